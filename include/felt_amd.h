@@ -1,0 +1,169 @@
+/*
+ * felt_amd.h — C ABI of the MI355X (gfx950) sparse-field level-set engine.
+ *
+ * Every entry point replaces one member of Felt's header-only C++ API.  The C++ classes
+ * Felt::Surface<D,L> and Felt::Polys<S> in include/felt_amd/Surface.hpp are thin wrappers over
+ * these calls; the cited reference lines are the functions each call stands in for.
+ *
+ * Conventions
+ *   - Positions are world coordinates (the reference's VecDi), int32, D values per position,
+ *     x first.  A surface of size s has world offset -s/2 (C division), so it spans
+ *     [-s/2, s - s/2) per axis (Surface.hpp:1716-1719).
+ *   - Partition indices are the reference's children-grid PosIdx: row-major, last axis fastest,
+ *     over the children grid (Util.hpp:23-40; PartitionedMixin.hpp:131-141).
+ *   - Layer list index k = layer + L (Surface.hpp:672-675).
+ *   - Every call returns FELT_OK (0) or a negative FELT_E_* code; felt_last_error() returns a
+ *     thread-local message for the last failure.  The reference throws std::domain_error in the
+ *     same places when FELT_EXCEPTIONS is defined (Common.hpp:167-172); the C++ wrapper throws it.
+ *   - A handle owns one HIP stream; calls on one handle must come from one thread at a time.
+ *   - Query calls that fill caller arrays take a capacity and report the full count in *n; when
+ *     the capacity is too small they fill nothing and return FELT_E_CAPACITY.
+ */
+#ifndef FELT_AMD_H
+#define FELT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FELT_OK 0
+#define FELT_E_ARG (-1)        /* invalid argument (shape, layer, position outside the grid) */
+#define FELT_E_DEVICE (-2)     /* HIP runtime error */
+#define FELT_E_NOMEM (-3)      /* device allocation failed or budget exhausted */
+#define FELT_E_CAPACITY (-4)   /* output buffer too small; *n holds the required count */
+#define FELT_E_STATE (-5)      /* call not valid in the handle's current state */
+#define FELT_E_DELTA (-6)      /* |delta| > 1 or a layer jump > 1 (Surface.hpp:278-287, 441-459) */
+
+typedef struct felt_surface felt_surface;
+typedef struct felt_polys felt_polys;
+
+/* Speed functor descriptor.  The reference takes any functor returning the zero-layer delta
+ * (Surface.hpp:254-296, 1754-1802); the device path evaluates this closed set, all in fp32
+ * with no fused multiply-add:
+ *   FELT_SPEED_CONSTANT    delta = a
+ *   FELT_SPEED_CURVATURE   delta = b * curv(x)
+ *   FELT_SPEED_CONST_CURV  delta = a + b * curv(x)
+ *   FELT_SPEED_NOISE_CURV  delta = clamp(a * (1 + c * noise(x)) + b * curv(x), -1, 1)
+ * curv is Felt's difference-of-normals mean curvature (NumericMixin.hpp:72-156).  noise is
+ * value noise on a lattice of `cell` voxels: lattice values are U[-1,1) drawn from
+ * SplitMix64(seed ^ hash(lattice point)), trilinearly (bilinearly in 2D) interpolated. */
+enum {
+    FELT_SPEED_CONSTANT = 0,
+    FELT_SPEED_CURVATURE = 1,
+    FELT_SPEED_CONST_CURV = 2,
+    FELT_SPEED_NOISE_CURV = 3
+};
+typedef struct felt_speed {
+    int32_t kind;
+    float a, b, c;
+    uint64_t seed;
+    int32_t cell;
+    int32_t reserved;
+} felt_speed;
+
+/* Partition-list kinds for felt_surface_partition_list. */
+enum {
+    FELT_LIST_ISOGRID = 0,       /* isogrid().children().lookup().list(k) */
+    FELT_LIST_DELTA = 1,         /* delta(k)          (Surface.hpp:637-640) */
+    FELT_LIST_STATUS_CHANGE = 2, /* status_change(k)  (Surface.hpp:648-651) */
+    FELT_LIST_AFFECTED = 3       /* affected().children().lookup().list(k) (Surface.hpp:693-696) */
+};
+
+typedef struct felt_stats {
+    uint64_t active_isogrid_partitions; /* Surface::Stats (Surface.hpp:142-148, 623-629) */
+    uint64_t active_delta_partitions;
+    uint64_t band_voxels;               /* sum of the 2L+1 layer sizes now */
+    uint64_t last_voxel_updates;        /* voxel updates counted for the last update call */
+    uint64_t last_rounds;               /* distance/flush/expand rounds of the last update */
+    uint64_t nan_events;                /* layer_id() calls that saw NaN since creation */
+} felt_stats;
+
+const char* felt_last_error(void);
+/* Number of HIP devices visible, or a negative code. */
+int felt_device_count(int* n);
+
+/* ---- Surface lifecycle -------------------------------------------------------------- */
+/* Surface(size, size_partition) (Surface.hpp:163-173).  D is 2 or 3, 1 <= L <= 8. */
+int felt_surface_create(int D, int L, const int32_t* size, const int32_t* part, felt_surface** out);
+int felt_surface_destroy(felt_surface* s);
+/* Geometry: children-grid size (3 values, unused axes 1) and partition count. */
+int felt_surface_geometry(felt_surface* s, int32_t* children_size, uint64_t* num_partitions);
+
+/* seed(pos) — singularity seed (Surface.hpp:207-241). */
+int felt_surface_seed(felt_surface* s, const int32_t* pos);
+/* Load a dense signed-distance field (row-major over the whole grid, world order).  Not a
+ * reference member: it builds the state Surface::load (Surface.hpp:180-196) would hold for
+ * that field.  Partitions holding a band voxel are active with every voxel set from phi;
+ * band voxels are tracked in row-major order; other partitions stay inactive with background
+ * -(L+1) if their first voxel is negative, else L+1. */
+int felt_surface_load_dense(felt_surface* s, const float* phi, size_t n);
+
+/* ---- Stepping ----------------------------------------------------------------------- */
+/* update(fn) — global step (Surface.hpp:254-296, 739-762). */
+int felt_surface_update(felt_surface* s, const felt_speed* fn);
+/* update(pos_lo, pos_hi, fn) — bounded step (Surface.hpp:310-386, 404-430). */
+int felt_surface_update_box(felt_surface* s, const int32_t* lo, const int32_t* hi,
+                            const felt_speed* fn);
+/* update(fn) for a host functor: fn(pos) was evaluated by the caller for the listed zero-layer
+ * positions; every other zero-layer voxel gets delta 0 (Surface.hpp:254-296). */
+int felt_surface_update_deltas(felt_surface* s, size_t n, const int32_t* pos, const float* delta);
+/* Low-level trio update_start / delta / update_end (Surface.hpp:393-459). */
+int felt_surface_update_start(felt_surface* s);
+int felt_surface_delta(felt_surface* s, size_t n, const int32_t* pos, const float* delta);
+int felt_surface_update_end(felt_surface* s);
+
+/* ---- Read-back ---------------------------------------------------------------------- */
+/* isogrid().snapshot() into a caller buffer of prod(size) floats (PartitionedMixin.hpp:642-660). */
+int felt_surface_snapshot(felt_surface* s, float* dst, size_t n);
+/* get(pos) for n positions, background outside the grid (PartitionedMixin.hpp:594-601). */
+int felt_surface_get(felt_surface* s, size_t n, const int32_t* pos, float* out);
+/* Total size of layer `layer` (-L..L), summed over partitions. */
+int felt_surface_layer_size(felt_surface* s, int layer, uint64_t* n);
+/* Positions tracked in layer `layer` (order unspecified). */
+int felt_surface_layer_positions(felt_surface* s, int layer, int32_t* pos, size_t cap, size_t* n);
+/* Zero-layer positions inside [lo, min(hi+1, grid_hi)) (the voxels update(lo,hi,fn) visits,
+ * Surface.hpp:313-353). */
+int felt_surface_zero_layer_in_box(felt_surface* s, const int32_t* lo, const int32_t* hi,
+                                   int32_t* pos, size_t cap, size_t* n);
+/* Partition PosIdx list of a kind (FELT_LIST_*) and list index k = layer + L (order unspecified). */
+int felt_surface_partition_list(felt_surface* s, int kind, int k, uint64_t* out, size_t cap,
+                                size_t* n);
+/* Voxels in the affected lookup of list k (affected().leafs(k, fn)). */
+int felt_surface_affected_positions(felt_surface* s, int k, int32_t* pos, size_t cap, size_t* n);
+/* 1 per active isogrid partition, 0 otherwise, for all partitions. */
+int felt_surface_partition_active(felt_surface* s, uint8_t* out, size_t n);
+/* Number of layer-k voxels in partition pos_idx (layer_size helper, test_Surface.cpp:1720-1737). */
+int felt_surface_partition_layer_size(felt_surface* s, uint64_t pos_idx, int layer, uint64_t* n);
+int felt_surface_stats(felt_surface* s, felt_stats* out);
+/* Block until all queued device work of the handle is done. */
+int felt_surface_sync(felt_surface* s);
+
+/* ---- Polygonisation (Polys.hpp:29-201) ---------------------------------------------- */
+int felt_polys_create(felt_surface* s, felt_polys** out);
+int felt_polys_destroy(felt_polys* p);
+int felt_polys_notify(felt_polys* p);
+int felt_polys_march(felt_polys* p);
+int felt_polys_invalidate(felt_polys* p);
+/* changes(): partitions marched by the last march (Polys.hpp:198-201). */
+int felt_polys_changes(felt_polys* p, uint64_t* out, size_t cap, size_t* n);
+/* children().get(pos_idx): active flag and vertex / simplex counts (Poly.hpp:176-189). */
+int felt_polys_child_info(felt_polys* p, uint64_t pos_idx, int* active, uint64_t* nvtx,
+                          uint64_t* nspx);
+/* Mesh of one child: vtx gets nvtx*(D or 2D) floats (pos, then normal in 3D), spx gets nspx*D
+ * vertex indices local to the child (PolyMixin.hpp:34-68, 125-163). */
+int felt_polys_child_mesh(felt_polys* p, uint64_t pos_idx, float* vtx, size_t vtx_cap,
+                          uint32_t* spx, size_t spx_cap);
+/* Totals over all active children. */
+int felt_polys_totals(felt_polys* p, uint64_t* nvtx, uint64_t* nspx);
+
+/* ---- Tables ------------------------------------------------------------------------- */
+/* Copy of the packed marching-cubes triangle table the kernels use (256 entries). */
+int felt_mc_table(uint64_t* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FELT_AMD_H */
