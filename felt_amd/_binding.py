@@ -100,6 +100,7 @@ class Binding:
         L = self.lib
         sig = {
             "last_error": (C.c_char_p, []),
+            "device_count": (C.c_int, [C.POINTER(C.c_int)]),
             "surface_destroy": (C.c_int, [_P]),
             "surface_geometry": (C.c_int, [_P, _I32P, _U64P]),
             "surface_seed": (C.c_int, [_P, _I32P]),
@@ -124,6 +125,7 @@ class Binding:
             "polys_notify": (C.c_int, [_P]),
             "polys_march": (C.c_int, [_P]),
             "polys_invalidate": (C.c_int, [_P]),
+            "polys_mark": (C.c_int, [_P, C.c_uint64]),
             "polys_changes": (C.c_int, [_P, _U64P, _SZ, _SZP]),
             "polys_child_info": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int), _U64P, _U64P]),
             "polys_child_mesh": (C.c_int, [_P, C.c_uint64, _F32P, _SZ, _U32P, _SZ]),
@@ -141,6 +143,8 @@ class Binding:
             sig["polys_destroy"] = (C.c_int, [_P])
         self.fn = {}
         for name, (res, args) in sig.items():
+            if not hasattr(L, prefix + name) and name in ("device_count",):
+                continue
             f = getattr(L, prefix + name)
             f.restype = res
             f.argtypes = args
@@ -378,6 +382,10 @@ class Polys:
 
     def invalidate(self) -> None:
         self.b.call("polys_invalidate", self.h)
+
+    def mark(self, pos_idx: int) -> None:
+        """Queue one partition for the next march (Impl::Poly::Single use)."""
+        self.b.call("polys_mark", self.h, int(pos_idx))
 
     def changes(self) -> np.ndarray:
         n = C.c_size_t()

@@ -1,0 +1,234 @@
+// felt_dev.h — device-side data layout and stencil helpers shared by the surface and polygon
+// kernels (gfx950).
+//
+// Layout in HBM (one surface):
+//   page[np]        partition -> brick slot, or -1 when the partition is inactive
+//   pbg[np]         value every voxel of an inactive partition reads (the reference's child
+//                   background, GridMixin.hpp:587-621; PartitionedMixin.hpp:502-516)
+//   pcnt[np*NL]     voxels per partition per layer list; a partition is in the reference's
+//                   parent list k exactly when pcnt[p*NL+k] > 0 (PartitionedMixin.hpp:489-558)
+//   slot_part[cap]  brick slot -> partition, -1 when free
+//   phi[cap*pv]     signed distances, one brick of pv = prod(part) voxels per slot, row-major
+//                   inside the brick with the last axis fastest (Util.hpp:23-40)
+//   lay[cap*pv]     layer list of each voxel (-L..L) or NONE: replaces the reference's per-child
+//                   lookup grid and per-layer leaf lists (LookupMixin.hpp:225-387)
+// A voxel is named by v = slot*pv + leaf.  Grid coordinates g are world coordinates minus the
+// grid offset, so 0 <= g < size; partition coordinates are g / part.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace felt {
+
+constexpr int8_t NONE = -128;
+constexpr float kEps = 1.1920928955078125e-07f;  // FLT_EPSILON
+
+struct Geo {
+    int D, L, NL;
+    int size[3], part[3], csize[3], offset[3];
+    int lstr[3];  // leaf strides inside a brick
+    int cstr[3];  // partition strides in the children grid
+    int pv;
+    int np;
+};
+
+struct Pool {
+    int* page;
+    float* pbg;
+    int* pcnt;
+    int* slot_part;
+    float* phi;
+    int8_t* lay;
+};
+
+// iround(v + FLT_EPSILON), exact round-half-away-from-zero (Surface.hpp:1694-1697).  NaN has
+// no reference value; it maps outside the band and is counted.
+__device__ __forceinline__ int layer_of(float v, int L, int* nan_ctr) {
+    const float t = v + kEps;
+    if (t != t) {
+        atomicAdd(nan_ctr, 1);
+        return L + 1;
+    }
+    return (int)__builtin_roundf(t);
+}
+
+__device__ __forceinline__ bool in_band(int k, int L) { return k >= -L && k <= L; }
+
+template <int D>
+__device__ __forceinline__ void decode(const Geo& G, const Pool& P, uint32_t v, int& slot,
+                                       int loc[3], int g[3]) {
+    slot = (int)(v / (uint32_t)G.pv);
+    int l = (int)(v - (uint32_t)slot * (uint32_t)G.pv);
+    int p = P.slot_part[slot];
+#pragma unroll
+    for (int a = D - 1; a >= 0; a--) {
+        const int pc = p % G.csize[a];
+        p /= G.csize[a];
+        loc[a] = l % G.part[a];
+        l /= G.part[a];
+        g[a] = pc * G.part[a] + loc[a];
+    }
+}
+
+template <int D>
+__device__ __forceinline__ bool inside(const Geo& G, const int q[3]) {
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        if (q[a] < 0 || q[a] >= G.size[a]) return false;
+    return true;
+}
+
+template <int D>
+__device__ __forceinline__ int part_of(const Geo& G, const int q[3]) {
+    int p = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) p += (q[a] / G.part[a]) * G.cstr[a];
+    return p;
+}
+
+template <int D>
+__device__ __forceinline__ int leaf_of(const Geo& G, const int q[3]) {
+    int l = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) l += (q[a] % G.part[a]) * G.lstr[a];
+    return l;
+}
+
+// get(pos) of the partitioned isogrid (PartitionedMixin.hpp:594-601): L+1 outside the grid,
+// the partition background when inactive.
+template <int D>
+__device__ __forceinline__ float get_g(const Geo& G, const Pool& P, const int q[3]) {
+    if (!inside<D>(G, q)) return (float)(G.L + 1);
+    const int p = part_of<D>(G, q);
+    const int s = P.page[p];
+    if (s < 0) return P.pbg[p];
+    return P.phi[(size_t)s * G.pv + leaf_of<D>(G, q)];
+}
+
+// Value at g + d (d components in {-1,0,1}); reads the voxel's own brick directly when the
+// neighbour stays inside it and inside the grid.
+template <int D>
+__device__ __forceinline__ float nb(const Geo& G, const Pool& P, uint32_t v, const int loc[3],
+                                    const int g[3], int d0, int d1, int d2) {
+    const int d[3] = {d0, d1, d2};
+    bool same = true;
+    int off = 0;
+    int q[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const int la = loc[a] + d[a];
+        q[a] = g[a] + d[a];
+        if (la < 0 || la >= G.part[a] || q[a] >= G.size[a]) same = false;
+        off += d[a] * G.lstr[a];
+    }
+    if (same) return P.phi[(int64_t)v + off];
+    return get_g<D>(G, P, q);
+}
+
+// distance(pos, dist, dir) (Surface.hpp:1387-1440): signed minimum over in-grid face
+// neighbours in the direction of the zero curve, plus one step.
+template <int D>
+__device__ __forceinline__ float distance(const Geo& G, const Pool& P, uint32_t v,
+                                          const int loc[3], const int g[3], float self,
+                                          float dir) {
+    float dist = self * dir;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+#pragma unroll
+        for (int s = -1; s <= 1; s += 2) {
+            const int qa = g[a] + s;
+            if (qa < 0 || qa >= G.size[a]) continue;
+            const float dn = nb<D>(G, P, v, loc, g, a == 0 ? s : 0, a == 1 ? s : 0,
+                                   a == 2 ? s : 0);
+            if (dn * dir < dist) dist = dn * dir;
+        }
+    }
+    const float dn = dist * dir;
+    return dn + dir;
+}
+
+// Mean curvature by difference of normals (NumericMixin.hpp:72-156), same operation order as
+// the reference with no contraction: forward normals, backward normals, then
+// (dn0 + (dn1 + dn2)) / 2 in 3D or (dn0 + dn1) / 2 in 2D.
+template <int D>
+__device__ __forceinline__ float curvature(const Geo& G, const Pool& P, uint32_t v,
+                                           const int loc[3], const int g[3]) {
+    const float c = P.phi[v];
+    float nf[3], nbk[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int e[3] = {0, 0, 0};
+        e[a] = 1;
+        const float va = nb<D>(G, P, v, loc, g, e[0], e[1], e[2]) - c;
+        float sq = 0.0f;
+#pragma unroll
+        for (int b = 0; b < D; b++) {
+            if (b == a) continue;
+            int lo[3] = {e[0], e[1], e[2]}, hi[3] = {e[0], e[1], e[2]};
+            lo[b] -= 1;
+            hi[b] += 1;
+            const float vl = nb<D>(G, P, v, loc, g, lo[0], lo[1], lo[2]);
+            const float vh = nb<D>(G, P, v, loc, g, hi[0], hi[1], hi[2]);
+            const float t = (vh - vl) / 2.0f;
+            sq += t * t;
+        }
+        nf[a] = va / __builtin_sqrtf(va * va + sq);
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int e[3] = {0, 0, 0};
+        e[a] = -1;
+        const float va = c - nb<D>(G, P, v, loc, g, e[0], e[1], e[2]);
+        float sq = 0.0f;
+#pragma unroll
+        for (int b = 0; b < D; b++) {
+            if (b == a) continue;
+            int lo[3] = {e[0], e[1], e[2]}, hi[3] = {e[0], e[1], e[2]};
+            lo[b] -= 1;
+            hi[b] += 1;
+            const float vl = nb<D>(G, P, v, loc, g, lo[0], lo[1], lo[2]);
+            const float vh = nb<D>(G, P, v, loc, g, hi[0], hi[1], hi[2]);
+            const float t = (vh - vl) / 2.0f;
+            sq += t * t;
+        }
+        nbk[a] = va / __builtin_sqrtf(va * va + sq);
+    }
+    float dn[3];
+#pragma unroll
+    for (int a = 0; a < D; a++) dn[a] = nf[a] - nbk[a];
+    float s;
+    if (D == 3) s = dn[0] + (dn[1] + dn[D - 1]);
+    else s = dn[0] + dn[1];
+    return s / 2.0f;
+}
+
+// Wave-aggregated append: one atomic per wavefront.  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_append(int* counter, bool pred) {
+    const unsigned long long m = __ballot(pred);
+    const int lane = __lane_id();
+    if (m == 0ull) return -1;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    const unsigned long long below = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    return pred ? base + __popcll(below) : -1;
+}
+
+// Compare-and-swap of one byte through its aligned 32-bit word.
+__device__ __forceinline__ bool cas_byte(int8_t* addr, int8_t expect, int8_t desired) {
+    const uintptr_t a = (uintptr_t)addr;
+    unsigned int* w = (unsigned int*)(a & ~(uintptr_t)3);
+    const int sh = (int)(a & 3) * 8;
+    unsigned int old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (true) {
+        if ((int8_t)((old >> sh) & 0xffu) != expect) return false;
+        const unsigned int nw = (old & ~(0xffu << sh)) | ((unsigned int)(uint8_t)desired << sh);
+        const unsigned int prev = atomicCAS(w, old, nw);
+        if (prev == old) return true;
+        old = prev;
+    }
+}
+
+}  // namespace felt

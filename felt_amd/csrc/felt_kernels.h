@@ -1,0 +1,112 @@
+// felt_kernels.h — HIP kernels of the sparse-field update (gfx950).  Each kernel restates one
+// phase of Surface<D,L>::update (Surface.hpp:254-1440) over compacted voxel lists.
+#pragma once
+
+#include "felt_dev.h"
+#include "../../include/felt_amd.h"
+
+namespace felt {
+
+// Device counters of one surface.
+struct Ctr {
+    int free_top;   // free-slot stack height
+    int sc_n;       // status-change records this round
+    int lu_n[2];    // entries of the two lookups (affected / affected buffer)
+    int flag;       // a non-zero-layer status change happened this round
+    int need_n;     // partitions to activate
+    int touched_n;  // partitions that lost a voxel to untrack
+    int nan_events;
+    int act_n;      // active partitions (compaction)
+    int z_n;        // zero-layer delta list entries
+    int err;        // 1: brick pool exhausted
+    int misc;
+    int front[20];  // breadth-first frontier bounds of calc_affected
+};
+
+// Status-change record: the reference's StatusChangeGrid entry (Surface.hpp:1138-1178).
+struct SC {
+    uint32_t* v;
+    int8_t* from;
+    int16_t* to;
+};
+
+// A lookup: the reference's AffectedLookupGrid as a tagged voxel list (tag = layer).
+struct Lu {
+    uint32_t* v;
+    int8_t* tag;
+    float* d;  // distance scratch aligned with v
+};
+
+struct Speed {
+    int kind;
+    float a, b, c;
+    unsigned long long seed;
+    int cell;
+};
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// Value noise: lattice values U[-1,1) from SplitMix64 of the lattice point, N-linear
+// interpolation reducing axis 0 first, fp32 without contraction.
+template <int D>
+__device__ __forceinline__ float value_noise(const int w[3], unsigned long long seed, int cell) {
+    int base[3] = {0, 0, 0};
+    float t[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const int q = w[a] >= 0 ? w[a] / cell : -((-w[a] + cell - 1) / cell);
+        base[a] = q;
+        t[a] = (float)(w[a] - q * cell) / (float)cell;
+    }
+    const unsigned long long mul[3] = {0x9E3779B97F4A7C15ULL, 0xC2B2AE3D27D4EB4FULL,
+                                       0x165667B19E3779F9ULL};
+    float val[8];
+#pragma unroll
+    for (int i = 0; i < (1 << D); i++) {
+        unsigned long long key = 0;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+            key += (unsigned long long)(long long)(base[a] + ((i >> a) & 1)) * mul[a];
+        const unsigned long long h = splitmix64(seed ^ key);
+        const float u = (float)(h >> 40) * (1.0f / 16777216.0f);
+        val[i] = u * 2.0f - 1.0f;
+    }
+    int n = 1 << D;
+#pragma unroll
+    for (int axis = 0; axis < D; axis++) {
+        const float wt = t[axis];
+        for (int i = 0; i < n / 2; i++) val[i] = wt * val[2 * i + 1] + (1.0f - wt) * val[2 * i];
+        n /= 2;
+    }
+    return val[0];
+}
+
+template <int D>
+__device__ __forceinline__ float eval_speed(const Geo& G, const Pool& P, const Speed& sp,
+                                            uint32_t v, const int loc[3], const int g[3]) {
+    switch (sp.kind) {
+    case FELT_SPEED_CONSTANT:
+        return sp.a;
+    case FELT_SPEED_CURVATURE:
+        return sp.b * curvature<D>(G, P, v, loc, g);
+    case FELT_SPEED_CONST_CURV:
+        return sp.a + sp.b * curvature<D>(G, P, v, loc, g);
+    default: {
+        int w[3];
+#pragma unroll
+        for (int a = 0; a < D; a++) w[a] = g[a] + G.offset[a];
+        const float n = value_noise<D>(w, sp.seed, sp.cell);
+        float r = sp.a * (1.0f + sp.c * n) + sp.b * curvature<D>(G, P, v, loc, g);
+        if (r < -1.0f) r = -1.0f;
+        else if (r > 1.0f) r = 1.0f;
+        return r;
+    }
+    }
+}
+
+}  // namespace felt

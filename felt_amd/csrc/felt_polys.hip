@@ -1,0 +1,692 @@
+// felt_polys.hip — per-partition marching cubes / squares on the device (gfx950).
+//
+// Replaces Felt::Polys<S> and Impl::Poly::Single (Polys.hpp:29-201, Poly.hpp:17-368).  For each
+// partition to polygonise, one workgroup visits the band voxels of its brick; the cube of voxel
+// x spans x..x+1 on every axis (the reference's pos_calc = x - SpxGridPosOffset with its corner
+// table, PolyMixin.hpp:380-409).  A crossed edge yields one vertex per partition, as the
+// reference's per-partition lookup does (Poly.hpp:296-346); here the vertex belongs to the
+// band cube of lowest leaf index among the cubes of the partition that share the edge, so
+// vertex indices come from two block-wide exclusive scans and no atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "felt_polys.h"
+
+using namespace felt;
+
+namespace {
+
+constexpr int TPB = 256;
+
+#include "mc_tables.inc"
+
+__constant__ unsigned long long c_mc_tri[256];
+__constant__ unsigned short c_ms_seg[16];
+
+// Edge table in the reference's corner convention: offset of the edge's low end from pos_calc
+// and its axis (PolyMixin.hpp:271-276, 390-406).
+__constant__ int c_edge3[12][4] = {{0, 0, 0, 0},  {1, 0, -1, 2}, {0, 0, -1, 0}, {0, 0, -1, 2},
+                                   {0, 1, 0, 0},  {1, 1, -1, 2}, {0, 1, -1, 0}, {0, 1, -1, 2},
+                                   {0, 0, 0, 1},  {1, 0, 0, 1},  {1, 0, -1, 1}, {0, 0, -1, 1}};
+__constant__ int c_corner3[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, -1}, {0, 0, -1},
+                                    {0, 1, 0}, {1, 1, 0}, {1, 1, -1}, {0, 1, -1}};
+__constant__ int c_edge2[4][3] = {{0, 0, 0}, {1, 0, 1}, {0, 1, 0}, {0, 0, 1}};
+__constant__ int c_corner2[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+// For each axis, the edges along it (3D: 4, 2D: 2).
+__constant__ int c_axis_edges3[3][4] = {{0, 2, 4, 6}, {8, 9, 10, 11}, {1, 3, 5, 7}};
+__constant__ int c_axis_edges2[2][2] = {{0, 2}, {1, 3}};
+
+template <int D>
+struct McT;
+template <>
+struct McT<3> {
+    static constexpr int nedge = 12, ncorner = 8, per_axis = 4;
+    __device__ static int edge(int e, int a) { return c_edge3[e][a]; }
+    __device__ static int corner(int c, int a) { return c_corner3[c][a]; }
+    __device__ static int axis_edge(int axis, int i) { return c_axis_edges3[axis][i]; }
+    // pos_calc = pos - SpxGridPosOffset, SpxGridPosOffset = (0,0,-1)
+    __device__ static int calc_off(int a) { return a == 2 ? 1 : 0; }
+    __device__ static unsigned edges_of(unsigned m) {
+        const unsigned long long t = c_mc_tri[m];
+        const int n = (int)(t >> 60) * 3;
+        unsigned em = 0;
+        for (int i = 0; i < n; i++) em |= 1u << ((t >> (4 * i)) & 0xf);
+        return em;
+    }
+    __device__ static int nsimplex(unsigned m) { return (int)(c_mc_tri[m] >> 60); }
+    __device__ static int simplex_edge(unsigned m, int i, int j) {
+        return (int)((c_mc_tri[m] >> (4 * (i * 3 + j))) & 0xf);
+    }
+};
+template <>
+struct McT<2> {
+    static constexpr int nedge = 4, ncorner = 4, per_axis = 2;
+    __device__ static int edge(int e, int a) { return c_edge2[e][a]; }
+    __device__ static int corner(int c, int a) { return c_corner2[c][a]; }
+    __device__ static int axis_edge(int axis, int i) { return c_axis_edges2[axis][i]; }
+    __device__ static int calc_off(int) { return 0; }
+    __device__ static unsigned edges_of(unsigned m) {
+        const unsigned t = c_ms_seg[m];
+        const int n = (int)(t >> 12) * 2;
+        unsigned em = 0;
+        for (int i = 0; i < n; i++) em |= 1u << ((t >> (4 * i)) & 0xf);
+        return em;
+    }
+    __device__ static int nsimplex(unsigned m) { return (int)(c_ms_seg[m] >> 12); }
+    __device__ static int simplex_edge(unsigned m, int i, int j) {
+        return (int)((c_ms_seg[m] >> (4 * (i * 2 + j))) & 0xf);
+    }
+};
+
+// Corner mask of the cube based at grid position g (bit i: phi(corner i) > 0, Poly.hpp:356-368).
+template <int D>
+__device__ unsigned cube_mask(const Geo& G, const Pool& P, const int g[3]) {
+    unsigned m = 0;
+    for (int c = 0; c < McT<D>::ncorner; c++) {
+        int q[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) q[a] = g[a] + McT<D>::calc_off(a) + McT<D>::corner(c, a);
+        m |= (unsigned)(get_g<D>(G, P, q) > 0.0f) << c;
+    }
+    return m;
+}
+
+// Leaf index (within partition p's brick) of the band cube at grid base position b, or -1.
+template <int D>
+__device__ int band_cube(const Geo& G, const Pool& P, int s, const int pc[3], const int b[3]) {
+    int l = 0;
+    for (int a = 0; a < D; a++) {
+        const int la = b[a] - pc[a] * G.part[a];
+        if (la < 0 || la >= G.part[a] || b[a] >= G.size[a]) return -1;
+        l += la * G.lstr[a];
+    }
+    return P.lay[(size_t)s * G.pv + l] != NONE ? l : -1;
+}
+
+// Owner of the vertex on edge e of the cube at base b: the lowest-leaf band cube of the
+// partition among those sharing the edge.  Returns its leaf and the edge's index in its frame.
+template <int D>
+__device__ int edge_owner(const Geo& G, const Pool& P, int s, const int pc[3], const int b[3],
+                          int e, int* e_owner) {
+    int pa[3] = {0, 0, 0};
+    for (int a = 0; a < D; a++) pa[a] = b[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
+    const int axis = McT<D>::edge(e, D);
+    int best = -1, best_e = e;
+    for (int i = 0; i < McT<D>::per_axis; i++) {
+        const int f = McT<D>::axis_edge(axis, i);
+        int cb[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) cb[a] = pa[a] - McT<D>::edge(f, a) - McT<D>::calc_off(a);
+        const int l = band_cube<D>(G, P, s, pc, cb);
+        if (l >= 0 && (best < 0 || l < best)) {
+            best = l;
+            best_e = f;
+        }
+    }
+    *e_owner = best_e;
+    return best;
+}
+
+// Block-wide exclusive scan of two int arrays in place, returning the totals.
+__device__ void block_scan2(int* a, int* b, int n, int* tot_a, int* tot_b) {
+    __shared__ int sa[TPB], sb[TPB];
+    __shared__ int carry_a, carry_b;
+    if (threadIdx.x == 0) {
+        carry_a = 0;
+        carry_b = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < n; base += TPB) {
+        const int i = base + threadIdx.x;
+        const int va = i < n ? a[i] : 0, vb = i < n ? b[i] : 0;
+        sa[threadIdx.x] = va;
+        sb[threadIdx.x] = vb;
+        __syncthreads();
+        for (int off = 1; off < TPB; off <<= 1) {
+            const int xa = threadIdx.x >= off ? sa[threadIdx.x - off] : 0;
+            const int xb = threadIdx.x >= off ? sb[threadIdx.x - off] : 0;
+            __syncthreads();
+            sa[threadIdx.x] += xa;
+            sb[threadIdx.x] += xb;
+            __syncthreads();
+        }
+        if (i < n) {
+            a[i] = carry_a + sa[threadIdx.x] - va;
+            b[i] = carry_b + sb[threadIdx.x] - vb;
+        }
+        __syncthreads();
+        if (threadIdx.x == TPB - 1) {
+            carry_a += sa[TPB - 1];
+            carry_b += sb[TPB - 1];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *tot_a = carry_a;
+        *tot_b = carry_b;
+    }
+}
+
+// Pass 1: per band voxel, the cube mask, the edges whose vertex it owns, and its vertex and
+// simplex counts; then per-partition exclusive scans.
+template <int D>
+__global__ void k_march_count(Geo G, Pool P, const int* parts, uint8_t* cmask, uint16_t* own,
+                              int* vbase, int* tbase, int* totals) {
+    const int j = blockIdx.x;
+    const int p = parts[j];
+    const int s = P.page[p];
+    uint8_t* cm = cmask + (size_t)j * G.pv;
+    uint16_t* ow = own + (size_t)j * G.pv;
+    int* vb = vbase + (size_t)j * G.pv;
+    int* tb = tbase + (size_t)j * G.pv;
+    int pc[3] = {0, 0, 0}, rem = p;
+    for (int a = D - 1; a >= 0; a--) {
+        pc[a] = rem % G.csize[a];
+        rem /= G.csize[a];
+    }
+    for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
+        int nv = 0, nt = 0;
+        unsigned m = 0, om = 0;
+        if (s >= 0 && P.lay[(size_t)s * G.pv + l] != NONE) {
+            int g[3] = {0, 0, 0}, r = l;
+            for (int a = D - 1; a >= 0; a--) {
+                g[a] = pc[a] * G.part[a] + r % G.part[a];
+                r /= G.part[a];
+            }
+            m = cube_mask<D>(G, P, g);
+            const unsigned em = McT<D>::edges_of(m);
+            for (int e = 0; e < McT<D>::nedge; e++) {
+                if (!((em >> e) & 1)) continue;
+                int eo;
+                if (edge_owner<D>(G, P, s, pc, g, e, &eo) == l) om |= 1u << e;
+            }
+            nv = __popc(om);
+            nt = McT<D>::nsimplex(m);
+        }
+        cm[l] = (uint8_t)m;
+        ow[l] = (uint16_t)om;
+        vb[l] = nv;
+        tb[l] = nt;
+    }
+    __syncthreads();
+    block_scan2(vb, tb, G.pv, &totals[2 * j], &totals[2 * j + 1]);
+}
+
+// Felt's N-linear interpolation in world coordinates (NumericMixin.hpp:403-481).
+template <int D>
+__device__ float interp_w(const Geo& G, const Pool& P, const float w[3]) {
+    int fl[3] = {0, 0, 0};
+    float t[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < D; a++) {
+        const float f = floorf(w[a]);
+        fl[a] = (int)f;
+        t[a] = w[a] - f;
+    }
+    float val[8];
+    for (int i = 0; i < (1 << D); i++) {
+        int q[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) q[a] = fl[a] + ((i >> a) & 1) - G.offset[a];
+        val[i] = get_g<D>(G, P, q);
+    }
+    int n = 1 << D;
+    for (int axis = 0; axis < D; axis++) {
+        const float wt = t[axis];
+        for (int i = 0; i < n / 2; i++) val[i] = wt * val[2 * i + 1] + (1.0f - wt) * val[2 * i];
+        n /= 2;
+    }
+    return val[0];
+}
+
+template <int D>
+__device__ bool inside_wf(const Geo& G, const float w[3]) {
+    for (int a = 0; a < D; a++) {
+        if (w[a] >= (float)(G.offset[a] + G.size[a])) return false;
+        if (w[a] < (float)G.offset[a]) return false;
+    }
+    return true;
+}
+
+// Safe gradient at a real world position (NumericMixin.hpp:188-231), probe stepped -1,+2,-1.
+template <int D>
+__device__ void grad_f(const Geo& G, const Pool& P, const float w[3], float out[3]) {
+    float pt[3] = {w[0], w[1], w[2]};
+    const float centre = interp_w<D>(G, P, w);
+    for (int a = 0; a < D; a++) {
+        float back = centre, fwd = centre;
+        int order = 0;
+        pt[a] -= 1.0f;
+        if (inside_wf<D>(G, pt)) {
+            back = interp_w<D>(G, P, pt);
+            order++;
+        }
+        pt[a] += 2.0f;
+        if (inside_wf<D>(G, pt)) {
+            fwd = interp_w<D>(G, P, pt);
+            order++;
+        }
+        pt[a] -= 1.0f;
+        out[a] = order != 0 ? (fwd - back) / (float)order : 0.0f;
+        out[a] = out[a] / 1.0f;
+    }
+}
+
+// Safe gradient at an integer grid position.
+template <int D>
+__device__ void grad_i(const Geo& G, const Pool& P, const int g[3], float out[3]) {
+    int pt[3] = {g[0], g[1], g[2]};
+    const float centre = get_g<D>(G, P, g);
+    for (int a = 0; a < D; a++) {
+        float back = centre, fwd = centre;
+        int order = 0;
+        pt[a] -= 1;
+        if (inside<D>(G, pt)) {
+            back = get_g<D>(G, P, pt);
+            order++;
+        }
+        pt[a] += 2;
+        if (inside<D>(G, pt)) {
+            fwd = get_g<D>(G, P, pt);
+            order++;
+        }
+        pt[a] -= 1;
+        out[a] = order != 0 ? (fwd - back) / (float)order : 0.0f;
+        out[a] = out[a] / 1.0f;
+    }
+}
+
+__device__ void normalize3(float n[3]) {
+    const float sq = n[0] * n[0] + (n[1] * n[1] + n[2] * n[2]);
+    if (sq > 0.0f) {
+        const float r = __builtin_sqrtf(sq);
+        n[0] = n[0] / r;
+        n[1] = n[1] / r;
+        n[2] = n[2] / r;
+    }
+}
+
+// Vertex on the edge from grid position ga along axis (Poly.hpp:296-346; PolyMixin.hpp:142-148).
+template <int D>
+__device__ void make_vertex(const Geo& G, const Pool& P, const int ga[3], int axis, float* out) {
+    int gb[3] = {ga[0], ga[1], ga[2]};
+    gb[axis] += 1;
+    const float va = get_g<D>(G, P, ga), vb = get_g<D>(G, P, gb);
+    float pos[3] = {0.f, 0.f, 0.f}, n[3] = {0.f, 0.f, 0.f};
+    if (fabsf(va) <= kEps || fabsf(vb) <= kEps) {
+        const int* e = fabsf(va) <= kEps ? ga : gb;
+        for (int a = 0; a < D; a++) pos[a] = (float)(e[a] + G.offset[a]);
+        if (D == 3) grad_i<D>(G, P, e, n);
+    } else {
+        float mu;
+        if (fabsf(va - vb) <= kEps) mu = 0.5f;
+        else mu = va / (va - vb);
+        for (int a = 0; a < D; a++) {
+            const float fa = (float)(ga[a] + G.offset[a]), fb = (float)(gb[a] + G.offset[a]);
+            pos[a] = fa + (fb - fa) * mu;
+        }
+        if (D == 3) grad_f<D>(G, P, pos, n);
+    }
+    if (D == 3) {
+        normalize3(n);
+        for (int a = 0; a < 3; a++) {
+            out[a] = pos[a];
+            out[3 + a] = n[a];
+        }
+    } else {
+        out[0] = pos[0];
+        out[1] = pos[1];
+    }
+}
+
+// Pass 2: vertices of owned edges and simplices of every band cube.
+template <int D>
+__global__ void k_march_emit(Geo G, Pool P, const int* parts, const uint8_t* cmask,
+                             const uint16_t* own, const int* vbase, const int* tbase,
+                             const long long* seg_v, const long long* seg_t, float* vout,
+                             uint32_t* tout) {
+    const int j = blockIdx.x;
+    const int p = parts[j];
+    const int s = P.page[p];
+    if (s < 0) return;
+    const uint8_t* cm = cmask + (size_t)j * G.pv;
+    const uint16_t* ow = own + (size_t)j * G.pv;
+    const int* vb = vbase + (size_t)j * G.pv;
+    const int* tb = tbase + (size_t)j * G.pv;
+    const int vw = D == 3 ? 6 : 2;
+    float* vo = vout + seg_v[j] * vw;
+    uint32_t* to = tout + seg_t[j] * D;
+    int pc[3] = {0, 0, 0}, rem = p;
+    for (int a = D - 1; a >= 0; a--) {
+        pc[a] = rem % G.csize[a];
+        rem /= G.csize[a];
+    }
+    for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
+        if (P.lay[(size_t)s * G.pv + l] == NONE) continue;
+        const unsigned m = cm[l];
+        const int ns = McT<D>::nsimplex(m);
+        if (ns == 0) continue;
+        int g[3] = {0, 0, 0}, r = l;
+        for (int a = D - 1; a >= 0; a--) {
+            g[a] = pc[a] * G.part[a] + r % G.part[a];
+            r /= G.part[a];
+        }
+        const unsigned om = ow[l];
+        int rank = 0;
+        for (int e = 0; e < McT<D>::nedge; e++) {
+            if (!((om >> e) & 1)) continue;
+            int ga[3] = {0, 0, 0};
+            for (int a = 0; a < D; a++) ga[a] = g[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
+            make_vertex<D>(G, P, ga, McT<D>::edge(e, D), vo + (size_t)(vb[l] + rank) * vw);
+            rank++;
+        }
+        for (int i = 0; i < ns; i++) {
+            for (int k = 0; k < D; k++) {
+                const int e = McT<D>::simplex_edge(m, i, k);
+                int eo;
+                const int o = edge_owner<D>(G, P, s, pc, g, e, &eo);
+                const unsigned oom = ow[o];
+                const int idx = vb[o] + __popc(oom & ((1u << eo) - 1u));
+                to[(size_t)(tb[l] + i) * D + k] = (uint32_t)idx;
+            }
+        }
+    }
+}
+
+// Copies live mesh segments into a fresh arena when the old one is compacted.
+__global__ void k_copy_segments(const long long* src, const long long* dst, const long long* len,
+                                int n, const float* from, float* into) {
+    for (int j = blockIdx.x; j < n; j += gridDim.x)
+        for (long long i = threadIdx.x; i < len[j]; i += blockDim.x) into[dst[j] + i] = from[src[j] + i];
+}
+__global__ void k_copy_segments_u(const long long* src, const long long* dst,
+                                  const long long* len, int n, const uint32_t* from,
+                                  uint32_t* into) {
+    for (int j = blockIdx.x; j < n; j += gridDim.x)
+        for (long long i = threadIdx.x; i < len[j]; i += blockDim.x) into[dst[j] + i] = from[src[j] + i];
+}
+
+// Per partition: bit 0 in delta(0)|delta(2L), bit 1 in status_change(0)|status_change(2L),
+// bit 2 tracked by the isogrid in list 0 or 2L, bit 3 isogrid partition active.
+__global__ void k_notify_bits(Geo G, Pool P, const unsigned* dmask, const unsigned* smask,
+                              uint8_t* out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.np) return;
+    const unsigned outer = 1u | (1u << (G.NL - 1));
+    uint8_t b = 0;
+    if (dmask[p] & outer) b |= 1;
+    if (smask[p] & outer) b |= 2;
+    if (P.pcnt[p * G.NL] > 0 || P.pcnt[p * G.NL + G.NL - 1] > 0) b |= 4;
+    if (P.page[p] >= 0) b |= 8;
+    out[p] = b;
+}
+
+}  // namespace
+
+#define HIPX(expr)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return felt_fail(FELT_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define DISPATCHP(D_, KERNEL, GRID, BLOCK, ...)                                                \
+    do {                                                                                       \
+        if ((D_) == 2) hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, st, __VA_ARGS__);          \
+        else hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, st, __VA_ARGS__);                    \
+    } while (0)
+
+void felt_mc_table_host(uint64_t* out) {
+    for (int i = 0; i < 256; i++) out[i] = kMcTri[i];
+}
+
+int felt_polys_upload_tables() {
+    static bool done = false;
+    if (done) return FELT_OK;
+    HIPX(hipMemcpyToSymbol(HIP_SYMBOL(c_mc_tri), kMcTri, sizeof(kMcTri)));
+    HIPX(hipMemcpyToSymbol(HIP_SYMBOL(c_ms_seg), kMsSeg, sizeof(kMsSeg)));
+    done = true;
+    return FELT_OK;
+}
+
+void Changes::track(int64_t p) {
+    if (look[p] != -1) return;
+    look[p] = (int64_t)list.size();
+    list.push_back(p);
+}
+void Changes::untrack(int64_t p) {
+    const int64_t at = look[p];
+    if (at == -1) return;
+    const int64_t last = (int64_t)list.size() - 1;
+    if (at < last) {
+        const int64_t m = list[last];
+        list[at] = m;
+        look[m] = at;
+    }
+    look[p] = -1;
+    list.pop_back();
+}
+void Changes::reset() {
+    for (int64_t p : list) look[p] = -1;
+    list.clear();
+}
+
+int felt_polys_init(felt_polys* q) {
+    const Geo& G = q->s->G;
+    q->kids.assign(G.np, PolyKid{});
+    q->a.look.assign(G.np, -1);
+    q->b.look.assign(G.np, -1);
+    q->pending = &q->a;
+    q->done = &q->b;
+    q->vw = G.D == 3 ? 6 : 2;
+    return felt_polys_upload_tables();
+}
+
+void felt_polys_release(felt_polys* q) {
+    q->varena.release();
+    q->tarena.release();
+    q->cmask.release();
+    q->own.release();
+    q->vbase.release();
+    q->tbase.release();
+    q->tot.release();
+    q->dparts.release();
+    q->dseg.release();
+    q->bits.release();
+}
+
+// notify (Polys.hpp:92-133).
+int felt_polys_notify_impl(felt_polys* q) {
+    felt_surface* s = q->s;
+    const Geo& G = s->G;
+    hipStream_t st = s->st;
+    int rc;
+    if ((rc = q->bits.ensure(G.np))) return rc;
+    hipLaunchKernelGGL(k_notify_bits, dim3((G.np + TPB - 1) / TPB), dim3(TPB), 0, st, G, s->P,
+                       s->dmask, s->smask, q->bits.ptr);
+    std::vector<uint8_t> bits(G.np);
+    HIPX(hipMemcpyAsync(bits.data(), q->bits.ptr, G.np, hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    for (int p = 0; p < G.np; p++) {
+        if (bits[p] & 1) {
+            const bool act = q->kids[p].active || (bits[p] & 4);
+            if (act) q->pending->track(p);
+            else q->pending->untrack(p);
+        }
+    }
+    for (int p = 0; p < G.np; p++)
+        if ((bits[p] & 2) && q->kids[p].active) q->pending->track(p);
+    return FELT_OK;
+}
+
+// invalidate (Polys.hpp:171-191): the isogrid's outermost-layer partitions.
+int felt_polys_invalidate_impl(felt_polys* q) {
+    felt_surface* s = q->s;
+    const Geo& G = s->G;
+    int rc;
+    if ((rc = q->bits.ensure(G.np))) return rc;
+    hipLaunchKernelGGL(k_notify_bits, dim3((G.np + TPB - 1) / TPB), dim3(TPB), 0, s->st, G, s->P,
+                       s->dmask, s->smask, q->bits.ptr);
+    std::vector<uint8_t> bits(G.np);
+    HIPX(hipMemcpyAsync(bits.data(), q->bits.ptr, G.np, hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    q->pending->reset();
+    for (int p = 0; p < G.np; p++)
+        if (bits[p] & 4) q->pending->track(p);
+    return FELT_OK;
+}
+
+// Makes room for `nv` more vertices and `nt` more simplices, compacting live segments into a
+// larger arena when the bump pointer would overflow.
+static int arena_reserve(felt_polys* q, long long nv, long long nt) {
+    hipStream_t st = q->s->st;
+    if (q->vtop + nv <= (long long)(q->varena.cap / q->vw) &&
+        q->ttop + nt <= (long long)(q->tarena.cap / q->s->G.D))
+        return FELT_OK;
+    long long live_v = 0, live_t = 0;
+    std::vector<long long> sv, dv, lv, stt, dt, lt;
+    for (PolyKid& k : q->kids) {
+        if (!k.active) continue;
+        if (k.nv) {
+            sv.push_back(k.voff * q->vw);
+            dv.push_back(live_v * q->vw);
+            lv.push_back(k.nv * q->vw);
+        }
+        if (k.nt) {
+            stt.push_back(k.toff * q->s->G.D);
+            dt.push_back(live_t * q->s->G.D);
+            lt.push_back(k.nt * q->s->G.D);
+        }
+        live_v += k.nv;
+        live_t += k.nt;
+    }
+    const long long cap_v = std::max<long long>(1024, 2 * (live_v + nv));
+    const long long cap_t = std::max<long long>(1024, 2 * (live_t + nt));
+    DBuf<float> nvb;
+    DBuf<uint32_t> ntb;
+    int rc;
+    if ((rc = nvb.ensure((size_t)cap_v * q->vw))) return rc;
+    if ((rc = ntb.ensure((size_t)cap_t * q->s->G.D))) { nvb.release(); return rc; }
+    auto copy = [&](std::vector<long long>& a, std::vector<long long>& b, std::vector<long long>& c,
+                    bool is_f) -> int {
+        if (a.empty()) return FELT_OK;
+        const size_t n = a.size();
+        long long* d = nullptr;
+        HIPX(hipMalloc(&d, 3 * n * sizeof(long long)));
+        HIPX(hipMemcpyAsync(d, a.data(), n * 8, hipMemcpyHostToDevice, st));
+        HIPX(hipMemcpyAsync(d + n, b.data(), n * 8, hipMemcpyHostToDevice, st));
+        HIPX(hipMemcpyAsync(d + 2 * n, c.data(), n * 8, hipMemcpyHostToDevice, st));
+        const int grid = (int)std::min<size_t>(n, 65535);
+        if (is_f)
+            hipLaunchKernelGGL(k_copy_segments, dim3(grid), dim3(TPB), 0, st, d, d + n, d + 2 * n,
+                               (int)n, q->varena.ptr, nvb.ptr);
+        else
+            hipLaunchKernelGGL(k_copy_segments_u, dim3(grid), dim3(TPB), 0, st, d, d + n,
+                               d + 2 * n, (int)n, q->tarena.ptr, ntb.ptr);
+        HIPX(hipStreamSynchronize(st));
+        HIPX(hipFree(d));
+        return FELT_OK;
+    };
+    if ((rc = copy(sv, dv, lv, true))) return rc;
+    if ((rc = copy(stt, dt, lt, false))) return rc;
+    long long cv = 0, ct = 0;
+    for (PolyKid& k : q->kids) {
+        if (!k.active) continue;
+        k.voff = cv;
+        k.toff = ct;
+        cv += k.nv;
+        ct += k.nt;
+    }
+    q->varena.release();
+    q->tarena.release();
+    q->varena = nvb;
+    q->tarena = ntb;
+    q->vtop = live_v;
+    q->ttop = live_t;
+    return FELT_OK;
+}
+
+// march (Polys.hpp:138-165).
+int felt_polys_march_impl(felt_polys* q) {
+    felt_surface* s = q->s;
+    const Geo& G = s->G;
+    hipStream_t st = s->st;
+    int rc;
+    std::vector<int> active(G.np);
+    HIPX(hipMemcpyAsync(active.data(), s->P.page, G.np * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPX(hipStreamSynchronize(st));
+    std::vector<int> todo;
+    for (int64_t p : q->pending->list) {
+        PolyKid& k = q->kids[p];
+        if (active[p] >= 0) {
+            k.active = true;
+            k.nv = k.nt = 0;
+            todo.push_back((int)p);
+        } else if (k.active) {
+            k = PolyKid{};
+        }
+    }
+    const size_t per = (size_t)G.pv * (1 + 2 + 4 + 4);
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(4096, (size_t)(96u << 20) / per));
+    q->last_spx = 0;
+    for (size_t b0 = 0; b0 < todo.size(); b0 += batch) {
+        const int nb = (int)std::min(batch, todo.size() - b0);
+        if ((rc = q->dparts.ensure(nb))) return rc;
+        if ((rc = q->cmask.ensure((size_t)nb * G.pv))) return rc;
+        if ((rc = q->own.ensure((size_t)nb * G.pv))) return rc;
+        if ((rc = q->vbase.ensure((size_t)nb * G.pv))) return rc;
+        if ((rc = q->tbase.ensure((size_t)nb * G.pv))) return rc;
+        if ((rc = q->tot.ensure((size_t)nb * 2))) return rc;
+        if ((rc = q->dseg.ensure((size_t)nb * 2))) return rc;
+        HIPX(hipMemcpyAsync(q->dparts.ptr, todo.data() + b0, nb * sizeof(int),
+                            hipMemcpyHostToDevice, st));
+        DISPATCHP(G.D, k_march_count, dim3(nb), dim3(TPB), G, s->P, q->dparts.ptr, q->cmask.ptr,
+                  q->own.ptr, q->vbase.ptr, q->tbase.ptr, q->tot.ptr);
+        std::vector<int> tot(nb * 2);
+        HIPX(hipMemcpyAsync(tot.data(), q->tot.ptr, nb * 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+        long long sum_v = 0, sum_t = 0;
+        for (int j = 0; j < nb; j++) {
+            sum_v += tot[2 * j];
+            sum_t += tot[2 * j + 1];
+        }
+        if ((rc = arena_reserve(q, sum_v, sum_t))) return rc;
+        std::vector<long long> seg(nb * 2);
+        for (int j = 0; j < nb; j++) {
+            PolyKid& k = q->kids[todo[b0 + j]];
+            k.voff = q->vtop;
+            k.nv = tot[2 * j];
+            k.toff = q->ttop;
+            k.nt = tot[2 * j + 1];
+            seg[j] = k.voff;
+            seg[nb + j] = k.toff;
+            q->vtop += k.nv;
+            q->ttop += k.nt;
+        }
+        q->last_spx += sum_t;
+        HIPX(hipMemcpyAsync(q->dseg.ptr, seg.data(), nb * 2 * sizeof(long long),
+                            hipMemcpyHostToDevice, st));
+        DISPATCHP(G.D, k_march_emit, dim3(nb), dim3(TPB), G, s->P, q->dparts.ptr, q->cmask.ptr,
+                  q->own.ptr, q->vbase.ptr, q->tbase.ptr, q->dseg.ptr, q->dseg.ptr + nb,
+                  q->varena.ptr, q->tarena.ptr);
+    }
+    HIPX(hipStreamSynchronize(st));
+    std::swap(q->pending, q->done);
+    q->pending->reset();
+    return FELT_OK;
+}
+
+int felt_polys_child_mesh_impl(felt_polys* q, uint64_t p, float* vtx, size_t vcap, uint32_t* spx,
+                               size_t scap) {
+    const PolyKid& k = q->kids[p];
+    const int D = q->s->G.D;
+    if (vcap < (size_t)k.nv * q->vw || scap < (size_t)k.nt * D)
+        return felt_fail(FELT_E_CAPACITY, "mesh buffers too small");
+    if (k.nv)
+        HIPX(hipMemcpyAsync(vtx, q->varena.ptr + k.voff * q->vw, k.nv * q->vw * sizeof(float),
+                            hipMemcpyDeviceToHost, q->s->st));
+    if (k.nt)
+        HIPX(hipMemcpyAsync(spx, q->tarena.ptr + k.toff * D, k.nt * D * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, q->s->st));
+    HIPX(hipStreamSynchronize(q->s->st));
+    return FELT_OK;
+}

@@ -1,0 +1,90 @@
+// felt_surface.h — host-side state of one device-resident surface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "felt_kernels.h"
+
+// Growable device buffer (contents are not preserved when it grows).
+template <class T>
+struct DBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n);
+    void release();
+};
+
+struct felt_surface {
+    felt::Geo G{};
+    felt::Pool P{};
+    hipStream_t st = nullptr;
+    int cap = 0;  // brick pool slots
+    int* free_stack = nullptr;
+    felt::Ctr* ctr = nullptr;
+    felt::Ctr hc{};  // host copy of the counters
+    uint8_t* amark = nullptr;
+    int *ukey = nullptr, *tflag = nullptr, *tlist = nullptr, *nflag = nullptr, *nlist = nullptr;
+    unsigned *dmask = nullptr, *smask = nullptr;
+    int* act = nullptr;
+    // band compaction
+    DBuf<uint32_t> band;
+    DBuf<float> band_d;
+    DBuf<int> blk_cnt, blk_off;
+    std::vector<int> band_base;
+    // bounded-update zero layer
+    DBuf<uint32_t> zlist;
+    DBuf<float> zdelta;
+    int z_host_n = 0;
+    // lookups: 0 = affected, 1 = affected buffer
+    DBuf<uint32_t> lu_v[2];
+    DBuf<int8_t> lu_tag[2];
+    DBuf<float> lu_d[2];
+    // status-change records
+    DBuf<uint32_t> sc_v;
+    DBuf<int8_t> sc_from;
+    DBuf<int16_t> sc_to;
+    // scratch
+    DBuf<float> vox;
+    bool vox_zeroed = false;
+    DBuf<int> tmp_i;
+    DBuf<float> tmp_f;
+    DBuf<uint32_t> tmp_u;
+    int64_t band_cap_hint = 0;
+    bool in_update = false;
+    std::vector<std::pair<std::array<int, 3>, float>> pending_deltas;
+    std::map<std::array<int, 3>, size_t> pending_index;
+    uint64_t last_updates = 0, last_rounds = 0;
+    struct felt_polys* polys = nullptr;
+};
+
+extern thread_local std::string g_felt_err;
+int felt_fail(int code, const std::string& msg);
+
+int felt_surface_init(felt_surface* s);
+void felt_surface_release(felt_surface* s);
+felt::Lu felt_lu(felt_surface* s, int i);
+int felt_compact_band(felt_surface* s);
+int felt_update_start_impl(felt_surface* s);
+int felt_update_global(felt_surface* s, const felt_speed* fn);
+int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
+int felt_update_box_impl(felt_surface* s, const int32_t* lo, const int32_t* hi,
+                         const felt_speed* fn);
+int felt_delta_impl(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
+int felt_update_end_public(felt_surface* s);
+int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::vector<float>& val,
+                      const std::vector<int>& layer);
+int felt_load_dense_impl(felt_surface* s, const float* phi, size_t n);
+int felt_snapshot_impl(felt_surface* s, float* dst, size_t total);
+int felt_get_impl(felt_surface* s, size_t n, const int32_t* gpos, float* out);
+int felt_collect_layer_impl(felt_surface* s, int layer, std::vector<int>& out);
+int felt_lu_positions_impl(felt_surface* s, int lu_i, int tag, std::vector<int>& out);
+int felt_band_total(felt_surface* s, size_t* out);
+int felt_download(felt_surface* s, void* dst, const void* src, size_t bytes);
+int felt_sync_impl(felt_surface* s);

@@ -1,0 +1,1340 @@
+// felt_surface.hip — sparse-field level-set surface on one MI355X (gfx950).
+//
+// Device-resident replacement for Felt::Surface<D,L> (Surface.hpp:49-1811): the partitioned
+// isogrid is a brick pool with a partition page table; the per-partition per-layer tracking
+// lists are compacted index arrays rebuilt by count / scan / scatter passes; each phase of
+// update (speed evaluation, zero-layer apply, layered distance transform, status-change flush,
+// narrow-band expansion, affected-set search) is a kernel over those lists.  Host code only
+// sequences kernels and reads a handful of counters.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "felt_kernels.h"
+#include "felt_surface.h"
+
+using namespace felt;
+
+namespace {
+
+constexpr int TPB = 256;
+
+__global__ void k_fill_i32(int* p, int64_t n, int v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_init_free(int* stack, int cap) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x)
+        stack[i] = cap - 1 - i;
+}
+
+// ---- compaction of the narrow band into per-layer lists -------------------------------------
+
+template <int D>
+__global__ void k_active_parts(Geo G, Pool P, int* out, int* n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = p < G.np && P.page[p] >= 0;
+    const int at = wave_append(n, act);
+    if (act) out[at] = p;
+}
+
+// Per block: counts of band voxels per layer in one chunk of one active partition.
+template <int D>
+__global__ void k_band_count(Geo G, Pool P, const int* act, int nchunk, int chunk, int* counts) {
+    __shared__ int c[32];
+    const int b = blockIdx.x;
+    const int p = act[b / nchunk];
+    const int s = P.page[p];
+    const int start = (b % nchunk) * chunk;
+    if (threadIdx.x < G.NL) c[threadIdx.x] = 0;
+    __syncthreads();
+    const int8_t* lay = P.lay + (size_t)s * G.pv;
+    for (int i = start + threadIdx.x; i < start + chunk && i < G.pv; i += blockDim.x) {
+        const int k = lay[i];
+        if (k != NONE) atomicAdd(&c[k + G.L], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < G.NL) counts[b * G.NL + threadIdx.x] = c[threadIdx.x];
+}
+
+// Single block: layer-major exclusive scan of per-block counts.  base[k] is where layer k's
+// segment starts; base[NL] is the band total.
+__global__ void k_band_scan(const int* counts, int nblocks, int NL, int* offsets, int* base) {
+    __shared__ int part[TPB * 17];
+    __shared__ int tot[18];
+    const int t = threadIdx.x;
+    const int seg = (nblocks + TPB - 1) / TPB;
+    const int b0 = t * seg, b1 = min(nblocks, b0 + seg);
+    int sums[17];
+    for (int k = 0; k < NL; k++) sums[k] = 0;
+    for (int b = b0; b < b1; b++)
+        for (int k = 0; k < NL; k++) sums[k] += counts[b * NL + k];
+    for (int k = 0; k < NL; k++) part[k * TPB + t] = sums[k];
+    __syncthreads();
+    if (t < NL) {
+        int run = 0;
+        for (int i = 0; i < TPB; i++) {
+            const int x = part[t * TPB + i];
+            part[t * TPB + i] = run;
+            run += x;
+        }
+        tot[t] = run;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int k = 0; k < NL; k++) {
+            base[k] = run;
+            run += tot[k];
+        }
+        base[NL] = run;
+        tot[17] = 0;
+    }
+    __syncthreads();
+    int run[17];
+    for (int k = 0; k < NL; k++) run[k] = base[k] + part[k * TPB + t];
+    for (int b = b0; b < b1; b++)
+        for (int k = 0; k < NL; k++) {
+            offsets[b * NL + k] = run[k];
+            run[k] += counts[b * NL + k];
+        }
+}
+
+template <int D>
+__global__ void k_band_scatter(Geo G, Pool P, const int* act, int nchunk, int chunk,
+                               const int* offsets, uint32_t* band) {
+    __shared__ int c[32];
+    const int b = blockIdx.x;
+    const int p = act[b / nchunk];
+    const int s = P.page[p];
+    const int start = (b % nchunk) * chunk;
+    if (threadIdx.x < G.NL) c[threadIdx.x] = offsets[b * G.NL + threadIdx.x];
+    __syncthreads();
+    const int8_t* lay = P.lay + (size_t)s * G.pv;
+    for (int i = start + threadIdx.x; i < start + chunk && i < G.pv; i += blockDim.x) {
+        const int k = lay[i];
+        if (k != NONE) {
+            const int at = atomicAdd(&c[k + G.L], 1);
+            band[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)i;
+        }
+    }
+}
+
+// ---- speed, apply, distance ----------------------------------------------------------------
+
+template <int D>
+__global__ void k_speed(Geo G, Pool P, const uint32_t* list, int n, Speed sp, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = list[i];
+    int slot, loc[3], g[3];
+    decode<D>(G, P, v, slot, loc, g);
+    out[i] = eval_speed<D>(G, P, sp, v, loc, g);
+}
+
+// Appends a status change and, when the target is in the band, a buffer entry
+// (status_change, Surface.hpp:1138-1178).  All lanes of the wave must call it.
+__device__ __forceinline__ void record_change(bool pred, uint32_t v, int from, int to, int L,
+                                              SC sc, int* sc_n, Lu buf, int* buf_n) {
+    const int at = wave_append(sc_n, pred);
+    if (pred) {
+        sc.v[at] = v;
+        sc.from[at] = (int8_t)from;
+        sc.to[at] = (int16_t)to;
+    }
+    const bool inb = pred && to >= -L && to <= L;
+    const int bt = wave_append(buf_n, inb);
+    if (inb) {
+        buf.v[bt] = v;
+        buf.tag[bt] = (int8_t)to;
+    }
+}
+
+// update_zero_layer (Surface.hpp:920-979): phi += delta; a layer change is recorded but does
+// not raise the round flag.
+template <int D>
+__global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta, int n, SC sc,
+                        Lu buf, Ctr* ctr, int buf_i) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t v = 0;
+    int k = 0;
+    if (valid) {
+        v = list[i];
+        const float nv = P.phi[v] + delta[i];
+        k = layer_of(nv, G.L, &ctr->nan_events);
+        P.phi[v] = nv;
+    }
+    record_change(valid && k != 0, v, 0, k, G.L, sc, &ctr->sc_n, buf, &ctr->lu_n[buf_i]);
+}
+
+// update_distance pass 1 (Surface.hpp:1029-1073): d = distance() for every list entry of the
+// layer; tag == nullptr means the list is already one layer's segment.
+template <int D>
+__global__ void k_dist1(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
+                        float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (tag && tag[i] != layer) return;
+    const uint32_t v = list[i];
+    int slot, loc[3], g[3];
+    decode<D>(G, P, v, slot, loc, g);
+    const float side = layer < 0 ? -1.0f : 1.0f;
+    out[i] = distance<D>(G, P, v, loc, g, P.phi[v], side);
+}
+
+// update_distance pass 2 (Surface.hpp:1076-1121): phi = d and status change.
+template <int D>
+__global__ void k_dist2(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
+                        const float* d, SC sc, Lu buf, Ctr* ctr, int buf_i) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n && (tag == nullptr || tag[i] == layer);
+    uint32_t v = 0;
+    int k = layer;
+    if (valid) {
+        v = list[i];
+        const float x = d[i];
+        k = layer_of(x, G.L, &ctr->nan_events);
+        P.phi[v] = x;
+    }
+    const bool ch = valid && k != layer;
+    if (__ballot(ch) != 0ull && ch) ctr->flag = 1;
+    record_change(ch, v, layer, k, G.L, sc, &ctr->sc_n, buf, &ctr->lu_n[buf_i]);
+}
+
+// ---- flush, deactivation, expansion ---------------------------------------------------------
+
+// flush_status_change (Surface.hpp:1183-1244) with Untrack::retrack / untrack
+// (PartitionedMixin.hpp:489-558).  A partition that loses voxels to untrack is queued for the
+// emptiness check; its background becomes Distance(to) of the untrack that comes last in the
+// reference's flush order (highest from-layer).
+template <int D>
+__global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, int* ukey, int* tflag, int* tlist,
+                        Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = ctrc->sc_n;
+    const bool valid = i < n;
+    bool first = false;
+    int p = 0;
+    if (valid) {
+        const uint32_t v = sc.v[i];
+        const int from = sc.from[i], to = sc.to[i];
+        const int slot = (int)(v / (uint32_t)G.pv);
+        p = P.slot_part[slot];
+        if (to >= -G.L && to <= G.L) {
+            P.lay[v] = (int8_t)to;
+            atomicSub(&P.pcnt[p * G.NL + from + G.L], 1);
+            atomicAdd(&P.pcnt[p * G.NL + to + G.L], 1);
+        } else {
+            P.lay[v] = NONE;
+            P.phi[v] = (float)to;
+            atomicSub(&P.pcnt[p * G.NL + from + G.L], 1);
+            atomicMax(&ukey[p], ((from + G.L) << 16) | ((to + 32768) & 0xffff));
+            first = atomicExch(&tflag[p], 1) == 0;
+        }
+    }
+    const int at = wave_append(&ctr->touched_n, first);
+    if (first) tlist[at] = p;
+}
+
+__global__ void k_deactivate(Geo G, Pool P, const int* tlist, const Ctr* ctrc, int* ukey,
+                             int* tflag, int* free_stack, Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctrc->touched_n) return;
+    const int p = tlist[i];
+    tflag[p] = 0;
+    const int key = ukey[p];
+    ukey[p] = -1;
+    int sum = 0;
+    for (int k = 0; k < G.NL; k++) sum += P.pcnt[p * G.NL + k];
+    const int s = P.page[p];
+    if (sum == 0 && s >= 0) {
+        P.page[p] = -1;
+        P.slot_part[s] = -1;
+        P.pbg[p] = (float)((key & 0xffff) - 32768);
+        const int t = atomicAdd(&ctr->free_top, 1);
+        free_stack[t] = s;
+    }
+}
+
+// expand_narrow_band (Surface.hpp:1249-1355), phase 1: queue inactive partitions that an
+// expanding voxel's out-of-band neighbours fall in.
+template <int D>
+__global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag, int* nlist,
+                              Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = ctrc->sc_n;
+    bool cand = false;
+    int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+    if (i < n) {
+        const int from = sc.from[i], to = sc.to[i];
+        const int side = from < 0 ? -1 : 1;
+        if ((from == G.L || from == -G.L) && to == G.L * side - side) {
+            cand = true;
+            decode<D>(G, P, sc.v[i], slot, loc, g);
+        }
+    }
+    for (int a = 0; a < D; a++) {
+        for (int s = -1; s <= 1; s += 2) {
+            bool q_new = false;
+            int pq = 0;
+            if (cand) {
+                int q[3] = {g[0], g[1], g[2]};
+                q[a] += s;
+                if (inside<D>(G, q)) {
+                    pq = part_of<D>(G, q);
+                    if (P.page[pq] < 0 && !in_band(layer_of(P.pbg[pq], G.L, &ctr->nan_events), G.L))
+                        q_new = atomicExch(&nflag[pq], 1) == 0;
+                }
+            }
+            const int at = wave_append(&ctr->need_n, q_new);
+            if (q_new) nlist[at] = pq;
+        }
+    }
+}
+
+// Partition activation from the free-slot stack: the brick is filled with the partition's
+// background (GridMixin.hpp:75-81) and no voxel is tracked.
+__global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* nflag,
+                        int* free_stack, Ctr* ctr) {
+    __shared__ int slot_sh;
+    const int n = ctrc->need_n;
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        const int p = nlist[j];
+        if (threadIdx.x == 0) {
+            const int t = atomicSub(&ctr->free_top, 1) - 1;
+            int s = -1;
+            if (t >= 0) s = free_stack[t];
+            else {
+                atomicAdd(&ctr->free_top, 1);
+                ctr->err = 1;
+            }
+            slot_sh = s;
+            if (s >= 0) {
+                P.page[p] = s;
+                P.slot_part[s] = p;
+            }
+            nflag[p] = 0;
+        }
+        __syncthreads();
+        const int s = slot_sh;
+        if (s >= 0) {
+            const float bg = P.pbg[p];
+            float* ph = P.phi + (size_t)s * G.pv;
+            int8_t* ly = P.lay + (size_t)s * G.pv;
+            for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
+                ph[i] = bg;
+                ly[i] = NONE;
+            }
+            if (threadIdx.x < G.NL) P.pcnt[p * G.NL + threadIdx.x] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// expand_narrow_band phase 2: claim each out-of-band neighbour once and track it in layer
+// +-L with its distance (the value does not depend on which expanding voxel claims it).
+template <int D>
+__global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctrc->sc_n) return;
+    const int from = sc.from[i], to = sc.to[i];
+    const int side = from < 0 ? -1 : 1;
+    if (!((from == G.L || from == -G.L) && to == G.L * side - side)) return;
+    int slot, loc[3], g[3];
+    decode<D>(G, P, sc.v[i], slot, loc, g);
+    const int8_t target = (int8_t)(G.L * side);
+    for (int a = 0; a < D; a++) {
+        for (int s = -1; s <= 1; s += 2) {
+            int q[3] = {g[0], g[1], g[2]};
+            q[a] += s;
+            if (!inside<D>(G, q)) continue;
+            const int pq = part_of<D>(G, q);
+            const int sq = P.page[pq];
+            if (sq < 0) continue;  // activation failed (pool exhausted, flagged)
+            const uint32_t vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, q);
+            const float dq = __hip_atomic_load(&P.phi[vq], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+            if (in_band(layer_of(dq, G.L, &ctr->nan_events), G.L)) continue;
+            if (!cas_byte(&P.lay[vq], NONE, target)) continue;
+            int lq[3];
+            for (int b = 0; b < D; b++) lq[b] = q[b] % G.part[b];
+            const float nd = distance<D>(G, P, vq, lq, q, dq, (float)side);
+            __hip_atomic_store(&P.phi[vq], nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicAdd(&P.pcnt[pq * G.NL + target + G.L], 1);
+        }
+    }
+}
+
+// Partitions holding status changes from each layer this round (status_change(k) lists).
+__global__ void k_smask(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctrc->sc_n) return;
+    const int slot = (int)(sc.v[i] / (uint32_t)G.pv);
+    const int p = P.slot_part[slot];
+    atomicOr(&smask[p], 1u << (sc.from[i] + G.L));
+}
+
+__global__ void k_dmask_pcnt(Geo G, Pool P, unsigned* dmask) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= G.np) return;
+    unsigned m = 0;
+    for (int k = 0; k < G.NL; k++)
+        if (P.pcnt[p * G.NL + k] > 0) m |= 1u << k;
+    dmask[p] = m;
+}
+
+__global__ void k_dmask_lu(Geo G, Pool P, Lu lu, const int* n, unsigned* dmask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n) return;
+    const int slot = (int)(lu.v[i] / (uint32_t)G.pv);
+    atomicOr(&dmask[P.slot_part[slot]], 1u << (lu.tag[i] + G.L));
+}
+
+// ---- bounded update: zero layer in a box, affected-set search -------------------------------
+
+template <int D>
+__global__ void k_box_zero(Geo G, Pool P, const int* parts, int nparts, int lo0, int lo1, int lo2,
+                           int hi0, int hi1, int hi2, uint32_t* z, Ctr* ctr) {
+    const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    for (int j = blockIdx.x; j < nparts; j += gridDim.x) {
+        const int p = parts[j];
+        const int s = P.page[p];
+        if (s < 0) continue;
+        int pc[3], rem = p;
+        for (int a = D - 1; a >= 0; a--) {
+            pc[a] = rem % G.csize[a];
+            rem /= G.csize[a];
+        }
+        for (int base = 0; base < G.pv; base += blockDim.x) {
+            const int l = base + threadIdx.x;
+            bool take = false;
+            if (l < G.pv && P.lay[(size_t)s * G.pv + l] == 0) {
+                take = true;
+                int r = l;
+                for (int a = D - 1; a >= 0; a--) {
+                    const int la = r % G.part[a];
+                    r /= G.part[a];
+                    const int ga = pc[a] * G.part[a] + la;
+                    if (ga < lo[a] || ga >= hi[a]) take = false;
+                }
+            }
+            const int at = wave_append(&ctr->z_n, take);
+            if (take) z[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)l;
+        }
+    }
+}
+
+__global__ void k_lu_seed(Lu lu, const uint32_t* z, int n, uint8_t* amark, int* lu_n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    lu.v[i] = z[i];
+    lu.tag[i] = 0;
+    amark[z[i]] = 1;
+    if (i == 0) *lu_n = n;
+}
+
+__global__ void k_snap_front(Ctr* ctr, int lu_i, int j) { ctr->front[j] = ctr->lu_n[lu_i]; }
+
+// calc_affected (Surface.hpp:780-913), one breadth-first round: face neighbours of the last
+// round's voxels whose current layer is in the band join the set once.
+template <int D>
+__global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j,
+                                 uint8_t* amark) {
+    const int lo = ctr->front[j - 1], hi = ctr->front[j];
+    const int n = hi - lo;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const int i = base + threadIdx.x;
+        const bool valid = i < n;
+        int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        if (valid) decode<D>(G, P, lu.v[lo + i], slot, loc, g);
+        for (int a = 0; a < D; a++) {
+            for (int s = -1; s <= 1; s += 2) {
+                bool add = false;
+                uint32_t vq = 0;
+                int kq = 0;
+                if (valid) {
+                    int q[3] = {g[0], g[1], g[2]};
+                    q[a] += s;
+                    if (inside<D>(G, q)) {
+                        const int pq = part_of<D>(G, q);
+                        const int sq = P.page[pq];
+                        const float dq = sq < 0 ? P.pbg[pq]
+                                                : P.phi[(size_t)sq * G.pv + leaf_of<D>(G, q)];
+                        kq = layer_of(dq, G.L, &ctr->nan_events);
+                        if (in_band(kq, G.L) && sq >= 0) {
+                            vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, q);
+                            add = cas_byte((int8_t*)&amark[vq], 0, 1);
+                        }
+                    }
+                }
+                const int at = wave_append(&ctr->lu_n[lu_i], add);
+                if (add) {
+                    lu.v[at] = vq;
+                    lu.tag[at] = (int8_t)kq;
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_unmark(Lu lu, const int* n, uint8_t* amark) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *n) amark[lu.v[i]] = 0;
+}
+
+// ---- seeding, loading, read-back ------------------------------------------------------------
+
+template <int D>
+__global__ void k_mark_parts(Geo G, Pool P, const int* gpos, int n, int* nflag, int* nlist,
+                             Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool q_new = false;
+    int p = 0;
+    if (i < n) {
+        int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
+        p = part_of<D>(G, q);
+        if (P.page[p] < 0) q_new = atomicExch(&nflag[p], 1) == 0;
+    }
+    const int at = wave_append(&ctr->need_n, q_new);
+    if (q_new) nlist[at] = p;
+}
+
+// Seed / track: phi = value always, list membership only when untracked (first-list-wins,
+// LookupMixin.hpp:298-331; TrackedMixin.hpp ByValue::track).
+template <int D>
+__global__ void k_track_points(Geo G, Pool P, const int* gpos, const float* val, const int* layer,
+                               int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
+    const int p = part_of<D>(G, q);
+    const int s = P.page[p];
+    if (s < 0) return;
+    const size_t v = (size_t)s * G.pv + leaf_of<D>(G, q);
+    P.phi[v] = val[i];
+    if (P.lay[v] == NONE) {
+        P.lay[v] = (int8_t)layer[i];
+        atomicAdd(&P.pcnt[p * G.NL + layer[i] + G.L], 1);
+    }
+}
+
+// Dense load: per partition, decide whether it holds a band voxel, and the background of the
+// ones that do not (sign of the lowest-corner voxel).
+template <int D>
+__global__ void k_load_scan(Geo G, Pool P, const float* dense, int* has, Ctr* ctr) {
+    __shared__ int any;
+    const int p = blockIdx.x;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    int pc[3] = {0, 0, 0}, rem = p;
+    for (int a = D - 1; a >= 0; a--) {
+        pc[a] = rem % G.csize[a];
+        rem /= G.csize[a];
+    }
+    bool first_found = false;
+    for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
+        int r = l, q[3] = {0, 0, 0};
+        for (int a = D - 1; a >= 0; a--) {
+            q[a] = pc[a] * G.part[a] + r % G.part[a];
+            r /= G.part[a];
+        }
+        if (!inside<D>(G, q)) continue;
+        size_t idx = 0;
+        for (int a = 0; a < D; a++) idx = idx * G.size[a] + q[a];
+        if (in_band(layer_of(dense[idx], G.L, &ctr->nan_events), G.L)) any = 1;
+        (void)first_found;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        has[p] = any;
+        int q[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) q[a] = pc[a] * G.part[a];
+        if (inside<D>(G, q)) {
+            size_t idx = 0;
+            for (int a = 0; a < D; a++) idx = idx * G.size[a] + q[a];
+            P.pbg[p] = dense[idx] < 0.0f ? -(float)(G.L + 1) : (float)(G.L + 1);
+        }
+    }
+}
+
+__global__ void k_list_from_flags(const int* has, int np, int* nlist, int* nflag, Ctr* ctr) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool take = p < np && has[p];
+    const int at = wave_append(&ctr->need_n, take);
+    if (take) {
+        nlist[at] = p;
+        nflag[p] = 1;
+    }
+}
+
+template <int D>
+__global__ void k_load_fill(Geo G, Pool P, const float* dense, const int* has, Ctr* ctr) {
+    const int p = blockIdx.x;
+    if (!has[p]) return;
+    const int s = P.page[p];
+    if (s < 0) return;
+    int pc[3] = {0, 0, 0}, rem = p;
+    for (int a = D - 1; a >= 0; a--) {
+        pc[a] = rem % G.csize[a];
+        rem /= G.csize[a];
+    }
+    for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
+        int r = l, q[3] = {0, 0, 0};
+        for (int a = D - 1; a >= 0; a--) {
+            q[a] = pc[a] * G.part[a] + r % G.part[a];
+            r /= G.part[a];
+        }
+        if (!inside<D>(G, q)) continue;
+        size_t idx = 0;
+        for (int a = 0; a < D; a++) idx = idx * G.size[a] + q[a];
+        const float x = dense[idx];
+        const size_t v = (size_t)s * G.pv + l;
+        P.phi[v] = x;
+        const int k = layer_of(x, G.L, &ctr->nan_events);
+        if (in_band(k, G.L)) {
+            P.lay[v] = (int8_t)k;
+            atomicAdd(&P.pcnt[p * G.NL + k + G.L], 1);
+        }
+    }
+}
+
+template <int D>
+__global__ void k_snapshot(Geo G, Pool P, float* out, int64_t total) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int q[3] = {0, 0, 0};
+        int64_t r = i;
+        for (int a = D - 1; a >= 0; a--) {
+            q[a] = (int)(r % G.size[a]);
+            r /= G.size[a];
+        }
+        out[i] = get_g<D>(G, P, q);
+    }
+}
+
+template <int D>
+__global__ void k_get(Geo G, Pool P, const int* gpos, int n, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
+    out[i] = get_g<D>(G, P, q);
+}
+
+// Positions of voxels tracked in `layer` (tag filter) over all active partitions.
+template <int D>
+__global__ void k_collect_layer(Geo G, Pool P, int layer, int* out, int cap, int* n) {
+    const int p = blockIdx.x;
+    const int s = P.page[p];
+    if (s < 0) return;
+    int pc[3] = {0, 0, 0}, rem = p;
+    for (int a = D - 1; a >= 0; a--) {
+        pc[a] = rem % G.csize[a];
+        rem /= G.csize[a];
+    }
+    for (int base = 0; base < G.pv; base += blockDim.x) {
+        const int l = base + threadIdx.x;
+        const bool take = l < G.pv && P.lay[(size_t)s * G.pv + l] == layer;
+        const int at = wave_append(n, take);
+        if (take && at < cap) {
+            int r = l;
+            for (int a = D - 1; a >= 0; a--) {
+                out[at * 3 + a] = pc[a] * G.part[a] + r % G.part[a] + G.offset[a];
+                r /= G.part[a];
+            }
+        }
+    }
+}
+
+template <int D>
+__global__ void k_lu_positions(Geo G, Pool P, Lu lu, const int* n, int tag, int* out, int cap,
+                               int* cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool take = i < *n && lu.tag[i] == tag;
+    int slot = 0, loc[3], g[3] = {0, 0, 0};
+    if (take) decode<D>(G, P, lu.v[i], slot, loc, g);
+    const int at = wave_append(cnt, take);
+    if (take && at < cap)
+        for (int a = 0; a < D; a++) out[at * 3 + a] = g[a] + G.offset[a];
+}
+
+template <int D>
+__global__ void k_voxel_ids(Geo G, Pool P, const int* gpos, int n, uint32_t* out, int* bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
+    const int p = part_of<D>(G, q);
+    const int s = P.page[p];
+    if (s < 0) {
+        atomicAdd(bad, 1);
+        out[i] = 0;
+        return;
+    }
+    const uint32_t v = (uint32_t)s * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, q);
+    if (P.lay[v] != 0) atomicAdd(bad, 1);
+    out[i] = v;
+}
+
+__global__ void k_vox_set(const uint32_t* ids, const float* vals, int m, float* vox) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) vox[ids[i]] = vals[i];
+}
+__global__ void k_vox_gather(const uint32_t* list, int n, float* vox, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        out[i] = vox[list[i]];
+        vox[list[i]] = 0.0f;
+    }
+}
+
+__global__ void k_band_total(Geo G, Pool P, int* out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = 0;
+    if (p < G.np)
+        for (int k = 0; k < G.NL; k++) s += P.pcnt[p * G.NL + k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
+inline int blocks_for(int64_t n) { return (int)std::max<int64_t>(1, (n + TPB - 1) / TPB); }
+
+}  // namespace
+
+// =============================================================================================
+// Host side
+// =============================================================================================
+
+thread_local std::string g_felt_err;
+
+int felt_fail(int code, const std::string& msg) {
+    g_felt_err = msg;
+    return code;
+}
+
+#define HIPX(expr)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return felt_fail(FELT_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int DBuf<T>::ensure(size_t n) {
+    if (n <= cap) return FELT_OK;
+    size_t nc = std::max(n, cap + cap / 2);
+    nc = std::max<size_t>(nc, 256);
+    if (ptr) HIPX(hipFree(ptr));
+    ptr = nullptr;
+    cap = 0;
+    HIPX(hipMalloc(&ptr, nc * sizeof(T)));
+    cap = nc;
+    return FELT_OK;
+}
+template <class T>
+void DBuf<T>::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+template struct DBuf<uint32_t>;
+template struct DBuf<int8_t>;
+template struct DBuf<int16_t>;
+template struct DBuf<float>;
+template struct DBuf<int>;
+template struct DBuf<long long>;
+template struct DBuf<uint16_t>;
+template struct DBuf<uint8_t>;
+
+#define DISPATCH(D_, KERNEL, GRID, BLOCK, ...)                                                 \
+    do {                                                                                       \
+        if ((D_) == 2) hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, s->st, __VA_ARGS__);      \
+        else hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, s->st, __VA_ARGS__);                \
+    } while (0)
+#define LAUNCH(KERNEL, GRID, BLOCK, ...) hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, s->st, __VA_ARGS__)
+
+static int read_ctr(felt_surface* s) {
+    HIPX(hipMemcpyAsync(&s->hc, s->ctr, sizeof(Ctr), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    if (s->hc.err) return felt_fail(FELT_E_NOMEM, "brick pool exhausted");
+    return FELT_OK;
+}
+
+static int zero_ctr_field(felt_surface* s, int* field) {
+    HIPX(hipMemsetAsync(field, 0, sizeof(int), s->st));
+    return FELT_OK;
+}
+
+int felt_surface_init(felt_surface* s) {
+    Geo& G = s->G;
+    HIPX(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    const int64_t np = G.np, cap = s->cap;
+    HIPX(hipMalloc(&s->P.page, np * sizeof(int)));
+    HIPX(hipMalloc(&s->P.pbg, np * sizeof(float)));
+    HIPX(hipMalloc(&s->P.pcnt, np * G.NL * sizeof(int)));
+    HIPX(hipMalloc(&s->P.slot_part, cap * sizeof(int)));
+    HIPX(hipMalloc(&s->P.phi, cap * (int64_t)G.pv * sizeof(float)));
+    HIPX(hipMalloc(&s->P.lay, cap * (int64_t)G.pv));
+    HIPX(hipMalloc(&s->amark, cap * (int64_t)G.pv));
+    HIPX(hipMalloc(&s->free_stack, cap * sizeof(int)));
+    HIPX(hipMalloc(&s->ctr, sizeof(Ctr)));
+    HIPX(hipMalloc(&s->ukey, np * sizeof(int)));
+    HIPX(hipMalloc(&s->tflag, np * sizeof(int)));
+    HIPX(hipMalloc(&s->tlist, np * sizeof(int)));
+    HIPX(hipMalloc(&s->nflag, np * sizeof(int)));
+    HIPX(hipMalloc(&s->nlist, np * sizeof(int)));
+    HIPX(hipMalloc(&s->dmask, np * sizeof(unsigned)));
+    HIPX(hipMalloc(&s->smask, np * sizeof(unsigned)));
+    HIPX(hipMalloc(&s->act, np * sizeof(int)));
+    const int gb = 1024;
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->P.page, np, -1);
+    float bg = (float)(G.L + 1);
+    int bgi;
+    std::memcpy(&bgi, &bg, 4);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->P.pbg, np, bgi);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->P.pcnt, np * G.NL, 0);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->P.slot_part, cap, -1);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->ukey, np, -1);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->tflag, np, 0);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->nflag, np, 0);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->dmask, np, 0);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->smask, np, 0);
+    HIPX(hipMemsetAsync(s->P.lay, 0x80, cap * (int64_t)G.pv, s->st));
+    HIPX(hipMemsetAsync(s->amark, 0, cap * (int64_t)G.pv, s->st));
+    LAUNCH(k_init_free, dim3(gb), dim3(TPB), s->free_stack, (int)cap);
+    Ctr c;
+    std::memset(&c, 0, sizeof(c));
+    c.free_top = (int)cap;
+    HIPX(hipMemcpyAsync(s->ctr, &c, sizeof(Ctr), hipMemcpyHostToDevice, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}
+
+void felt_surface_release(felt_surface* s) {
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
+                    s->amark, s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag,
+                    s->nlist, s->dmask, s->smask, s->act};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    s->band.release();
+    s->band_d.release();
+    s->blk_cnt.release();
+    s->blk_off.release();
+    s->zlist.release();
+    s->zdelta.release();
+    for (int i = 0; i < 2; i++) {
+        s->lu_v[i].release();
+        s->lu_tag[i].release();
+        s->lu_d[i].release();
+    }
+    s->sc_v.release();
+    s->sc_from.release();
+    s->sc_to.release();
+    s->vox.release();
+    s->tmp_i.release();
+    s->tmp_f.release();
+    s->tmp_u.release();
+    if (s->st) (void)hipStreamDestroy(s->st);
+    s->st = nullptr;
+}
+
+Lu felt_lu(felt_surface* s, int i) { return Lu{s->lu_v[i].ptr, s->lu_tag[i].ptr, s->lu_d[i].ptr}; }
+static SC sc_of(felt_surface* s) { return SC{s->sc_v.ptr, s->sc_from.ptr, s->sc_to.ptr}; }
+
+// Capacity of every per-step list: no list in a step holds more entries than the band had at
+// its start.
+static int ensure_step_capacity(felt_surface* s, size_t band) {
+    const size_t n = band + 64;
+    int rc;
+    for (int i = 0; i < 2; i++) {
+        if ((rc = s->lu_v[i].ensure(n))) return rc;
+        if ((rc = s->lu_tag[i].ensure(n))) return rc;
+        if ((rc = s->lu_d[i].ensure(n))) return rc;
+    }
+    if ((rc = s->sc_v.ensure(n))) return rc;
+    if ((rc = s->sc_from.ensure(n))) return rc;
+    if ((rc = s->sc_to.ensure(n))) return rc;
+    if ((rc = s->zlist.ensure(n))) return rc;
+    if ((rc = s->zdelta.ensure(n))) return rc;
+    return FELT_OK;
+}
+
+// Band compaction: per-layer segments of the band list, s->band_base[k]..[k+1].
+int felt_compact_band(felt_surface* s) {
+    const Geo& G = s->G;
+    int rc;
+    HIPX(hipMemsetAsync(&s->ctr->act_n, 0, sizeof(int), s->st));
+    DISPATCH(G.D, k_active_parts, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->act,
+             &s->ctr->act_n);
+    if ((rc = read_ctr(s))) return rc;
+    const int nact = s->hc.act_n;
+    const int chunk = std::min(G.pv, 4096);
+    const int nchunk = (G.pv + chunk - 1) / chunk;
+    const int nblocks = nact * nchunk;
+    s->band_base.assign(G.NL + 1, 0);
+    if (nblocks == 0) return FELT_OK;
+    if ((rc = s->blk_cnt.ensure((size_t)nblocks * G.NL + G.NL + 1))) return rc;
+    if ((rc = s->blk_off.ensure((size_t)nblocks * G.NL))) return rc;
+    DISPATCH(G.D, k_band_count, dim3(nblocks), dim3(TPB), G, s->P, s->act, nchunk, chunk,
+             s->blk_cnt.ptr);
+    int* base = s->blk_cnt.ptr + (size_t)nblocks * G.NL;
+    LAUNCH(k_band_scan, dim3(1), dim3(TPB), s->blk_cnt.ptr, nblocks, G.NL, s->blk_off.ptr, base);
+    HIPX(hipMemcpyAsync(s->band_base.data(), base, (G.NL + 1) * sizeof(int),
+                        hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    const size_t total = (size_t)s->band_base[G.NL];
+    if ((rc = s->band.ensure(total + 64))) return rc;
+    if ((rc = s->band_d.ensure(total + 64))) return rc;
+    DISPATCH(G.D, k_band_scatter, dim3(nblocks), dim3(TPB), G, s->P, s->act, nchunk, chunk,
+             s->blk_off.ptr, s->band.ptr);
+    return FELT_OK;
+}
+
+// status_change partition sets are those of the last round only; the buffer `buf_i` collects
+// this round's in-band targets.
+static int distance_round(felt_surface* s, bool from_band, int look_i, int look_n, int buf_i) {
+    const Geo& G = s->G;
+    const SC sc = sc_of(s);
+    const Lu buf = felt_lu(s, buf_i);
+    for (int pass = 0; pass < 2; pass++) {
+        const int side = pass == 0 ? -1 : 1;
+        for (int m = 1; m <= G.L; m++) {
+            const int layer = side * m;
+            const uint32_t* list;
+            const int8_t* tag;
+            float* d;
+            int n;
+            if (from_band) {
+                const int k = layer + G.L;
+                list = s->band.ptr + s->band_base[k];
+                tag = nullptr;
+                d = s->band_d.ptr + s->band_base[k];
+                n = s->band_base[k + 1] - s->band_base[k];
+            } else {
+                list = s->lu_v[look_i].ptr;
+                tag = s->lu_tag[look_i].ptr;
+                d = s->lu_d[look_i].ptr;
+                n = look_n;
+            }
+            if (n == 0) continue;
+            DISPATCH(G.D, k_dist1, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d);
+            DISPATCH(G.D, k_dist2, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d,
+                     sc, buf, s->ctr, buf_i);
+        }
+    }
+    return FELT_OK;
+}
+
+// flush_status_change then expand_narrow_band, sized by the status-change count bound.
+static int flush_expand(felt_surface* s, int sc_bound) {
+    const Geo& G = s->G;
+    const SC sc = sc_of(s);
+    if (sc_bound <= 0) return FELT_OK;
+    const int nb = blocks_for(sc_bound);
+    LAUNCH(k_smask, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->smask);
+    HIPX(hipMemsetAsync(&s->ctr->touched_n, 0, sizeof(int), s->st));
+    DISPATCH(G.D, k_flush, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag, s->tlist,
+             s->ctr);
+    LAUNCH(k_deactivate, dim3(blocks_for(std::min<int64_t>(G.np, sc_bound))), dim3(TPB), G, s->P,
+           s->tlist, s->ctr, s->ukey, s->tflag, s->free_stack, s->ctr);
+    HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
+    DISPATCH(G.D, k_expand_mark, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->nflag, s->nlist,
+             s->ctr);
+    LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
+           s->free_stack, s->ctr);
+    DISPATCH(G.D, k_expand_claim, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ctr);
+    return FELT_OK;
+}
+
+static int reset_round(felt_surface* s) {
+    HIPX(hipMemsetAsync(&s->ctr->sc_n, 0, sizeof(int), s->st));
+    HIPX(hipMemsetAsync(&s->ctr->flag, 0, sizeof(int), s->st));
+    HIPX(hipMemsetAsync(s->smask, 0, s->G.np * sizeof(unsigned), s->st));
+    return FELT_OK;
+}
+
+// update_start (Surface.hpp:393-399): delta, affected, buffer and status-change state cleared.
+int felt_update_start_impl(felt_surface* s) {
+    HIPX(hipMemsetAsync(s->dmask, 0, s->G.np * sizeof(unsigned), s->st));
+    HIPX(hipMemsetAsync(&s->ctr->lu_n[0], 0, 2 * sizeof(int), s->st));
+    HIPX(hipMemsetAsync(&s->ctr->z_n, 0, sizeof(int), s->st));
+    int rc = reset_round(s);
+    if (rc) return rc;
+    s->z_host_n = 0;
+    s->in_update = true;
+    s->pending_deltas.clear();
+    s->pending_index.clear();
+    return FELT_OK;
+}
+
+// Global step (Surface.hpp:254-296, 739-762).  `delta_fn` fills s->band_d over the zero-layer
+// segment.
+template <class DeltaFn>
+static int global_step(felt_surface* s, DeltaFn&& delta_fn) {
+    const Geo& G = s->G;
+    int rc;
+    if ((rc = felt_update_start_impl(s))) return rc;
+    if ((rc = felt_compact_band(s))) return rc;
+    const size_t band = (size_t)s->band_base[G.NL];
+    if ((rc = ensure_step_capacity(s, band))) return rc;
+    LAUNCH(k_dmask_pcnt, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->dmask);
+    const int z0 = s->band_base[G.L], zn = s->band_base[G.L + 1] - z0;
+    // update_zero_layer into the affected buffer (index 1)
+    if (zn > 0) {
+        if ((rc = delta_fn(s->band.ptr + z0, zn, s->band_d.ptr + z0))) return rc;
+        DISPATCH(G.D, k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->band.ptr + z0,
+                 s->band_d.ptr + z0, zn, sc_of(s), felt_lu(s, 1), s->ctr, 1);
+    }
+    if ((rc = distance_round(s, true, 0, 0, 1))) return rc;
+    if ((rc = flush_expand(s, (int)band))) return rc;
+    if ((rc = read_ctr(s))) return rc;
+    int rounds = 1;
+    if (s->hc.flag) {
+        int look = 1, buf = 0;
+        while (true) {
+            if ((rc = reset_round(s))) return rc;
+            const int look_n = s->hc.lu_n[look];
+            if ((rc = distance_round(s, false, look, look_n, buf))) return rc;
+            if ((rc = flush_expand(s, look_n))) return rc;
+            HIPX(hipMemsetAsync(&s->ctr->lu_n[look], 0, sizeof(int), s->st));
+            std::swap(look, buf);
+            rounds++;
+            if ((rc = read_ctr(s))) return rc;
+            if (!s->hc.flag) break;
+        }
+    }
+    s->last_updates = band;
+    s->last_rounds = rounds;
+    s->in_update = false;
+    return FELT_OK;
+}
+
+int felt_update_global(felt_surface* s, const felt_speed* fn) {
+    Speed sp{fn->kind, fn->a, fn->b, fn->c, (unsigned long long)fn->seed, fn->cell};
+    if (fn->kind == FELT_SPEED_NOISE_CURV && fn->cell <= 0)
+        return felt_fail(FELT_E_ARG, "noise cell must be positive");
+    return global_step(s, [&](const uint32_t* list, int n, float* out) -> int {
+        DISPATCH(s->G.D, k_speed, dim3(blocks_for(n)), dim3(TPB), s->G, s->P, list, n, sp, out);
+        return FELT_OK;
+    });
+}
+
+// Host-evaluated deltas: scattered through a per-voxel scratch onto the zero-layer list.
+int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals) {
+    int rc;
+    const size_t pool = (size_t)s->cap * s->G.pv;
+    float* before = s->vox.ptr;
+    if ((rc = s->vox.ensure(pool))) return rc;
+    if (s->vox.ptr != before) s->vox_zeroed = false;
+    if (!s->vox_zeroed) {
+        HIPX(hipMemsetAsync(s->vox.ptr, 0, pool * sizeof(float), s->st));
+        s->vox_zeroed = true;
+    }
+    if ((rc = s->tmp_i.ensure(m * 3 + 1))) return rc;
+    if ((rc = s->tmp_f.ensure(m + 1))) return rc;
+    if ((rc = s->tmp_u.ensure(m + 1))) return rc;
+    if (m) {
+        HIPX(hipMemcpyAsync(s->tmp_i.ptr, gpos, m * 3 * sizeof(int), hipMemcpyHostToDevice, s->st));
+        HIPX(hipMemcpyAsync(s->tmp_f.ptr, vals, m * sizeof(float), hipMemcpyHostToDevice, s->st));
+        HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
+        DISPATCH(s->G.D, k_voxel_ids, dim3(blocks_for(m)), dim3(TPB), s->G, s->P, s->tmp_i.ptr,
+                 (int)m, s->tmp_u.ptr, &s->ctr->misc);
+        if ((rc = read_ctr(s))) return rc;
+        if (s->hc.misc) return felt_fail(FELT_E_DELTA, "delta at a position outside the zero layer");
+        LAUNCH(k_vox_set, dim3(blocks_for(m)), dim3(TPB), s->tmp_u.ptr, s->tmp_f.ptr, (int)m,
+               s->vox.ptr);
+    }
+    return global_step(s, [&](const uint32_t* list, int n, float* out) -> int {
+        LAUNCH(k_vox_gather, dim3(blocks_for(n)), dim3(TPB), list, n, s->vox.ptr, out);
+        return FELT_OK;
+    });
+}
+
+// update_end (Surface.hpp:404-430) over the zero-layer list s->zlist / s->zdelta.
+int felt_update_end_impl(felt_surface* s) {
+    const Geo& G = s->G;
+    int rc;
+    const int zn = s->z_host_n;
+    // calc_affected into lookup 0 (m_grid_affected)
+    Lu A = felt_lu(s, 0);
+    if (zn > 0)
+        LAUNCH(k_lu_seed, dim3(blocks_for(zn)), dim3(TPB), A, s->zlist.ptr, zn, s->amark,
+               &s->ctr->lu_n[0]);
+    HIPX(hipMemsetAsync(&s->ctr->front[0], 0, sizeof(int), s->st));
+    LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, 1);
+    for (int j = 1; j <= G.L; j++) {
+        DISPATCH(G.D, k_affected_round, dim3(std::min(4096, std::max(1, blocks_for(s->band_cap_hint)))),
+                 dim3(TPB), G, s->P, A, s->ctr, 0, j, s->amark);
+        LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, j + 1);
+    }
+    if ((rc = read_ctr(s))) return rc;
+    const int an = s->hc.lu_n[0];
+    LAUNCH(k_unmark, dim3(blocks_for(an)), dim3(TPB), A, &s->ctr->lu_n[0], s->amark);
+    LAUNCH(k_dmask_lu, dim3(blocks_for(an)), dim3(TPB), G, s->P, A, &s->ctr->lu_n[0], s->dmask);
+    if (zn > 0)
+        DISPATCH(G.D, k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr,
+                 s->zdelta.ptr, zn, sc_of(s), felt_lu(s, 1), s->ctr, 1);
+    int look = 0, buf = 1;
+    if ((rc = distance_round(s, false, look, an, buf))) return rc;
+    if ((rc = flush_expand(s, an))) return rc;
+    if ((rc = read_ctr(s))) return rc;
+    int rounds = 1;
+    while (s->hc.flag) {
+        HIPX(hipMemsetAsync(&s->ctr->lu_n[look], 0, sizeof(int), s->st));
+        std::swap(look, buf);
+        if ((rc = reset_round(s))) return rc;
+        const int look_n = s->hc.lu_n[look];
+        if ((rc = distance_round(s, false, look, look_n, buf))) return rc;
+        if ((rc = flush_expand(s, look_n))) return rc;
+        rounds++;
+        if ((rc = read_ctr(s))) return rc;
+    }
+    s->last_updates = (uint64_t)an;
+    s->last_rounds = rounds;
+    s->in_update = false;
+    return FELT_OK;
+}
+
+int felt_band_total(felt_surface* s, size_t* out) {
+    HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
+    LAUNCH(k_band_total, dim3(blocks_for(s->G.np)), dim3(TPB), s->G, s->P, &s->ctr->misc);
+    int rc = read_ctr(s);
+    if (rc) return rc;
+    *out = (size_t)s->hc.misc;
+    return FELT_OK;
+}
+
+// Bounded step (Surface.hpp:310-386).
+int felt_update_box_impl(felt_surface* s, const int32_t* lo, const int32_t* hi,
+                         const felt_speed* fn) {
+    const Geo& G = s->G;
+    int rc;
+    if ((rc = felt_update_start_impl(s))) return rc;
+    size_t band;
+    if ((rc = felt_band_total(s, &band))) return rc;
+    if ((rc = ensure_step_capacity(s, band))) return rc;
+    s->band_cap_hint = (int64_t)band;
+    // partition box clamped to the grid, voxel box [lo, min(hi+1, grid_hi))
+    int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0}, vlo[3] = {0, 0, 0}, vhi[3] = {1, 1, 1};
+    bool empty = false;
+    for (int a = 0; a < G.D; a++) {
+        auto pos_child = [&](int w) {
+            // (w - offset) / part with C truncation, then the children offset cancels
+            return (w - G.offset[a]) / G.part[a];
+        };
+        const int cgl = pos_child(G.offset[a]);
+        const int cgh = pos_child(G.offset[a] + G.size[a] - 1);
+        clo[a] = std::max(cgl, pos_child(lo[a]));
+        chi[a] = std::min(cgh, pos_child(hi[a]));
+        if (chi[a] < clo[a]) empty = true;
+        vlo[a] = lo[a] - G.offset[a];
+        vhi[a] = std::min(G.size[a], hi[a] + 1 - G.offset[a]);
+    }
+    int zn = 0;
+    if (!empty) {
+        std::vector<int> parts;
+        int n0 = chi[0] - clo[0] + 1, n1 = G.D > 1 ? chi[1] - clo[1] + 1 : 1,
+            n2 = G.D > 2 ? chi[2] - clo[2] + 1 : 1;
+        parts.reserve((size_t)n0 * n1 * n2);
+        for (int x = clo[0]; x <= chi[0]; x++)
+            for (int y = clo[1]; y <= (G.D > 1 ? chi[1] : clo[1]); y++)
+                for (int z = clo[2]; z <= (G.D > 2 ? chi[2] : clo[2]); z++)
+                    parts.push_back(x * G.cstr[0] + (G.D > 1 ? y * G.cstr[1] : 0) +
+                                    (G.D > 2 ? z * G.cstr[2] : 0));
+        if ((rc = s->tmp_i.ensure(parts.size() + 1))) return rc;
+        HIPX(hipMemcpyAsync(s->tmp_i.ptr, parts.data(), parts.size() * sizeof(int),
+                            hipMemcpyHostToDevice, s->st));
+        DISPATCH(G.D, k_box_zero, dim3((unsigned)std::min<size_t>(parts.size(), 65535)), dim3(TPB),
+                 G, s->P, s->tmp_i.ptr, (int)parts.size(), vlo[0], vlo[1], vlo[2], vhi[0], vhi[1],
+                 vhi[2], s->zlist.ptr, s->ctr);
+        if ((rc = read_ctr(s))) return rc;
+        zn = s->hc.z_n;
+    }
+    s->z_host_n = zn;
+    if (zn > 0) {
+        Speed sp{fn->kind, fn->a, fn->b, fn->c, (unsigned long long)fn->seed, fn->cell};
+        DISPATCH(G.D, k_speed, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr, zn, sp,
+                 s->zdelta.ptr);
+    }
+    return felt_update_end_impl(s);
+}
+
+int felt_delta_impl(felt_surface* s, size_t m, const int32_t* gpos, const float* vals) {
+    int rc;
+    if (!s->in_update) return felt_fail(FELT_E_STATE, "delta() outside update_start/update_end");
+    (void)rc;
+    for (size_t i = 0; i < m; i++) {
+        const std::array<int, 3> key{gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
+        auto it = s->pending_index.find(key);
+        if (it != s->pending_index.end()) {
+            s->pending_deltas[it->second].second = vals[i];
+        } else {
+            s->pending_index.emplace(key, s->pending_deltas.size());
+            s->pending_deltas.emplace_back(key, vals[i]);
+        }
+    }
+    return FELT_OK;
+}
+
+int felt_update_end_public(felt_surface* s) {
+    int rc;
+    if (!s->in_update) return felt_fail(FELT_E_STATE, "update_end() without update_start()");
+    size_t band;
+    if ((rc = felt_band_total(s, &band))) return rc;
+    if ((rc = ensure_step_capacity(s, band + s->pending_deltas.size()))) return rc;
+    s->band_cap_hint = (int64_t)band;
+    const size_t m = s->pending_deltas.size();
+    if (m) {
+        std::vector<int> gp(m * 3);
+        std::vector<float> dv(m);
+        for (size_t i = 0; i < m; i++) {
+            for (int a = 0; a < 3; a++) gp[i * 3 + a] = s->pending_deltas[i].first[a];
+            dv[i] = s->pending_deltas[i].second;
+        }
+        if ((rc = s->tmp_i.ensure(m * 3 + 1))) return rc;
+        HIPX(hipMemcpyAsync(s->tmp_i.ptr, gp.data(), m * 3 * sizeof(int), hipMemcpyHostToDevice,
+                            s->st));
+        HIPX(hipMemcpyAsync(s->zdelta.ptr, dv.data(), m * sizeof(float), hipMemcpyHostToDevice,
+                            s->st));
+        HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
+        DISPATCH(s->G.D, k_voxel_ids, dim3(blocks_for(m)), dim3(TPB), s->G, s->P, s->tmp_i.ptr,
+                 (int)m, s->zlist.ptr, &s->ctr->misc);
+        if ((rc = read_ctr(s))) return rc;
+        if (s->hc.misc) {
+            s->in_update = false;
+            return felt_fail(FELT_E_DELTA, "delta at a position outside the zero layer");
+        }
+    }
+    s->z_host_n = (int)m;
+    s->pending_deltas.clear();
+    s->pending_index.clear();
+    return felt_update_end_impl(s);
+}
+
+// Tracks points (seed and similar): activate partitions, then write values and lists.
+int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::vector<float>& val,
+                      const std::vector<int>& layer) {
+    const Geo& G = s->G;
+    int rc;
+    const int n = (int)val.size();
+    if (n == 0) return FELT_OK;
+    if ((rc = s->tmp_i.ensure((size_t)n * 4 + 1))) return rc;
+    if ((rc = s->tmp_f.ensure((size_t)n + 1))) return rc;
+    int* dpos = s->tmp_i.ptr;
+    int* dlay = s->tmp_i.ptr + (size_t)n * 3;
+    HIPX(hipMemcpyAsync(dpos, gpos.data(), (size_t)n * 3 * sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIPX(hipMemcpyAsync(dlay, layer.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIPX(hipMemcpyAsync(s->tmp_f.ptr, val.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice,
+                        s->st));
+    HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
+    DISPATCH(G.D, k_mark_parts, dim3(blocks_for(n)), dim3(TPB), G, s->P, dpos, n, s->nflag,
+             s->nlist, s->ctr);
+    LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
+           s->free_stack, s->ctr);
+    DISPATCH(G.D, k_track_points, dim3(blocks_for(n)), dim3(TPB), G, s->P, dpos, s->tmp_f.ptr,
+             dlay, n);
+    return read_ctr(s);
+}
+
+int felt_load_dense_impl(felt_surface* s, const float* phi, size_t n) {
+    const Geo& G = s->G;
+    int rc;
+    float* dense = nullptr;
+    HIPX(hipMalloc(&dense, n * sizeof(float)));
+    hipError_t e = hipMemcpyAsync(dense, phi, n * sizeof(float), hipMemcpyHostToDevice, s->st);
+    if (e != hipSuccess) {
+        (void)hipFree(dense);
+        return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    }
+    if ((rc = s->tmp_i.ensure((size_t)G.np + 1))) { (void)hipFree(dense); return rc; }
+    DISPATCH(G.D, k_load_scan, dim3(G.np), dim3(TPB), G, s->P, dense, s->tmp_i.ptr, s->ctr);
+    HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
+    LAUNCH(k_list_from_flags, dim3(blocks_for(G.np)), dim3(TPB), s->tmp_i.ptr, G.np, s->nlist,
+           s->nflag, s->ctr);
+    LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
+           s->free_stack, s->ctr);
+    DISPATCH(G.D, k_load_fill, dim3(G.np), dim3(TPB), G, s->P, dense, s->tmp_i.ptr, s->ctr);
+    rc = read_ctr(s);
+    (void)hipFree(dense);
+    return rc;
+}
+
+int felt_snapshot_impl(felt_surface* s, float* dst, size_t total) {
+    int rc;
+    if ((rc = s->tmp_f.ensure(total))) return rc;
+    DISPATCH(s->G.D, k_snapshot, dim3((unsigned)std::min<int64_t>(65535, blocks_for(total))),
+             dim3(TPB), s->G, s->P, s->tmp_f.ptr, (int64_t)total);
+    HIPX(hipMemcpyAsync(dst, s->tmp_f.ptr, total * sizeof(float), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}
+
+int felt_get_impl(felt_surface* s, size_t n, const int32_t* gpos, float* out) {
+    int rc;
+    if (n == 0) return FELT_OK;
+    if ((rc = s->tmp_i.ensure(n * 3 + 1))) return rc;
+    if ((rc = s->tmp_f.ensure(n + 1))) return rc;
+    HIPX(hipMemcpyAsync(s->tmp_i.ptr, gpos, n * 3 * sizeof(int), hipMemcpyHostToDevice, s->st));
+    DISPATCH(s->G.D, k_get, dim3(blocks_for(n)), dim3(TPB), s->G, s->P, s->tmp_i.ptr, (int)n,
+             s->tmp_f.ptr);
+    HIPX(hipMemcpyAsync(out, s->tmp_f.ptr, n * sizeof(float), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}
+
+int felt_collect_layer_impl(felt_surface* s, int layer, std::vector<int>& out) {
+    int rc;
+    uint64_t n64 = 0;
+    std::vector<int> cnt(s->G.np * s->G.NL);
+    HIPX(hipMemcpyAsync(cnt.data(), s->P.pcnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    for (int p = 0; p < s->G.np; p++) n64 += (uint64_t)cnt[(size_t)p * s->G.NL + layer + s->G.L];
+    const size_t n = (size_t)n64;
+    if ((rc = s->tmp_i.ensure(n * 3 + 3))) return rc;
+    HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
+    DISPATCH(s->G.D, k_collect_layer, dim3(s->G.np), dim3(TPB), s->G, s->P, layer, s->tmp_i.ptr,
+             (int)n, &s->ctr->misc);
+    if ((rc = read_ctr(s))) return rc;
+    if ((size_t)s->hc.misc != n) return felt_fail(FELT_E_STATE, "layer count mismatch");
+    out.resize(n * 3);
+    HIPX(hipMemcpyAsync(out.data(), s->tmp_i.ptr, n * 3 * sizeof(int), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}
+
+int felt_lu_positions_impl(felt_surface* s, int lu_i, int tag, std::vector<int>& out) {
+    int rc;
+    if ((rc = read_ctr(s))) return rc;
+    const int n = s->hc.lu_n[lu_i];
+    if ((rc = s->tmp_i.ensure((size_t)n * 3 + 3))) return rc;
+    HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
+    if (n > 0)
+        DISPATCH(s->G.D, k_lu_positions, dim3(blocks_for(n)), dim3(TPB), s->G, s->P,
+                 felt_lu(s, lu_i), &s->ctr->lu_n[lu_i], tag, s->tmp_i.ptr, n, &s->ctr->misc);
+    if ((rc = read_ctr(s))) return rc;
+    const int m = s->hc.misc;
+    out.resize((size_t)m * 3);
+    if (m)
+        HIPX(hipMemcpyAsync(out.data(), s->tmp_i.ptr, (size_t)m * 3 * sizeof(int),
+                            hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}
+
+int felt_download(felt_surface* s, void* dst, const void* src, size_t bytes) {
+    HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}
+
+int felt_sync_impl(felt_surface* s) {
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}

@@ -1,0 +1,142 @@
+"""Synthetic surface workloads (seeded, host-generated) shared by tests and the benchmark.
+
+Random numbers come from SplitMix64 (Steele, Lea & Flood 2014) with floats drawn as
+(u >> 40) * 2^-24, so the same seed gives the same inputs on every machine.  Initial fields are
+computed in float64 and stored as float32: phi = clamp(distance-to-surface, -(L+1), L+1).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        u = (self.next_u64() >> 40) * (1.0 / 16777216.0)
+        return lo + (hi - lo) * u
+
+    def randint(self, lo: int, hi: int) -> int:
+        """Integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1)) if hi > lo else lo
+
+
+def grid_coords(size: Sequence[int]):
+    """World coordinate arrays (offset -size/2, C division) with indexing 'ij'."""
+    axes = [np.arange(s, dtype=np.float64) - (s // 2) for s in size]
+    return np.meshgrid(*axes, indexing="ij")
+
+
+def spheres_phi(size: Sequence[int], L: int, centres, radii) -> np.ndarray:
+    """min over spheres of (|x - c| - r), clamped to +-(L+1)."""
+    X = grid_coords(size)
+    phi = np.full(tuple(size), np.inf)
+    for c, r in zip(centres, radii):
+        d2 = sum((X[a] - c[a]) ** 2 for a in range(len(size)))
+        phi = np.minimum(phi, np.sqrt(d2) - r)
+    return np.clip(phi, -(L + 1), L + 1).astype(np.float32)
+
+
+def _real_sph_harm(l: int, m: int, theta: np.ndarray, ph: np.ndarray) -> np.ndarray:
+    """Real spherical harmonic (unnormalised associated Legendre times cos/sin)."""
+    x = np.cos(theta)
+    am = abs(m)
+    # associated Legendre P_l^|m|(x) by recurrence
+    pmm = np.ones_like(x)
+    if am > 0:
+        somx2 = np.sqrt(np.maximum(0.0, 1 - x * x))
+        fact = 1.0
+        for _ in range(am):
+            pmm = -pmm * fact * somx2
+            fact += 2.0
+    if l == am:
+        plm = pmm
+    else:
+        pmmp1 = x * (2 * am + 1) * pmm
+        if l == am + 1:
+            plm = pmmp1
+        else:
+            pll = pmmp1
+            for ll in range(am + 2, l + 1):
+                pll = ((2 * ll - 1) * x * pmmp1 - (ll + am - 1) * pmm) / (ll - am)
+                pmm, pmmp1 = pmmp1, pll
+            plm = pll
+    norm = math.sqrt((2 * l + 1) / (4 * math.pi) * math.factorial(l - am) / math.factorial(l + am))
+    if m > 0:
+        return math.sqrt(2) * norm * plm * np.cos(am * ph)
+    if m < 0:
+        return math.sqrt(2) * norm * plm * np.sin(am * ph)
+    return norm * plm
+
+
+def harmonic_sphere_phi(size: Sequence[int], L: int, radius: float, seed: int,
+                        n_terms: int = 8, lmax: int = 6, amp: float = 6.0) -> np.ndarray:
+    """Sphere of `radius` at the centre, radially perturbed by a sum of real spherical
+    harmonics (degree <= lmax) with amplitudes U[0, amp), normalised so the largest single
+    term reaches its amplitude."""
+    rng = SplitMix64(seed)
+    terms = []
+    for _ in range(n_terms):
+        l = rng.randint(0, lmax)
+        m = rng.randint(-l, l)
+        terms.append((l, m, rng.uniform(0.0, amp)))
+    X = grid_coords(size)
+    r = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
+    theta = np.arccos(np.clip(X[2] / np.maximum(r, 1e-12), -1.0, 1.0))
+    ph = np.arctan2(X[1], X[0])
+    R = np.full_like(r, radius)
+    for l, m, a in terms:
+        Y = _real_sph_harm(l, m, theta, ph)
+        peak = max(1e-12, float(np.abs(_real_sph_harm(l, m, np.linspace(0, math.pi, 181)[:, None],
+                                                          np.linspace(-math.pi, math.pi,
+                                                                      361)[None, :])).max()))
+        R = R + a * Y / peak
+    return np.clip(r - R, -(L + 1), L + 1).astype(np.float32)
+
+
+@dataclass
+class SurfaceSpec:
+    D: int
+    L: int
+    size: Tuple[int, ...]
+    part: Tuple[int, ...]
+    centres: List[Tuple[float, ...]] = field(default_factory=list)
+    radii: List[float] = field(default_factory=list)
+    speed: int = 0  # index into SPEEDS
+
+    def phi(self) -> np.ndarray:
+        return spheres_phi(self.size, self.L, self.centres, self.radii)
+
+
+def mixed_surfaces(n: int, seed: int, edge_lo: int, edge_hi: int, D: int = 3, L: int = 3,
+                   part_small: int = 8, part_big: int = 16, small_below: int = 96,
+                   fixed_part: int | None = None) -> List[SurfaceSpec]:
+    """Independent surfaces of uneven size: edge log-uniform in [edge_lo, edge_hi] rounded to
+    a multiple of the partition size, 1-8 spheres of radius U[0.05, 0.3] * edge."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        e = math.exp(rng.uniform(math.log(edge_lo), math.log(edge_hi)))
+        p = fixed_part if fixed_part else (part_small if e < small_below else part_big)
+        edge = max(p, int(round(e / p)) * p)
+        ns = rng.randint(1, 8)
+        cs, rs = [], []
+        for _ in range(ns):
+            rs.append(rng.uniform(0.05, 0.3) * edge)
+            cs.append(tuple(rng.uniform(-edge / 2, edge / 2) for _ in range(D)))
+        out.append(SurfaceSpec(D, L, (edge,) * D, (p,) * D, cs, rs, rng.randint(0, 2)))
+    return out

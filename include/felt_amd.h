@@ -147,6 +147,9 @@ int felt_polys_destroy(felt_polys* p);
 int felt_polys_notify(felt_polys* p);
 int felt_polys_march(felt_polys* p);
 int felt_polys_invalidate(felt_polys* p);
+/* Queue one partition for the next march, as driving Impl::Poly::Single::march on that child
+ * directly does (Poly.hpp:138-147; test_Poly.cpp:29-60). */
+int felt_polys_mark(felt_polys* p, uint64_t pos_idx);
 /* changes(): partitions marched by the last march (Polys.hpp:198-201). */
 int felt_polys_changes(felt_polys* p, uint64_t* out, size_t cap, size_t* n);
 /* children().get(pos_idx): active flag and vertex / simplex counts (Poly.hpp:176-189). */

@@ -106,6 +106,7 @@ struct Handle {
     virtual int polys_notify() = 0;
     virtual int polys_march() = 0;
     virtual int polys_invalidate() = 0;
+    virtual int polys_mark(uint64_t p) = 0;
     virtual int polys_changes(uint64_t* out, size_t cap, size_t* n) = 0;
     virtual int polys_child_info(uint64_t p, int* active, uint64_t* nv, uint64_t* ns) = 0;
     virtual int polys_child_mesh(uint64_t p, float* vtx, size_t vcap, uint32_t* spx, size_t scap) = 0;
@@ -317,6 +318,12 @@ struct HandleT final : Handle {
         polys->invalidate();
         return FELT_OK;
     }
+    int polys_mark(uint64_t p) override {
+        if (int rc = need_polys()) return rc;
+        if (p >= s.g.np) return fail(FELT_E_ARG, "partition out of range");
+        polys->pending->track(p);
+        return FELT_OK;
+    }
     int polys_changes(uint64_t* out, size_t cap, size_t* n) override {
         if (int rc = need_polys()) return rc;
         const RawList& lst = polys->done->list;
@@ -446,6 +453,7 @@ OAPI int felt_oracle_polys_create(felt_oracle_surface* s) { return H(s)->polys_c
 OAPI int felt_oracle_polys_notify(felt_oracle_surface* s) { return H(s)->polys_notify(); }
 OAPI int felt_oracle_polys_march(felt_oracle_surface* s) { return H(s)->polys_march(); }
 OAPI int felt_oracle_polys_invalidate(felt_oracle_surface* s) { return H(s)->polys_invalidate(); }
+OAPI int felt_oracle_polys_mark(felt_oracle_surface* s, uint64_t p) { return H(s)->polys_mark(p); }
 OAPI int felt_oracle_polys_changes(felt_oracle_surface* s, uint64_t* out, size_t cap, size_t* n) { return H(s)->polys_changes(out, cap, n); }
 OAPI int felt_oracle_polys_child_info(felt_oracle_surface* s, uint64_t p, int* active, uint64_t* nv, uint64_t* ns) {
     return H(s)->polys_child_info(p, active, nv, ns);

@@ -1,0 +1,181 @@
+"""HIP surface vs the CPU oracle on seeded workloads, compared after every step: layer
+membership, active partitions, isogrid/delta/status-change partition lists bit-exact, phi
+within 1e-5 (tolerance PHI_TOL in parity.py), meshes as unordered sets within 1e-5."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib
+import parity
+from felt_amd import workloads
+from felt_amd._binding import speed_const_curv, speed_constant, speed_curvature, speed_noise_curv
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(D, L, size, part):
+    import felt_amd
+    return felt_amd.Surface(D, L, size, part), oracle_lib.surface(D, L, size, part)
+
+
+def step_both(g, c, how, *args):
+    getattr(g, how)(*args)
+    getattr(c, how)(*args)
+
+
+def check(g, c, where, snapshot=True):
+    d = parity.state_diff(g, c, snapshot=snapshot)
+    assert not d, f"{where}: {d}"
+
+
+def test_c1_plain_config():
+    """128^3 isogrid in 16^3 partitions, singularity seed, 40 steps of -0.5 then 40 of
+    0.2*curvature, compared after every step; then a full march compared as meshes."""
+    import felt_amd
+    g, c = pair(3, 3, (128,) * 3, (16,) * 3)
+    step_both(g, c, "seed", (0, 0, 0))
+    check(g, c, "seed")
+    t0 = time.time()
+    for i in range(80):
+        sp = speed_constant(-0.5) if i < 40 else speed_curvature(0.2)
+        step_both(g, c, "update", sp)
+        check(g, c, f"step {i}", snapshot=(i % 5 == 4 or i >= 75))
+        if i % 10 == 9:
+            print(f"[c1] step {i + 1} ok, band {g.stats().band_voxels}, "
+                  f"{time.time() - t0:.1f}s", flush=True)
+    assert c.stats().nan_events == g.stats().nan_events
+    pg, pc = felt_amd.Polys(g), oracle_lib.polys(c)
+    pg.invalidate(); pc.invalidate()
+    pg.march(); pc.march()
+    d = parity.mesh_diff(pg, g, pc, c)
+    assert not d, d
+    assert pg.totals()[1] > 0
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_2d_circles_global_and_bounded(seed):
+    rng = workloads.SplitMix64(seed)
+    size, part, L = (96, 80), (8, 8), 2
+    cs = [(rng.uniform(-30, 30), rng.uniform(-25, 25)) for _ in range(4)]
+    rs = [rng.uniform(4, 14) for _ in range(4)]
+    phi = workloads.spheres_phi(size, L, cs, rs)
+    g, c = pair(2, L, size, part)
+    step_both(g, c, "load_dense", phi)
+    check(g, c, "load")
+    speeds = [speed_constant(-0.5), speed_curvature(0.3), speed_const_curv(-0.3, 0.2),
+              speed_constant(0.4), speed_noise_curv(-0.3, 0.1, 0.5, 99, 16)]
+    for i in range(30):
+        sp = speeds[rng.randint(0, len(speeds) - 1)]
+        if rng.uniform() < 0.5:
+            step_both(g, c, "update", sp)
+        else:
+            lo = [rng.randint(-50, 40) for _ in range(2)]
+            hi = [x + rng.randint(0, 60) for x in lo]
+            step_both(g, c, "update_box", lo, hi, sp)
+        check(g, c, f"seed {seed} step {i}")
+    print(f"[2d] seed {seed} ok, band {g.stats().band_voxels}", flush=True)
+
+
+def test_3d_spheres_bounded_noise():
+    rng = workloads.SplitMix64(7)
+    size, part, L = (48, 48, 40), (8, 8, 8), 3
+    cs = [tuple(rng.uniform(-15, 15) for _ in range(3)) for _ in range(3)]
+    rs = [rng.uniform(4, 10) for _ in range(3)]
+    g, c = pair(3, L, size, part)
+    step_both(g, c, "load_dense", workloads.spheres_phi(size, L, cs, rs))
+    check(g, c, "load")
+    for i in range(24):
+        sp = speed_noise_curv(-0.3, 0.1, 0.5, 1234, 16) if i % 3 else speed_curvature(0.25)
+        if i % 2:
+            lo = [rng.randint(-24, 10) for _ in range(3)]
+            hi = [x + rng.randint(4, 30) for x in lo]
+            step_both(g, c, "update_box", lo, hi, sp)
+        else:
+            step_both(g, c, "update", sp)
+        check(g, c, f"step {i}")
+    print(f"[3d] ok, band {g.stats().band_voxels}", flush=True)
+
+
+def test_uneven_partitions_and_grid_edge():
+    """Grid sizes not divisible by the partition size (children grid +1 on every axis,
+    PartitionedMixin.hpp:131-141), with the surface growing into the grid boundary."""
+    g, c = pair(3, 2, (20, 17, 13), (6, 6, 6))
+    step_both(g, c, "seed", (3, 2, 1))
+    for i in range(14):
+        step_both(g, c, "update", speed_constant(-0.7 if i < 10 else 0.35))
+        check(g, c, f"step {i}")
+
+
+def test_collapse_and_reexpand():
+    g, c = pair(3, 3, (40,) * 3, (8,) * 3)
+    step_both(g, c, "load_dense", workloads.spheres_phi((40,) * 3, 3, [(0.5, -1.0, 2.0)], [9.0]))
+    for i in range(16):
+        step_both(g, c, "update", speed_constant(0.9))
+        check(g, c, f"contract {i}")
+    assert g.layer_size(0) == 0
+    step_both(g, c, "seed", (0, 0, 0))
+    for i in range(6):
+        step_both(g, c, "update", speed_constant(-1.0))
+        check(g, c, f"expand {i}")
+
+
+def test_host_functor_and_low_level_api():
+    g, c = pair(2, 3, (24, 24), (4, 4))
+    step_both(g, c, "seed", (-3, -3))
+    step_both(g, c, "seed", (4, 5))
+    pts = {(-3, -2): -1.0, (4, 4): 1.0}
+    step_both(g, c, "update", lambda p: pts.get(tuple(p), 0.0))
+    check(g, c, "functor")
+    for i in range(5):
+        z = [tuple(int(x) for x in p) for p in c.layer_positions(0)]
+        pick = sorted(z)[i % len(z)]
+        for s in (g, c):
+            s.update_start()
+            s.delta([pick], [-0.8])
+            s.update_end()
+        check(g, c, f"local {i}")
+        for k in range(7):
+            a = {tuple(int(x) for x in p) for p in g.affected_positions(k)}
+            b = {tuple(int(x) for x in p) for p in c.affected_positions(k)}
+            assert a == b, (i, k)
+
+
+def test_polys_incremental_parity():
+    import felt_amd
+    rng = workloads.SplitMix64(11)
+    size, part, L = (40, 36, 32), (8, 6, 8), 3
+    cs = [tuple(rng.uniform(-10, 10) for _ in range(3)) for _ in range(2)]
+    g, c = pair(3, L, size, part)
+    step_both(g, c, "load_dense", workloads.spheres_phi(size, L, cs, [6.0, 8.5]))
+    pg, pc = felt_amd.Polys(g), oracle_lib.polys(c)
+    for p in (pg, pc):
+        p.invalidate()
+        p.march()
+    assert not parity.mesh_diff(pg, g, pc, c)
+    for i in range(12):
+        sp = speed_const_curv(-0.4, 0.2) if i < 8 else speed_constant(0.6)
+        if i % 3 == 2:
+            step_both(g, c, "update_box", [-10, -10, -10], [5, 8, 3], sp)
+        else:
+            step_both(g, c, "update", sp)
+        pg.notify(); pc.notify()
+        if i % 2:
+            pg.march(); pc.march()
+            d = parity.mesh_diff(pg, g, pc, c)
+            assert not d, (i, d)
+    print(f"[polys] ok, triangles {pg.totals()[1]}", flush=True)
+
+
+def test_polys_2d_parity():
+    import felt_amd
+    size, part, L = (64, 48), (8, 8), 2
+    g, c = pair(2, L, size, part)
+    step_both(g, c, "load_dense", workloads.spheres_phi(size, L, [(-10, 3), (12, -5)], [7.5, 9.0]))
+    pg, pc = felt_amd.Polys(g), oracle_lib.polys(c)
+    for i in range(6):
+        step_both(g, c, "update", speed_const_curv(-0.5, 0.1))
+        pg.notify(); pc.notify()
+        pg.march(); pc.march()
+        d = parity.mesh_diff(pg, g, pc, c)
+        assert not d, (i, d)
