@@ -397,6 +397,29 @@ API int felt_surface_stats(felt_surface* s, felt_stats* st) {
     return FELT_OK;
 }
 
+API int felt_surface_profile(felt_surface* s, int enable) {
+    CHECK_S(s);
+    if (int rc = prof_collect(s)) return rc;
+    s->prof.on = enable != 0;
+    for (int k = 0; k < PK_COUNT; k++) {
+        s->prof.launches[k] = 0;
+        s->prof.ms[k] = 0.0;
+        s->prof.items[k] = 0;
+    }
+    return FELT_OK;
+}
+
+API int felt_surface_profile_read(felt_surface* s, int kernel, uint64_t* launches, double* ms,
+                                  uint64_t* items) {
+    CHECK_S(s);
+    if (kernel < 0 || kernel >= PK_COUNT) return felt_fail(FELT_E_ARG, "unknown kernel id");
+    if (int rc = prof_collect(s)) return rc;
+    *launches = (uint64_t)s->prof.launches[kernel];
+    *ms = s->prof.ms[kernel];
+    *items = (uint64_t)s->prof.items[kernel];
+    return FELT_OK;
+}
+
 API int felt_surface_sync(felt_surface* s) {
     CHECK_S(s);
     return felt_sync_impl(s);

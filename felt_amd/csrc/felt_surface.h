@@ -21,6 +21,23 @@ struct DBuf {
     void release();
 };
 
+// Optional per-kernel timing with HIP events on the surface's stream.
+enum ProfKernel { PK_SPEED, PK_APPLY, PK_DIST1, PK_DIST2, PK_FLUSH, PK_EXPAND, PK_COMPACT,
+                  PK_AFFECTED, PK_MARCH_COUNT, PK_MARCH_EMIT, PK_COUNT };
+struct ProfRec {
+    hipEvent_t a, b;
+    int kernel;
+    long long items;
+};
+struct Prof {
+    bool on = false;
+    std::vector<ProfRec> pending;
+    std::vector<hipEvent_t> spare;
+    long long launches[PK_COUNT] = {};
+    double ms[PK_COUNT] = {};
+    long long items[PK_COUNT] = {};
+};
+
 struct felt_surface {
     felt::Geo G{};
     felt::Pool P{};
@@ -62,7 +79,13 @@ struct felt_surface {
     std::map<std::array<int, 3>, size_t> pending_index;
     uint64_t last_updates = 0, last_rounds = 0;
     struct felt_polys* polys = nullptr;
+    Prof prof;
 };
+
+// Brackets one launch with events when profiling is on.
+void prof_begin(felt_surface* s, int kernel, long long items);
+void prof_end(felt_surface* s);
+int prof_collect(felt_surface* s);
 
 extern thread_local std::string g_felt_err;
 int felt_fail(int code, const std::string& msg);

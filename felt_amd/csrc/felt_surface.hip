@@ -22,6 +22,7 @@ using namespace felt;
 namespace {
 
 constexpr int TPB = 256;
+constexpr int kMaxRounds = 100000;
 
 __global__ void k_fill_i32(int* p, int64_t n, int v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -725,6 +726,45 @@ int felt_fail(int code, const std::string& msg) {
             return felt_fail(FELT_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+static hipEvent_t prof_event(felt_surface* s) {
+    if (!s->prof.spare.empty()) {
+        hipEvent_t e = s->prof.spare.back();
+        s->prof.spare.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void prof_begin(felt_surface* s, int kernel, long long items) {
+    if (!s->prof.on) return;
+    ProfRec r{prof_event(s), prof_event(s), kernel, items};
+    (void)hipEventRecord(r.a, s->st);
+    s->prof.pending.push_back(r);
+}
+
+void prof_end(felt_surface* s) {
+    if (!s->prof.on || s->prof.pending.empty()) return;
+    (void)hipEventRecord(s->prof.pending.back().b, s->st);
+}
+
+int prof_collect(felt_surface* s) {
+    if (s->prof.pending.empty()) return FELT_OK;
+    HIPX(hipStreamSynchronize(s->st));
+    for (ProfRec& r : s->prof.pending) {
+        float ms = 0.f;
+        HIPX(hipEventElapsedTime(&ms, r.a, r.b));
+        s->prof.launches[r.kernel]++;
+        s->prof.ms[r.kernel] += ms;
+        s->prof.items[r.kernel] += r.items;
+        s->prof.spare.push_back(r.a);
+        s->prof.spare.push_back(r.b);
+    }
+    s->prof.pending.clear();
+    return FELT_OK;
+}
+
 template <class T>
 int DBuf<T>::ensure(size_t n) {
     if (n <= cap) return FELT_OK;
@@ -780,8 +820,9 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->P.pcnt, np * G.NL * sizeof(int)));
     HIPX(hipMalloc(&s->P.slot_part, cap * sizeof(int)));
     HIPX(hipMalloc(&s->P.phi, cap * (int64_t)G.pv * sizeof(float)));
-    HIPX(hipMalloc(&s->P.lay, cap * (int64_t)G.pv));
-    HIPX(hipMalloc(&s->amark, cap * (int64_t)G.pv));
+    // +4: byte compare-and-swap goes through the enclosing aligned 32-bit word
+    HIPX(hipMalloc(&s->P.lay, cap * (int64_t)G.pv + 4));
+    HIPX(hipMalloc(&s->amark, cap * (int64_t)G.pv + 4));
     HIPX(hipMalloc(&s->free_stack, cap * sizeof(int)));
     HIPX(hipMalloc(&s->ctr, sizeof(Ctr)));
     HIPX(hipMalloc(&s->ukey, np * sizeof(int)));
@@ -818,6 +859,13 @@ int felt_surface_init(felt_surface* s) {
 
 void felt_surface_release(felt_surface* s) {
     if (s->st) (void)hipStreamSynchronize(s->st);
+    for (ProfRec& r : s->prof.pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : s->prof.spare) (void)hipEventDestroy(e);
+    s->prof.pending.clear();
+    s->prof.spare.clear();
     void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
                     s->amark, s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag,
                     s->nlist, s->dmask, s->smask, s->act};
@@ -875,6 +923,7 @@ int felt_compact_band(felt_surface* s) {
              &s->ctr->act_n);
     if ((rc = read_ctr(s))) return rc;
     const int nact = s->hc.act_n;
+    s->band_base.assign(G.NL + 1, 0);
     const int chunk = std::min(G.pv, 4096);
     const int nchunk = (G.pv + chunk - 1) / chunk;
     const int nblocks = nact * nchunk;
@@ -882,8 +931,10 @@ int felt_compact_band(felt_surface* s) {
     if (nblocks == 0) return FELT_OK;
     if ((rc = s->blk_cnt.ensure((size_t)nblocks * G.NL + G.NL + 1))) return rc;
     if ((rc = s->blk_off.ensure((size_t)nblocks * G.NL))) return rc;
+    prof_begin(s, PK_COMPACT, (long long)nact * G.pv);
     DISPATCH(G.D, k_band_count, dim3(nblocks), dim3(TPB), G, s->P, s->act, nchunk, chunk,
              s->blk_cnt.ptr);
+    prof_end(s);
     int* base = s->blk_cnt.ptr + (size_t)nblocks * G.NL;
     LAUNCH(k_band_scan, dim3(1), dim3(TPB), s->blk_cnt.ptr, nblocks, G.NL, s->blk_off.ptr, base);
     HIPX(hipMemcpyAsync(s->band_base.data(), base, (G.NL + 1) * sizeof(int),
@@ -924,9 +975,13 @@ static int distance_round(felt_surface* s, bool from_band, int look_i, int look_
                 n = look_n;
             }
             if (n == 0) continue;
+            prof_begin(s, PK_DIST1, n);
             DISPATCH(G.D, k_dist1, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d);
+            prof_end(s);
+            prof_begin(s, PK_DIST2, n);
             DISPATCH(G.D, k_dist2, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d,
                      sc, buf, s->ctr, buf_i);
+            prof_end(s);
         }
     }
     return FELT_OK;
@@ -940,16 +995,20 @@ static int flush_expand(felt_surface* s, int sc_bound) {
     const int nb = blocks_for(sc_bound);
     LAUNCH(k_smask, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->smask);
     HIPX(hipMemsetAsync(&s->ctr->touched_n, 0, sizeof(int), s->st));
+    prof_begin(s, PK_FLUSH, 0);
     DISPATCH(G.D, k_flush, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag, s->tlist,
              s->ctr);
+    prof_end(s);
     LAUNCH(k_deactivate, dim3(blocks_for(std::min<int64_t>(G.np, sc_bound))), dim3(TPB), G, s->P,
            s->tlist, s->ctr, s->ukey, s->tflag, s->free_stack, s->ctr);
     HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
+    prof_begin(s, PK_EXPAND, 0);
     DISPATCH(G.D, k_expand_mark, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->nflag, s->nlist,
              s->ctr);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
            s->free_stack, s->ctr);
     DISPATCH(G.D, k_expand_claim, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ctr);
+    prof_end(s);
     return FELT_OK;
 }
 
@@ -989,8 +1048,10 @@ static int global_step(felt_surface* s, DeltaFn&& delta_fn) {
     // update_zero_layer into the affected buffer (index 1)
     if (zn > 0) {
         if ((rc = delta_fn(s->band.ptr + z0, zn, s->band_d.ptr + z0))) return rc;
+        prof_begin(s, PK_APPLY, zn);
         DISPATCH(G.D, k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->band.ptr + z0,
                  s->band_d.ptr + z0, zn, sc_of(s), felt_lu(s, 1), s->ctr, 1);
+        prof_end(s);
     }
     if ((rc = distance_round(s, true, 0, 0, 1))) return rc;
     if ((rc = flush_expand(s, (int)band))) return rc;
@@ -1008,6 +1069,7 @@ static int global_step(felt_surface* s, DeltaFn&& delta_fn) {
             rounds++;
             if ((rc = read_ctr(s))) return rc;
             if (!s->hc.flag) break;
+            if (rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
         }
     }
     s->last_updates = band;
@@ -1021,7 +1083,9 @@ int felt_update_global(felt_surface* s, const felt_speed* fn) {
     if (fn->kind == FELT_SPEED_NOISE_CURV && fn->cell <= 0)
         return felt_fail(FELT_E_ARG, "noise cell must be positive");
     return global_step(s, [&](const uint32_t* list, int n, float* out) -> int {
+        prof_begin(s, PK_SPEED, n);
         DISPATCH(s->G.D, k_speed, dim3(blocks_for(n)), dim3(TPB), s->G, s->P, list, n, sp, out);
+        prof_end(s);
         return FELT_OK;
     });
 }
@@ -1095,6 +1159,7 @@ int felt_update_end_impl(felt_surface* s) {
         if ((rc = flush_expand(s, look_n))) return rc;
         rounds++;
         if ((rc = read_ctr(s))) return rc;
+        if (rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
     }
     s->last_updates = (uint64_t)an;
     s->last_rounds = rounds;

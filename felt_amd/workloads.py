@@ -94,18 +94,37 @@ def harmonic_sphere_phi(size: Sequence[int], L: int, radius: float, seed: int,
         l = rng.randint(0, lmax)
         m = rng.randint(-l, l)
         terms.append((l, m, rng.uniform(0.0, amp)))
-    X = grid_coords(size)
-    r = np.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
-    theta = np.arccos(np.clip(X[2] / np.maximum(r, 1e-12), -1.0, 1.0))
-    ph = np.arctan2(X[1], X[0])
-    R = np.full_like(r, radius)
-    for l, m, a in terms:
-        Y = _real_sph_harm(l, m, theta, ph)
-        peak = max(1e-12, float(np.abs(_real_sph_harm(l, m, np.linspace(0, math.pi, 181)[:, None],
-                                                          np.linspace(-math.pi, math.pi,
-                                                                      361)[None, :])).max()))
-        R = R + a * Y / peak
-    return np.clip(r - R, -(L + 1), L + 1).astype(np.float32)
+    peaks = []
+    for l, m, _ in terms:
+        peaks.append(max(1e-12, float(np.abs(_real_sph_harm(
+            l, m, np.linspace(0, math.pi, 181)[:, None],
+            np.linspace(-math.pi, math.pi, 361)[None, :])).max())))
+    reach = sum(a for _, _, a in terms) + L + 2
+    out = np.empty(tuple(size), dtype=np.float32)
+    ys = np.arange(size[1], dtype=np.float64) - (size[1] // 2)
+    zs = np.arange(size[2], dtype=np.float64) - (size[2] // 2)
+    Y2, Z2 = np.meshgrid(ys, zs, indexing="ij")
+    slab = 16
+    for x0 in range(0, size[0], slab):
+        xs = np.arange(x0, min(size[0], x0 + slab), dtype=np.float64) - (size[0] // 2)
+        X = xs[:, None, None] + 0.0 * Y2[None]
+        Yb = np.broadcast_to(Y2[None], X.shape)
+        Zb = np.broadcast_to(Z2[None], X.shape)
+        r = np.sqrt(X ** 2 + Yb ** 2 + Zb ** 2)
+        phi = r - radius
+        # the perturbation moves the surface by at most sum(amplitudes): outside that shell
+        # the clamped value is already decided
+        near = np.abs(phi) <= reach
+        if near.any():
+            rn, xn, yn, zn = r[near], X[near], Yb[near], Zb[near]
+            theta = np.arccos(np.clip(zn / np.maximum(rn, 1e-12), -1.0, 1.0))
+            ph = np.arctan2(yn, xn)
+            R = np.full_like(rn, radius)
+            for (l, m, a), pk in zip(terms, peaks):
+                R += a * _real_sph_harm(l, m, theta, ph) / pk
+            phi[near] = rn - R
+        out[x0:x0 + len(xs)] = np.clip(phi, -(L + 1), L + 1)
+    return out
 
 
 @dataclass

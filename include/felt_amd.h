@@ -138,6 +138,14 @@ int felt_surface_partition_active(felt_surface* s, uint8_t* out, size_t n);
 /* Number of layer-k voxels in partition pos_idx (layer_size helper, test_Surface.cpp:1720-1737). */
 int felt_surface_partition_layer_size(felt_surface* s, uint64_t pos_idx, int layer, uint64_t* n);
 int felt_surface_stats(felt_surface* s, felt_stats* out);
+/* Kernel timing with HIP events on the handle's stream, for measurement.  Not a reference
+ * member (the reference has no profiling hooks).  Kernel ids: 0 speed, 1 zero-layer apply,
+ * 2 distance pass 1, 3 distance pass 2, 4 flush, 5 expand (three kernels), 6 band count,
+ * 7 affected search, 8 march count, 9 march emit.  `items` sums the list entries the launches
+ * processed.  Enabling or disabling clears the totals. */
+int felt_surface_profile(felt_surface* s, int enable);
+int felt_surface_profile_read(felt_surface* s, int kernel, uint64_t* launches, double* ms,
+                              uint64_t* items);
 /* Block until all queued device work of the handle is done. */
 int felt_surface_sync(felt_surface* s);
 
