@@ -1,0 +1,79 @@
+// C++ use of Felt::Surface / Felt::Polys over the HIP library, written as the reference's own
+// tests are (test_Surface.cpp:49-160, 1031-1053; test_Poly.cpp:278-358).  Exit status 0 when
+// every check holds; prints the first failure otherwise.
+#include <felt_amd/Surface.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <set>
+
+#define CHECK(c)                                                   \
+    do {                                                           \
+        if (!(c)) {                                                \
+            std::printf("FAILED %s:%d: %s\n", __FILE__, __LINE__, #c); \
+            return 1;                                              \
+        }                                                          \
+    } while (0)
+
+using namespace Felt;
+
+int main() {
+    {
+        // 9x9 2-layer surface, 3x3 partitions: seed then expand by a host lambda of each shape
+        Surface<2, 2> surface(VecDi<2>{9, 9}, VecDi<2>{3, 3});
+        surface.seed(VecDi<2>{0, 0});
+        CHECK(surface.isogrid().get(VecDi<2>{0, 0}) == 0);
+        CHECK(surface.layer_size(0) == 1 && surface.layer_size(1) == 4 && surface.layer_size(2) == 8);
+        surface.update([](const VecDi<2>&, const Surface<2, 2>::IsoGrid&) { return -1.0f; });
+        CHECK(surface.layer_size(-1) == 1 && surface.layer_size(0) == 4);
+        surface.update([](const VecDi<2>&) { return 1.0f; });
+        CHECK(surface.layer_size(-1) == 0 && surface.layer_size(0) == 1);
+        surface.update([]() { return -1.0f; });
+        CHECK(surface.layer_size(2) == 12);
+        const std::vector<float> snap = surface.isogrid().snapshot();
+        CHECK(snap.size() == 81 && snap[4 * 9 + 4] == -1.0f);
+        surface.update(VecDi<2>{1, 0}, VecDi<2>{3, 3}, []() { return 0.0f; });
+        CHECK(surface.delta(Surface<2, 2>::layer_idx(0)).size() >= 1);
+    }
+    {
+        // 3D layer counts 1/6/18/38/66 (test_Surface.cpp:1031-1053), device functor
+        Surface<3, 3> surface(VecDi<3>{11, 11, 11}, VecDi<3>{3, 3, 3});
+        surface.seed(VecDi<3>{0, 0, 0});
+        surface.update(speed_constant(-1.0f));
+        CHECK(surface.layer_size(-1) == 1 && surface.layer_size(0) == 6 &&
+              surface.layer_size(1) == 18 && surface.layer_size(2) == 38 &&
+              surface.layer_size(3) == 66);
+        surface.update_start();
+        surface.delta(VecDi<3>{0, 1, 0}, -1.0f);
+        surface.update_end();
+        bool threw = false;
+        try {
+            surface.update_start();
+            surface.delta(VecDi<3>{5, 5, 5}, -0.5f);  // not a zero-layer point
+            surface.update_end();
+        } catch (const std::domain_error&) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+    {
+        // Polys counts in the central partition (test_Poly.cpp:329-358)
+        Surface<3, 3> surface(VecDi<3>{15, 15, 15}, VecDi<3>{5, 5, 5});
+        Polys<Surface<3, 3>> polys(surface);
+        surface.seed(VecDi<3>{0, 0, 0});
+        surface.update(speed_constant(-1.0f));
+        polys.notify();
+        polys.march();
+        const PosIdx centre = (1 * 3 + 1) * 3 + 1;
+        const auto child = polys.child(centre);
+        CHECK(child.vtxs().size() == 30 && child.spxs().size() == 56);
+        CHECK(polys.changes().size() == 7);
+        for (const auto& v : child.vtxs()) {
+            const float n = std::sqrt(v.norm[0] * v.norm[0] + v.norm[1] * v.norm[1] +
+                                      v.norm[2] * v.norm[2]);
+            CHECK(std::fabs(n - 1.0f) < 1e-5f);
+        }
+    }
+    std::printf("surface_api: all checks passed\n");
+    return 0;
+}
