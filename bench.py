@@ -102,8 +102,7 @@ def main():
         t0 = time.perf_counter()
         s.update(sp)
         polys.notify()
-        st = s.stats()
-        updates += int(st.last_voxel_updates)
+        updates += s.step_info()[0]
         if (i + 1) % mc_every == 0:
             tm = time.perf_counter()
             polys.march()

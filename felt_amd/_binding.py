@@ -122,6 +122,7 @@ class Binding:
             "surface_partition_layer_size": (C.c_int, [_P, C.c_uint64, C.c_int, _U64P]),
             "surface_stats": (C.c_int, [_P, C.POINTER(felt_stats)]),
             "surface_sync": (C.c_int, [_P]),
+            "surface_step_info": (C.c_int, [_P, _U64P, _U64P]),
             "polys_notify": (C.c_int, [_P]),
             "polys_march": (C.c_int, [_P]),
             "polys_invalidate": (C.c_int, [_P]),
@@ -143,7 +144,7 @@ class Binding:
             sig["polys_destroy"] = (C.c_int, [_P])
         self.fn = {}
         for name, (res, args) in sig.items():
-            if not hasattr(L, prefix + name) and name in ("device_count",):
+            if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info"):
                 continue
             f = getattr(L, prefix + name)
             f.restype = res
@@ -337,6 +338,15 @@ class Surface:
         st = felt_stats()
         self.b.call("surface_stats", self.h, C.byref(st))
         return st
+
+    def step_info(self):
+        """(voxel updates, rounds) of the last update call."""
+        u, r = C.c_uint64(), C.c_uint64()
+        if "surface_step_info" in self.b.fn:
+            self.b.call("surface_step_info", self.h, C.byref(u), C.byref(r))
+            return int(u.value), int(r.value)
+        st = self.stats()
+        return int(st.last_voxel_updates), int(st.last_rounds)
 
     def sync(self) -> None:
         self.b.call("surface_sync", self.h)

@@ -397,6 +397,13 @@ API int felt_surface_stats(felt_surface* s, felt_stats* st) {
     return FELT_OK;
 }
 
+API int felt_surface_step_info(felt_surface* s, uint64_t* voxel_updates, uint64_t* rounds) {
+    CHECK_S(s);
+    *voxel_updates = s->last_updates;
+    *rounds = s->last_rounds;
+    return FELT_OK;
+}
+
 API int felt_surface_profile(felt_surface* s, int enable) {
     CHECK_S(s);
     if (int rc = prof_collect(s)) return rc;

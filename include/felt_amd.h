@@ -138,6 +138,8 @@ int felt_surface_partition_active(felt_surface* s, uint8_t* out, size_t n);
 /* Number of layer-k voxels in partition pos_idx (layer_size helper, test_Surface.cpp:1720-1737). */
 int felt_surface_partition_layer_size(felt_surface* s, uint64_t pos_idx, int layer, uint64_t* n);
 int felt_surface_stats(felt_surface* s, felt_stats* out);
+/* Voxel updates and rounds of the last update call, without device traffic. */
+int felt_surface_step_info(felt_surface* s, uint64_t* voxel_updates, uint64_t* rounds);
 /* Kernel timing with HIP events on the handle's stream, for measurement.  Not a reference
  * member (the reference has no profiling hooks).  Kernel ids: 0 speed, 1 zero-layer apply,
  * 2 distance pass 1, 3 distance pass 2, 4 flush, 5 expand (three kernels), 6 band count,
