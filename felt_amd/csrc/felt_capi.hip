@@ -109,6 +109,11 @@ API int felt_surface_create(int D, int L, const int32_t* size, const int32_t* pa
     G.cstr[2] = 1;
     G.cstr[1] = G.csize[2];
     G.cstr[0] = G.csize[1] * G.csize[2];
+    for (int a = 0; a < 3; a++) {
+        G.fpart[a] = make_fastdiv((uint32_t)G.part[a]);
+        G.fcsize[a] = make_fastdiv((uint32_t)G.csize[a]);
+    }
+    G.fpv = make_fastdiv((uint32_t)G.pv);
     felt_surface* s = new (std::nothrow) felt_surface();
     if (!s) return felt_fail(FELT_E_NOMEM, "host allocation failed");
     s->G = G;

@@ -24,6 +24,30 @@ namespace felt {
 constexpr int8_t NONE = -128;
 constexpr float kEps = 1.1920928955078125e-07f;  // FLT_EPSILON
 
+// Division by a run-time invariant divisor with one high multiply (Granlund & Montgomery
+// 1994; Warren, Hacker's Delight 10-8): s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1,
+// q = (t + ((n - t) >> 1)) >> (s - 1) with t = mulhi(n, m).  Exact for every 32-bit n.
+struct FastDiv {
+    uint32_t d, m;
+    int s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f{d, 0u, 0};
+    if (d <= 1) return f;
+    int s = 0;
+    while ((1ull << s) < d) s++;
+    f.s = s;
+    f.m = (uint32_t)((((1ull << 32) * ((1ull << s) - d)) / d) + 1);
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    if (f.d <= 1) return n;
+    const uint32_t t = __umulhi(n, f.m);
+    return (t + ((n - t) >> 1)) >> (f.s - 1);
+}
+
 struct Geo {
     int D, L, NL;
     int size[3], part[3], csize[3], offset[3];
@@ -31,6 +55,7 @@ struct Geo {
     int cstr[3];  // partition strides in the children grid
     int pv;
     int np;
+    FastDiv fpart[3], fcsize[3], fpv;
 };
 
 struct Pool {
@@ -58,16 +83,42 @@ __device__ __forceinline__ bool in_band(int k, int L) { return k >= -L && k <= L
 template <int D>
 __device__ __forceinline__ void decode(const Geo& G, const Pool& P, uint32_t v, int& slot,
                                        int loc[3], int g[3]) {
-    slot = (int)(v / (uint32_t)G.pv);
-    int l = (int)(v - (uint32_t)slot * (uint32_t)G.pv);
-    int p = P.slot_part[slot];
+    slot = (int)fdiv(v, G.fpv);
+    uint32_t l = v - (uint32_t)slot * (uint32_t)G.pv;
+    uint32_t p = (uint32_t)P.slot_part[slot];
 #pragma unroll
     for (int a = D - 1; a >= 0; a--) {
-        const int pc = p % G.csize[a];
-        p /= G.csize[a];
-        loc[a] = l % G.part[a];
-        l /= G.part[a];
+        const uint32_t pq = fdiv(p, G.fcsize[a]);
+        const int pc = (int)(p - pq * (uint32_t)G.csize[a]);
+        p = pq;
+        const uint32_t lq = fdiv(l, G.fpart[a]);
+        loc[a] = (int)(l - lq * (uint32_t)G.part[a]);
+        l = lq;
         g[a] = pc * G.part[a] + loc[a];
+    }
+}
+
+// Partition coordinates of partition index p.
+template <int D>
+__device__ __forceinline__ void part_coords(const Geo& G, int p, int pc[3]) {
+    uint32_t r = (uint32_t)p;
+#pragma unroll
+    for (int a = D - 1; a >= 0; a--) {
+        const uint32_t q = fdiv(r, G.fcsize[a]);
+        pc[a] = (int)(r - q * (uint32_t)G.csize[a]);
+        r = q;
+    }
+}
+
+// Brick coordinates of leaf index l.
+template <int D>
+__device__ __forceinline__ void leaf_coords(const Geo& G, int l, int lc[3]) {
+    uint32_t r = (uint32_t)l;
+#pragma unroll
+    for (int a = D - 1; a >= 0; a--) {
+        const uint32_t q = fdiv(r, G.fpart[a]);
+        lc[a] = (int)(r - q * (uint32_t)G.part[a]);
+        r = q;
     }
 }
 
@@ -83,7 +134,7 @@ template <int D>
 __device__ __forceinline__ int part_of(const Geo& G, const int q[3]) {
     int p = 0;
 #pragma unroll
-    for (int a = 0; a < D; a++) p += (q[a] / G.part[a]) * G.cstr[a];
+    for (int a = 0; a < D; a++) p += (int)fdiv((uint32_t)q[a], G.fpart[a]) * G.cstr[a];
     return p;
 }
 
@@ -91,8 +142,24 @@ template <int D>
 __device__ __forceinline__ int leaf_of(const Geo& G, const int q[3]) {
     int l = 0;
 #pragma unroll
-    for (int a = 0; a < D; a++) l += (q[a] % G.part[a]) * G.lstr[a];
+    for (int a = 0; a < D; a++) {
+        const int qa = (int)fdiv((uint32_t)q[a], G.fpart[a]);
+        l += (q[a] - qa * G.part[a]) * G.lstr[a];
+    }
     return l;
+}
+
+// Partition and leaf of an in-grid position together.
+template <int D>
+__device__ __forceinline__ void locate(const Geo& G, const int q[3], int& p, int& l) {
+    p = 0;
+    l = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const int qa = (int)fdiv((uint32_t)q[a], G.fpart[a]);
+        p += qa * G.cstr[a];
+        l += (q[a] - qa * G.part[a]) * G.lstr[a];
+    }
 }
 
 // get(pos) of the partitioned isogrid (PartitionedMixin.hpp:594-601): L+1 outside the grid,
@@ -100,10 +167,11 @@ __device__ __forceinline__ int leaf_of(const Geo& G, const int q[3]) {
 template <int D>
 __device__ __forceinline__ float get_g(const Geo& G, const Pool& P, const int q[3]) {
     if (!inside<D>(G, q)) return (float)(G.L + 1);
-    const int p = part_of<D>(G, q);
+    int p, l;
+    locate<D>(G, q, p, l);
     const int s = P.page[p];
     if (s < 0) return P.pbg[p];
-    return P.phi[(size_t)s * G.pv + leaf_of<D>(G, q)];
+    return P.phi[(size_t)s * G.pv + l];
 }
 
 // Value at g + d (d components in {-1,0,1}); reads the voxel's own brick directly when the

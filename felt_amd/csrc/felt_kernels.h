@@ -34,7 +34,8 @@ struct SC {
 struct Lu {
     uint32_t* v;
     int8_t* tag;
-    float* d;  // distance scratch aligned with v
+    float* d;    // distance scratch aligned with v
+    int8_t* tk;  // layer of the new value, aligned with v
 };
 
 struct Speed {

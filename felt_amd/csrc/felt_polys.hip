@@ -128,43 +128,51 @@ __device__ int edge_owner(const Geo& G, const Pool& P, int s, const int pc[3], c
     return best;
 }
 
-// Block-wide exclusive scan of two int arrays in place, returning the totals.
+// Block-wide exclusive scan of two int arrays in place, returning the totals.  Thread t owns
+// a contiguous segment; segment sums are scanned with wavefront shuffles.
 __device__ void block_scan2(int* a, int* b, int n, int* tot_a, int* tot_b) {
-    __shared__ int sa[TPB], sb[TPB];
-    __shared__ int carry_a, carry_b;
-    if (threadIdx.x == 0) {
-        carry_a = 0;
-        carry_b = 0;
+    __shared__ int wa[TPB / 64], wb[TPB / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int seg = (n + TPB - 1) / TPB;
+    const int i0 = min(n, t * seg), i1 = min(n, i0 + seg);
+    int sa = 0, sb = 0;
+    for (int i = i0; i < i1; i++) {
+        sa += a[i];
+        sb += b[i];
+    }
+    int xa = sa, xb = sb;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int ya = __shfl_up(xa, o), yb = __shfl_up(xb, o);
+        if (lane >= o) {
+            xa += ya;
+            xb += yb;
+        }
+    }
+    if (lane == 63) {
+        wa[wave] = xa;
+        wb[wave] = xb;
     }
     __syncthreads();
-    for (int base = 0; base < n; base += TPB) {
-        const int i = base + threadIdx.x;
-        const int va = i < n ? a[i] : 0, vb = i < n ? b[i] : 0;
-        sa[threadIdx.x] = va;
-        sb[threadIdx.x] = vb;
-        __syncthreads();
-        for (int off = 1; off < TPB; off <<= 1) {
-            const int xa = threadIdx.x >= off ? sa[threadIdx.x - off] : 0;
-            const int xb = threadIdx.x >= off ? sb[threadIdx.x - off] : 0;
-            __syncthreads();
-            sa[threadIdx.x] += xa;
-            sb[threadIdx.x] += xb;
-            __syncthreads();
+    int pa = 0, pb = 0, ta = 0, tb = 0;
+    for (int w = 0; w < TPB / 64; w++) {
+        if (w < wave) {
+            pa += wa[w];
+            pb += wb[w];
         }
-        if (i < n) {
-            a[i] = carry_a + sa[threadIdx.x] - va;
-            b[i] = carry_b + sb[threadIdx.x] - vb;
-        }
-        __syncthreads();
-        if (threadIdx.x == TPB - 1) {
-            carry_a += sa[TPB - 1];
-            carry_b += sb[TPB - 1];
-        }
-        __syncthreads();
+        ta += wa[w];
+        tb += wb[w];
     }
-    if (threadIdx.x == 0) {
-        *tot_a = carry_a;
-        *tot_b = carry_b;
+    int ra = pa + xa - sa, rb = pb + xb - sb;
+    for (int i = i0; i < i1; i++) {
+        const int va = a[i], vb = b[i];
+        a[i] = ra;
+        b[i] = rb;
+        ra += va;
+        rb += vb;
+    }
+    if (t == 0) {
+        *tot_a = ta;
+        *tot_b = tb;
     }
 }
 
@@ -180,20 +188,15 @@ __global__ void k_march_count(Geo G, Pool P, const int* parts, uint8_t* cmask, u
     uint16_t* ow = own + (size_t)j * G.pv;
     int* vb = vbase + (size_t)j * G.pv;
     int* tb = tbase + (size_t)j * G.pv;
-    int pc[3] = {0, 0, 0}, rem = p;
-    for (int a = D - 1; a >= 0; a--) {
-        pc[a] = rem % G.csize[a];
-        rem /= G.csize[a];
-    }
+    int pc[3] = {0, 0, 0};
+    part_coords<D>(G, p, pc);
     for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
         int nv = 0, nt = 0;
         unsigned m = 0, om = 0;
         if (s >= 0 && P.lay[(size_t)s * G.pv + l] != NONE) {
-            int g[3] = {0, 0, 0}, r = l;
-            for (int a = D - 1; a >= 0; a--) {
-                g[a] = pc[a] * G.part[a] + r % G.part[a];
-                r /= G.part[a];
-            }
+            int g[3] = {0, 0, 0};
+            leaf_coords<D>(G, l, g);
+            for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
             m = cube_mask<D>(G, P, g);
             const unsigned em = McT<D>::edges_of(m);
             for (int e = 0; e < McT<D>::nedge; e++) {
@@ -355,21 +358,16 @@ __global__ void k_march_emit(Geo G, Pool P, const int* parts, const uint8_t* cma
     const int vw = D == 3 ? 6 : 2;
     float* vo = vout + seg_v[j] * vw;
     uint32_t* to = tout + seg_t[j] * D;
-    int pc[3] = {0, 0, 0}, rem = p;
-    for (int a = D - 1; a >= 0; a--) {
-        pc[a] = rem % G.csize[a];
-        rem /= G.csize[a];
-    }
+    int pc[3] = {0, 0, 0};
+    part_coords<D>(G, p, pc);
     for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
         if (P.lay[(size_t)s * G.pv + l] == NONE) continue;
         const unsigned m = cm[l];
         const int ns = McT<D>::nsimplex(m);
         if (ns == 0) continue;
-        int g[3] = {0, 0, 0}, r = l;
-        for (int a = D - 1; a >= 0; a--) {
-            g[a] = pc[a] * G.part[a] + r % G.part[a];
-            r /= G.part[a];
-        }
+        int g[3] = {0, 0, 0};
+        leaf_coords<D>(G, l, g);
+        for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
         const unsigned om = ow[l];
         int rank = 0;
         for (int e = 0; e < McT<D>::nedge; e++) {

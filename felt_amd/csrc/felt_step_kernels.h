@@ -1,0 +1,490 @@
+// felt_step_kernels.h — kernels of one update step (included by felt_surface.hip only).
+//
+// Each round of Surface::update_end[_global] (Surface.hpp:404-430, 739-762) runs over one
+// "processed list" of voxels (the band, the affected set or the last round's buffer): every
+// entry carries its voxel id v, its layer at the start of the round (tag) and the layer its new
+// value falls in (tk).  Status changes are the entries with tk != tag; they are compacted into
+// the status-change records and the next round's buffer by count / scan / scatter passes, so no
+// list append contends on a global atomic.  Per-partition counters are updated with one atomic
+// per (wavefront, partition, layer).
+#pragma once
+
+#include "felt_kernels.h"
+
+namespace felt {
+namespace step {
+
+constexpr int BT = 1024;  // threads of the scan / compaction blocks
+constexpr int ITEMS = 4;  // list entries per thread in the change compaction
+
+// Exclusive scan of one int per thread over a block of BT threads; returns the block total.
+__device__ __forceinline__ int block_excl_scan(int val, int* total) {
+    __shared__ int wsum[BT / 64];
+    __shared__ int wpre[BT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = val;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        const int nw = BT / 64;
+        int w = lane < nw ? wsum[lane] : 0;
+        int z = w;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(z, o);
+            if (lane >= o) z += y;
+        }
+        if (lane < nw) wpre[lane] = z - w;
+        if (lane == nw - 1) wsum[0] = z;  // block total (wsum[0] no longer needed)
+    }
+    __syncthreads();
+    const int res = wpre[wave] + x - val;
+    *total = wsum[0];
+    __syncthreads();
+    return res;
+}
+
+// Adds +1 to pcnt[p][inc] and -1 to pcnt[p][dec] (-1 = none) for every active lane, with one
+// atomic per distinct (partition, layer) of the wavefront.  All lanes must call it.
+__device__ __forceinline__ void agg_pcnt(int* pcnt, int NL, int p, int dec, int inc) {
+    const int lane = __lane_id();
+    unsigned long long todo = __ballot(p >= 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int p0 = __shfl(p, leader);
+        const bool mine = p >= 0 && p == p0;
+        const unsigned long long mm = __ballot(mine);
+        for (int k = 0; k < NL; k++) {
+            const int d = __popcll(__ballot(mine && inc == k)) - __popcll(__ballot(mine && dec == k));
+            if (lane == leader && d != 0) atomicAdd(&pcnt[p0 * NL + k], d);
+        }
+        todo &= ~mm;
+    }
+}
+
+// ---- band compaction (one block per active partition) ---------------------------------------
+
+template <int D>
+__global__ void k_band_count(Geo G, Pool P, const int* act, int* counts) {
+    __shared__ int c[32];
+    const int p = act[blockIdx.x];
+    const int s = P.page[p];
+    if (threadIdx.x < 32) c[threadIdx.x] = 0;
+    __syncthreads();
+    const int8_t* lay = P.lay + (size_t)s * G.pv;
+    for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
+        const int k = lay[i];
+        if (k != NONE) atomicAdd(&c[k + G.L], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < G.NL) counts[blockIdx.x * G.NL + threadIdx.x] = c[threadIdx.x];
+}
+
+// Single block: layer-major exclusive scan of counts[n][NL] into offsets; base[k] is where
+// layer k's segment starts and base[NL] the band total.
+__global__ void k_band_scan(const int* counts, int n, int NL, int* offsets, int* base) {
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int k = 0; k < NL; k++) {
+        if (threadIdx.x == 0) base[k] = carry;
+        __syncthreads();
+        for (int b0 = 0; b0 < n; b0 += BT) {
+            const int b = b0 + threadIdx.x;
+            const int v = b < n ? counts[b * NL + k] : 0;
+            int tot;
+            const int ex = block_excl_scan(v, &tot);
+            if (b < n) offsets[b * NL + k] = carry + ex;
+            __syncthreads();
+            if (threadIdx.x == 0) carry += tot;
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) base[NL] = carry;
+}
+
+template <int D>
+__global__ void k_band_scatter(Geo G, Pool P, const int* act, const int* offsets, uint32_t* band,
+                               int8_t* tag) {
+    __shared__ int c[32];
+    const int p = act[blockIdx.x];
+    const int s = P.page[p];
+    if (threadIdx.x < G.NL) c[threadIdx.x] = offsets[blockIdx.x * G.NL + threadIdx.x];
+    __syncthreads();
+    const int8_t* lay = P.lay + (size_t)s * G.pv;
+    for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
+        const int k = lay[i];
+        if (k != NONE) {
+            const int at = atomicAdd(&c[k + G.L], 1);
+            band[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)i;
+            tag[at] = (int8_t)k;
+        }
+    }
+}
+
+// ---- per-round processing --------------------------------------------------------------------
+
+__global__ void k_tk_init(const int8_t* tag, int8_t* tk, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) tk[i] = tag[i];
+}
+
+__device__ __forceinline__ int8_t clamp_k(int k) { return (int8_t)(k < -100 ? -100 : (k > 100 ? 100 : k)); }
+
+template <int D>
+__global__ void k_speed(Geo G, Pool P, const uint32_t* list, int n, Speed sp, float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = list[i];
+    int slot, loc[3], g[3];
+    decode<D>(G, P, v, slot, loc, g);
+    out[i] = eval_speed<D>(G, P, sp, v, loc, g);
+}
+
+// update_zero_layer (Surface.hpp:920-979): phi += delta; tk = new layer.
+__global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta, int n,
+                        int8_t* tk, int* nan_ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = list[i];
+    const float nv = P.phi[v] + delta[i];
+    P.phi[v] = nv;
+    tk[i] = clamp_k(layer_of(nv, G.L, nan_ctr));
+}
+
+// update_distance pass 1 (Surface.hpp:1029-1073).
+template <int D>
+__global__ void k_dist1(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
+                        float* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (tag && tag[i] != layer) return;
+    const uint32_t v = list[i];
+    int slot, loc[3], g[3];
+    decode<D>(G, P, v, slot, loc, g);
+    const float side = layer < 0 ? -1.0f : 1.0f;
+    out[i] = distance<D>(G, P, v, loc, g, P.phi[v], side);
+}
+
+// update_distance pass 2 (Surface.hpp:1076-1121): phi = d; tk = new layer.
+__global__ void k_dist2(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
+                        const float* d, int8_t* tk, int* nan_ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (tag && tag[i] != layer) return;
+    const uint32_t v = list[i];
+    const float x = d[i];
+    P.phi[v] = x;
+    tk[i] = clamp_k(layer_of(x, G.L, nan_ctr));
+}
+
+// Change compaction, pass 1: per block the number of status changes and of in-band targets;
+// raises the round flag for a change out of a non-zero layer (Surface.hpp:1116).
+__global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, int n, int* counts,
+                            Ctr* ctr) {
+    const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
+    int nsc = 0, nbf = 0;
+    bool fl = false;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int i = base + j * BT;
+        if (i < n) {
+            const int a = tag[i], b = tk[i];
+            if (a != b) {
+                nsc++;
+                if (b >= -G.L && b <= G.L) nbf++;
+                if (a != 0) fl = true;
+            }
+        }
+    }
+    int tsc, tbf;
+    block_excl_scan(nsc, &tsc);
+    block_excl_scan(nbf, &tbf);
+    if (__ballot(fl) != 0ull && fl) ctr->flag = 1;
+    if (threadIdx.x == 0) {
+        counts[2 * blockIdx.x] = tsc;
+        counts[2 * blockIdx.x + 1] = tbf;
+    }
+}
+
+// Pass 2 (single block): exclusive scan of the per-block counts; the totals become the round's
+// status-change count and the buffer's entry count.
+__global__ void k_chg_scan(int* counts, int nb, int* sc_n, int* buf_n) {
+    __shared__ int carry[2];
+    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += BT) {
+        const int b = b0 + threadIdx.x;
+        const int a = b < nb ? counts[2 * b] : 0, c = b < nb ? counts[2 * b + 1] : 0;
+        int ta, tc;
+        const int ea = block_excl_scan(a, &ta);
+        const int ec = block_excl_scan(c, &tc);
+        if (b < nb) {
+            counts[2 * b] = carry[0] + ea;
+            counts[2 * b + 1] = carry[1] + ec;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            carry[0] += ta;
+            carry[1] += tc;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *sc_n = carry[0];
+        *buf_n = carry[1];
+    }
+}
+
+// Pass 3: status-change records (status_change, Surface.hpp:1138-1178) and the buffer entries
+// of the in-band targets, in list order within each block.
+__global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, const int8_t* tk, int n,
+                              const int* offsets, SC sc, Lu buf) {
+    const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
+    int nsc = 0, nbf = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int i = base + j * BT;
+        if (i < n && tag[i] != tk[i]) {
+            nsc++;
+            if (tk[i] >= -G.L && tk[i] <= G.L) nbf++;
+        }
+    }
+    int tsc, tbf;
+    int osc = block_excl_scan(nsc, &tsc) + offsets[2 * blockIdx.x];
+    int obf = block_excl_scan(nbf, &tbf) + offsets[2 * blockIdx.x + 1];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int i = base + j * BT;
+        if (i < n) {
+            const int a = tag[i], b = tk[i];
+            if (a != b) {
+                const uint32_t v = lv[i];
+                sc.v[osc] = v;
+                sc.from[osc] = (int8_t)a;
+                sc.to[osc] = (int16_t)b;
+                osc++;
+                if (b >= -G.L && b <= G.L) {
+                    buf.v[obf] = v;
+                    buf.tag[obf] = (int8_t)b;
+                    obf++;
+                }
+            }
+        }
+    }
+}
+
+// Partitions holding status changes from each layer this round (status_change(k) lists), one
+// atomic per distinct partition of a wavefront.
+__global__ void k_smask(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = ctrc->sc_n;
+    int p = -1, from = 0;
+    if (i < n) {
+        p = P.slot_part[fdiv(sc.v[i], G.fpv)];
+        from = sc.from[i] + G.L;
+    }
+    const int lane = __lane_id();
+    unsigned long long todo = __ballot(p >= 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int p0 = __shfl(p, leader);
+        const bool mine = p >= 0 && p == p0;
+        unsigned bits = 0;
+        for (int k = 0; k < G.NL; k++)
+            if (__ballot(mine && from == k)) bits |= 1u << k;
+        if (lane == leader) atomicOr(&smask[p0], bits);
+        todo &= ~__ballot(mine);
+    }
+}
+
+// flush_status_change (Surface.hpp:1183-1244) with Untrack::retrack / untrack
+// (PartitionedMixin.hpp:489-558).  A partition that loses voxels to untrack is queued for the
+// emptiness check; its background becomes Distance(to) of the untrack that comes last in the
+// reference's flush order (the highest from-layer).
+__global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, int* ukey, int* tflag, int* tlist,
+                        Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = ctrc->sc_n;
+    int p = -1, dec = -1, inc = -1, key = -1;
+    if (i < n) {
+        const uint32_t v = sc.v[i];
+        const int from = sc.from[i], to = sc.to[i];
+        p = P.slot_part[fdiv(v, G.fpv)];
+        dec = from + G.L;
+        if (to >= -G.L && to <= G.L) {
+            P.lay[v] = (int8_t)to;
+            inc = to + G.L;
+        } else {
+            P.lay[v] = NONE;
+            P.phi[v] = (float)to;
+            key = ((from + G.L) << 16) | ((to + 32768) & 0xffff);
+        }
+    }
+    agg_pcnt(P.pcnt, G.NL, p, dec, inc);
+    // untracks: one atomicMax per distinct partition of the wave
+    const int lane = __lane_id();
+    const int pu = key >= 0 ? p : -1;
+    unsigned long long todo = __ballot(pu >= 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int p0 = __shfl(pu, leader);
+        const bool mine = pu >= 0 && pu == p0;
+        int kmax = mine ? key : -1;
+        for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
+        if (lane == leader) {
+            atomicMax(&ukey[p0], kmax);
+            if (atomicExch(&tflag[p0], 1) == 0) tlist[atomicAdd(&ctr->touched_n, 1)] = p0;
+        }
+        todo &= ~__ballot(mine);
+    }
+}
+
+__global__ void k_deactivate(Geo G, Pool P, const int* tlist, const Ctr* ctrc, int* ukey,
+                             int* tflag, int* free_stack, Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctrc->touched_n) return;
+    const int p = tlist[i];
+    tflag[p] = 0;
+    const int key = ukey[p];
+    ukey[p] = -1;
+    int sum = 0;
+    for (int k = 0; k < G.NL; k++) sum += P.pcnt[p * G.NL + k];
+    const int s = P.page[p];
+    if (sum == 0 && s >= 0) {
+        P.page[p] = -1;
+        P.slot_part[s] = -1;
+        P.pbg[p] = (float)((key & 0xffff) - 32768);
+        const int t = atomicAdd(&ctr->free_top, 1);
+        free_stack[t] = s;
+    }
+}
+
+__device__ __forceinline__ bool expanding(const Geo& G, int from, int to) {
+    const int side = from < 0 ? -1 : 1;
+    return (from == G.L || from == -G.L) && to == G.L * side - side;
+}
+
+// expand_narrow_band (Surface.hpp:1249-1355), phase 1: queue inactive partitions that an
+// expanding voxel's out-of-band neighbours fall in.
+template <int D>
+__global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag, int* nlist,
+                              Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = ctrc->sc_n;
+    bool cand = false;
+    int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+    if (i < n && expanding(G, sc.from[i], sc.to[i])) {
+        cand = true;
+        decode<D>(G, P, sc.v[i], slot, loc, g);
+    }
+    for (int a = 0; a < D; a++) {
+        for (int s = -1; s <= 1; s += 2) {
+            bool q_new = false;
+            int pq = 0;
+            if (cand) {
+                int q[3] = {g[0], g[1], g[2]};
+                q[a] += s;
+                if (inside<D>(G, q)) {
+                    pq = part_of<D>(G, q);
+                    if (P.page[pq] < 0 &&
+                        !in_band(layer_of(P.pbg[pq], G.L, &ctr->nan_events), G.L))
+                        q_new = atomicExch(&nflag[pq], 1) == 0;
+                }
+            }
+            const int at = wave_append(&ctr->need_n, q_new);
+            if (q_new) nlist[at] = pq;
+        }
+    }
+}
+
+// Partition activation from the free-slot stack: the brick is filled with the partition's
+// background (GridMixin.hpp:75-81) and no voxel is tracked.
+__global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* nflag,
+                        int* free_stack, Ctr* ctr) {
+    __shared__ int slot_sh;
+    const int n = ctrc->need_n;
+    for (int j = blockIdx.x; j < n; j += gridDim.x) {
+        const int p = nlist[j];
+        if (threadIdx.x == 0) {
+            const int t = atomicSub(&ctr->free_top, 1) - 1;
+            int s = -1;
+            if (t >= 0) s = free_stack[t];
+            else {
+                atomicAdd(&ctr->free_top, 1);
+                ctr->err = 1;
+            }
+            slot_sh = s;
+            if (s >= 0) {
+                P.page[p] = s;
+                P.slot_part[s] = p;
+            }
+            nflag[p] = 0;
+        }
+        __syncthreads();
+        const int s = slot_sh;
+        if (s >= 0) {
+            const float bg = P.pbg[p];
+            float* ph = P.phi + (size_t)s * G.pv;
+            int8_t* ly = P.lay + (size_t)s * G.pv;
+            for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
+                ph[i] = bg;
+                ly[i] = NONE;
+            }
+            if (threadIdx.x < G.NL) P.pcnt[p * G.NL + threadIdx.x] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// expand_narrow_band phase 2: claim each out-of-band neighbour once and track it in layer
+// +-L with its distance (the value does not depend on which expanding voxel claims it).
+template <int D>
+__global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cand = false;
+    int side = 1, slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+    if (i < ctrc->sc_n && expanding(G, sc.from[i], sc.to[i])) {
+        cand = true;
+        side = sc.from[i] < 0 ? -1 : 1;
+        decode<D>(G, P, sc.v[i], slot, loc, g);
+    }
+    const int8_t target = (int8_t)(G.L * side);
+    for (int a = 0; a < D; a++) {
+        for (int s = -1; s <= 1; s += 2) {
+            int claimed_p = -1;
+            if (cand) {
+                int q[3] = {g[0], g[1], g[2]};
+                q[a] += s;
+                if (inside<D>(G, q)) {
+                    int pq, lq;
+                    locate<D>(G, q, pq, lq);
+                    const int sq = P.page[pq];
+                    if (sq >= 0) {  // else activation failed (pool exhausted, flagged)
+                        const uint32_t vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
+                        const float dq = __hip_atomic_load(&P.phi[vq], __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+                        if (!in_band(layer_of(dq, G.L, &ctr->nan_events), G.L) &&
+                            cas_byte(&P.lay[vq], NONE, target)) {
+                            int lc[3];
+                            leaf_coords<D>(G, lq, lc);
+                            const float nd = distance<D>(G, P, vq, lc, q, dq, (float)side);
+                            __hip_atomic_store(&P.phi[vq], nd, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                            claimed_p = pq;
+                        }
+                    }
+                }
+            }
+            agg_pcnt(P.pcnt, G.NL, claimed_p, -1, claimed_p >= 0 ? target + G.L : -1);
+        }
+    }
+}
+
+}  // namespace step
+}  // namespace felt

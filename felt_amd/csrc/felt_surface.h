@@ -53,6 +53,8 @@ struct felt_surface {
     // band compaction
     DBuf<uint32_t> band;
     DBuf<float> band_d;
+    DBuf<int8_t> band_tag, band_tk;
+    DBuf<int> chg_cnt;
     DBuf<int> blk_cnt, blk_off;
     std::vector<int> band_base;
     // bounded-update zero layer
@@ -63,6 +65,7 @@ struct felt_surface {
     DBuf<uint32_t> lu_v[2];
     DBuf<int8_t> lu_tag[2];
     DBuf<float> lu_d[2];
+    DBuf<int8_t> lu_tk[2];
     // status-change records
     DBuf<uint32_t> sc_v;
     DBuf<int8_t> sc_from;

@@ -15,9 +15,11 @@
 #include <vector>
 
 #include "felt_kernels.h"
+#include "felt_step_kernels.h"
 #include "felt_surface.h"
 
 using namespace felt;
+using namespace felt::step;
 
 namespace {
 
@@ -45,343 +47,6 @@ __global__ void k_active_parts(Geo G, Pool P, int* out, int* n) {
     if (act) out[at] = p;
 }
 
-// Per block: counts of band voxels per layer in one chunk of one active partition.
-template <int D>
-__global__ void k_band_count(Geo G, Pool P, const int* act, int nchunk, int chunk, int* counts) {
-    __shared__ int c[32];
-    const int b = blockIdx.x;
-    const int p = act[b / nchunk];
-    const int s = P.page[p];
-    const int start = (b % nchunk) * chunk;
-    if (threadIdx.x < G.NL) c[threadIdx.x] = 0;
-    __syncthreads();
-    const int8_t* lay = P.lay + (size_t)s * G.pv;
-    for (int i = start + threadIdx.x; i < start + chunk && i < G.pv; i += blockDim.x) {
-        const int k = lay[i];
-        if (k != NONE) atomicAdd(&c[k + G.L], 1);
-    }
-    __syncthreads();
-    if (threadIdx.x < G.NL) counts[b * G.NL + threadIdx.x] = c[threadIdx.x];
-}
-
-// Single block: layer-major exclusive scan of per-block counts.  base[k] is where layer k's
-// segment starts; base[NL] is the band total.
-__global__ void k_band_scan(const int* counts, int nblocks, int NL, int* offsets, int* base) {
-    __shared__ int part[TPB * 17];
-    __shared__ int tot[18];
-    const int t = threadIdx.x;
-    const int seg = (nblocks + TPB - 1) / TPB;
-    const int b0 = t * seg, b1 = min(nblocks, b0 + seg);
-    int sums[17];
-    for (int k = 0; k < NL; k++) sums[k] = 0;
-    for (int b = b0; b < b1; b++)
-        for (int k = 0; k < NL; k++) sums[k] += counts[b * NL + k];
-    for (int k = 0; k < NL; k++) part[k * TPB + t] = sums[k];
-    __syncthreads();
-    if (t < NL) {
-        int run = 0;
-        for (int i = 0; i < TPB; i++) {
-            const int x = part[t * TPB + i];
-            part[t * TPB + i] = run;
-            run += x;
-        }
-        tot[t] = run;
-    }
-    __syncthreads();
-    if (t == 0) {
-        int run = 0;
-        for (int k = 0; k < NL; k++) {
-            base[k] = run;
-            run += tot[k];
-        }
-        base[NL] = run;
-        tot[17] = 0;
-    }
-    __syncthreads();
-    int run[17];
-    for (int k = 0; k < NL; k++) run[k] = base[k] + part[k * TPB + t];
-    for (int b = b0; b < b1; b++)
-        for (int k = 0; k < NL; k++) {
-            offsets[b * NL + k] = run[k];
-            run[k] += counts[b * NL + k];
-        }
-}
-
-template <int D>
-__global__ void k_band_scatter(Geo G, Pool P, const int* act, int nchunk, int chunk,
-                               const int* offsets, uint32_t* band) {
-    __shared__ int c[32];
-    const int b = blockIdx.x;
-    const int p = act[b / nchunk];
-    const int s = P.page[p];
-    const int start = (b % nchunk) * chunk;
-    if (threadIdx.x < G.NL) c[threadIdx.x] = offsets[b * G.NL + threadIdx.x];
-    __syncthreads();
-    const int8_t* lay = P.lay + (size_t)s * G.pv;
-    for (int i = start + threadIdx.x; i < start + chunk && i < G.pv; i += blockDim.x) {
-        const int k = lay[i];
-        if (k != NONE) {
-            const int at = atomicAdd(&c[k + G.L], 1);
-            band[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)i;
-        }
-    }
-}
-
-// ---- speed, apply, distance ----------------------------------------------------------------
-
-template <int D>
-__global__ void k_speed(Geo G, Pool P, const uint32_t* list, int n, Speed sp, float* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = list[i];
-    int slot, loc[3], g[3];
-    decode<D>(G, P, v, slot, loc, g);
-    out[i] = eval_speed<D>(G, P, sp, v, loc, g);
-}
-
-// Appends a status change and, when the target is in the band, a buffer entry
-// (status_change, Surface.hpp:1138-1178).  All lanes of the wave must call it.
-__device__ __forceinline__ void record_change(bool pred, uint32_t v, int from, int to, int L,
-                                              SC sc, int* sc_n, Lu buf, int* buf_n) {
-    const int at = wave_append(sc_n, pred);
-    if (pred) {
-        sc.v[at] = v;
-        sc.from[at] = (int8_t)from;
-        sc.to[at] = (int16_t)to;
-    }
-    const bool inb = pred && to >= -L && to <= L;
-    const int bt = wave_append(buf_n, inb);
-    if (inb) {
-        buf.v[bt] = v;
-        buf.tag[bt] = (int8_t)to;
-    }
-}
-
-// update_zero_layer (Surface.hpp:920-979): phi += delta; a layer change is recorded but does
-// not raise the round flag.
-template <int D>
-__global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta, int n, SC sc,
-                        Lu buf, Ctr* ctr, int buf_i) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t v = 0;
-    int k = 0;
-    if (valid) {
-        v = list[i];
-        const float nv = P.phi[v] + delta[i];
-        k = layer_of(nv, G.L, &ctr->nan_events);
-        P.phi[v] = nv;
-    }
-    record_change(valid && k != 0, v, 0, k, G.L, sc, &ctr->sc_n, buf, &ctr->lu_n[buf_i]);
-}
-
-// update_distance pass 1 (Surface.hpp:1029-1073): d = distance() for every list entry of the
-// layer; tag == nullptr means the list is already one layer's segment.
-template <int D>
-__global__ void k_dist1(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
-                        float* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (tag && tag[i] != layer) return;
-    const uint32_t v = list[i];
-    int slot, loc[3], g[3];
-    decode<D>(G, P, v, slot, loc, g);
-    const float side = layer < 0 ? -1.0f : 1.0f;
-    out[i] = distance<D>(G, P, v, loc, g, P.phi[v], side);
-}
-
-// update_distance pass 2 (Surface.hpp:1076-1121): phi = d and status change.
-template <int D>
-__global__ void k_dist2(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
-                        const float* d, SC sc, Lu buf, Ctr* ctr, int buf_i) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n && (tag == nullptr || tag[i] == layer);
-    uint32_t v = 0;
-    int k = layer;
-    if (valid) {
-        v = list[i];
-        const float x = d[i];
-        k = layer_of(x, G.L, &ctr->nan_events);
-        P.phi[v] = x;
-    }
-    const bool ch = valid && k != layer;
-    if (__ballot(ch) != 0ull && ch) ctr->flag = 1;
-    record_change(ch, v, layer, k, G.L, sc, &ctr->sc_n, buf, &ctr->lu_n[buf_i]);
-}
-
-// ---- flush, deactivation, expansion ---------------------------------------------------------
-
-// flush_status_change (Surface.hpp:1183-1244) with Untrack::retrack / untrack
-// (PartitionedMixin.hpp:489-558).  A partition that loses voxels to untrack is queued for the
-// emptiness check; its background becomes Distance(to) of the untrack that comes last in the
-// reference's flush order (highest from-layer).
-template <int D>
-__global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, int* ukey, int* tflag, int* tlist,
-                        Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = ctrc->sc_n;
-    const bool valid = i < n;
-    bool first = false;
-    int p = 0;
-    if (valid) {
-        const uint32_t v = sc.v[i];
-        const int from = sc.from[i], to = sc.to[i];
-        const int slot = (int)(v / (uint32_t)G.pv);
-        p = P.slot_part[slot];
-        if (to >= -G.L && to <= G.L) {
-            P.lay[v] = (int8_t)to;
-            atomicSub(&P.pcnt[p * G.NL + from + G.L], 1);
-            atomicAdd(&P.pcnt[p * G.NL + to + G.L], 1);
-        } else {
-            P.lay[v] = NONE;
-            P.phi[v] = (float)to;
-            atomicSub(&P.pcnt[p * G.NL + from + G.L], 1);
-            atomicMax(&ukey[p], ((from + G.L) << 16) | ((to + 32768) & 0xffff));
-            first = atomicExch(&tflag[p], 1) == 0;
-        }
-    }
-    const int at = wave_append(&ctr->touched_n, first);
-    if (first) tlist[at] = p;
-}
-
-__global__ void k_deactivate(Geo G, Pool P, const int* tlist, const Ctr* ctrc, int* ukey,
-                             int* tflag, int* free_stack, Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ctrc->touched_n) return;
-    const int p = tlist[i];
-    tflag[p] = 0;
-    const int key = ukey[p];
-    ukey[p] = -1;
-    int sum = 0;
-    for (int k = 0; k < G.NL; k++) sum += P.pcnt[p * G.NL + k];
-    const int s = P.page[p];
-    if (sum == 0 && s >= 0) {
-        P.page[p] = -1;
-        P.slot_part[s] = -1;
-        P.pbg[p] = (float)((key & 0xffff) - 32768);
-        const int t = atomicAdd(&ctr->free_top, 1);
-        free_stack[t] = s;
-    }
-}
-
-// expand_narrow_band (Surface.hpp:1249-1355), phase 1: queue inactive partitions that an
-// expanding voxel's out-of-band neighbours fall in.
-template <int D>
-__global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag, int* nlist,
-                              Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = ctrc->sc_n;
-    bool cand = false;
-    int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
-    if (i < n) {
-        const int from = sc.from[i], to = sc.to[i];
-        const int side = from < 0 ? -1 : 1;
-        if ((from == G.L || from == -G.L) && to == G.L * side - side) {
-            cand = true;
-            decode<D>(G, P, sc.v[i], slot, loc, g);
-        }
-    }
-    for (int a = 0; a < D; a++) {
-        for (int s = -1; s <= 1; s += 2) {
-            bool q_new = false;
-            int pq = 0;
-            if (cand) {
-                int q[3] = {g[0], g[1], g[2]};
-                q[a] += s;
-                if (inside<D>(G, q)) {
-                    pq = part_of<D>(G, q);
-                    if (P.page[pq] < 0 && !in_band(layer_of(P.pbg[pq], G.L, &ctr->nan_events), G.L))
-                        q_new = atomicExch(&nflag[pq], 1) == 0;
-                }
-            }
-            const int at = wave_append(&ctr->need_n, q_new);
-            if (q_new) nlist[at] = pq;
-        }
-    }
-}
-
-// Partition activation from the free-slot stack: the brick is filled with the partition's
-// background (GridMixin.hpp:75-81) and no voxel is tracked.
-__global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* nflag,
-                        int* free_stack, Ctr* ctr) {
-    __shared__ int slot_sh;
-    const int n = ctrc->need_n;
-    for (int j = blockIdx.x; j < n; j += gridDim.x) {
-        const int p = nlist[j];
-        if (threadIdx.x == 0) {
-            const int t = atomicSub(&ctr->free_top, 1) - 1;
-            int s = -1;
-            if (t >= 0) s = free_stack[t];
-            else {
-                atomicAdd(&ctr->free_top, 1);
-                ctr->err = 1;
-            }
-            slot_sh = s;
-            if (s >= 0) {
-                P.page[p] = s;
-                P.slot_part[s] = p;
-            }
-            nflag[p] = 0;
-        }
-        __syncthreads();
-        const int s = slot_sh;
-        if (s >= 0) {
-            const float bg = P.pbg[p];
-            float* ph = P.phi + (size_t)s * G.pv;
-            int8_t* ly = P.lay + (size_t)s * G.pv;
-            for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
-                ph[i] = bg;
-                ly[i] = NONE;
-            }
-            if (threadIdx.x < G.NL) P.pcnt[p * G.NL + threadIdx.x] = 0;
-        }
-        __syncthreads();
-    }
-}
-
-// expand_narrow_band phase 2: claim each out-of-band neighbour once and track it in layer
-// +-L with its distance (the value does not depend on which expanding voxel claims it).
-template <int D>
-__global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ctrc->sc_n) return;
-    const int from = sc.from[i], to = sc.to[i];
-    const int side = from < 0 ? -1 : 1;
-    if (!((from == G.L || from == -G.L) && to == G.L * side - side)) return;
-    int slot, loc[3], g[3];
-    decode<D>(G, P, sc.v[i], slot, loc, g);
-    const int8_t target = (int8_t)(G.L * side);
-    for (int a = 0; a < D; a++) {
-        for (int s = -1; s <= 1; s += 2) {
-            int q[3] = {g[0], g[1], g[2]};
-            q[a] += s;
-            if (!inside<D>(G, q)) continue;
-            const int pq = part_of<D>(G, q);
-            const int sq = P.page[pq];
-            if (sq < 0) continue;  // activation failed (pool exhausted, flagged)
-            const uint32_t vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, q);
-            const float dq = __hip_atomic_load(&P.phi[vq], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-            if (in_band(layer_of(dq, G.L, &ctr->nan_events), G.L)) continue;
-            if (!cas_byte(&P.lay[vq], NONE, target)) continue;
-            int lq[3];
-            for (int b = 0; b < D; b++) lq[b] = q[b] % G.part[b];
-            const float nd = distance<D>(G, P, vq, lq, q, dq, (float)side);
-            __hip_atomic_store(&P.phi[vq], nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            atomicAdd(&P.pcnt[pq * G.NL + target + G.L], 1);
-        }
-    }
-}
-
-// Partitions holding status changes from each layer this round (status_change(k) lists).
-__global__ void k_smask(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ctrc->sc_n) return;
-    const int slot = (int)(sc.v[i] / (uint32_t)G.pv);
-    const int p = P.slot_part[slot];
-    atomicOr(&smask[p], 1u << (sc.from[i] + G.L));
-}
-
 __global__ void k_dmask_pcnt(Geo G, Pool P, unsigned* dmask) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= G.np) return;
@@ -394,7 +59,7 @@ __global__ void k_dmask_pcnt(Geo G, Pool P, unsigned* dmask) {
 __global__ void k_dmask_lu(Geo G, Pool P, Lu lu, const int* n, unsigned* dmask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *n) return;
-    const int slot = (int)(lu.v[i] / (uint32_t)G.pv);
+    const int slot = (int)fdiv(lu.v[i], G.fpv);
     atomicOr(&dmask[P.slot_part[slot]], 1u << (lu.tag[i] + G.L));
 }
 
@@ -408,21 +73,17 @@ __global__ void k_box_zero(Geo G, Pool P, const int* parts, int nparts, int lo0,
         const int p = parts[j];
         const int s = P.page[p];
         if (s < 0) continue;
-        int pc[3], rem = p;
-        for (int a = D - 1; a >= 0; a--) {
-            pc[a] = rem % G.csize[a];
-            rem /= G.csize[a];
-        }
+        int pc[3];
+        part_coords<D>(G, p, pc);
         for (int base = 0; base < G.pv; base += blockDim.x) {
             const int l = base + threadIdx.x;
             bool take = false;
             if (l < G.pv && P.lay[(size_t)s * G.pv + l] == 0) {
                 take = true;
-                int r = l;
-                for (int a = D - 1; a >= 0; a--) {
-                    const int la = r % G.part[a];
-                    r /= G.part[a];
-                    const int ga = pc[a] * G.part[a] + la;
+                int lc[3];
+                leaf_coords<D>(G, l, lc);
+                for (int a = 0; a < D; a++) {
+                    const int ga = pc[a] * G.part[a] + lc[a];
                     if (ga < lo[a] || ga >= hi[a]) take = false;
                 }
             }
@@ -635,21 +296,16 @@ __global__ void k_collect_layer(Geo G, Pool P, int layer, int* out, int cap, int
     const int p = blockIdx.x;
     const int s = P.page[p];
     if (s < 0) return;
-    int pc[3] = {0, 0, 0}, rem = p;
-    for (int a = D - 1; a >= 0; a--) {
-        pc[a] = rem % G.csize[a];
-        rem /= G.csize[a];
-    }
+    int pc[3] = {0, 0, 0};
+    part_coords<D>(G, p, pc);
     for (int base = 0; base < G.pv; base += blockDim.x) {
         const int l = base + threadIdx.x;
         const bool take = l < G.pv && P.lay[(size_t)s * G.pv + l] == layer;
         const int at = wave_append(n, take);
         if (take && at < cap) {
-            int r = l;
-            for (int a = D - 1; a >= 0; a--) {
-                out[at * 3 + a] = pc[a] * G.part[a] + r % G.part[a] + G.offset[a];
-                r /= G.part[a];
-            }
+            int lc[3];
+            leaf_coords<D>(G, l, lc);
+            for (int a = 0; a < D; a++) out[at * 3 + a] = pc[a] * G.part[a] + lc[a] + G.offset[a];
         }
     }
 }
@@ -873,6 +529,10 @@ void felt_surface_release(felt_surface* s) {
         if (p) (void)hipFree(p);
     s->band.release();
     s->band_d.release();
+    s->band_tag.release();
+    s->band_tk.release();
+    s->chg_cnt.release();
+    for (int i = 0; i < 2; i++) s->lu_tk[i].release();
     s->blk_cnt.release();
     s->blk_off.release();
     s->zlist.release();
@@ -893,7 +553,9 @@ void felt_surface_release(felt_surface* s) {
     s->st = nullptr;
 }
 
-Lu felt_lu(felt_surface* s, int i) { return Lu{s->lu_v[i].ptr, s->lu_tag[i].ptr, s->lu_d[i].ptr}; }
+Lu felt_lu(felt_surface* s, int i) {
+    return Lu{s->lu_v[i].ptr, s->lu_tag[i].ptr, s->lu_d[i].ptr, s->lu_tk[i].ptr};
+}
 static SC sc_of(felt_surface* s) { return SC{s->sc_v.ptr, s->sc_from.ptr, s->sc_to.ptr}; }
 
 // Capacity of every per-step list: no list in a step holds more entries than the band had at
@@ -905,16 +567,19 @@ static int ensure_step_capacity(felt_surface* s, size_t band) {
         if ((rc = s->lu_v[i].ensure(n))) return rc;
         if ((rc = s->lu_tag[i].ensure(n))) return rc;
         if ((rc = s->lu_d[i].ensure(n))) return rc;
+        if ((rc = s->lu_tk[i].ensure(n))) return rc;
     }
     if ((rc = s->sc_v.ensure(n))) return rc;
     if ((rc = s->sc_from.ensure(n))) return rc;
     if ((rc = s->sc_to.ensure(n))) return rc;
     if ((rc = s->zlist.ensure(n))) return rc;
     if ((rc = s->zdelta.ensure(n))) return rc;
+    if ((rc = s->chg_cnt.ensure(2 * (n / (BT * ITEMS) + 2)))) return rc;
     return FELT_OK;
 }
 
-// Band compaction: per-layer segments of the band list, s->band_base[k]..[k+1].
+// Band compaction: per-layer segments of the band list, s->band_base[k]..[k+1], one block per
+// active partition.
 int felt_compact_band(felt_surface* s) {
     const Geo& G = s->G;
     int rc;
@@ -924,36 +589,32 @@ int felt_compact_band(felt_surface* s) {
     if ((rc = read_ctr(s))) return rc;
     const int nact = s->hc.act_n;
     s->band_base.assign(G.NL + 1, 0);
-    const int chunk = std::min(G.pv, 4096);
-    const int nchunk = (G.pv + chunk - 1) / chunk;
-    const int nblocks = nact * nchunk;
-    s->band_base.assign(G.NL + 1, 0);
-    if (nblocks == 0) return FELT_OK;
-    if ((rc = s->blk_cnt.ensure((size_t)nblocks * G.NL + G.NL + 1))) return rc;
-    if ((rc = s->blk_off.ensure((size_t)nblocks * G.NL))) return rc;
+    if (nact == 0) return FELT_OK;
+    if ((rc = s->blk_cnt.ensure((size_t)nact * G.NL + G.NL + 1))) return rc;
+    if ((rc = s->blk_off.ensure((size_t)nact * G.NL))) return rc;
+    const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
     prof_begin(s, PK_COMPACT, (long long)nact * G.pv);
-    DISPATCH(G.D, k_band_count, dim3(nblocks), dim3(TPB), G, s->P, s->act, nchunk, chunk,
-             s->blk_cnt.ptr);
+    DISPATCH(G.D, k_band_count, dim3(nact), dim3(bt), G, s->P, s->act, s->blk_cnt.ptr);
     prof_end(s);
-    int* base = s->blk_cnt.ptr + (size_t)nblocks * G.NL;
-    LAUNCH(k_band_scan, dim3(1), dim3(TPB), s->blk_cnt.ptr, nblocks, G.NL, s->blk_off.ptr, base);
+    int* base = s->blk_cnt.ptr + (size_t)nact * G.NL;
+    LAUNCH(k_band_scan, dim3(1), dim3(BT), s->blk_cnt.ptr, nact, G.NL, s->blk_off.ptr, base);
     HIPX(hipMemcpyAsync(s->band_base.data(), base, (G.NL + 1) * sizeof(int),
                         hipMemcpyDeviceToHost, s->st));
     HIPX(hipStreamSynchronize(s->st));
     const size_t total = (size_t)s->band_base[G.NL];
     if ((rc = s->band.ensure(total + 64))) return rc;
     if ((rc = s->band_d.ensure(total + 64))) return rc;
-    DISPATCH(G.D, k_band_scatter, dim3(nblocks), dim3(TPB), G, s->P, s->act, nchunk, chunk,
-             s->blk_off.ptr, s->band.ptr);
+    if ((rc = s->band_tag.ensure(total + 64))) return rc;
+    if ((rc = s->band_tk.ensure(total + 64))) return rc;
+    DISPATCH(G.D, k_band_scatter, dim3(nact), dim3(bt), G, s->P, s->act, s->blk_off.ptr,
+             s->band.ptr, s->band_tag.ptr);
     return FELT_OK;
 }
 
-// status_change partition sets are those of the last round only; the buffer `buf_i` collects
-// this round's in-band targets.
-static int distance_round(felt_surface* s, bool from_band, int look_i, int look_n, int buf_i) {
+// The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
+// 1..L, each a Jacobi pass (distance into d, then phi = d and tk = its layer).
+static int distance_passes(felt_surface* s, bool from_band, int look_i, int look_n) {
     const Geo& G = s->G;
-    const SC sc = sc_of(s);
-    const Lu buf = felt_lu(s, buf_i);
     for (int pass = 0; pass < 2; pass++) {
         const int side = pass == 0 ? -1 : 1;
         for (int m = 1; m <= G.L; m++) {
@@ -961,17 +622,20 @@ static int distance_round(felt_surface* s, bool from_band, int look_i, int look_
             const uint32_t* list;
             const int8_t* tag;
             float* d;
+            int8_t* tk;
             int n;
             if (from_band) {
                 const int k = layer + G.L;
                 list = s->band.ptr + s->band_base[k];
                 tag = nullptr;
                 d = s->band_d.ptr + s->band_base[k];
+                tk = s->band_tk.ptr + s->band_base[k];
                 n = s->band_base[k + 1] - s->band_base[k];
             } else {
                 list = s->lu_v[look_i].ptr;
                 tag = s->lu_tag[look_i].ptr;
                 d = s->lu_d[look_i].ptr;
+                tk = s->lu_tk[look_i].ptr;
                 n = look_n;
             }
             if (n == 0) continue;
@@ -979,15 +643,31 @@ static int distance_round(felt_surface* s, bool from_band, int look_i, int look_
             DISPATCH(G.D, k_dist1, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d);
             prof_end(s);
             prof_begin(s, PK_DIST2, n);
-            DISPATCH(G.D, k_dist2, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d,
-                     sc, buf, s->ctr, buf_i);
+            LAUNCH(k_dist2, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d, tk,
+                   &s->ctr->nan_events);
             prof_end(s);
         }
     }
     return FELT_OK;
 }
 
-// flush_status_change then expand_narrow_band, sized by the status-change count bound.
+// Status changes of the round's processed list -> status-change records and the buffer
+// lookup `buf_i` (its previous contents are replaced).
+static int compact_changes(felt_surface* s, const uint32_t* v, const int8_t* tag,
+                           const int8_t* tk, int n, int buf_i) {
+    const Geo& G = s->G;
+    if (n == 0) return FELT_OK;
+    const int nb = (n + BT * ITEMS - 1) / (BT * ITEMS);
+    int rc;
+    if ((rc = s->chg_cnt.ensure(2 * (size_t)nb))) return rc;
+    LAUNCH(k_chg_count, dim3(nb), dim3(BT), G, tag, tk, n, s->chg_cnt.ptr, s->ctr);
+    LAUNCH(k_chg_scan, dim3(1), dim3(BT), s->chg_cnt.ptr, nb, &s->ctr->sc_n, &s->ctr->lu_n[buf_i]);
+    LAUNCH(k_chg_scatter, dim3(nb), dim3(BT), G, v, tag, tk, n, s->chg_cnt.ptr, sc_of(s),
+           felt_lu(s, buf_i));
+    return FELT_OK;
+}
+
+// flush_status_change then expand_narrow_band, sized by a bound on the status-change count.
 static int flush_expand(felt_surface* s, int sc_bound) {
     const Geo& G = s->G;
     const SC sc = sc_of(s);
@@ -996,8 +676,7 @@ static int flush_expand(felt_surface* s, int sc_bound) {
     LAUNCH(k_smask, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->smask);
     HIPX(hipMemsetAsync(&s->ctr->touched_n, 0, sizeof(int), s->st));
     prof_begin(s, PK_FLUSH, 0);
-    DISPATCH(G.D, k_flush, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag, s->tlist,
-             s->ctr);
+    LAUNCH(k_flush, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag, s->tlist, s->ctr);
     prof_end(s);
     LAUNCH(k_deactivate, dim3(blocks_for(std::min<int64_t>(G.np, sc_bound))), dim3(TPB), G, s->P,
            s->tlist, s->ctr, s->ukey, s->tflag, s->free_stack, s->ctr);
@@ -1017,6 +696,20 @@ static int reset_round(felt_surface* s) {
     HIPX(hipMemsetAsync(&s->ctr->flag, 0, sizeof(int), s->st));
     HIPX(hipMemsetAsync(s->smask, 0, s->G.np * sizeof(unsigned), s->st));
     return FELT_OK;
+}
+
+// A round over lookup `look` (rounds after the first): distance passes over its entries,
+// changes into lookup `buf`, then flush and expand.
+static int lookup_round(felt_surface* s, int look, int look_n, int buf) {
+    int rc;
+    if (look_n > 0)
+        LAUNCH(k_tk_init, dim3(blocks_for(look_n)), dim3(TPB), s->lu_tag[look].ptr,
+               s->lu_tk[look].ptr, look_n);
+    if ((rc = distance_passes(s, false, look, look_n))) return rc;
+    if ((rc = compact_changes(s, s->lu_v[look].ptr, s->lu_tag[look].ptr, s->lu_tk[look].ptr,
+                              look_n, buf)))
+        return rc;
+    return flush_expand(s, look_n);
 }
 
 // update_start (Surface.hpp:393-399): delta, affected, buffer and status-change state cleared.
@@ -1045,15 +738,17 @@ static int global_step(felt_surface* s, DeltaFn&& delta_fn) {
     if ((rc = ensure_step_capacity(s, band))) return rc;
     LAUNCH(k_dmask_pcnt, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->dmask);
     const int z0 = s->band_base[G.L], zn = s->band_base[G.L + 1] - z0;
-    // update_zero_layer into the affected buffer (index 1)
+    // update_zero_layer; its changes go to the affected buffer (lookup 1)
     if (zn > 0) {
         if ((rc = delta_fn(s->band.ptr + z0, zn, s->band_d.ptr + z0))) return rc;
         prof_begin(s, PK_APPLY, zn);
-        DISPATCH(G.D, k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->band.ptr + z0,
-                 s->band_d.ptr + z0, zn, sc_of(s), felt_lu(s, 1), s->ctr, 1);
+        LAUNCH(k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->band.ptr + z0,
+               s->band_d.ptr + z0, zn, s->band_tk.ptr + z0, &s->ctr->nan_events);
         prof_end(s);
     }
-    if ((rc = distance_round(s, true, 0, 0, 1))) return rc;
+    if ((rc = distance_passes(s, true, 0, 0))) return rc;
+    if ((rc = compact_changes(s, s->band.ptr, s->band_tag.ptr, s->band_tk.ptr, (int)band, 1)))
+        return rc;
     if ((rc = flush_expand(s, (int)band))) return rc;
     if ((rc = read_ctr(s))) return rc;
     int rounds = 1;
@@ -1061,9 +756,7 @@ static int global_step(felt_surface* s, DeltaFn&& delta_fn) {
         int look = 1, buf = 0;
         while (true) {
             if ((rc = reset_round(s))) return rc;
-            const int look_n = s->hc.lu_n[look];
-            if ((rc = distance_round(s, false, look, look_n, buf))) return rc;
-            if ((rc = flush_expand(s, look_n))) return rc;
+            if ((rc = lookup_round(s, look, s->hc.lu_n[look], buf))) return rc;
             HIPX(hipMemsetAsync(&s->ctr->lu_n[look], 0, sizeof(int), s->st));
             std::swap(look, buf);
             rounds++;
@@ -1126,27 +819,36 @@ int felt_update_end_impl(felt_surface* s) {
     const Geo& G = s->G;
     int rc;
     const int zn = s->z_host_n;
-    // calc_affected into lookup 0 (m_grid_affected)
+    // calc_affected into lookup 0 (m_grid_affected); its first zn entries are the delta list
     Lu A = felt_lu(s, 0);
     if (zn > 0)
         LAUNCH(k_lu_seed, dim3(blocks_for(zn)), dim3(TPB), A, s->zlist.ptr, zn, s->amark,
                &s->ctr->lu_n[0]);
     HIPX(hipMemsetAsync(&s->ctr->front[0], 0, sizeof(int), s->st));
     LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, 1);
+    const int bfs_grid = std::min(4096, std::max(1, blocks_for(s->band_cap_hint)));
     for (int j = 1; j <= G.L; j++) {
-        DISPATCH(G.D, k_affected_round, dim3(std::min(4096, std::max(1, blocks_for(s->band_cap_hint)))),
-                 dim3(TPB), G, s->P, A, s->ctr, 0, j, s->amark);
+        prof_begin(s, PK_AFFECTED, 0);
+        DISPATCH(G.D, k_affected_round, dim3(bfs_grid), dim3(TPB), G, s->P, A, s->ctr, 0, j,
+                 s->amark);
+        prof_end(s);
         LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, j + 1);
     }
     if ((rc = read_ctr(s))) return rc;
     const int an = s->hc.lu_n[0];
     LAUNCH(k_unmark, dim3(blocks_for(an)), dim3(TPB), A, &s->ctr->lu_n[0], s->amark);
     LAUNCH(k_dmask_lu, dim3(blocks_for(an)), dim3(TPB), G, s->P, A, &s->ctr->lu_n[0], s->dmask);
-    if (zn > 0)
-        DISPATCH(G.D, k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr,
-                 s->zdelta.ptr, zn, sc_of(s), felt_lu(s, 1), s->ctr, 1);
+    if (an > 0)
+        LAUNCH(k_tk_init, dim3(blocks_for(an)), dim3(TPB), A.tag, A.tk, an);
+    if (zn > 0) {
+        prof_begin(s, PK_APPLY, zn);
+        LAUNCH(k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr, s->zdelta.ptr, zn,
+               A.tk, &s->ctr->nan_events);
+        prof_end(s);
+    }
     int look = 0, buf = 1;
-    if ((rc = distance_round(s, false, look, an, buf))) return rc;
+    if ((rc = distance_passes(s, false, look, an))) return rc;
+    if ((rc = compact_changes(s, A.v, A.tag, A.tk, an, buf))) return rc;
     if ((rc = flush_expand(s, an))) return rc;
     if ((rc = read_ctr(s))) return rc;
     int rounds = 1;
@@ -1154,9 +856,7 @@ int felt_update_end_impl(felt_surface* s) {
         HIPX(hipMemsetAsync(&s->ctr->lu_n[look], 0, sizeof(int), s->st));
         std::swap(look, buf);
         if ((rc = reset_round(s))) return rc;
-        const int look_n = s->hc.lu_n[look];
-        if ((rc = distance_round(s, false, look, look_n, buf))) return rc;
-        if ((rc = flush_expand(s, look_n))) return rc;
+        if ((rc = lookup_round(s, look, s->hc.lu_n[look], buf))) return rc;
         rounds++;
         if ((rc = read_ctr(s))) return rc;
         if (rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
