@@ -35,7 +35,7 @@ struct felt_polys {
     long long last_spx = 0;
     DBuf<uint8_t> cmask;
     DBuf<uint16_t> own;
-    DBuf<int> vbase, tbase, tot, dparts;
+    DBuf<int> vbase, tbase, tot, dparts, pstart, vst, item_j, item_l, nv, nt, vleaf;
     DBuf<long long> dseg;
     DBuf<uint8_t> bits;
 };

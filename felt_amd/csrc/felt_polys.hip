@@ -128,94 +128,6 @@ __device__ int edge_owner(const Geo& G, const Pool& P, int s, const int pc[3], c
     return best;
 }
 
-// Block-wide exclusive scan of two int arrays in place, returning the totals.  Thread t owns
-// a contiguous segment; segment sums are scanned with wavefront shuffles.
-__device__ void block_scan2(int* a, int* b, int n, int* tot_a, int* tot_b) {
-    __shared__ int wa[TPB / 64], wb[TPB / 64];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int seg = (n + TPB - 1) / TPB;
-    const int i0 = min(n, t * seg), i1 = min(n, i0 + seg);
-    int sa = 0, sb = 0;
-    for (int i = i0; i < i1; i++) {
-        sa += a[i];
-        sb += b[i];
-    }
-    int xa = sa, xb = sb;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int ya = __shfl_up(xa, o), yb = __shfl_up(xb, o);
-        if (lane >= o) {
-            xa += ya;
-            xb += yb;
-        }
-    }
-    if (lane == 63) {
-        wa[wave] = xa;
-        wb[wave] = xb;
-    }
-    __syncthreads();
-    int pa = 0, pb = 0, ta = 0, tb = 0;
-    for (int w = 0; w < TPB / 64; w++) {
-        if (w < wave) {
-            pa += wa[w];
-            pb += wb[w];
-        }
-        ta += wa[w];
-        tb += wb[w];
-    }
-    int ra = pa + xa - sa, rb = pb + xb - sb;
-    for (int i = i0; i < i1; i++) {
-        const int va = a[i], vb = b[i];
-        a[i] = ra;
-        b[i] = rb;
-        ra += va;
-        rb += vb;
-    }
-    if (t == 0) {
-        *tot_a = ta;
-        *tot_b = tb;
-    }
-}
-
-// Pass 1: per band voxel, the cube mask, the edges whose vertex it owns, and its vertex and
-// simplex counts; then per-partition exclusive scans.
-template <int D>
-__global__ void k_march_count(Geo G, Pool P, const int* parts, uint8_t* cmask, uint16_t* own,
-                              int* vbase, int* tbase, int* totals) {
-    const int j = blockIdx.x;
-    const int p = parts[j];
-    const int s = P.page[p];
-    uint8_t* cm = cmask + (size_t)j * G.pv;
-    uint16_t* ow = own + (size_t)j * G.pv;
-    int* vb = vbase + (size_t)j * G.pv;
-    int* tb = tbase + (size_t)j * G.pv;
-    int pc[3] = {0, 0, 0};
-    part_coords<D>(G, p, pc);
-    for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
-        int nv = 0, nt = 0;
-        unsigned m = 0, om = 0;
-        if (s >= 0 && P.lay[(size_t)s * G.pv + l] != NONE) {
-            int g[3] = {0, 0, 0};
-            leaf_coords<D>(G, l, g);
-            for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
-            m = cube_mask<D>(G, P, g);
-            const unsigned em = McT<D>::edges_of(m);
-            for (int e = 0; e < McT<D>::nedge; e++) {
-                if (!((em >> e) & 1)) continue;
-                int eo;
-                if (edge_owner<D>(G, P, s, pc, g, e, &eo) == l) om |= 1u << e;
-            }
-            nv = __popc(om);
-            nt = McT<D>::nsimplex(m);
-        }
-        cm[l] = (uint8_t)m;
-        ow[l] = (uint16_t)om;
-        vb[l] = nv;
-        tb[l] = nt;
-    }
-    __syncthreads();
-    block_scan2(vb, tb, G.pv, &totals[2 * j], &totals[2 * j + 1]);
-}
-
 // Felt's N-linear interpolation in world coordinates (NumericMixin.hpp:403-481).
 template <int D>
 __device__ float interp_w(const Geo& G, const Pool& P, const float w[3]) {
@@ -341,51 +253,240 @@ __device__ void make_vertex(const Geo& G, const Pool& P, const int ga[3], int ax
     }
 }
 
-// Pass 2: vertices of owned edges and simplices of every band cube.
+// ---- data-parallel march: one thread per band voxel of the partitions being marched ----------
+
+constexpr int BT = 1024;
+constexpr int ITEMS = 4;
+
+__device__ __forceinline__ int block_excl_scan(int val, int* total) {
+    __shared__ int wsum[BT / 64];
+    __shared__ int wpre[BT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = val;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        const int nw = BT / 64;
+        const int w = lane < nw ? wsum[lane] : 0;
+        int z = w;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(z, o);
+            if (lane >= o) z += y;
+        }
+        if (lane < nw) wpre[lane] = z - w;
+        if (lane == nw - 1) wsum[0] = z;
+    }
+    __syncthreads();
+    const int res = wpre[wave] + x - val;
+    *total = wsum[0];
+    __syncthreads();
+    return res;
+}
+
+// Band voxels per marched partition.
 template <int D>
-__global__ void k_march_emit(Geo G, Pool P, const int* parts, const uint8_t* cmask,
-                             const uint16_t* own, const int* vbase, const int* tbase,
-                             const long long* seg_v, const long long* seg_t, float* vout,
-                             uint32_t* tout) {
-    const int j = blockIdx.x;
+__global__ void k_mb_count(Geo G, Pool P, const int* parts, int* cnt) {
+    __shared__ int c;
+    if (threadIdx.x == 0) c = 0;
+    __syncthreads();
+    const int s = P.page[parts[blockIdx.x]];
+    int mine = 0;
+    if (s >= 0)
+        for (int l = threadIdx.x; l < G.pv; l += blockDim.x)
+            mine += P.lay[(size_t)s * G.pv + l] != NONE;
+    atomicAdd(&c, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+}
+
+// Single block: start[j] = exclusive prefix of cnt, start[nb] = total.
+__global__ void k_mb_scan(const int* cnt, int nb, int* start) {
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += BT) {
+        const int b = b0 + threadIdx.x;
+        int tot;
+        const int ex = block_excl_scan(b < nb ? cnt[b] : 0, &tot);
+        if (b < nb) start[b] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[nb] = carry;
+}
+
+template <int D>
+__global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, int* item_j,
+                             int* item_l) {
+    __shared__ int c;
+    if (threadIdx.x == 0) c = start[blockIdx.x];
+    __syncthreads();
+    const int s = P.page[parts[blockIdx.x]];
+    if (s < 0) return;
+    for (int l = threadIdx.x; l < G.pv; l += blockDim.x)
+        if (P.lay[(size_t)s * G.pv + l] != NONE) {
+            const int at = atomicAdd(&c, 1);
+            item_j[at] = blockIdx.x;
+            item_l[at] = l;
+        }
+}
+
+// Cube mask, owned edges and vertex / simplex counts of each band voxel.
+template <int D>
+__global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
+                          int M, uint8_t* cmask, uint16_t* own, int* nv, int* nt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int j = item_j[i], l = item_l[i];
     const int p = parts[j];
     const int s = P.page[p];
-    if (s < 0) return;
-    const uint8_t* cm = cmask + (size_t)j * G.pv;
-    const uint16_t* ow = own + (size_t)j * G.pv;
-    const int* vb = vbase + (size_t)j * G.pv;
-    const int* tb = tbase + (size_t)j * G.pv;
-    const int vw = D == 3 ? 6 : 2;
-    float* vo = vout + seg_v[j] * vw;
-    uint32_t* to = tout + seg_t[j] * D;
-    int pc[3] = {0, 0, 0};
+    int pc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
     part_coords<D>(G, p, pc);
-    for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
-        if (P.lay[(size_t)s * G.pv + l] == NONE) continue;
-        const unsigned m = cm[l];
-        const int ns = McT<D>::nsimplex(m);
-        if (ns == 0) continue;
-        int g[3] = {0, 0, 0};
-        leaf_coords<D>(G, l, g);
-        for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
-        const unsigned om = ow[l];
-        int rank = 0;
-        for (int e = 0; e < McT<D>::nedge; e++) {
-            if (!((om >> e) & 1)) continue;
-            int ga[3] = {0, 0, 0};
-            for (int a = 0; a < D; a++) ga[a] = g[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
-            make_vertex<D>(G, P, ga, McT<D>::edge(e, D), vo + (size_t)(vb[l] + rank) * vw);
-            rank++;
+    leaf_coords<D>(G, l, g);
+    for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
+    const unsigned m = cube_mask<D>(G, P, g);
+    const unsigned em = McT<D>::edges_of(m);
+    unsigned om = 0;
+    for (int e = 0; e < McT<D>::nedge; e++) {
+        if (!((em >> e) & 1)) continue;
+        int eo;
+        if (edge_owner<D>(G, P, s, pc, g, e, &eo) == l) om |= 1u << e;
+    }
+    cmask[(size_t)j * G.pv + l] = (uint8_t)m;
+    own[(size_t)j * G.pv + l] = (uint16_t)om;
+    nv[i] = __popc(om);
+    nt[i] = McT<D>::nsimplex(m);
+}
+
+// Device-wide exclusive scan of two int arrays: block sums, top-level scan, apply.
+__global__ void k_s2_blocks(const int* a, const int* b, int n, int* bs) {
+    const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
+    int sa = 0, sb = 0;
+    for (int k = 0; k < ITEMS; k++) {
+        const int i = base + k * BT;
+        if (i < n) {
+            sa += a[i];
+            sb += b[i];
         }
-        for (int i = 0; i < ns; i++) {
-            for (int k = 0; k < D; k++) {
-                const int e = McT<D>::simplex_edge(m, i, k);
-                int eo;
-                const int o = edge_owner<D>(G, P, s, pc, g, e, &eo);
-                const unsigned oom = ow[o];
-                const int idx = vb[o] + __popc(oom & ((1u << eo) - 1u));
-                to[(size_t)(tb[l] + i) * D + k] = (uint32_t)idx;
-            }
+    }
+    int ta, tb;
+    block_excl_scan(sa, &ta);
+    block_excl_scan(sb, &tb);
+    if (threadIdx.x == 0) {
+        bs[2 * blockIdx.x] = ta;
+        bs[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+__global__ void k_s2_top(int* bs, int nblk, int* tot) {
+    __shared__ int carry[2];
+    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nblk; b0 += BT) {
+        const int b = b0 + threadIdx.x;
+        const int x = b < nblk ? bs[2 * b] : 0, y = b < nblk ? bs[2 * b + 1] : 0;
+        int tx, ty;
+        const int ex = block_excl_scan(x, &tx);
+        const int ey = block_excl_scan(y, &ty);
+        if (b < nblk) {
+            bs[2 * b] = carry[0] + ex;
+            bs[2 * b + 1] = carry[1] + ey;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            carry[0] += tx;
+            carry[1] += ty;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tot[0] = carry[0];
+        tot[1] = carry[1];
+    }
+}
+
+// Each thread scans ITEMS consecutive entries so that the output order is the input order.
+__global__ void k_s2_apply(const int* a, const int* b, int n, const int* bs, int* ea, int* eb) {
+    const int i0 = (blockIdx.x * BT + threadIdx.x) * ITEMS;
+    int sa = 0, sb = 0;
+    for (int k = 0; k < ITEMS; k++)
+        if (i0 + k < n) {
+            sa += a[i0 + k];
+            sb += b[i0 + k];
+        }
+    int ta, tb;
+    int ra = block_excl_scan(sa, &ta) + bs[2 * blockIdx.x];
+    int rb = block_excl_scan(sb, &tb) + bs[2 * blockIdx.x + 1];
+    for (int k = 0; k < ITEMS; k++)
+        if (i0 + k < n) {
+            const int va = a[i0 + k], vb = b[i0 + k];
+            ea[i0 + k] = ra;
+            eb[i0 + k] = rb;
+            ra += va;
+            rb += vb;
+        }
+}
+
+// Vertex base of each band voxel, by leaf, for the owner lookups of the emit pass.
+__global__ void k_m_leafbase(const int* item_j, const int* item_l, int M, int pv, const int* vb,
+                             int* vleaf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) vleaf[(size_t)item_j[i] * pv + item_l[i]] = vb[i];
+}
+
+// First vertex / simplex of each marched partition; entry nb holds the totals.
+__global__ void k_m_partstart(const int* start, int nb, const int* vb, const int* tb, int M,
+                              const int* tot, int* vst, int* tst) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > nb) return;
+    const int it = start[j];
+    vst[j] = it < M ? vb[it] : tot[0];
+    tst[j] = it < M ? tb[it] : tot[1];
+}
+
+// Vertices of owned edges and simplices of every band cube (Poly.hpp:198-287), simplex vertex
+// indices local to the partition.
+template <int D>
+__global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
+                         int M, const uint8_t* cmask, const uint16_t* own, const int* vb,
+                         const int* tb, const int* vleaf, const int* vst, float* vout,
+                         uint32_t* tout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const int j = item_j[i], l = item_l[i];
+    const size_t jb = (size_t)j * G.pv;
+    const unsigned m = cmask[jb + l];
+    const int ns = McT<D>::nsimplex(m);
+    if (ns == 0) return;
+    const int p = parts[j];
+    const int s = P.page[p];
+    int pc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+    part_coords<D>(G, p, pc);
+    leaf_coords<D>(G, l, g);
+    for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
+    const int vw = D == 3 ? 6 : 2;
+    const unsigned om = own[jb + l];
+    int rank = 0;
+    for (int e = 0; e < McT<D>::nedge; e++) {
+        if (!((om >> e) & 1)) continue;
+        int ga[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) ga[a] = g[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
+        make_vertex<D>(G, P, ga, McT<D>::edge(e, D), vout + (size_t)(vb[i] + rank) * vw);
+        rank++;
+    }
+    const int base = vst[j];
+    for (int t = 0; t < ns; t++) {
+        for (int k = 0; k < D; k++) {
+            const int e = McT<D>::simplex_edge(m, t, k);
+            int eo;
+            const int o = edge_owner<D>(G, P, s, pc, g, e, &eo);
+            const int idx = vleaf[jb + o] + __popc((unsigned)own[jb + o] & ((1u << eo) - 1u)) - base;
+            tout[(size_t)(tb[i] + t) * D + k] = (uint32_t)idx;
         }
     }
 }
@@ -490,6 +591,13 @@ void felt_polys_release(felt_polys* q) {
     q->dparts.release();
     q->dseg.release();
     q->bits.release();
+    q->pstart.release();
+    q->vst.release();
+    q->item_j.release();
+    q->item_l.release();
+    q->nv.release();
+    q->nt.release();
+    q->vleaf.release();
 }
 
 // notify (Polys.hpp:92-133).
@@ -623,49 +731,74 @@ int felt_polys_march_impl(felt_polys* q) {
             k = PolyKid{};
         }
     }
-    const size_t per = (size_t)G.pv * (1 + 2 + 4 + 4);
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(4096, (size_t)(96u << 20) / per));
+    const size_t per = (size_t)G.pv * (1 + 2 + 4);
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(8192, (size_t)(192u << 20) / per));
+    const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
     q->last_spx = 0;
     for (size_t b0 = 0; b0 < todo.size(); b0 += batch) {
         const int nb = (int)std::min(batch, todo.size() - b0);
         if ((rc = q->dparts.ensure(nb))) return rc;
-        if ((rc = q->cmask.ensure((size_t)nb * G.pv))) return rc;
-        if ((rc = q->own.ensure((size_t)nb * G.pv))) return rc;
-        if ((rc = q->vbase.ensure((size_t)nb * G.pv))) return rc;
-        if ((rc = q->tbase.ensure((size_t)nb * G.pv))) return rc;
-        if ((rc = q->tot.ensure((size_t)nb * 2))) return rc;
-        if ((rc = q->dseg.ensure((size_t)nb * 2))) return rc;
+        if ((rc = q->pstart.ensure(nb + 1))) return rc;
+        if ((rc = q->vst.ensure(2 * (nb + 1) + 2))) return rc;
         HIPX(hipMemcpyAsync(q->dparts.ptr, todo.data() + b0, nb * sizeof(int),
                             hipMemcpyHostToDevice, st));
-        DISPATCHP(G.D, k_march_count, dim3(nb), dim3(TPB), G, s->P, q->dparts.ptr, q->cmask.ptr,
-                  q->own.ptr, q->vbase.ptr, q->tbase.ptr, q->tot.ptr);
-        std::vector<int> tot(nb * 2);
-        HIPX(hipMemcpyAsync(tot.data(), q->tot.ptr, nb * 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        DISPATCHP(G.D, k_mb_count, dim3(nb), dim3(bt), G, s->P, q->dparts.ptr, q->pstart.ptr);
+        hipLaunchKernelGGL(k_mb_scan, dim3(1), dim3(BT), 0, st, q->pstart.ptr, nb, q->pstart.ptr);
+        int M = 0;
+        HIPX(hipMemcpyAsync(&M, q->pstart.ptr + nb, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPX(hipStreamSynchronize(st));
-        long long sum_v = 0, sum_t = 0;
-        for (int j = 0; j < nb; j++) {
-            sum_v += tot[2 * j];
-            sum_t += tot[2 * j + 1];
-        }
-        if ((rc = arena_reserve(q, sum_v, sum_t))) return rc;
-        std::vector<long long> seg(nb * 2);
+        if (M == 0) continue;
+        const int nblk = (M + BT * ITEMS - 1) / (BT * ITEMS);
+        if ((rc = q->item_j.ensure(M))) return rc;
+        if ((rc = q->item_l.ensure(M))) return rc;
+        if ((rc = q->nv.ensure(M))) return rc;
+        if ((rc = q->nt.ensure(M))) return rc;
+        if ((rc = q->vbase.ensure(M))) return rc;
+        if ((rc = q->tbase.ensure(M))) return rc;
+        if ((rc = q->tot.ensure(2 * (size_t)nblk + 2))) return rc;
+        if ((rc = q->cmask.ensure((size_t)nb * G.pv))) return rc;
+        if ((rc = q->own.ensure((size_t)nb * G.pv))) return rc;
+        if ((rc = q->vleaf.ensure((size_t)nb * G.pv))) return rc;
+        DISPATCHP(G.D, k_mb_scatter, dim3(nb), dim3(bt), G, s->P, q->dparts.ptr, q->pstart.ptr,
+                  q->item_j.ptr, q->item_l.ptr);
+        prof_begin(s, PK_MARCH_COUNT, M);
+        DISPATCHP(G.D, k_m_count, dim3((M + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
+                  q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->nv.ptr, q->nt.ptr);
+        prof_end(s);
+        int* bs = q->tot.ptr;
+        int* totals = q->tot.ptr + 2 * nblk;
+        hipLaunchKernelGGL(k_s2_blocks, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs);
+        hipLaunchKernelGGL(k_s2_top, dim3(1), dim3(BT), 0, st, bs, nblk, totals);
+        hipLaunchKernelGGL(k_s2_apply, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs,
+                           q->vbase.ptr, q->tbase.ptr);
+        hipLaunchKernelGGL(k_m_leafbase, dim3((M + TPB - 1) / TPB), dim3(TPB), 0, st, q->item_j.ptr,
+                           q->item_l.ptr, M, G.pv, q->vbase.ptr, q->vleaf.ptr);
+        int* vst = q->vst.ptr;
+        int* tst = q->vst.ptr + (nb + 1);
+        hipLaunchKernelGGL(k_m_partstart, dim3((nb + 1 + TPB - 1) / TPB), dim3(TPB), 0, st,
+                           q->pstart.ptr, nb, q->vbase.ptr, q->tbase.ptr, M, totals, vst, tst);
+        std::vector<int> hv(2 * (nb + 1));
+        HIPX(hipMemcpyAsync(hv.data(), vst, 2 * (nb + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+        const long long NV = hv[nb], NT = hv[(nb + 1) + nb];
+        if ((rc = arena_reserve(q, NV, NT))) return rc;
+        const long long vb0 = q->vtop, tb0 = q->ttop;
         for (int j = 0; j < nb; j++) {
             PolyKid& k = q->kids[todo[b0 + j]];
-            k.voff = q->vtop;
-            k.nv = tot[2 * j];
-            k.toff = q->ttop;
-            k.nt = tot[2 * j + 1];
-            seg[j] = k.voff;
-            seg[nb + j] = k.toff;
-            q->vtop += k.nv;
-            q->ttop += k.nt;
+            k.voff = vb0 + hv[j];
+            k.nv = hv[j + 1] - hv[j];
+            k.toff = tb0 + hv[(nb + 1) + j];
+            k.nt = hv[(nb + 1) + j + 1] - hv[(nb + 1) + j];
         }
-        q->last_spx += sum_t;
-        HIPX(hipMemcpyAsync(q->dseg.ptr, seg.data(), nb * 2 * sizeof(long long),
-                            hipMemcpyHostToDevice, st));
-        DISPATCHP(G.D, k_march_emit, dim3(nb), dim3(TPB), G, s->P, q->dparts.ptr, q->cmask.ptr,
-                  q->own.ptr, q->vbase.ptr, q->tbase.ptr, q->dseg.ptr, q->dseg.ptr + nb,
-                  q->varena.ptr, q->tarena.ptr);
+        q->vtop += NV;
+        q->ttop += NT;
+        q->last_spx += NT;
+        prof_begin(s, PK_MARCH_EMIT, M);
+        DISPATCHP(G.D, k_m_emit, dim3((M + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
+                  q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->vbase.ptr,
+                  q->tbase.ptr, q->vleaf.ptr, vst, q->varena.ptr + vb0 * q->vw,
+                  q->tarena.ptr + tb0 * G.D);
+        prof_end(s);
     }
     HIPX(hipStreamSynchronize(st));
     std::swap(q->pending, q->done);
