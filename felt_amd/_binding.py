@@ -123,6 +123,7 @@ class Binding:
             "surface_stats": (C.c_int, [_P, C.POINTER(felt_stats)]),
             "surface_sync": (C.c_int, [_P]),
             "surface_step_info": (C.c_int, [_P, _U64P, _U64P]),
+            "surface_set_graph": (C.c_int, [_P, C.c_int]),
             "polys_notify": (C.c_int, [_P]),
             "polys_march": (C.c_int, [_P]),
             "polys_invalidate": (C.c_int, [_P]),
@@ -144,7 +145,8 @@ class Binding:
             sig["polys_destroy"] = (C.c_int, [_P])
         self.fn = {}
         for name, (res, args) in sig.items():
-            if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info"):
+            if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info",
+                                                            "surface_set_graph"):
                 continue
             f = getattr(L, prefix + name)
             f.restype = res
@@ -338,6 +340,12 @@ class Surface:
         st = felt_stats()
         self.b.call("surface_stats", self.h, C.byref(st))
         return st
+
+    def set_graph(self, enable: bool) -> None:
+        """Capture global device-functor steps as one HIP graph (default) or launch kernels
+        one by one."""
+        if "surface_set_graph" in self.b.fn:
+            self.b.call("surface_set_graph", self.h, int(bool(enable)))
 
     def step_info(self):
         """(voxel updates, rounds) of the last update call."""

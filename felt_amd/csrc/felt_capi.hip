@@ -402,6 +402,12 @@ API int felt_surface_stats(felt_surface* s, felt_stats* st) {
     return FELT_OK;
 }
 
+API int felt_surface_set_graph(felt_surface* s, int enable) {
+    CHECK_S(s);
+    s->use_graph = enable != 0;
+    return FELT_OK;
+}
+
 API int felt_surface_step_info(felt_surface* s, uint64_t* voxel_updates, uint64_t* rounds) {
     CHECK_S(s);
     *voxel_updates = s->last_updates;

@@ -8,19 +8,25 @@
 namespace felt {
 
 // Device counters of one surface.
+// Every list length a kernel needs lives here, so launches never wait on the host.  A span
+// {lo, hi} is a half-open range of list positions.
 struct Ctr {
-    int free_top;   // free-slot stack height
-    int sc_n;       // status-change records this round
-    int lu_n[2];    // entries of the two lookups (affected / affected buffer)
-    int flag;       // a non-zero-layer status change happened this round
-    int need_n;     // partitions to activate
-    int touched_n;  // partitions that lost a voxel to untrack
+    int free_top;        // free-slot stack height
+    int sc_n;            // status-change records this round
+    int lu_span[2][2];   // {0, entries} of the two lookups (affected / affected buffer)
+    int flag;            // a non-zero-layer status change happened this round
+    int need_n;          // partitions to activate
+    int touched_n;       // partitions that lost a voxel to untrack
     int nan_events;
-    int act_n;      // active partitions (compaction)
-    int z_n;        // zero-layer delta list entries
-    int err;        // 1: brick pool exhausted
+    int act_n;           // active partitions
+    int z_span[2];       // {0, zero-layer delta entries} of a bounded update
+    int band_span[2];    // {0, band entries}
+    int err;             // 1: brick pool exhausted
     int misc;
-    int front[20];  // breadth-first frontier bounds of calc_affected
+    int one;             // constant 1: the live flag of a round that always runs
+    int live[8];         // round r of an unrolled step runs while live[r] != 0
+    int band_total;      // tracked voxels, refreshed at the end of an unrolled step
+    int front[20];       // breadth-first frontier bounds of calc_affected
 };
 
 // Status-change record: the reference's StatusChangeGrid entry (Surface.hpp:1138-1178).

@@ -7,6 +7,10 @@
 // the status-change records and the next round's buffer by count / scan / scatter passes, so no
 // list append contends on a global atomic.  Per-partition counters are updated with one atomic
 // per (wavefront, partition, layer).
+//
+// Every kernel reads its list bounds from device memory (a span {lo, hi}) and loops with a
+// grid stride, and every kernel of a round takes the round's live flag, so a whole step can be
+// launched without reading anything back; rounds that turn out not to be needed exit at once.
 #pragma once
 
 #include "felt_kernels.h"
@@ -67,27 +71,77 @@ __device__ __forceinline__ void agg_pcnt(int* pcnt, int NL, int p, int dec, int 
     }
 }
 
-// ---- band compaction (one block per active partition) ---------------------------------------
+// ---- step and round control -----------------------------------------------------------------
+
+// Start of a step: counters of the previous step cleared, first round live.
+__global__ void k_step_reset(Ctr* c) {
+    c->sc_n = 0;
+    c->flag = 0;
+    c->lu_span[0][0] = c->lu_span[0][1] = 0;
+    c->lu_span[1][0] = c->lu_span[1][1] = 0;
+    c->z_span[0] = c->z_span[1] = 0;
+    c->band_span[0] = c->band_span[1] = 0;
+    c->touched_n = 0;
+    c->need_n = 0;
+    c->act_n = 0;
+    c->one = 1;
+    c->live[0] = 1;
+    for (int r = 1; r < 8; r++) c->live[r] = 0;
+}
+
+// Round r runs only if round r-1 ran and changed an outer layer (Surface.hpp:752-760); its
+// status-change state starts empty (m_grid_status_change.reset).
+__global__ void k_round_gate(Ctr* c, int r) {
+    const int live = c->live[r - 1] && c->flag;
+    c->live[r] = live;
+    if (live) {
+        c->flag = 0;
+        c->sc_n = 0;
+        c->touched_n = 0;
+        c->need_n = 0;
+    }
+}
+
+__global__ void k_fill_u32_live(unsigned* p, int n, unsigned v, const int* live) {
+    if (!*live) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Empties a lookup (plookup->reset, Surface.hpp:424, 758).
+__global__ void k_lu_clear(Ctr* c, int lu, const int* live) {
+    if (*live) c->lu_span[lu][1] = 0;
+}
+
+__global__ void k_flush_reset(Ctr* c, const int* live) {
+    if (*live) {
+        c->touched_n = 0;
+        c->need_n = 0;
+    }
+}
+
+// ---- band compaction (one block per partition; inactive partitions count nothing) -----------
 
 template <int D>
-__global__ void k_band_count(Geo G, Pool P, const int* act, int* counts) {
+__global__ void k_band_count(Geo G, Pool P, int* counts) {
     __shared__ int c[32];
-    const int p = act[blockIdx.x];
+    const int p = blockIdx.x;
     const int s = P.page[p];
     if (threadIdx.x < 32) c[threadIdx.x] = 0;
     __syncthreads();
-    const int8_t* lay = P.lay + (size_t)s * G.pv;
-    for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
-        const int k = lay[i];
-        if (k != NONE) atomicAdd(&c[k + G.L], 1);
+    if (s >= 0) {
+        const int8_t* lay = P.lay + (size_t)s * G.pv;
+        for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
+            const int k = lay[i];
+            if (k != NONE) atomicAdd(&c[k + G.L], 1);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < G.NL) counts[blockIdx.x * G.NL + threadIdx.x] = c[threadIdx.x];
+    if (threadIdx.x < G.NL) counts[p * G.NL + threadIdx.x] = c[threadIdx.x];
 }
 
 // Single block: layer-major exclusive scan of counts[n][NL] into offsets; base[k] is where
-// layer k's segment starts and base[NL] the band total.
-__global__ void k_band_scan(const int* counts, int n, int NL, int* offsets, int* base) {
+// layer k's segment starts and base[NL] the band total (also the band span's end).
+__global__ void k_band_scan(const int* counts, int n, int NL, int* offsets, int* base, Ctr* c) {
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
@@ -105,16 +159,20 @@ __global__ void k_band_scan(const int* counts, int n, int NL, int* offsets, int*
             __syncthreads();
         }
     }
-    if (threadIdx.x == 0) base[NL] = carry;
+    if (threadIdx.x == 0) {
+        base[NL] = carry;
+        c->band_span[0] = 0;
+        c->band_span[1] = carry;
+    }
 }
 
 template <int D>
-__global__ void k_band_scatter(Geo G, Pool P, const int* act, const int* offsets, uint32_t* band,
-                               int8_t* tag) {
+__global__ void k_band_scatter(Geo G, Pool P, const int* offsets, uint32_t* band, int8_t* tag) {
     __shared__ int c[32];
-    const int p = act[blockIdx.x];
+    const int p = blockIdx.x;
     const int s = P.page[p];
-    if (threadIdx.x < G.NL) c[threadIdx.x] = offsets[blockIdx.x * G.NL + threadIdx.x];
+    if (s < 0) return;
+    if (threadIdx.x < G.NL) c[threadIdx.x] = offsets[p * G.NL + threadIdx.x];
     __syncthreads();
     const int8_t* lay = P.lay + (size_t)s * G.pv;
     for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
@@ -129,64 +187,76 @@ __global__ void k_band_scatter(Geo G, Pool P, const int* act, const int* offsets
 
 // ---- per-round processing --------------------------------------------------------------------
 
-__global__ void k_tk_init(const int8_t* tag, int8_t* tk, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) tk[i] = tag[i];
+#define FELT_SPAN_LOOP(i, rng)                                                                 \
+    for (int i = (rng)[0] + blockIdx.x * blockDim.x + threadIdx.x, i##_hi = (rng)[1]; i < i##_hi; \
+         i += gridDim.x * blockDim.x)
+
+__global__ void k_tk_init(const int8_t* tag, int8_t* tk, const int* rng, const int* live) {
+    if (!*live) return;
+    FELT_SPAN_LOOP(i, rng) tk[i] = tag[i];
 }
 
 __device__ __forceinline__ int8_t clamp_k(int k) { return (int8_t)(k < -100 ? -100 : (k > 100 ? 100 : k)); }
 
 template <int D>
-__global__ void k_speed(Geo G, Pool P, const uint32_t* list, int n, Speed sp, float* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = list[i];
-    int slot, loc[3], g[3];
-    decode<D>(G, P, v, slot, loc, g);
-    out[i] = eval_speed<D>(G, P, sp, v, loc, g);
+__global__ void k_speed(Geo G, Pool P, const uint32_t* list, const int* rng, const Speed* sp,
+                        float* out) {
+    const Speed s = *sp;
+    FELT_SPAN_LOOP(i, rng) {
+        const uint32_t v = list[i];
+        int slot, loc[3], g[3];
+        decode<D>(G, P, v, slot, loc, g);
+        out[i] = eval_speed<D>(G, P, s, v, loc, g);
+    }
 }
 
-// update_zero_layer (Surface.hpp:920-979): phi += delta; tk = new layer.
-__global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta, int n,
+// update_zero_layer (Surface.hpp:920-979): phi += delta; tk = new layer.  tk is indexed like
+// list, from the span's start.
+__global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta, const int* rng,
                         int8_t* tk, int* nan_ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = list[i];
-    const float nv = P.phi[v] + delta[i];
-    P.phi[v] = nv;
-    tk[i] = clamp_k(layer_of(nv, G.L, nan_ctr));
+    FELT_SPAN_LOOP(i, rng) {
+        const uint32_t v = list[i];
+        const float nv = P.phi[v] + delta[i];
+        P.phi[v] = nv;
+        tk[i] = clamp_k(layer_of(nv, G.L, nan_ctr));
+    }
 }
 
 // update_distance pass 1 (Surface.hpp:1029-1073).
 template <int D>
-__global__ void k_dist1(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
-                        float* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (tag && tag[i] != layer) return;
-    const uint32_t v = list[i];
-    int slot, loc[3], g[3];
-    decode<D>(G, P, v, slot, loc, g);
+__global__ void k_dist1(Geo G, Pool P, const uint32_t* list, const int8_t* tag, const int* rng,
+                        int layer, float* out, const int* live) {
+    if (!*live) return;
     const float side = layer < 0 ? -1.0f : 1.0f;
-    out[i] = distance<D>(G, P, v, loc, g, P.phi[v], side);
+    FELT_SPAN_LOOP(i, rng) {
+        if (tag && tag[i] != layer) continue;
+        const uint32_t v = list[i];
+        int slot, loc[3], g[3];
+        decode<D>(G, P, v, slot, loc, g);
+        out[i] = distance<D>(G, P, v, loc, g, P.phi[v], side);
+    }
 }
 
 // update_distance pass 2 (Surface.hpp:1076-1121): phi = d; tk = new layer.
-__global__ void k_dist2(Geo G, Pool P, const uint32_t* list, const int8_t* tag, int n, int layer,
-                        const float* d, int8_t* tk, int* nan_ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (tag && tag[i] != layer) return;
-    const uint32_t v = list[i];
-    const float x = d[i];
-    P.phi[v] = x;
-    tk[i] = clamp_k(layer_of(x, G.L, nan_ctr));
+__global__ void k_dist2(Geo G, Pool P, const uint32_t* list, const int8_t* tag, const int* rng,
+                        int layer, const float* d, int8_t* tk, int* nan_ctr, const int* live) {
+    if (!*live) return;
+    FELT_SPAN_LOOP(i, rng) {
+        if (tag && tag[i] != layer) continue;
+        const uint32_t v = list[i];
+        const float x = d[i];
+        P.phi[v] = x;
+        tk[i] = clamp_k(layer_of(x, G.L, nan_ctr));
+    }
 }
 
 // Change compaction, pass 1: per block the number of status changes and of in-band targets;
-// raises the round flag for a change out of a non-zero layer (Surface.hpp:1116).
-__global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, int n, int* counts,
-                            Ctr* ctr) {
+// raises the round flag for a change out of a non-zero layer (Surface.hpp:1116).  The grid is
+// sized for the list's capacity; blocks past the span's end count zero.
+__global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, const int* rng, int* counts,
+                            Ctr* ctr, const int* live) {
+    if (!*live) return;
+    const int n = rng[1];
     const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
     int nsc = 0, nbf = 0;
     bool fl = false;
@@ -213,8 +283,9 @@ __global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, int n, i
 }
 
 // Pass 2 (single block): exclusive scan of the per-block counts; the totals become the round's
-// status-change count and the buffer's entry count.
-__global__ void k_chg_scan(int* counts, int nb, int* sc_n, int* buf_n) {
+// status-change count and the buffer lookup's span.
+__global__ void k_chg_scan(int* counts, int nb, Ctr* ctr, int buf, const int* live) {
+    if (!*live) return;
     __shared__ int carry[2];
     if (threadIdx.x == 0) carry[0] = carry[1] = 0;
     __syncthreads();
@@ -236,15 +307,18 @@ __global__ void k_chg_scan(int* counts, int nb, int* sc_n, int* buf_n) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        *sc_n = carry[0];
-        *buf_n = carry[1];
+        ctr->sc_n = carry[0];
+        ctr->lu_span[buf][0] = 0;
+        ctr->lu_span[buf][1] = carry[1];
     }
 }
 
 // Pass 3: status-change records (status_change, Surface.hpp:1138-1178) and the buffer entries
 // of the in-band targets, in list order within each block.
-__global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, const int8_t* tk, int n,
-                              const int* offsets, SC sc, Lu buf) {
+__global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, const int8_t* tk,
+                              const int* rng, const int* offsets, SC sc, Lu buf, const int* live) {
+    if (!*live) return;
+    const int n = rng[1];
     const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
     int nsc = 0, nbf = 0;
 #pragma unroll
@@ -279,27 +353,36 @@ __global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, cons
     }
 }
 
+// Loops over a device-sized list with a grid stride, keeping every wavefront's lanes together
+// (the wave-wide helpers need all lanes).
+#define FELT_WAVE_LOOP(i, valid, n)                                                            \
+    for (int i##_b = blockIdx.x * blockDim.x, i##_n = (n); i##_b < i##_n;                      \
+         i##_b += gridDim.x * blockDim.x)                                                      \
+        for (int i = i##_b + (int)threadIdx.x, valid = i < i##_n, i##_once = 1; i##_once;     \
+             i##_once = 0)
+
 // Partitions holding status changes from each layer this round (status_change(k) lists), one
 // atomic per distinct partition of a wavefront.
-__global__ void k_smask(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = ctrc->sc_n;
-    int p = -1, from = 0;
-    if (i < n) {
-        p = P.slot_part[fdiv(sc.v[i], G.fpv)];
-        from = sc.from[i] + G.L;
-    }
-    const int lane = __lane_id();
-    unsigned long long todo = __ballot(p >= 0);
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int p0 = __shfl(p, leader);
-        const bool mine = p >= 0 && p == p0;
-        unsigned bits = 0;
-        for (int k = 0; k < G.NL; k++)
-            if (__ballot(mine && from == k)) bits |= 1u << k;
-        if (lane == leader) atomicOr(&smask[p0], bits);
-        todo &= ~__ballot(mine);
+__global__ void k_smask(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask, const int* live) {
+    if (!*live) return;
+    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
+        int p = -1, from = 0;
+        if (valid) {
+            p = P.slot_part[fdiv(sc.v[i], G.fpv)];
+            from = sc.from[i] + G.L;
+        }
+        const int lane = __lane_id();
+        unsigned long long todo = __ballot(p >= 0);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int p0 = __shfl(p, leader);
+            const bool mine = p >= 0 && p == p0;
+            unsigned bits = 0;
+            for (int k = 0; k < G.NL; k++)
+                if (__ballot(mine && from == k)) bits |= 1u << k;
+            if (lane == leader) atomicOr(&smask[p0], bits);
+            todo &= ~__ballot(mine);
+        }
     }
 }
 
@@ -308,60 +391,63 @@ __global__ void k_smask(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask) 
 // emptiness check; its background becomes Distance(to) of the untrack that comes last in the
 // reference's flush order (the highest from-layer).
 __global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, int* ukey, int* tflag, int* tlist,
-                        Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = ctrc->sc_n;
-    int p = -1, dec = -1, inc = -1, key = -1;
-    if (i < n) {
-        const uint32_t v = sc.v[i];
-        const int from = sc.from[i], to = sc.to[i];
-        p = P.slot_part[fdiv(v, G.fpv)];
-        dec = from + G.L;
-        if (to >= -G.L && to <= G.L) {
-            P.lay[v] = (int8_t)to;
-            inc = to + G.L;
-        } else {
-            P.lay[v] = NONE;
-            P.phi[v] = (float)to;
-            key = ((from + G.L) << 16) | ((to + 32768) & 0xffff);
+                        Ctr* ctr, const int* live) {
+    if (!*live) return;
+    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
+        int p = -1, dec = -1, inc = -1, key = -1;
+        if (valid) {
+            const uint32_t v = sc.v[i];
+            const int from = sc.from[i], to = sc.to[i];
+            p = P.slot_part[fdiv(v, G.fpv)];
+            dec = from + G.L;
+            if (to >= -G.L && to <= G.L) {
+                P.lay[v] = (int8_t)to;
+                inc = to + G.L;
+            } else {
+                P.lay[v] = NONE;
+                P.phi[v] = (float)to;
+                key = ((from + G.L) << 16) | ((to + 32768) & 0xffff);
+            }
         }
-    }
-    agg_pcnt(P.pcnt, G.NL, p, dec, inc);
-    // untracks: one atomicMax per distinct partition of the wave
-    const int lane = __lane_id();
-    const int pu = key >= 0 ? p : -1;
-    unsigned long long todo = __ballot(pu >= 0);
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const int p0 = __shfl(pu, leader);
-        const bool mine = pu >= 0 && pu == p0;
-        int kmax = mine ? key : -1;
-        for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
-        if (lane == leader) {
-            atomicMax(&ukey[p0], kmax);
-            if (atomicExch(&tflag[p0], 1) == 0) tlist[atomicAdd(&ctr->touched_n, 1)] = p0;
+        agg_pcnt(P.pcnt, G.NL, p, dec, inc);
+        // untracks: one atomicMax per distinct partition of the wave
+        const int lane = __lane_id();
+        const int pu = key >= 0 ? p : -1;
+        unsigned long long todo = __ballot(pu >= 0);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int p0 = __shfl(pu, leader);
+            const bool mine = pu >= 0 && pu == p0;
+            int kmax = mine ? key : -1;
+            for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
+            if (lane == leader) {
+                atomicMax(&ukey[p0], kmax);
+                if (atomicExch(&tflag[p0], 1) == 0) tlist[atomicAdd(&ctr->touched_n, 1)] = p0;
+            }
+            todo &= ~__ballot(mine);
         }
-        todo &= ~__ballot(mine);
     }
 }
 
 __global__ void k_deactivate(Geo G, Pool P, const int* tlist, const Ctr* ctrc, int* ukey,
-                             int* tflag, int* free_stack, Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ctrc->touched_n) return;
-    const int p = tlist[i];
-    tflag[p] = 0;
-    const int key = ukey[p];
-    ukey[p] = -1;
-    int sum = 0;
-    for (int k = 0; k < G.NL; k++) sum += P.pcnt[p * G.NL + k];
-    const int s = P.page[p];
-    if (sum == 0 && s >= 0) {
-        P.page[p] = -1;
-        P.slot_part[s] = -1;
-        P.pbg[p] = (float)((key & 0xffff) - 32768);
-        const int t = atomicAdd(&ctr->free_top, 1);
-        free_stack[t] = s;
+                             int* tflag, int* free_stack, Ctr* ctr, const int* live) {
+    if (!*live) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ctrc->touched_n;
+         i += gridDim.x * blockDim.x) {
+        const int p = tlist[i];
+        tflag[p] = 0;
+        const int key = ukey[p];
+        ukey[p] = -1;
+        int sum = 0;
+        for (int k = 0; k < G.NL; k++) sum += P.pcnt[p * G.NL + k];
+        const int s = P.page[p];
+        if (sum == 0 && s >= 0) {
+            P.page[p] = -1;
+            P.slot_part[s] = -1;
+            P.pbg[p] = (float)((key & 0xffff) - 32768);
+            const int t = atomicAdd(&ctr->free_top, 1);
+            free_stack[t] = s;
+        }
     }
 }
 
@@ -374,31 +460,32 @@ __device__ __forceinline__ bool expanding(const Geo& G, int from, int to) {
 // expanding voxel's out-of-band neighbours fall in.
 template <int D>
 __global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag, int* nlist,
-                              Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = ctrc->sc_n;
-    bool cand = false;
-    int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
-    if (i < n && expanding(G, sc.from[i], sc.to[i])) {
-        cand = true;
-        decode<D>(G, P, sc.v[i], slot, loc, g);
-    }
-    for (int a = 0; a < D; a++) {
-        for (int s = -1; s <= 1; s += 2) {
-            bool q_new = false;
-            int pq = 0;
-            if (cand) {
-                int q[3] = {g[0], g[1], g[2]};
-                q[a] += s;
-                if (inside<D>(G, q)) {
-                    pq = part_of<D>(G, q);
-                    if (P.page[pq] < 0 &&
-                        !in_band(layer_of(P.pbg[pq], G.L, &ctr->nan_events), G.L))
-                        q_new = atomicExch(&nflag[pq], 1) == 0;
+                              Ctr* ctr, const int* live) {
+    if (!*live) return;
+    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
+        bool cand = false;
+        int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        if (valid && expanding(G, sc.from[i], sc.to[i])) {
+            cand = true;
+            decode<D>(G, P, sc.v[i], slot, loc, g);
+        }
+        for (int a = 0; a < D; a++) {
+            for (int s = -1; s <= 1; s += 2) {
+                bool q_new = false;
+                int pq = 0;
+                if (cand) {
+                    int q[3] = {g[0], g[1], g[2]};
+                    q[a] += s;
+                    if (inside<D>(G, q)) {
+                        pq = part_of<D>(G, q);
+                        if (P.page[pq] < 0 &&
+                            !in_band(layer_of(P.pbg[pq], G.L, &ctr->nan_events), G.L))
+                            q_new = atomicExch(&nflag[pq], 1) == 0;
+                    }
                 }
+                const int at = wave_append(&ctr->need_n, q_new);
+                if (q_new) nlist[at] = pq;
             }
-            const int at = wave_append(&ctr->need_n, q_new);
-            if (q_new) nlist[at] = pq;
         }
     }
 }
@@ -406,7 +493,8 @@ __global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag,
 // Partition activation from the free-slot stack: the brick is filled with the partition's
 // background (GridMixin.hpp:75-81) and no voxel is tracked.
 __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* nflag,
-                        int* free_stack, Ctr* ctr) {
+                        int* free_stack, Ctr* ctr, const int* live) {
+    if (!*live) return;
     __shared__ int slot_sh;
     const int n = ctrc->need_n;
     for (int j = blockIdx.x; j < n; j += gridDim.x) {
@@ -445,46 +533,59 @@ __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* n
 // expand_narrow_band phase 2: claim each out-of-band neighbour once and track it in layer
 // +-L with its distance (the value does not depend on which expanding voxel claims it).
 template <int D>
-__global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool cand = false;
-    int side = 1, slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
-    if (i < ctrc->sc_n && expanding(G, sc.from[i], sc.to[i])) {
-        cand = true;
-        side = sc.from[i] < 0 ? -1 : 1;
-        decode<D>(G, P, sc.v[i], slot, loc, g);
-    }
-    const int8_t target = (int8_t)(G.L * side);
-    for (int a = 0; a < D; a++) {
-        for (int s = -1; s <= 1; s += 2) {
-            int claimed_p = -1;
-            if (cand) {
-                int q[3] = {g[0], g[1], g[2]};
-                q[a] += s;
-                if (inside<D>(G, q)) {
-                    int pq, lq;
-                    locate<D>(G, q, pq, lq);
-                    const int sq = P.page[pq];
-                    if (sq >= 0) {  // else activation failed (pool exhausted, flagged)
-                        const uint32_t vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
-                        const float dq = __hip_atomic_load(&P.phi[vq], __ATOMIC_RELAXED,
-                                                           __HIP_MEMORY_SCOPE_AGENT);
-                        if (!in_band(layer_of(dq, G.L, &ctr->nan_events), G.L) &&
-                            cas_byte(&P.lay[vq], NONE, target)) {
-                            int lc[3];
-                            leaf_coords<D>(G, lq, lc);
-                            const float nd = distance<D>(G, P, vq, lc, q, dq, (float)side);
-                            __hip_atomic_store(&P.phi[vq], nd, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                            claimed_p = pq;
+__global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, const int* live) {
+    if (!*live) return;
+    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
+        bool cand = false;
+        int side = 1, slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        if (valid && expanding(G, sc.from[i], sc.to[i])) {
+            cand = true;
+            side = sc.from[i] < 0 ? -1 : 1;
+            decode<D>(G, P, sc.v[i], slot, loc, g);
+        }
+        const int8_t target = (int8_t)(G.L * side);
+        for (int a = 0; a < D; a++) {
+            for (int s = -1; s <= 1; s += 2) {
+                int claimed_p = -1;
+                if (cand) {
+                    int q[3] = {g[0], g[1], g[2]};
+                    q[a] += s;
+                    if (inside<D>(G, q)) {
+                        int pq, lq;
+                        locate<D>(G, q, pq, lq);
+                        const int sq = P.page[pq];
+                        if (sq >= 0) {  // else activation failed (pool exhausted, flagged)
+                            const uint32_t vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
+                            const float dq = __hip_atomic_load(&P.phi[vq], __ATOMIC_RELAXED,
+                                                               __HIP_MEMORY_SCOPE_AGENT);
+                            if (!in_band(layer_of(dq, G.L, &ctr->nan_events), G.L) &&
+                                cas_byte(&P.lay[vq], NONE, target)) {
+                                int lc[3];
+                                leaf_coords<D>(G, lq, lc);
+                                const float nd = distance<D>(G, P, vq, lc, q, dq, (float)side);
+                                __hip_atomic_store(&P.phi[vq], nd, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                                claimed_p = pq;
+                            }
                         }
                     }
                 }
+                agg_pcnt(P.pcnt, G.NL, claimed_p, -1, claimed_p >= 0 ? target + G.L : -1);
             }
-            agg_pcnt(P.pcnt, G.NL, claimed_p, -1, claimed_p >= 0 ? target + G.L : -1);
         }
     }
 }
+
+// Tracked voxels in total (sum of pcnt), for sizing the next step's lists.
+__global__ void k_band_total2(Geo G, Pool P, Ctr* c) {
+    int s = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < G.np; p += gridDim.x * blockDim.x)
+        for (int k = 0; k < G.NL; k++) s += P.pcnt[p * G.NL + k];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&c->band_total, s);
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
 
 }  // namespace step
 }  // namespace felt

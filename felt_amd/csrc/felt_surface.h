@@ -29,9 +29,12 @@ struct ProfRec {
     int kernel;
     long long items;
 };
+// items >= 0: entries processed; items < 0: band segment -(items+1), resolved after the step.
 struct Prof {
     bool on = false;
+    bool capturing = false;
     std::vector<ProfRec> pending;
+    std::vector<ProfRec> graph_recs;  // event pairs recorded by every replay of the step graph
     std::vector<hipEvent_t> spare;
     long long launches[PK_COUNT] = {};
     double ms[PK_COUNT] = {};
@@ -76,7 +79,18 @@ struct felt_surface {
     DBuf<int> tmp_i;
     DBuf<float> tmp_f;
     DBuf<uint32_t> tmp_u;
-    int64_t band_cap_hint = 0;
+    int* bbase = nullptr;           // band segment starts, NL+1 entries (device)
+    felt::Speed* d_speed = nullptr;  // speed descriptor read by the speed kernel (device)
+    felt::Speed* h_speed = nullptr;  // pinned staging copy
+    size_t list_cap = 0;             // capacity of every per-step list
+    std::vector<int> hbase;          // host copy of bbase after a profiled step
+    bool band_known = false;         // band_now is the current tracked-voxel count
+    size_t band_now = 0;
+    int unroll = 3;                  // rounds of a global step launched without host reads
+    bool use_graph = true;
+    bool graph_valid = false;
+    bool graph_prof = false;         // the captured graph holds profiling event nodes
+    hipGraphExec_t graph_exec = nullptr;
     bool in_update = false;
     std::vector<std::pair<std::array<int, 3>, float>> pending_deltas;
     std::map<std::array<int, 3>, size_t> pending_index;
@@ -96,7 +110,6 @@ int felt_fail(int code, const std::string& msg);
 int felt_surface_init(felt_surface* s);
 void felt_surface_release(felt_surface* s);
 felt::Lu felt_lu(felt_surface* s, int i);
-int felt_compact_band(felt_surface* s);
 int felt_update_start_impl(felt_surface* s);
 int felt_update_global(felt_surface* s, const felt_speed* fn);
 int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);

@@ -56,11 +56,12 @@ __global__ void k_dmask_pcnt(Geo G, Pool P, unsigned* dmask) {
     dmask[p] = m;
 }
 
-__global__ void k_dmask_lu(Geo G, Pool P, Lu lu, const int* n, unsigned* dmask) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *n) return;
-    const int slot = (int)fdiv(lu.v[i], G.fpv);
-    atomicOr(&dmask[P.slot_part[slot]], 1u << (lu.tag[i] + G.L));
+__global__ void k_dmask_lu(Geo G, Pool P, Lu lu, const int* rng, unsigned* dmask) {
+    for (int i = rng[0] + blockIdx.x * blockDim.x + threadIdx.x; i < rng[1];
+         i += gridDim.x * blockDim.x) {
+        const int slot = (int)fdiv(lu.v[i], G.fpv);
+        atomicOr(&dmask[P.slot_part[slot]], 1u << (lu.tag[i] + G.L));
+    }
 }
 
 // ---- bounded update: zero layer in a box, affected-set search -------------------------------
@@ -87,22 +88,25 @@ __global__ void k_box_zero(Geo G, Pool P, const int* parts, int nparts, int lo0,
                     if (ga < lo[a] || ga >= hi[a]) take = false;
                 }
             }
-            const int at = wave_append(&ctr->z_n, take);
+            const int at = wave_append(&ctr->z_span[1], take);
             if (take) z[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)l;
         }
     }
 }
 
-__global__ void k_lu_seed(Lu lu, const uint32_t* z, int n, uint8_t* amark, int* lu_n) {
+__global__ void k_lu_seed(Lu lu, const uint32_t* z, int n, uint8_t* amark, int* lu_span) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     lu.v[i] = z[i];
     lu.tag[i] = 0;
     amark[z[i]] = 1;
-    if (i == 0) *lu_n = n;
+    if (i == 0) {
+        lu_span[0] = 0;
+        lu_span[1] = n;
+    }
 }
 
-__global__ void k_snap_front(Ctr* ctr, int lu_i, int j) { ctr->front[j] = ctr->lu_n[lu_i]; }
+__global__ void k_snap_front(Ctr* ctr, int lu_i, int j) { ctr->front[j] = ctr->lu_span[lu_i][1]; }
 
 // calc_affected (Surface.hpp:780-913), one breadth-first round: face neighbours of the last
 // round's voxels whose current layer is in the band join the set once.
@@ -137,7 +141,7 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
                         }
                     }
                 }
-                const int at = wave_append(&ctr->lu_n[lu_i], add);
+                const int at = wave_append(&ctr->lu_span[lu_i][1], add);
                 if (add) {
                     lu.v[at] = vq;
                     lu.tag[at] = (int8_t)kq;
@@ -147,9 +151,10 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
     }
 }
 
-__global__ void k_unmark(Lu lu, const int* n, uint8_t* amark) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < *n) amark[lu.v[i]] = 0;
+__global__ void k_unmark(Lu lu, const int* rng, uint8_t* amark) {
+    for (int i = rng[0] + blockIdx.x * blockDim.x + threadIdx.x; i < rng[1];
+         i += gridDim.x * blockDim.x)
+        amark[lu.v[i]] = 0;
 }
 
 // ---- seeding, loading, read-back ------------------------------------------------------------
@@ -343,12 +348,17 @@ __global__ void k_vox_set(const uint32_t* ids, const float* vals, int m, float* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < m) vox[ids[i]] = vals[i];
 }
-__global__ void k_vox_gather(const uint32_t* list, int n, float* vox, float* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+__global__ void k_vox_gather(const uint32_t* list, const int* rng, float* vox, float* out) {
+    for (int i = rng[0] + blockIdx.x * blockDim.x + threadIdx.x; i < rng[1];
+         i += gridDim.x * blockDim.x) {
         out[i] = vox[list[i]];
         vox[list[i]] = 0.0f;
     }
+}
+
+__global__ void k_set_span(int* span, int n) {
+    span[0] = 0;
+    span[1] = n;
 }
 
 __global__ void k_band_total(Geo G, Pool P, int* out) {
@@ -397,28 +407,56 @@ void prof_begin(felt_surface* s, int kernel, long long items) {
     if (!s->prof.on) return;
     ProfRec r{prof_event(s), prof_event(s), kernel, items};
     (void)hipEventRecord(r.a, s->st);
-    s->prof.pending.push_back(r);
+    (s->prof.capturing ? s->prof.graph_recs : s->prof.pending).push_back(r);
 }
 
 void prof_end(felt_surface* s) {
-    if (!s->prof.on || s->prof.pending.empty()) return;
-    (void)hipEventRecord(s->prof.pending.back().b, s->st);
+    if (!s->prof.on) return;
+    std::vector<ProfRec>& v = s->prof.capturing ? s->prof.graph_recs : s->prof.pending;
+    if (!v.empty()) (void)hipEventRecord(v.back().b, s->st);
+}
+
+static long long prof_items(felt_surface* s, long long items) {
+    if (items >= 0) return items;
+    const int k = (int)(-items - 1);
+    if ((int)s->hbase.size() <= k + 1) return 0;
+    return s->hbase[k + 1] - s->hbase[k];
+}
+
+static int prof_account(felt_surface* s, const ProfRec& r) {
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, r.a, r.b));
+    s->prof.launches[r.kernel]++;
+    s->prof.ms[r.kernel] += ms;
+    s->prof.items[r.kernel] += prof_items(s, r.items);
+    return FELT_OK;
 }
 
 int prof_collect(felt_surface* s) {
     if (s->prof.pending.empty()) return FELT_OK;
     HIPX(hipStreamSynchronize(s->st));
     for (ProfRec& r : s->prof.pending) {
-        float ms = 0.f;
-        HIPX(hipEventElapsedTime(&ms, r.a, r.b));
-        s->prof.launches[r.kernel]++;
-        s->prof.ms[r.kernel] += ms;
-        s->prof.items[r.kernel] += r.items;
+        if (int rc = prof_account(s, r)) return rc;
         s->prof.spare.push_back(r.a);
         s->prof.spare.push_back(r.b);
     }
     s->prof.pending.clear();
     return FELT_OK;
+}
+
+// After a replay of the step graph: the event pairs it recorded.
+static int prof_collect_graph(felt_surface* s) {
+    for (const ProfRec& r : s->prof.graph_recs)
+        if (int rc = prof_account(s, r)) return rc;
+    return FELT_OK;
+}
+
+static void prof_drop_graph(felt_surface* s) {
+    for (ProfRec& r : s->prof.graph_recs) {
+        s->prof.spare.push_back(r.a);
+        s->prof.spare.push_back(r.b);
+    }
+    s->prof.graph_recs.clear();
 }
 
 template <class T>
@@ -457,6 +495,11 @@ template struct DBuf<uint8_t>;
 
 static int read_ctr(felt_surface* s) {
     HIPX(hipMemcpyAsync(&s->hc, s->ctr, sizeof(Ctr), hipMemcpyDeviceToHost, s->st));
+    if (s->prof.on && s->bbase) {
+        s->hbase.resize(s->G.NL + 1);
+        HIPX(hipMemcpyAsync(s->hbase.data(), s->bbase, (s->G.NL + 1) * sizeof(int),
+                            hipMemcpyDeviceToHost, s->st));
+    }
     HIPX(hipStreamSynchronize(s->st));
     if (s->hc.err) return felt_fail(FELT_E_NOMEM, "brick pool exhausted");
     return FELT_OK;
@@ -489,6 +532,11 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->dmask, np * sizeof(unsigned)));
     HIPX(hipMalloc(&s->smask, np * sizeof(unsigned)));
     HIPX(hipMalloc(&s->act, np * sizeof(int)));
+    HIPX(hipMalloc(&s->bbase, (G.NL + 2) * sizeof(int)));
+    HIPX(hipMalloc(&s->d_speed, sizeof(Speed)));
+    HIPX(hipHostMalloc((void**)&s->h_speed, sizeof(Speed), hipHostMallocDefault));
+    if (s->blk_cnt.ensure((size_t)np * G.NL + G.NL + 1)) return FELT_E_DEVICE;
+    if (s->blk_off.ensure((size_t)np * G.NL)) return FELT_E_DEVICE;
     const int gb = 1024;
     LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->P.page, np, -1);
     float bg = (float)(G.L + 1);
@@ -508,6 +556,8 @@ int felt_surface_init(felt_surface* s) {
     Ctr c;
     std::memset(&c, 0, sizeof(c));
     c.free_top = (int)cap;
+    c.one = 1;
+    c.live[0] = 1;
     HIPX(hipMemcpyAsync(s->ctr, &c, sizeof(Ctr), hipMemcpyHostToDevice, s->st));
     HIPX(hipStreamSynchronize(s->st));
     return FELT_OK;
@@ -519,12 +569,21 @@ void felt_surface_release(felt_surface* s) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
+    for (ProfRec& r : s->prof.graph_recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    s->prof.graph_recs.clear();
     for (hipEvent_t e : s->prof.spare) (void)hipEventDestroy(e);
     s->prof.pending.clear();
     s->prof.spare.clear();
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    s->graph_exec = nullptr;
+    if (s->h_speed) (void)hipHostFree(s->h_speed);
+    s->h_speed = nullptr;
     void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
                     s->amark, s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag,
-                    s->nlist, s->dmask, s->smask, s->act};
+                    s->nlist, s->dmask, s->smask, s->act, s->bbase, s->d_speed};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s->band.release();
@@ -557,11 +616,14 @@ Lu felt_lu(felt_surface* s, int i) {
     return Lu{s->lu_v[i].ptr, s->lu_tag[i].ptr, s->lu_d[i].ptr, s->lu_tk[i].ptr};
 }
 static SC sc_of(felt_surface* s) { return SC{s->sc_v.ptr, s->sc_from.ptr, s->sc_to.ptr}; }
+static int* lu_span(felt_surface* s, int i) { return &s->ctr->lu_span[i][0]; }
+static const int* always(felt_surface* s) { return &s->ctr->one; }
 
 // Capacity of every per-step list: no list in a step holds more entries than the band had at
-// its start.
+// its start.  Growth changes buffer addresses, so a captured step graph is rebuilt.
 static int ensure_step_capacity(felt_surface* s, size_t band) {
-    const size_t n = band + 64;
+    if (band + 64 <= s->list_cap) return FELT_OK;
+    const size_t n = std::max<size_t>(band + band / 4 + 1024, s->list_cap + s->list_cap / 2);
     int rc;
     for (int i = 0; i < 2; i++) {
         if ((rc = s->lu_v[i].ensure(n))) return rc;
@@ -574,123 +636,121 @@ static int ensure_step_capacity(felt_surface* s, size_t band) {
     if ((rc = s->sc_to.ensure(n))) return rc;
     if ((rc = s->zlist.ensure(n))) return rc;
     if ((rc = s->zdelta.ensure(n))) return rc;
+    if ((rc = s->band.ensure(n))) return rc;
+    if ((rc = s->band_d.ensure(n))) return rc;
+    if ((rc = s->band_tag.ensure(n))) return rc;
+    if ((rc = s->band_tk.ensure(n))) return rc;
     if ((rc = s->chg_cnt.ensure(2 * (n / (BT * ITEMS) + 2)))) return rc;
+    s->list_cap = n;
+    s->graph_valid = false;
     return FELT_OK;
 }
 
-// Band compaction: per-layer segments of the band list, s->band_base[k]..[k+1], one block per
-// active partition.
-int felt_compact_band(felt_surface* s) {
+static int grid_for_lists(felt_surface* s) {
+    return (int)std::min<size_t>(4096, std::max<size_t>(1, (s->list_cap + TPB - 1) / TPB));
+}
+static int chg_blocks(felt_surface* s) {
+    return (int)((s->list_cap + BT * ITEMS - 1) / (BT * ITEMS));
+}
+
+// Band compaction into per-layer segments [bbase[k], bbase[k+1]) — one block per partition.
+static void compact_band(felt_surface* s) {
     const Geo& G = s->G;
-    int rc;
-    HIPX(hipMemsetAsync(&s->ctr->act_n, 0, sizeof(int), s->st));
-    DISPATCH(G.D, k_active_parts, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->act,
-             &s->ctr->act_n);
-    if ((rc = read_ctr(s))) return rc;
-    const int nact = s->hc.act_n;
-    s->band_base.assign(G.NL + 1, 0);
-    if (nact == 0) return FELT_OK;
-    if ((rc = s->blk_cnt.ensure((size_t)nact * G.NL + G.NL + 1))) return rc;
-    if ((rc = s->blk_off.ensure((size_t)nact * G.NL))) return rc;
     const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
-    prof_begin(s, PK_COMPACT, (long long)nact * G.pv);
-    DISPATCH(G.D, k_band_count, dim3(nact), dim3(bt), G, s->P, s->act, s->blk_cnt.ptr);
+    prof_begin(s, PK_COMPACT, (long long)G.np * G.pv);
+    DISPATCH(G.D, k_band_count, dim3(G.np), dim3(bt), G, s->P, s->blk_cnt.ptr);
     prof_end(s);
-    int* base = s->blk_cnt.ptr + (size_t)nact * G.NL;
-    LAUNCH(k_band_scan, dim3(1), dim3(BT), s->blk_cnt.ptr, nact, G.NL, s->blk_off.ptr, base);
-    HIPX(hipMemcpyAsync(s->band_base.data(), base, (G.NL + 1) * sizeof(int),
-                        hipMemcpyDeviceToHost, s->st));
-    HIPX(hipStreamSynchronize(s->st));
-    const size_t total = (size_t)s->band_base[G.NL];
-    if ((rc = s->band.ensure(total + 64))) return rc;
-    if ((rc = s->band_d.ensure(total + 64))) return rc;
-    if ((rc = s->band_tag.ensure(total + 64))) return rc;
-    if ((rc = s->band_tk.ensure(total + 64))) return rc;
-    DISPATCH(G.D, k_band_scatter, dim3(nact), dim3(bt), G, s->P, s->act, s->blk_off.ptr,
-             s->band.ptr, s->band_tag.ptr);
-    return FELT_OK;
+    LAUNCH(k_band_scan, dim3(1), dim3(BT), s->blk_cnt.ptr, G.np, G.NL, s->blk_off.ptr, s->bbase,
+           s->ctr);
+    DISPATCH(G.D, k_band_scatter, dim3(G.np), dim3(bt), G, s->P, s->blk_off.ptr, s->band.ptr,
+             s->band_tag.ptr);
 }
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
 // 1..L, each a Jacobi pass (distance into d, then phi = d and tk = its layer).
-static int distance_passes(felt_surface* s, bool from_band, int look_i, int look_n) {
+static void distance_passes(felt_surface* s, bool from_band, int look, const int* live) {
     const Geo& G = s->G;
+    const int grid = grid_for_lists(s);
     for (int pass = 0; pass < 2; pass++) {
         const int side = pass == 0 ? -1 : 1;
         for (int m = 1; m <= G.L; m++) {
             const int layer = side * m;
             const uint32_t* list;
             const int8_t* tag;
+            const int* rng;
             float* d;
             int8_t* tk;
-            int n;
             if (from_band) {
-                const int k = layer + G.L;
-                list = s->band.ptr + s->band_base[k];
+                list = s->band.ptr;
                 tag = nullptr;
-                d = s->band_d.ptr + s->band_base[k];
-                tk = s->band_tk.ptr + s->band_base[k];
-                n = s->band_base[k + 1] - s->band_base[k];
+                rng = s->bbase + (layer + G.L);
+                d = s->band_d.ptr;
+                tk = s->band_tk.ptr;
             } else {
-                list = s->lu_v[look_i].ptr;
-                tag = s->lu_tag[look_i].ptr;
-                d = s->lu_d[look_i].ptr;
-                tk = s->lu_tk[look_i].ptr;
-                n = look_n;
+                list = s->lu_v[look].ptr;
+                tag = s->lu_tag[look].ptr;
+                rng = lu_span(s, look);
+                d = s->lu_d[look].ptr;
+                tk = s->lu_tk[look].ptr;
             }
-            if (n == 0) continue;
-            prof_begin(s, PK_DIST1, n);
-            DISPATCH(G.D, k_dist1, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d);
+            const long long spec = from_band ? -(long long)(layer + G.L) - 1 : 0;
+            prof_begin(s, PK_DIST1, spec);
+            DISPATCH(G.D, k_dist1, dim3(grid), dim3(TPB), G, s->P, list, tag, rng, layer, d, live);
             prof_end(s);
-            prof_begin(s, PK_DIST2, n);
-            LAUNCH(k_dist2, dim3(blocks_for(n)), dim3(TPB), G, s->P, list, tag, n, layer, d, tk,
-                   &s->ctr->nan_events);
+            prof_begin(s, PK_DIST2, spec);
+            LAUNCH(k_dist2, dim3(grid), dim3(TPB), G, s->P, list, tag, rng, layer, d, tk,
+                   &s->ctr->nan_events, live);
             prof_end(s);
         }
     }
-    return FELT_OK;
 }
 
 // Status changes of the round's processed list -> status-change records and the buffer
-// lookup `buf_i` (its previous contents are replaced).
-static int compact_changes(felt_surface* s, const uint32_t* v, const int8_t* tag,
-                           const int8_t* tk, int n, int buf_i) {
+// lookup `buf` (its previous contents are replaced).
+static void compact_changes(felt_surface* s, const uint32_t* v, const int8_t* tag,
+                            const int8_t* tk, const int* rng, int buf, const int* live) {
     const Geo& G = s->G;
-    if (n == 0) return FELT_OK;
-    const int nb = (n + BT * ITEMS - 1) / (BT * ITEMS);
-    int rc;
-    if ((rc = s->chg_cnt.ensure(2 * (size_t)nb))) return rc;
-    LAUNCH(k_chg_count, dim3(nb), dim3(BT), G, tag, tk, n, s->chg_cnt.ptr, s->ctr);
-    LAUNCH(k_chg_scan, dim3(1), dim3(BT), s->chg_cnt.ptr, nb, &s->ctr->sc_n, &s->ctr->lu_n[buf_i]);
-    LAUNCH(k_chg_scatter, dim3(nb), dim3(BT), G, v, tag, tk, n, s->chg_cnt.ptr, sc_of(s),
-           felt_lu(s, buf_i));
-    return FELT_OK;
+    const int nb = chg_blocks(s);
+    LAUNCH(k_chg_count, dim3(nb), dim3(BT), G, tag, tk, rng, s->chg_cnt.ptr, s->ctr, live);
+    LAUNCH(k_chg_scan, dim3(1), dim3(BT), s->chg_cnt.ptr, nb, s->ctr, buf, live);
+    LAUNCH(k_chg_scatter, dim3(nb), dim3(BT), G, v, tag, tk, rng, s->chg_cnt.ptr, sc_of(s),
+           felt_lu(s, buf), live);
 }
 
-// flush_status_change then expand_narrow_band, sized by a bound on the status-change count.
-static int flush_expand(felt_surface* s, int sc_bound) {
+// flush_status_change then expand_narrow_band.
+static void flush_expand(felt_surface* s, const int* live) {
     const Geo& G = s->G;
     const SC sc = sc_of(s);
-    if (sc_bound <= 0) return FELT_OK;
-    const int nb = blocks_for(sc_bound);
-    LAUNCH(k_smask, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->smask);
-    HIPX(hipMemsetAsync(&s->ctr->touched_n, 0, sizeof(int), s->st));
+    const int grid = grid_for_lists(s);
+    LAUNCH(k_smask, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->smask, live);
+    LAUNCH(k_flush_reset, dim3(1), dim3(1), s->ctr, live);
     prof_begin(s, PK_FLUSH, 0);
-    LAUNCH(k_flush, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag, s->tlist, s->ctr);
+    LAUNCH(k_flush, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag, s->tlist, s->ctr,
+           live);
     prof_end(s);
-    LAUNCH(k_deactivate, dim3(blocks_for(std::min<int64_t>(G.np, sc_bound))), dim3(TPB), G, s->P,
-           s->tlist, s->ctr, s->ukey, s->tflag, s->free_stack, s->ctr);
-    HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
+    LAUNCH(k_deactivate, dim3(std::min(grid, blocks_for(G.np))), dim3(TPB), G, s->P, s->tlist,
+           s->ctr, s->ukey, s->tflag, s->free_stack, s->ctr, live);
     prof_begin(s, PK_EXPAND, 0);
-    DISPATCH(G.D, k_expand_mark, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->nflag, s->nlist,
-             s->ctr);
+    DISPATCH(G.D, k_expand_mark, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->nflag, s->nlist,
+             s->ctr, live);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
-           s->free_stack, s->ctr);
-    DISPATCH(G.D, k_expand_claim, dim3(nb), dim3(TPB), G, s->P, sc, s->ctr, s->ctr);
+           s->free_stack, s->ctr, live);
+    DISPATCH(G.D, k_expand_claim, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->ctr, live);
     prof_end(s);
-    return FELT_OK;
 }
 
+// A round over lookup `look` (rounds after the first): distance passes over its entries,
+// changes into lookup `buf`, then flush and expand.
+static void lookup_round(felt_surface* s, int look, int buf, const int* live) {
+    LAUNCH(k_tk_init, dim3(grid_for_lists(s)), dim3(TPB), s->lu_tag[look].ptr, s->lu_tk[look].ptr,
+           lu_span(s, look), live);
+    distance_passes(s, false, look, live);
+    compact_changes(s, s->lu_v[look].ptr, s->lu_tag[look].ptr, s->lu_tk[look].ptr,
+                    lu_span(s, look), buf, live);
+    flush_expand(s, live);
+}
+
+// Host-driven reset of a round's status-change state (m_grid_status_change.reset).
 static int reset_round(felt_surface* s) {
     HIPX(hipMemsetAsync(&s->ctr->sc_n, 0, sizeof(int), s->st));
     HIPX(hipMemsetAsync(&s->ctr->flag, 0, sizeof(int), s->st));
@@ -698,27 +758,13 @@ static int reset_round(felt_surface* s) {
     return FELT_OK;
 }
 
-// A round over lookup `look` (rounds after the first): distance passes over its entries,
-// changes into lookup `buf`, then flush and expand.
-static int lookup_round(felt_surface* s, int look, int look_n, int buf) {
-    int rc;
-    if (look_n > 0)
-        LAUNCH(k_tk_init, dim3(blocks_for(look_n)), dim3(TPB), s->lu_tag[look].ptr,
-               s->lu_tk[look].ptr, look_n);
-    if ((rc = distance_passes(s, false, look, look_n))) return rc;
-    if ((rc = compact_changes(s, s->lu_v[look].ptr, s->lu_tag[look].ptr, s->lu_tk[look].ptr,
-                              look_n, buf)))
-        return rc;
-    return flush_expand(s, look_n);
-}
-
 // update_start (Surface.hpp:393-399): delta, affected, buffer and status-change state cleared.
 int felt_update_start_impl(felt_surface* s) {
-    HIPX(hipMemsetAsync(s->dmask, 0, s->G.np * sizeof(unsigned), s->st));
-    HIPX(hipMemsetAsync(&s->ctr->lu_n[0], 0, 2 * sizeof(int), s->st));
-    HIPX(hipMemsetAsync(&s->ctr->z_n, 0, sizeof(int), s->st));
-    int rc = reset_round(s);
-    if (rc) return rc;
+    LAUNCH(k_step_reset, dim3(1), dim3(1), s->ctr);
+    LAUNCH(k_fill_i32, dim3(std::min(1024, blocks_for(s->G.np))), dim3(TPB), (int*)s->dmask,
+           (int64_t)s->G.np, 0);
+    LAUNCH(k_fill_i32, dim3(std::min(1024, blocks_for(s->G.np))), dim3(TPB), (int*)s->smask,
+           (int64_t)s->G.np, 0);
     s->z_host_n = 0;
     s->in_update = true;
     s->pending_deltas.clear();
@@ -726,44 +772,113 @@ int felt_update_start_impl(felt_surface* s) {
     return FELT_OK;
 }
 
-// Global step (Surface.hpp:254-296, 739-762).  `delta_fn` fills s->band_d over the zero-layer
-// segment.
-template <class DeltaFn>
-static int global_step(felt_surface* s, DeltaFn&& delta_fn) {
+// The kernels of a global step up to the last unrolled round (no host reads): step reset,
+// band compaction, delta of the zero layer (`zero_fn`), zero-layer apply, round 1 over the
+// band, then rounds 2..R gated on the previous round's flag.  Lookup roles follow
+// update_end_global (Surface.hpp:739-762): round 1 buffers into lookup 1; round r >= 2 reads
+// lookup (r even ? 1 : 0), buffers into the other, then empties the one it read.
+template <class ZeroFn>
+static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
     const Geo& G = s->G;
-    int rc;
-    if ((rc = felt_update_start_impl(s))) return rc;
-    if ((rc = felt_compact_band(s))) return rc;
-    const size_t band = (size_t)s->band_base[G.NL];
-    if ((rc = ensure_step_capacity(s, band))) return rc;
+    felt_update_start_impl(s);
+    compact_band(s);
     LAUNCH(k_dmask_pcnt, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->dmask);
-    const int z0 = s->band_base[G.L], zn = s->band_base[G.L + 1] - z0;
-    // update_zero_layer; its changes go to the affected buffer (lookup 1)
-    if (zn > 0) {
-        if ((rc = delta_fn(s->band.ptr + z0, zn, s->band_d.ptr + z0))) return rc;
-        prof_begin(s, PK_APPLY, zn);
-        LAUNCH(k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->band.ptr + z0,
-               s->band_d.ptr + z0, zn, s->band_tk.ptr + z0, &s->ctr->nan_events);
-        prof_end(s);
+    const int* zrng = s->bbase + G.L;
+    zero_fn(zrng);
+    prof_begin(s, PK_APPLY, -(long long)G.L - 1);
+    LAUNCH(k_apply, dim3(grid_for_lists(s)), dim3(TPB), G, s->P, s->band.ptr, s->band_d.ptr, zrng,
+           s->band_tk.ptr, &s->ctr->nan_events);
+    prof_end(s);
+    distance_passes(s, true, 0, always(s));
+    compact_changes(s, s->band.ptr, s->band_tag.ptr, s->band_tk.ptr, &s->ctr->band_span[0], 1,
+                    always(s));
+    flush_expand(s, always(s));
+    for (int r = 1; r < s->unroll; r++) {
+        const int look = r % 2 == 1 ? 1 : 0, buf = 1 - look;
+        LAUNCH(k_round_gate, dim3(1), dim3(1), s->ctr, r);
+        const int* live = &s->ctr->live[r];
+        LAUNCH(k_fill_u32_live, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), s->smask, G.np,
+               0u, live);
+        lookup_round(s, look, buf, live);
+        LAUNCH(k_lu_clear, dim3(1), dim3(1), s->ctr, look, live);
     }
-    if ((rc = distance_passes(s, true, 0, 0))) return rc;
-    if ((rc = compact_changes(s, s->band.ptr, s->band_tag.ptr, s->band_tk.ptr, (int)band, 1)))
-        return rc;
-    if ((rc = flush_expand(s, (int)band))) return rc;
-    if ((rc = read_ctr(s))) return rc;
-    int rounds = 1;
-    if (s->hc.flag) {
-        int look = 1, buf = 0;
-        while (true) {
-            if ((rc = reset_round(s))) return rc;
-            if ((rc = lookup_round(s, look, s->hc.lu_n[look], buf))) return rc;
-            HIPX(hipMemsetAsync(&s->ctr->lu_n[look], 0, sizeof(int), s->st));
-            std::swap(look, buf);
-            rounds++;
-            if ((rc = read_ctr(s))) return rc;
-            if (!s->hc.flag) break;
-            if (rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
+    LAUNCH(k_zero_int, dim3(1), dim3(1), &s->ctr->band_total);
+    LAUNCH(k_band_total2, dim3(std::min(256, blocks_for(G.np))), dim3(TPB), G, s->P, s->ctr);
+}
+
+// Rounds after the unrolled ones, driven from the host (rare).
+static int finish_rounds(felt_surface* s, int look, int buf, int* rounds) {
+    int rc;
+    while (s->hc.flag) {
+        if ((rc = reset_round(s))) return rc;
+        lookup_round(s, look, buf, always(s));
+        HIPX(hipMemsetAsync(&s->ctr->lu_span[look][1], 0, sizeof(int), s->st));
+        std::swap(look, buf);
+        (*rounds)++;
+        if ((rc = read_ctr(s))) return rc;
+        if (*rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
+    }
+    return FELT_OK;
+}
+
+// Makes sure the step lists can hold the current band (one host read when it is unknown).
+static int size_for_band(felt_surface* s) {
+    int rc;
+    if (!s->band_known) {
+        size_t band;
+        if ((rc = felt_band_total(s, &band))) return rc;
+        s->band_now = band;
+        s->band_known = true;
+    }
+    return ensure_step_capacity(s, s->band_now);
+}
+
+// Global step (Surface.hpp:254-296, 739-762).  `zero_fn(rng)` writes the zero layer's deltas
+// into band_d over the span rng.  With `use_graph` the kernel sequence is captured once and
+// replayed while list capacities and the speed layout are unchanged.
+template <class ZeroFn>
+static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
+    int rc;
+    if ((rc = size_for_band(s))) return rc;
+    const bool graph = use_graph;
+    if (graph) {
+        if (!s->graph_valid || s->graph_prof != s->prof.on) {
+            if (s->graph_exec) {
+                (void)hipGraphExecDestroy(s->graph_exec);
+                s->graph_exec = nullptr;
+            }
+            prof_drop_graph(s);
+            s->prof.capturing = true;
+            HIPX(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+            global_step_kernels(s, zero_fn);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s->st, &g);
+            s->prof.capturing = false;
+            if (ec != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(ec));
+            hipError_t e = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+            s->graph_valid = true;
+            s->graph_prof = s->prof.on;
         }
+        HIPX(hipGraphLaunch(s->graph_exec, s->st));
+    } else {
+        global_step_kernels(s, zero_fn);
+    }
+    if ((rc = read_ctr(s))) return rc;
+    if (graph && s->prof.on && (rc = prof_collect_graph(s))) return rc;
+    int rounds = 1;
+    for (int r = 1; r < s->unroll; r++) rounds += s->hc.live[r] ? 1 : 0;
+    const size_t band = (size_t)s->hc.band_span[1];
+    // still changing after the last unrolled round: continue from the host
+    if (s->hc.live[s->unroll - 1] && s->hc.flag) {
+        const int r = s->unroll;  // next round number
+        const int look = r % 2 == 1 ? 1 : 0, buf = 1 - look;
+        if ((rc = finish_rounds(s, look, buf, &rounds))) return rc;
+        s->band_known = false;
+    } else {
+        s->band_now = (size_t)s->hc.band_total;
+        s->band_known = true;
     }
     s->last_updates = band;
     s->last_rounds = rounds;
@@ -775,12 +890,14 @@ int felt_update_global(felt_surface* s, const felt_speed* fn) {
     Speed sp{fn->kind, fn->a, fn->b, fn->c, (unsigned long long)fn->seed, fn->cell};
     if (fn->kind == FELT_SPEED_NOISE_CURV && fn->cell <= 0)
         return felt_fail(FELT_E_ARG, "noise cell must be positive");
-    return global_step(s, [&](const uint32_t* list, int n, float* out) -> int {
-        prof_begin(s, PK_SPEED, n);
-        DISPATCH(s->G.D, k_speed, dim3(blocks_for(n)), dim3(TPB), s->G, s->P, list, n, sp, out);
+    *s->h_speed = sp;
+    HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, sizeof(Speed), hipMemcpyHostToDevice, s->st));
+    return global_step(s, [&](const int* rng) {
+        prof_begin(s, PK_SPEED, -(long long)s->G.L - 1);
+        DISPATCH(s->G.D, k_speed, dim3(grid_for_lists(s)), dim3(TPB), s->G, s->P, s->band.ptr, rng,
+                 s->d_speed, s->band_d.ptr);
         prof_end(s);
-        return FELT_OK;
-    });
+    }, s->use_graph);
 }
 
 // Host-evaluated deltas: scattered through a per-voxel scratch onto the zero-layer list.
@@ -808,13 +925,14 @@ int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, co
         LAUNCH(k_vox_set, dim3(blocks_for(m)), dim3(TPB), s->tmp_u.ptr, s->tmp_f.ptr, (int)m,
                s->vox.ptr);
     }
-    return global_step(s, [&](const uint32_t* list, int n, float* out) -> int {
-        LAUNCH(k_vox_gather, dim3(blocks_for(n)), dim3(TPB), list, n, s->vox.ptr, out);
-        return FELT_OK;
-    });
+    return global_step(s, [&](const int* rng) {
+        LAUNCH(k_vox_gather, dim3(grid_for_lists(s)), dim3(TPB), s->band.ptr, rng, s->vox.ptr,
+               s->band_d.ptr);
+    }, false);
 }
 
-// update_end (Surface.hpp:404-430) over the zero-layer list s->zlist / s->zdelta.
+// update_end (Surface.hpp:404-430) over the zero-layer list s->zlist / s->zdelta, whose span
+// is ctr->z_span.
 int felt_update_end_impl(felt_surface* s) {
     const Geo& G = s->G;
     int rc;
@@ -823,10 +941,10 @@ int felt_update_end_impl(felt_surface* s) {
     Lu A = felt_lu(s, 0);
     if (zn > 0)
         LAUNCH(k_lu_seed, dim3(blocks_for(zn)), dim3(TPB), A, s->zlist.ptr, zn, s->amark,
-               &s->ctr->lu_n[0]);
+               lu_span(s, 0));
     HIPX(hipMemsetAsync(&s->ctr->front[0], 0, sizeof(int), s->st));
     LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, 1);
-    const int bfs_grid = std::min(4096, std::max(1, blocks_for(s->band_cap_hint)));
+    const int bfs_grid = grid_for_lists(s);
     for (int j = 1; j <= G.L; j++) {
         prof_begin(s, PK_AFFECTED, 0);
         DISPATCH(G.D, k_affected_round, dim3(bfs_grid), dim3(TPB), G, s->P, A, s->ctr, 0, j,
@@ -834,29 +952,25 @@ int felt_update_end_impl(felt_surface* s) {
         prof_end(s);
         LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, j + 1);
     }
-    if ((rc = read_ctr(s))) return rc;
-    const int an = s->hc.lu_n[0];
-    LAUNCH(k_unmark, dim3(blocks_for(an)), dim3(TPB), A, &s->ctr->lu_n[0], s->amark);
-    LAUNCH(k_dmask_lu, dim3(blocks_for(an)), dim3(TPB), G, s->P, A, &s->ctr->lu_n[0], s->dmask);
-    if (an > 0)
-        LAUNCH(k_tk_init, dim3(blocks_for(an)), dim3(TPB), A.tag, A.tk, an);
-    if (zn > 0) {
-        prof_begin(s, PK_APPLY, zn);
-        LAUNCH(k_apply, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr, s->zdelta.ptr, zn,
-               A.tk, &s->ctr->nan_events);
-        prof_end(s);
-    }
+    LAUNCH(k_unmark, dim3(bfs_grid), dim3(TPB), A, lu_span(s, 0), s->amark);
+    LAUNCH(k_dmask_lu, dim3(bfs_grid), dim3(TPB), G, s->P, A, lu_span(s, 0), s->dmask);
+    LAUNCH(k_tk_init, dim3(bfs_grid), dim3(TPB), A.tag, A.tk, lu_span(s, 0), always(s));
+    prof_begin(s, PK_APPLY, 0);
+    LAUNCH(k_apply, dim3(bfs_grid), dim3(TPB), G, s->P, s->zlist.ptr, s->zdelta.ptr,
+           &s->ctr->z_span[0], A.tk, &s->ctr->nan_events);
+    prof_end(s);
     int look = 0, buf = 1;
-    if ((rc = distance_passes(s, false, look, an))) return rc;
-    if ((rc = compact_changes(s, A.v, A.tag, A.tk, an, buf))) return rc;
-    if ((rc = flush_expand(s, an))) return rc;
+    distance_passes(s, false, look, always(s));
+    compact_changes(s, A.v, A.tag, A.tk, lu_span(s, 0), buf, always(s));
+    flush_expand(s, always(s));
     if ((rc = read_ctr(s))) return rc;
+    const int an = s->hc.lu_span[0][1];
     int rounds = 1;
     while (s->hc.flag) {
-        HIPX(hipMemsetAsync(&s->ctr->lu_n[look], 0, sizeof(int), s->st));
+        HIPX(hipMemsetAsync(&s->ctr->lu_span[look][1], 0, sizeof(int), s->st));
         std::swap(look, buf);
         if ((rc = reset_round(s))) return rc;
-        if ((rc = lookup_round(s, look, s->hc.lu_n[look], buf))) return rc;
+        lookup_round(s, look, buf, always(s));
         rounds++;
         if ((rc = read_ctr(s))) return rc;
         if (rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
@@ -864,6 +978,7 @@ int felt_update_end_impl(felt_surface* s) {
     s->last_updates = (uint64_t)an;
     s->last_rounds = rounds;
     s->in_update = false;
+    s->band_known = false;
     return FELT_OK;
 }
 
@@ -881,11 +996,8 @@ int felt_update_box_impl(felt_surface* s, const int32_t* lo, const int32_t* hi,
                          const felt_speed* fn) {
     const Geo& G = s->G;
     int rc;
+    if ((rc = size_for_band(s))) return rc;
     if ((rc = felt_update_start_impl(s))) return rc;
-    size_t band;
-    if ((rc = felt_band_total(s, &band))) return rc;
-    if ((rc = ensure_step_capacity(s, band))) return rc;
-    s->band_cap_hint = (int64_t)band;
     // partition box clamped to the grid, voxel box [lo, min(hi+1, grid_hi))
     int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0}, vlo[3] = {0, 0, 0}, vhi[3] = {1, 1, 1};
     bool empty = false;
@@ -920,13 +1032,15 @@ int felt_update_box_impl(felt_surface* s, const int32_t* lo, const int32_t* hi,
                  G, s->P, s->tmp_i.ptr, (int)parts.size(), vlo[0], vlo[1], vlo[2], vhi[0], vhi[1],
                  vhi[2], s->zlist.ptr, s->ctr);
         if ((rc = read_ctr(s))) return rc;
-        zn = s->hc.z_n;
+        zn = s->hc.z_span[1];
     }
     s->z_host_n = zn;
     if (zn > 0) {
         Speed sp{fn->kind, fn->a, fn->b, fn->c, (unsigned long long)fn->seed, fn->cell};
-        DISPATCH(G.D, k_speed, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr, zn, sp,
-                 s->zdelta.ptr);
+        *s->h_speed = sp;
+        HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, sizeof(Speed), hipMemcpyHostToDevice, s->st));
+        DISPATCH(G.D, k_speed, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr,
+                 &s->ctr->z_span[0], s->d_speed, s->zdelta.ptr);
     }
     return felt_update_end_impl(s);
 }
@@ -951,10 +1065,8 @@ int felt_delta_impl(felt_surface* s, size_t m, const int32_t* gpos, const float*
 int felt_update_end_public(felt_surface* s) {
     int rc;
     if (!s->in_update) return felt_fail(FELT_E_STATE, "update_end() without update_start()");
-    size_t band;
-    if ((rc = felt_band_total(s, &band))) return rc;
-    if ((rc = ensure_step_capacity(s, band + s->pending_deltas.size()))) return rc;
-    s->band_cap_hint = (int64_t)band;
+    if ((rc = size_for_band(s))) return rc;
+    if ((rc = ensure_step_capacity(s, s->band_now + s->pending_deltas.size()))) return rc;
     const size_t m = s->pending_deltas.size();
     if (m) {
         std::vector<int> gp(m * 3);
@@ -978,6 +1090,7 @@ int felt_update_end_public(felt_surface* s) {
         }
     }
     s->z_host_n = (int)m;
+    LAUNCH(k_set_span, dim3(1), dim3(1), &s->ctr->z_span[0], (int)m);
     s->pending_deltas.clear();
     s->pending_index.clear();
     return felt_update_end_impl(s);
@@ -988,6 +1101,7 @@ int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::
                       const std::vector<int>& layer) {
     const Geo& G = s->G;
     int rc;
+    s->band_known = false;
     const int n = (int)val.size();
     if (n == 0) return FELT_OK;
     if ((rc = s->tmp_i.ensure((size_t)n * 4 + 1))) return rc;
@@ -1002,7 +1116,7 @@ int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::
     DISPATCH(G.D, k_mark_parts, dim3(blocks_for(n)), dim3(TPB), G, s->P, dpos, n, s->nflag,
              s->nlist, s->ctr);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
-           s->free_stack, s->ctr);
+           s->free_stack, s->ctr, always(s));
     DISPATCH(G.D, k_track_points, dim3(blocks_for(n)), dim3(TPB), G, s->P, dpos, s->tmp_f.ptr,
              dlay, n);
     return read_ctr(s);
@@ -1011,6 +1125,7 @@ int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::
 int felt_load_dense_impl(felt_surface* s, const float* phi, size_t n) {
     const Geo& G = s->G;
     int rc;
+    s->band_known = false;
     float* dense = nullptr;
     HIPX(hipMalloc(&dense, n * sizeof(float)));
     hipError_t e = hipMemcpyAsync(dense, phi, n * sizeof(float), hipMemcpyHostToDevice, s->st);
@@ -1024,7 +1139,7 @@ int felt_load_dense_impl(felt_surface* s, const float* phi, size_t n) {
     LAUNCH(k_list_from_flags, dim3(blocks_for(G.np)), dim3(TPB), s->tmp_i.ptr, G.np, s->nlist,
            s->nflag, s->ctr);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
-           s->free_stack, s->ctr);
+           s->free_stack, s->ctr, always(s));
     DISPATCH(G.D, k_load_fill, dim3(G.np), dim3(TPB), G, s->P, dense, s->tmp_i.ptr, s->ctr);
     rc = read_ctr(s);
     (void)hipFree(dense);
@@ -1077,12 +1192,12 @@ int felt_collect_layer_impl(felt_surface* s, int layer, std::vector<int>& out) {
 int felt_lu_positions_impl(felt_surface* s, int lu_i, int tag, std::vector<int>& out) {
     int rc;
     if ((rc = read_ctr(s))) return rc;
-    const int n = s->hc.lu_n[lu_i];
+    const int n = s->hc.lu_span[lu_i][1];
     if ((rc = s->tmp_i.ensure((size_t)n * 3 + 3))) return rc;
     HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
     if (n > 0)
         DISPATCH(s->G.D, k_lu_positions, dim3(blocks_for(n)), dim3(TPB), s->G, s->P,
-                 felt_lu(s, lu_i), &s->ctr->lu_n[lu_i], tag, s->tmp_i.ptr, n, &s->ctr->misc);
+                 felt_lu(s, lu_i), &s->ctr->lu_span[lu_i][1], tag, s->tmp_i.ptr, n, &s->ctr->misc);
     if ((rc = read_ctr(s))) return rc;
     const int m = s->hc.misc;
     out.resize((size_t)m * 3);

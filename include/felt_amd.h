@@ -138,6 +138,10 @@ int felt_surface_partition_active(felt_surface* s, uint8_t* out, size_t n);
 /* Number of layer-k voxels in partition pos_idx (layer_size helper, test_Surface.cpp:1720-1737). */
 int felt_surface_partition_layer_size(felt_surface* s, uint64_t pos_idx, int layer, uint64_t* n);
 int felt_surface_stats(felt_surface* s, felt_stats* out);
+/* Launch a global update(fn) with a device functor as one captured HIP graph (default on):
+ * the first three rounds run without host reads, later rounds are launched from the host.
+ * Off: the same kernels are launched one by one.  Not a reference member. */
+int felt_surface_set_graph(felt_surface* s, int enable);
 /* Voxel updates and rounds of the last update call, without device traffic. */
 int felt_surface_step_info(felt_surface* s, uint64_t* voxel_updates, uint64_t* rounds);
 /* Kernel timing with HIP events on the handle's stream, for measurement.  Not a reference

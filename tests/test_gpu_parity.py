@@ -29,11 +29,14 @@ def check(g, c, where, snapshot=True):
     assert not d, f"{where}: {d}"
 
 
-def test_c1_plain_config():
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "launches"])
+def test_c1_plain_config(graph):
     """128^3 isogrid in 16^3 partitions, singularity seed, 40 steps of -0.5 then 40 of
-    0.2*curvature, compared after every step; then a full march compared as meshes."""
+    0.2*curvature, compared after every step; then a full march compared as meshes.  Run with
+    the step captured as one graph and with kernels launched one by one."""
     import felt_amd
     g, c = pair(3, 3, (128,) * 3, (16,) * 3)
+    g.set_graph(graph)
     step_both(g, c, "seed", (0, 0, 0))
     check(g, c, "seed")
     t0 = time.time()
