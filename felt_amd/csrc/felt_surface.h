@@ -23,7 +23,8 @@ struct DBuf {
 
 // Optional per-kernel timing with HIP events on the surface's stream.
 enum ProfKernel { PK_SPEED, PK_APPLY, PK_DIST1, PK_DIST2, PK_FLUSH, PK_EXPAND, PK_COMPACT,
-                  PK_AFFECTED, PK_MARCH_COUNT, PK_MARCH_EMIT, PK_COUNT };
+                  PK_AFFECTED, PK_MARCH_COUNT, PK_MARCH_EMIT, PK_DIST1_LATE, PK_DIST2_LATE,
+                  PK_COUNT };
 struct ProfRec {
     hipEvent_t a, b;
     int kernel;
@@ -32,9 +33,7 @@ struct ProfRec {
 // items >= 0: entries processed; items < 0: band segment -(items+1), resolved after the step.
 struct Prof {
     bool on = false;
-    bool capturing = false;
     std::vector<ProfRec> pending;
-    std::vector<ProfRec> graph_recs;  // event pairs recorded by every replay of the step graph
     std::vector<hipEvent_t> spare;
     long long launches[PK_COUNT] = {};
     double ms[PK_COUNT] = {};
@@ -89,7 +88,6 @@ struct felt_surface {
     int unroll = 3;                  // rounds of a global step launched without host reads
     bool use_graph = true;
     bool graph_valid = false;
-    bool graph_prof = false;         // the captured graph holds profiling event nodes
     hipGraphExec_t graph_exec = nullptr;
     bool in_update = false;
     std::vector<std::pair<std::array<int, 3>, float>> pending_deltas;

@@ -6,6 +6,7 @@
 // update (speed evaluation, zero-layer apply, layered distance transform, status-change flush,
 // narrow-band expansion, affected-set search) is a kernel over those lists.  Host code only
 // sequences kernels and reads a handful of counters.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -407,13 +408,12 @@ void prof_begin(felt_surface* s, int kernel, long long items) {
     if (!s->prof.on) return;
     ProfRec r{prof_event(s), prof_event(s), kernel, items};
     (void)hipEventRecord(r.a, s->st);
-    (s->prof.capturing ? s->prof.graph_recs : s->prof.pending).push_back(r);
+    s->prof.pending.push_back(r);
 }
 
 void prof_end(felt_surface* s) {
-    if (!s->prof.on) return;
-    std::vector<ProfRec>& v = s->prof.capturing ? s->prof.graph_recs : s->prof.pending;
-    if (!v.empty()) (void)hipEventRecord(v.back().b, s->st);
+    if (!s->prof.on || s->prof.pending.empty()) return;
+    (void)hipEventRecord(s->prof.pending.back().b, s->prof.pending.empty() ? s->st : s->st);
 }
 
 static long long prof_items(felt_surface* s, long long items) {
@@ -442,21 +442,6 @@ int prof_collect(felt_surface* s) {
     }
     s->prof.pending.clear();
     return FELT_OK;
-}
-
-// After a replay of the step graph: the event pairs it recorded.
-static int prof_collect_graph(felt_surface* s) {
-    for (const ProfRec& r : s->prof.graph_recs)
-        if (int rc = prof_account(s, r)) return rc;
-    return FELT_OK;
-}
-
-static void prof_drop_graph(felt_surface* s) {
-    for (ProfRec& r : s->prof.graph_recs) {
-        s->prof.spare.push_back(r.a);
-        s->prof.spare.push_back(r.b);
-    }
-    s->prof.graph_recs.clear();
 }
 
 template <class T>
@@ -492,6 +477,23 @@ template struct DBuf<uint8_t>;
         else hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, s->st, __VA_ARGS__);                \
     } while (0)
 #define LAUNCH(KERNEL, GRID, BLOCK, ...) hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, s->st, __VA_ARGS__)
+
+// Launch timed by events attached to the dispatch itself when profiling is on.
+#define PLAUNCH(KID, SPEC, KERNEL, GRID, BLOCK, ...)                                            \
+    do {                                                                                       \
+        if (s->prof.on) {                                                                      \
+            ProfRec r_{prof_event(s), prof_event(s), KID, SPEC};                               \
+            hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, s->st, r_.a, r_.b, 0, __VA_ARGS__);  \
+            s->prof.pending.push_back(r_);                                                     \
+        } else {                                                                               \
+            hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, s->st, __VA_ARGS__);                    \
+        }                                                                                      \
+    } while (0)
+#define PDISPATCH(KID, SPEC, D_, KERNEL, GRID, BLOCK, ...)                                      \
+    do {                                                                                       \
+        if ((D_) == 2) PLAUNCH(KID, SPEC, KERNEL<2>, GRID, BLOCK, __VA_ARGS__);                \
+        else PLAUNCH(KID, SPEC, KERNEL<3>, GRID, BLOCK, __VA_ARGS__);                          \
+    } while (0)
 
 static int read_ctr(felt_surface* s) {
     HIPX(hipMemcpyAsync(&s->hc, s->ctr, sizeof(Ctr), hipMemcpyDeviceToHost, s->st));
@@ -569,11 +571,6 @@ void felt_surface_release(felt_surface* s) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
     }
-    for (ProfRec& r : s->prof.graph_recs) {
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
-    }
-    s->prof.graph_recs.clear();
     for (hipEvent_t e : s->prof.spare) (void)hipEventDestroy(e);
     s->prof.pending.clear();
     s->prof.spare.clear();
@@ -657,9 +654,8 @@ static int chg_blocks(felt_surface* s) {
 static void compact_band(felt_surface* s) {
     const Geo& G = s->G;
     const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
-    prof_begin(s, PK_COMPACT, (long long)G.np * G.pv);
-    DISPATCH(G.D, k_band_count, dim3(G.np), dim3(bt), G, s->P, s->blk_cnt.ptr);
-    prof_end(s);
+    PDISPATCH(PK_COMPACT, (long long)G.np * G.pv, G.D, k_band_count, dim3(G.np), dim3(bt), G,
+              s->P, s->blk_cnt.ptr);
     LAUNCH(k_band_scan, dim3(1), dim3(BT), s->blk_cnt.ptr, G.np, G.NL, s->blk_off.ptr, s->bbase,
            s->ctr);
     DISPATCH(G.D, k_band_scatter, dim3(G.np), dim3(bt), G, s->P, s->blk_off.ptr, s->band.ptr,
@@ -693,14 +689,12 @@ static void distance_passes(felt_surface* s, bool from_band, int look, const int
                 d = s->lu_d[look].ptr;
                 tk = s->lu_tk[look].ptr;
             }
+            // only the first round's launches are attributed item counts (band segments)
             const long long spec = from_band ? -(long long)(layer + G.L) - 1 : 0;
-            prof_begin(s, PK_DIST1, spec);
-            DISPATCH(G.D, k_dist1, dim3(grid), dim3(TPB), G, s->P, list, tag, rng, layer, d, live);
-            prof_end(s);
-            prof_begin(s, PK_DIST2, spec);
-            LAUNCH(k_dist2, dim3(grid), dim3(TPB), G, s->P, list, tag, rng, layer, d, tk,
-                   &s->ctr->nan_events, live);
-            prof_end(s);
+            PDISPATCH(from_band ? PK_DIST1 : PK_DIST1_LATE, spec, G.D, k_dist1, dim3(grid),
+                      dim3(TPB), G, s->P, list, tag, rng, layer, d, live);
+            PLAUNCH(from_band ? PK_DIST2 : PK_DIST2_LATE, spec, k_dist2, dim3(grid), dim3(TPB), G,
+                    s->P, list, tag, rng, layer, d, tk, &s->ctr->nan_events, live);
         }
     }
 }
@@ -724,10 +718,8 @@ static void flush_expand(felt_surface* s, const int* live) {
     const int grid = grid_for_lists(s);
     LAUNCH(k_smask, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->smask, live);
     LAUNCH(k_flush_reset, dim3(1), dim3(1), s->ctr, live);
-    prof_begin(s, PK_FLUSH, 0);
-    LAUNCH(k_flush, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag, s->tlist, s->ctr,
-           live);
-    prof_end(s);
+    PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag,
+            s->tlist, s->ctr, live);
     LAUNCH(k_deactivate, dim3(std::min(grid, blocks_for(G.np))), dim3(TPB), G, s->P, s->tlist,
            s->ctr, s->ukey, s->tflag, s->free_stack, s->ctr, live);
     prof_begin(s, PK_EXPAND, 0);
@@ -785,10 +777,8 @@ static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
     LAUNCH(k_dmask_pcnt, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->dmask);
     const int* zrng = s->bbase + G.L;
     zero_fn(zrng);
-    prof_begin(s, PK_APPLY, -(long long)G.L - 1);
-    LAUNCH(k_apply, dim3(grid_for_lists(s)), dim3(TPB), G, s->P, s->band.ptr, s->band_d.ptr, zrng,
-           s->band_tk.ptr, &s->ctr->nan_events);
-    prof_end(s);
+    PLAUNCH(PK_APPLY, -(long long)G.L - 1, k_apply, dim3(grid_for_lists(s)), dim3(TPB), G, s->P,
+            s->band.ptr, s->band_d.ptr, zrng, s->band_tk.ptr, &s->ctr->nan_events);
     distance_passes(s, true, 0, always(s));
     compact_changes(s, s->band.ptr, s->band_tag.ptr, s->band_tk.ptr, &s->ctr->band_span[0], 1,
                     always(s));
@@ -840,33 +830,30 @@ template <class ZeroFn>
 static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
     int rc;
     if ((rc = size_for_band(s))) return rc;
-    const bool graph = use_graph;
+    // Kernel timing attaches events to individual dispatches, which a replayed graph does not
+    // allow, so a profiled step launches its kernels one by one.
+    const bool graph = use_graph && !s->prof.on;
     if (graph) {
-        if (!s->graph_valid || s->graph_prof != s->prof.on) {
+        if (!s->graph_valid) {
             if (s->graph_exec) {
                 (void)hipGraphExecDestroy(s->graph_exec);
                 s->graph_exec = nullptr;
             }
-            prof_drop_graph(s);
-            s->prof.capturing = true;
             HIPX(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
             global_step_kernels(s, zero_fn);
             hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(s->st, &g);
-            s->prof.capturing = false;
-            if (ec != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(ec));
+            HIPX(hipStreamEndCapture(s->st, &g));
             hipError_t e = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
             s->graph_valid = true;
-            s->graph_prof = s->prof.on;
         }
         HIPX(hipGraphLaunch(s->graph_exec, s->st));
     } else {
         global_step_kernels(s, zero_fn);
     }
     if ((rc = read_ctr(s))) return rc;
-    if (graph && s->prof.on && (rc = prof_collect_graph(s))) return rc;
+    if (s->prof.on && (rc = prof_collect(s))) return rc;
     int rounds = 1;
     for (int r = 1; r < s->unroll; r++) rounds += s->hc.live[r] ? 1 : 0;
     const size_t band = (size_t)s->hc.band_span[1];
@@ -893,10 +880,8 @@ int felt_update_global(felt_surface* s, const felt_speed* fn) {
     *s->h_speed = sp;
     HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, sizeof(Speed), hipMemcpyHostToDevice, s->st));
     return global_step(s, [&](const int* rng) {
-        prof_begin(s, PK_SPEED, -(long long)s->G.L - 1);
-        DISPATCH(s->G.D, k_speed, dim3(grid_for_lists(s)), dim3(TPB), s->G, s->P, s->band.ptr, rng,
-                 s->d_speed, s->band_d.ptr);
-        prof_end(s);
+        PDISPATCH(PK_SPEED, -(long long)s->G.L - 1, s->G.D, k_speed, dim3(grid_for_lists(s)),
+                  dim3(TPB), s->G, s->P, s->band.ptr, rng, s->d_speed, s->band_d.ptr);
     }, s->use_graph);
 }
 
