@@ -14,7 +14,6 @@ timed region starts with the surface resident in HBM.
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -27,13 +26,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "narrow-band voxel updates/s, batched+served vs plain on GPU; p99 step latency"
 HBM_PEAK = 8.0e12
-# algorithmic bytes per list entry of each timed kernel
-KERNELS = {
+# algorithmic bytes per list entry of each timed pass
+PASSES = {
     0: ("speed (curvature + noise functor)", 12),   # index, phi(x) read, delta write
-    1: ("zero-layer apply", 16),                    # index, delta read, phi read, phi write
-    2: ("distance pass 1", 12),                     # index, phi(x) read, d write
-    3: ("distance pass 2", 12),                     # index, d read, phi write
-    6: ("band compaction count", 1),                # one layer byte per brick voxel
+    1: ("distance pass 1", 12),                     # index, phi(x) read, d write
+    2: ("band compaction", 1),                      # one layer byte per pool voxel scanned
 }
 
 
@@ -63,7 +60,7 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--part", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--prof-steps", type=int, default=20)
+    ap.add_argument("--pass-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -73,7 +70,7 @@ def main():
     L = 3
     phi = make_workload(args.size, args.part, L)
     import felt_amd
-    lib = felt_amd.lib()
+    felt_amd.lib()
     s = felt_amd.Surface(3, L, (args.size,) * 3, (args.part,) * 3)
     t = time.time()
     s.load_dense(phi)
@@ -91,11 +88,6 @@ def main():
             polys.march()
     s.sync()
     log(f"[bench] {args.warmup} warmup steps done")
-    fprof = lib.lib.felt_surface_profile
-    fprof.argtypes = [ctypes.c_void_p, ctypes.c_int]
-    fread = lib.lib.felt_surface_profile_read
-    fread.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
-                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]
 
     def run(steps, first, lat, tally):
         for i in range(steps):
@@ -120,22 +112,16 @@ def main():
     s.sync()
     total = time.perf_counter() - t_all
     updates, march_s, spx = tally["updates"], tally["march_s"], tally["spx"]
-    # kernel durations: HIP events attached to each dispatch (hipExtLaunchKernelGGL) on the
-    # surface's stream, over a profiling pass that continues the same run
-    fprof(s.h, 1)
-    run(args.prof_steps, args.warmup + args.steps, [], {"updates": 0, "march_s": 0.0, "spx": 0})
-    s.sync()
+    # kernel durations: each pass's launches of one step, captured as a graph on the surface's
+    # stream and replayed between two HIP events on the final state of the timed run
     ktab = {}
-    for k, (name, bpi) in KERNELS.items():
-        n, ms, items = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_uint64()
-        fread(s.h, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(items))
-        if n.value:
-            ktab[k] = (name, bpi, n.value, ms.value, items.value)
-    fprof(s.h, 0)
-    dom = max(ktab, key=lambda k: ktab[k][3])
-    name, bpi, n, ms, items = ktab[dom]
-    avg_s = ms / 1e3 / n
-    bytes_per_launch = bpi * items / n
+    for k, (name, bpi) in PASSES.items():
+        us, items, n = s.time_pass(k, args.pass_reps)
+        ktab[k] = (name, bpi, n, us, items)
+    dom = max((k for k in ktab if k != 2), key=lambda k: ktab[k][2] * ktab[k][3])
+    name, bpi, n, us, items = ktab[dom]
+    avg_s = us / 1e6
+    bytes_per_launch = bpi * items
     achieved = bytes_per_launch / avg_s
     lat_ms = np.array(lat) * 1e3
     final = s.stats()
@@ -173,11 +159,12 @@ def main():
             "traffic": None,
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
-            "timing": f"HIP events attached to the dispatches of {args.prof_steps} steps run "
-                      "right after the timed region (profiled steps launch kernels one by one)",
+            "timing": f"the pass's launches of one step on the final state, captured as a "
+                      f"HIP graph and replayed {args.pass_reps} times between two HIP events "
+                      "on the surface's stream",
         },
-        "kernels": {v[0]: {"launches": v[2], "total_ms": v[3], "items": v[4]}
-                    for v in ktab.values()},
+        "kernels": {v[0]: {"launches_per_step": v[2], "us_per_launch": v[3],
+                           "items_per_launch": v[4]} for v in ktab.values()},
     }
     s.close()
     if not args.no_cpu:

@@ -30,6 +30,10 @@ LIST_DELTA = 1
 LIST_STATUS_CHANGE = 2
 LIST_AFFECTED = 3
 
+PASS_SPEED = 0
+PASS_DIST1 = 1
+PASS_BAND = 2
+
 
 class FeltError(RuntimeError):
     """Raised for a non-zero status; the reference throws std::domain_error there."""
@@ -124,6 +128,8 @@ class Binding:
             "surface_sync": (C.c_int, [_P]),
             "surface_step_info": (C.c_int, [_P, _U64P, _U64P]),
             "surface_set_graph": (C.c_int, [_P, C.c_int]),
+            "surface_time_pass": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_int)]),
             "polys_notify": (C.c_int, [_P]),
             "polys_march": (C.c_int, [_P]),
             "polys_invalidate": (C.c_int, [_P]),
@@ -146,7 +152,8 @@ class Binding:
         self.fn = {}
         for name, (res, args) in sig.items():
             if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info",
-                                                            "surface_set_graph"):
+                                                            "surface_set_graph",
+                                                            "surface_time_pass"):
                 continue
             f = getattr(L, prefix + name)
             f.restype = res
@@ -355,6 +362,14 @@ class Surface:
             return int(u.value), int(r.value)
         st = self.stats()
         return int(st.last_voxel_updates), int(st.last_rounds)
+
+    def time_pass(self, pass_id: int, reps: int = 20):
+        """(microseconds per launch, entries per launch, launches) of one pass of a global step
+        replayed on the current state (felt_surface_time_pass)."""
+        us, items, n = C.c_double(), C.c_double(), C.c_int()
+        self.b.call("surface_time_pass", self.h, int(pass_id), int(reps), C.byref(us),
+                    C.byref(items), C.byref(n))
+        return float(us.value), float(items.value), int(n.value)
 
     def sync(self) -> None:
         self.b.call("surface_sync", self.h)

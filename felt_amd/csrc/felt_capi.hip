@@ -419,6 +419,14 @@ API int felt_surface_profile(felt_surface* s, int enable) {
     CHECK_S(s);
     if (int rc = prof_collect(s)) return rc;
     s->prof.on = enable != 0;
+    if (s->prof.on) {
+        // 3 ms of the wall clock: longer than the host takes to enqueue one step
+        int dev = 0, khz = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+        if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+        s->prof.hold_ticks = 3LL * khz;
+    }
     for (int k = 0; k < PK_COUNT; k++) {
         s->prof.launches[k] = 0;
         s->prof.ms[k] = 0.0;
@@ -436,6 +444,15 @@ API int felt_surface_profile_read(felt_surface* s, int kernel, uint64_t* launche
     *ms = s->prof.ms[kernel];
     *items = (uint64_t)s->prof.items[kernel];
     return FELT_OK;
+}
+
+API int felt_surface_time_pass(felt_surface* s, int pass, int reps, double* us_per_launch,
+                               double* items_per_launch, int* launches) {
+    CHECK_S(s);
+    if (!us_per_launch || !items_per_launch || !launches)
+        return felt_fail(FELT_E_ARG, "null output");
+    if (int rc = prof_collect(s)) return rc;
+    return felt_time_pass_impl(s, pass, reps, us_per_launch, items_per_launch, launches);
 }
 
 API int felt_surface_sync(felt_surface* s) {

@@ -222,14 +222,15 @@ __global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta,
     }
 }
 
-// update_distance pass 1 (Surface.hpp:1029-1073).
-template <int D>
-__global__ void k_dist1(Geo G, Pool P, const uint32_t* list, const int8_t* tag, const int* rng,
-                        int layer, float* out, const int* live) {
-    if (!*live) return;
+// update_distance pass 1 (Surface.hpp:1029-1073) over the entries of one layer: d = the
+// distance from the layer's neighbours.  Tagged lists hold every layer and are filtered by tag.
+template <int D, bool Tagged>
+__device__ __forceinline__ void dist1_body(const Geo& G, const Pool& P, const uint32_t* list,
+                                           const int8_t* tag, const int* rng, int layer,
+                                           float* out) {
     const float side = layer < 0 ? -1.0f : 1.0f;
     FELT_SPAN_LOOP(i, rng) {
-        if (tag && tag[i] != layer) continue;
+        if (Tagged && tag[i] != layer) continue;
         const uint32_t v = list[i];
         int slot, loc[3], g[3];
         decode<D>(G, P, v, slot, loc, g);
@@ -238,16 +239,46 @@ __global__ void k_dist1(Geo G, Pool P, const uint32_t* list, const int8_t* tag, 
 }
 
 // update_distance pass 2 (Surface.hpp:1076-1121): phi = d; tk = new layer.
-__global__ void k_dist2(Geo G, Pool P, const uint32_t* list, const int8_t* tag, const int* rng,
-                        int layer, const float* d, int8_t* tk, int* nan_ctr, const int* live) {
-    if (!*live) return;
+template <bool Tagged>
+__device__ __forceinline__ void dist2_body(const Geo& G, const Pool& P, const uint32_t* list,
+                                           const int8_t* tag, const int* rng, int layer,
+                                           const float* d, int8_t* tk, int* nan_ctr) {
     FELT_SPAN_LOOP(i, rng) {
-        if (tag && tag[i] != layer) continue;
+        if (Tagged && tag[i] != layer) continue;
         const uint32_t v = list[i];
         const float x = d[i];
         P.phi[v] = x;
         tk[i] = clamp_k(layer_of(x, G.L, nan_ctr));
     }
+}
+
+// First round of a global step: the span is the layer's segment of the band list.
+template <int D>
+__global__ void k_dist1_band(Geo G, Pool P, const uint32_t* list, const int* rng, int layer,
+                             float* out, const int* live) {
+    if (!*live) return;
+    dist1_body<D, false>(G, P, list, nullptr, rng, layer, out);
+}
+
+__global__ void k_dist2_band(Geo G, Pool P, const uint32_t* list, const int* rng, int layer,
+                             const float* d, int8_t* tk, int* nan_ctr, const int* live) {
+    if (!*live) return;
+    dist2_body<false>(G, P, list, nullptr, rng, layer, d, tk, nan_ctr);
+}
+
+// Later rounds and bounded updates: a lookup holding every layer, filtered by tag.
+template <int D>
+__global__ void k_dist1_lookup(Geo G, Pool P, const uint32_t* list, const int8_t* tag,
+                               const int* rng, int layer, float* out, const int* live) {
+    if (!*live) return;
+    dist1_body<D, true>(G, P, list, tag, rng, layer, out);
+}
+
+__global__ void k_dist2_lookup(Geo G, Pool P, const uint32_t* list, const int8_t* tag,
+                               const int* rng, int layer, const float* d, int8_t* tk,
+                               int* nan_ctr, const int* live) {
+    if (!*live) return;
+    dist2_body<true>(G, P, list, tag, rng, layer, d, tk, nan_ctr);
 }
 
 // Change compaction, pass 1: per block the number of status changes and of in-band targets;

@@ -33,6 +33,7 @@ struct ProfRec {
 // items >= 0: entries processed; items < 0: band segment -(items+1), resolved after the step.
 struct Prof {
     bool on = false;
+    long long hold_ticks = 0;  // wall-clock ticks a profiled step waits behind k_hold
     std::vector<ProfRec> pending;
     std::vector<hipEvent_t> spare;
     long long launches[PK_COUNT] = {};
@@ -81,6 +82,7 @@ struct felt_surface {
     int* bbase = nullptr;           // band segment starts, NL+1 entries (device)
     felt::Speed* d_speed = nullptr;  // speed descriptor read by the speed kernel (device)
     felt::Speed* h_speed = nullptr;  // pinned staging copy
+    bool speed_valid = false;        // d_speed holds the last global update's functor
     size_t list_cap = 0;             // capacity of every per-step list
     std::vector<int> hbase;          // host copy of bbase after a profiled step
     bool band_known = false;         // band_now is the current tracked-voxel count
@@ -110,6 +112,8 @@ void felt_surface_release(felt_surface* s);
 felt::Lu felt_lu(felt_surface* s, int i);
 int felt_update_start_impl(felt_surface* s);
 int felt_update_global(felt_surface* s, const felt_speed* fn);
+int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double* items,
+                        int* launches);
 int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
 int felt_update_box_impl(felt_surface* s, const int32_t* lo, const int32_t* hi,
                          const felt_speed* fn);

@@ -33,6 +33,17 @@ __global__ void k_fill_i32(int* p, int64_t n, int v) {
         p[i] = v;
 }
 
+// Keeps the stream busy for `ticks` of the constant-rate wall clock (at most 2^20 sleeps), so
+// that the host has enqueued every kernel of a profiled step before the first one starts and
+// each dispatch's events then bracket the kernel alone, not the host's enqueue time.
+__global__ void k_hold(long long ticks) {
+    const long long t0 = (long long)wall_clock64();
+    for (int i = 0; i < (1 << 20); i++) {
+        if ((long long)wall_clock64() - t0 >= ticks) break;
+        __builtin_amdgcn_s_sleep(127);
+    }
+}
+
 __global__ void k_init_free(int* stack, int cap) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x)
         stack[i] = cap - 1 - i;
@@ -690,11 +701,18 @@ static void distance_passes(felt_surface* s, bool from_band, int look, const int
                 tk = s->lu_tk[look].ptr;
             }
             // only the first round's launches are attributed item counts (band segments)
-            const long long spec = from_band ? -(long long)(layer + G.L) - 1 : 0;
-            PDISPATCH(from_band ? PK_DIST1 : PK_DIST1_LATE, spec, G.D, k_dist1, dim3(grid),
-                      dim3(TPB), G, s->P, list, tag, rng, layer, d, live);
-            PLAUNCH(from_band ? PK_DIST2 : PK_DIST2_LATE, spec, k_dist2, dim3(grid), dim3(TPB), G,
-                    s->P, list, tag, rng, layer, d, tk, &s->ctr->nan_events, live);
+            if (from_band) {
+                const long long spec = -(long long)(layer + G.L) - 1;
+                PDISPATCH(PK_DIST1, spec, G.D, k_dist1_band, dim3(grid), dim3(TPB), G, s->P, list,
+                          rng, layer, d, live);
+                PLAUNCH(PK_DIST2, spec, k_dist2_band, dim3(grid), dim3(TPB), G, s->P, list, rng,
+                        layer, d, tk, &s->ctr->nan_events, live);
+            } else {
+                PDISPATCH(PK_DIST1_LATE, 0, G.D, k_dist1_lookup, dim3(grid), dim3(TPB), G, s->P,
+                          list, tag, rng, layer, d, live);
+                PLAUNCH(PK_DIST2_LATE, 0, k_dist2_lookup, dim3(grid), dim3(TPB), G, s->P, list, tag,
+                        rng, layer, d, tk, &s->ctr->nan_events, live);
+            }
         }
     }
 }
@@ -833,6 +851,8 @@ static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
     // Kernel timing attaches events to individual dispatches, which a replayed graph does not
     // allow, so a profiled step launches its kernels one by one.
     const bool graph = use_graph && !s->prof.on;
+    if (s->prof.on && s->prof.hold_ticks > 0)
+        LAUNCH(k_hold, dim3(1), dim3(64), s->prof.hold_ticks);
     if (graph) {
         if (!s->graph_valid) {
             if (s->graph_exec) {
@@ -879,10 +899,83 @@ int felt_update_global(felt_surface* s, const felt_speed* fn) {
         return felt_fail(FELT_E_ARG, "noise cell must be positive");
     *s->h_speed = sp;
     HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, sizeof(Speed), hipMemcpyHostToDevice, s->st));
+    s->speed_valid = true;
     return global_step(s, [&](const int* rng) {
         PDISPATCH(PK_SPEED, -(long long)s->G.L - 1, s->G.D, k_speed, dim3(grid_for_lists(s)),
                   dim3(TPB), s->G, s->P, s->band.ptr, rng, s->d_speed, s->band_d.ptr);
     }, s->use_graph);
+}
+
+// Times one pass of a global step on the current state (felt_surface_time_pass).
+int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double* items,
+                        int* launches) {
+    if (s->in_update) return felt_fail(FELT_E_STATE, "timing inside update_start/update_end");
+    if (pass < FELT_PASS_SPEED || pass > FELT_PASS_BAND) return felt_fail(FELT_E_ARG, "unknown pass");
+    if (reps < 1) return felt_fail(FELT_E_ARG, "reps must be positive");
+    if (pass == FELT_PASS_SPEED && !s->speed_valid)
+        return felt_fail(FELT_E_STATE, "no device functor has been run");
+    int rc;
+    if ((rc = size_for_band(s))) return rc;
+    const Geo& G = s->G;
+    const bool prof = s->prof.on;
+    s->prof.on = false;  // plain launches inside the capture
+    LAUNCH(k_fill_i32, dim3(1), dim3(1), &s->ctr->one, (int64_t)1, 1);
+    compact_band(s);
+    std::vector<int> hb(G.NL + 1);
+    HIPX(hipMemcpyAsync(hb.data(), s->bbase, (G.NL + 1) * sizeof(int), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    // The graph holds `reps` copies of the pass, so graph-launch latency is paid once.
+    int n = 0;
+    long long entries = 0;
+    HIPX(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < reps; r++) {
+        n = 0;
+        entries = 0;
+        if (pass == FELT_PASS_SPEED) {
+            DISPATCH(G.D, k_speed, dim3(grid_for_lists(s)), dim3(TPB), G, s->P, s->band.ptr,
+                     s->bbase + G.L, s->d_speed, s->band_d.ptr);
+            n = 1;
+            entries = hb[G.L + 1] - hb[G.L];
+        } else if (pass == FELT_PASS_DIST1) {
+            for (int layer = -G.L; layer <= G.L; layer++) {
+                if (layer == 0) continue;
+                const int k = layer + G.L;
+                DISPATCH(G.D, k_dist1_band, dim3(grid_for_lists(s)), dim3(TPB), G, s->P,
+                         s->band.ptr, s->bbase + k, layer, s->band_d.ptr, always(s));
+                n++;
+                entries += hb[k + 1] - hb[k];
+            }
+        } else {
+            compact_band(s);
+            n = 3;
+            entries = (long long)G.np * G.pv;
+        }
+    }
+    hipGraph_t g = nullptr;
+    HIPX(hipStreamEndCapture(s->st, &g));
+    s->prof.on = prof;
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    hipEvent_t a = nullptr, b = nullptr;
+    float ms = 0.f;
+    e = hipGraphLaunch(ge, s->st);  // warm-up replay
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, s->st);
+    if (e == hipSuccess) e = hipGraphLaunch(ge, s->st);
+    if (e == hipSuccess) e = hipEventRecord(b, s->st);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipGraphExecDestroy(ge);
+    if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    *us = (double)ms * 1000.0 / ((double)reps * n);
+    *items = (double)entries / n;
+    *launches = n;
+    return FELT_OK;
 }
 
 // Host-evaluated deltas: scattered through a per-voxel scratch onto the zero-layer list.

@@ -155,6 +155,21 @@ int felt_surface_step_info(felt_surface* s, uint64_t* voxel_updates, uint64_t* r
 int felt_surface_profile(felt_surface* s, int enable);
 int felt_surface_profile_read(felt_surface* s, int kernel, uint64_t* launches, double* ms,
                               uint64_t* items);
+/* Times one pass of a global step on the surface's current state.  The band list is rebuilt
+ * as a step starts, then `reps` copies of the pass's launches of one step are captured as one
+ * graph, which is replayed once to warm up and once between two HIP events on the handle's
+ * stream.  Passes:
+ * FELT_PASS_SPEED, one launch of the last global update's device functor over the zero layer;
+ * FELT_PASS_DIST1, the first-round distance pass 1, one launch per non-zero layer;
+ * FELT_PASS_BAND, band compaction (three kernels).  They write only scratch (deltas, distances,
+ * the band list), so the surface is unchanged.  Outputs: microseconds per launch, list entries
+ * per launch (for FELT_PASS_BAND the pool voxels scanned), launches per replay.  Not a
+ * reference member. */
+#define FELT_PASS_SPEED 0
+#define FELT_PASS_DIST1 1
+#define FELT_PASS_BAND 2
+int felt_surface_time_pass(felt_surface* s, int pass, int reps, double* us_per_launch,
+                           double* items_per_launch, int* launches);
 /* Block until all queued device work of the handle is done. */
 int felt_surface_sync(felt_surface* s);
 
