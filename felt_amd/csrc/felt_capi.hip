@@ -33,13 +33,13 @@ int layer_host(float v) {
     return (int)std::round(t);
 }
 
-// World position -> grid coordinates (3 ints, unused axes 0); false outside the grid.
-bool to_grid(const Geo& G, const int32_t* w, int* g) {
+// World position -> member grid coordinates (3 ints, unused axes 0); false outside the grid.
+bool to_grid(const Mem& M, int D, const int32_t* w, int* g) {
     bool in = true;
     for (int a = 0; a < 3; a++) {
-        if (a < G.D) {
-            g[a] = w[a] - G.offset[a];
-            if (g[a] < 0 || g[a] >= G.size[a]) in = false;
+        if (a < D) {
+            g[a] = w[a] - M.offset[a];
+            if (g[a] < 0 || g[a] >= M.size[a]) in = false;
         } else {
             g[a] = 0;
         }
@@ -47,11 +47,11 @@ bool to_grid(const Geo& G, const int32_t* w, int* g) {
     return in;
 }
 
-int check_deltas(const Geo& G, size_t n, const int32_t* pos, const float* d,
+int check_deltas(const felt_surface* s, size_t n, const int32_t* pos, const float* d,
                  std::vector<int>& gpos) {
     gpos.resize(n * 3);
     for (size_t i = 0; i < n; i++) {
-        if (!to_grid(G, pos + i * G.D, gpos.data() + i * 3))
+        if (!to_grid(s->mems[0], s->G.D, pos + i * s->G.D, gpos.data() + i * 3))
             return felt_fail(FELT_E_ARG, "delta position outside the grid");
         const int k = layer_host(d[i]);
         if (k < -1 || k > 1) return felt_fail(FELT_E_DELTA, "delta would jump more than one layer");
@@ -59,65 +59,77 @@ int check_deltas(const Geo& G, size_t n, const int32_t* pos, const float* d,
     return FELT_OK;
 }
 
-}  // namespace
-
-API const char* felt_last_error(void) { return g_felt_err.c_str(); }
-
-API int felt_device_count(int* n) {
-    if (!n) return felt_fail(FELT_E_ARG, "null");
-    hipError_t e = hipGetDeviceCount(n);
-    if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
-    return FELT_OK;
-}
-
-API int felt_surface_create(int D, int L, const int32_t* size, const int32_t* part,
-                            felt_surface** out) {
-    if ((D != 2 && D != 3) || L < 1 || L > 8 || !size || !part || !out)
+// A pool of n members of sizes[i*D..] sharing D, L and the partition size.  Each member's
+// geometry follows Surface(size, part) (Surface.hpp:163-173): offset -size/2, children grid
+// size/part, one more partition per axis when any axis does not divide
+// (PartitionedMixin.hpp:131-141).  The brick pool holds every partition unless max_bricks
+// limits it; voxel ids are 32-bit, so it never exceeds 2^32 / prod(part) bricks.
+int make_pool(int D, int L, const int32_t* part, int n, const int32_t* sizes, uint64_t max_bricks,
+              felt_surface** out) {
+    if ((D != 2 && D != 3) || L < 1 || L > 8 || !sizes || !part || !out || n < 1)
         return felt_fail(FELT_E_ARG, "bad surface arguments (D in {2,3}, 1 <= L <= 8)");
     Geo G{};
     G.D = D;
     G.L = L;
     G.NL = 2 * L + 1;
-    bool uneven = false;
-    int64_t np = 1, pv = 1;
+    G.nmem = n;
+    int64_t pv = 1;
     for (int a = 0; a < 3; a++) {
-        if (a < D) {
-            if (size[a] < 1 || part[a] < 1) return felt_fail(FELT_E_ARG, "sizes must be positive");
-            G.size[a] = size[a];
-            G.part[a] = part[a];
-            G.offset[a] = -1 * size[a] / 2;                       // Surface.hpp:1716-1719
-            G.csize[a] = size[a] / part[a];
-            if (G.csize[a] * part[a] != size[a]) uneven = true;
-        } else {
-            G.size[a] = G.part[a] = G.csize[a] = 1;
-            G.offset[a] = 0;
-        }
-    }
-    if (uneven)                                                    // PartitionedMixin.hpp:131-141
-        for (int a = 0; a < D; a++) G.csize[a] += 1;
-    for (int a = 0; a < 3; a++) {
-        np *= G.csize[a];
+        if (a < D && part[a] < 1) return felt_fail(FELT_E_ARG, "sizes must be positive");
+        G.part[a] = a < D ? part[a] : 1;
         pv *= G.part[a];
     }
-    if (np > (1 << 30) || pv > (1 << 24) || np * pv > (int64_t)0xffffffffll)
-        return felt_fail(FELT_E_ARG, "grid too large for 32-bit voxel indices");
-    G.np = (int)np;
+    if (pv > (1 << 24)) return felt_fail(FELT_E_ARG, "partition too large");
     G.pv = (int)pv;
     G.lstr[2] = 1;
     G.lstr[1] = G.part[2];
     G.lstr[0] = G.part[1] * G.part[2];
-    G.cstr[2] = 1;
-    G.cstr[1] = G.csize[2];
-    G.cstr[0] = G.csize[1] * G.csize[2];
-    for (int a = 0; a < 3; a++) {
-        G.fpart[a] = make_fastdiv((uint32_t)G.part[a]);
-        G.fcsize[a] = make_fastdiv((uint32_t)G.csize[a]);
-    }
+    for (int a = 0; a < 3; a++) G.fpart[a] = make_fastdiv((uint32_t)G.part[a]);
     G.fpv = make_fastdiv((uint32_t)G.pv);
+    std::vector<Mem> mems((size_t)n);
+    int64_t np_total = 0;
+    for (int i = 0; i < n; i++) {
+        Mem& M = mems[i];
+        std::memset(&M, 0, sizeof(Mem));
+        bool uneven = false;
+        for (int a = 0; a < 3; a++) {
+            if (a < D) {
+                const int sz = sizes[(size_t)i * D + a];
+                if (sz < 1) return felt_fail(FELT_E_ARG, "sizes must be positive");
+                M.size[a] = sz;
+                M.offset[a] = -1 * sz / 2;  // Surface.hpp:1716-1719
+                M.csize[a] = sz / G.part[a];
+                if (M.csize[a] * G.part[a] != sz) uneven = true;
+            } else {
+                M.size[a] = M.csize[a] = 1;
+                M.offset[a] = 0;
+            }
+        }
+        if (uneven)  // PartitionedMixin.hpp:131-141
+            for (int a = 0; a < D; a++) M.csize[a] += 1;
+        int64_t np = 1;
+        for (int a = 0; a < 3; a++) np *= M.csize[a];
+        M.cstr[2] = 1;
+        M.cstr[1] = M.csize[2];
+        M.cstr[0] = M.csize[1] * M.csize[2];
+        for (int a = 0; a < 3; a++) M.fcsize[a] = make_fastdiv((uint32_t)M.csize[a]);
+        M.pbase = (int)np_total;
+        M.np = (int)std::min<int64_t>(np, 1 << 30);
+        np_total += np;
+        if (np_total > (1 << 30)) return felt_fail(FELT_E_ARG, "too many partitions");
+    }
+    G.np = (int)np_total;
+    int64_t cap = np_total;
+    const int64_t id_limit = (int64_t)0xffffffffll / pv;
+    if (max_bricks > 0) cap = std::min<int64_t>(cap, (int64_t)max_bricks);
+    if (n == 1 && cap > id_limit)
+        return felt_fail(FELT_E_ARG, "grid too large for 32-bit voxel indices");
+    cap = std::min(cap, id_limit);
     felt_surface* s = new (std::nothrow) felt_surface();
     if (!s) return felt_fail(FELT_E_NOMEM, "host allocation failed");
     s->G = G;
-    s->cap = G.np;
+    s->mems = std::move(mems);
+    s->cap = (int)cap;
     int rc = felt_surface_init(s);
     if (rc) {
         felt_surface_release(s);
@@ -128,23 +140,14 @@ API int felt_surface_create(int D, int L, const int32_t* size, const int32_t* pa
     return FELT_OK;
 }
 
-API int felt_surface_destroy(felt_surface* s) {
-    if (!s) return FELT_OK;
-    felt_surface_release(s);
-    delete s;
-    return FELT_OK;
-}
-
-API int felt_surface_geometry(felt_surface* s, int32_t* cs, uint64_t* np) {
-    CHECK_S(s);
-    for (int a = 0; a < 3; a++) cs[a] = s->G.csize[a];
-    *np = (uint64_t)s->G.np;
-    return FELT_OK;
+size_t grid_total(const felt_surface* s, int m) {
+    size_t t = 1;
+    for (int a = 0; a < s->G.D; a++) t *= (size_t)s->mems[m].size[a];
+    return t;
 }
 
 // seed (Surface.hpp:207-241): L1 distances on the (2L+1)^D window, tracked by layer.
-API int felt_surface_seed(felt_surface* s, const int32_t* pos) {
-    CHECK_S(s);
+int seed_m(felt_surface* s, int m, const int32_t* pos) {
     const Geo& G = s->G;
     const int w = 2 * G.L + 1;
     int64_t n = 1;
@@ -168,49 +171,156 @@ API int felt_surface_seed(felt_surface* s, const int32_t* pos) {
         const int k = layer_host((float)dist);
         if (k < -G.L || k > G.L) continue;
         int g[3];
-        if (!to_grid(G, wp, g)) continue;
+        if (!to_grid(s->mems[m], G.D, wp, g)) continue;
         gp.insert(gp.end(), g, g + 3);
         val.push_back((float)dist);
         lay.push_back(k);
     }
-    return felt_track_points(s, gp, val, lay);
+    return felt_track_points(s, m, gp, val, lay);
+}
+
+int check_speed(const felt_speed* fn) {
+    if (!fn || fn->kind < 0 || fn->kind > FELT_SPEED_NOISE_CURV)
+        return felt_fail(FELT_E_ARG, "bad speed descriptor");
+    if (fn->kind == FELT_SPEED_NOISE_CURV && fn->cell <= 0)
+        return felt_fail(FELT_E_ARG, "noise cell must be positive");
+    return FELT_OK;
+}
+
+int emit_positions(int D, const std::vector<int>& p3, int32_t* pos, size_t cap, size_t* n) {
+    const size_t k = p3.size() / 3;
+    *n = k;
+    if (cap < k) return felt_fail(FELT_E_CAPACITY, "position buffer too small");
+    for (size_t i = 0; i < k; i++)
+        for (int a = 0; a < D; a++) pos[i * D + a] = p3[i * 3 + a];
+    return FELT_OK;
+}
+
+// Member m's slice of the per-partition layer counts.
+int member_counts(felt_surface* s, int m, std::vector<int>& cnt) {
+    const Mem& M = s->mems[m];
+    cnt.resize((size_t)M.np * s->G.NL);
+    return felt_download(s, cnt.data(), s->P.pcnt + (size_t)M.pbase * s->G.NL,
+                         cnt.size() * sizeof(int));
+}
+
+int layer_size_m(felt_surface* s, int m, int layer, uint64_t* n) {
+    const Geo& G = s->G;
+    if (layer < -G.L || layer > G.L) return felt_fail(FELT_E_ARG, "layer out of range");
+    std::vector<int> cnt;
+    if (int rc = member_counts(s, m, cnt)) return rc;
+    uint64_t t = 0;
+    for (int p = 0; p < s->mems[m].np; p++) t += (uint64_t)cnt[(size_t)p * G.NL + layer + G.L];
+    *n = t;
+    return FELT_OK;
+}
+
+int layer_positions_m(felt_surface* s, int m, int layer, int32_t* pos, size_t cap, size_t* n) {
+    if (layer < -s->G.L || layer > s->G.L) return felt_fail(FELT_E_ARG, "layer out of range");
+    std::vector<int> p3;
+    if (int rc = felt_collect_layer_impl(s, m, layer, p3)) return rc;
+    return emit_positions(s->G.D, p3, pos, cap, n);
+}
+
+// Surface::Stats (Surface.hpp:623-629, 1732-1742): the longest parent list of each grid.
+int stats_m(felt_surface* s, int m, felt_stats* st) {
+    const Geo& G = s->G;
+    const Mem& M = s->mems[m];
+    std::vector<int> cnt;
+    std::vector<unsigned> dm(M.np);
+    if (int rc = member_counts(s, m, cnt)) return rc;
+    if (int rc = felt_download(s, dm.data(), s->dmask + M.pbase, dm.size() * sizeof(unsigned)))
+        return rc;
+    if (int rc = felt_download(s, &s->hc, s->ctr, sizeof(Ctr))) return rc;
+    uint64_t best_i = 0, best_d = 0, band = 0;
+    for (int k = 0; k < G.NL; k++) {
+        uint64_t ni = 0, nd = 0;
+        for (int p = 0; p < M.np; p++) {
+            const int c = cnt[(size_t)p * G.NL + k];
+            if (c > 0) ni++;
+            band += (uint64_t)c;
+            if ((dm[p] >> k) & 1u) nd++;
+        }
+        best_i = std::max(best_i, ni);
+        best_d = std::max(best_d, nd);
+    }
+    st->active_isogrid_partitions = best_i;
+    st->active_delta_partitions = best_d;
+    st->band_voxels = band;
+    st->last_voxel_updates = s->last_updates;
+    st->last_rounds = s->last_rounds;
+    st->nan_events = (uint64_t)s->hc.nan_events;
+    return FELT_OK;
+}
+
+}  // namespace
+
+API const char* felt_last_error(void) { return g_felt_err.c_str(); }
+
+API int felt_device_count(int* n) {
+    if (!n) return felt_fail(FELT_E_ARG, "null");
+    hipError_t e = hipGetDeviceCount(n);
+    if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    return FELT_OK;
+}
+
+API int felt_surface_create(int D, int L, const int32_t* size, const int32_t* part,
+                            felt_surface** out) {
+    if (!size) return felt_fail(FELT_E_ARG, "null size");
+    return make_pool(D, L, part, 1, size, 0, out);
+}
+
+API int felt_surface_destroy(felt_surface* s) {
+    if (!s) return FELT_OK;
+    felt_surface_release(s);
+    delete s;
+    return FELT_OK;
+}
+
+API int felt_surface_geometry(felt_surface* s, int32_t* cs, uint64_t* np) {
+    CHECK_S(s);
+    for (int a = 0; a < 3; a++) cs[a] = s->mems[0].csize[a];
+    *np = (uint64_t)s->mems[0].np;
+    return FELT_OK;
+}
+
+API int felt_surface_seed(felt_surface* s, const int32_t* pos) {
+    CHECK_S(s);
+    if (!pos) return felt_fail(FELT_E_ARG, "null position");
+    return seed_m(s, 0, pos);
 }
 
 API int felt_surface_load_dense(felt_surface* s, const float* phi, size_t n) {
     CHECK_S(s);
-    size_t total = 1;
-    for (int a = 0; a < s->G.D; a++) total *= (size_t)s->G.size[a];
-    if (n != total || !phi) return felt_fail(FELT_E_ARG, "load_dense: size mismatch");
-    return felt_load_dense_impl(s, phi, n);
+    if (n != grid_total(s, 0) || !phi) return felt_fail(FELT_E_ARG, "load_dense: size mismatch");
+    return felt_load_dense_impl(s, 0, phi, n);
 }
 
 API int felt_surface_update(felt_surface* s, const felt_speed* fn) {
     CHECK_S(s);
-    if (!fn || fn->kind < 0 || fn->kind > FELT_SPEED_NOISE_CURV)
-        return felt_fail(FELT_E_ARG, "bad speed descriptor");
+    if (int rc = check_speed(fn)) return rc;
     return felt_update_global(s, fn);
 }
 
 API int felt_surface_update_box(felt_surface* s, const int32_t* lo, const int32_t* hi,
                                 const felt_speed* fn) {
     CHECK_S(s);
-    if (!fn || !lo || !hi || fn->kind < 0 || fn->kind > FELT_SPEED_NOISE_CURV)
-        return felt_fail(FELT_E_ARG, "bad arguments");
-    if (fn->kind == FELT_SPEED_NOISE_CURV && fn->cell <= 0)
-        return felt_fail(FELT_E_ARG, "noise cell must be positive");
+    if (!lo || !hi) return felt_fail(FELT_E_ARG, "bad arguments");
+    if (int rc = check_speed(fn)) return rc;
     int l3[3] = {0, 0, 0}, h3[3] = {0, 0, 0};
     for (int a = 0; a < s->G.D; a++) {
         l3[a] = lo[a];
         h3[a] = hi[a];
     }
-    return felt_update_box_impl(s, l3, h3, fn);
+    const int id = 0;
+    return felt_update_boxes_impl(s, 1, &id, l3, h3, fn);
 }
 
 API int felt_surface_update_deltas(felt_surface* s, size_t n, const int32_t* pos,
                                    const float* delta) {
     CHECK_S(s);
     std::vector<int> g;
-    if (int rc = check_deltas(s->G, n, pos, delta, g)) return rc;
+    if (int rc = check_deltas(s, n, pos, delta, g)) return rc;
     return felt_update_global_deltas(s, n, g.data(), delta);
 }
 
@@ -222,7 +332,7 @@ API int felt_surface_update_start(felt_surface* s) {
 API int felt_surface_delta(felt_surface* s, size_t n, const int32_t* pos, const float* delta) {
     CHECK_S(s);
     std::vector<int> g;
-    if (int rc = check_deltas(s->G, n, pos, delta, g)) return rc;
+    if (int rc = check_deltas(s, n, pos, delta, g)) return rc;
     return felt_delta_impl(s, n, g.data(), delta);
 }
 
@@ -233,100 +343,80 @@ API int felt_surface_update_end(felt_surface* s) {
 
 API int felt_surface_snapshot(felt_surface* s, float* dst, size_t n) {
     CHECK_S(s);
-    size_t total = 1;
-    for (int a = 0; a < s->G.D; a++) total *= (size_t)s->G.size[a];
+    const size_t total = grid_total(s, 0);
     if (n < total) return felt_fail(FELT_E_CAPACITY, "snapshot buffer too small");
-    return felt_snapshot_impl(s, dst, total);
+    return felt_snapshot_impl(s, 0, dst, total);
 }
 
 API int felt_surface_get(felt_surface* s, size_t n, const int32_t* pos, float* out) {
     CHECK_S(s);
     std::vector<int> g(n * 3);
-    std::vector<int> outside;
     for (size_t i = 0; i < n; i++)
-        if (!to_grid(s->G, pos + i * s->G.D, g.data() + i * 3)) {
+        if (!to_grid(s->mems[0], s->G.D, pos + i * s->G.D, g.data() + i * 3)) {
             // out-of-grid reads return the grid background; keep them off the device
             for (int a = 0; a < 3; a++) g[i * 3 + a] = -1;
         }
-    return felt_get_impl(s, n, g.data(), out);
+    return felt_get_impl(s, 0, n, g.data(), out);
 }
 
 API int felt_surface_layer_size(felt_surface* s, int layer, uint64_t* n) {
     CHECK_S(s);
-    const Geo& G = s->G;
-    if (layer < -G.L || layer > G.L) return felt_fail(FELT_E_ARG, "layer out of range");
-    std::vector<int> cnt((size_t)G.np * G.NL);
-    if (int rc = felt_download(s, cnt.data(), s->P.pcnt, cnt.size() * sizeof(int))) return rc;
-    uint64_t t = 0;
-    for (int p = 0; p < G.np; p++) t += (uint64_t)cnt[(size_t)p * G.NL + layer + G.L];
-    *n = t;
-    return FELT_OK;
-}
-
-static int emit_positions(const Geo& G, const std::vector<int>& p3, int32_t* pos, size_t cap,
-                          size_t* n) {
-    const size_t m = p3.size() / 3;
-    *n = m;
-    if (cap < m) return felt_fail(FELT_E_CAPACITY, "position buffer too small");
-    for (size_t i = 0; i < m; i++)
-        for (int a = 0; a < G.D; a++) pos[i * G.D + a] = p3[i * 3 + a];
-    return FELT_OK;
+    return layer_size_m(s, 0, layer, n);
 }
 
 API int felt_surface_layer_positions(felt_surface* s, int layer, int32_t* pos, size_t cap,
                                      size_t* n) {
     CHECK_S(s);
-    if (layer < -s->G.L || layer > s->G.L) return felt_fail(FELT_E_ARG, "layer out of range");
-    std::vector<int> p3;
-    if (int rc = felt_collect_layer_impl(s, layer, p3)) return rc;
-    return emit_positions(s->G, p3, pos, cap, n);
+    return layer_positions_m(s, 0, layer, pos, cap, n);
 }
 
 API int felt_surface_zero_layer_in_box(felt_surface* s, const int32_t* lo, const int32_t* hi,
                                        int32_t* pos, size_t cap, size_t* n) {
     CHECK_S(s);
     const Geo& G = s->G;
+    const Mem& M = s->mems[0];
     std::vector<int> p3;
-    if (int rc = felt_collect_layer_impl(s, 0, p3)) return rc;
+    if (int rc = felt_collect_layer_impl(s, 0, 0, p3)) return rc;
     std::vector<int> kept;
     for (size_t i = 0; i < p3.size() / 3; i++) {
         bool in = true;
         for (int a = 0; a < G.D; a++) {
-            const int hb = std::min(G.offset[a] + G.size[a], hi[a] + 1);
+            const int hb = std::min(M.offset[a] + M.size[a], hi[a] + 1);
             if (p3[i * 3 + a] < lo[a] || p3[i * 3 + a] >= hb) in = false;
         }
         if (in) kept.insert(kept.end(), p3.begin() + i * 3, p3.begin() + i * 3 + 3);
     }
-    return emit_positions(G, kept, pos, cap, n);
+    return emit_positions(G.D, kept, pos, cap, n);
 }
 
 API int felt_surface_partition_list(felt_surface* s, int kind, int k, uint64_t* out, size_t cap,
                                     size_t* n) {
     CHECK_S(s);
     const Geo& G = s->G;
+    const Mem& M = s->mems[0];
     if (k < 0 || k >= G.NL) return felt_fail(FELT_E_ARG, "list index out of range");
     std::vector<uint64_t> parts;
     if (kind == FELT_LIST_ISOGRID) {
-        std::vector<int> cnt((size_t)G.np * G.NL);
-        if (int rc = felt_download(s, cnt.data(), s->P.pcnt, cnt.size() * sizeof(int))) return rc;
-        for (int p = 0; p < G.np; p++)
+        std::vector<int> cnt;
+        if (int rc = member_counts(s, 0, cnt)) return rc;
+        for (int p = 0; p < M.np; p++)
             if (cnt[(size_t)p * G.NL + k] > 0) parts.push_back(p);
     } else if (kind == FELT_LIST_DELTA || kind == FELT_LIST_STATUS_CHANGE) {
-        std::vector<unsigned> m(G.np);
-        if (int rc = felt_download(s, m.data(), kind == FELT_LIST_DELTA ? s->dmask : s->smask,
-                                   m.size() * sizeof(unsigned)))
+        std::vector<unsigned> mk(M.np);
+        if (int rc = felt_download(s, mk.data(), kind == FELT_LIST_DELTA ? s->dmask : s->smask,
+                                   mk.size() * sizeof(unsigned)))
             return rc;
-        for (int p = 0; p < G.np; p++)
-            if ((m[p] >> k) & 1u) parts.push_back(p);
+        for (int p = 0; p < M.np; p++)
+            if ((mk[p] >> k) & 1u) parts.push_back(p);
     } else if (kind == FELT_LIST_AFFECTED) {
         std::vector<int> p3;
-        if (int rc = felt_lu_positions_impl(s, 0, k - G.L, p3)) return rc;
-        std::vector<char> seen(G.np, 0);
+        if (int rc = felt_lu_positions_impl(s, 0, 0, k - G.L, p3)) return rc;
+        std::vector<char> seen(M.np, 0);
         for (size_t i = 0; i < p3.size() / 3; i++) {
             int g[3];
-            to_grid(G, p3.data() + i * 3, g);
+            to_grid(M, G.D, p3.data() + i * 3, g);
             int p = 0;
-            for (int a = 0; a < G.D; a++) p += (g[a] / G.part[a]) * G.cstr[a];
+            for (int a = 0; a < G.D; a++) p += (g[a] / G.part[a]) * M.cstr[a];
             if (!seen[p]) {
                 seen[p] = 1;
                 parts.push_back(p);
@@ -347,24 +437,24 @@ API int felt_surface_affected_positions(felt_surface* s, int k, int32_t* pos, si
     CHECK_S(s);
     if (k < 0 || k >= s->G.NL) return felt_fail(FELT_E_ARG, "list index out of range");
     std::vector<int> p3;
-    if (int rc = felt_lu_positions_impl(s, 0, k - s->G.L, p3)) return rc;
-    return emit_positions(s->G, p3, pos, cap, n);
+    if (int rc = felt_lu_positions_impl(s, 0, 0, k - s->G.L, p3)) return rc;
+    return emit_positions(s->G.D, p3, pos, cap, n);
 }
 
 API int felt_surface_partition_active(felt_surface* s, uint8_t* out, size_t n) {
     CHECK_S(s);
-    const Geo& G = s->G;
-    if (n < (size_t)G.np) return felt_fail(FELT_E_CAPACITY, "buffer too small");
-    std::vector<int> page(G.np);
+    const Mem& M = s->mems[0];
+    if (n < (size_t)M.np) return felt_fail(FELT_E_CAPACITY, "buffer too small");
+    std::vector<int> page(M.np);
     if (int rc = felt_download(s, page.data(), s->P.page, page.size() * sizeof(int))) return rc;
-    for (int p = 0; p < G.np; p++) out[p] = page[p] >= 0 ? 1 : 0;
+    for (int p = 0; p < M.np; p++) out[p] = page[p] >= 0 ? 1 : 0;
     return FELT_OK;
 }
 
 API int felt_surface_partition_layer_size(felt_surface* s, uint64_t p, int layer, uint64_t* n) {
     CHECK_S(s);
     const Geo& G = s->G;
-    if (p >= (uint64_t)G.np || layer < -G.L || layer > G.L)
+    if (p >= (uint64_t)s->mems[0].np || layer < -G.L || layer > G.L)
         return felt_fail(FELT_E_ARG, "argument out of range");
     int c = 0;
     if (int rc = felt_download(s, &c, s->P.pcnt + p * G.NL + layer + G.L, sizeof(int))) return rc;
@@ -372,34 +462,9 @@ API int felt_surface_partition_layer_size(felt_surface* s, uint64_t p, int layer
     return FELT_OK;
 }
 
-// Surface::Stats (Surface.hpp:623-629, 1732-1742): the longest parent list of each grid.
 API int felt_surface_stats(felt_surface* s, felt_stats* st) {
     CHECK_S(s);
-    const Geo& G = s->G;
-    std::vector<int> cnt((size_t)G.np * G.NL);
-    std::vector<unsigned> dm(G.np);
-    if (int rc = felt_download(s, cnt.data(), s->P.pcnt, cnt.size() * sizeof(int))) return rc;
-    if (int rc = felt_download(s, dm.data(), s->dmask, dm.size() * sizeof(unsigned))) return rc;
-    if (int rc = felt_download(s, &s->hc, s->ctr, sizeof(Ctr))) return rc;
-    uint64_t best_i = 0, best_d = 0, band = 0;
-    for (int k = 0; k < G.NL; k++) {
-        uint64_t ni = 0, nd = 0;
-        for (int p = 0; p < G.np; p++) {
-            const int c = cnt[(size_t)p * G.NL + k];
-            if (c > 0) ni++;
-            band += (uint64_t)c;
-            if ((dm[p] >> k) & 1u) nd++;
-        }
-        best_i = std::max(best_i, ni);
-        best_d = std::max(best_d, nd);
-    }
-    st->active_isogrid_partitions = best_i;
-    st->active_delta_partitions = best_d;
-    st->band_voxels = band;
-    st->last_voxel_updates = s->last_updates;
-    st->last_rounds = s->last_rounds;
-    st->nan_events = (uint64_t)s->hc.nan_events;
-    return FELT_OK;
+    return stats_m(s, 0, st);
 }
 
 API int felt_surface_set_graph(felt_surface* s, int enable) {
@@ -499,7 +564,7 @@ API int felt_polys_invalidate(felt_polys* q) {
 
 API int felt_polys_mark(felt_polys* q, uint64_t p) {
     CHECK_P(q);
-    if (p >= (uint64_t)q->s->G.np) return felt_fail(FELT_E_ARG, "partition out of range");
+    if (p >= (uint64_t)q->s->mems[0].np) return felt_fail(FELT_E_ARG, "partition out of range");
     q->pending->track((int64_t)p);
     return FELT_OK;
 }
@@ -516,7 +581,7 @@ API int felt_polys_changes(felt_polys* q, uint64_t* out, size_t cap, size_t* n) 
 API int felt_polys_child_info(felt_polys* q, uint64_t p, int* active, uint64_t* nv,
                               uint64_t* ns) {
     CHECK_P(q);
-    if (p >= (uint64_t)q->s->G.np) return felt_fail(FELT_E_ARG, "partition out of range");
+    if (p >= (uint64_t)q->s->mems[0].np) return felt_fail(FELT_E_ARG, "partition out of range");
     const PolyKid& k = q->kids[p];
     *active = k.active ? 1 : 0;
     *nv = (uint64_t)k.nv;
@@ -527,7 +592,7 @@ API int felt_polys_child_info(felt_polys* q, uint64_t p, int* active, uint64_t* 
 API int felt_polys_child_mesh(felt_polys* q, uint64_t p, float* vtx, size_t vcap, uint32_t* spx,
                               size_t scap) {
     CHECK_P(q);
-    if (p >= (uint64_t)q->s->G.np) return felt_fail(FELT_E_ARG, "partition out of range");
+    if (p >= (uint64_t)q->s->mems[0].np) return felt_fail(FELT_E_ARG, "partition out of range");
     return felt_polys_child_mesh_impl(q, p, vtx, vcap, spx, scap);
 }
 
@@ -541,6 +606,211 @@ API int felt_polys_totals(felt_polys* q, uint64_t* nv, uint64_t* ns) {
     *nv = v;
     *ns = t;
     return FELT_OK;
+}
+
+// ---- batched surfaces -------------------------------------------------------------------
+
+struct felt_batch {
+    felt_surface* s = nullptr;
+};
+
+#define CHECK_B(b) \
+    if (!(b) || !(b)->s) return felt_fail(FELT_E_ARG, "null batch handle")
+#define CHECK_M(b, i) \
+    if ((i) < 0 || (i) >= (b)->s->G.nmem) return felt_fail(FELT_E_ARG, "member out of range")
+
+API int felt_batch_create(int D, int L, const int32_t* part, int32_t n, const int32_t* sizes,
+                          uint64_t max_bricks, felt_batch** out) {
+    if (!out) return felt_fail(FELT_E_ARG, "null output");
+    felt_batch* b = new (std::nothrow) felt_batch();
+    if (!b) return felt_fail(FELT_E_NOMEM, "host allocation failed");
+    if (int rc = make_pool(D, L, part, n, sizes, max_bricks, &b->s)) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return FELT_OK;
+}
+
+API int felt_batch_destroy(felt_batch* b) {
+    if (!b) return FELT_OK;
+    if (b->s) {
+        felt_surface_release(b->s);
+        delete b->s;
+    }
+    delete b;
+    return FELT_OK;
+}
+
+API int felt_batch_count(felt_batch* b, int32_t* n) {
+    CHECK_B(b);
+    *n = b->s->G.nmem;
+    return FELT_OK;
+}
+
+API int felt_batch_memory(felt_batch* b, uint64_t* bytes, uint64_t* used, uint64_t* total) {
+    CHECK_B(b);
+    if (int rc = felt_download(b->s, &b->s->hc, b->s->ctr, sizeof(Ctr))) return rc;
+    *bytes = felt_device_bytes(b->s);
+    *total = (uint64_t)b->s->cap;
+    *used = (uint64_t)(b->s->cap - b->s->hc.free_top);
+    return FELT_OK;
+}
+
+API int felt_batch_seed(felt_batch* b, int32_t i, const int32_t* pos) {
+    CHECK_B(b);
+    CHECK_M(b, i);
+    if (!pos) return felt_fail(FELT_E_ARG, "null position");
+    return seed_m(b->s, i, pos);
+}
+
+API int felt_batch_load_dense(felt_batch* b, int32_t i, const float* phi, size_t n) {
+    CHECK_B(b);
+    CHECK_M(b, i);
+    if (n != grid_total(b->s, i) || !phi) return felt_fail(FELT_E_ARG, "load_dense: size mismatch");
+    return felt_load_dense_impl(b->s, i, phi, n);
+}
+
+API int felt_batch_load_spheres(felt_batch* b, int32_t i, int32_t n, const double* centres,
+                                const double* radii) {
+    CHECK_B(b);
+    CHECK_M(b, i);
+    if (n < 1 || !centres || !radii) return felt_fail(FELT_E_ARG, "need at least one sphere");
+    const int D = b->s->G.D;
+    std::vector<double> c3((size_t)n * 3, 0.0);
+    for (int k = 0; k < n; k++)
+        for (int a = 0; a < D; a++) c3[(size_t)k * 3 + a] = centres[(size_t)k * D + a];
+    return felt_load_spheres_impl(b->s, i, n, c3.data(), radii);
+}
+
+API int felt_batch_update(felt_batch* b, const felt_speed* speeds) {
+    CHECK_B(b);
+    if (!speeds) return felt_fail(FELT_E_ARG, "null speeds");
+    for (int i = 0; i < b->s->G.nmem; i++)
+        if (int rc = check_speed(speeds + i)) return rc;
+    return felt_update_global(b->s, speeds);
+}
+
+API int felt_batch_update_boxes(felt_batch* b, int32_t n, const int32_t* ids, const int32_t* lo,
+                                const int32_t* hi, const felt_speed* speeds) {
+    CHECK_B(b);
+    if (n < 0 || (n > 0 && (!ids || !lo || !hi || !speeds)))
+        return felt_fail(FELT_E_ARG, "bad arguments");
+    felt_surface* s = b->s;
+    const int D = s->G.D;
+    std::vector<char> seen(s->G.nmem, 0);
+    std::vector<int32_t> l3((size_t)n * 3, 0), h3((size_t)n * 3, 0);
+    std::vector<int> id(n);
+    for (int j = 0; j < n; j++) {
+        if (ids[j] < 0 || ids[j] >= s->G.nmem) return felt_fail(FELT_E_ARG, "member out of range");
+        if (seen[ids[j]]) return felt_fail(FELT_E_ARG, "a member appears twice in one step");
+        seen[ids[j]] = 1;
+        if (int rc = check_speed(speeds + j)) return rc;
+        id[j] = ids[j];
+        for (int a = 0; a < D; a++) {
+            l3[(size_t)j * 3 + a] = lo[(size_t)j * D + a];
+            h3[(size_t)j * 3 + a] = hi[(size_t)j * D + a];
+        }
+    }
+    return felt_update_boxes_impl(s, n, id.data(), l3.data(), h3.data(), speeds);
+}
+
+API int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n) {
+    CHECK_B(b);
+    CHECK_M(b, i);
+    const size_t total = grid_total(b->s, i);
+    if (n < total) return felt_fail(FELT_E_CAPACITY, "snapshot buffer too small");
+    return felt_snapshot_impl(b->s, i, dst, total);
+}
+
+API int felt_batch_layer_size(felt_batch* b, int32_t i, int layer, uint64_t* n) {
+    CHECK_B(b);
+    CHECK_M(b, i);
+    return layer_size_m(b->s, i, layer, n);
+}
+
+API int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos, size_t cap,
+                                   size_t* n) {
+    CHECK_B(b);
+    CHECK_M(b, i);
+    return layer_positions_m(b->s, i, layer, pos, cap, n);
+}
+
+API int felt_batch_stats(felt_batch* b, int32_t i, felt_stats* st) {
+    CHECK_B(b);
+    CHECK_M(b, i);
+    return stats_m(b->s, i, st);
+}
+
+API int felt_batch_step_info(felt_batch* b, uint64_t* voxel_updates, uint64_t* rounds) {
+    CHECK_B(b);
+    *voxel_updates = b->s->last_updates;
+    *rounds = b->s->last_rounds;
+    return FELT_OK;
+}
+
+API int felt_batch_set_graph(felt_batch* b, int enable) {
+    CHECK_B(b);
+    b->s->use_graph = enable != 0;
+    return FELT_OK;
+}
+
+API int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launch,
+                             double* items_per_launch, int* launches) {
+    CHECK_B(b);
+    return felt_surface_time_pass(b->s, pass, reps, us_per_launch, items_per_launch, launches);
+}
+
+API int felt_batch_sync(felt_batch* b) {
+    CHECK_B(b);
+    return felt_sync_impl(b->s);
+}
+
+API int felt_batch_polys_create(felt_batch* b, felt_polys** out) {
+    CHECK_B(b);
+    return felt_polys_create(b->s, out);
+}
+
+// Pool partition of member i's partition pos_idx.
+static int pool_part(felt_polys* q, int32_t i, uint64_t pos_idx, uint64_t* p) {
+    if (i < 0 || i >= q->s->G.nmem) return felt_fail(FELT_E_ARG, "member out of range");
+    const Mem& M = q->s->mems[i];
+    if (pos_idx >= (uint64_t)M.np) return felt_fail(FELT_E_ARG, "partition out of range");
+    *p = (uint64_t)M.pbase + pos_idx;
+    return FELT_OK;
+}
+
+API int felt_batch_polys_changes(felt_polys* q, int32_t i, uint64_t* out, size_t cap, size_t* n) {
+    CHECK_P(q);
+    if (i < 0 || i >= q->s->G.nmem) return felt_fail(FELT_E_ARG, "member out of range");
+    const Mem& M = q->s->mems[i];
+    std::vector<uint64_t> mine;
+    for (int64_t p : q->done->list)
+        if (p >= M.pbase && p < (int64_t)M.pbase + M.np) mine.push_back((uint64_t)(p - M.pbase));
+    *n = mine.size();
+    if (cap < mine.size()) return felt_fail(FELT_E_CAPACITY, "buffer too small");
+    std::copy(mine.begin(), mine.end(), out);
+    return FELT_OK;
+}
+
+API int felt_batch_polys_child_info(felt_polys* q, int32_t i, uint64_t pos_idx, int* active,
+                                    uint64_t* nv, uint64_t* ns) {
+    CHECK_P(q);
+    uint64_t p;
+    if (int rc = pool_part(q, i, pos_idx, &p)) return rc;
+    const PolyKid& k = q->kids[p];
+    *active = k.active ? 1 : 0;
+    *nv = (uint64_t)k.nv;
+    *ns = (uint64_t)k.nt;
+    return FELT_OK;
+}
+
+API int felt_batch_polys_child_mesh(felt_polys* q, int32_t i, uint64_t pos_idx, float* vtx,
+                                    size_t vcap, uint32_t* spx, size_t scap) {
+    CHECK_P(q);
+    uint64_t p;
+    if (int rc = pool_part(q, i, pos_idx, &p)) return rc;
+    return felt_polys_child_mesh_impl(q, p, vtx, vcap, spx, scap);
 }
 
 API int felt_mc_table(uint64_t* out, size_t n) {

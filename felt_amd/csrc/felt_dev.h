@@ -1,19 +1,25 @@
 // felt_dev.h — device-side data layout and stencil helpers shared by the surface and polygon
 // kernels (gfx950).
 //
-// Layout in HBM (one surface):
+// One handle owns a pool of one or more member surfaces of the same dimension, layer count and
+// partition size (a single Felt::Surface is a pool of one).  Their partitions share one index
+// space: member m owns pool partitions [pbase_m, pbase_m + np_m), row-major over its children
+// grid.  Layout in HBM:
 //   page[np]        partition -> brick slot, or -1 when the partition is inactive
 //   pbg[np]         value every voxel of an inactive partition reads (the reference's child
 //                   background, GridMixin.hpp:587-621; PartitionedMixin.hpp:502-516)
 //   pcnt[np*NL]     voxels per partition per layer list; a partition is in the reference's
 //                   parent list k exactly when pcnt[p*NL+k] > 0 (PartitionedMixin.hpp:489-558)
+//   part_mem[np]    partition -> member surface
 //   slot_part[cap]  brick slot -> partition, -1 when free
+//   slot_org[cap]   brick slot -> grid coordinates of its partition's first voxel, and member
 //   phi[cap*pv]     signed distances, one brick of pv = prod(part) voxels per slot, row-major
 //                   inside the brick with the last axis fastest (Util.hpp:23-40)
 //   lay[cap*pv]     layer list of each voxel (-L..L) or NONE: replaces the reference's per-child
 //                   lookup grid and per-layer leaf lists (LookupMixin.hpp:225-387)
+//   mem[nmem]       member geometry
 // A voxel is named by v = slot*pv + leaf.  Grid coordinates g are world coordinates minus the
-// grid offset, so 0 <= g < size; partition coordinates are g / part.
+// member's grid offset, so 0 <= g < size; partition coordinates are g / part.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,14 +54,27 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
     return (t + ((n - t) >> 1)) >> (f.s - 1);
 }
 
+// Geometry shared by every member of a pool.
 struct Geo {
     int D, L, NL;
-    int size[3], part[3], csize[3], offset[3];
+    int part[3];
     int lstr[3];  // leaf strides inside a brick
-    int cstr[3];  // partition strides in the children grid
     int pv;
-    int np;
-    FastDiv fpart[3], fcsize[3], fpv;
+    int np;       // partitions of all members
+    int nmem;     // member surfaces
+    FastDiv fpart[3], fpv;
+};
+
+// One member surface: its grid and children grid (Surface.hpp:163-173; PartitionedMixin.hpp:
+// 131-141).  The first two 16-byte rows are read per voxel by the stencil kernels.
+struct alignas(16) Mem {
+    int size[3];
+    int pbase;      // its first pool partition
+    int cstr[3];    // partition strides in its children grid
+    int np;         // its partitions
+    int csize[3];
+    int offset[3];  // world position of grid position 0: -size/2 (Surface.hpp:1716-1719)
+    FastDiv fcsize[3];
 };
 
 struct Pool {
@@ -65,7 +84,27 @@ struct Pool {
     int* slot_part;
     float* phi;
     int8_t* lay;
+    int4* slot_org;
+    const Mem* mem;
+    const int* part_mem;
 };
+
+// A voxel of the band being processed: id, member, coordinates in its brick and in its grid,
+// and its grid's size.
+struct Vx {
+    uint32_t v;
+    int m;
+    int loc[3];
+    int g[3];
+    int size[3];
+};
+
+__device__ __forceinline__ int4 mem_row0(const Pool& P, int m) {
+    return *reinterpret_cast<const int4*>(&P.mem[m].size[0]);
+}
+__device__ __forceinline__ int4 mem_row1(const Pool& P, int m) {
+    return *reinterpret_cast<const int4*>(&P.mem[m].cstr[0]);
+}
 
 // iround(v + FLT_EPSILON), exact round-half-away-from-zero (Surface.hpp:1694-1697).  NaN has
 // no reference value; it maps outside the band and is counted.
@@ -81,134 +120,130 @@ __device__ __forceinline__ int layer_of(float v, int L, int* nan_ctr) {
 __device__ __forceinline__ bool in_band(int k, int L) { return k >= -L && k <= L; }
 
 template <int D>
-__device__ __forceinline__ void decode(const Geo& G, const Pool& P, uint32_t v, int& slot,
-                                       int loc[3], int g[3]) {
-    slot = (int)fdiv(v, G.fpv);
-    uint32_t l = v - (uint32_t)slot * (uint32_t)G.pv;
-    uint32_t p = (uint32_t)P.slot_part[slot];
+__device__ __forceinline__ void leaf_coords(const Geo& G, uint32_t l, int lc[3]) {
 #pragma unroll
     for (int a = D - 1; a >= 0; a--) {
-        const uint32_t pq = fdiv(p, G.fcsize[a]);
-        const int pc = (int)(p - pq * (uint32_t)G.csize[a]);
-        p = pq;
-        const uint32_t lq = fdiv(l, G.fpart[a]);
-        loc[a] = (int)(l - lq * (uint32_t)G.part[a]);
-        l = lq;
-        g[a] = pc * G.part[a] + loc[a];
+        const uint32_t q = fdiv(l, G.fpart[a]);
+        lc[a] = (int)(l - q * (uint32_t)G.part[a]);
+        l = q;
     }
 }
 
-// Partition coordinates of partition index p.
 template <int D>
-__device__ __forceinline__ void part_coords(const Geo& G, int p, int pc[3]) {
+__device__ __forceinline__ void decode(const Geo& G, const Pool& P, uint32_t v, Vx& x) {
+    const uint32_t slot = fdiv(v, G.fpv);
+    x.v = v;
+#pragma unroll
+    for (int a = D; a < 3; a++) {
+        x.loc[a] = 0;
+        x.g[a] = 0;
+        x.size[a] = 1;
+    }
+    leaf_coords<D>(G, v - slot * (uint32_t)G.pv, x.loc);
+    const int4 o = P.slot_org[slot];
+    x.m = o.w;
+    const int4 r0 = mem_row0(P, o.w);
+    x.g[0] = o.x + x.loc[0];
+    x.size[0] = r0.x;
+    if (D > 1) {
+        x.g[1] = o.y + x.loc[1];
+        x.size[1] = r0.y;
+    }
+    if (D > 2) {
+        x.g[2] = o.z + x.loc[2];
+        x.size[2] = r0.z;
+    }
+}
+
+// Member-local partition coordinates of local partition index p.
+template <int D>
+__device__ __forceinline__ void part_coords(const Mem& M, int p, int pc[3]) {
     uint32_t r = (uint32_t)p;
 #pragma unroll
     for (int a = D - 1; a >= 0; a--) {
-        const uint32_t q = fdiv(r, G.fcsize[a]);
-        pc[a] = (int)(r - q * (uint32_t)G.csize[a]);
-        r = q;
-    }
-}
-
-// Brick coordinates of leaf index l.
-template <int D>
-__device__ __forceinline__ void leaf_coords(const Geo& G, int l, int lc[3]) {
-    uint32_t r = (uint32_t)l;
-#pragma unroll
-    for (int a = D - 1; a >= 0; a--) {
-        const uint32_t q = fdiv(r, G.fpart[a]);
-        lc[a] = (int)(r - q * (uint32_t)G.part[a]);
+        const uint32_t q = fdiv(r, M.fcsize[a]);
+        pc[a] = (int)(r - q * (uint32_t)M.csize[a]);
         r = q;
     }
 }
 
 template <int D>
-__device__ __forceinline__ bool inside(const Geo& G, const int q[3]) {
+__device__ __forceinline__ bool inside(const int size[3], const int q[3]) {
 #pragma unroll
     for (int a = 0; a < D; a++)
-        if (q[a] < 0 || q[a] >= G.size[a]) return false;
+        if (q[a] < 0 || q[a] >= size[a]) return false;
     return true;
 }
 
+// Pool partition and leaf of an in-grid position of member m.
 template <int D>
-__device__ __forceinline__ int part_of(const Geo& G, const int q[3]) {
-    int p = 0;
-#pragma unroll
-    for (int a = 0; a < D; a++) p += (int)fdiv((uint32_t)q[a], G.fpart[a]) * G.cstr[a];
-    return p;
-}
-
-template <int D>
-__device__ __forceinline__ int leaf_of(const Geo& G, const int q[3]) {
-    int l = 0;
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        const int qa = (int)fdiv((uint32_t)q[a], G.fpart[a]);
-        l += (q[a] - qa * G.part[a]) * G.lstr[a];
-    }
-    return l;
-}
-
-// Partition and leaf of an in-grid position together.
-template <int D>
-__device__ __forceinline__ void locate(const Geo& G, const int q[3], int& p, int& l) {
-    p = 0;
+__device__ __forceinline__ void locate(const Geo& G, const Pool& P, int m, const int q[3], int& p,
+                                       int& l) {
+    const int4 r0 = mem_row0(P, m), r1 = mem_row1(P, m);
+    const int cs[3] = {r1.x, r1.y, r1.z};
+    p = r0.w;
     l = 0;
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const int qa = (int)fdiv((uint32_t)q[a], G.fpart[a]);
-        p += qa * G.cstr[a];
+        p += qa * cs[a];
         l += (q[a] - qa * G.part[a]) * G.lstr[a];
     }
 }
 
-// get(pos) of the partitioned isogrid (PartitionedMixin.hpp:594-601): L+1 outside the grid,
-// the partition background when inactive.
 template <int D>
-__device__ __forceinline__ float get_g(const Geo& G, const Pool& P, const int q[3]) {
-    if (!inside<D>(G, q)) return (float)(G.L + 1);
+__device__ __forceinline__ int part_of(const Geo& G, const Pool& P, int m, const int q[3]) {
     int p, l;
-    locate<D>(G, q, p, l);
+    locate<D>(G, P, m, q, p, l);
+    return p;
+}
+
+// get(pos) of member m's partitioned isogrid (PartitionedMixin.hpp:594-601): L+1 outside the
+// grid, the partition background when inactive.
+template <int D>
+__device__ __forceinline__ float get_g(const Geo& G, const Pool& P, int m, const int size[3],
+                                       const int q[3]) {
+    if (!inside<D>(size, q)) return (float)(G.L + 1);
+    int p, l;
+    locate<D>(G, P, m, q, p, l);
     const int s = P.page[p];
     if (s < 0) return P.pbg[p];
     return P.phi[(size_t)s * G.pv + l];
 }
 
-// Value at g + d (d components in {-1,0,1}); reads the voxel's own brick directly when the
+// Value at x.g + d (d components in {-1,0,1}); reads the voxel's own brick directly when the
 // neighbour stays inside it and inside the grid.
 template <int D>
-__device__ __forceinline__ float nb(const Geo& G, const Pool& P, uint32_t v, const int loc[3],
-                                    const int g[3], int d0, int d1, int d2) {
+__device__ __forceinline__ float nb(const Geo& G, const Pool& P, const Vx& x, int d0, int d1,
+                                    int d2) {
     const int d[3] = {d0, d1, d2};
     bool same = true;
     int off = 0;
     int q[3];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        const int la = loc[a] + d[a];
-        q[a] = g[a] + d[a];
-        if (la < 0 || la >= G.part[a] || q[a] >= G.size[a]) same = false;
+        const int la = x.loc[a] + d[a];
+        q[a] = x.g[a] + d[a];
+        if (la < 0 || la >= G.part[a] || q[a] >= x.size[a]) same = false;
         off += d[a] * G.lstr[a];
     }
-    if (same) return P.phi[(int64_t)v + off];
-    return get_g<D>(G, P, q);
+    if (same) return P.phi[(int64_t)x.v + off];
+    return get_g<D>(G, P, x.m, x.size, q);
 }
 
 // distance(pos, dist, dir) (Surface.hpp:1387-1440): signed minimum over in-grid face
 // neighbours in the direction of the zero curve, plus one step.
 template <int D>
-__device__ __forceinline__ float distance(const Geo& G, const Pool& P, uint32_t v,
-                                          const int loc[3], const int g[3], float self,
+__device__ __forceinline__ float distance(const Geo& G, const Pool& P, const Vx& x, float self,
                                           float dir) {
     float dist = self * dir;
 #pragma unroll
     for (int a = 0; a < D; a++) {
 #pragma unroll
         for (int s = -1; s <= 1; s += 2) {
-            const int qa = g[a] + s;
-            if (qa < 0 || qa >= G.size[a]) continue;
-            const float dn = nb<D>(G, P, v, loc, g, a == 0 ? s : 0, a == 1 ? s : 0,
-                                   a == 2 ? s : 0);
+            const int qa = x.g[a] + s;
+            if (qa < 0 || qa >= x.size[a]) continue;
+            const float dn = nb<D>(G, P, x, a == 0 ? s : 0, a == 1 ? s : 0, a == 2 ? s : 0);
             if (dn * dir < dist) dist = dn * dir;
         }
     }
@@ -220,15 +255,14 @@ __device__ __forceinline__ float distance(const Geo& G, const Pool& P, uint32_t 
 // the reference with no contraction: forward normals, backward normals, then
 // (dn0 + (dn1 + dn2)) / 2 in 3D or (dn0 + dn1) / 2 in 2D.
 template <int D>
-__device__ __forceinline__ float curvature(const Geo& G, const Pool& P, uint32_t v,
-                                           const int loc[3], const int g[3]) {
-    const float c = P.phi[v];
+__device__ __forceinline__ float curvature(const Geo& G, const Pool& P, const Vx& x) {
+    const float c = P.phi[x.v];
     float nf[3], nbk[3];
 #pragma unroll
     for (int a = 0; a < D; a++) {
         int e[3] = {0, 0, 0};
         e[a] = 1;
-        const float va = nb<D>(G, P, v, loc, g, e[0], e[1], e[2]) - c;
+        const float va = nb<D>(G, P, x, e[0], e[1], e[2]) - c;
         float sq = 0.0f;
 #pragma unroll
         for (int b = 0; b < D; b++) {
@@ -236,8 +270,8 @@ __device__ __forceinline__ float curvature(const Geo& G, const Pool& P, uint32_t
             int lo[3] = {e[0], e[1], e[2]}, hi[3] = {e[0], e[1], e[2]};
             lo[b] -= 1;
             hi[b] += 1;
-            const float vl = nb<D>(G, P, v, loc, g, lo[0], lo[1], lo[2]);
-            const float vh = nb<D>(G, P, v, loc, g, hi[0], hi[1], hi[2]);
+            const float vl = nb<D>(G, P, x, lo[0], lo[1], lo[2]);
+            const float vh = nb<D>(G, P, x, hi[0], hi[1], hi[2]);
             const float t = (vh - vl) / 2.0f;
             sq += t * t;
         }
@@ -247,7 +281,7 @@ __device__ __forceinline__ float curvature(const Geo& G, const Pool& P, uint32_t
     for (int a = 0; a < D; a++) {
         int e[3] = {0, 0, 0};
         e[a] = -1;
-        const float va = c - nb<D>(G, P, v, loc, g, e[0], e[1], e[2]);
+        const float va = c - nb<D>(G, P, x, e[0], e[1], e[2]);
         float sq = 0.0f;
 #pragma unroll
         for (int b = 0; b < D; b++) {
@@ -255,8 +289,8 @@ __device__ __forceinline__ float curvature(const Geo& G, const Pool& P, uint32_t
             int lo[3] = {e[0], e[1], e[2]}, hi[3] = {e[0], e[1], e[2]};
             lo[b] -= 1;
             hi[b] += 1;
-            const float vl = nb<D>(G, P, v, loc, g, lo[0], lo[1], lo[2]);
-            const float vh = nb<D>(G, P, v, loc, g, hi[0], hi[1], hi[2]);
+            const float vl = nb<D>(G, P, x, lo[0], lo[1], lo[2]);
+            const float vh = nb<D>(G, P, x, hi[0], hi[1], hi[2]);
             const float t = (vh - vl) / 2.0f;
             sq += t * t;
         }

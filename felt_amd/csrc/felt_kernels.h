@@ -93,22 +93,24 @@ __device__ __forceinline__ float value_noise(const int w[3], unsigned long long 
     return val[0];
 }
 
+// The speed functor of member x.m at voxel x; `offset` is the member's grid offset (noise is a
+// function of world position).
 template <int D>
 __device__ __forceinline__ float eval_speed(const Geo& G, const Pool& P, const Speed& sp,
-                                            uint32_t v, const int loc[3], const int g[3]) {
+                                            const Vx& x, const int offset[3]) {
     switch (sp.kind) {
     case FELT_SPEED_CONSTANT:
         return sp.a;
     case FELT_SPEED_CURVATURE:
-        return sp.b * curvature<D>(G, P, v, loc, g);
+        return sp.b * curvature<D>(G, P, x);
     case FELT_SPEED_CONST_CURV:
-        return sp.a + sp.b * curvature<D>(G, P, v, loc, g);
+        return sp.a + sp.b * curvature<D>(G, P, x);
     default: {
         int w[3];
 #pragma unroll
-        for (int a = 0; a < D; a++) w[a] = g[a] + G.offset[a];
+        for (int a = 0; a < D; a++) w[a] = x.g[a] + offset[a];
         const float n = value_noise<D>(w, sp.seed, sp.cell);
-        float r = sp.a * (1.0f + sp.c * n) + sp.b * curvature<D>(G, P, v, loc, g);
+        float r = sp.a * (1.0f + sp.c * n) + sp.b * curvature<D>(G, P, x);
         if (r < -1.0f) r = -1.0f;
         else if (r > 1.0f) r = 1.0f;
         return r;

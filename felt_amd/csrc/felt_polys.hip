@@ -81,25 +81,39 @@ struct McT<2> {
     }
 };
 
+// The member a marched voxel belongs to: its grid size and world offset.
+struct MemV {
+    int m;
+    int size[3];
+    int offset[3];
+};
+
+__device__ __forceinline__ MemV mem_view(const Pool& P, int m) {
+    const Mem& M = P.mem[m];
+    return MemV{m, {M.size[0], M.size[1], M.size[2]}, {M.offset[0], M.offset[1], M.offset[2]}};
+}
+
 // Corner mask of the cube based at grid position g (bit i: phi(corner i) > 0, Poly.hpp:356-368).
 template <int D>
-__device__ unsigned cube_mask(const Geo& G, const Pool& P, const int g[3]) {
+__device__ unsigned cube_mask(const Geo& G, const Pool& P, const MemV& V, const int g[3]) {
     unsigned m = 0;
     for (int c = 0; c < McT<D>::ncorner; c++) {
         int q[3] = {0, 0, 0};
         for (int a = 0; a < D; a++) q[a] = g[a] + McT<D>::calc_off(a) + McT<D>::corner(c, a);
-        m |= (unsigned)(get_g<D>(G, P, q) > 0.0f) << c;
+        m |= (unsigned)(get_g<D>(G, P, V.m, V.size, q) > 0.0f) << c;
     }
     return m;
 }
 
-// Leaf index (within partition p's brick) of the band cube at grid base position b, or -1.
+// Leaf index (within the brick in slot s, whose first voxel is at grid position org) of the
+// band cube at grid base position b, or -1.
 template <int D>
-__device__ int band_cube(const Geo& G, const Pool& P, int s, const int pc[3], const int b[3]) {
+__device__ int band_cube(const Geo& G, const Pool& P, const MemV& V, int s, const int org[3],
+                         const int b[3]) {
     int l = 0;
     for (int a = 0; a < D; a++) {
-        const int la = b[a] - pc[a] * G.part[a];
-        if (la < 0 || la >= G.part[a] || b[a] >= G.size[a]) return -1;
+        const int la = b[a] - org[a];
+        if (la < 0 || la >= G.part[a] || b[a] >= V.size[a]) return -1;
         l += la * G.lstr[a];
     }
     return P.lay[(size_t)s * G.pv + l] != NONE ? l : -1;
@@ -108,8 +122,8 @@ __device__ int band_cube(const Geo& G, const Pool& P, int s, const int pc[3], co
 // Owner of the vertex on edge e of the cube at base b: the lowest-leaf band cube of the
 // partition among those sharing the edge.  Returns its leaf and the edge's index in its frame.
 template <int D>
-__device__ int edge_owner(const Geo& G, const Pool& P, int s, const int pc[3], const int b[3],
-                          int e, int* e_owner) {
+__device__ int edge_owner(const Geo& G, const Pool& P, const MemV& V, int s, const int org[3],
+                          const int b[3], int e, int* e_owner) {
     int pa[3] = {0, 0, 0};
     for (int a = 0; a < D; a++) pa[a] = b[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
     const int axis = McT<D>::edge(e, D);
@@ -118,7 +132,7 @@ __device__ int edge_owner(const Geo& G, const Pool& P, int s, const int pc[3], c
         const int f = McT<D>::axis_edge(axis, i);
         int cb[3] = {0, 0, 0};
         for (int a = 0; a < D; a++) cb[a] = pa[a] - McT<D>::edge(f, a) - McT<D>::calc_off(a);
-        const int l = band_cube<D>(G, P, s, pc, cb);
+        const int l = band_cube<D>(G, P, V, s, org, cb);
         if (l >= 0 && (best < 0 || l < best)) {
             best = l;
             best_e = f;
@@ -130,7 +144,7 @@ __device__ int edge_owner(const Geo& G, const Pool& P, int s, const int pc[3], c
 
 // Felt's N-linear interpolation in world coordinates (NumericMixin.hpp:403-481).
 template <int D>
-__device__ float interp_w(const Geo& G, const Pool& P, const float w[3]) {
+__device__ float interp_w(const Geo& G, const Pool& P, const MemV& V, const float w[3]) {
     int fl[3] = {0, 0, 0};
     float t[3] = {0.f, 0.f, 0.f};
     for (int a = 0; a < D; a++) {
@@ -141,8 +155,8 @@ __device__ float interp_w(const Geo& G, const Pool& P, const float w[3]) {
     float val[8];
     for (int i = 0; i < (1 << D); i++) {
         int q[3] = {0, 0, 0};
-        for (int a = 0; a < D; a++) q[a] = fl[a] + ((i >> a) & 1) - G.offset[a];
-        val[i] = get_g<D>(G, P, q);
+        for (int a = 0; a < D; a++) q[a] = fl[a] + ((i >> a) & 1) - V.offset[a];
+        val[i] = get_g<D>(G, P, V.m, V.size, q);
     }
     int n = 1 << D;
     for (int axis = 0; axis < D; axis++) {
@@ -154,30 +168,31 @@ __device__ float interp_w(const Geo& G, const Pool& P, const float w[3]) {
 }
 
 template <int D>
-__device__ bool inside_wf(const Geo& G, const float w[3]) {
+__device__ bool inside_wf(const MemV& V, const float w[3]) {
     for (int a = 0; a < D; a++) {
-        if (w[a] >= (float)(G.offset[a] + G.size[a])) return false;
-        if (w[a] < (float)G.offset[a]) return false;
+        if (w[a] >= (float)(V.offset[a] + V.size[a])) return false;
+        if (w[a] < (float)V.offset[a]) return false;
     }
     return true;
 }
 
 // Safe gradient at a real world position (NumericMixin.hpp:188-231), probe stepped -1,+2,-1.
 template <int D>
-__device__ void grad_f(const Geo& G, const Pool& P, const float w[3], float out[3]) {
+__device__ void grad_f(const Geo& G, const Pool& P, const MemV& V, const float w[3],
+                       float out[3]) {
     float pt[3] = {w[0], w[1], w[2]};
-    const float centre = interp_w<D>(G, P, w);
+    const float centre = interp_w<D>(G, P, V, w);
     for (int a = 0; a < D; a++) {
         float back = centre, fwd = centre;
         int order = 0;
         pt[a] -= 1.0f;
-        if (inside_wf<D>(G, pt)) {
-            back = interp_w<D>(G, P, pt);
+        if (inside_wf<D>(V, pt)) {
+            back = interp_w<D>(G, P, V, pt);
             order++;
         }
         pt[a] += 2.0f;
-        if (inside_wf<D>(G, pt)) {
-            fwd = interp_w<D>(G, P, pt);
+        if (inside_wf<D>(V, pt)) {
+            fwd = interp_w<D>(G, P, V, pt);
             order++;
         }
         pt[a] -= 1.0f;
@@ -188,20 +203,20 @@ __device__ void grad_f(const Geo& G, const Pool& P, const float w[3], float out[
 
 // Safe gradient at an integer grid position.
 template <int D>
-__device__ void grad_i(const Geo& G, const Pool& P, const int g[3], float out[3]) {
+__device__ void grad_i(const Geo& G, const Pool& P, const MemV& V, const int g[3], float out[3]) {
     int pt[3] = {g[0], g[1], g[2]};
-    const float centre = get_g<D>(G, P, g);
+    const float centre = get_g<D>(G, P, V.m, V.size, g);
     for (int a = 0; a < D; a++) {
         float back = centre, fwd = centre;
         int order = 0;
         pt[a] -= 1;
-        if (inside<D>(G, pt)) {
-            back = get_g<D>(G, P, pt);
+        if (inside<D>(V.size, pt)) {
+            back = get_g<D>(G, P, V.m, V.size, pt);
             order++;
         }
         pt[a] += 2;
-        if (inside<D>(G, pt)) {
-            fwd = get_g<D>(G, P, pt);
+        if (inside<D>(V.size, pt)) {
+            fwd = get_g<D>(G, P, V.m, V.size, pt);
             order++;
         }
         pt[a] -= 1;
@@ -222,24 +237,25 @@ __device__ void normalize3(float n[3]) {
 
 // Vertex on the edge from grid position ga along axis (Poly.hpp:296-346; PolyMixin.hpp:142-148).
 template <int D>
-__device__ void make_vertex(const Geo& G, const Pool& P, const int ga[3], int axis, float* out) {
+__device__ void make_vertex(const Geo& G, const Pool& P, const MemV& V, const int ga[3], int axis,
+                            float* out) {
     int gb[3] = {ga[0], ga[1], ga[2]};
     gb[axis] += 1;
-    const float va = get_g<D>(G, P, ga), vb = get_g<D>(G, P, gb);
+    const float va = get_g<D>(G, P, V.m, V.size, ga), vb = get_g<D>(G, P, V.m, V.size, gb);
     float pos[3] = {0.f, 0.f, 0.f}, n[3] = {0.f, 0.f, 0.f};
     if (fabsf(va) <= kEps || fabsf(vb) <= kEps) {
         const int* e = fabsf(va) <= kEps ? ga : gb;
-        for (int a = 0; a < D; a++) pos[a] = (float)(e[a] + G.offset[a]);
-        if (D == 3) grad_i<D>(G, P, e, n);
+        for (int a = 0; a < D; a++) pos[a] = (float)(e[a] + V.offset[a]);
+        if (D == 3) grad_i<D>(G, P, V, e, n);
     } else {
         float mu;
         if (fabsf(va - vb) <= kEps) mu = 0.5f;
         else mu = va / (va - vb);
         for (int a = 0; a < D; a++) {
-            const float fa = (float)(ga[a] + G.offset[a]), fb = (float)(gb[a] + G.offset[a]);
+            const float fa = (float)(ga[a] + V.offset[a]), fb = (float)(gb[a] + V.offset[a]);
             pos[a] = fa + (fb - fa) * mu;
         }
-        if (D == 3) grad_f<D>(G, P, pos, n);
+        if (D == 3) grad_f<D>(G, P, V, pos, n);
     }
     if (D == 3) {
         normalize3(n);
@@ -345,17 +361,19 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
     const int j = item_j[i], l = item_l[i];
     const int p = parts[j];
     const int s = P.page[p];
-    int pc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
-    part_coords<D>(G, p, pc);
-    leaf_coords<D>(G, l, g);
-    for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
-    const unsigned m = cube_mask<D>(G, P, g);
+    const int4 o = P.slot_org[s];
+    const MemV V = mem_view(P, o.w);
+    const int org[3] = {o.x, o.y, o.z};
+    int g[3] = {0, 0, 0};
+    leaf_coords<D>(G, (uint32_t)l, g);
+    for (int a = 0; a < D; a++) g[a] += org[a];
+    const unsigned m = cube_mask<D>(G, P, V, g);
     const unsigned em = McT<D>::edges_of(m);
     unsigned om = 0;
     for (int e = 0; e < McT<D>::nedge; e++) {
         if (!((em >> e) & 1)) continue;
         int eo;
-        if (edge_owner<D>(G, P, s, pc, g, e, &eo) == l) om |= 1u << e;
+        if (edge_owner<D>(G, P, V, s, org, g, e, &eo) == l) om |= 1u << e;
     }
     cmask[(size_t)j * G.pv + l] = (uint8_t)m;
     own[(size_t)j * G.pv + l] = (uint16_t)om;
@@ -465,10 +483,12 @@ __global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, con
     if (ns == 0) return;
     const int p = parts[j];
     const int s = P.page[p];
-    int pc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
-    part_coords<D>(G, p, pc);
-    leaf_coords<D>(G, l, g);
-    for (int a = 0; a < D; a++) g[a] += pc[a] * G.part[a];
+    const int4 o = P.slot_org[s];
+    const MemV V = mem_view(P, o.w);
+    const int org[3] = {o.x, o.y, o.z};
+    int g[3] = {0, 0, 0};
+    leaf_coords<D>(G, (uint32_t)l, g);
+    for (int a = 0; a < D; a++) g[a] += org[a];
     const int vw = D == 3 ? 6 : 2;
     const unsigned om = own[jb + l];
     int rank = 0;
@@ -476,7 +496,7 @@ __global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, con
         if (!((om >> e) & 1)) continue;
         int ga[3] = {0, 0, 0};
         for (int a = 0; a < D; a++) ga[a] = g[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
-        make_vertex<D>(G, P, ga, McT<D>::edge(e, D), vout + (size_t)(vb[i] + rank) * vw);
+        make_vertex<D>(G, P, V, ga, McT<D>::edge(e, D), vout + (size_t)(vb[i] + rank) * vw);
         rank++;
     }
     const int base = vst[j];
@@ -484,8 +504,8 @@ __global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, con
         for (int k = 0; k < D; k++) {
             const int e = McT<D>::simplex_edge(m, t, k);
             int eo;
-            const int o = edge_owner<D>(G, P, s, pc, g, e, &eo);
-            const int idx = vleaf[jb + o] + __popc((unsigned)own[jb + o] & ((1u << eo) - 1u)) - base;
+            const int ow = edge_owner<D>(G, P, V, s, org, g, e, &eo);
+            const int idx = vleaf[jb + ow] + __popc((unsigned)own[jb + ow] & ((1u << eo) - 1u)) - base;
             tout[(size_t)(tb[i] + t) * D + k] = (uint32_t)idx;
         }
     }

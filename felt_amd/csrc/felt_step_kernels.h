@@ -201,12 +201,13 @@ __device__ __forceinline__ int8_t clamp_k(int k) { return (int8_t)(k < -100 ? -1
 template <int D>
 __global__ void k_speed(Geo G, Pool P, const uint32_t* list, const int* rng, const Speed* sp,
                         float* out) {
-    const Speed s = *sp;
     FELT_SPAN_LOOP(i, rng) {
-        const uint32_t v = list[i];
-        int slot, loc[3], g[3];
-        decode<D>(G, P, v, slot, loc, g);
-        out[i] = eval_speed<D>(G, P, s, v, loc, g);
+        Vx x;
+        decode<D>(G, P, list[i], x);
+        const Speed s = sp[x.m];
+        const Mem& M = P.mem[x.m];
+        const int off[3] = {M.offset[0], M.offset[1], M.offset[2]};
+        out[i] = eval_speed<D>(G, P, s, x, off);
     }
 }
 
@@ -231,10 +232,9 @@ __device__ __forceinline__ void dist1_body(const Geo& G, const Pool& P, const ui
     const float side = layer < 0 ? -1.0f : 1.0f;
     FELT_SPAN_LOOP(i, rng) {
         if (Tagged && tag[i] != layer) continue;
-        const uint32_t v = list[i];
-        int slot, loc[3], g[3];
-        decode<D>(G, P, v, slot, loc, g);
-        out[i] = distance<D>(G, P, v, loc, g, P.phi[v], side);
+        Vx x;
+        decode<D>(G, P, list[i], x);
+        out[i] = distance<D>(G, P, x, P.phi[x.v], side);
     }
 }
 
@@ -495,20 +495,20 @@ __global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag,
     if (!*live) return;
     FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
         bool cand = false;
-        int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        Vx x;
         if (valid && expanding(G, sc.from[i], sc.to[i])) {
             cand = true;
-            decode<D>(G, P, sc.v[i], slot, loc, g);
+            decode<D>(G, P, sc.v[i], x);
         }
         for (int a = 0; a < D; a++) {
             for (int s = -1; s <= 1; s += 2) {
                 bool q_new = false;
                 int pq = 0;
                 if (cand) {
-                    int q[3] = {g[0], g[1], g[2]};
+                    int q[3] = {x.g[0], x.g[1], x.g[2]};
                     q[a] += s;
-                    if (inside<D>(G, q)) {
-                        pq = part_of<D>(G, q);
+                    if (inside<D>(x.size, q)) {
+                        pq = part_of<D>(G, P, x.m, q);
                         if (P.page[pq] < 0 &&
                             !in_band(layer_of(P.pbg[pq], G.L, &ctr->nan_events), G.L))
                             q_new = atomicExch(&nflag[pq], 1) == 0;
@@ -542,6 +542,12 @@ __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* n
             if (s >= 0) {
                 P.page[p] = s;
                 P.slot_part[s] = p;
+                const int m = P.part_mem[p];
+                const Mem& M = P.mem[m];
+                int pc[3] = {0, 0, 0};
+                part_coords<3>(M, p - M.pbase, pc);
+                P.slot_org[s] = make_int4(pc[0] * G.part[0], pc[1] * G.part[1],
+                                          pc[2] * G.part[2], m);
             }
             nflag[p] = 0;
         }
@@ -568,22 +574,23 @@ __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, 
     if (!*live) return;
     FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
         bool cand = false;
-        int side = 1, slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+        int side = 1;
+        Vx x;
         if (valid && expanding(G, sc.from[i], sc.to[i])) {
             cand = true;
             side = sc.from[i] < 0 ? -1 : 1;
-            decode<D>(G, P, sc.v[i], slot, loc, g);
+            decode<D>(G, P, sc.v[i], x);
         }
         const int8_t target = (int8_t)(G.L * side);
         for (int a = 0; a < D; a++) {
             for (int s = -1; s <= 1; s += 2) {
                 int claimed_p = -1;
                 if (cand) {
-                    int q[3] = {g[0], g[1], g[2]};
+                    int q[3] = {x.g[0], x.g[1], x.g[2]};
                     q[a] += s;
-                    if (inside<D>(G, q)) {
+                    if (inside<D>(x.size, q)) {
                         int pq, lq;
-                        locate<D>(G, q, pq, lq);
+                        locate<D>(G, P, x.m, q, pq, lq);
                         const int sq = P.page[pq];
                         if (sq >= 0) {  // else activation failed (pool exhausted, flagged)
                             const uint32_t vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
@@ -591,9 +598,16 @@ __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, 
                                                                __HIP_MEMORY_SCOPE_AGENT);
                             if (!in_band(layer_of(dq, G.L, &ctr->nan_events), G.L) &&
                                 cas_byte(&P.lay[vq], NONE, target)) {
-                                int lc[3];
-                                leaf_coords<D>(G, lq, lc);
-                                const float nd = distance<D>(G, P, vq, lc, q, dq, (float)side);
+                                Vx y;
+                                y.v = vq;
+                                y.m = x.m;
+                                for (int b = 0; b < 3; b++) {
+                                    y.loc[b] = 0;
+                                    y.g[b] = q[b];
+                                    y.size[b] = x.size[b];
+                                }
+                                leaf_coords<D>(G, (uint32_t)lq, y.loc);
+                                const float nd = distance<D>(G, P, y, dq, (float)side);
                                 __hip_atomic_store(&P.phi[vq], nd, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
                                 claimed_p = pq;

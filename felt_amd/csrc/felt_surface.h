@@ -1,4 +1,5 @@
-// felt_surface.h — host-side state of one device-resident surface.
+// felt_surface.h — host-side state of one device-resident pool of member surfaces (a single
+// surface is a pool of one member).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,6 +45,7 @@ struct Prof {
 struct felt_surface {
     felt::Geo G{};
     felt::Pool P{};
+    std::vector<felt::Mem> mems;     // member geometry (host copy of P.mem)
     hipStream_t st = nullptr;
     int cap = 0;  // brick pool slots
     int* free_stack = nullptr;
@@ -80,9 +82,9 @@ struct felt_surface {
     DBuf<float> tmp_f;
     DBuf<uint32_t> tmp_u;
     int* bbase = nullptr;           // band segment starts, NL+1 entries (device)
-    felt::Speed* d_speed = nullptr;  // speed descriptor read by the speed kernel (device)
-    felt::Speed* h_speed = nullptr;  // pinned staging copy
-    bool speed_valid = false;        // d_speed holds the last global update's functor
+    felt::Speed* d_speed = nullptr;  // per-member speed descriptors read by the speed kernel
+    felt::Speed* h_speed = nullptr;  // pinned staging copy (nmem entries)
+    bool speed_valid = false;        // d_speed holds the last global update's functors
     size_t list_cap = 0;             // capacity of every per-step list
     std::vector<int> hbase;          // host copy of bbase after a profiled step
     bool band_known = false;         // band_now is the current tracked-voxel count
@@ -111,21 +113,33 @@ int felt_surface_init(felt_surface* s);
 void felt_surface_release(felt_surface* s);
 felt::Lu felt_lu(felt_surface* s, int i);
 int felt_update_start_impl(felt_surface* s);
-int felt_update_global(felt_surface* s, const felt_speed* fn);
+// Global step of every member, member i with speed fns[i].
+int felt_update_global(felt_surface* s, const felt_speed* fns);
 int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double* items,
                         int* launches);
 int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
-int felt_update_box_impl(felt_surface* s, const int32_t* lo, const int32_t* hi,
-                         const felt_speed* fn);
+// Bounded step of n distinct members ids[j], each over its world box [lo, hi] (3 ints per
+// entry) with speed fns[j]; other members are not touched.
+int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t* lo,
+                           const int32_t* hi, const felt_speed* fns);
 int felt_delta_impl(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
 int felt_update_end_public(felt_surface* s);
-int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::vector<float>& val,
-                      const std::vector<int>& layer);
-int felt_load_dense_impl(felt_surface* s, const float* phi, size_t n);
-int felt_snapshot_impl(felt_surface* s, float* dst, size_t total);
-int felt_get_impl(felt_surface* s, size_t n, const int32_t* gpos, float* out);
-int felt_collect_layer_impl(felt_surface* s, int layer, std::vector<int>& out);
-int felt_lu_positions_impl(felt_surface* s, int lu_i, int tag, std::vector<int>& out);
+// Per-member operations; positions are member grid coordinates (3 ints per entry) unless
+// stated.
+int felt_track_points(felt_surface* s, int m, const std::vector<int>& gpos,
+                      const std::vector<float>& val, const std::vector<int>& layer);
+int felt_load_dense_impl(felt_surface* s, int m, const float* phi, size_t n);
+// Loads the field min_k(|x - c_k| - r_k) (world coordinates, fp64) as load_dense would.
+int felt_load_spheres_impl(felt_surface* s, int m, int n, const double* centres,
+                           const double* radii);
+int felt_snapshot_impl(felt_surface* s, int m, float* dst, size_t total);
+int felt_get_impl(felt_surface* s, int m, size_t n, const int32_t* gpos, float* out);
+// World positions of member m's voxels tracked in `layer`.
+int felt_collect_layer_impl(felt_surface* s, int m, int layer, std::vector<int>& out);
+// World positions of member m's entries of lookup lu_i with tag `tag`.
+int felt_lu_positions_impl(felt_surface* s, int m, int lu_i, int tag, std::vector<int>& out);
 int felt_band_total(felt_surface* s, size_t* out);
 int felt_download(felt_surface* s, void* dst, const void* src, size_t bytes);
+// Device bytes held by the handle.
+uint64_t felt_device_bytes(const felt_surface* s);
 int felt_sync_impl(felt_surface* s);

@@ -51,14 +51,6 @@ __global__ void k_init_free(int* stack, int cap) {
 
 // ---- compaction of the narrow band into per-layer lists -------------------------------------
 
-template <int D>
-__global__ void k_active_parts(Geo G, Pool P, int* out, int* n) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool act = p < G.np && P.page[p] >= 0;
-    const int at = wave_append(n, act);
-    if (act) out[at] = p;
-}
-
 __global__ void k_dmask_pcnt(Geo G, Pool P, unsigned* dmask) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= G.np) return;
@@ -78,26 +70,29 @@ __global__ void k_dmask_lu(Geo G, Pool P, Lu lu, const int* rng, unsigned* dmask
 
 // ---- bounded update: zero layer in a box, affected-set search -------------------------------
 
+// Zero-layer voxels of the listed partitions that lie in their box, appended to z.  Partition j
+// uses box pbox[j]: boxes holds {lo[3], hi[3]} per box in its member's grid coordinates,
+// half-open.
 template <int D>
-__global__ void k_box_zero(Geo G, Pool P, const int* parts, int nparts, int lo0, int lo1, int lo2,
-                           int hi0, int hi1, int hi2, uint32_t* z, Ctr* ctr) {
-    const int lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+__global__ void k_box_zero(Geo G, Pool P, const int* parts, const int* pbox, int nparts,
+                           const int* boxes, uint32_t* z, Ctr* ctr) {
     for (int j = blockIdx.x; j < nparts; j += gridDim.x) {
         const int p = parts[j];
         const int s = P.page[p];
         if (s < 0) continue;
-        int pc[3];
-        part_coords<D>(G, p, pc);
+        const int4 o = P.slot_org[s];
+        const int org[3] = {o.x, o.y, o.z};
+        const int* bx = boxes + 6 * pbox[j];
         for (int base = 0; base < G.pv; base += blockDim.x) {
             const int l = base + threadIdx.x;
             bool take = false;
             if (l < G.pv && P.lay[(size_t)s * G.pv + l] == 0) {
                 take = true;
-                int lc[3];
-                leaf_coords<D>(G, l, lc);
+                int lc[3] = {0, 0, 0};
+                leaf_coords<D>(G, (uint32_t)l, lc);
                 for (int a = 0; a < D; a++) {
-                    const int ga = pc[a] * G.part[a] + lc[a];
-                    if (ga < lo[a] || ga >= hi[a]) take = false;
+                    const int ga = org[a] + lc[a];
+                    if (ga < bx[a] || ga >= bx[3 + a]) take = false;
                 }
             }
             const int at = wave_append(&ctr->z_span[1], take);
@@ -131,24 +126,24 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const int i = base + threadIdx.x;
         const bool valid = i < n;
-        int slot = 0, loc[3] = {0, 0, 0}, g[3] = {0, 0, 0};
-        if (valid) decode<D>(G, P, lu.v[lo + i], slot, loc, g);
+        Vx x;
+        if (valid) decode<D>(G, P, lu.v[lo + i], x);
         for (int a = 0; a < D; a++) {
             for (int s = -1; s <= 1; s += 2) {
                 bool add = false;
                 uint32_t vq = 0;
                 int kq = 0;
                 if (valid) {
-                    int q[3] = {g[0], g[1], g[2]};
+                    int q[3] = {x.g[0], x.g[1], x.g[2]};
                     q[a] += s;
-                    if (inside<D>(G, q)) {
-                        const int pq = part_of<D>(G, q);
+                    if (inside<D>(x.size, q)) {
+                        int pq, lq;
+                        locate<D>(G, P, x.m, q, pq, lq);
                         const int sq = P.page[pq];
-                        const float dq = sq < 0 ? P.pbg[pq]
-                                                : P.phi[(size_t)sq * G.pv + leaf_of<D>(G, q)];
+                        const float dq = sq < 0 ? P.pbg[pq] : P.phi[(size_t)sq * G.pv + lq];
                         kq = layer_of(dq, G.L, &ctr->nan_events);
                         if (in_band(kq, G.L) && sq >= 0) {
-                            vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, q);
+                            vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
                             add = cas_byte((int8_t*)&amark[vq], 0, 1);
                         }
                     }
@@ -172,14 +167,14 @@ __global__ void k_unmark(Lu lu, const int* rng, uint8_t* amark) {
 // ---- seeding, loading, read-back ------------------------------------------------------------
 
 template <int D>
-__global__ void k_mark_parts(Geo G, Pool P, const int* gpos, int n, int* nflag, int* nlist,
+__global__ void k_mark_parts(Geo G, Pool P, int m, const int* gpos, int n, int* nflag, int* nlist,
                              Ctr* ctr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool q_new = false;
     int p = 0;
     if (i < n) {
         int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
-        p = part_of<D>(G, q);
+        p = part_of<D>(G, P, m, q);
         if (P.page[p] < 0) q_new = atomicExch(&nflag[p], 1) == 0;
     }
     const int at = wave_append(&ctr->need_n, q_new);
@@ -189,15 +184,16 @@ __global__ void k_mark_parts(Geo G, Pool P, const int* gpos, int n, int* nflag, 
 // Seed / track: phi = value always, list membership only when untracked (first-list-wins,
 // LookupMixin.hpp:298-331; TrackedMixin.hpp ByValue::track).
 template <int D>
-__global__ void k_track_points(Geo G, Pool P, const int* gpos, const float* val, const int* layer,
-                               int n) {
+__global__ void k_track_points(Geo G, Pool P, int m, const int* gpos, const float* val,
+                               const int* layer, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
-    const int p = part_of<D>(G, q);
+    int p, l;
+    locate<D>(G, P, m, q, p, l);
     const int s = P.page[p];
     if (s < 0) return;
-    const size_t v = (size_t)s * G.pv + leaf_of<D>(G, q);
+    const size_t v = (size_t)s * G.pv + l;
     P.phi[v] = val[i];
     if (P.lay[v] == NONE) {
         P.lay[v] = (int8_t)layer[i];
@@ -205,76 +201,107 @@ __global__ void k_track_points(Geo G, Pool P, const int* gpos, const float* val,
     }
 }
 
-// Dense load: per partition, decide whether it holds a band voxel, and the background of the
-// ones that do not (sign of the lowest-corner voxel).
+// Fields a member can be loaded from: value at an in-grid position q.
+struct DenseField {
+    const float* d;  // row-major over the member's grid, last axis fastest
+    int size[3];
+    __device__ float at(const int q[3], int D) const {
+        size_t idx = 0;
+        for (int a = 0; a < D; a++) idx = idx * size[a] + q[a];
+        return d[idx];
+    }
+};
+
+// min_k(|x - c_k| - r_k) in fp64 at world position x = q + offset, clamped to +-(L+1) and
+// rounded to fp32; |x - c|^2 sums the axes in order without contraction.
+struct SphereField {
+    const double* c;  // 3 per sphere
+    const double* r;
+    int n;
+    int offset[3];
+    double lim;
+    __device__ float at(const int q[3], int D) const {
+        double best = 0.0;
+        for (int k = 0; k < n; k++) {
+            double sq = 0.0;
+            for (int a = 0; a < D; a++) {
+                const double d = (double)(q[a] + offset[a]) - c[3 * k + a];
+                sq = sq + d * d;
+            }
+            const double v = __builtin_sqrt(sq) - r[k];
+            if (k == 0 || v < best) best = v;
+        }
+        if (best < -lim) best = -lim;
+        if (best > lim) best = lim;
+        return (float)best;
+    }
+};
+
+// Grid position of leaf l of member-local partition pc.
 template <int D>
-__global__ void k_load_scan(Geo G, Pool P, const float* dense, int* has, Ctr* ctr) {
+__device__ __forceinline__ void leaf_pos(const Geo& G, const int pc[3], int l, int q[3]) {
+    int lc[3] = {0, 0, 0};
+    leaf_coords<D>(G, (uint32_t)l, lc);
+    for (int a = 0; a < 3; a++) q[a] = pc[a] * G.part[a] + lc[a];
+}
+
+// Load, pass 1 (one block per partition of member m): does the partition hold a band voxel,
+// and the background of the ones that do not (sign of the lowest-corner voxel).
+template <int D, class F>
+__global__ void k_load_scan(Geo G, Pool P, int m, F f, int* has, Ctr* ctr) {
     __shared__ int any;
-    const int p = blockIdx.x;
+    const Mem& M = P.mem[m];
+    const int lp = blockIdx.x;
+    const int p = M.pbase + lp;
     if (threadIdx.x == 0) any = 0;
     __syncthreads();
-    int pc[3] = {0, 0, 0}, rem = p;
-    for (int a = D - 1; a >= 0; a--) {
-        pc[a] = rem % G.csize[a];
-        rem /= G.csize[a];
-    }
-    bool first_found = false;
+    int pc[3] = {0, 0, 0};
+    part_coords<D>(M, lp, pc);
     for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
-        int r = l, q[3] = {0, 0, 0};
-        for (int a = D - 1; a >= 0; a--) {
-            q[a] = pc[a] * G.part[a] + r % G.part[a];
-            r /= G.part[a];
-        }
-        if (!inside<D>(G, q)) continue;
-        size_t idx = 0;
-        for (int a = 0; a < D; a++) idx = idx * G.size[a] + q[a];
-        if (in_band(layer_of(dense[idx], G.L, &ctr->nan_events), G.L)) any = 1;
-        (void)first_found;
+        int q[3];
+        leaf_pos<D>(G, pc, l, q);
+        if (!inside<D>(M.size, q)) continue;
+        if (in_band(layer_of(f.at(q, D), G.L, &ctr->nan_events), G.L)) any = 1;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        has[p] = any;
+        has[lp] = any;
         int q[3] = {0, 0, 0};
         for (int a = 0; a < D; a++) q[a] = pc[a] * G.part[a];
-        if (inside<D>(G, q)) {
-            size_t idx = 0;
-            for (int a = 0; a < D; a++) idx = idx * G.size[a] + q[a];
-            P.pbg[p] = dense[idx] < 0.0f ? -(float)(G.L + 1) : (float)(G.L + 1);
-        }
+        if (inside<D>(M.size, q))
+            P.pbg[p] = f.at(q, D) < 0.0f ? -(float)(G.L + 1) : (float)(G.L + 1);
     }
 }
 
-__global__ void k_list_from_flags(const int* has, int np, int* nlist, int* nflag, Ctr* ctr) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool take = p < np && has[p];
+// has[] of member m's partitions -> the activation list (pool partition indices).
+__global__ void k_list_from_flags(Pool P, int m, const int* has, int np, int* nlist, int* nflag,
+                                  Ctr* ctr) {
+    const int lp = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool take = lp < np && has[lp];
     const int at = wave_append(&ctr->need_n, take);
     if (take) {
+        const int p = P.mem[m].pbase + lp;
         nlist[at] = p;
         nflag[p] = 1;
     }
 }
 
-template <int D>
-__global__ void k_load_fill(Geo G, Pool P, const float* dense, const int* has, Ctr* ctr) {
-    const int p = blockIdx.x;
-    if (!has[p]) return;
+// Load, pass 2: values and layers of the activated partitions.
+template <int D, class F>
+__global__ void k_load_fill(Geo G, Pool P, int m, F f, const int* has, Ctr* ctr) {
+    const Mem& M = P.mem[m];
+    const int lp = blockIdx.x;
+    if (!has[lp]) return;
+    const int p = M.pbase + lp;
     const int s = P.page[p];
     if (s < 0) return;
-    int pc[3] = {0, 0, 0}, rem = p;
-    for (int a = D - 1; a >= 0; a--) {
-        pc[a] = rem % G.csize[a];
-        rem /= G.csize[a];
-    }
+    int pc[3] = {0, 0, 0};
+    part_coords<D>(M, lp, pc);
     for (int l = threadIdx.x; l < G.pv; l += blockDim.x) {
-        int r = l, q[3] = {0, 0, 0};
-        for (int a = D - 1; a >= 0; a--) {
-            q[a] = pc[a] * G.part[a] + r % G.part[a];
-            r /= G.part[a];
-        }
-        if (!inside<D>(G, q)) continue;
-        size_t idx = 0;
-        for (int a = 0; a < D; a++) idx = idx * G.size[a] + q[a];
-        const float x = dense[idx];
+        int q[3];
+        leaf_pos<D>(G, pc, l, q);
+        if (!inside<D>(M.size, q)) continue;
+        const float x = f.at(q, D);
         const size_t v = (size_t)s * G.pv + l;
         P.phi[v] = x;
         const int k = layer_of(x, G.L, &ctr->nan_events);
@@ -286,72 +313,83 @@ __global__ void k_load_fill(Geo G, Pool P, const float* dense, const int* has, C
 }
 
 template <int D>
-__global__ void k_snapshot(Geo G, Pool P, float* out, int64_t total) {
+__global__ void k_snapshot(Geo G, Pool P, int m, float* out, int64_t total) {
+    const Mem& M = P.mem[m];
+    const int size[3] = {M.size[0], M.size[1], M.size[2]};
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         int q[3] = {0, 0, 0};
         int64_t r = i;
         for (int a = D - 1; a >= 0; a--) {
-            q[a] = (int)(r % G.size[a]);
-            r /= G.size[a];
+            q[a] = (int)(r % size[a]);
+            r /= size[a];
         }
-        out[i] = get_g<D>(G, P, q);
+        out[i] = get_g<D>(G, P, m, size, q);
     }
 }
 
 template <int D>
-__global__ void k_get(Geo G, Pool P, const int* gpos, int n, float* out) {
+__global__ void k_get(Geo G, Pool P, int m, const int* gpos, int n, float* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const Mem& M = P.mem[m];
+    const int size[3] = {M.size[0], M.size[1], M.size[2]};
     int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
-    out[i] = get_g<D>(G, P, q);
+    out[i] = get_g<D>(G, P, m, size, q);
 }
 
-// Positions of voxels tracked in `layer` (tag filter) over all active partitions.
+// World positions of member m's voxels tracked in `layer` (one block per member partition).
 template <int D>
-__global__ void k_collect_layer(Geo G, Pool P, int layer, int* out, int cap, int* n) {
-    const int p = blockIdx.x;
+__global__ void k_collect_layer(Geo G, Pool P, int m, int layer, int* out, int cap, int* n) {
+    const Mem& M = P.mem[m];
+    const int p = M.pbase + blockIdx.x;
     const int s = P.page[p];
     if (s < 0) return;
-    int pc[3] = {0, 0, 0};
-    part_coords<D>(G, p, pc);
+    const int4 o = P.slot_org[s];
+    const int org[3] = {o.x, o.y, o.z};
     for (int base = 0; base < G.pv; base += blockDim.x) {
         const int l = base + threadIdx.x;
         const bool take = l < G.pv && P.lay[(size_t)s * G.pv + l] == layer;
         const int at = wave_append(n, take);
         if (take && at < cap) {
-            int lc[3];
-            leaf_coords<D>(G, l, lc);
-            for (int a = 0; a < D; a++) out[at * 3 + a] = pc[a] * G.part[a] + lc[a] + G.offset[a];
+            int lc[3] = {0, 0, 0};
+            leaf_coords<D>(G, (uint32_t)l, lc);
+            for (int a = 0; a < D; a++) out[at * 3 + a] = org[a] + lc[a] + M.offset[a];
         }
     }
 }
 
+// World positions of the lookup entries of member m with tag `tag`.
 template <int D>
-__global__ void k_lu_positions(Geo G, Pool P, Lu lu, const int* n, int tag, int* out, int cap,
-                               int* cnt) {
+__global__ void k_lu_positions(Geo G, Pool P, Lu lu, const int* n, int m, int tag, int* out,
+                               int cap, int* cnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool take = i < *n && lu.tag[i] == tag;
-    int slot = 0, loc[3], g[3] = {0, 0, 0};
-    if (take) decode<D>(G, P, lu.v[i], slot, loc, g);
+    bool take = i < *n && lu.tag[i] == tag;
+    Vx x;
+    if (take) {
+        decode<D>(G, P, lu.v[i], x);
+        take = x.m == m;
+    }
     const int at = wave_append(cnt, take);
     if (take && at < cap)
-        for (int a = 0; a < D; a++) out[at * 3 + a] = g[a] + G.offset[a];
+        for (int a = 0; a < D; a++) out[at * 3 + a] = x.g[a] + P.mem[m].offset[a];
 }
 
 template <int D>
-__global__ void k_voxel_ids(Geo G, Pool P, const int* gpos, int n, uint32_t* out, int* bad) {
+__global__ void k_voxel_ids(Geo G, Pool P, int m, const int* gpos, int n, uint32_t* out,
+                            int* bad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int q[3] = {gpos[i * 3], gpos[i * 3 + 1], gpos[i * 3 + 2]};
-    const int p = part_of<D>(G, q);
+    int p, l;
+    locate<D>(G, P, m, q, p, l);
     const int s = P.page[p];
     if (s < 0) {
         atomicAdd(bad, 1);
         out[i] = 0;
         return;
     }
-    const uint32_t v = (uint32_t)s * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, q);
+    const uint32_t v = (uint32_t)s * (uint32_t)G.pv + (uint32_t)l;
     if (P.lay[v] != 0) atomicAdd(bad, 1);
     out[i] = v;
 }
@@ -531,6 +569,20 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->P.pbg, np * sizeof(float)));
     HIPX(hipMalloc(&s->P.pcnt, np * G.NL * sizeof(int)));
     HIPX(hipMalloc(&s->P.slot_part, cap * sizeof(int)));
+    HIPX(hipMalloc(&s->P.slot_org, cap * sizeof(int4)));
+    {
+        Mem* dm = nullptr;
+        int* pm = nullptr;
+        HIPX(hipMalloc(&dm, s->mems.size() * sizeof(Mem)));
+        HIPX(hipMalloc(&pm, np * sizeof(int)));
+        s->P.mem = dm;
+        s->P.part_mem = pm;
+        std::vector<int> owner(np);
+        for (size_t m = 0; m < s->mems.size(); m++)
+            for (int i = 0; i < s->mems[m].np; i++) owner[s->mems[m].pbase + i] = (int)m;
+        HIPX(hipMemcpy(dm, s->mems.data(), s->mems.size() * sizeof(Mem), hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(pm, owner.data(), np * sizeof(int), hipMemcpyHostToDevice));
+    }
     HIPX(hipMalloc(&s->P.phi, cap * (int64_t)G.pv * sizeof(float)));
     // +4: byte compare-and-swap goes through the enclosing aligned 32-bit word
     HIPX(hipMalloc(&s->P.lay, cap * (int64_t)G.pv + 4));
@@ -546,8 +598,9 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->smask, np * sizeof(unsigned)));
     HIPX(hipMalloc(&s->act, np * sizeof(int)));
     HIPX(hipMalloc(&s->bbase, (G.NL + 2) * sizeof(int)));
-    HIPX(hipMalloc(&s->d_speed, sizeof(Speed)));
-    HIPX(hipHostMalloc((void**)&s->h_speed, sizeof(Speed), hipHostMallocDefault));
+    HIPX(hipMalloc(&s->d_speed, G.nmem * sizeof(Speed)));
+    HIPX(hipHostMalloc((void**)&s->h_speed, G.nmem * sizeof(Speed), hipHostMallocDefault));
+    std::memset(s->h_speed, 0, G.nmem * sizeof(Speed));
     if (s->blk_cnt.ensure((size_t)np * G.NL + G.NL + 1)) return FELT_E_DEVICE;
     if (s->blk_off.ensure((size_t)np * G.NL)) return FELT_E_DEVICE;
     const int gb = 1024;
@@ -590,7 +643,7 @@ void felt_surface_release(felt_surface* s) {
     if (s->h_speed) (void)hipHostFree(s->h_speed);
     s->h_speed = nullptr;
     void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
-                    s->amark, s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag,
+                    s->P.slot_org, (void*)s->P.mem, (void*)s->P.part_mem, s->amark, s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag,
                     s->nlist, s->dmask, s->smask, s->act, s->bbase, s->d_speed};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -893,12 +946,18 @@ static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
     return FELT_OK;
 }
 
-int felt_update_global(felt_surface* s, const felt_speed* fn) {
-    Speed sp{fn->kind, fn->a, fn->b, fn->c, (unsigned long long)fn->seed, fn->cell};
-    if (fn->kind == FELT_SPEED_NOISE_CURV && fn->cell <= 0)
-        return felt_fail(FELT_E_ARG, "noise cell must be positive");
-    *s->h_speed = sp;
-    HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, sizeof(Speed), hipMemcpyHostToDevice, s->st));
+static Speed to_speed(const felt_speed& f) {
+    return Speed{f.kind, f.a, f.b, f.c, (unsigned long long)f.seed, f.cell};
+}
+
+int felt_update_global(felt_surface* s, const felt_speed* fns) {
+    for (int m = 0; m < s->G.nmem; m++) {
+        if (fns[m].kind == FELT_SPEED_NOISE_CURV && fns[m].cell <= 0)
+            return felt_fail(FELT_E_ARG, "noise cell must be positive");
+        s->h_speed[m] = to_speed(fns[m]);
+    }
+    HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, s->G.nmem * sizeof(Speed), hipMemcpyHostToDevice,
+                        s->st));
     s->speed_valid = true;
     return global_step(s, [&](const int* rng) {
         PDISPATCH(PK_SPEED, -(long long)s->G.L - 1, s->G.D, k_speed, dim3(grid_for_lists(s)),
@@ -996,8 +1055,8 @@ int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, co
         HIPX(hipMemcpyAsync(s->tmp_i.ptr, gpos, m * 3 * sizeof(int), hipMemcpyHostToDevice, s->st));
         HIPX(hipMemcpyAsync(s->tmp_f.ptr, vals, m * sizeof(float), hipMemcpyHostToDevice, s->st));
         HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
-        DISPATCH(s->G.D, k_voxel_ids, dim3(blocks_for(m)), dim3(TPB), s->G, s->P, s->tmp_i.ptr,
-                 (int)m, s->tmp_u.ptr, &s->ctr->misc);
+        DISPATCH(s->G.D, k_voxel_ids, dim3(blocks_for(m)), dim3(TPB), s->G, s->P, 0,
+                 s->tmp_i.ptr, (int)m, s->tmp_u.ptr, &s->ctr->misc);
         if ((rc = read_ctr(s))) return rc;
         if (s->hc.misc) return felt_fail(FELT_E_DELTA, "delta at a position outside the zero layer");
         LAUNCH(k_vox_set, dim3(blocks_for(m)), dim3(TPB), s->tmp_u.ptr, s->tmp_f.ptr, (int)m,
@@ -1069,54 +1128,67 @@ int felt_band_total(felt_surface* s, size_t* out) {
     return FELT_OK;
 }
 
-// Bounded step (Surface.hpp:310-386).
-int felt_update_box_impl(felt_surface* s, const int32_t* lo, const int32_t* hi,
-                         const felt_speed* fn) {
+// Bounded step (Surface.hpp:310-386) of members ids[0..n): each member's partitions that
+// overlap its box (clamped to the grid) are searched for zero-layer voxels with
+// lo <= x < min(hi + 1, grid_hi), which take that member's speed; then update_end over all of
+// them at once.
+int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t* lo,
+                           const int32_t* hi, const felt_speed* fns) {
     const Geo& G = s->G;
     int rc;
     if ((rc = size_for_band(s))) return rc;
     if ((rc = felt_update_start_impl(s))) return rc;
-    // partition box clamped to the grid, voxel box [lo, min(hi+1, grid_hi))
-    int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0}, vlo[3] = {0, 0, 0}, vhi[3] = {1, 1, 1};
-    bool empty = false;
-    for (int a = 0; a < G.D; a++) {
-        auto pos_child = [&](int w) {
+    std::vector<int> parts, pbox, boxes;
+    for (int j = 0; j < n; j++) {
+        const Mem& M = s->mems[ids[j]];
+        const int32_t* l3 = lo + 3 * j;
+        const int32_t* h3 = hi + 3 * j;
+        int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0}, vlo[3] = {0, 0, 0}, vhi[3] = {1, 1, 1};
+        bool empty = false;
+        for (int a = 0; a < G.D; a++) {
             // (w - offset) / part with C truncation, then the children offset cancels
-            return (w - G.offset[a]) / G.part[a];
-        };
-        const int cgl = pos_child(G.offset[a]);
-        const int cgh = pos_child(G.offset[a] + G.size[a] - 1);
-        clo[a] = std::max(cgl, pos_child(lo[a]));
-        chi[a] = std::min(cgh, pos_child(hi[a]));
-        if (chi[a] < clo[a]) empty = true;
-        vlo[a] = lo[a] - G.offset[a];
-        vhi[a] = std::min(G.size[a], hi[a] + 1 - G.offset[a]);
-    }
-    int zn = 0;
-    if (!empty) {
-        std::vector<int> parts;
-        int n0 = chi[0] - clo[0] + 1, n1 = G.D > 1 ? chi[1] - clo[1] + 1 : 1,
-            n2 = G.D > 2 ? chi[2] - clo[2] + 1 : 1;
-        parts.reserve((size_t)n0 * n1 * n2);
+            auto pos_child = [&](int w) { return (w - M.offset[a]) / G.part[a]; };
+            const int cgl = pos_child(M.offset[a]);
+            const int cgh = pos_child(M.offset[a] + M.size[a] - 1);
+            clo[a] = std::max(cgl, pos_child(l3[a]));
+            chi[a] = std::min(cgh, pos_child(h3[a]));
+            if (chi[a] < clo[a]) empty = true;
+            vlo[a] = l3[a] - M.offset[a];
+            vhi[a] = std::min(M.size[a], h3[a] + 1 - M.offset[a]);
+        }
+        s->h_speed[ids[j]] = to_speed(fns[j]);
+        if (empty) continue;
+        const int b = (int)boxes.size() / 6;
+        for (int a = 0; a < 3; a++) boxes.push_back(vlo[a]);
+        for (int a = 0; a < 3; a++) boxes.push_back(vhi[a]);
         for (int x = clo[0]; x <= chi[0]; x++)
             for (int y = clo[1]; y <= (G.D > 1 ? chi[1] : clo[1]); y++)
-                for (int z = clo[2]; z <= (G.D > 2 ? chi[2] : clo[2]); z++)
-                    parts.push_back(x * G.cstr[0] + (G.D > 1 ? y * G.cstr[1] : 0) +
-                                    (G.D > 2 ? z * G.cstr[2] : 0));
-        if ((rc = s->tmp_i.ensure(parts.size() + 1))) return rc;
-        HIPX(hipMemcpyAsync(s->tmp_i.ptr, parts.data(), parts.size() * sizeof(int),
+                for (int z = clo[2]; z <= (G.D > 2 ? chi[2] : clo[2]); z++) {
+                    parts.push_back(M.pbase + x * M.cstr[0] + (G.D > 1 ? y * M.cstr[1] : 0) +
+                                    (G.D > 2 ? z * M.cstr[2] : 0));
+                    pbox.push_back(b);
+                }
+    }
+    int zn = 0;
+    if (!parts.empty()) {
+        const size_t np = parts.size();
+        if ((rc = s->tmp_i.ensure(2 * np + boxes.size() + 1))) return rc;
+        HIPX(hipMemcpyAsync(s->tmp_i.ptr, parts.data(), np * sizeof(int), hipMemcpyHostToDevice,
+                            s->st));
+        HIPX(hipMemcpyAsync(s->tmp_i.ptr + np, pbox.data(), np * sizeof(int),
                             hipMemcpyHostToDevice, s->st));
-        DISPATCH(G.D, k_box_zero, dim3((unsigned)std::min<size_t>(parts.size(), 65535)), dim3(TPB),
-                 G, s->P, s->tmp_i.ptr, (int)parts.size(), vlo[0], vlo[1], vlo[2], vhi[0], vhi[1],
-                 vhi[2], s->zlist.ptr, s->ctr);
+        HIPX(hipMemcpyAsync(s->tmp_i.ptr + 2 * np, boxes.data(), boxes.size() * sizeof(int),
+                            hipMemcpyHostToDevice, s->st));
+        DISPATCH(G.D, k_box_zero, dim3((unsigned)std::min<size_t>(np, 65535)), dim3(TPB), G, s->P,
+                 s->tmp_i.ptr, s->tmp_i.ptr + np, (int)np, s->tmp_i.ptr + 2 * np, s->zlist.ptr,
+                 s->ctr);
         if ((rc = read_ctr(s))) return rc;
         zn = s->hc.z_span[1];
     }
     s->z_host_n = zn;
     if (zn > 0) {
-        Speed sp{fn->kind, fn->a, fn->b, fn->c, (unsigned long long)fn->seed, fn->cell};
-        *s->h_speed = sp;
-        HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, sizeof(Speed), hipMemcpyHostToDevice, s->st));
+        HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, G.nmem * sizeof(Speed), hipMemcpyHostToDevice,
+                            s->st));
         DISPATCH(G.D, k_speed, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr,
                  &s->ctr->z_span[0], s->d_speed, s->zdelta.ptr);
     }
@@ -1159,8 +1231,8 @@ int felt_update_end_public(felt_surface* s) {
         HIPX(hipMemcpyAsync(s->zdelta.ptr, dv.data(), m * sizeof(float), hipMemcpyHostToDevice,
                             s->st));
         HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
-        DISPATCH(s->G.D, k_voxel_ids, dim3(blocks_for(m)), dim3(TPB), s->G, s->P, s->tmp_i.ptr,
-                 (int)m, s->zlist.ptr, &s->ctr->misc);
+        DISPATCH(s->G.D, k_voxel_ids, dim3(blocks_for(m)), dim3(TPB), s->G, s->P, 0,
+                 s->tmp_i.ptr, (int)m, s->zlist.ptr, &s->ctr->misc);
         if ((rc = read_ctr(s))) return rc;
         if (s->hc.misc) {
             s->in_update = false;
@@ -1174,9 +1246,10 @@ int felt_update_end_public(felt_surface* s) {
     return felt_update_end_impl(s);
 }
 
-// Tracks points (seed and similar): activate partitions, then write values and lists.
-int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::vector<float>& val,
-                      const std::vector<int>& layer) {
+// Tracks points (seed and similar) of member m: activate partitions, then write values and
+// lists.
+int felt_track_points(felt_surface* s, int m, const std::vector<int>& gpos,
+                      const std::vector<float>& val, const std::vector<int>& layer) {
     const Geo& G = s->G;
     int rc;
     s->band_known = false;
@@ -1191,19 +1264,39 @@ int felt_track_points(felt_surface* s, const std::vector<int>& gpos, const std::
     HIPX(hipMemcpyAsync(s->tmp_f.ptr, val.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice,
                         s->st));
     HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
-    DISPATCH(G.D, k_mark_parts, dim3(blocks_for(n)), dim3(TPB), G, s->P, dpos, n, s->nflag,
+    DISPATCH(G.D, k_mark_parts, dim3(blocks_for(n)), dim3(TPB), G, s->P, m, dpos, n, s->nflag,
              s->nlist, s->ctr);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
            s->free_stack, s->ctr, always(s));
-    DISPATCH(G.D, k_track_points, dim3(blocks_for(n)), dim3(TPB), G, s->P, dpos, s->tmp_f.ptr,
+    DISPATCH(G.D, k_track_points, dim3(blocks_for(n)), dim3(TPB), G, s->P, m, dpos, s->tmp_f.ptr,
              dlay, n);
     return read_ctr(s);
 }
 
-int felt_load_dense_impl(felt_surface* s, const float* phi, size_t n) {
+// Load of member m from a field (Surface::load semantics restated for a dense field): every
+// partition holding a band voxel is activated and filled, the others get +-(L+1) by the sign of
+// their lowest-corner value.
+template <class F>
+static int load_field(felt_surface* s, int m, const F& f) {
     const Geo& G = s->G;
+    const Mem& M = s->mems[m];
     int rc;
     s->band_known = false;
+    if ((rc = s->tmp_i.ensure((size_t)M.np + 1))) return rc;
+    const int bt = std::min(1024, std::max(64, ((G.pv + 63) / 64) * 64));
+    auto scan = G.D == 2 ? k_load_scan<2, F> : k_load_scan<3, F>;
+    auto fill = G.D == 2 ? k_load_fill<2, F> : k_load_fill<3, F>;
+    hipLaunchKernelGGL(scan, dim3(M.np), dim3(bt), 0, s->st, G, s->P, m, f, s->tmp_i.ptr, s->ctr);
+    HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
+    LAUNCH(k_list_from_flags, dim3(blocks_for(M.np)), dim3(TPB), s->P, m, s->tmp_i.ptr, M.np,
+           s->nlist, s->nflag, s->ctr);
+    LAUNCH(k_alloc, dim3(std::min(1024, M.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
+           s->free_stack, s->ctr, always(s));
+    hipLaunchKernelGGL(fill, dim3(M.np), dim3(bt), 0, s->st, G, s->P, m, f, s->tmp_i.ptr, s->ctr);
+    return read_ctr(s);
+}
+
+int felt_load_dense_impl(felt_surface* s, int m, const float* phi, size_t n) {
     float* dense = nullptr;
     HIPX(hipMalloc(&dense, n * sizeof(float)));
     hipError_t e = hipMemcpyAsync(dense, phi, n * sizeof(float), hipMemcpyHostToDevice, s->st);
@@ -1211,54 +1304,73 @@ int felt_load_dense_impl(felt_surface* s, const float* phi, size_t n) {
         (void)hipFree(dense);
         return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
     }
-    if ((rc = s->tmp_i.ensure((size_t)G.np + 1))) { (void)hipFree(dense); return rc; }
-    DISPATCH(G.D, k_load_scan, dim3(G.np), dim3(TPB), G, s->P, dense, s->tmp_i.ptr, s->ctr);
-    HIPX(hipMemsetAsync(&s->ctr->need_n, 0, sizeof(int), s->st));
-    LAUNCH(k_list_from_flags, dim3(blocks_for(G.np)), dim3(TPB), s->tmp_i.ptr, G.np, s->nlist,
-           s->nflag, s->ctr);
-    LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
-           s->free_stack, s->ctr, always(s));
-    DISPATCH(G.D, k_load_fill, dim3(G.np), dim3(TPB), G, s->P, dense, s->tmp_i.ptr, s->ctr);
-    rc = read_ctr(s);
+    DenseField f{dense, {s->mems[m].size[0], s->mems[m].size[1], s->mems[m].size[2]}};
+    const int rc = load_field(s, m, f);
+    (void)hipStreamSynchronize(s->st);
     (void)hipFree(dense);
     return rc;
 }
 
-int felt_snapshot_impl(felt_surface* s, float* dst, size_t total) {
+int felt_load_spheres_impl(felt_surface* s, int m, int n, const double* centres,
+                           const double* radii) {
+    double* d = nullptr;
+    HIPX(hipMalloc(&d, (size_t)n * 4 * sizeof(double) + 8));
+    hipError_t e = hipMemcpyAsync(d, centres, (size_t)n * 3 * sizeof(double),
+                                  hipMemcpyHostToDevice, s->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d + 3 * (size_t)n, radii, (size_t)n * sizeof(double),
+                           hipMemcpyHostToDevice, s->st);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    }
+    const Mem& M = s->mems[m];
+    SphereField f{d, d + 3 * (size_t)n, n, {M.offset[0], M.offset[1], M.offset[2]},
+                  (double)(s->G.L + 1)};
+    const int rc = load_field(s, m, f);
+    (void)hipStreamSynchronize(s->st);
+    (void)hipFree(d);
+    return rc;
+}
+
+int felt_snapshot_impl(felt_surface* s, int m, float* dst, size_t total) {
     int rc;
     if ((rc = s->tmp_f.ensure(total))) return rc;
     DISPATCH(s->G.D, k_snapshot, dim3((unsigned)std::min<int64_t>(65535, blocks_for(total))),
-             dim3(TPB), s->G, s->P, s->tmp_f.ptr, (int64_t)total);
+             dim3(TPB), s->G, s->P, m, s->tmp_f.ptr, (int64_t)total);
     HIPX(hipMemcpyAsync(dst, s->tmp_f.ptr, total * sizeof(float), hipMemcpyDeviceToHost, s->st));
     HIPX(hipStreamSynchronize(s->st));
     return FELT_OK;
 }
 
-int felt_get_impl(felt_surface* s, size_t n, const int32_t* gpos, float* out) {
+int felt_get_impl(felt_surface* s, int m, size_t n, const int32_t* gpos, float* out) {
     int rc;
     if (n == 0) return FELT_OK;
     if ((rc = s->tmp_i.ensure(n * 3 + 1))) return rc;
     if ((rc = s->tmp_f.ensure(n + 1))) return rc;
     HIPX(hipMemcpyAsync(s->tmp_i.ptr, gpos, n * 3 * sizeof(int), hipMemcpyHostToDevice, s->st));
-    DISPATCH(s->G.D, k_get, dim3(blocks_for(n)), dim3(TPB), s->G, s->P, s->tmp_i.ptr, (int)n,
+    DISPATCH(s->G.D, k_get, dim3(blocks_for(n)), dim3(TPB), s->G, s->P, m, s->tmp_i.ptr, (int)n,
              s->tmp_f.ptr);
     HIPX(hipMemcpyAsync(out, s->tmp_f.ptr, n * sizeof(float), hipMemcpyDeviceToHost, s->st));
     HIPX(hipStreamSynchronize(s->st));
     return FELT_OK;
 }
 
-int felt_collect_layer_impl(felt_surface* s, int layer, std::vector<int>& out) {
+int felt_collect_layer_impl(felt_surface* s, int m, int layer, std::vector<int>& out) {
     int rc;
+    const Geo& G = s->G;
+    const Mem& M = s->mems[m];
     uint64_t n64 = 0;
-    std::vector<int> cnt(s->G.np * s->G.NL);
-    HIPX(hipMemcpyAsync(cnt.data(), s->P.pcnt, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, s->st));
+    std::vector<int> cnt((size_t)M.np * G.NL);
+    HIPX(hipMemcpyAsync(cnt.data(), s->P.pcnt + (size_t)M.pbase * G.NL, cnt.size() * sizeof(int),
+                        hipMemcpyDeviceToHost, s->st));
     HIPX(hipStreamSynchronize(s->st));
-    for (int p = 0; p < s->G.np; p++) n64 += (uint64_t)cnt[(size_t)p * s->G.NL + layer + s->G.L];
+    for (int p = 0; p < M.np; p++) n64 += (uint64_t)cnt[(size_t)p * G.NL + layer + G.L];
     const size_t n = (size_t)n64;
     if ((rc = s->tmp_i.ensure(n * 3 + 3))) return rc;
     HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
-    DISPATCH(s->G.D, k_collect_layer, dim3(s->G.np), dim3(TPB), s->G, s->P, layer, s->tmp_i.ptr,
-             (int)n, &s->ctr->misc);
+    DISPATCH(G.D, k_collect_layer, dim3(M.np), dim3(TPB), G, s->P, m, layer, s->tmp_i.ptr, (int)n,
+             &s->ctr->misc);
     if ((rc = read_ctr(s))) return rc;
     if ((size_t)s->hc.misc != n) return felt_fail(FELT_E_STATE, "layer count mismatch");
     out.resize(n * 3);
@@ -1267,7 +1379,7 @@ int felt_collect_layer_impl(felt_surface* s, int layer, std::vector<int>& out) {
     return FELT_OK;
 }
 
-int felt_lu_positions_impl(felt_surface* s, int lu_i, int tag, std::vector<int>& out) {
+int felt_lu_positions_impl(felt_surface* s, int m, int lu_i, int tag, std::vector<int>& out) {
     int rc;
     if ((rc = read_ctr(s))) return rc;
     const int n = s->hc.lu_span[lu_i][1];
@@ -1275,12 +1387,13 @@ int felt_lu_positions_impl(felt_surface* s, int lu_i, int tag, std::vector<int>&
     HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
     if (n > 0)
         DISPATCH(s->G.D, k_lu_positions, dim3(blocks_for(n)), dim3(TPB), s->G, s->P,
-                 felt_lu(s, lu_i), &s->ctr->lu_span[lu_i][1], tag, s->tmp_i.ptr, n, &s->ctr->misc);
+                 felt_lu(s, lu_i), &s->ctr->lu_span[lu_i][1], m, tag, s->tmp_i.ptr, n,
+                 &s->ctr->misc);
     if ((rc = read_ctr(s))) return rc;
-    const int m = s->hc.misc;
-    out.resize((size_t)m * 3);
-    if (m)
-        HIPX(hipMemcpyAsync(out.data(), s->tmp_i.ptr, (size_t)m * 3 * sizeof(int),
+    const int k = s->hc.misc;
+    out.resize((size_t)k * 3);
+    if (k)
+        HIPX(hipMemcpyAsync(out.data(), s->tmp_i.ptr, (size_t)k * 3 * sizeof(int),
                             hipMemcpyDeviceToHost, s->st));
     HIPX(hipStreamSynchronize(s->st));
     return FELT_OK;
@@ -1290,6 +1403,20 @@ int felt_download(felt_surface* s, void* dst, const void* src, size_t bytes) {
     HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->st));
     HIPX(hipStreamSynchronize(s->st));
     return FELT_OK;
+}
+
+uint64_t felt_device_bytes(const felt_surface* s) {
+    const Geo& G = s->G;
+    const uint64_t np = (uint64_t)G.np, cap = (uint64_t)s->cap, pv = (uint64_t)G.pv;
+    // phi, lay, amark per voxel; slot tables; partition tables
+    uint64_t b = cap * pv * (sizeof(float) + 2) + cap * (sizeof(int) * 2 + sizeof(int4));
+    b += np * (sizeof(int) * 9 + sizeof(float) + sizeof(int) * G.NL * 3);
+    b += (uint64_t)s->mems.size() * (sizeof(Mem) + sizeof(Speed));
+    // per-step lists: band (4+4+1+1), two lookups (4+1+4+1), status changes (4+1+2),
+    // zero-layer list (4+4)
+    b += (uint64_t)s->list_cap * (10 + 2 * 10 + 7 + 8);
+    b += (uint64_t)s->vox.cap * sizeof(float);
+    return b;
 }
 
 int felt_sync_impl(felt_surface* s) {

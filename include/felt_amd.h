@@ -194,6 +194,57 @@ int felt_polys_child_mesh(felt_polys* p, uint64_t pos_idx, float* vtx, size_t vt
 /* Totals over all active children. */
 int felt_polys_totals(felt_polys* p, uint64_t* nvtx, uint64_t* nspx);
 
+/* ---- Batched surfaces --------------------------------------------------------------- */
+/* Not a reference type.  A batch is a pool of n independent surfaces that share D, L and the
+ * partition size and each have their own grid size.  Member i behaves exactly as a
+ * felt_surface of its size (same layers, values and meshes bit for bit), but one step of many
+ * members is one launch sequence over all their bricks (for a global step, one replay of a
+ * captured HIP graph).  Member positions are world coordinates of that member's grid
+ * (offset -size/2); member partition indices are its own children-grid PosIdx. */
+typedef struct felt_batch felt_batch;
+/* sizes: n*D grid sizes.  max_bricks: brick pool capacity, 0 for one brick per partition of
+ * every member; either way at most 2^32 / prod(part) bricks (32-bit voxel ids). */
+int felt_batch_create(int D, int L, const int32_t* part, int32_t n, const int32_t* sizes,
+                      uint64_t max_bricks, felt_batch** out);
+int felt_batch_destroy(felt_batch* b);
+int felt_batch_count(felt_batch* b, int32_t* n);
+/* Device bytes held by the batch (bricks, tables, step lists) and brick slots in use / total. */
+int felt_batch_memory(felt_batch* b, uint64_t* device_bytes, uint64_t* bricks_used,
+                      uint64_t* bricks_total);
+int felt_batch_seed(felt_batch* b, int32_t i, const int32_t* pos);
+int felt_batch_load_dense(felt_batch* b, int32_t i, const float* phi, size_t n);
+/* Loads member i with the field min_k(|x - c_k| - r_k) over n spheres (world centres, D per
+ * sphere), evaluated in fp64, clamped to +-(L+1) and rounded to fp32, with the semantics of
+ * felt_surface_load_dense for that dense field. */
+int felt_batch_load_spheres(felt_batch* b, int32_t i, int32_t n, const double* centres,
+                            const double* radii);
+/* update(fn) of every member, member i with speeds[i], as one step (Surface.hpp:254-296). */
+int felt_batch_update(felt_batch* b, const felt_speed* speeds);
+/* update(lo, hi, fn) of the n distinct members ids[j], member ids[j] over its box
+ * lo[j*D..], hi[j*D..] with speeds[j], as one step (Surface.hpp:310-386); the other members
+ * are not touched. */
+int felt_batch_update_boxes(felt_batch* b, int32_t n, const int32_t* ids, const int32_t* lo,
+                            const int32_t* hi, const felt_speed* speeds);
+int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n);
+int felt_batch_layer_size(felt_batch* b, int32_t i, int layer, uint64_t* n);
+int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos, size_t cap,
+                               size_t* n);
+/* Stats of member i; last_voxel_updates / last_rounds are those of the last batch step. */
+int felt_batch_stats(felt_batch* b, int32_t i, felt_stats* out);
+int felt_batch_step_info(felt_batch* b, uint64_t* voxel_updates, uint64_t* rounds);
+int felt_batch_set_graph(felt_batch* b, int enable);
+int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launch,
+                         double* items_per_launch, int* launches);
+int felt_batch_sync(felt_batch* b);
+/* Polygonisation of every member: felt_polys_notify / march / invalidate / totals / destroy
+ * act on all members at once; the per-member queries below take the member index. */
+int felt_batch_polys_create(felt_batch* b, felt_polys** out);
+int felt_batch_polys_changes(felt_polys* p, int32_t i, uint64_t* out, size_t cap, size_t* n);
+int felt_batch_polys_child_info(felt_polys* p, int32_t i, uint64_t pos_idx, int* active,
+                                uint64_t* nvtx, uint64_t* nspx);
+int felt_batch_polys_child_mesh(felt_polys* p, int32_t i, uint64_t pos_idx, float* vtx,
+                                size_t vtx_cap, uint32_t* spx, size_t spx_cap);
+
 /* ---- Tables ------------------------------------------------------------------------- */
 /* Copy of the packed marching-cubes triangle table the kernels use (256 entries). */
 int felt_mc_table(uint64_t* out, size_t n);
