@@ -127,6 +127,13 @@ def harmonic_sphere_phi(size: Sequence[int], L: int, radius: float, seed: int,
     return out
 
 
+def speed_of(index: int):
+    """The three per-surface speeds of the mixed workloads: constant -0.4, 0.15 * curvature,
+    -0.3 + 0.1 * curvature."""
+    from ._binding import speed_const_curv, speed_constant, speed_curvature
+    return [speed_constant(-0.4), speed_curvature(0.15), speed_const_curv(-0.3, 0.1)][index]
+
+
 @dataclass
 class SurfaceSpec:
     D: int
@@ -135,7 +142,7 @@ class SurfaceSpec:
     part: Tuple[int, ...]
     centres: List[Tuple[float, ...]] = field(default_factory=list)
     radii: List[float] = field(default_factory=list)
-    speed: int = 0  # index into SPEEDS
+    speed: int = 0  # speed_of(speed)
 
     def phi(self) -> np.ndarray:
         return spheres_phi(self.size, self.L, self.centres, self.radii)
