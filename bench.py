@@ -1,17 +1,24 @@
 #!/usr/bin/env python3
-"""Narrow-band voxel updates/s of Felt's sparse-field level-set update on one MI355X.
+"""Narrow-band voxel updates/s of Felt's sparse-field level-set update on one MI355X:
+batched vs plain on the mixed-size workload, plus the single large surface.
 
-Workload (one GPU): a single large surface on the plain GPU path — Surface<3,3>, float32,
-512^3 isogrid in 32^3 partitions, a sphere of radius 150 at the centre radially perturbed by
-8 real spherical harmonics (degree <= 6, amplitudes U[0,6) voxels, seed 0xFE1702), speed
-clamp(-0.3 * (1 + 0.5 * n(x)) + 0.1 * curvature, -1, 1) with n seeded value noise on a
-64-voxel lattice; incremental marching cubes of the changed partitions every 10 steps.
+Headline workload (mixed sizes): 256 independent Surface<3,3> in float32.  Grid edge
+log-uniform in [32, 256] rounded to a multiple of the partition size (8^3 partitions below an
+edge of 96, else 16^3), 1-8 spheres of radius U[0.05, 0.3] * edge with uniform centres (phi =
+min over spheres), per-surface speed drawn from {-0.4, 0.15 curv, -0.3 + 0.1 curv}, seed
+0xFE1703.  A step advances every surface by one update(fn).
+  - batched: the surfaces of each partition size form one batch (felt_batch_update: one
+    captured HIP graph replay per batch per step);
+  - plain: one felt_surface per surface, update(fn) called surface after surface.
+Voxel updates per step = band size at the start of the step (all 2L+1 layers), summed over
+surfaces.  `value` is the batched rate; the plain rate and their ratio sit beside it.
 
-A step is one update(fn) over the whole narrow band (plus notify(), and march() every 10th
-step).  Voxel updates per step = band size at the start of the step (all 2L+1 layers).  The
-timed region starts with the surface resident in HBM.
+Second workload (single large surface): Surface<3,3> 512^3 in 32^3 partitions, sphere r=150
+perturbed by 8 spherical harmonics (seed 0xFE1702), speed clamp(-0.3(1+0.5n)+0.1curv, -1, 1),
+incremental marching cubes every 10 steps.
 
-Prints one JSON line on stdout; progress goes to stderr.
+The timed regions start with the surfaces resident in HBM.  Prints one JSON line on stdout;
+progress goes to stderr.
 """
 import argparse
 import json
@@ -26,11 +33,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "narrow-band voxel updates/s, batched+served vs plain on GPU; p99 step latency"
 HBM_PEAK = 8.0e12
+MIXED_SEED = 0xFE1703
 # algorithmic bytes per list entry of each timed pass
 PASSES = {
-    0: ("speed (curvature + noise functor)", 12),   # index, phi(x) read, delta write
-    1: ("distance pass 1", 12),                     # index, phi(x) read, d write
-    2: ("band compaction", 1),                      # one layer byte per pool voxel scanned
+    0: ("speed (curvature functor)", 12),   # index, phi(x) read, delta write
+    1: ("distance pass 1", 12),             # index, phi(x) read, d write
+    2: ("band compaction", 1),              # one layer byte per pool voxel scanned
 }
 
 
@@ -38,145 +46,173 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(size=512, part=32, L=3):
+def pctl(x, q):
+    return float(np.percentile(np.asarray(x) * 1e3, q)) if len(x) else None
+
+
+def mixed_workload(n):
     from felt_amd import workloads
-    t = time.time()
-    phi = workloads.harmonic_sphere_phi((size,) * 3, L, 150.0 * size / 512.0, 0xFE1702,
-                                        amp=6.0 * size / 512.0)
-    log(f"[bench] workload field {size}^3 built in {time.time() - t:.1f}s")
-    return phi
+    return workloads.mixed_surfaces(n, MIXED_SEED, 32, 256)
 
 
-def speed():
-    from felt_amd import speed_noise_curv
-    return speed_noise_curv(-0.3, 0.1, 0.5, 0xFE1702, 64)
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--size", type=int, default=512)
-    ap.add_argument("--part", type=int, default=32)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--pass-reps", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs a surface on a single MI355X and has "
-              "no multi-GPU path (see DESIGN.md for how it would shard).", file=sys.stderr)
-        sys.exit(2)
-    L = 3
-    phi = make_workload(args.size, args.part, L)
-    import felt_amd
-    felt_amd.lib()
-    s = felt_amd.Surface(3, L, (args.size,) * 3, (args.part,) * 3)
-    t = time.time()
-    s.load_dense(phi)
-    s.sync()
-    log(f"[bench] loaded onto GPU in {time.time() - t:.1f}s, band {s.stats().band_voxels}")
-    polys = felt_amd.Polys(s)
-    polys.invalidate()
-    polys.march()
-    sp = speed()
-    mc_every = 10
-    for i in range(args.warmup):
-        s.update(sp)
-        polys.notify()
-        if (i + 1) % mc_every == 0:
-            polys.march()
-    s.sync()
-    log(f"[bench] {args.warmup} warmup steps done")
-
-    def run(steps, first, lat, tally):
-        for i in range(steps):
-            t0 = time.perf_counter()
-            s.update(sp)
-            polys.notify()
-            tally["updates"] += s.step_info()[0]
-            if (first + i + 1) % mc_every == 0:
-                tm = time.perf_counter()
-                polys.march()
-                tally["march_s"] += time.perf_counter() - tm
-                tally["spx"] += sum(polys.child_info(int(p))[2] for p in polys.changes())
-            lat.append(time.perf_counter() - t0)
-            if (i + 1) % 50 == 0:
-                log(f"[bench] step {i + 1}/{steps}")
-
-    # timed region: each update is one replay of the captured step graph
-    lat, tally = [], {"updates": 0, "march_s": 0.0, "spx": 0}
-    s.sync()
-    t_all = time.perf_counter()
-    run(args.steps, args.warmup, lat, tally)
-    s.sync()
-    total = time.perf_counter() - t_all
-    updates, march_s, spx = tally["updates"], tally["march_s"], tally["spx"]
-    # kernel durations: each pass's launches of one step, captured as a graph on the surface's
-    # stream and replayed between two HIP events on the final state of the timed run
+def time_passes(obj, reps):
+    """Per-pass timing on the current state; returns (table, roofline of the dominant pass)."""
     ktab = {}
     for k, (name, bpi) in PASSES.items():
-        us, items, n = s.time_pass(k, args.pass_reps)
+        us, items, n = obj.time_pass(k, reps)
         ktab[k] = (name, bpi, n, us, items)
     dom = max((k for k in ktab if k != 2), key=lambda k: ktab[k][2] * ktab[k][3])
     name, bpi, n, us, items = ktab[dom]
-    avg_s = us / 1e6
-    bytes_per_launch = bpi * items
-    achieved = bytes_per_launch / avg_s
-    lat_ms = np.array(lat) * 1e3
-    final = s.stats()
-    result = {
-        "metric": METRIC,
-        "value": updates / total,
-        "unit": "voxel updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": total / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"single large surface, plain GPU path: Surface<3,3> {args.size}^3 isogrid "
-                        f"in {args.part}^3 partitions, harmonic-perturbed sphere r="
-                        f"{150.0 * args.size / 512:g}, speed clamp(-0.3(1+0.5n)+0.1curv), "
-                        "incremental marching cubes every 10 steps",
-            "size": args.size, "partition": args.part, "layers": L,
-            "band_voxels_final": int(final.band_voxels),
-        },
-        "p50_step_ms": float(np.percentile(lat_ms, 50)),
-        "p99_step_ms": float(np.percentile(lat_ms, 99)),
-        "triangles_per_s": spx / march_s if march_s > 0 else None,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": name,
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": None,
-            "bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": avg_s * 1e6,
-            "timing": f"the pass's launches of one step on the final state, captured as a "
-                      f"HIP graph and replayed {args.pass_reps} times between two HIP events "
-                      "on the surface's stream",
-        },
-        "kernels": {v[0]: {"launches_per_step": v[2], "us_per_launch": v[3],
-                           "items_per_launch": v[4]} for v in ktab.values()},
-    }
+    achieved = bpi * items / (us / 1e6)
+    roof = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+            "bytes_per_launch": bpi * items, "avg_launch_us": us,
+            "timing": f"the pass's launches of one step on the final state, {reps} copies "
+                      "captured in one HIP graph and replayed between two HIP events on the "
+                      "handle's stream"}
+    table = {v[0]: {"launches_per_step": v[2], "us_per_launch": v[3], "items_per_launch": v[4]}
+             for v in ktab.values()}
+    return table, roof
+
+
+def run_mixed(args):
+    import felt_amd
+    from felt_amd import workloads
+    from felt_amd.batch import Batch
+    specs = mixed_workload(args.surfaces)
+    groups = {}
+    for i, sp in enumerate(specs):
+        groups.setdefault(sp.part[0], []).append(i)
+    lib = felt_amd.lib().lib
+    # ---- batched ----
+    t = time.time()
+    batches = []
+    for p, ids in sorted(groups.items()):
+        bt = Batch(lib, 3, 3, [specs[i].size for i in ids], (p,) * 3)
+        for j, i in enumerate(ids):
+            bt.load_spheres(j, specs[i].centres, specs[i].radii)
+        speeds = [workloads.speed_of(specs[i].speed) for i in ids]
+        batches.append((bt, speeds))
+    log(f"[bench] mixed: {len(specs)} surfaces in {len(batches)} batches loaded in "
+        f"{time.time() - t:.1f}s")
+    for _ in range(args.warmup):
+        for bt, speeds in batches:
+            bt.update(speeds)
+    lat, ups = [], 0
+    t_all = time.perf_counter()
+    for step in range(args.steps):
+        t0 = time.perf_counter()
+        for bt, speeds in batches:
+            bt.update(speeds)
+            ups += bt.step_info()[0]
+        lat.append(time.perf_counter() - t0)
+        if (step + 1) % 25 == 0:
+            log(f"[bench] batched step {step + 1}/{args.steps}")
+    total = time.perf_counter() - t_all
+    batched = {"value": ups / total, "ms_per_step": total / args.steps * 1e3,
+               "p50_step_ms": pctl(lat, 50), "p99_step_ms": pctl(lat, 99),
+               "voxel_updates_per_step": ups / args.steps}
+    big = max(batches, key=lambda b: b[0].memory()[1])[0]
+    ktab, roof = time_passes(big, args.pass_reps)
+    mem = [b.memory() for b, _ in batches]
+    for bt, _ in batches:
+        bt.close()
+    # ---- plain: one surface per launch sequence ----
+    t = time.time()
+    plain = []
+    for sp in specs:
+        s = felt_amd.Surface(3, 3, sp.size, sp.part)
+        s.load_spheres(sp.centres, sp.radii)
+        plain.append((s, workloads.speed_of(sp.speed)))
+    log(f"[bench] plain: {len(plain)} surfaces loaded in {time.time() - t:.1f}s")
+    for _ in range(args.warmup):
+        for s, f in plain:
+            s.update(f)
+    lat_p, ups_p = [], 0
+    t_all = time.perf_counter()
+    for step in range(args.plain_steps):
+        t0 = time.perf_counter()
+        for s, f in plain:
+            s.update(f)
+            ups_p += s.step_info()[0]
+        lat_p.append(time.perf_counter() - t0)
+        if (step + 1) % 10 == 0:
+            log(f"[bench] plain step {step + 1}/{args.plain_steps}")
+    total_p = time.perf_counter() - t_all
+    for s, _ in plain:
+        s.close()
+    plain_res = {"value": ups_p / total_p, "ms_per_step": total_p / args.plain_steps * 1e3,
+                 "p50_step_ms": pctl(lat_p, 50), "p99_step_ms": pctl(lat_p, 99),
+                 "steps": args.plain_steps, "voxel_updates_per_step": ups_p / args.plain_steps}
+    cfg = {"workload": f"mixed-size batch: {len(specs)} independent Surface<3,3> float32, edge "
+                       "log-uniform 32-256 (8^3 partitions below 96, else 16^3), 1-8 spheres "
+                       "r=U[0.05,0.3]*edge, speeds {-0.4, 0.15curv, -0.3+0.1curv}, seed "
+                       f"0x{MIXED_SEED:X}; one update(fn) of every surface per step",
+           "surfaces": len(specs),
+           "batches": {f"{p}^3": len(ids) for p, ids in sorted(groups.items())},
+           "device_bytes": int(sum(m[0] for m in mem)),
+           "bricks_in_use": int(sum(m[1] for m in mem))}
+    return specs, batched, plain_res, ktab, roof, cfg
+
+
+def run_single(args):
+    import felt_amd
+    from felt_amd import speed_noise_curv, workloads
+    size, part, L = args.size, args.part, 3
+    t = time.time()
+    phi = workloads.harmonic_sphere_phi((size,) * 3, L, 150.0 * size / 512.0, 0xFE1702,
+                                        amp=6.0 * size / 512.0)
+    log(f"[bench] single: field {size}^3 built in {time.time() - t:.1f}s")
+    s = felt_amd.Surface(3, L, (size,) * 3, (part,) * 3)
+    s.load_dense(phi)
+    del phi
+    polys = felt_amd.Polys(s)
+    polys.invalidate()
+    polys.march()
+    sp = speed_noise_curv(-0.3, 0.1, 0.5, 0xFE1702, 64)
+    for i in range(args.warmup):
+        s.update(sp)
+        polys.notify()
+    s.sync()
+    lat, ups, march_s, spx = [], 0, 0.0, 0
+    t_all = time.perf_counter()
+    for i in range(args.single_steps):
+        t0 = time.perf_counter()
+        s.update(sp)
+        polys.notify()
+        ups += s.step_info()[0]
+        if (i + 1) % 10 == 0:
+            tm = time.perf_counter()
+            polys.march()
+            march_s += time.perf_counter() - tm
+            spx += sum(polys.child_info(int(p))[2] for p in polys.changes())
+        lat.append(time.perf_counter() - t0)
+    s.sync()
+    total = time.perf_counter() - t_all
+    ktab, roof = time_passes(s, args.pass_reps)
+    res = {"workload": f"single large surface, plain GPU path: Surface<3,3> {size}^3 isogrid in "
+                       f"{part}^3 partitions, harmonic-perturbed sphere r={150.0 * size / 512:g}, "
+                       "speed clamp(-0.3(1+0.5n)+0.1curv), incremental marching cubes every 10 "
+                       "steps",
+           "value": ups / total, "steps": args.single_steps,
+           "ms_per_step": total / args.single_steps * 1e3,
+           "p50_step_ms": pctl(lat, 50), "p99_step_ms": pctl(lat, 99),
+           "triangles_per_s": spx / march_s if march_s > 0 else None,
+           "band_voxels_final": int(s.stats().band_voxels),
+           "roofline": roof, "kernels": ktab}
+    polys.close()
     s.close()
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(phi, args.size, args.part, L, args.cpu_seconds)
-    print(json.dumps(result), flush=True)
+    log(f"[bench] single: {res['value'] / 1e9:.2f} G voxel updates/s")
+    return res
 
 
-def cpu_baseline(phi, size, part, L, budget_s):
-    """The CPU restatement (oracle/, OpenMP over partitions like the reference) on the same
-    initial state, for a bounded number of steps."""
+def cpu_baseline(specs, budget_s):
+    """The CPU restatement (oracle/, OpenMP over partitions like the reference) stepping the
+    mixed workload's surfaces one after another from their initial state, until the time
+    budget is spent."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
+    from felt_amd import workloads
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -184,24 +220,77 @@ def cpu_baseline(phi, size, part, L, budget_s):
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
     b = oracle_lib.binding()
     b.lib.felt_oracle_set_threads(cores)
-    c = oracle_lib.surface(3, L, (size,) * 3, (part,) * 3)
-    c.b.lib.felt_oracle_surface_set_omp(c.h, 1)
-    t = time.time()
-    c.load_dense(phi)
-    log(f"[bench] oracle loaded in {time.time() - t:.1f}s")
-    sp = speed()
     done, ups, el = 0, 0, 0.0
-    while el < budget_s and done < 50:
+    for sp in specs:
+        if el >= budget_s:
+            break
+        c = oracle_lib.surface(3, 3, sp.size, sp.part)
+        c.b.lib.felt_oracle_surface_set_omp(c.h, 1)
+        c.load_dense(sp.phi())
+        f = workloads.speed_of(sp.speed)
         t0 = time.perf_counter()
-        c.update(sp)
+        c.update(f)
         el += time.perf_counter() - t0
         ups += int(c.stats().last_voxel_updates)
         done += 1
-        log(f"[bench] cpu step {done}: {el:.1f}s")
+        c.close()
+    log(f"[bench] cpu: {done} surfaces stepped in {el:.1f}s")
     return {"value": ups / el, "unit": "voxel updates/s", "cores": cores, "kind": "port",
-            "sample": f"first {done} update steps of the same {size}^3 workload from the same "
-                      f"initial state ({el:.1f}s of CPU work), CPU restatement with OpenMP "
-                      "over partitions"}
+            "sample": f"one update step of each of the first {done} surfaces of the same mixed "
+                      f"workload from the same initial state ({el:.1f}s of CPU work), CPU "
+                      "restatement with OpenMP over partitions"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--surfaces", type=int, default=256)
+    ap.add_argument("--single-steps", type=int, default=60)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--part", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--pass-reps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs its surfaces on a single MI355X and "
+              "has no multi-GPU path (DESIGN.md describes how it would shard).", file=sys.stderr)
+        sys.exit(2)
+    import felt_amd
+    felt_amd.lib()
+    specs, batched, plain, ktab, roof, cfg = run_mixed(args)
+    log(f"[bench] batched {batched['value'] / 1e9:.2f} G/s, plain {plain['value'] / 1e9:.2f} G/s")
+    result = {
+        "metric": METRIC,
+        "value": batched["value"],
+        "unit": "voxel updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": batched["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": cfg,
+        "p50_step_ms": batched["p50_step_ms"],
+        "p99_step_ms": batched["p99_step_ms"],
+        "voxel_updates_per_step": batched["voxel_updates_per_step"],
+        "plain": plain,
+        "batched_vs_plain": batched["value"] / plain["value"],
+        "roofline": roof,
+        "kernels": ktab,
+    }
+    if not args.no_single:
+        result["single_surface"] = run_single(args)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(specs, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -128,6 +128,8 @@ class Binding:
             "surface_sync": (C.c_int, [_P]),
             "surface_step_info": (C.c_int, [_P, _U64P, _U64P]),
             "surface_set_graph": (C.c_int, [_P, C.c_int]),
+            "surface_load_spheres": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double)]),
             "surface_time_pass": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.POINTER(C.c_int)]),
             "polys_notify": (C.c_int, [_P]),
@@ -153,7 +155,8 @@ class Binding:
         for name, (res, args) in sig.items():
             if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info",
                                                             "surface_set_graph",
-                                                            "surface_time_pass"):
+                                                            "surface_time_pass",
+                                                            "surface_load_spheres"):
                 continue
             f = getattr(L, prefix + name)
             f.restype = res
@@ -233,6 +236,13 @@ class Surface:
     def load_dense(self, phi: np.ndarray) -> None:
         a = np.ascontiguousarray(phi, dtype=np.float32).reshape(-1)
         self.b.call("surface_load_dense", self.h, _ptr(a, _F32P), a.size)
+
+    def load_spheres(self, centres, radii) -> None:
+        """Device-side load of min_k(|x - c_k| - r_k) (felt_surface_load_spheres)."""
+        c = np.ascontiguousarray(np.asarray(centres, dtype=np.float64).reshape(-1))
+        r = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).reshape(-1))
+        self.b.call("surface_load_spheres", self.h, int(r.size),
+                    c.ctypes.data_as(C.POINTER(C.c_double)), r.ctypes.data_as(C.POINTER(C.c_double)))
 
     def update(self, fn) -> None:
         """``update(fn)``: fn is a felt_speed descriptor (device functor) or a Python callable

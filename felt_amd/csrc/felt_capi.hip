@@ -118,6 +118,7 @@ int make_pool(int D, int L, const int32_t* part, int n, const int32_t* sizes, ui
         np_total += np;
         if (np_total > (1 << 30)) return felt_fail(FELT_E_ARG, "too many partitions");
     }
+    if (np_total * G.NL >= ((int64_t)1 << 31)) return felt_fail(FELT_E_ARG, "too many partitions");
     G.np = (int)np_total;
     int64_t cap = np_total;
     const int64_t id_limit = (int64_t)0xffffffffll / pv;
@@ -294,6 +295,22 @@ API int felt_surface_load_dense(felt_surface* s, const float* phi, size_t n) {
     CHECK_S(s);
     if (n != grid_total(s, 0) || !phi) return felt_fail(FELT_E_ARG, "load_dense: size mismatch");
     return felt_load_dense_impl(s, 0, phi, n);
+}
+
+static int load_spheres_m(felt_surface* s, int m, int32_t n, const double* centres,
+                          const double* radii) {
+    if (n < 1 || !centres || !radii) return felt_fail(FELT_E_ARG, "need at least one sphere");
+    const int D = s->G.D;
+    std::vector<double> c3((size_t)n * 3, 0.0);
+    for (int k = 0; k < n; k++)
+        for (int a = 0; a < D; a++) c3[(size_t)k * 3 + a] = centres[(size_t)k * D + a];
+    return felt_load_spheres_impl(s, m, n, c3.data(), radii);
+}
+
+API int felt_surface_load_spheres(felt_surface* s, int32_t n, const double* centres,
+                                  const double* radii) {
+    CHECK_S(s);
+    return load_spheres_m(s, 0, n, centres, radii);
 }
 
 API int felt_surface_update(felt_surface* s, const felt_speed* fn) {
@@ -675,12 +692,7 @@ API int felt_batch_load_spheres(felt_batch* b, int32_t i, int32_t n, const doubl
                                 const double* radii) {
     CHECK_B(b);
     CHECK_M(b, i);
-    if (n < 1 || !centres || !radii) return felt_fail(FELT_E_ARG, "need at least one sphere");
-    const int D = b->s->G.D;
-    std::vector<double> c3((size_t)n * 3, 0.0);
-    for (int k = 0; k < n; k++)
-        for (int a = 0; a < D; a++) c3[(size_t)k * 3 + a] = centres[(size_t)k * D + a];
-    return felt_load_spheres_impl(b->s, i, n, c3.data(), radii);
+    return load_spheres_m(b->s, i, n, centres, radii);
 }
 
 API int felt_batch_update(felt_batch* b, const felt_speed* speeds) {

@@ -119,45 +119,51 @@ __global__ void k_flush_reset(Ctr* c, const int* live) {
     }
 }
 
-// ---- band compaction (one block per partition; inactive partitions count nothing) -----------
+// ---- band compaction ---------------------------------------------------------------------
+// The band list is layer-major: segment k holds the layer k-L voxels, partitions in index
+// order within a segment.  Entry e = k*np + p of that order holds pcnt[p*NL + k] voxels, so
+// the segment offsets are an exclusive scan of the per-partition layer counts, and a block
+// per active partition then writes its voxels.
 
-template <int D>
-__global__ void k_band_count(Geo G, Pool P, int* counts) {
-    __shared__ int c[32];
-    const int p = blockIdx.x;
-    const int s = P.page[p];
-    if (threadIdx.x < 32) c[threadIdx.x] = 0;
-    __syncthreads();
-    if (s >= 0) {
-        const int8_t* lay = P.lay + (size_t)s * G.pv;
-        for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
-            const int k = lay[i];
-            if (k != NONE) atomicAdd(&c[k + G.L], 1);
+constexpr int SCAN_ITEMS = 8;  // consecutive scan entries per thread
+
+// Sum of this thread's SCAN_ITEMS consecutive entries starting at e0.
+__device__ __forceinline__ int bscan_sum(const int* pcnt, int np, int NL, int e0, int n) {
+    int k = e0 / np, p = e0 - k * np, sum = 0;
+    for (int j = 0; j < SCAN_ITEMS && e0 + j < n; j++) {
+        sum += pcnt[p * NL + k];
+        if (++p == np) {
+            p = 0;
+            k++;
         }
     }
-    __syncthreads();
-    if (threadIdx.x < G.NL) counts[p * G.NL + threadIdx.x] = c[threadIdx.x];
+    return sum;
 }
 
-// Single block: layer-major exclusive scan of counts[n][NL] into offsets; base[k] is where
-// layer k's segment starts and base[NL] the band total (also the band span's end).
-__global__ void k_band_scan(const int* counts, int n, int NL, int* offsets, int* base, Ctr* c) {
+// Pass 1: per block the total of its BT * SCAN_ITEMS entries.
+__global__ void k_bscan_blocks(const int* pcnt, int np, int NL, int* bs) {
+    const int n = np * NL;
+    const int e0 = (blockIdx.x * BT + threadIdx.x) * SCAN_ITEMS;
+    const int sum = e0 < n ? bscan_sum(pcnt, np, NL, e0, n) : 0;
+    int tot;
+    block_excl_scan(sum, &tot);
+    if (threadIdx.x == 0) bs[blockIdx.x] = tot;
+}
+
+// Pass 2 (single block): exclusive scan of the block totals in place; the total is the band
+// size (base[NL] and the band span's end).
+__global__ void k_bscan_top(int* bs, int nb, int NL, int* base, Ctr* c) {
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int k = 0; k < NL; k++) {
-        if (threadIdx.x == 0) base[k] = carry;
+    for (int b0 = 0; b0 < nb; b0 += BT) {
+        const int b = b0 + threadIdx.x;
+        int tot;
+        const int ex = block_excl_scan(b < nb ? bs[b] : 0, &tot);
+        if (b < nb) bs[b] = carry + ex;
         __syncthreads();
-        for (int b0 = 0; b0 < n; b0 += BT) {
-            const int b = b0 + threadIdx.x;
-            const int v = b < n ? counts[b * NL + k] : 0;
-            int tot;
-            const int ex = block_excl_scan(v, &tot);
-            if (b < n) offsets[b * NL + k] = carry + ex;
-            __syncthreads();
-            if (threadIdx.x == 0) carry += tot;
-            __syncthreads();
-        }
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
         base[NL] = carry;
@@ -166,21 +172,53 @@ __global__ void k_band_scan(const int* counts, int n, int NL, int* offsets, int*
     }
 }
 
+// Pass 3: each entry's offset; the first entry of each layer gives that segment's start.
+__global__ void k_bscan_apply(const int* pcnt, int np, int NL, const int* bs, int* offsets,
+                              int* base) {
+    const int n = np * NL;
+    const int e0 = (blockIdx.x * BT + threadIdx.x) * SCAN_ITEMS;
+    const int sum = e0 < n ? bscan_sum(pcnt, np, NL, e0, n) : 0;
+    int tot;
+    int at = block_excl_scan(sum, &tot) + bs[blockIdx.x];
+    if (e0 >= n) return;
+    int k = e0 / np, p = e0 - k * np;
+    for (int j = 0; j < SCAN_ITEMS && e0 + j < n; j++) {
+        if (p == 0) base[k] = at;
+        offsets[p * NL + k] = at;
+        at += pcnt[p * NL + k];
+        if (++p == np) {
+            p = 0;
+            k++;
+        }
+    }
+}
+
+// Pass 4 (one block per partition): the partition's band voxels into their segments.  A
+// voxel beyond its partition's count (layers and counts disagreeing) is not written and is
+// reported through err.
 template <int D>
-__global__ void k_band_scatter(Geo G, Pool P, const int* offsets, uint32_t* band, int8_t* tag) {
-    __shared__ int c[32];
+__global__ void k_band_scatter(Geo G, Pool P, const int* offsets, uint32_t* band, int8_t* tag,
+                               Ctr* ctr) {
+    __shared__ int c[32], e[32];
     const int p = blockIdx.x;
     const int s = P.page[p];
     if (s < 0) return;
-    if (threadIdx.x < G.NL) c[threadIdx.x] = offsets[p * G.NL + threadIdx.x];
+    if (threadIdx.x < G.NL) {
+        c[threadIdx.x] = offsets[p * G.NL + threadIdx.x];
+        e[threadIdx.x] = c[threadIdx.x] + P.pcnt[p * G.NL + threadIdx.x];
+    }
     __syncthreads();
     const int8_t* lay = P.lay + (size_t)s * G.pv;
     for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
         const int k = lay[i];
         if (k != NONE) {
             const int at = atomicAdd(&c[k + G.L], 1);
-            band[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)i;
-            tag[at] = (int8_t)k;
+            if (at < e[k + G.L]) {
+                band[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)i;
+                tag[at] = (int8_t)k;
+            } else {
+                ctr->err = 2;
+            }
         }
     }
 }

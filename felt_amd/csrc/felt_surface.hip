@@ -552,7 +552,8 @@ static int read_ctr(felt_surface* s) {
                             hipMemcpyDeviceToHost, s->st));
     }
     HIPX(hipStreamSynchronize(s->st));
-    if (s->hc.err) return felt_fail(FELT_E_NOMEM, "brick pool exhausted");
+    if (s->hc.err == 1) return felt_fail(FELT_E_NOMEM, "brick pool exhausted");
+    if (s->hc.err) return felt_fail(FELT_E_STATE, "layer counts and layer lists disagree");
     return FELT_OK;
 }
 
@@ -714,16 +715,23 @@ static int chg_blocks(felt_surface* s) {
     return (int)((s->list_cap + BT * ITEMS - 1) / (BT * ITEMS));
 }
 
-// Band compaction into per-layer segments [bbase[k], bbase[k+1]) — one block per partition.
+// Band compaction into per-layer segments [bbase[k], bbase[k+1]): scan of the partition
+// layer counts, then one block per partition.
+static int bscan_blocks(const Geo& G) {
+    return (int)(((int64_t)G.np * G.NL + (int64_t)BT * SCAN_ITEMS - 1) / ((int64_t)BT * SCAN_ITEMS));
+}
+
 static void compact_band(felt_surface* s) {
     const Geo& G = s->G;
     const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
-    PDISPATCH(PK_COMPACT, (long long)G.np * G.pv, G.D, k_band_count, dim3(G.np), dim3(bt), G,
-              s->P, s->blk_cnt.ptr);
-    LAUNCH(k_band_scan, dim3(1), dim3(BT), s->blk_cnt.ptr, G.np, G.NL, s->blk_off.ptr, s->bbase,
-           s->ctr);
+    const int nb = bscan_blocks(G);
+    PLAUNCH(PK_COMPACT, (long long)G.np * G.NL, k_bscan_blocks, dim3(nb), dim3(BT), s->P.pcnt,
+            G.np, G.NL, s->blk_cnt.ptr);
+    LAUNCH(k_bscan_top, dim3(1), dim3(BT), s->blk_cnt.ptr, nb, G.NL, s->bbase, s->ctr);
+    LAUNCH(k_bscan_apply, dim3(nb), dim3(BT), s->P.pcnt, G.np, G.NL, s->blk_cnt.ptr,
+           s->blk_off.ptr, s->bbase);
     DISPATCH(G.D, k_band_scatter, dim3(G.np), dim3(bt), G, s->P, s->blk_off.ptr, s->band.ptr,
-             s->band_tag.ptr);
+             s->band_tag.ptr, s->ctr);
 }
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
@@ -1006,7 +1014,7 @@ int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double*
             }
         } else {
             compact_band(s);
-            n = 3;
+            n = 4;
             entries = (long long)G.np * G.pv;
         }
     }

@@ -101,6 +101,12 @@ int felt_surface_seed(felt_surface* s, const int32_t* pos);
  * -(L+1) if their first voxel is negative, else L+1. */
 int felt_surface_load_dense(felt_surface* s, const float* phi, size_t n);
 
+/* Load the field min_k(|x - c_k| - r_k) over n spheres (world centres, D per sphere),
+ * evaluated on the device in fp64, clamped to +-(L+1) and rounded to fp32, exactly as
+ * felt_surface_load_dense of that dense field.  Not a reference member (workload setup). */
+int felt_surface_load_spheres(felt_surface* s, int32_t n, const double* centres,
+                              const double* radii);
+
 /* ---- Stepping ----------------------------------------------------------------------- */
 /* update(fn) — global step (Surface.hpp:254-296, 739-762). */
 int felt_surface_update(felt_surface* s, const felt_speed* fn);
@@ -161,7 +167,7 @@ int felt_surface_profile_read(felt_surface* s, int kernel, uint64_t* launches, d
  * stream.  Passes:
  * FELT_PASS_SPEED, one launch of the last global update's device functor over the zero layer;
  * FELT_PASS_DIST1, the first-round distance pass 1, one launch per non-zero layer;
- * FELT_PASS_BAND, band compaction (three kernels).  They write only scratch (deltas, distances,
+ * FELT_PASS_BAND, band compaction (four kernels).  They write only scratch (deltas, distances,
  * the band list), so the surface is unchanged.  Outputs: microseconds per launch, list entries
  * per launch (for FELT_PASS_BAND the pool voxels scanned), launches per replay.  Not a
  * reference member. */
