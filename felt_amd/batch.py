@@ -83,12 +83,16 @@ class Batch:
     """n independent surfaces sharing D, L and the partition size."""
 
     def __init__(self, lib: C.CDLL, D: int, L: int, sizes: Sequence[Sequence[int]],
-                 part: Sequence[int], max_bricks: int = 0):
+                 part: Sequence[int], max_bricks: int = 0, handle=None):
         self.api = _Api(lib)
         self.D, self.L = D, L
         self.sizes = [tuple(int(x) for x in s) for s in sizes]
         self.part = tuple(int(x) for x in part)
         self.n = len(self.sizes)
+        self.own = handle is None
+        if handle is not None:  # borrowed from a server
+            self.h = handle
+            return
         h = _P()
         sz = _arr(self.sizes, np.int32)
         pt = _arr(self.part, np.int32)
@@ -97,9 +101,9 @@ class Batch:
         self.h = h
 
     def close(self):
-        if self.h:
+        if self.h and self.own:
             self.api("felt_batch_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -193,17 +197,21 @@ class Batch:
 class BatchPolys:
     """Polygonisation of every member of a batch (Polys.hpp:29-201 per member)."""
 
-    def __init__(self, batch: Batch):
+    def __init__(self, batch: Batch, handle=None):
         self.bt = batch
         self.api = batch.api
+        self.own = handle is None
+        if handle is not None:  # borrowed from a server
+            self.h = handle
+            return
         h = _P()
         self.api("felt_batch_polys_create", batch.h, C.byref(h))
         self.h = h
 
     def close(self):
-        if self.h:
+        if self.h and self.own:
             self.api("felt_polys_destroy", self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

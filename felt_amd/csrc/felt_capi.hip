@@ -627,10 +627,6 @@ API int felt_polys_totals(felt_polys* q, uint64_t* nv, uint64_t* ns) {
 
 // ---- batched surfaces -------------------------------------------------------------------
 
-struct felt_batch {
-    felt_surface* s = nullptr;
-};
-
 #define CHECK_B(b) \
     if (!(b) || !(b)->s) return felt_fail(FELT_E_ARG, "null batch handle")
 #define CHECK_M(b, i) \

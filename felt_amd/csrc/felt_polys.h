@@ -38,6 +38,7 @@ struct felt_polys {
     DBuf<int> vbase, tbase, tot, dparts, pstart, vst, item_j, item_l, nv, nt, vleaf;
     DBuf<long long> dseg;
     DBuf<uint8_t> bits;
+    DBuf<int> plist;  // partition lists exchanged with the host
 };
 
 int felt_polys_init(felt_polys* q);

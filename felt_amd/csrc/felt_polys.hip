@@ -524,19 +524,33 @@ __global__ void k_copy_segments_u(const long long* src, const long long* dst,
         for (long long i = threadIdx.x; i < len[j]; i += blockDim.x) into[dst[j] + i] = from[src[j] + i];
 }
 
-// Per partition: bit 0 in delta(0)|delta(2L), bit 1 in status_change(0)|status_change(2L),
-// bit 2 tracked by the isogrid in list 0 or 2L, bit 3 isogrid partition active.
-__global__ void k_notify_bits(Geo G, Pool P, const unsigned* dmask, const unsigned* smask,
-                              uint8_t* out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= G.np) return;
-    const unsigned outer = 1u | (1u << (G.NL - 1));
-    uint8_t b = 0;
-    if (dmask[p] & outer) b |= 1;
-    if (smask[p] & outer) b |= 2;
-    if (P.pcnt[p * G.NL] > 0 || P.pcnt[p * G.NL + G.NL - 1] > 0) b |= 4;
-    if (P.page[p] >= 0) b |= 8;
-    out[p] = b;
+// Partitions whose bits meet `want`, as (partition, bits) pairs: bit 0 in
+// delta(0)|delta(2L), bit 1 in status_change(0)|status_change(2L), bit 2 tracked by the
+// isogrid in list 0 or 2L.
+__global__ void k_part_bits(Geo G, Pool P, const unsigned* dmask, const unsigned* smask,
+                            unsigned want, int* out, int* cnt) {
+    for (int b0 = blockIdx.x * blockDim.x; b0 < G.np; b0 += gridDim.x * blockDim.x) {
+        const int p = b0 + threadIdx.x;
+        unsigned b = 0;
+        if (p < G.np) {
+            const unsigned outer = 1u | (1u << (G.NL - 1));
+            if (dmask[p] & outer) b |= 1;
+            if (smask[p] & outer) b |= 2;
+            if (P.pcnt[p * G.NL] > 0 || P.pcnt[p * G.NL + G.NL - 1] > 0) b |= 4;
+        }
+        const bool take = (b & want) != 0;
+        const int at = wave_append(cnt, take);
+        if (take) {
+            out[2 * at] = p;
+            out[2 * at + 1] = (int)b;
+        }
+    }
+}
+
+// Whether each listed partition is active in the isogrid.
+__global__ void k_gather_active(Pool P, const int* parts, int n, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = P.page[parts[i]] >= 0;
 }
 
 }  // namespace
@@ -611,6 +625,7 @@ void felt_polys_release(felt_polys* q) {
     q->dparts.release();
     q->dseg.release();
     q->bits.release();
+    q->plist.release();
     q->pstart.release();
     q->vst.release();
     q->item_j.release();
@@ -620,44 +635,56 @@ void felt_polys_release(felt_polys* q) {
     q->vleaf.release();
 }
 
-// notify (Polys.hpp:92-133).
-int felt_polys_notify_impl(felt_polys* q) {
+// Partitions whose notify bits meet `want`, sorted by partition index.
+static int part_bits(felt_polys* q, unsigned want, std::vector<std::pair<int, int>>& out) {
     felt_surface* s = q->s;
     const Geo& G = s->G;
     hipStream_t st = s->st;
     int rc;
-    if ((rc = q->bits.ensure(G.np))) return rc;
-    hipLaunchKernelGGL(k_notify_bits, dim3((G.np + TPB - 1) / TPB), dim3(TPB), 0, st, G, s->P,
-                       s->dmask, s->smask, q->bits.ptr);
-    std::vector<uint8_t> bits(G.np);
-    HIPX(hipMemcpyAsync(bits.data(), q->bits.ptr, G.np, hipMemcpyDeviceToHost, st));
+    if ((rc = q->plist.ensure(2 * (size_t)G.np + 2))) return rc;
+    int* cnt = q->plist.ptr + 2 * (size_t)G.np;
+    HIPX(hipMemsetAsync(cnt, 0, sizeof(int), st));
+    const int grid = (int)std::min<int64_t>(4096, ((int64_t)G.np + TPB - 1) / TPB);
+    hipLaunchKernelGGL(k_part_bits, dim3(grid), dim3(TPB), 0, st, G, s->P, s->dmask, s->smask,
+                       want, q->plist.ptr, cnt);
+    int n = 0;
+    HIPX(hipMemcpyAsync(&n, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPX(hipStreamSynchronize(st));
-    for (int p = 0; p < G.np; p++) {
-        if (bits[p] & 1) {
-            const bool act = q->kids[p].active || (bits[p] & 4);
+    std::vector<int> raw(2 * (size_t)n);
+    if (n) {
+        HIPX(hipMemcpyAsync(raw.data(), q->plist.ptr, raw.size() * sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+    }
+    out.resize(n);
+    for (int i = 0; i < n; i++) out[i] = {raw[2 * i], raw[2 * i + 1]};
+    std::sort(out.begin(), out.end());
+    return FELT_OK;
+}
+
+// notify (Polys.hpp:92-133).
+int felt_polys_notify_impl(felt_polys* q) {
+    std::vector<std::pair<int, int>> l;
+    if (int rc = part_bits(q, 3u, l)) return rc;
+    for (const auto& e : l) {
+        const int p = e.first, bits = e.second;
+        if (bits & 1) {
+            const bool act = q->kids[p].active || (bits & 4);
             if (act) q->pending->track(p);
             else q->pending->untrack(p);
         }
     }
-    for (int p = 0; p < G.np; p++)
-        if ((bits[p] & 2) && q->kids[p].active) q->pending->track(p);
+    for (const auto& e : l)
+        if ((e.second & 2) && q->kids[e.first].active) q->pending->track(e.first);
     return FELT_OK;
 }
 
 // invalidate (Polys.hpp:171-191): the isogrid's outermost-layer partitions.
 int felt_polys_invalidate_impl(felt_polys* q) {
-    felt_surface* s = q->s;
-    const Geo& G = s->G;
-    int rc;
-    if ((rc = q->bits.ensure(G.np))) return rc;
-    hipLaunchKernelGGL(k_notify_bits, dim3((G.np + TPB - 1) / TPB), dim3(TPB), 0, s->st, G, s->P,
-                       s->dmask, s->smask, q->bits.ptr);
-    std::vector<uint8_t> bits(G.np);
-    HIPX(hipMemcpyAsync(bits.data(), q->bits.ptr, G.np, hipMemcpyDeviceToHost, s->st));
-    HIPX(hipStreamSynchronize(s->st));
+    std::vector<std::pair<int, int>> l;
+    if (int rc = part_bits(q, 4u, l)) return rc;
     q->pending->reset();
-    for (int p = 0; p < G.np; p++)
-        if (bits[p] & 4) q->pending->track(p);
+    for (const auto& e : l) q->pending->track(e.first);
     return FELT_OK;
 }
 
@@ -737,13 +764,24 @@ int felt_polys_march_impl(felt_polys* q) {
     const Geo& G = s->G;
     hipStream_t st = s->st;
     int rc;
-    std::vector<int> active(G.np);
-    HIPX(hipMemcpyAsync(active.data(), s->P.page, G.np * sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPX(hipStreamSynchronize(st));
+    const std::vector<int64_t>& pend = q->pending->list;
+    const size_t npend = pend.size();
+    std::vector<int> pl(pend.begin(), pend.end()), active(npend);
+    if (npend) {
+        if ((rc = q->plist.ensure(2 * npend + 2))) return rc;
+        HIPX(hipMemcpyAsync(q->plist.ptr, pl.data(), npend * sizeof(int), hipMemcpyHostToDevice,
+                            st));
+        hipLaunchKernelGGL(k_gather_active, dim3((unsigned)((npend + TPB - 1) / TPB)), dim3(TPB),
+                           0, st, s->P, q->plist.ptr, (int)npend, q->plist.ptr + npend);
+        HIPX(hipMemcpyAsync(active.data(), q->plist.ptr + npend, npend * sizeof(int),
+                            hipMemcpyDeviceToHost, st));
+        HIPX(hipStreamSynchronize(st));
+    }
     std::vector<int> todo;
-    for (int64_t p : q->pending->list) {
+    for (size_t i = 0; i < npend; i++) {
+        const int64_t p = pend[i];
         PolyKid& k = q->kids[p];
-        if (active[p] >= 0) {
+        if (active[i]) {
             k.active = true;
             k.nv = k.nt = 0;
             todo.push_back((int)p);

@@ -1,0 +1,279 @@
+// felt_server.hip — a request loop over one batch of resident surfaces (gfx950).
+//
+// Requests (surface, box, speed, steps) are queued as they arrive.  Each tick admits queued
+// requests in arrival order while three limits hold: one request in flight per surface, at
+// most `max_active` in flight, and the step lists the in-flight set needs fit the HBM budget
+// beside the resident bricks.  Then every in-flight request advances by one bounded update
+// (Surface::update(lo, hi, fn), Surface.hpp:310-386) — all of them in one batched step over
+// the shared brick pool — followed by repolygonisation of the changed partitions
+// (Polys::notify / march, Polys.hpp:92-165).  Requests that have run their steps complete and
+// free their slot for the next tick, so work that arrives while others run joins at the next
+// step (continuous batching).  No host code touches voxel data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <vector>
+
+#include "../../include/felt_amd.h"
+#include "felt_polys.h"
+#include "felt_surface.h"
+
+using namespace felt;
+
+#define API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+// Bytes of step lists per listed voxel: band (index, delta, tag, new layer), two lookups,
+// status-change records, zero-layer list.
+constexpr uint64_t kListBytes = 10 + 2 * 10 + 7 + 8;
+
+double now_s() {
+    using clk = std::chrono::steady_clock;
+    return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = q / 100.0 * (double)(v.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - (double)i;
+    return i + 1 < v.size() ? v[i] * (1.0 - f) + v[i + 1] * f : v[i];
+}
+
+struct Req {
+    felt_request r;
+    double t_submit;
+    int left;
+};
+
+}  // namespace
+
+struct felt_server {
+    felt_batch* batch = nullptr;
+    felt_surface* s = nullptr;
+    felt_polys* polys = nullptr;
+    uint64_t budget = 0;
+    int max_active = 0;
+    std::deque<Req> queue;
+    std::vector<Req> active;
+    std::vector<char> busy;            // a request of this surface is in flight
+    std::vector<long long> band;       // tracked voxels per surface, refreshed after its steps
+    std::vector<uint64_t> done;        // completed request ids not yet polled
+    std::vector<double> tick_ms, req_ms;
+    uint64_t ticks = 0, completed = 0, deferred = 0, voxel_updates = 0;
+    double busy_s = 0.0;
+};
+
+namespace {
+
+int refresh_bands(felt_server* v, const std::vector<int>& ids) {
+    std::vector<long long> b;
+    if (int rc = felt_member_bands(v->s, ids.data(), (int)ids.size(), b)) return rc;
+    for (size_t j = 0; j < ids.size(); j++) v->band[ids[j]] = b[j];
+    return FELT_OK;
+}
+
+// Step-list bytes after a step of the in-flight set plus a surface of `extra` voxels (the
+// lists only grow).
+uint64_t list_bytes(const felt_server* v, long long extra) {
+    long long sum = extra;
+    for (const Req& q : v->active) sum += v->band[q.r.surface];
+    return (uint64_t)felt_list_cap_for(v->s, (size_t)sum) * kListBytes;
+}
+
+// Bytes of everything but the step lists.
+uint64_t resident_bytes(const felt_server* v) {
+    const felt_surface* s = v->s;
+    return felt_device_bytes(s) - (uint64_t)s->list_cap * kListBytes;
+}
+
+void admit(felt_server* v) {
+    const uint64_t fixed = resident_bytes(v);
+    for (auto it = v->queue.begin(); it != v->queue.end();) {
+        if ((int)v->active.size() >= v->max_active) break;
+        const int m = it->r.surface;
+        if (v->busy[m]) {
+            ++it;
+            continue;
+        }
+        if (!v->active.empty() && fixed + list_bytes(v, v->band[m]) > v->budget) {
+            v->deferred++;
+            break;  // keep arrival order: later requests wait too
+        }
+        v->busy[m] = 1;
+        v->active.push_back(*it);
+        it = v->queue.erase(it);
+    }
+}
+
+}  // namespace
+
+API int felt_server_create(int D, int L, const int32_t* part, int32_t n, const int32_t* sizes,
+                           uint64_t budget_bytes, int32_t max_active, int32_t polygonise,
+                           felt_server** out) {
+    if (!out || max_active < 1 || budget_bytes == 0)
+        return felt_fail(FELT_E_ARG, "bad server arguments");
+    // bricks take at most three quarters of the budget
+    uint64_t pv = 1;
+    for (int a = 0; a < D && part; a++) pv *= (uint64_t)std::max(1, part[a]);
+    const uint64_t brick_bytes = pv * (sizeof(float) + 2) + 32;
+    const uint64_t max_bricks = std::max<uint64_t>(1, budget_bytes / 4 * 3 / brick_bytes);
+    felt_server* v = new (std::nothrow) felt_server();
+    if (!v) return felt_fail(FELT_E_NOMEM, "host allocation failed");
+    if (int rc = felt_batch_create(D, L, part, n, sizes, max_bricks, &v->batch)) {
+        delete v;
+        return rc;
+    }
+    v->s = v->batch->s;
+    if (felt_device_bytes(v->s) > budget_bytes) {
+        felt_batch_destroy(v->batch);
+        delete v;
+        return felt_fail(FELT_E_NOMEM, "resident surfaces exceed the HBM budget");
+    }
+    if (polygonise) {
+        if (int rc = felt_batch_polys_create(v->batch, &v->polys)) {
+            felt_batch_destroy(v->batch);
+            delete v;
+            return rc;
+        }
+    }
+    v->budget = budget_bytes;
+    v->max_active = max_active;
+    v->busy.assign(n, 0);
+    v->band.assign(n, 0);
+    *out = v;
+    return FELT_OK;
+}
+
+API int felt_server_destroy(felt_server* v) {
+    if (!v) return FELT_OK;
+    if (v->polys) felt_polys_destroy(v->polys);
+    felt_batch_destroy(v->batch);
+    delete v;
+    return FELT_OK;
+}
+
+API int felt_server_batch(felt_server* v, felt_batch** out) {
+    if (!v || !out) return felt_fail(FELT_E_ARG, "null argument");
+    *out = v->batch;
+    return FELT_OK;
+}
+
+API int felt_server_polys(felt_server* v, felt_polys** out) {
+    if (!v || !out) return felt_fail(FELT_E_ARG, "null argument");
+    if (!v->polys) return felt_fail(FELT_E_STATE, "server created without polygonisation");
+    *out = v->polys;
+    return FELT_OK;
+}
+
+// Marks the surfaces' current bands known and, with polygonisation, marches every surface.
+API int felt_server_start(felt_server* v) {
+    if (!v) return felt_fail(FELT_E_ARG, "null server");
+    std::vector<int> all(v->s->G.nmem);
+    for (int i = 0; i < v->s->G.nmem; i++) all[i] = i;
+    if (int rc = refresh_bands(v, all)) return rc;
+    if (v->polys) {
+        if (int rc = felt_polys_invalidate_impl(v->polys)) return rc;
+        if (int rc = felt_polys_march_impl(v->polys)) return rc;
+    }
+    return FELT_OK;
+}
+
+API int felt_server_submit(felt_server* v, const felt_request* r, size_t n) {
+    if (!v || (n && !r)) return felt_fail(FELT_E_ARG, "null argument");
+    const double t = now_s();
+    for (size_t i = 0; i < n; i++) {
+        const felt_request& q = r[i];
+        if (q.surface < 0 || q.surface >= v->s->G.nmem)
+            return felt_fail(FELT_E_ARG, "request surface out of range");
+        if (q.steps < 1) return felt_fail(FELT_E_ARG, "request needs at least one step");
+        if (q.speed.kind < 0 || q.speed.kind > FELT_SPEED_NOISE_CURV ||
+            (q.speed.kind == FELT_SPEED_NOISE_CURV && q.speed.cell <= 0))
+            return felt_fail(FELT_E_ARG, "bad speed descriptor");
+        v->queue.push_back(Req{q, t, q.steps});
+    }
+    return FELT_OK;
+}
+
+API int felt_server_tick(felt_server* v, int32_t* stepped) {
+    if (!v) return felt_fail(FELT_E_ARG, "null server");
+    admit(v);
+    if (stepped) *stepped = (int32_t)v->active.size();
+    if (v->active.empty()) return FELT_OK;
+    const double t0 = now_s();
+    const int n = (int)v->active.size();
+    const int D = v->s->G.D;
+    std::vector<int> ids(n);
+    std::vector<int32_t> lo((size_t)n * 3, 0), hi((size_t)n * 3, 0);
+    std::vector<felt_speed> sp(n);
+    for (int j = 0; j < n; j++) {
+        const felt_request& q = v->active[j].r;
+        ids[j] = q.surface;
+        for (int a = 0; a < D; a++) {
+            lo[(size_t)j * 3 + a] = q.lo[a];
+            hi[(size_t)j * 3 + a] = q.hi[a];
+        }
+        sp[j] = q.speed;
+    }
+    int rc = felt_update_boxes_impl(v->s, n, ids.data(), lo.data(), hi.data(), sp.data());
+    if (!rc && v->polys) rc = felt_polys_notify_impl(v->polys);
+    if (!rc && v->polys) rc = felt_polys_march_impl(v->polys);
+    if (!rc) rc = refresh_bands(v, ids);
+    if (rc) return rc;
+    const double t1 = now_s();
+    v->tick_ms.push_back((t1 - t0) * 1e3);
+    v->busy_s += t1 - t0;
+    v->ticks++;
+    v->voxel_updates += v->s->last_updates;
+    std::vector<Req> keep;
+    keep.reserve(v->active.size());
+    for (Req& q : v->active) {
+        if (--q.left > 0) {
+            keep.push_back(q);
+        } else {
+            v->busy[q.r.surface] = 0;
+            v->done.push_back(q.r.id);
+            v->req_ms.push_back((t1 - q.t_submit) * 1e3);
+            v->completed++;
+        }
+    }
+    v->active.swap(keep);
+    return FELT_OK;
+}
+
+API int felt_server_poll(felt_server* v, uint64_t* ids, size_t cap, size_t* n) {
+    if (!v || !n) return felt_fail(FELT_E_ARG, "null argument");
+    const size_t k = std::min(cap, v->done.size());
+    if (k) std::copy(v->done.begin(), v->done.begin() + (long)k, ids);
+    v->done.erase(v->done.begin(), v->done.begin() + (long)k);
+    *n = k;
+    return FELT_OK;
+}
+
+API int felt_server_read_stats(felt_server* v, felt_server_stats* st) {
+    if (!v || !st) return felt_fail(FELT_E_ARG, "null argument");
+    std::memset(st, 0, sizeof(*st));
+    st->ticks = v->ticks;
+    st->requests_done = v->completed;
+    st->requests_queued = v->queue.size();
+    st->requests_active = v->active.size();
+    st->admissions_deferred = v->deferred;
+    st->voxel_updates = v->voxel_updates;
+    st->busy_s = v->busy_s;
+    st->step_p50_ms = percentile(v->tick_ms, 50);
+    st->step_p99_ms = percentile(v->tick_ms, 99);
+    st->request_p50_ms = percentile(v->req_ms, 50);
+    st->request_p99_ms = percentile(v->req_ms, 99);
+    if (int rc = felt_download(v->s, &v->s->hc, v->s->ctr, sizeof(Ctr))) return rc;
+    st->device_bytes = felt_device_bytes(v->s);
+    st->budget_bytes = v->budget;
+    st->bricks_total = (uint64_t)v->s->cap;
+    st->bricks_used = (uint64_t)(v->s->cap - v->s->hc.free_top);
+    return FELT_OK;
+}

@@ -101,6 +101,11 @@ struct felt_surface {
     Prof prof;
 };
 
+// A batch handle: a pool of member surfaces.
+struct felt_batch {
+    felt_surface* s = nullptr;
+};
+
 // Brackets one launch with events when profiling is on.
 void prof_begin(felt_surface* s, int kernel, long long items);
 void prof_end(felt_surface* s);
@@ -140,6 +145,10 @@ int felt_collect_layer_impl(felt_surface* s, int m, int layer, std::vector<int>&
 int felt_lu_positions_impl(felt_surface* s, int m, int lu_i, int tag, std::vector<int>& out);
 int felt_band_total(felt_surface* s, size_t* out);
 int felt_download(felt_surface* s, void* dst, const void* src, size_t bytes);
+// Tracked voxels of members ids[0..n).
+int felt_member_bands(felt_surface* s, const int* ids, int n, std::vector<long long>& out);
+// Capacity the step lists would grow to for a step over `band` listed voxels.
+size_t felt_list_cap_for(const felt_surface* s, size_t band);
 // Device bytes held by the handle.
 uint64_t felt_device_bytes(const felt_surface* s);
 int felt_sync_impl(felt_surface* s);

@@ -411,6 +411,24 @@ __global__ void k_set_span(int* span, int n) {
     span[1] = n;
 }
 
+// Tracked voxels of each listed member (one block per member).
+__global__ void k_member_band(Geo G, Pool P, const int* ids, long long* out) {
+    const Mem& M = P.mem[ids[blockIdx.x]];
+    const int64_t n = (int64_t)M.np * G.NL;
+    const int* c = P.pcnt + (int64_t)M.pbase * G.NL;
+    long long s = 0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += c[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    __shared__ long long part[16];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        out[blockIdx.x] = t;
+    }
+}
+
 __global__ void k_band_total(Geo G, Pool P, int* out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int s = 0;
@@ -683,9 +701,14 @@ static const int* always(felt_surface* s) { return &s->ctr->one; }
 
 // Capacity of every per-step list: no list in a step holds more entries than the band had at
 // its start.  Growth changes buffer addresses, so a captured step graph is rebuilt.
+size_t felt_list_cap_for(const felt_surface* s, size_t band) {
+    if (band + 64 <= s->list_cap) return s->list_cap;
+    return std::max<size_t>(band + band / 4 + 1024, s->list_cap + s->list_cap / 2);
+}
+
 static int ensure_step_capacity(felt_surface* s, size_t band) {
     if (band + 64 <= s->list_cap) return FELT_OK;
-    const size_t n = std::max<size_t>(band + band / 4 + 1024, s->list_cap + s->list_cap / 2);
+    const size_t n = felt_list_cap_for(s, band);
     int rc;
     for (int i = 0; i < 2; i++) {
         if ((rc = s->lu_v[i].ensure(n))) return rc;
@@ -1144,7 +1167,16 @@ int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t
                            const int32_t* hi, const felt_speed* fns) {
     const Geo& G = s->G;
     int rc;
-    if ((rc = size_for_band(s))) return rc;
+    if (G.nmem == 1) {
+        if ((rc = size_for_band(s))) return rc;
+    } else {
+        // lists of a bounded step hold at most the stepped members' bands
+        std::vector<long long> bands;
+        if ((rc = felt_member_bands(s, ids, n, bands))) return rc;
+        long long sum = 0;
+        for (long long b : bands) sum += b;
+        if ((rc = ensure_step_capacity(s, (size_t)sum))) return rc;
+    }
     if ((rc = felt_update_start_impl(s))) return rc;
     std::vector<int> parts, pbox, boxes;
     for (int j = 0; j < n; j++) {
@@ -1409,6 +1441,20 @@ int felt_lu_positions_impl(felt_surface* s, int m, int lu_i, int tag, std::vecto
 
 int felt_download(felt_surface* s, void* dst, const void* src, size_t bytes) {
     HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
+}
+
+int felt_member_bands(felt_surface* s, const int* ids, int n, std::vector<long long>& out) {
+    out.assign(n, 0);
+    if (n == 0) return FELT_OK;
+    int rc;
+    if ((rc = s->tmp_i.ensure((size_t)n + 1))) return rc;
+    if ((rc = s->tmp_u.ensure(2 * (size_t)n + 2))) return rc;
+    long long* d = reinterpret_cast<long long*>(s->tmp_u.ptr);
+    HIPX(hipMemcpyAsync(s->tmp_i.ptr, ids, n * sizeof(int), hipMemcpyHostToDevice, s->st));
+    LAUNCH(k_member_band, dim3(n), dim3(TPB), s->G, s->P, s->tmp_i.ptr, d);
+    HIPX(hipMemcpyAsync(out.data(), d, n * sizeof(long long), hipMemcpyDeviceToHost, s->st));
     HIPX(hipStreamSynchronize(s->st));
     return FELT_OK;
 }

@@ -251,6 +251,53 @@ int felt_batch_polys_child_info(felt_polys* p, int32_t i, uint64_t pos_idx, int*
 int felt_batch_polys_child_mesh(felt_polys* p, int32_t i, uint64_t pos_idx, float* vtx,
                                 size_t vtx_cap, uint32_t* spx, size_t spx_cap);
 
+/* ---- Served surfaces ---------------------------------------------------------------- */
+/* Not a reference type.  A server keeps n surfaces (one batch) resident and runs requests
+ * against them: each request is `steps` bounded updates update(lo, hi, fn) of one surface
+ * (Surface.hpp:310-386), each followed by repolygonisation of the changed partitions
+ * (Polys.hpp:92-165) when the server polygonises.  felt_server_tick admits queued requests in
+ * arrival order — one in flight per surface, at most max_active in flight, and only while the
+ * resident bricks plus the step lists of the in-flight set fit budget_bytes — then advances
+ * every in-flight request by one step, all of them in one batched step.  Requests submitted
+ * while others run join at the next tick.  A surface's result depends only on its own requests
+ * and their order, never on what else is in flight. */
+typedef struct felt_server felt_server;
+typedef struct felt_request {
+    int32_t surface;       /* member index */
+    int32_t lo[3], hi[3];  /* world box, D values used */
+    felt_speed speed;
+    int32_t steps;
+    int32_t reserved;
+    uint64_t id;           /* caller's id, returned by felt_server_poll */
+} felt_request;
+typedef struct felt_server_stats {
+    uint64_t ticks;                /* batched steps run */
+    uint64_t requests_done, requests_queued, requests_active;
+    uint64_t admissions_deferred;  /* ticks whose admission stopped at the budget */
+    uint64_t voxel_updates;        /* sum over ticks of the affected-set sizes */
+    double busy_s;                 /* wall time inside ticks' device work */
+    double step_p50_ms, step_p99_ms;        /* per-tick latency */
+    double request_p50_ms, request_p99_ms;  /* submit to completion */
+    uint64_t device_bytes, budget_bytes, bricks_used, bricks_total;
+} felt_server_stats;
+/* sizes: n*D grid sizes.  Bricks get at most three quarters of budget_bytes. */
+int felt_server_create(int D, int L, const int32_t* part, int32_t n, const int32_t* sizes,
+                       uint64_t budget_bytes, int32_t max_active, int32_t polygonise,
+                       felt_server** out);
+int felt_server_destroy(felt_server* v);
+/* The server's batch (borrowed): load surfaces through it before felt_server_start. */
+int felt_server_batch(felt_server* v, felt_batch** out);
+/* The server's polygoniser (borrowed), for mesh queries with felt_batch_polys_*. */
+int felt_server_polys(felt_server* v, felt_polys** out);
+/* Reads every surface's band size and, when polygonising, marches every surface once. */
+int felt_server_start(felt_server* v);
+int felt_server_submit(felt_server* v, const felt_request* r, size_t n);
+/* One batched step; *stepped = requests advanced (0 when nothing is in flight). */
+int felt_server_tick(felt_server* v, int32_t* stepped);
+/* Ids of requests completed since the last poll (up to cap). */
+int felt_server_poll(felt_server* v, uint64_t* ids, size_t cap, size_t* n);
+int felt_server_read_stats(felt_server* v, felt_server_stats* out);
+
 /* ---- Tables ------------------------------------------------------------------------- */
 /* Copy of the packed marching-cubes triangle table the kernels use (256 entries). */
 int felt_mc_table(uint64_t* out, size_t n);
