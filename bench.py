@@ -206,6 +206,119 @@ def run_single(args):
     return res
 
 
+SERVED_SEED = 0xFE1704
+
+
+def served_requests(specs, window_s, rate, seed):
+    """Poisson arrivals: (time, surface, box, speed index, steps) per request."""
+    import math
+    from felt_amd import workloads
+    rng = workloads.SplitMix64(seed)
+    out, t = [], 0.0
+    while True:
+        t += -math.log(1.0 - rng.uniform()) / rate
+        if t >= window_s:
+            break
+        i = rng.randint(0, len(specs) - 1)
+        lo, hi = [], []
+        for s in specs[i].size:
+            side = max(1, int(rng.uniform(0.05, 1.0) * s))
+            a = rng.randint(-(s // 2), s - s // 2 - side)
+            lo.append(a)
+            hi.append(a + side - 1)
+        out.append((t, i, lo, hi, rng.randint(0, 2), rng.randint(1, 50)))
+    return out
+
+
+def run_served(args):
+    """Served stream: resident surfaces, Poisson request arrivals, continuous batching."""
+    import felt_amd
+    from felt_amd import workloads
+    from felt_amd.server import Server, request
+    lib = felt_amd.lib().lib
+    specs = workloads.mixed_surfaces(args.served_surfaces, SERVED_SEED, 64, 384, fixed_part=16)
+    sizes = [sp.size for sp in specs]
+    budget = int(args.budget_frac * 288e9)
+    arrivals = served_requests(specs, args.served_window, args.rate, SERVED_SEED ^ 0x5EED)
+
+    def make(max_active):
+        t = time.time()
+        srv = Server(lib, 3, 3, sizes, (16,) * 3, budget, max_active, polygonise=True)
+        for i, sp in enumerate(specs):
+            srv.batch.load_spheres(i, sp.centres, sp.radii)
+        srv.start()
+        log(f"[bench] served: {len(specs)} surfaces resident in {time.time() - t:.1f}s, "
+            f"{srv.stats()['device_bytes'] / 2**30:.1f} GiB")
+        return srv
+
+    def reqs(sel):
+        return [request(i, lo, hi, workloads.speed_of(k), n, rid)
+                for rid, (_, i, lo, hi, k, n) in sel]
+
+    # batched, in real time: arrivals are submitted when their time has come
+    srv = make(args.max_active)
+    todo = list(enumerate(arrivals))
+    total = len(todo)
+    active_sum, ticks = 0, 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        k = 0
+        while k < len(todo) and todo[k][1][0] <= now:
+            k += 1
+        if k:
+            srv.submit(reqs(todo[:k]))
+            todo = todo[k:]
+        st = srv.tick()
+        if st == 0:
+            if not todo and srv.stats()["requests_done"] >= total:
+                break
+            if todo:
+                time.sleep(max(0.0, min(0.001, todo[0][1][0] - (time.perf_counter() - t0))))
+            continue
+        active_sum += st
+        ticks += 1
+        srv.poll()
+        if ticks % 100 == 0:
+            log(f"[bench] served tick {ticks}: {srv.stats()['requests_done']}/{total} done")
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    res = {"workload": f"served stream: {len(specs)} resident Surface<3,3> float32, edge "
+                       "log-uniform 64-384, 16^3 partitions, 1-8 spheres, seed "
+                       f"0x{SERVED_SEED:X}; Poisson arrivals at {args.rate:g}/s for "
+                       f"{args.served_window:g}s; request = update(lo,hi,fn) over a box of "
+                       "U[0.05,1]*edge per axis, speed from the three, 1-50 steps, each step "
+                       "followed by repolygonisation of the changed partitions",
+           "requests": total, "ticks": st["ticks"], "wall_s": wall,
+           "value": st["voxel_updates"] / wall,
+           "capacity": st["voxel_updates"] / st["busy_s"] if st["busy_s"] else None,
+           "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
+           "request_p50_ms": st["request_p50_ms"], "request_p99_ms": st["request_p99_ms"],
+           "mean_in_flight": active_sum / max(1, ticks),
+           "phase_ms": dict(zip(["bounded update", "notify", "march", "band refresh"],
+                                st["phase_ms"])),
+           "admissions_deferred": st["admissions_deferred"],
+           "budget_bytes": budget, "device_bytes": st["device_bytes"],
+           "bricks_used": st["bricks_used"], "bricks_total": st["bricks_total"]}
+    # plain: the same stream one request step at a time (no batching), for a bounded sample
+    srv = make(1)
+    srv.submit(reqs(list(enumerate(arrivals))))
+    for _ in range(args.plain_ticks):
+        if srv.tick() == 0:
+            break
+    pst = srv.stats()
+    srv.close()
+    res["plain"] = {"capacity": pst["voxel_updates"] / pst["busy_s"] if pst["busy_s"] else None,
+                    "ticks": pst["ticks"], "step_p50_ms": pst["step_p50_ms"],
+                    "step_p99_ms": pst["step_p99_ms"]}
+    if res["capacity"] and res["plain"]["capacity"]:
+        res["served_vs_plain"] = res["capacity"] / res["plain"]["capacity"]
+    log(f"[bench] served: {res['value'] / 1e9:.2f} G/s wall, capacity "
+        f"{(res['capacity'] or 0) / 1e9:.2f} G/s, p99 step {res['step_p99_ms']:.2f} ms")
+    return res
+
+
 def cpu_baseline(specs, budget_s):
     """The CPU restatement (oracle/, OpenMP over partitions like the reference) stepping the
     mixed workload's surfaces one after another from their initial state, until the time
@@ -255,6 +368,14 @@ def main():
     ap.add_argument("--pass-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-only", action="store_true")
+    ap.add_argument("--served-surfaces", type=int, default=1000)
+    ap.add_argument("--served-window", type=float, default=2.0)
+    ap.add_argument("--rate", type=float, default=2000.0)
+    ap.add_argument("--budget-frac", type=float, default=0.25)
+    ap.add_argument("--max-active", type=int, default=1000)
+    ap.add_argument("--plain-ticks", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs its surfaces on a single MI355X and "
@@ -262,6 +383,9 @@ def main():
         sys.exit(2)
     import felt_amd
     felt_amd.lib()
+    if args.served_only:
+        print(json.dumps({"served": run_served(args)}), flush=True)
+        return
     specs, batched, plain, ktab, roof, cfg = run_mixed(args)
     log(f"[bench] batched {batched['value'] / 1e9:.2f} G/s, plain {plain['value'] / 1e9:.2f} G/s")
     result = {
@@ -286,6 +410,8 @@ def main():
         "roofline": roof,
         "kernels": ktab,
     }
+    if not args.no_served:
+        result["served"] = run_served(args)
     if not args.no_single:
         result["single_surface"] = run_single(args)
     if not args.no_cpu:

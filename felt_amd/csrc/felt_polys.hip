@@ -81,16 +81,35 @@ struct McT<2> {
     }
 };
 
-// The member a marched voxel belongs to: its grid size and world offset.
+// The member a marched voxel belongs to (grid size, world offset) and the brick being marched
+// (slot, grid position of its first voxel): reads inside that brick skip the page table.
 struct MemV {
     int m;
     int size[3];
     int offset[3];
+    int slot;
+    int org[3];
 };
 
-__device__ __forceinline__ MemV mem_view(const Pool& P, int m) {
-    const Mem& M = P.mem[m];
-    return MemV{m, {M.size[0], M.size[1], M.size[2]}, {M.offset[0], M.offset[1], M.offset[2]}};
+__device__ __forceinline__ MemV mem_view(const Pool& P, int slot) {
+    const int4 o = P.slot_org[slot];
+    const Mem& M = P.mem[o.w];
+    return MemV{o.w, {M.size[0], M.size[1], M.size[2]}, {M.offset[0], M.offset[1], M.offset[2]},
+                slot, {o.x, o.y, o.z}};
+}
+
+// get(pos) at grid position q of the member.
+template <int D>
+__device__ __forceinline__ float get_v(const Geo& G, const Pool& P, const MemV& V, const int q[3]) {
+    int l = 0;
+    bool in = true;
+    for (int a = 0; a < D; a++) {
+        const int la = q[a] - V.org[a];
+        if (la < 0 || la >= G.part[a] || q[a] >= V.size[a]) in = false;
+        l += la * G.lstr[a];
+    }
+    if (in) return P.phi[(size_t)V.slot * G.pv + l];
+    return get_g<D>(G, P, V.m, V.size, q);
 }
 
 // Corner mask of the cube based at grid position g (bit i: phi(corner i) > 0, Poly.hpp:356-368).
@@ -100,7 +119,7 @@ __device__ unsigned cube_mask(const Geo& G, const Pool& P, const MemV& V, const 
     for (int c = 0; c < McT<D>::ncorner; c++) {
         int q[3] = {0, 0, 0};
         for (int a = 0; a < D; a++) q[a] = g[a] + McT<D>::calc_off(a) + McT<D>::corner(c, a);
-        m |= (unsigned)(get_g<D>(G, P, V.m, V.size, q) > 0.0f) << c;
+        m |= (unsigned)(get_v<D>(G, P, V, q) > 0.0f) << c;
     }
     return m;
 }
@@ -156,7 +175,7 @@ __device__ float interp_w(const Geo& G, const Pool& P, const MemV& V, const floa
     for (int i = 0; i < (1 << D); i++) {
         int q[3] = {0, 0, 0};
         for (int a = 0; a < D; a++) q[a] = fl[a] + ((i >> a) & 1) - V.offset[a];
-        val[i] = get_g<D>(G, P, V.m, V.size, q);
+        val[i] = get_v<D>(G, P, V, q);
     }
     int n = 1 << D;
     for (int axis = 0; axis < D; axis++) {
@@ -205,18 +224,18 @@ __device__ void grad_f(const Geo& G, const Pool& P, const MemV& V, const float w
 template <int D>
 __device__ void grad_i(const Geo& G, const Pool& P, const MemV& V, const int g[3], float out[3]) {
     int pt[3] = {g[0], g[1], g[2]};
-    const float centre = get_g<D>(G, P, V.m, V.size, g);
+    const float centre = get_v<D>(G, P, V, g);
     for (int a = 0; a < D; a++) {
         float back = centre, fwd = centre;
         int order = 0;
         pt[a] -= 1;
         if (inside<D>(V.size, pt)) {
-            back = get_g<D>(G, P, V.m, V.size, pt);
+            back = get_v<D>(G, P, V, pt);
             order++;
         }
         pt[a] += 2;
         if (inside<D>(V.size, pt)) {
-            fwd = get_g<D>(G, P, V.m, V.size, pt);
+            fwd = get_v<D>(G, P, V, pt);
             order++;
         }
         pt[a] -= 1;
@@ -241,7 +260,7 @@ __device__ void make_vertex(const Geo& G, const Pool& P, const MemV& V, const in
                             float* out) {
     int gb[3] = {ga[0], ga[1], ga[2]};
     gb[axis] += 1;
-    const float va = get_g<D>(G, P, V.m, V.size, ga), vb = get_g<D>(G, P, V.m, V.size, gb);
+    const float va = get_v<D>(G, P, V, ga), vb = get_v<D>(G, P, V, gb);
     float pos[3] = {0.f, 0.f, 0.f}, n[3] = {0.f, 0.f, 0.f};
     if (fabsf(va) <= kEps || fabsf(vb) <= kEps) {
         const int* e = fabsf(va) <= kEps ? ga : gb;
@@ -361,9 +380,8 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
     const int j = item_j[i], l = item_l[i];
     const int p = parts[j];
     const int s = P.page[p];
-    const int4 o = P.slot_org[s];
-    const MemV V = mem_view(P, o.w);
-    const int org[3] = {o.x, o.y, o.z};
+    const MemV V = mem_view(P, s);
+    const int* org = V.org;
     int g[3] = {0, 0, 0};
     leaf_coords<D>(G, (uint32_t)l, g);
     for (int a = 0; a < D; a++) g[a] += org[a];
@@ -483,9 +501,8 @@ __global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, con
     if (ns == 0) return;
     const int p = parts[j];
     const int s = P.page[p];
-    const int4 o = P.slot_org[s];
-    const MemV V = mem_view(P, o.w);
-    const int org[3] = {o.x, o.y, o.z};
+    const MemV V = mem_view(P, s);
+    const int* org = V.org;
     int g[3] = {0, 0, 0};
     leaf_coords<D>(G, (uint32_t)l, g);
     for (int a = 0; a < D; a++) g[a] += org[a];

@@ -68,6 +68,7 @@ struct felt_server {
     std::vector<double> tick_ms, req_ms;
     uint64_t ticks = 0, completed = 0, deferred = 0, voxel_updates = 0;
     double busy_s = 0.0;
+    double phase_s[4] = {0, 0, 0, 0};  // bounded update, notify, march, band refresh
 };
 
 namespace {
@@ -221,10 +222,20 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
         }
         sp[j] = q.speed;
     }
+    double tp = t0;
+    auto phase = [&](int k) {
+        const double t = now_s();
+        v->phase_s[k] += t - tp;
+        tp = t;
+    };
     int rc = felt_update_boxes_impl(v->s, n, ids.data(), lo.data(), hi.data(), sp.data());
+    phase(0);
     if (!rc && v->polys) rc = felt_polys_notify_impl(v->polys);
+    phase(1);
     if (!rc && v->polys) rc = felt_polys_march_impl(v->polys);
+    phase(2);
     if (!rc) rc = refresh_bands(v, ids);
+    phase(3);
     if (rc) return rc;
     const double t1 = now_s();
     v->tick_ms.push_back((t1 - t0) * 1e3);
@@ -275,5 +286,7 @@ API int felt_server_read_stats(felt_server* v, felt_server_stats* st) {
     st->budget_bytes = v->budget;
     st->bricks_total = (uint64_t)v->s->cap;
     st->bricks_used = (uint64_t)(v->s->cap - v->s->hc.free_top);
+    for (int k = 0; k < 4; k++)
+        st->phase_ms[k] = v->ticks ? v->phase_s[k] * 1e3 / (double)v->ticks : 0.0;
     return FELT_OK;
 }

@@ -86,6 +86,7 @@ struct felt_surface {
     felt::Speed* h_speed = nullptr;  // pinned staging copy (nmem entries)
     bool speed_valid = false;        // d_speed holds the last global update's functors
     size_t list_cap = 0;             // capacity of every per-step list
+    int64_t hint = -1;               // exact entries of the current round's lists, -1: unknown
     std::vector<int> hbase;          // host copy of bbase after a profiled step
     bool band_known = false;         // band_now is the current tracked-voxel count
     size_t band_now = 0;

@@ -60,11 +60,28 @@ __global__ void k_dmask_pcnt(Geo G, Pool P, unsigned* dmask) {
     dmask[p] = m;
 }
 
+// delta(k) partition lists of the affected set, one atomic per distinct partition of a wave.
 __global__ void k_dmask_lu(Geo G, Pool P, Lu lu, const int* rng, unsigned* dmask) {
-    for (int i = rng[0] + blockIdx.x * blockDim.x + threadIdx.x; i < rng[1];
-         i += gridDim.x * blockDim.x) {
-        const int slot = (int)fdiv(lu.v[i], G.fpv);
-        atomicOr(&dmask[P.slot_part[slot]], 1u << (lu.tag[i] + G.L));
+    const int hi = rng[1];
+    for (int b = rng[0] + blockIdx.x * blockDim.x; b < hi; b += gridDim.x * blockDim.x) {
+        const int i = b + threadIdx.x;
+        int p = -1;
+        unsigned bit = 0;
+        if (i < hi) {
+            p = P.slot_part[fdiv(lu.v[i], G.fpv)];
+            bit = 1u << (lu.tag[i] + G.L);
+        }
+        const int lane = __lane_id();
+        unsigned long long todo = __ballot(p >= 0);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int p0 = __shfl(p, leader);
+            const bool mine = p >= 0 && p == p0;
+            unsigned bits = mine ? bit : 0u;
+            for (int o = 32; o > 0; o >>= 1) bits |= __shfl_xor(bits, o);
+            if (lane == leader) atomicOr(&dmask[p0], bits);
+            todo &= ~__ballot(mine);
+        }
     }
 }
 
@@ -74,30 +91,48 @@ __global__ void k_dmask_lu(Geo G, Pool P, Lu lu, const int* rng, unsigned* dmask
 // uses box pbox[j]: boxes holds {lo[3], hi[3]} per box in its member's grid coordinates,
 // half-open.
 template <int D>
+__device__ __forceinline__ bool box_zero_take(const Geo& G, const Pool& P, int s, int l,
+                                              const int org[3], const int* bx) {
+    if (l >= G.pv || P.lay[(size_t)s * G.pv + l] != 0) return false;
+    int lc[3] = {0, 0, 0};
+    leaf_coords<D>(G, (uint32_t)l, lc);
+    for (int a = 0; a < D; a++) {
+        const int ga = org[a] + lc[a];
+        if (ga < bx[a] || ga >= bx[3 + a]) return false;
+    }
+    return true;
+}
+
+// One block per listed partition: count its voxels in the box, reserve that many entries with
+// one global atomic, then write them.
+template <int D>
 __global__ void k_box_zero(Geo G, Pool P, const int* parts, const int* pbox, int nparts,
                            const int* boxes, uint32_t* z, Ctr* ctr) {
+    __shared__ int cnt, base;
     for (int j = blockIdx.x; j < nparts; j += gridDim.x) {
         const int p = parts[j];
         const int s = P.page[p];
-        if (s < 0) continue;
+        if (s < 0) continue;  // uniform over the block
         const int4 o = P.slot_org[s];
         const int org[3] = {o.x, o.y, o.z};
         const int* bx = boxes + 6 * pbox[j];
-        for (int base = 0; base < G.pv; base += blockDim.x) {
-            const int l = base + threadIdx.x;
-            bool take = false;
-            if (l < G.pv && P.lay[(size_t)s * G.pv + l] == 0) {
-                take = true;
-                int lc[3] = {0, 0, 0};
-                leaf_coords<D>(G, (uint32_t)l, lc);
-                for (int a = 0; a < D; a++) {
-                    const int ga = org[a] + lc[a];
-                    if (ga < bx[a] || ga >= bx[3 + a]) take = false;
-                }
-            }
-            const int at = wave_append(&ctr->z_span[1], take);
-            if (take) z[at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)l;
+        if (threadIdx.x == 0) cnt = 0;
+        __syncthreads();
+        for (int b = 0; b < G.pv; b += blockDim.x)
+            wave_append(&cnt, box_zero_take<D>(G, P, s, b + threadIdx.x, org, bx));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            base = cnt ? atomicAdd(&ctr->z_span[1], cnt) : 0;
+            cnt = 0;
         }
+        __syncthreads();
+        for (int b = 0; b < G.pv; b += blockDim.x) {
+            const int l = b + threadIdx.x;
+            const bool take = box_zero_take<D>(G, P, s, l, org, bx);
+            const int at = wave_append(&cnt, take);
+            if (take) z[base + at] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)l;
+        }
+        __syncthreads();
     }
 }
 
@@ -144,7 +179,8 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
                         kq = layer_of(dq, G.L, &ctr->nan_events);
                         if (in_band(kq, G.L) && sq >= 0) {
                             vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
-                            add = cas_byte((int8_t*)&amark[vq], 0, 1);
+                            // most neighbours are already in the set: test before the atomic
+                            add = amark[vq] == 0 && cas_byte((int8_t*)&amark[vq], 0, 1);
                         }
                     }
                 }
@@ -731,11 +767,16 @@ static int ensure_step_capacity(felt_surface* s, size_t band) {
     return FELT_OK;
 }
 
+// Grids of the per-round kernels: sized for the lists' capacity, or for the exact entry count
+// of the round when the host has read it (rounds driven from the host).
+static size_t list_entries(felt_surface* s) {
+    return s->hint >= 0 ? (size_t)s->hint : s->list_cap;
+}
 static int grid_for_lists(felt_surface* s) {
-    return (int)std::min<size_t>(4096, std::max<size_t>(1, (s->list_cap + TPB - 1) / TPB));
+    return (int)std::min<size_t>(4096, std::max<size_t>(1, (list_entries(s) + TPB - 1) / TPB));
 }
 static int chg_blocks(felt_surface* s) {
-    return (int)((s->list_cap + BT * ITEMS - 1) / (BT * ITEMS));
+    return (int)std::max<size_t>(1, (list_entries(s) + BT * ITEMS - 1) / (BT * ITEMS));
 }
 
 // Band compaction into per-layer segments [bbase[k], bbase[k+1]): scan of the partition
@@ -900,17 +941,25 @@ static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
 
 // Rounds after the unrolled ones, driven from the host (rare).
 static int finish_rounds(felt_surface* s, int look, int buf, int* rounds) {
-    int rc;
+    int rc = FELT_OK;
     while (s->hc.flag) {
-        if ((rc = reset_round(s))) return rc;
+        s->hint = s->hc.lu_span[look][1];
+        if ((rc = reset_round(s))) break;
         lookup_round(s, look, buf, always(s));
-        HIPX(hipMemsetAsync(&s->ctr->lu_span[look][1], 0, sizeof(int), s->st));
+        if (hipMemsetAsync(&s->ctr->lu_span[look][1], 0, sizeof(int), s->st) != hipSuccess) {
+            rc = felt_fail(FELT_E_DEVICE, "hipMemsetAsync failed");
+            break;
+        }
         std::swap(look, buf);
         (*rounds)++;
-        if ((rc = read_ctr(s))) return rc;
-        if (*rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
+        if ((rc = read_ctr(s))) break;
+        if (*rounds > kMaxRounds) {
+            rc = felt_fail(FELT_E_STATE, "round loop did not converge");
+            break;
+        }
     }
-    return FELT_OK;
+    s->hint = -1;
+    return rc;
 }
 
 // Makes sure the step lists can hold the current band (one host read when it is unknown).
@@ -1120,29 +1169,39 @@ int felt_update_end_impl(felt_surface* s) {
         prof_end(s);
         LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, j + 1);
     }
-    LAUNCH(k_unmark, dim3(bfs_grid), dim3(TPB), A, lu_span(s, 0), s->amark);
-    LAUNCH(k_dmask_lu, dim3(bfs_grid), dim3(TPB), G, s->P, A, lu_span(s, 0), s->dmask);
-    LAUNCH(k_tk_init, dim3(bfs_grid), dim3(TPB), A.tag, A.tk, lu_span(s, 0), always(s));
+    // the affected set's size sizes every grid of the rounds below
+    if ((rc = read_ctr(s))) return rc;
+    const int an = s->hc.lu_span[0][1];
+    s->hint = an;
+    const int grid = grid_for_lists(s);
+    LAUNCH(k_unmark, dim3(grid), dim3(TPB), A, lu_span(s, 0), s->amark);
+    LAUNCH(k_dmask_lu, dim3(grid), dim3(TPB), G, s->P, A, lu_span(s, 0), s->dmask);
+    LAUNCH(k_tk_init, dim3(grid), dim3(TPB), A.tag, A.tk, lu_span(s, 0), always(s));
     prof_begin(s, PK_APPLY, 0);
-    LAUNCH(k_apply, dim3(bfs_grid), dim3(TPB), G, s->P, s->zlist.ptr, s->zdelta.ptr,
+    LAUNCH(k_apply, dim3(grid), dim3(TPB), G, s->P, s->zlist.ptr, s->zdelta.ptr,
            &s->ctr->z_span[0], A.tk, &s->ctr->nan_events);
     prof_end(s);
     int look = 0, buf = 1;
     distance_passes(s, false, look, always(s));
     compact_changes(s, A.v, A.tag, A.tk, lu_span(s, 0), buf, always(s));
     flush_expand(s, always(s));
-    if ((rc = read_ctr(s))) return rc;
-    const int an = s->hc.lu_span[0][1];
+    if ((rc = read_ctr(s))) { s->hint = -1; return rc; }
     int rounds = 1;
     while (s->hc.flag) {
         HIPX(hipMemsetAsync(&s->ctr->lu_span[look][1], 0, sizeof(int), s->st));
         std::swap(look, buf);
-        if ((rc = reset_round(s))) return rc;
+        s->hint = s->hc.lu_span[look][1];
+        if ((rc = reset_round(s))) break;
         lookup_round(s, look, buf, always(s));
         rounds++;
-        if ((rc = read_ctr(s))) return rc;
-        if (rounds > kMaxRounds) return felt_fail(FELT_E_STATE, "round loop did not converge");
+        if ((rc = read_ctr(s))) break;
+        if (rounds > kMaxRounds) {
+            rc = felt_fail(FELT_E_STATE, "round loop did not converge");
+            break;
+        }
     }
+    s->hint = -1;
+    if (rc) return rc;
     s->last_updates = (uint64_t)an;
     s->last_rounds = rounds;
     s->in_update = false;

@@ -30,7 +30,8 @@ class felt_server_stats(C.Structure):
                 ("busy_s", C.c_double), ("step_p50_ms", C.c_double), ("step_p99_ms", C.c_double),
                 ("request_p50_ms", C.c_double), ("request_p99_ms", C.c_double),
                 ("device_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
-                ("bricks_used", C.c_uint64), ("bricks_total", C.c_uint64)]
+                ("bricks_used", C.c_uint64), ("bricks_total", C.c_uint64),
+                ("phase_ms", C.c_double * 4)]
 
 
 _SIGS = {
@@ -120,4 +121,6 @@ class Server:
     def stats(self) -> dict:
         st = felt_server_stats()
         self.api("felt_server_read_stats", self.h, C.byref(st))
-        return {name: getattr(st, name) for name, _ in felt_server_stats._fields_}
+        out = {name: getattr(st, name) for name, _ in felt_server_stats._fields_}
+        out["phase_ms"] = [float(x) for x in st.phase_ms]
+        return out

@@ -279,6 +279,7 @@ typedef struct felt_server_stats {
     double step_p50_ms, step_p99_ms;        /* per-tick latency */
     double request_p50_ms, request_p99_ms;  /* submit to completion */
     uint64_t device_bytes, budget_bytes, bricks_used, bricks_total;
+    double phase_ms[4];  /* mean per tick: bounded update, notify, march, band refresh */
 } felt_server_stats;
 /* sizes: n*D grid sizes.  Bricks get at most three quarters of budget_bytes. */
 int felt_server_create(int D, int L, const int32_t* part, int32_t n, const int32_t* sizes,
