@@ -318,6 +318,23 @@ __device__ __forceinline__ int wave_append(int* counter, bool pred) {
     return pred ? base + __popcll(below) : -1;
 }
 
+// Wave-aggregated append of k entries per lane: one atomic per wavefront; returns the lane's
+// first index.  Every lane of the wave must call it.
+__device__ __forceinline__ int wave_append_n(int* counter, int k) {
+    const int lane = __lane_id();
+    int x = k;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    const int total = __shfl(x, 63);
+    int base = 0;
+    if (lane == 63 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + x - k;
+}
+
 // Compare-and-swap of one byte through its aligned 32-bit word.
 __device__ __forceinline__ bool cas_byte(int8_t* addr, int8_t expect, int8_t desired) {
     const uintptr_t a = (uintptr_t)addr;

@@ -150,8 +150,33 @@ __global__ void k_lu_seed(Lu lu, const uint32_t* z, int n, uint8_t* amark, int* 
 
 __global__ void k_snap_front(Ctr* ctr, int lu_i, int j) { ctr->front[j] = ctr->lu_span[lu_i][1]; }
 
+// Face neighbour d = (axis a, side sd) of voxel x for the affected search: its id and value
+// when its partition is active (active = true), else the partition background.
+template <int D>
+__device__ __forceinline__ float aff_neighbour(const Geo& G, const Pool& P, const Vx& x, int a,
+                                               int sd, const int q[3], uint32_t& vq,
+                                               bool& active) {
+    const int la = x.loc[a] + sd;
+    if (la >= 0 && la < G.part[a]) {  // same brick
+        vq = (uint32_t)((int64_t)x.v + sd * G.lstr[a]);
+        active = true;
+        return P.phi[vq];
+    }
+    int pq, lq;
+    locate<D>(G, P, x.m, q, pq, lq);
+    const int sq = P.page[pq];
+    if (sq < 0) {
+        active = false;
+        return P.pbg[pq];
+    }
+    vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
+    active = true;
+    return P.phi[vq];
+}
+
 // calc_affected (Surface.hpp:780-913), one breadth-first round: face neighbours of the last
-// round's voxels whose current layer is in the band join the set once.
+// round's voxels whose current layer is in the band join the set once.  Each lane claims its
+// new neighbours first, then the wave reserves all their entries with one atomic.
 template <int D>
 __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j,
                                  uint8_t* amark) {
@@ -162,32 +187,48 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
         const int i = base + threadIdx.x;
         const bool valid = i < n;
         Vx x;
-        if (valid) decode<D>(G, P, lu.v[lo + i], x);
-        for (int a = 0; a < D; a++) {
-            for (int s = -1; s <= 1; s += 2) {
-                bool add = false;
-                uint32_t vq = 0;
-                int kq = 0;
-                if (valid) {
+        unsigned mask = 0, tags = 0;  // claimed directions; their layers, 4 bits each
+        if (valid) {
+            decode<D>(G, P, lu.v[lo + i], x);
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int sd = t == 0 ? -1 : 1;
                     int q[3] = {x.g[0], x.g[1], x.g[2]};
-                    q[a] += s;
-                    if (inside<D>(x.size, q)) {
-                        int pq, lq;
-                        locate<D>(G, P, x.m, q, pq, lq);
-                        const int sq = P.page[pq];
-                        const float dq = sq < 0 ? P.pbg[pq] : P.phi[(size_t)sq * G.pv + lq];
-                        kq = layer_of(dq, G.L, &ctr->nan_events);
-                        if (in_band(kq, G.L) && sq >= 0) {
-                            vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
-                            // most neighbours are already in the set: test before the atomic
-                            add = amark[vq] == 0 && cas_byte((int8_t*)&amark[vq], 0, 1);
-                        }
+                    q[a] += sd;
+                    if (!inside<D>(x.size, q)) continue;
+                    uint32_t vq = 0;
+                    bool active;
+                    const float dq = aff_neighbour<D>(G, P, x, a, sd, q, vq, active);
+                    const int kq = layer_of(dq, G.L, &ctr->nan_events);
+                    // most neighbours are already in the set: test before the atomic
+                    if (in_band(kq, G.L) && active && amark[vq] == 0 &&
+                        cas_byte((int8_t*)&amark[vq], 0, 1)) {
+                        const int d = 2 * a + t;
+                        mask |= 1u << d;
+                        tags |= (unsigned)(kq + 8) << (4 * d);
                     }
                 }
-                const int at = wave_append(&ctr->lu_span[lu_i][1], add);
-                if (add) {
+            }
+        }
+        int at = wave_append_n(&ctr->lu_span[lu_i][1], __popc(mask));
+        if (mask) {
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int d = 2 * a + t;
+                    if (!((mask >> d) & 1u)) continue;
+                    const int sd = t == 0 ? -1 : 1;
+                    int q[3] = {x.g[0], x.g[1], x.g[2]};
+                    q[a] += sd;
+                    uint32_t vq = 0;
+                    bool active;
+                    (void)aff_neighbour<D>(G, P, x, a, sd, q, vq, active);
                     lu.v[at] = vq;
-                    lu.tag[at] = (int8_t)kq;
+                    lu.tag[at] = (int8_t)((int)((tags >> (4 * d)) & 15u) - 8);
+                    at++;
                 }
             }
         }
