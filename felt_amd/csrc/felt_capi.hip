@@ -821,6 +821,12 @@ API int felt_batch_polys_child_mesh(felt_polys* q, int32_t i, uint64_t pos_idx, 
     return felt_polys_child_mesh_impl(q, p, vtx, vcap, spx, scap);
 }
 
+API int felt_ms_table(uint32_t* out, size_t n) {
+    if (!out || n < 16) return felt_fail(FELT_E_CAPACITY, "need 16 entries");
+    felt_ms_table_host(out);
+    return FELT_OK;
+}
+
 API int felt_mc_table(uint64_t* out, size_t n) {
     if (!out || n < 256) return felt_fail(FELT_E_CAPACITY, "need 256 entries");
     felt_mc_table_host(out);

@@ -49,3 +49,4 @@ int felt_polys_march_impl(felt_polys* q);
 int felt_polys_child_mesh_impl(felt_polys* q, uint64_t p, float* vtx, size_t vcap, uint32_t* spx,
                                size_t scap);
 void felt_mc_table_host(uint64_t* out);
+void felt_ms_table_host(uint32_t* out);

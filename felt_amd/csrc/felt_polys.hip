@@ -24,7 +24,7 @@ constexpr int TPB = 256;
 #include "mc_tables.inc"
 
 __constant__ unsigned long long c_mc_tri[256];
-__constant__ unsigned short c_ms_seg[16];
+__constant__ unsigned int c_ms_seg[16];
 
 // Edge table in the reference's corner convention: offset of the edge's low end from pos_calc
 // and its axis (PolyMixin.hpp:271-276, 390-406).
@@ -70,12 +70,12 @@ struct McT<2> {
     __device__ static int calc_off(int) { return 0; }
     __device__ static unsigned edges_of(unsigned m) {
         const unsigned t = c_ms_seg[m];
-        const int n = (int)(t >> 12) * 2;
+        const int n = (int)(t >> 28) * 2;
         unsigned em = 0;
         for (int i = 0; i < n; i++) em |= 1u << ((t >> (4 * i)) & 0xf);
         return em;
     }
-    __device__ static int nsimplex(unsigned m) { return (int)(c_ms_seg[m] >> 12); }
+    __device__ static int nsimplex(unsigned m) { return (int)(c_ms_seg[m] >> 28); }
     __device__ static int simplex_edge(unsigned m, int i, int j) {
         return (int)((c_ms_seg[m] >> (4 * (i * 2 + j))) & 0xf);
     }
@@ -587,6 +587,10 @@ __global__ void k_gather_active(Pool P, const int* parts, int n, int* out) {
 
 void felt_mc_table_host(uint64_t* out) {
     for (int i = 0; i < 256; i++) out[i] = kMcTri[i];
+}
+
+void felt_ms_table_host(uint32_t* out) {
+    for (int i = 0; i < 16; i++) out[i] = kMsSeg[i];
 }
 
 int felt_polys_upload_tables() {

@@ -42,13 +42,29 @@ def grid_coords(size: Sequence[int]):
 
 
 def spheres_phi(size: Sequence[int], L: int, centres, radii) -> np.ndarray:
-    """min over spheres of (|x - c| - r), clamped to +-(L+1)."""
-    X = grid_coords(size)
-    phi = np.full(tuple(size), np.inf)
+    """min over spheres of (|x - c| - r), clamped to +-(L+1), in float64 then float32.  Each
+    sphere is evaluated only inside its bounding box grown by L+1: outside it |x - c| - r > L+1,
+    which the clamp maps to L+1 anyway."""
+    D = len(size)
+    lim = float(L + 1)
+    phi = np.full(tuple(size), lim)
+    off = [s // 2 for s in size]
     for c, r in zip(centres, radii):
-        d2 = sum((X[a] - c[a]) ** 2 for a in range(len(size)))
-        phi = np.minimum(phi, np.sqrt(d2) - r)
-    return np.clip(phi, -(L + 1), L + 1).astype(np.float32)
+        sl, ax = [], []
+        for a in range(D):
+            lo = max(0, int(math.floor(c[a] - r - lim)) + off[a])
+            hi = min(size[a], int(math.ceil(c[a] + r + lim)) + off[a] + 1)
+            if lo >= hi:
+                break
+            sl.append(slice(lo, hi))
+            ax.append(np.arange(lo, hi, dtype=np.float64) - off[a])
+        if len(sl) < D:
+            continue
+        X = np.meshgrid(*ax, indexing="ij")
+        d2 = sum((X[a] - c[a]) ** 2 for a in range(D))
+        sub = phi[tuple(sl)]
+        np.minimum(sub, np.sqrt(d2) - r, out=sub)
+    return np.clip(phi, -lim, lim).astype(np.float32)
 
 
 def _real_sph_harm(l: int, m: int, theta: np.ndarray, ph: np.ndarray) -> np.ndarray:

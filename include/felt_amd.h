@@ -302,6 +302,9 @@ int felt_server_read_stats(felt_server* v, felt_server_stats* out);
 /* ---- Tables ------------------------------------------------------------------------- */
 /* Copy of the packed marching-cubes triangle table the kernels use (256 entries). */
 int felt_mc_table(uint64_t* out, size_t n);
+/* Copy of the packed marching-squares segment table (16 entries: segment count in bits 28-31,
+ * edge k of the segment list in bits 4k..4k+3). */
+int felt_ms_table(uint32_t* out, size_t n);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,29 @@ def test_library_exports_every_declared_symbol():
     assert len(declared()) >= 30
 
 
+def test_ms_table_matches_oracle_and_topology():
+    """Marching-squares segments: same edge lists as the oracle's table, and every segment
+    joins two edges whose endpoints differ in sign (Felt's square corners and edges,
+    PolyMixin.hpp:221-265)."""
+    import felt_amd
+    lib = ctypes.CDLL(felt_amd.LIB_PATH)
+    out = np.zeros(16, dtype=np.uint32)
+    assert lib.felt_ms_table(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 16) == 0
+    src = open(os.path.join(ROOT, "oracle", "felt_oracle.hpp")).read()
+    blk = src[src.index("kSeg2[16] = {"):]
+    rows = re.findall(r'"([0-9a-f]*)"', blk[:blk.index("};")])
+    assert len(rows) == 16
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    edges = [(0, 1), (1, 2), (3, 2), (0, 3)]  # corner pairs of edges 0..3
+    for m in range(16):
+        t = int(out[m])
+        n = t >> 28
+        got = "".join("%x" % ((t >> (4 * k)) & 0xF) for k in range(2 * n))
+        assert got == rows[m], (m, got, rows[m])
+        crossed = {e for e, (a, b) in enumerate(edges) if ((m >> a) & 1) != ((m >> b) & 1)}
+        assert set(int(c, 16) for c in got) == crossed, m
+
+
 def test_mc_table_matches_oracle_and_topology():
     import felt_amd
     lib = ctypes.CDLL(felt_amd.LIB_PATH)
