@@ -117,6 +117,16 @@ def run_mixed(args):
     mem = [b.memory() for b, _ in batches]
     for bt, _ in batches:
         bt.close()
+    cfg = {"workload": f"mixed-size batch: {len(specs)} independent Surface<3,3> float32, edge "
+                       "log-uniform 32-256 (8^3 partitions below 96, else 16^3), 1-8 spheres "
+                       "r=U[0.05,0.3]*edge, speeds {-0.4, 0.15curv, -0.3+0.1curv}, seed "
+                       f"0x{MIXED_SEED:X}; one update(fn) of every surface per step",
+           "surfaces": len(specs),
+           "batches": {f"{p}^3": len(ids) for p, ids in sorted(groups.items())},
+           "device_bytes": int(sum(m[0] for m in mem)),
+           "bricks_in_use": int(sum(m[1] for m in mem))}
+    if args.plain_steps <= 0:
+        return specs, batched, None, ktab, roof, cfg
     # ---- plain: one surface per launch sequence ----
     t = time.time()
     plain = []
@@ -144,14 +154,6 @@ def run_mixed(args):
     plain_res = {"value": ups_p / total_p, "ms_per_step": total_p / args.plain_steps * 1e3,
                  "p50_step_ms": pctl(lat_p, 50), "p99_step_ms": pctl(lat_p, 99),
                  "steps": args.plain_steps, "voxel_updates_per_step": ups_p / args.plain_steps}
-    cfg = {"workload": f"mixed-size batch: {len(specs)} independent Surface<3,3> float32, edge "
-                       "log-uniform 32-256 (8^3 partitions below 96, else 16^3), 1-8 spheres "
-                       "r=U[0.05,0.3]*edge, speeds {-0.4, 0.15curv, -0.3+0.1curv}, seed "
-                       f"0x{MIXED_SEED:X}; one update(fn) of every surface per step",
-           "surfaces": len(specs),
-           "batches": {f"{p}^3": len(ids) for p, ids in sorted(groups.items())},
-           "device_bytes": int(sum(m[0] for m in mem)),
-           "bricks_in_use": int(sum(m[1] for m in mem))}
     return specs, batched, plain_res, ktab, roof, cfg
 
 
@@ -370,6 +372,7 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-only", action="store_true")
+    ap.add_argument("--single-only", action="store_true")
     ap.add_argument("--served-surfaces", type=int, default=1000)
     ap.add_argument("--served-window", type=float, default=2.0)
     ap.add_argument("--rate", type=float, default=2000.0)
@@ -386,8 +389,12 @@ def main():
     if args.served_only:
         print(json.dumps({"served": run_served(args)}), flush=True)
         return
+    if args.single_only:
+        print(json.dumps({"single_surface": run_single(args)}), flush=True)
+        return
     specs, batched, plain, ktab, roof, cfg = run_mixed(args)
-    log(f"[bench] batched {batched['value'] / 1e9:.2f} G/s, plain {plain['value'] / 1e9:.2f} G/s")
+    log(f"[bench] batched {batched['value'] / 1e9:.2f} G/s" +
+        (f", plain {plain['value'] / 1e9:.2f} G/s" if plain else ""))
     result = {
         "metric": METRIC,
         "value": batched["value"],
@@ -406,7 +413,7 @@ def main():
         "p99_step_ms": batched["p99_step_ms"],
         "voxel_updates_per_step": batched["voxel_updates_per_step"],
         "plain": plain,
-        "batched_vs_plain": batched["value"] / plain["value"],
+        "batched_vs_plain": batched["value"] / plain["value"] if plain else None,
         "roofline": roof,
         "kernels": ktab,
     }

@@ -33,13 +33,19 @@ constexpr float kEps = 1.1920928955078125e-07f;  // FLT_EPSILON
 // Division by a run-time invariant divisor with one high multiply (Granlund & Montgomery
 // 1994; Warren, Hacker's Delight 10-8): s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1,
 // q = (t + ((n - t) >> 1)) >> (s - 1) with t = mulhi(n, m).  Exact for every 32-bit n.
+// A power-of-two divisor (sh >= 0) divides by a shift.
 struct FastDiv {
     uint32_t d, m;
     int s;
+    int sh;
 };
 
 inline FastDiv make_fastdiv(uint32_t d) {
-    FastDiv f{d, 0u, 0};
+    FastDiv f{d, 0u, 0, -1};
+    if (d != 0 && (d & (d - 1)) == 0) {
+        while ((1u << (f.sh + 1)) != d) f.sh++;
+        f.sh++;
+    }
     if (d <= 1) return f;
     int s = 0;
     while ((1ull << s) < d) s++;
@@ -49,6 +55,7 @@ inline FastDiv make_fastdiv(uint32_t d) {
 }
 
 __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    if (f.sh >= 0) return n >> f.sh;
     if (f.d <= 1) return n;
     const uint32_t t = __umulhi(n, f.m);
     return (t + ((n - t) >> 1)) >> (f.s - 1);

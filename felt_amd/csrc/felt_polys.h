@@ -26,6 +26,7 @@ struct felt_polys {
     felt_surface* s = nullptr;
     std::vector<PolyKid> kids;
     Changes a, b;
+    Changes alive;  // children with a mesh (active)
     Changes* pending = nullptr;
     Changes* done = nullptr;
     int vw = 6;
@@ -35,7 +36,7 @@ struct felt_polys {
     long long last_spx = 0;
     DBuf<uint8_t> cmask;
     DBuf<uint16_t> own;
-    DBuf<int> vbase, tbase, tot, dparts, pstart, vst, item_j, item_l, nv, nt, vleaf;
+    DBuf<int> vbase, tbase, tot, dparts, pstart, vst, item_j, item_l, nv, nt, vleaf, work;
     DBuf<long long> dseg;
     DBuf<uint8_t> bits;
     DBuf<int> plist;  // partition lists exchanged with the host

@@ -371,10 +371,12 @@ __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, 
         }
 }
 
-// Cube mask, owned edges and vertex / simplex counts of each band voxel.
+// Cube mask, owned edges and vertex / simplex counts of each band voxel; the voxels with
+// anything to emit are listed in `work` so that the emit pass runs on them alone.
 template <int D>
 __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
-                          int M, uint8_t* cmask, uint16_t* own, int* nv, int* nt) {
+                          int M, uint8_t* cmask, uint16_t* own, int* nv, int* nt, int* work,
+                          int* nwork) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     const int j = item_j[i], l = item_l[i];
@@ -397,6 +399,9 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
     own[(size_t)j * G.pv + l] = (uint16_t)om;
     nv[i] = __popc(om);
     nt[i] = McT<D>::nsimplex(m);
+    const bool busy = om != 0 || McT<D>::nsimplex(m) > 0;
+    const int at = wave_append(nwork, busy);
+    if (busy) work[at] = i;
 }
 
 // Device-wide exclusive scan of two int arrays: block sums, top-level scan, apply.
@@ -489,11 +494,12 @@ __global__ void k_m_partstart(const int* start, int nb, const int* vb, const int
 // indices local to the partition.
 template <int D>
 __global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
-                         int M, const uint8_t* cmask, const uint16_t* own, const int* vb,
-                         const int* tb, const int* vleaf, const int* vst, float* vout,
-                         uint32_t* tout) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
+                         const int* work, int nwork, const uint8_t* cmask, const uint16_t* own,
+                         const int* vb, const int* tb, const int* vleaf, const int* vst,
+                         float* vout, uint32_t* tout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nwork) return;
+    const int i = work[t];
     const int j = item_j[i], l = item_l[i];
     const size_t jb = (size_t)j * G.pv;
     const unsigned m = cmask[jb + l];
@@ -628,6 +634,7 @@ int felt_polys_init(felt_polys* q) {
     const Geo& G = q->s->G;
     q->kids.assign(G.np, PolyKid{});
     q->a.look.assign(G.np, -1);
+    q->alive.look.assign(G.np, -1);
     q->b.look.assign(G.np, -1);
     q->pending = &q->a;
     q->done = &q->b;
@@ -654,6 +661,7 @@ void felt_polys_release(felt_polys* q) {
     q->nv.release();
     q->nt.release();
     q->vleaf.release();
+    q->work.release();
 }
 
 // Partitions whose notify bits meet `want`, sorted by partition index.
@@ -718,8 +726,8 @@ static int arena_reserve(felt_polys* q, long long nv, long long nt) {
         return FELT_OK;
     long long live_v = 0, live_t = 0;
     std::vector<long long> sv, dv, lv, stt, dt, lt;
-    for (PolyKid& k : q->kids) {
-        if (!k.active) continue;
+    for (int64_t p : q->alive.list) {
+        PolyKid& k = q->kids[p];
         if (k.nv) {
             sv.push_back(k.voff * q->vw);
             dv.push_back(live_v * q->vw);
@@ -763,8 +771,8 @@ static int arena_reserve(felt_polys* q, long long nv, long long nt) {
     if ((rc = copy(sv, dv, lv, true))) return rc;
     if ((rc = copy(stt, dt, lt, false))) return rc;
     long long cv = 0, ct = 0;
-    for (PolyKid& k : q->kids) {
-        if (!k.active) continue;
+    for (int64_t p : q->alive.list) {
+        PolyKid& k = q->kids[p];
         k.voff = cv;
         k.toff = ct;
         cv += k.nv;
@@ -804,10 +812,12 @@ int felt_polys_march_impl(felt_polys* q) {
         PolyKid& k = q->kids[p];
         if (active[i]) {
             k.active = true;
+            q->alive.track(p);
             k.nv = k.nt = 0;
             todo.push_back((int)p);
         } else if (k.active) {
             k = PolyKid{};
+            q->alive.untrack(p);
         }
     }
     const size_t per = (size_t)G.pv * (1 + 2 + 4);
@@ -834,18 +844,22 @@ int felt_polys_march_impl(felt_polys* q) {
         if ((rc = q->nt.ensure(M))) return rc;
         if ((rc = q->vbase.ensure(M))) return rc;
         if ((rc = q->tbase.ensure(M))) return rc;
-        if ((rc = q->tot.ensure(2 * (size_t)nblk + 2))) return rc;
+        if ((rc = q->tot.ensure(2 * (size_t)nblk + 3))) return rc;
+        if ((rc = q->work.ensure(M))) return rc;
         if ((rc = q->cmask.ensure((size_t)nb * G.pv))) return rc;
         if ((rc = q->own.ensure((size_t)nb * G.pv))) return rc;
         if ((rc = q->vleaf.ensure((size_t)nb * G.pv))) return rc;
         DISPATCHP(G.D, k_mb_scatter, dim3(nb), dim3(bt), G, s->P, q->dparts.ptr, q->pstart.ptr,
                   q->item_j.ptr, q->item_l.ptr);
-        prof_begin(s, PK_MARCH_COUNT, M);
-        DISPATCHP(G.D, k_m_count, dim3((M + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
-                  q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->nv.ptr, q->nt.ptr);
-        prof_end(s);
         int* bs = q->tot.ptr;
         int* totals = q->tot.ptr + 2 * nblk;
+        int* nwork = q->tot.ptr + 2 * nblk + 2;
+        HIPX(hipMemsetAsync(nwork, 0, sizeof(int), st));
+        prof_begin(s, PK_MARCH_COUNT, M);
+        DISPATCHP(G.D, k_m_count, dim3((M + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
+                  q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->nv.ptr, q->nt.ptr,
+                  q->work.ptr, nwork);
+        prof_end(s);
         hipLaunchKernelGGL(k_s2_blocks, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs);
         hipLaunchKernelGGL(k_s2_top, dim3(1), dim3(BT), 0, st, bs, nblk, totals);
         hipLaunchKernelGGL(k_s2_apply, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs,
@@ -857,7 +871,9 @@ int felt_polys_march_impl(felt_polys* q) {
         hipLaunchKernelGGL(k_m_partstart, dim3((nb + 1 + TPB - 1) / TPB), dim3(TPB), 0, st,
                            q->pstart.ptr, nb, q->vbase.ptr, q->tbase.ptr, M, totals, vst, tst);
         std::vector<int> hv(2 * (nb + 1));
+        int W = 0;
         HIPX(hipMemcpyAsync(hv.data(), vst, 2 * (nb + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPX(hipMemcpyAsync(&W, nwork, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPX(hipStreamSynchronize(st));
         const long long NV = hv[nb], NT = hv[(nb + 1) + nb];
         if ((rc = arena_reserve(q, NV, NT))) return rc;
@@ -872,11 +888,12 @@ int felt_polys_march_impl(felt_polys* q) {
         q->vtop += NV;
         q->ttop += NT;
         q->last_spx += NT;
-        prof_begin(s, PK_MARCH_EMIT, M);
-        DISPATCHP(G.D, k_m_emit, dim3((M + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
-                  q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->vbase.ptr,
-                  q->tbase.ptr, q->vleaf.ptr, vst, q->varena.ptr + vb0 * q->vw,
-                  q->tarena.ptr + tb0 * G.D);
+        prof_begin(s, PK_MARCH_EMIT, W);
+        if (W > 0)
+            DISPATCHP(G.D, k_m_emit, dim3((W + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
+                      q->item_j.ptr, q->item_l.ptr, q->work.ptr, W, q->cmask.ptr, q->own.ptr,
+                      q->vbase.ptr, q->tbase.ptr, q->vleaf.ptr, vst, q->varena.ptr + vb0 * q->vw,
+                      q->tarena.ptr + tb0 * G.D);
         prof_end(s);
     }
     HIPX(hipStreamSynchronize(st));

@@ -828,15 +828,14 @@ static int bscan_blocks(const Geo& G) {
 
 static void compact_band(felt_surface* s) {
     const Geo& G = s->G;
-    const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
     const int nb = bscan_blocks(G);
     PLAUNCH(PK_COMPACT, (long long)G.np * G.NL, k_bscan_blocks, dim3(nb), dim3(BT), s->P.pcnt,
             G.np, G.NL, s->blk_cnt.ptr);
     LAUNCH(k_bscan_top, dim3(1), dim3(BT), s->blk_cnt.ptr, nb, G.NL, s->bbase, s->ctr);
     LAUNCH(k_bscan_apply, dim3(nb), dim3(BT), s->P.pcnt, G.np, G.NL, s->blk_cnt.ptr,
            s->blk_off.ptr, s->bbase);
-    DISPATCH(G.D, k_band_scatter, dim3(G.np), dim3(bt), G, s->P, s->blk_off.ptr, s->band.ptr,
-             s->band_tag.ptr, s->ctr);
+    DISPATCH(G.D, k_band_scatter, dim3(std::min(G.np, 4096)), dim3(TPB), G, s->P, s->blk_off.ptr,
+             s->band.ptr, s->band_tag.ptr, s->ctr);
 }
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
