@@ -7,8 +7,9 @@ log-uniform in [32, 256] rounded to a multiple of the partition size (8^3 partit
 edge of 96, else 16^3), 1-8 spheres of radius U[0.05, 0.3] * edge with uniform centres (phi =
 min over spheres), per-surface speed drawn from {-0.4, 0.15 curv, -0.3 + 0.1 curv}, seed
 0xFE1703.  A step advances every surface by one update(fn).
-  - batched: the surfaces of each partition size form one batch (felt_batch_update: one
-    captured HIP graph replay per batch per step);
+  - batched: the surfaces of each partition size form one batch; a step queues one captured
+    HIP graph replay per batch (felt_batch_update_async, one stream each, so the two overlap)
+    and then waits for both;
   - plain: one felt_surface per surface, update(fn) called surface after surface.
 Voxel updates per step = band size at the start of the step (all 2L+1 layers), summed over
 surfaces.  `value` is the batched rate; the plain rate and their ratio sit beside it.
@@ -102,8 +103,11 @@ def run_mixed(args):
     t_all = time.perf_counter()
     for step in range(args.steps):
         t0 = time.perf_counter()
+        # queue every batch's step (one stream each) before waiting, so they overlap
         for bt, speeds in batches:
-            bt.update(speeds)
+            bt.update_async(speeds)
+        for bt, _ in batches:
+            bt.wait()
             ups += bt.step_info()[0]
         lat.append(time.perf_counter() - t0)
         if (step + 1) % 25 == 0:

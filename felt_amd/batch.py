@@ -33,6 +33,8 @@ _SIGS = {
     "felt_batch_load_spheres": [_P, C.c_int32, C.c_int32, _F64P, _F64P],
     "felt_batch_update": [_P, C.POINTER(felt_speed)],
     "felt_batch_update_boxes": [_P, C.c_int32, _I32P, _I32P, _I32P, C.POINTER(felt_speed)],
+    "felt_batch_update_async": [_P, C.POINTER(felt_speed)],
+    "felt_batch_wait": [_P],
     "felt_batch_snapshot": [_P, C.c_int32, _F32P, _SZ],
     "felt_batch_layer_size": [_P, C.c_int32, C.c_int, _U64P],
     "felt_batch_layer_positions": [_P, C.c_int32, C.c_int, _I32P, _SZ, _SZP],
@@ -138,6 +140,14 @@ class Batch:
         """update(fn) of every member, member i with speeds[i]."""
         arr = (felt_speed * self.n)(*speeds)
         self.api("felt_batch_update", self.h, arr)
+
+    def update_async(self, speeds: Sequence[felt_speed]) -> None:
+        """Queue update(fn) of every member; finish with wait()."""
+        self._speeds = (felt_speed * self.n)(*speeds)  # kept alive until the launch returns
+        self.api("felt_batch_update_async", self.h, self._speeds)
+
+    def wait(self) -> None:
+        self.api("felt_batch_wait", self.h)
 
     def update_boxes(self, ids: Sequence[int], lo, hi, speeds: Sequence[felt_speed]) -> None:
         """update(lo, hi, fn) of the listed members as one step."""

@@ -629,6 +629,8 @@ API int felt_polys_totals(felt_polys* q, uint64_t* nv, uint64_t* ns) {
 
 #define CHECK_B(b) \
     if (!(b) || !(b)->s) return felt_fail(FELT_E_ARG, "null batch handle")
+#define CHECK_IDLE(b) \
+    if ((b)->s->step_pending) return felt_fail(FELT_E_STATE, "a step is in flight: felt_batch_wait first")
 #define CHECK_M(b, i) \
     if ((i) < 0 || (i) >= (b)->s->G.nmem) return felt_fail(FELT_E_ARG, "member out of range")
 
@@ -641,6 +643,7 @@ API int felt_batch_create(int D, int L, const int32_t* part, int32_t n, const in
         delete b;
         return rc;
     }
+    b->s->auto_unroll = true;
     *out = b;
     return FELT_OK;
 }
@@ -663,6 +666,7 @@ API int felt_batch_count(felt_batch* b, int32_t* n) {
 
 API int felt_batch_memory(felt_batch* b, uint64_t* bytes, uint64_t* used, uint64_t* total) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     if (int rc = felt_download(b->s, &b->s->hc, b->s->ctr, sizeof(Ctr))) return rc;
     *bytes = felt_device_bytes(b->s);
     *total = (uint64_t)b->s->cap;
@@ -672,6 +676,7 @@ API int felt_batch_memory(felt_batch* b, uint64_t* bytes, uint64_t* used, uint64
 
 API int felt_batch_seed(felt_batch* b, int32_t i, const int32_t* pos) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     CHECK_M(b, i);
     if (!pos) return felt_fail(FELT_E_ARG, "null position");
     return seed_m(b->s, i, pos);
@@ -679,6 +684,7 @@ API int felt_batch_seed(felt_batch* b, int32_t i, const int32_t* pos) {
 
 API int felt_batch_load_dense(felt_batch* b, int32_t i, const float* phi, size_t n) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     CHECK_M(b, i);
     if (n != grid_total(b->s, i) || !phi) return felt_fail(FELT_E_ARG, "load_dense: size mismatch");
     return felt_load_dense_impl(b->s, i, phi, n);
@@ -687,12 +693,14 @@ API int felt_batch_load_dense(felt_batch* b, int32_t i, const float* phi, size_t
 API int felt_batch_load_spheres(felt_batch* b, int32_t i, int32_t n, const double* centres,
                                 const double* radii) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     CHECK_M(b, i);
     return load_spheres_m(b->s, i, n, centres, radii);
 }
 
 API int felt_batch_update(felt_batch* b, const felt_speed* speeds) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     if (!speeds) return felt_fail(FELT_E_ARG, "null speeds");
     for (int i = 0; i < b->s->G.nmem; i++)
         if (int rc = check_speed(speeds + i)) return rc;
@@ -702,6 +710,7 @@ API int felt_batch_update(felt_batch* b, const felt_speed* speeds) {
 API int felt_batch_update_boxes(felt_batch* b, int32_t n, const int32_t* ids, const int32_t* lo,
                                 const int32_t* hi, const felt_speed* speeds) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     if (n < 0 || (n > 0 && (!ids || !lo || !hi || !speeds)))
         return felt_fail(FELT_E_ARG, "bad arguments");
     felt_surface* s = b->s;
@@ -725,6 +734,7 @@ API int felt_batch_update_boxes(felt_batch* b, int32_t n, const int32_t* ids, co
 
 API int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     CHECK_M(b, i);
     const size_t total = grid_total(b->s, i);
     if (n < total) return felt_fail(FELT_E_CAPACITY, "snapshot buffer too small");
@@ -733,6 +743,7 @@ API int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n) {
 
 API int felt_batch_layer_size(felt_batch* b, int32_t i, int layer, uint64_t* n) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     CHECK_M(b, i);
     return layer_size_m(b->s, i, layer, n);
 }
@@ -740,14 +751,30 @@ API int felt_batch_layer_size(felt_batch* b, int32_t i, int layer, uint64_t* n) 
 API int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos, size_t cap,
                                    size_t* n) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     CHECK_M(b, i);
     return layer_positions_m(b->s, i, layer, pos, cap, n);
 }
 
 API int felt_batch_stats(felt_batch* b, int32_t i, felt_stats* st) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     CHECK_M(b, i);
     return stats_m(b->s, i, st);
+}
+
+API int felt_batch_update_async(felt_batch* b, const felt_speed* speeds) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    if (!speeds) return felt_fail(FELT_E_ARG, "null speeds");
+    for (int i = 0; i < b->s->G.nmem; i++)
+        if (int rc = check_speed(speeds + i)) return rc;
+    return felt_update_global_launch(b->s, speeds);
+}
+
+API int felt_batch_wait(felt_batch* b) {
+    CHECK_B(b);
+    return felt_global_step_finish(b->s);
 }
 
 API int felt_batch_step_info(felt_batch* b, uint64_t* voxel_updates, uint64_t* rounds) {
@@ -759,6 +786,7 @@ API int felt_batch_step_info(felt_batch* b, uint64_t* voxel_updates, uint64_t* r
 
 API int felt_batch_set_graph(felt_batch* b, int enable) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     b->s->use_graph = enable != 0;
     return FELT_OK;
 }
@@ -766,6 +794,7 @@ API int felt_batch_set_graph(felt_batch* b, int enable) {
 API int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launch,
                              double* items_per_launch, int* launches) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     return felt_surface_time_pass(b->s, pass, reps, us_per_launch, items_per_launch, launches);
 }
 
@@ -776,6 +805,7 @@ API int felt_batch_sync(felt_batch* b) {
 
 API int felt_batch_polys_create(felt_batch* b, felt_polys** out) {
     CHECK_B(b);
+    CHECK_IDLE(b);
     return felt_polys_create(b->s, out);
 }
 

@@ -91,6 +91,8 @@ struct felt_surface {
     bool band_known = false;         // band_now is the current tracked-voxel count
     size_t band_now = 0;
     int unroll = 3;                  // rounds of a global step launched without host reads
+    bool auto_unroll = false;        // raise `unroll` when steps need more rounds (batches)
+    bool step_pending = false;       // a global step is queued and not yet finished
     bool use_graph = true;
     bool graph_valid = false;
     hipGraphExec_t graph_exec = nullptr;
@@ -121,6 +123,10 @@ felt::Lu felt_lu(felt_surface* s, int i);
 int felt_update_start_impl(felt_surface* s);
 // Global step of every member, member i with speed fns[i].
 int felt_update_global(felt_surface* s, const felt_speed* fns);
+// The same split in two: queue the step on the handle's stream, then finish it (counter read
+// and any rounds beyond the unrolled ones).
+int felt_update_global_launch(felt_surface* s, const felt_speed* fns);
+int felt_global_step_finish(felt_surface* s);
 int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double* items,
                         int* launches);
 int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);

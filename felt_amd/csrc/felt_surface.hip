@@ -26,6 +26,7 @@ namespace {
 
 constexpr int TPB = 256;
 constexpr int kMaxRounds = 100000;
+constexpr int kMaxUnroll = 8;  // Ctr::live has one flag per unrolled round
 
 __global__ void k_fill_i32(int* p, int64_t n, int v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -1014,11 +1015,12 @@ static int size_for_band(felt_surface* s) {
     return ensure_step_capacity(s, s->band_now);
 }
 
-// Global step (Surface.hpp:254-296, 739-762).  `zero_fn(rng)` writes the zero layer's deltas
-// into band_d over the span rng.  With `use_graph` the kernel sequence is captured once and
-// replayed while list capacities and the speed layout are unchanged.
+// Global step (Surface.hpp:254-296, 739-762), launch half: `zero_fn(rng)` writes the zero
+// layer's deltas into band_d over the span rng.  With `use_graph` the kernel sequence is
+// captured once and replayed while list capacities, the speed layout and the unrolled round
+// count are unchanged.  Returns with the step queued on the handle's stream.
 template <class ZeroFn>
-static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
+static int global_step_launch(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
     int rc;
     if ((rc = size_for_band(s))) return rc;
     // Kernel timing attaches events to individual dispatches, which a replayed graph does not
@@ -1045,6 +1047,17 @@ static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
     } else {
         global_step_kernels(s, zero_fn);
     }
+    s->step_pending = true;
+    return FELT_OK;
+}
+
+// Finish half: one counter read, then any rounds beyond the unrolled ones from the host.  With
+// auto_unroll, a step that needed more rounds than are unrolled raises the unrolled count (up
+// to 8) for the next capture.
+int felt_global_step_finish(felt_surface* s) {
+    int rc;
+    if (!s->step_pending) return FELT_OK;
+    s->step_pending = false;
     if ((rc = read_ctr(s))) return rc;
     if (s->prof.on && (rc = prof_collect(s))) return rc;
     int rounds = 1;
@@ -1056,6 +1069,10 @@ static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
         const int look = r % 2 == 1 ? 1 : 0, buf = 1 - look;
         if ((rc = finish_rounds(s, look, buf, &rounds))) return rc;
         s->band_known = false;
+        if (s->auto_unroll && s->unroll < kMaxUnroll) {
+            s->unroll = std::min(kMaxUnroll, rounds);
+            s->graph_valid = false;
+        }
     } else {
         s->band_now = (size_t)s->hc.band_total;
         s->band_known = true;
@@ -1066,11 +1083,18 @@ static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
     return FELT_OK;
 }
 
+template <class ZeroFn>
+static int global_step(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
+    int rc;
+    if ((rc = global_step_launch(s, zero_fn, use_graph))) return rc;
+    return felt_global_step_finish(s);
+}
+
 static Speed to_speed(const felt_speed& f) {
     return Speed{f.kind, f.a, f.b, f.c, (unsigned long long)f.seed, f.cell};
 }
 
-int felt_update_global(felt_surface* s, const felt_speed* fns) {
+int felt_update_global_launch(felt_surface* s, const felt_speed* fns) {
     for (int m = 0; m < s->G.nmem; m++) {
         if (fns[m].kind == FELT_SPEED_NOISE_CURV && fns[m].cell <= 0)
             return felt_fail(FELT_E_ARG, "noise cell must be positive");
@@ -1079,10 +1103,16 @@ int felt_update_global(felt_surface* s, const felt_speed* fns) {
     HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, s->G.nmem * sizeof(Speed), hipMemcpyHostToDevice,
                         s->st));
     s->speed_valid = true;
-    return global_step(s, [&](const int* rng) {
+    return global_step_launch(s, [&](const int* rng) {
         PDISPATCH(PK_SPEED, -(long long)s->G.L - 1, s->G.D, k_speed, dim3(grid_for_lists(s)),
                   dim3(TPB), s->G, s->P, s->band.ptr, rng, s->d_speed, s->band_d.ptr);
     }, s->use_graph);
+}
+
+int felt_update_global(felt_surface* s, const felt_speed* fns) {
+    int rc;
+    if ((rc = felt_update_global_launch(s, fns))) return rc;
+    return felt_global_step_finish(s);
 }
 
 // Times one pass of a global step on the current state (felt_surface_time_pass).

@@ -231,6 +231,11 @@ int felt_batch_update(felt_batch* b, const felt_speed* speeds);
  * are not touched. */
 int felt_batch_update_boxes(felt_batch* b, int32_t n, const int32_t* ids, const int32_t* lo,
                             const int32_t* hi, const felt_speed* speeds);
+/* felt_batch_update split in two: queue the step on the batch's stream and return; then finish
+ * it.  Steps of different batches queued before waiting run concurrently on the device.  Until
+ * felt_batch_wait, every other call on the batch fails with FELT_E_STATE. */
+int felt_batch_update_async(felt_batch* b, const felt_speed* speeds);
+int felt_batch_wait(felt_batch* b);
 int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n);
 int felt_batch_layer_size(felt_batch* b, int32_t i, int layer, uint64_t* n);
 int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos, size_t cap,

@@ -182,3 +182,28 @@ def test_batch_polys():
             compare(f"step {step}")
     assert bp.totals()[1] > 0
     print("[batch polys] ok", flush=True)
+
+
+def test_two_batches_async_overlap():
+    """Steps of two batches queued together (two streams) before waiting: every member still
+    matches its oracle, and calls between queue and wait are refused."""
+    from felt_amd._binding import FeltError
+    b8, c8 = make_batch(3, 3, [(32, 32, 32), (24, 40, 24), (40, 40, 40)], (8, 8, 8), 41)
+    b16, c16 = make_batch(3, 3, [(48, 48, 48), (64, 32, 48)], (16, 16, 16), 42)
+    s8 = [SPEEDS[i % 3] for i in range(3)]
+    s16 = [SPEEDS[(i + 1) % 3] for i in range(2)]
+    for step in range(8):
+        b8.update_async(s8)
+        b16.update_async(s16)
+        if step == 0:
+            with pytest.raises(FeltError):
+                b8.snapshot(0)
+        b8.wait()
+        b16.wait()
+        for c, sp in zip(c8, s8):
+            c.update(sp)
+        for c, sp in zip(c16, s16):
+            c.update(sp)
+        check_all(b8, c8, f"8^3 step {step}", snapshot=step % 4 == 3)
+        check_all(b16, c16, f"16^3 step {step}", snapshot=step % 4 == 3)
+    print("[batch async] ok", flush=True)
