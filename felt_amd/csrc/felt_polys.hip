@@ -377,31 +377,41 @@ template <int D>
 __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
                           int M, uint8_t* cmask, uint16_t* own, int* nv, int* nt, int* work,
                           int* nwork) {
+    __shared__ int s_cnt, s_base;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const int j = item_j[i], l = item_l[i];
-    const int p = parts[j];
-    const int s = P.page[p];
-    const MemV V = mem_view(P, s);
-    const int* org = V.org;
-    int g[3] = {0, 0, 0};
-    leaf_coords<D>(G, (uint32_t)l, g);
-    for (int a = 0; a < D; a++) g[a] += org[a];
-    const unsigned m = cube_mask<D>(G, P, V, g);
-    const unsigned em = McT<D>::edges_of(m);
-    unsigned om = 0;
-    for (int e = 0; e < McT<D>::nedge; e++) {
-        if (!((em >> e) & 1)) continue;
-        int eo;
-        if (edge_owner<D>(G, P, V, s, org, g, e, &eo) == l) om |= 1u << e;
+    const bool valid = i < M;
+    bool busy = false;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    if (valid) {
+        const int j = item_j[i], l = item_l[i];
+        const int p = parts[j];
+        const int s = P.page[p];
+        const MemV V = mem_view(P, s);
+        const int* org = V.org;
+        int g[3] = {0, 0, 0};
+        leaf_coords<D>(G, (uint32_t)l, g);
+        for (int a = 0; a < D; a++) g[a] += org[a];
+        const unsigned m = cube_mask<D>(G, P, V, g);
+        const unsigned em = McT<D>::edges_of(m);
+        unsigned om = 0;
+        for (int e = 0; e < McT<D>::nedge; e++) {
+            if (!((em >> e) & 1)) continue;
+            int eo;
+            if (edge_owner<D>(G, P, V, s, org, g, e, &eo) == l) om |= 1u << e;
+        }
+        cmask[(size_t)j * G.pv + l] = (uint8_t)m;
+        own[(size_t)j * G.pv + l] = (uint16_t)om;
+        nv[i] = __popc(om);
+        nt[i] = McT<D>::nsimplex(m);
+        busy = nt[i] > 0;  // an owned edge is a crossed edge, so it has simplices too
     }
-    cmask[(size_t)j * G.pv + l] = (uint8_t)m;
-    own[(size_t)j * G.pv + l] = (uint16_t)om;
-    nv[i] = __popc(om);
-    nt[i] = McT<D>::nsimplex(m);
-    const bool busy = om != 0 || McT<D>::nsimplex(m) > 0;
-    const int at = wave_append(nwork, busy);
-    if (busy) work[at] = i;
+    // the busy voxels, listed with one global atomic per block
+    const int local = wave_append(&s_cnt, busy);
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(nwork, s_cnt) : 0;
+    __syncthreads();
+    if (busy) work[s_base + local] = i;
 }
 
 // Device-wide exclusive scan of two int arrays: block sums, top-level scan, apply.

@@ -213,6 +213,7 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
     std::vector<int> ids(n);
     std::vector<int32_t> lo((size_t)n * 3, 0), hi((size_t)n * 3, 0);
     std::vector<felt_speed> sp(n);
+    std::vector<long long> bands(n);
     for (int j = 0; j < n; j++) {
         const felt_request& q = v->active[j].r;
         ids[j] = q.surface;
@@ -221,6 +222,7 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
             hi[(size_t)j * 3 + a] = q.hi[a];
         }
         sp[j] = q.speed;
+        bands[j] = v->band[q.surface];
     }
     double tp = t0;
     auto phase = [&](int k) {
@@ -228,7 +230,8 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
         v->phase_s[k] += t - tp;
         tp = t;
     };
-    int rc = felt_update_boxes_impl(v->s, n, ids.data(), lo.data(), hi.data(), sp.data());
+    int rc = felt_update_boxes_impl(v->s, n, ids.data(), lo.data(), hi.data(), sp.data(),
+                                    bands.data());
     phase(0);
     if (!rc && v->polys) rc = felt_polys_notify_impl(v->polys);
     phase(1);

@@ -131,9 +131,11 @@ int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double*
                         int* launches);
 int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
 // Bounded step of n distinct members ids[j], each over its world box [lo, hi] (3 ints per
-// entry) with speed fns[j]; other members are not touched.
+// entry) with speed fns[j]; other members are not touched.  `bands` (optional) holds the
+// members' current band sizes, which size the step's lists; without it they are counted.
 int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t* lo,
-                           const int32_t* hi, const felt_speed* fns);
+                           const int32_t* hi, const felt_speed* fns,
+                           const long long* bands = nullptr);
 int felt_delta_impl(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
 int felt_update_end_public(felt_surface* s);
 // Per-member operations; positions are member grid coordinates (3 ints per entry) unless

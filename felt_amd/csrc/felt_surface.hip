@@ -1293,17 +1293,20 @@ int felt_band_total(felt_surface* s, size_t* out) {
 // lo <= x < min(hi + 1, grid_hi), which take that member's speed; then update_end over all of
 // them at once.
 int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t* lo,
-                           const int32_t* hi, const felt_speed* fns) {
+                           const int32_t* hi, const felt_speed* fns, const long long* bands) {
     const Geo& G = s->G;
     int rc;
     if (G.nmem == 1) {
         if ((rc = size_for_band(s))) return rc;
     } else {
         // lists of a bounded step hold at most the stepped members' bands
-        std::vector<long long> bands;
-        if ((rc = felt_member_bands(s, ids, n, bands))) return rc;
+        std::vector<long long> b;
+        if (!bands) {
+            if ((rc = felt_member_bands(s, ids, n, b))) return rc;
+            bands = b.data();
+        }
         long long sum = 0;
-        for (long long b : bands) sum += b;
+        for (int j = 0; j < n; j++) sum += bands[j];
         if ((rc = ensure_step_capacity(s, (size_t)sum))) return rc;
     }
     if ((rc = felt_update_start_impl(s))) return rc;
