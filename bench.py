@@ -247,9 +247,10 @@ def run_served(args):
     budget = int(args.budget_frac * 288e9)
     arrivals = served_requests(specs, args.served_window, args.rate, SERVED_SEED ^ 0x5EED)
 
-    def make(max_active):
+    def make(max_active, target_ms=0.0):
         t = time.time()
         srv = Server(lib, 3, 3, sizes, (16,) * 3, budget, max_active, polygonise=True)
+        srv.set_target(target_ms)
         for i, sp in enumerate(specs):
             srv.batch.load_spheres(i, sp.centres, sp.radii)
         srv.start()
@@ -261,52 +262,60 @@ def run_served(args):
         return [request(i, lo, hi, workloads.speed_of(k), n, rid)
                 for rid, (_, i, lo, hi, k, n) in sel]
 
-    # batched, in real time: arrivals are submitted when their time has come
-    srv = make(args.max_active)
-    todo = list(enumerate(arrivals))
-    total = len(todo)
-    active_sum, ticks = 0, 0
-    t0 = time.perf_counter()
-    while True:
-        now = time.perf_counter() - t0
-        k = 0
-        while k < len(todo) and todo[k][1][0] <= now:
-            k += 1
-        if k:
-            srv.submit(reqs(todo[:k]))
-            todo = todo[k:]
-        st = srv.tick()
-        if st == 0:
-            if not todo and srv.stats()["requests_done"] >= total:
-                break
-            if todo:
-                time.sleep(max(0.0, min(0.001, todo[0][1][0] - (time.perf_counter() - t0))))
-            continue
-        active_sum += st
-        ticks += 1
-        srv.poll()
-        if ticks % 100 == 0:
-            log(f"[bench] served tick {ticks}: {srv.stats()['requests_done']}/{total} done")
-    wall = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
-    res = {"workload": f"served stream: {len(specs)} resident Surface<3,3> float32, edge "
+    def stream(target_ms):
+        """The arrival stream in real time (arrivals are submitted when their time has come)
+        through a server with the given tick-time target."""
+        srv = make(args.max_active, target_ms)
+        todo = list(enumerate(arrivals))
+        total = len(todo)
+        active_sum, ticks = 0, 0
+        t0 = time.perf_counter()
+        while True:
+            now = time.perf_counter() - t0
+            k = 0
+            while k < len(todo) and todo[k][1][0] <= now:
+                k += 1
+            if k:
+                srv.submit(reqs(todo[:k]))
+                todo = todo[k:]
+            st = srv.tick()
+            if st == 0:
+                if not todo and srv.stats()["requests_done"] >= total:
+                    break
+                if todo:
+                    time.sleep(max(0.0, min(0.001, todo[0][1][0] - (time.perf_counter() - t0))))
+                continue
+            active_sum += st
+            ticks += 1
+            srv.poll()
+            if ticks % 200 == 0:
+                log(f"[bench] served tick {ticks}: {srv.stats()['requests_done']}/{total} done")
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        return {"tick_target_ms": target_ms or None,
+                "requests": total, "ticks": st["ticks"], "wall_s": wall,
+                "value": st["voxel_updates"] / wall,
+                "capacity": st["voxel_updates"] / st["busy_s"] if st["busy_s"] else None,
+                "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
+                "request_p50_ms": st["request_p50_ms"], "request_p99_ms": st["request_p99_ms"],
+                "mean_in_flight": active_sum / max(1, ticks),
+                "phase_ms": dict(zip(["bounded update", "notify", "march", "band refresh"],
+                                     st["phase_ms"])),
+                "admissions_deferred": st["admissions_deferred"],
+                "budget_bytes": budget, "device_bytes": st["device_bytes"],
+                "bricks_used": st["bricks_used"], "bricks_total": st["bricks_total"]}
+
+    res = stream(args.tick_target)
+    res["workload"] = (f"served stream: {len(specs)} resident Surface<3,3> float32, edge "
                        "log-uniform 64-384, 16^3 partitions, 1-8 spheres, seed "
                        f"0x{SERVED_SEED:X}; Poisson arrivals at {args.rate:g}/s for "
                        f"{args.served_window:g}s; request = update(lo,hi,fn) over a box of "
                        "U[0.05,1]*edge per axis, speed from the three, 1-50 steps, each step "
-                       "followed by repolygonisation of the changed partitions",
-           "requests": total, "ticks": st["ticks"], "wall_s": wall,
-           "value": st["voxel_updates"] / wall,
-           "capacity": st["voxel_updates"] / st["busy_s"] if st["busy_s"] else None,
-           "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
-           "request_p50_ms": st["request_p50_ms"], "request_p99_ms": st["request_p99_ms"],
-           "mean_in_flight": active_sum / max(1, ticks),
-           "phase_ms": dict(zip(["bounded update", "notify", "march", "band refresh"],
-                                st["phase_ms"])),
-           "admissions_deferred": st["admissions_deferred"],
-           "budget_bytes": budget, "device_bytes": st["device_bytes"],
-           "bricks_used": st["bricks_used"], "bricks_total": st["bricks_total"]}
+                       "followed by repolygonisation of the changed partitions")
+    if args.tick_target:
+        # the same stream with no tick-time target: largest batches, highest throughput
+        res["throughput_mode"] = stream(0.0)
     # plain: the same stream one request step at a time (no batching), for a bounded sample
     srv = make(1)
     srv.submit(reqs(list(enumerate(arrivals))))
@@ -320,6 +329,9 @@ def run_served(args):
                     "step_p99_ms": pst["step_p99_ms"]}
     if res["capacity"] and res["plain"]["capacity"]:
         res["served_vs_plain"] = res["capacity"] / res["plain"]["capacity"]
+    tm = res.get("throughput_mode")
+    if tm and tm["capacity"] and res["plain"]["capacity"]:
+        tm["served_vs_plain"] = tm["capacity"] / res["plain"]["capacity"]
     log(f"[bench] served: {res['value'] / 1e9:.2f} G/s wall, capacity "
         f"{(res['capacity'] or 0) / 1e9:.2f} G/s, p99 step {res['step_p99_ms']:.2f} ms")
     return res
@@ -382,6 +394,7 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--budget-frac", type=float, default=0.25)
     ap.add_argument("--max-active", type=int, default=1000)
+    ap.add_argument("--tick-target", type=float, default=5.0)
     ap.add_argument("--plain-ticks", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:

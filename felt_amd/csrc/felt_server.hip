@@ -50,6 +50,20 @@ struct Req {
     felt_request r;
     double t_submit;
     int left;
+    double frac;  // the box's share of its grid, clamped to [0, 1]
+};
+
+// Tick-time model of the latency target: t = fixed + unit * work, work = sum over in-flight
+// requests of band * box share (about the affected set).  Both terms are running averages.
+struct TickModel {
+    double fixed_ms = 2.0;
+    double unit_ms = 2e-7;
+    double predict(double work) const { return fixed_ms + unit_ms * work; }
+    void observe(double ms, double work) {
+        const double f = std::max(0.0, ms - unit_ms * work);
+        fixed_ms = 0.95 * fixed_ms + 0.05 * std::min(f, ms);
+        if (work > 0) unit_ms = 0.9 * unit_ms + 0.1 * std::max(0.0, ms - fixed_ms) / work;
+    }
 };
 
 }  // namespace
@@ -66,6 +80,8 @@ struct felt_server {
     std::vector<long long> band;       // tracked voxels per surface, refreshed after its steps
     std::vector<uint64_t> done;        // completed request ids not yet polled
     std::vector<double> tick_ms, req_ms;
+    double target_ms = 0.0;            // tick-time target of admission (0: none)
+    TickModel model;
     uint64_t ticks = 0, completed = 0, deferred = 0, voxel_updates = 0;
     double busy_s = 0.0;
     double phase_s[4] = {0, 0, 0, 0};  // bounded update, notify, march, band refresh
@@ -94,8 +110,17 @@ uint64_t resident_bytes(const felt_server* v) {
     return felt_device_bytes(s) - (uint64_t)s->list_cap * kListBytes;
 }
 
+double work_of(const felt_server* v, const Req& q) { return (double)v->band[q.r.surface] * q.frac; }
+
+double active_work(const felt_server* v) {
+    double w = 0;
+    for (const Req& q : v->active) w += work_of(v, q);
+    return w;
+}
+
 void admit(felt_server* v) {
     const uint64_t fixed = resident_bytes(v);
+    double work = active_work(v);
     for (auto it = v->queue.begin(); it != v->queue.end();) {
         if ((int)v->active.size() >= v->max_active) break;
         const int m = it->r.surface;
@@ -107,6 +132,11 @@ void admit(felt_server* v) {
             v->deferred++;
             break;  // keep arrival order: later requests wait too
         }
+        const double w = work_of(v, *it);
+        if (!v->active.empty() && v->target_ms > 0 &&
+            v->model.predict(work + w) > v->target_ms)
+            break;  // the tick would overrun the latency target
+        work += w;
         v->busy[m] = 1;
         v->active.push_back(*it);
         it = v->queue.erase(it);
@@ -186,6 +216,12 @@ API int felt_server_start(felt_server* v) {
     return FELT_OK;
 }
 
+API int felt_server_set_target(felt_server* v, double tick_ms) {
+    if (!v || tick_ms < 0) return felt_fail(FELT_E_ARG, "bad arguments");
+    v->target_ms = tick_ms;
+    return FELT_OK;
+}
+
 API int felt_server_submit(felt_server* v, const felt_request* r, size_t n) {
     if (!v || (n && !r)) return felt_fail(FELT_E_ARG, "null argument");
     const double t = now_s();
@@ -197,7 +233,14 @@ API int felt_server_submit(felt_server* v, const felt_request* r, size_t n) {
         if (q.speed.kind < 0 || q.speed.kind > FELT_SPEED_NOISE_CURV ||
             (q.speed.kind == FELT_SPEED_NOISE_CURV && q.speed.cell <= 0))
             return felt_fail(FELT_E_ARG, "bad speed descriptor");
-        v->queue.push_back(Req{q, t, q.steps});
+        const Mem& M = v->s->mems[q.surface];
+        double frac = 1.0;
+        for (int a = 0; a < v->s->G.D; a++) {
+            const int lo = std::max(q.lo[a], M.offset[a]);
+            const int hi = std::min(q.hi[a], M.offset[a] + M.size[a] - 1);
+            frac *= hi >= lo ? (double)(hi - lo + 1) / M.size[a] : 0.0;
+        }
+        v->queue.push_back(Req{q, t, q.steps, frac});
     }
     return FELT_OK;
 }
@@ -208,6 +251,7 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
     if (stepped) *stepped = (int32_t)v->active.size();
     if (v->active.empty()) return FELT_OK;
     const double t0 = now_s();
+    const double work = active_work(v);
     const int n = (int)v->active.size();
     const int D = v->s->G.D;
     std::vector<int> ids(n);
@@ -242,6 +286,7 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
     if (rc) return rc;
     const double t1 = now_s();
     v->tick_ms.push_back((t1 - t0) * 1e3);
+    v->model.observe((t1 - t0) * 1e3, work);
     v->busy_s += t1 - t0;
     v->ticks++;
     v->voxel_updates += v->s->last_updates;

@@ -65,7 +65,6 @@ struct felt_surface {
     // bounded-update zero layer
     DBuf<uint32_t> zlist;
     DBuf<float> zdelta;
-    int z_host_n = 0;
     // lookups: 0 = affected, 1 = affected buffer
     DBuf<uint32_t> lu_v[2];
     DBuf<int8_t> lu_tag[2];

@@ -137,13 +137,15 @@ __global__ void k_box_zero(Geo G, Pool P, const int* parts, const int* pbox, int
     }
 }
 
-__global__ void k_lu_seed(Lu lu, const uint32_t* z, int n, uint8_t* amark, int* lu_span) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    lu.v[i] = z[i];
-    lu.tag[i] = 0;
-    amark[z[i]] = 1;
-    if (i == 0) {
+// The affected set starts as the zero-layer list z (its length in z_n, on the device).
+__global__ void k_lu_seed(Lu lu, const uint32_t* z, const int* z_n, uint8_t* amark, int* lu_span) {
+    const int n = *z_n;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        lu.v[i] = z[i];
+        lu.tag[i] = 0;
+        amark[z[i]] = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         lu_span[0] = 0;
         lu_span[1] = n;
     }
@@ -941,7 +943,6 @@ int felt_update_start_impl(felt_surface* s) {
            (int64_t)s->G.np, 0);
     LAUNCH(k_fill_i32, dim3(std::min(1024, blocks_for(s->G.np))), dim3(TPB), (int*)s->smask,
            (int64_t)s->G.np, 0);
-    s->z_host_n = 0;
     s->in_update = true;
     s->pending_deltas.clear();
     s->pending_index.clear();
@@ -1223,12 +1224,10 @@ int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, co
 int felt_update_end_impl(felt_surface* s) {
     const Geo& G = s->G;
     int rc;
-    const int zn = s->z_host_n;
-    // calc_affected into lookup 0 (m_grid_affected); its first zn entries are the delta list
+    // calc_affected into lookup 0 (m_grid_affected); its first entries are the delta list
     Lu A = felt_lu(s, 0);
-    if (zn > 0)
-        LAUNCH(k_lu_seed, dim3(blocks_for(zn)), dim3(TPB), A, s->zlist.ptr, zn, s->amark,
-               lu_span(s, 0));
+    LAUNCH(k_lu_seed, dim3(grid_for_lists(s)), dim3(TPB), A, s->zlist.ptr, &s->ctr->z_span[1],
+           s->amark, lu_span(s, 0));
     HIPX(hipMemsetAsync(&s->ctr->front[0], 0, sizeof(int), s->st));
     LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, 1);
     const int bfs_grid = grid_for_lists(s);
@@ -1341,7 +1340,6 @@ int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t
                     pbox.push_back(b);
                 }
     }
-    int zn = 0;
     if (!parts.empty()) {
         const size_t np = parts.size();
         if ((rc = s->tmp_i.ensure(2 * np + boxes.size() + 1))) return rc;
@@ -1354,16 +1352,12 @@ int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t
         DISPATCH(G.D, k_box_zero, dim3((unsigned)std::min<size_t>(np, 65535)), dim3(TPB), G, s->P,
                  s->tmp_i.ptr, s->tmp_i.ptr + np, (int)np, s->tmp_i.ptr + 2 * np, s->zlist.ptr,
                  s->ctr);
-        if ((rc = read_ctr(s))) return rc;
-        zn = s->hc.z_span[1];
     }
-    s->z_host_n = zn;
-    if (zn > 0) {
-        HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, G.nmem * sizeof(Speed), hipMemcpyHostToDevice,
-                            s->st));
-        DISPATCH(G.D, k_speed, dim3(blocks_for(zn)), dim3(TPB), G, s->P, s->zlist.ptr,
-                 &s->ctr->z_span[0], s->d_speed, s->zdelta.ptr);
-    }
+    // the zero-layer count stays on the device: the speed pass strides over its span
+    HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, G.nmem * sizeof(Speed), hipMemcpyHostToDevice,
+                        s->st));
+    DISPATCH(G.D, k_speed, dim3(grid_for_lists(s)), dim3(TPB), G, s->P, s->zlist.ptr,
+             &s->ctr->z_span[0], s->d_speed, s->zdelta.ptr);
     return felt_update_end_impl(s);
 }
 
@@ -1411,7 +1405,6 @@ int felt_update_end_public(felt_surface* s) {
             return felt_fail(FELT_E_DELTA, "delta at a position outside the zero layer");
         }
     }
-    s->z_host_n = (int)m;
     LAUNCH(k_set_span, dim3(1), dim3(1), &s->ctr->z_span[0], (int)m);
     s->pending_deltas.clear();
     s->pending_index.clear();

@@ -41,6 +41,7 @@ _SIGS = {
     "felt_server_batch": [_P, C.POINTER(_P)],
     "felt_server_polys": [_P, C.POINTER(_P)],
     "felt_server_start": [_P],
+    "felt_server_set_target": [_P, C.c_double],
     "felt_server_submit": [_P, C.POINTER(felt_request), C.c_size_t],
     "felt_server_tick": [_P, C.POINTER(C.c_int32)],
     "felt_server_poll": [_P, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_size_t)],
@@ -99,6 +100,10 @@ class Server:
 
     def start(self) -> None:
         self.api("felt_server_start", self.h)
+
+    def set_target(self, tick_ms: float) -> None:
+        """Admit requests only while the predicted tick time stays within tick_ms (0: off)."""
+        self.api("felt_server_set_target", self.h, float(tick_ms))
 
     def submit(self, reqs: Sequence[felt_request]) -> None:
         if not reqs:

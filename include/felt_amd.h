@@ -297,6 +297,10 @@ int felt_server_batch(felt_server* v, felt_batch** out);
 int felt_server_polys(felt_server* v, felt_polys** out);
 /* Reads every surface's band size and, when polygonising, marches every surface once. */
 int felt_server_start(felt_server* v);
+/* Latency target for admission: a request joins a tick only while a running model of tick
+ * time (a fixed part plus a cost per band voxel inside the in-flight boxes) stays within
+ * tick_ms; a request is always admitted into an empty tick.  0 (the default) disables it. */
+int felt_server_set_target(felt_server* v, double tick_ms);
 int felt_server_submit(felt_server* v, const felt_request* r, size_t n);
 /* One batched step; *stepped = requests advanced (0 when nothing is in flight). */
 int felt_server_tick(felt_server* v, int32_t* stepped);
