@@ -338,9 +338,9 @@ def run_served(args):
 
 
 def cpu_baseline(specs, budget_s):
-    """The CPU restatement (oracle/, OpenMP over partitions like the reference) stepping the
-    mixed workload's surfaces one after another from their initial state, until the time
-    budget is spent."""
+    """The CPU restatement (oracle/, OpenMP over partitions like the reference) on the mixed
+    workload from the same initial state: every surface is loaded, then passes step each surface
+    once in turn until the time budget is spent (only the update calls are timed)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     from felt_amd import workloads
@@ -351,25 +351,30 @@ def cpu_baseline(specs, budget_s):
     cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
     b = oracle_lib.binding()
     b.lib.felt_oracle_set_threads(cores)
-    done, ups, el = 0, 0, 0.0
+    surfaces = []
     for sp in specs:
-        if el >= budget_s:
-            break
         c = oracle_lib.surface(3, 3, sp.size, sp.part)
         c.b.lib.felt_oracle_surface_set_omp(c.h, 1)
         c.load_dense(sp.phi())
-        f = workloads.speed_of(sp.speed)
-        t0 = time.perf_counter()
-        c.update(f)
-        el += time.perf_counter() - t0
-        ups += int(c.stats().last_voxel_updates)
-        done += 1
+        surfaces.append((c, workloads.speed_of(sp.speed)))
+    passes, steps, ups, el = 0, 0, 0, 0.0
+    while el < budget_s and passes < 50:
+        for c, f in surfaces:
+            t0 = time.perf_counter()
+            c.update(f)
+            el += time.perf_counter() - t0
+            ups += int(c.stats().last_voxel_updates)
+            steps += 1
+            if el >= budget_s:
+                break
+        passes += 1
+        log(f"[bench] cpu pass {passes}: {el:.1f}s")
+    for c, _ in surfaces:
         c.close()
-    log(f"[bench] cpu: {done} surfaces stepped in {el:.1f}s")
     return {"value": ups / el, "unit": "voxel updates/s", "cores": cores, "kind": "port",
-            "sample": f"one update step of each of the first {done} surfaces of the same mixed "
-                      f"workload from the same initial state ({el:.1f}s of CPU work), CPU "
-                      "restatement with OpenMP over partitions"}
+            "sample": f"{steps} update steps ({passes} passes over the {len(specs)} surfaces) of "
+                      f"the same mixed workload from the same initial state ({el:.1f}s of CPU "
+                      "work), CPU restatement with OpenMP over partitions"}
 
 
 def main():
