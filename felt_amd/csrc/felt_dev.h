@@ -13,6 +13,7 @@
 //   part_mem[np]    partition -> member surface
 //   slot_part[cap]  brick slot -> partition, -1 when free
 //   slot_org[cap]   brick slot -> grid coordinates of its partition's first voxel, and member
+//                   (bits 0-29) with bit 30 set when the brick touches a face of its grid
 //   phi[cap*pv]     signed distances, one brick of pv = prod(part) voxels per slot, row-major
 //                   inside the brick with the last axis fastest (Util.hpp:23-40)
 //   lay[cap*pv]     layer list of each voxel (-L..L) or NONE: replaces the reference's per-child
@@ -96,8 +97,15 @@ struct Pool {
     const int* part_mem;
 };
 
+// slot_org.w: the member and whether the brick touches a face of its member's grid.
+constexpr int kMemberMask = 0x3fffffff;
+constexpr int kEdgeBrick = 1 << 30;
+constexpr int kFar = 0x7fffffff;  // a grid size no neighbour of an inner brick reaches
+
 // A voxel of the band being processed: id, member, coordinates in its brick and in its grid,
-// and its grid's size.
+// and its grid's size.  Every face neighbour of a voxel of a brick that touches no grid face is
+// inside the grid, so for those bricks `size` is left at kFar and the member's geometry is read
+// only when a neighbour lies in another brick.
 struct Vx {
     uint32_t v;
     int m;
@@ -148,8 +156,9 @@ __device__ __forceinline__ void decode(const Geo& G, const Pool& P, uint32_t v, 
     }
     leaf_coords<D>(G, v - slot * (uint32_t)G.pv, x.loc);
     const int4 o = P.slot_org[slot];
-    x.m = o.w;
-    const int4 r0 = mem_row0(P, o.w);
+    x.m = o.w & kMemberMask;
+    int4 r0 = make_int4(kFar, kFar, kFar, 0);
+    if (o.w & kEdgeBrick) r0 = mem_row0(P, x.m);
     x.g[0] = o.x + x.loc[0];
     x.size[0] = r0.x;
     if (D > 1) {

@@ -93,8 +93,9 @@ struct MemV {
 
 __device__ __forceinline__ MemV mem_view(const Pool& P, int slot) {
     const int4 o = P.slot_org[slot];
-    const Mem& M = P.mem[o.w];
-    return MemV{o.w, {M.size[0], M.size[1], M.size[2]}, {M.offset[0], M.offset[1], M.offset[2]},
+    const int m = o.w & kMemberMask;
+    const Mem& M = P.mem[m];
+    return MemV{m, {M.size[0], M.size[1], M.size[2]}, {M.offset[0], M.offset[1], M.offset[2]},
                 slot, {o.x, o.y, o.z}};
 }
 

@@ -668,8 +668,11 @@ __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* n
                 const Mem& M = P.mem[m];
                 int pc[3] = {0, 0, 0};
                 part_coords<3>(M, p - M.pbase, pc);
+                bool edge = false;
+                for (int a = 0; a < G.D; a++)
+                    if (pc[a] == 0 || (pc[a] + 1) * G.part[a] >= M.size[a]) edge = true;
                 P.slot_org[s] = make_int4(pc[0] * G.part[0], pc[1] * G.part[1],
-                                          pc[2] * G.part[2], m);
+                                          pc[2] * G.part[2], m | (edge ? kEdgeBrick : 0));
             }
             nflag[p] = 0;
         }
@@ -720,13 +723,16 @@ __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, 
                                                                __HIP_MEMORY_SCOPE_AGENT);
                             if (!in_band(layer_of(dq, G.L, &ctr->nan_events), G.L) &&
                                 cas_byte(&P.lay[vq], NONE, target)) {
+                                // the neighbour's brick may touch a grid face where x's
+                                // does not: give it the real grid size
+                                const int4 r0 = mem_row0(P, x.m);
                                 Vx y;
                                 y.v = vq;
                                 y.m = x.m;
                                 for (int b = 0; b < 3; b++) {
                                     y.loc[b] = 0;
                                     y.g[b] = q[b];
-                                    y.size[b] = x.size[b];
+                                    y.size[b] = b < D ? (b == 0 ? r0.x : (b == 1 ? r0.y : r0.z)) : 1;
                                 }
                                 leaf_coords<D>(G, (uint32_t)lq, y.loc);
                                 const float nd = distance<D>(G, P, y, dq, (float)side);
