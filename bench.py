@@ -196,6 +196,7 @@ def run_single(args):
     s.sync()
     total = time.perf_counter() - t_all
     ktab, roof = time_passes(s, args.pass_reps)
+    ray = ray_bench(s, size, args.rays) if args.rays > 0 else None
     res = {"workload": f"single large surface, plain GPU path: Surface<3,3> {size}^3 isogrid in "
                        f"{part}^3 partitions, harmonic-perturbed sphere r={150.0 * size / 512:g}, "
                        "speed clamp(-0.3(1+0.5n)+0.1curv), incremental marching cubes every 10 "
@@ -205,11 +206,34 @@ def run_single(args):
            "p50_step_ms": pctl(lat, 50), "p99_step_ms": pctl(lat, 99),
            "triangles_per_s": spx / march_s if march_s > 0 else None,
            "band_voxels_final": int(s.stats().band_voxels),
-           "roofline": roof, "kernels": ktab}
+           "roofline": roof, "kernels": ktab, "ray": ray}
     polys.close()
     s.close()
     log(f"[bench] single: {res['value'] / 1e9:.2f} G voxel updates/s")
     return res
+
+
+def ray_bench(s, size, n):
+    """Surface::ray throughput on the final state: n rays from a shell of radius 0.45*size about
+    the centre, aimed at points within 0.1*size of it; rays resident on the device."""
+    import numpy as np
+    rng = np.random.default_rng(0xFE1705)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    o = 0.45 * size * u
+    tgt = rng.uniform(-0.1 * size, 0.1 * size, size=(n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o, d = o.astype(np.float32), d.astype(np.float32)
+    us = s.time_ray(o, d, reps=5)
+    k = min(n, 1 << 16)
+    hits = s.ray(o[:k], d[:k])
+    frac = float(np.mean(~np.all(hits == np.float32(3.4028235e38), axis=1)))
+    log(f"[bench] ray: {n} rays in {us / 1e3:.2f} ms, {n / us:.1f} M rays/s, hit fraction {frac:.3f}")
+    return {"rays": n, "us_per_launch": us, "rays_per_s": n / (us * 1e-6), "hit_fraction": frac,
+            "workload": "Surface::ray from a shell of radius 0.45*size aimed within 0.1*size of "
+                        "the centre, on the final state; one thread per ray, timed with HIP events "
+                        "over 5 launches"}
 
 
 SERVED_SEED = 0xFE1704
@@ -389,6 +413,8 @@ def main():
     ap.add_argument("--part", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--pass-reps", type=int, default=20)
+    ap.add_argument("--rays", type=int, default=1 << 20, help="rays cast on the single surface "
+                    "(0 skips)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-served", action="store_true")

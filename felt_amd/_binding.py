@@ -19,6 +19,7 @@ FELT_E_NOMEM = -3
 FELT_E_CAPACITY = -4
 FELT_E_STATE = -5
 FELT_E_DELTA = -6
+RAY_MISS = float(np.finfo(np.float32).max)  # Surface::ray_miss, every component
 
 SPEED_CONSTANT = 0
 SPEED_CURVATURE = 1
@@ -132,6 +133,9 @@ class Binding:
                                                C.POINTER(C.c_double)]),
             "surface_time_pass": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+            "surface_ray": (C.c_int, [_P, _SZ, _F32P, _F32P, _F32P]),
+            "surface_time_ray": (C.c_int, [_P, _SZ, _F32P, _F32P, C.c_int,
+                                           C.POINTER(C.c_double)]),
             "polys_notify": (C.c_int, [_P]),
             "polys_march": (C.c_int, [_P]),
             "polys_invalidate": (C.c_int, [_P]),
@@ -156,7 +160,9 @@ class Binding:
             if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info",
                                                             "surface_set_graph",
                                                             "surface_time_pass",
-                                                            "surface_load_spheres"):
+                                                            "surface_load_spheres",
+                                                            "surface_ray",
+                                                            "surface_time_ray"):
                 continue
             f = getattr(L, prefix + name)
             f.restype = res
@@ -294,6 +300,26 @@ class Surface:
         out = np.empty(len(p), dtype=np.float32)
         self.b.call("surface_get", self.h, len(p), _ptr(p, _I32P), _ptr(out, _F32P))
         return out
+
+    def ray(self, origins, dirs) -> np.ndarray:
+        """Surface::ray for each (origin, direction) row; a miss reads RAY_MISS (FLT_MAX)."""
+        o = np.ascontiguousarray(np.asarray(origins, dtype=np.float32).reshape(-1, self.D))
+        d = np.ascontiguousarray(np.asarray(dirs, dtype=np.float32).reshape(-1, self.D))
+        if o.shape != d.shape:
+            raise ValueError("origins and directions differ in count")
+        out = np.empty_like(o)
+        self.b.call("surface_ray", self.h, len(o), _ptr(o, _F32P), _ptr(d, _F32P),
+                    _ptr(out, _F32P))
+        return out
+
+    def time_ray(self, origins, dirs, reps: int = 10) -> float:
+        """Average device microseconds of one ray launch over these rays."""
+        o = np.ascontiguousarray(np.asarray(origins, dtype=np.float32).reshape(-1, self.D))
+        d = np.ascontiguousarray(np.asarray(dirs, dtype=np.float32).reshape(-1, self.D))
+        us = C.c_double()
+        self.b.call("surface_time_ray", self.h, len(o), _ptr(o, _F32P), _ptr(d, _F32P),
+                    int(reps), C.byref(us))
+        return float(us.value)
 
     def layer_size(self, layer: int) -> int:
         n = C.c_uint64()

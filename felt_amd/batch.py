@@ -39,6 +39,7 @@ _SIGS = {
     "felt_batch_layer_size": [_P, C.c_int32, C.c_int, _U64P],
     "felt_batch_layer_positions": [_P, C.c_int32, C.c_int, _I32P, _SZ, _SZP],
     "felt_batch_stats": [_P, C.c_int32, C.POINTER(felt_stats)],
+    "felt_batch_ray": [_P, _SZ, _I32P, _F32P, _F32P, _F32P],
     "felt_batch_step_info": [_P, _U64P, _U64P],
     "felt_batch_set_graph": [_P, C.c_int],
     "felt_batch_time_pass": [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -185,6 +186,18 @@ class Batch:
         st = felt_stats()
         self.api("felt_batch_stats", self.h, int(i), C.byref(st))
         return st
+
+    def ray(self, members, origins, dirs) -> np.ndarray:
+        """Surface::ray of ray i against member members[i]; a miss reads FLT_MAX."""
+        mv = _arr(members, np.int32)
+        o = np.ascontiguousarray(np.asarray(origins, dtype=np.float32).reshape(-1, self.D))
+        d = np.ascontiguousarray(np.asarray(dirs, dtype=np.float32).reshape(-1, self.D))
+        if not (len(mv) == len(o) == len(d)):
+            raise ValueError("members, origins and directions differ in count")
+        out = np.empty_like(o)
+        self.api("felt_batch_ray", self.h, len(o), _ptr(mv, _I32P), _ptr(o, _F32P),
+                 _ptr(d, _F32P), _ptr(out, _F32P))
+        return out
 
     def step_info(self):
         u, r = C.c_uint64(), C.c_uint64()

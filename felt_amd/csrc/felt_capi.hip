@@ -376,6 +376,31 @@ API int felt_surface_get(felt_surface* s, size_t n, const int32_t* pos, float* o
     return felt_get_impl(s, 0, n, g.data(), out);
 }
 
+API int felt_surface_ray(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                         float* out) {
+    CHECK_S(s);
+    if (n && (!origins || !dirs || !out)) return felt_fail(FELT_E_ARG, "null ray array");
+    return felt_ray_impl(s, nullptr, 0, n, origins, dirs, out);
+}
+
+API int felt_surface_ray_device(felt_surface* s, size_t n, const float* origins,
+                                const float* dirs, float* out) {
+    CHECK_S(s);
+    if (n && (!origins || !dirs || !out)) return felt_fail(FELT_E_ARG, "null ray array");
+    if (n > (size_t)0x7fffffff / 4) return felt_fail(FELT_E_ARG, "too many rays");
+    int rc = felt_ray_device(s, nullptr, 0, n, origins, dirs, out);
+    if (rc) return rc;
+    if (hipStreamSynchronize(s->st) != hipSuccess) return felt_fail(FELT_E_DEVICE, "ray kernel failed");
+    return FELT_OK;
+}
+
+API int felt_surface_time_ray(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                              int reps, double* us) {
+    CHECK_S(s);
+    if (!origins || !dirs || !us) return felt_fail(FELT_E_ARG, "null argument");
+    return felt_time_ray_impl(s, n, origins, dirs, reps, us);
+}
+
 API int felt_surface_layer_size(felt_surface* s, int layer, uint64_t* n) {
     CHECK_S(s);
     return layer_size_m(s, 0, layer, n);
@@ -730,6 +755,14 @@ API int felt_batch_update_boxes(felt_batch* b, int32_t n, const int32_t* ids, co
         }
     }
     return felt_update_boxes_impl(s, n, id.data(), l3.data(), h3.data(), speeds);
+}
+
+API int felt_batch_ray(felt_batch* b, size_t n, const int32_t* members, const float* origins,
+                       const float* dirs, float* out) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    if (n && (!members || !origins || !dirs || !out)) return felt_fail(FELT_E_ARG, "null ray array");
+    return felt_ray_impl(b->s, members, 0, n, origins, dirs, out);
 }
 
 API int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n) {

@@ -160,3 +160,12 @@ size_t felt_list_cap_for(const felt_surface* s, size_t band);
 // Device bytes held by the handle.
 uint64_t felt_device_bytes(const felt_surface* s);
 int felt_sync_impl(felt_surface* s);
+// Surface::ray for n rays of member m0 (or members[i] when given), host arrays in and out.
+int felt_ray_impl(felt_surface* s, const int32_t* members, int m0, size_t n, const float* origins,
+                  const float* dirs, float* out);
+// The same on device arrays, queued on the surface's stream.
+int felt_ray_device(felt_surface* s, const int* d_members, int m0, size_t n, const float* d_org,
+                    const float* d_dir, float* d_out);
+// Average device time of one launch over n rays of member 0, reps launches between two events.
+int felt_time_ray_impl(felt_surface* s, size_t n, const float* origins, const float* dirs, int reps,
+                       double* us);

@@ -144,6 +144,21 @@ int felt_surface_partition_active(felt_surface* s, uint8_t* out, size_t n);
 /* Number of layer-k voxels in partition pos_idx (layer_size helper, test_Surface.cpp:1720-1737). */
 int felt_surface_partition_layer_size(felt_surface* s, uint64_t pos_idx, int layer, uint64_t* n);
 int felt_surface_stats(felt_surface* s, felt_stats* out);
+/* ray(origin, dir) for n rays (Surface.hpp:468-616): origins and dirs hold n*D floats in world
+ * coordinates, dirs unit vectors; out receives n*D floats, the hit on the zero curve or FLT_MAX
+ * in every component for a miss (Surface::ray_miss, Surface.hpp:1805-1807).  Rays that are not
+ * finite, start beyond 2^20 or whose direction is not of unit length fail with FELT_E_ARG. */
+int felt_surface_ray(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                     float* out);
+/* The same with origins, dirs and out already in device memory (hipMalloc'd, n*D floats each);
+ * returns once the results are written. */
+int felt_surface_ray_device(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                            float* out);
+/* Measurement: the average device time in microseconds of one felt_surface_ray launch over the
+ * given rays (copied to the device once), from `reps` launches bracketed by two HIP events on the
+ * surface's stream. */
+int felt_surface_time_ray(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                          int reps, double* us);
 /* Launch a global update(fn) with a device functor as one captured HIP graph (default on):
  * the first three rounds run without host reads, later rounds are launched from the host.
  * Off: the same kernels are launched one by one.  Not a reference member. */
@@ -242,6 +257,9 @@ int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos
                                size_t* n);
 /* Stats of member i; last_voxel_updates / last_rounds are those of the last batch step. */
 int felt_batch_stats(felt_batch* b, int32_t i, felt_stats* out);
+/* felt_surface_ray over the members of a batch: ray i is cast against member members[i]. */
+int felt_batch_ray(felt_batch* b, size_t n, const int32_t* members, const float* origins,
+                   const float* dirs, float* out);
 int felt_batch_step_info(felt_batch* b, uint64_t* voxel_updates, uint64_t* rounds);
 int felt_batch_set_graph(felt_batch* b, int enable);
 int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launch,

@@ -7,6 +7,7 @@
 //   update_start(), delta(pos, v), update_end()   Surface.hpp:393-459
 //   isogrid().snapshot(), isogrid().get(pos)      PartitionedMixin.hpp:594-660
 //   delta(k), status_change(k), layer_idx(id), stats(), is_intersected(p)  Surface.hpp:623-696
+//   ray(origin, dir), ray_miss                 Surface.hpp:468-616, 1805-1807
 //   Polys(surface), notify(), march(), invalidate(), changes(), children().get(p)
 //                                            Polys.hpp:56-201, Poly.hpp:176-189
 // Functors: a felt_speed descriptor runs on the device; a host callable with one of the
@@ -18,6 +19,7 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -47,6 +49,12 @@ inline void check(int rc) {
 template <int D>
 VecDi<D> constant(int v) {
     VecDi<D> a{};
+    a.fill(v);
+    return a;
+}
+template <int D>
+VecDf<D> constant_f(float v) {
+    VecDf<D> a{};
     a.fill(v);
     return a;
 }
@@ -93,6 +101,9 @@ public:
         ListIdx active_isogrid_partitions;
         ListIdx active_delta_partitions;
     };
+    // What ray() returns for a miss: every component the largest float.
+    static inline const VecDf<D> ray_miss =
+        detail::constant_f<D>(std::numeric_limits<Distance>::max());
 
     explicit Surface(const VecDi<D>& size, const VecDi<D>& size_partition = detail::constant<D>(8))
         : m_size(size), m_part(size_partition) {
@@ -140,6 +151,23 @@ public:
         return parts(FELT_LIST_STATUS_CHANGE, layer_idx_);
     }
     static constexpr TupleIdx layer_idx(LayerId id) { return TupleIdx(id + L); }
+    // Cast a ray to the zero layer; ray_miss when nothing is hit.  dir is a unit vector.
+    VecDf<D> ray(const VecDf<D>& origin, const VecDf<D>& dir) const {
+        VecDf<D> hit{};
+        detail::check(felt_surface_ray(m_h.get(), 1, origin.data(), dir.data(), hit.data()));
+        return hit;
+    }
+    // Many rays in one launch.
+    std::vector<VecDf<D>> ray(const std::vector<VecDf<D>>& origins,
+                              const std::vector<VecDf<D>>& dirs) const {
+        if (origins.size() != dirs.size())
+            throw std::domain_error("origins and directions differ in count");
+        std::vector<VecDf<D>> hits(origins.size());
+        if (hits.empty()) return hits;
+        detail::check(felt_surface_ray(m_h.get(), origins.size(), origins.data()->data(),
+                                       dirs.data()->data(), hits.data()->data()));
+        return hits;
+    }
     Stats stats() const {
         felt_stats st{};
         detail::check(felt_surface_stats(m_h.get(), &st));

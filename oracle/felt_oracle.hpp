@@ -403,6 +403,12 @@ public:
         return int(std::round(t));
     }
     int layer_id(const Vi<D>& pos) { return layer_id(iso.get(pos)); }
+    /* The same without counting NaN (read-only queries such as ray). */
+    int layer_id_peek(R v) const {
+        R t = v + R(FLT_EPSILON);
+        if (std::isnan(t)) return L + 1;
+        return int(std::round(t));
+    }
 
     template <class Fn>
     static void neighs(Vi<D> pos, Fn&& fn) {            // NumericMixin.hpp:489-500
@@ -563,6 +569,150 @@ public:
         }
         for (int a = 0; a < D; a++) gv[a] = gv[a] / R(1);
         return gv;
+    }
+
+    /* ---- raycasting (Surface.hpp:468-616, 1450-1598) ---- */
+    static constexpr R TINY = R(0.00001f);                // Surface.hpp:67
+    static Vf ray_miss() {                                // Surface.hpp:1805-1807
+        Vf m{};
+        m.fill(std::numeric_limits<R>::max());
+        return m;
+    }
+    static R dotv(const Vf& a, const Vf& b) {             // Eigen dot: v0 + (v1 + v2)
+        Vf p{};
+        for (int i = 0; i < D; i++) p[i] = a[i] * b[i];
+        return sum(p);
+    }
+    static void normalize(Vf& n) {
+        Vf p{};
+        for (int i = 0; i < D; i++) p[i] = n[i] * n[i];
+        const R z = sum(p);
+        if (z > R(0)) {
+            const R r = std::sqrt(z);
+            for (int i = 0; i < D; i++) n[i] = n[i] / r;
+        }
+    }
+    bool child_inside(Idx p, const Vf& pos) const {       // Util.hpp:94-106 on the child
+        const Vi<D> o = g.child_offset(p);
+        for (int a = 0; a < D; a++) {
+            if (float(pos[a]) >= float(o[a] + g.part[a])) return false;
+            if (float(pos[a]) < float(o[a])) return false;
+        }
+        return true;
+    }
+    /* Ray within one child: half-voxel march until the zero layer, then converge. */
+    Vf ray_child(const Vf& origin, const Vf& dir, Idx p) const {
+        Vf ps = origin;
+        R t = R(0);
+        while (child_inside(p, ps)) {
+            if (layer_id_peek(interp(ps)) == 0) {
+                Vf n = grad(ps);
+                normalize(n);
+                if (dotv(n, dir) < R(0)) {
+                    for (int step = 0; step < 100; step++) {
+                        const R dist = interp(ps);
+                        for (int i = 0; i < D; i++) ps[i] = ps[i] - n[i] * dist;
+                        if (!g.inside(ps)) return ray_miss();
+                        if (std::abs(dist) <= TINY || dotv(n, dir) >= R(0)) break;
+                        n = grad(ps);
+                        normalize(n);
+                    }
+                    return ps;
+                }
+            }
+            t += R(0.5f);
+            for (int i = 0; i < D; i++) ps[i] = origin[i] + dir[i] * t;
+        }
+        return ray_miss();
+    }
+    R round_to_next(int dim, R dir, R pos) const {
+        R v = (pos - R(g.offset[dim])) / R(g.part[dim]);
+        v = dir == R(-1) ? std::floor(v) : std::ceil(v);
+        v *= R(g.part[dim]);
+        v += R(g.offset[dim]);
+        return v;
+    }
+    struct ChildHit {
+        Vf pos;
+        Idx p;
+    };
+    /* Intersect the ray with the plane normal . x + d = 0 (normal -dir_dim along dim), step
+     * TINY past it, and keep the child there if it tracks layer 0 or +-1. */
+    bool ray_check_track_child(std::vector<ChildHit>& hits, const Vf& o, const Vf& dir, int dim,
+                               R normal_dim, R d) const {
+        const R t = -(d + normal_dim * o[dim]) / (normal_dim * dir[dim]);
+        Vf pi{};
+        for (int i = 0; i < D; i++) pi[i] = (o[i] + dir[i] * t) + dir[i] * TINY;
+        for (int i = 0; i < D; i++)
+            if ((dir[i] > R(0) && pi[i] > R(g.size[i])) || (dir[i] < R(0) && pi[i] < R(g.offset[i])))
+                return false;
+        if (!g.inside(pi)) return true;
+        Vi<D> pf{};
+        for (int i = 0; i < D; i++) pf[i] = int(std::floor(pi[i]));
+        const Idx p = g.child_of(pf);
+        const Kid<R>& k = iso.kids[p];
+        if (k.lists[layer_idx(0)].size() || k.lists[layer_idx(1)].size() ||
+            k.lists[layer_idx(-1)].size())
+            hits.push_back(ChildHit{pi, p});
+        return true;
+    }
+    Vf ray(const Vf& o, const Vf& dir) const {
+        if (g.inside(o)) {
+            Vi<D> oi{};
+            for (int a = 0; a < D; a++) oi[a] = int(o[a]);    // Eigen cast: truncation
+            const Vf h = ray_child(o, dir, g.child_of(oi));
+            if (h != ray_miss()) return h;
+        }
+        std::vector<ChildHit> hits;
+        for (int dim = 0; dim < D; dim++) {
+            const R dir_dim = R((R(0) < dir[dim]) - (dir[dim] < R(0)));
+            if (dir_dim == R(0)) continue;
+            R ppd = round_to_next(dim, dir_dim, o[dim]);
+            Vf plane{};
+            plane[dim] = ppd;
+            if (!g.inside(plane)) {
+                R gd;
+                if (dir_dim == R(-1)) {
+                    gd = R(g.offset[dim] + g.size[dim]);
+                    if (ppd < gd) continue;
+                } else {
+                    gd = R(g.offset[dim]);
+                    if (ppd > gd) continue;
+                }
+                plane[dim] = gd;
+            }
+            const R normal_dim = -dir_dim;
+            ray_check_track_child(hits, o, dir, dim, normal_dim, plane[dim] * dir_dim);
+            ppd = round_to_next(dim, dir_dim, plane[dim]);
+            if (ppd != plane[dim]) {
+                plane[dim] = ppd;
+                if (!ray_check_track_child(hits, o, dir, dim, normal_dim, plane[dim] * dir_dim))
+                    continue;
+            }
+            const R cs = R(g.part[dim]);
+            while (true) {
+                plane[dim] += dir_dim * cs;
+                if (!ray_check_track_child(hits, o, dir, dim, normal_dim, plane[dim] * dir_dim))
+                    break;
+            }
+        }
+        // front to back (ties keep insertion order), then drop repeats of a child
+        auto dist2 = [&](const ChildHit& h) {
+            Vf dv{};
+            for (int i = 0; i < D; i++) dv[i] = h.pos[i] - o[i];
+            return dotv(dv, dv);
+        };
+        std::stable_sort(hits.begin(), hits.end(), [&](const ChildHit& a, const ChildHit& b) {
+            return dist2(a) < dist2(b);
+        });
+        hits.erase(std::unique(hits.begin(), hits.end(),
+                               [](const ChildHit& a, const ChildHit& b) { return a.p == b.p; }),
+                   hits.end());
+        for (const ChildHit& h : hits) {
+            const Vf hit = ray_child(h.pos, dir, h.p);
+            if (hit != ray_miss()) return hit;
+        }
+        return ray_miss();
     }
 
     /* ---- update API ---- */

@@ -101,6 +101,7 @@ struct Handle {
     virtual int stats(felt_stats* st) = 0;
     virtual int geometry(int32_t* cs, uint64_t* np) = 0;
     virtual void set_omp(bool on) = 0;
+    virtual int ray(size_t n, const float* origin, const float* dir, float* out) = 0;
     /* polys */
     virtual int polys_create() = 0;
     virtual int polys_notify() = 0;
@@ -190,6 +191,19 @@ struct HandleT final : Handle {
         Vi<D> r{};
         for (int a = 0; a < D; a++) r[a] = v[a] - s.g.offset[a];
         return r;
+    }
+    /* ray(origin, dir) per ray (Surface.hpp:468-616); a miss gives ray_miss (FLT_MAX). */
+    int ray(size_t n, const float* origin, const float* dir, float* out) override {
+        for (size_t i = 0; i < n; i++) {
+            typename Surface<D, R>::Vf o{}, d{};
+            for (int a = 0; a < D; a++) {
+                o[a] = R(origin[i * D + a]);
+                d[a] = R(dir[i * D + a]);
+            }
+            const auto h = s.ray(o, d);
+            for (int a = 0; a < D; a++) out[i * D + a] = float(h[a]);
+        }
+        return FELT_OK;
     }
     int update_start() override {
         s.update_start();
@@ -447,6 +461,9 @@ OAPI int felt_oracle_surface_partition_layer_size(felt_oracle_surface* s, uint64
     return H(s)->partition_layer_size(p, layer, n);
 }
 OAPI int felt_oracle_surface_stats(felt_oracle_surface* s, felt_stats* st) { return H(s)->stats(st); }
+OAPI int felt_oracle_surface_ray(felt_oracle_surface* s, size_t n, const float* origin, const float* dir, float* out) {
+    return H(s)->ray(n, origin, dir, out);
+}
 OAPI int felt_oracle_surface_sync(felt_oracle_surface*) { return FELT_OK; }
 
 OAPI int felt_oracle_polys_create(felt_oracle_surface* s) { return H(s)->polys_create(); }

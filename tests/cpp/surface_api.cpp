@@ -2,6 +2,7 @@
 // tests are (test_Surface.cpp:49-160, 1031-1053; test_Poly.cpp:278-358).  Exit status 0 when
 // every check holds; prints the first failure otherwise.
 #include <felt_amd/Surface.hpp>
+#include <hip/hip_runtime_api.h>
 
 #include <cmath>
 #include <cstdio>
@@ -73,6 +74,36 @@ int main() {
                                       v.norm[2] * v.norm[2]);
             CHECK(std::fabs(n - 1.0f) < 1e-5f);
         }
+    }
+    {
+        // raycasting (test_Surface.cpp:1278-1330): radius-3 surface, rays along +x
+        Surface<3, 3> surface(VecDi<3>{32, 32, 32}, VecDi<3>{5, 5, 5});
+        surface.seed(VecDi<3>{0, 0, 0});
+        for (int i = 0; i < 3; i++) surface.update([]() { return -1.0f; });
+        const VecDf<3> hit = surface.ray(VecDf<3>{-35.0f, 0, 0}, VecDf<3>{1, 0, 0});
+        CHECK(std::fabs(hit[0] + 3.0f) < 1e-4f && std::fabs(hit[1]) < 1e-4f &&
+              std::fabs(hit[2]) < 1e-4f);
+        CHECK(surface.ray(VecDf<3>{0, 0, 0}, VecDf<3>{1, 0, 0}) == surface.ray_miss);
+        const auto hits = surface.ray({VecDf<3>{-6.0f, 0, 0}, VecDf<3>{0, 0, 0}},
+                                      {VecDf<3>{1, 0, 0}, VecDf<3>{1, 0, 0}});
+        CHECK(hits.size() == 2 && std::fabs(hits[0][0] + 3.0f) < 1e-4f && hits[1] == surface.ray_miss);
+        // the same rays from device memory (felt_surface_ray_device)
+        const float org[6] = {-6.0f, 0, 0, 0, 0, 0}, dir[6] = {1, 0, 0, 1, 0, 0};
+        float *d_org = nullptr, *d_dir = nullptr, *d_out = nullptr, out[6] = {};
+        CHECK(hipMalloc(&d_org, sizeof(org)) == hipSuccess);
+        CHECK(hipMalloc(&d_dir, sizeof(dir)) == hipSuccess);
+        CHECK(hipMalloc(&d_out, sizeof(out)) == hipSuccess);
+        CHECK(hipMemcpy(d_org, org, sizeof(org), hipMemcpyHostToDevice) == hipSuccess);
+        CHECK(hipMemcpy(d_dir, dir, sizeof(dir), hipMemcpyHostToDevice) == hipSuccess);
+        CHECK(felt_surface_ray_device(surface.handle(), 2, d_org, d_dir, d_out) == FELT_OK);
+        CHECK(hipMemcpy(out, d_out, sizeof(out), hipMemcpyDeviceToHost) == hipSuccess);
+        for (int a = 0; a < 3; a++) {
+            CHECK(out[a] == hits[0][a]);
+            CHECK(out[3 + a] == surface.ray_miss[a]);
+        }
+        (void)hipFree(d_org);
+        (void)hipFree(d_dir);
+        (void)hipFree(d_out);
     }
     std::printf("surface_api: all checks passed\n");
     return 0;
