@@ -134,6 +134,8 @@ class Binding:
             "surface_time_pass": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.POINTER(C.c_int)]),
             "surface_ray": (C.c_int, [_P, _SZ, _F32P, _F32P, _F32P]),
+            "surface_save": (C.c_int, [_P, C.POINTER(C.c_uint8), _SZ, _SZP]),
+            "surface_load": (C.c_int, [C.POINTER(C.c_uint8), _SZ, C.POINTER(_P)]),
             "surface_time_ray": (C.c_int, [_P, _SZ, _F32P, _F32P, C.c_int,
                                            C.POINTER(C.c_double)]),
             "polys_notify": (C.c_int, [_P]),
@@ -162,7 +164,9 @@ class Binding:
                                                             "surface_time_pass",
                                                             "surface_load_spheres",
                                                             "surface_ray",
-                                                            "surface_time_ray"):
+                                                            "surface_time_ray",
+                                                            "surface_save",
+                                                            "surface_load"):
                 continue
             f = getattr(L, prefix + name)
             f.restype = res
@@ -184,6 +188,31 @@ def _vec(v: Sequence[int], D: int) -> np.ndarray:
     return a
 
 
+def archive_geometry(data: bytes):
+    """(D, L, size, part) of a surface archive: the child size is its first Eigen vector, the
+    background child's list tuple holds 2L+1 lists, and the grid size is the isogrid's Size just
+    before the trailing Spatial fields (dx and two vectors)."""
+    import struct
+    if len(data) < 12:
+        raise FeltError(FELT_E_ARG, "not a surface archive")
+    D = struct.unpack_from("<i", data, 4)[0]
+    if D not in (2, 3):
+        raise FeltError(FELT_E_ARG, "not a surface archive")
+    V = 16 + 4 * D
+    part = struct.unpack_from(f"<{D}i", data, 4 + 16)
+    at = 4 + V
+    at += 4                                                   # background child: bg
+    at += 8 + 4 * struct.unpack_from("<Q", data, at)[0]        # its data
+    at += 8                                                   # its lookup's background
+    at += 8 + 8 * struct.unpack_from("<Q", data, at)[0]        # its lookup data
+    nl = struct.unpack_from("<i", data, at)[0]
+    end = len(data) - (4 + 2 * V) - 3 * V
+    size = struct.unpack_from(f"<{D}i", data, end + 16)
+    if nl < 3 or nl % 2 == 0:
+        raise FeltError(FELT_E_ARG, "not a surface archive")
+    return D, (nl - 1) // 2, tuple(size), tuple(part)
+
+
 class Surface:
     """Mirror of ``Felt::Surface<D,L>`` (Surface.hpp:49-696) over a felt ABI binding."""
 
@@ -202,12 +231,16 @@ class Surface:
                          C.byref(h))
         else:
             binding.call("surface_create", D, L, _ptr(sz, _I32P), _ptr(pt, _I32P), C.byref(h))
+        self._setup(h)
+
+    def _setup(self, h):
+        binding = self.b
         self.h = h
         self.offset = tuple(-(s // 2) for s in self.size)
         cs = np.zeros(3, np.int32)
         npart = C.c_uint64()
         binding.call("surface_geometry", self.h, _ptr(cs, _I32P), C.byref(npart))
-        self.children_size = tuple(int(x) for x in cs[:D])
+        self.children_size = tuple(int(x) for x in cs[:self.D])
         self.num_partitions = int(npart.value)
         self.children_offset = tuple(int(np.fix(o / p)) for o, p in zip(self.offset, self.part))
         self._polys = None
@@ -300,6 +333,28 @@ class Surface:
         out = np.empty(len(p), dtype=np.float32)
         self.b.call("surface_get", self.h, len(p), _ptr(p, _I32P), _ptr(out, _F32P))
         return out
+
+    def save(self) -> bytes:
+        """Surface::save: the isogrid as the reference's cereal binary archive."""
+        n = C.c_size_t()
+        self.b.call("surface_save", self.h, None, 0, C.byref(n))
+        buf = (C.c_uint8 * max(n.value, 1))()
+        self.b.call("surface_save", self.h, buf, n.value, C.byref(n))
+        return bytes(buf[: n.value])
+
+    @classmethod
+    def load(cls, binding: "Binding", data: bytes) -> "Surface":
+        """Surface::load: a new surface from a cereal archive of an isogrid."""
+        D, L, size, part = archive_geometry(data)
+        buf = (C.c_uint8 * (len(data) + 1)).from_buffer_copy(data + b"\0")
+        h = _P()
+        binding.call("surface_load", buf, len(data), C.byref(h))
+        obj = Surface.__new__(cls)
+        obj.b = binding
+        obj.D, obj.L, obj.NL = D, L, 2 * L + 1
+        obj.size, obj.part = size, part
+        obj._setup(h)
+        return obj
 
     def ray(self, origins, dirs) -> np.ndarray:
         """Surface::ray for each (origin, direction) row; a miss reads RAY_MISS (FLT_MAX)."""

@@ -40,6 +40,9 @@ _SIGS = {
     "felt_batch_layer_positions": [_P, C.c_int32, C.c_int, _I32P, _SZ, _SZP],
     "felt_batch_stats": [_P, C.c_int32, C.POINTER(felt_stats)],
     "felt_batch_ray": [_P, _SZ, _I32P, _F32P, _F32P, _F32P],
+    "felt_batch_save": [_P, C.c_int32, C.POINTER(C.c_uint8), _SZ, _SZP],
+    "felt_batch_load": [_P, C.c_int32, C.POINTER(C.c_uint8), _SZ],
+    "felt_batch_clear": [_P, C.c_int32],
     "felt_batch_step_info": [_P, _U64P, _U64P],
     "felt_batch_set_graph": [_P, C.c_int],
     "felt_batch_time_pass": [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -198,6 +201,23 @@ class Batch:
         self.api("felt_batch_ray", self.h, len(o), _ptr(mv, _I32P), _ptr(o, _F32P),
                  _ptr(d, _F32P), _ptr(out, _F32P))
         return out
+
+    def save(self, i: int) -> bytes:
+        """Surface::save of member i: the reference's cereal archive of its isogrid."""
+        n = C.c_size_t()
+        self.api("felt_batch_save", self.h, int(i), None, 0, C.byref(n))
+        buf = (C.c_uint8 * max(n.value, 1))()
+        self.api("felt_batch_save", self.h, int(i), buf, n.value, C.byref(n))
+        return bytes(buf[: n.value])
+
+    def load(self, i: int, data: bytes) -> None:
+        """Replace member i's state with an archive of the same geometry."""
+        buf = (C.c_uint8 * (len(data) + 1)).from_buffer_copy(data + b"\0")
+        self.api("felt_batch_load", self.h, int(i), buf, len(data))
+
+    def clear(self, i: int) -> None:
+        """Release member i's bricks; it becomes an empty grid."""
+        self.api("felt_batch_clear", self.h, int(i))
 
     def step_info(self):
         u, r = C.c_uint64(), C.c_uint64()

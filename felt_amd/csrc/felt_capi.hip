@@ -401,6 +401,29 @@ API int felt_surface_time_ray(felt_surface* s, size_t n, const float* origins, c
     return felt_time_ray_impl(s, n, origins, dirs, reps, us);
 }
 
+API int felt_surface_save(felt_surface* s, uint8_t* dst, size_t cap, size_t* n) {
+    CHECK_S(s);
+    if (!n) return felt_fail(FELT_E_ARG, "null size output");
+    if (s->in_update) return felt_fail(FELT_E_STATE, "save inside update_start / update_end");
+    return felt_save_impl(s, 0, dst, cap, n);
+}
+
+API int felt_surface_load(const uint8_t* src, size_t n, felt_surface** out) {
+    if (!src || !out) return felt_fail(FELT_E_ARG, "null argument");
+    int D = 0, L = 0, size[3] = {1, 1, 1}, part[3] = {1, 1, 1};
+    int rc = felt_archive_geometry(src, n, &D, &L, size, part);
+    if (rc) return rc;
+    felt_surface* s = nullptr;
+    if ((rc = make_pool(D, L, part, 1, size, 0, &s))) return rc;
+    if ((rc = felt_load_impl(s, 0, src, n))) {
+        felt_surface_release(s);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return FELT_OK;
+}
+
 API int felt_surface_layer_size(felt_surface* s, int layer, uint64_t* n) {
     CHECK_S(s);
     return layer_size_m(s, 0, layer, n);
@@ -763,6 +786,40 @@ API int felt_batch_ray(felt_batch* b, size_t n, const int32_t* members, const fl
     CHECK_IDLE(b);
     if (n && (!members || !origins || !dirs || !out)) return felt_fail(FELT_E_ARG, "null ray array");
     return felt_ray_impl(b->s, members, 0, n, origins, dirs, out);
+}
+
+API int felt_batch_save(felt_batch* b, int32_t i, uint8_t* dst, size_t cap, size_t* n) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    CHECK_M(b, i);
+    if (!n) return felt_fail(FELT_E_ARG, "null size output");
+    return felt_save_impl(b->s, i, dst, cap, n);
+}
+
+API int felt_batch_load(felt_batch* b, int32_t i, const uint8_t* src, size_t n) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    CHECK_M(b, i);
+    if (!src) return felt_fail(FELT_E_ARG, "null archive");
+    int D = 0, L = 0, size[3] = {1, 1, 1}, part[3] = {1, 1, 1};
+    int rc = felt_archive_geometry(src, n, &D, &L, size, part);
+    if (rc) return rc;
+    const felt_surface* s = b->s;
+    if (D != s->G.D || L != s->G.L) return felt_fail(FELT_E_ARG, "archive D or L differs from the batch's");
+    for (int a = 0; a < D; a++)
+        if (size[a] != s->mems[i].size[a] || part[a] != s->G.part[a])
+            return felt_fail(FELT_E_ARG, "archive grid differs from the member's");
+    return felt_load_impl(b->s, i, src, n);
+}
+
+API int felt_batch_clear(felt_batch* b, int32_t i) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    CHECK_M(b, i);
+    int rc = felt_member_clear_impl(b->s, i);
+    if (rc) return rc;
+    if (b->s->polys) return felt_polys_invalidate_impl(b->s->polys);
+    return FELT_OK;
 }
 
 API int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n) {

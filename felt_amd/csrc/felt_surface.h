@@ -166,6 +166,12 @@ int felt_ray_impl(felt_surface* s, const int32_t* members, int m0, size_t n, con
 // The same on device arrays, queued on the surface's stream.
 int felt_ray_device(felt_surface* s, const int* d_members, int m0, size_t n, const float* d_org,
                     const float* d_dir, float* d_out);
+// Checkpoints in the reference's on-disk format (cereal archive of the isogrid).
+int felt_archive_geometry(const uint8_t* src, size_t n, int* D, int* L, int* size, int* part);
+int felt_save_impl(felt_surface* s, int m, uint8_t* dst, size_t cap, size_t* n);
+int felt_load_impl(felt_surface* s, int m, const uint8_t* src, size_t n);
+int felt_member_clear_impl(felt_surface* s, int m);
+int felt_alloc_listed(felt_surface* s, int grid);
 // Average device time of one launch over n rays of member 0, reps launches between two events.
 int felt_time_ray_impl(felt_surface* s, size_t n, const float* origins, const float* dirs, int reps,
                        double* us);

@@ -348,7 +348,10 @@ __global__ void k_load_scan(Geo G, Pool P, int m, F f, int* has, Ctr* ctr) {
         has[lp] = any;
         int q[3] = {0, 0, 0};
         for (int a = 0; a < D; a++) q[a] = pc[a] * G.part[a];
-        if (inside<D>(M.size, q))
+        // an activated partition keeps the grid background, as a new child does when it is
+        // activated; an inactive one reads the side of the surface its first voxel is on
+        if (any) P.pbg[p] = (float)(G.L + 1);
+        else if (inside<D>(M.size, q))
             P.pbg[p] = f.at(q, D) < 0.0f ? -(float)(G.L + 1) : (float)(G.L + 1);
     }
 }
@@ -1459,6 +1462,14 @@ static int load_field(felt_surface* s, int m, const F& f) {
            s->free_stack, s->ctr, always(s));
     hipLaunchKernelGGL(fill, dim3(M.np), dim3(bt), 0, s->st, G, s->P, m, f, s->tmp_i.ptr, s->ctr);
     return read_ctr(s);
+}
+
+// Activates the ctr->need_n partitions listed in s->nlist (k_alloc), queued on the stream.
+int felt_alloc_listed(felt_surface* s, int grid) {
+    LAUNCH(k_alloc, dim3(std::max(1, grid)), dim3(TPB), s->G, s->P, s->nlist, s->ctr, s->nflag,
+           s->free_stack, s->ctr, always(s));
+    HIPX(hipGetLastError());
+    return FELT_OK;
 }
 
 int felt_load_dense_impl(felt_surface* s, int m, const float* phi, size_t n) {

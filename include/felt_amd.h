@@ -96,9 +96,9 @@ int felt_surface_geometry(felt_surface* s, int32_t* children_size, uint64_t* num
 int felt_surface_seed(felt_surface* s, const int32_t* pos);
 /* Load a dense signed-distance field (row-major over the whole grid, world order).  Not a
  * reference member: it builds the state Surface::load (Surface.hpp:180-196) would hold for
- * that field.  Partitions holding a band voxel are active with every voxel set from phi;
- * band voxels are tracked in row-major order; other partitions stay inactive with background
- * -(L+1) if their first voxel is negative, else L+1. */
+ * that field.  Partitions holding a band voxel are active with every voxel set from phi and
+ * keep the grid background L+1; band voxels are tracked in row-major order; other partitions
+ * stay inactive with background -(L+1) if their first voxel is negative, else L+1. */
 int felt_surface_load_dense(felt_surface* s, const float* phi, size_t n);
 
 /* Load the field min_k(|x - c_k| - r_k) over n spheres (world centres, D per sphere),
@@ -120,6 +120,17 @@ int felt_surface_update_deltas(felt_surface* s, size_t n, const int32_t* pos, co
 int felt_surface_update_start(felt_surface* s);
 int felt_surface_delta(felt_surface* s, size_t n, const int32_t* pos, const float* delta);
 int felt_surface_update_end(felt_surface* s);
+
+/* ---- Checkpoint / resume ------------------------------------------------------------ */
+/* Surface::save (Surface.hpp:180-183): the isogrid as the reference's cereal binary archive
+ * (IsoGrid::write, PartitionedMixin.hpp:692-699), which Surface::load reads back.  With dst NULL
+ * only *n is set to the archive's size; otherwise cap must be at least that.  Layer lists are
+ * written in ascending position order. */
+int felt_surface_save(felt_surface* s, uint8_t* dst, size_t cap, size_t* n);
+/* Surface::load (Surface.hpp:192-196): a new surface from such an archive.  D, L, size and
+ * partition size come from the archive; its offset must be -size/2 (as every Surface's is), its
+ * background L+1, and its parent lists must agree with its child lists, else FELT_E_ARG. */
+int felt_surface_load(const uint8_t* src, size_t n, felt_surface** out);
 
 /* ---- Read-back ---------------------------------------------------------------------- */
 /* isogrid().snapshot() into a caller buffer of prod(size) floats (PartitionedMixin.hpp:642-660). */
@@ -257,6 +268,12 @@ int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos
                                size_t* n);
 /* Stats of member i; last_voxel_updates / last_rounds are those of the last batch step. */
 int felt_batch_stats(felt_batch* b, int32_t i, felt_stats* out);
+/* felt_surface_save / load for member i of a batch.  A load replaces the member's state (its
+ * size, partition size and L must match the archive's); felt_batch_clear returns the member's
+ * bricks to the pool and leaves it the empty grid of a newly created surface (for eviction). */
+int felt_batch_save(felt_batch* b, int32_t i, uint8_t* dst, size_t cap, size_t* n);
+int felt_batch_load(felt_batch* b, int32_t i, const uint8_t* src, size_t n);
+int felt_batch_clear(felt_batch* b, int32_t i);
 /* felt_surface_ray over the members of a batch: ray i is cast against member members[i]. */
 int felt_batch_ray(felt_batch* b, size_t n, const int32_t* members, const float* origins,
                    const float* dirs, float* out);

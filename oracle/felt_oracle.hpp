@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <limits>
 #include <mutex>
 #include <utility>
@@ -358,6 +359,109 @@ struct PartGrid {
             for (Idx l : kids[p].lists[k]) fn(g->pos_of(p, l));
     }
 };
+
+/* ---- cereal binary archive of an IsoGrid (Surface.hpp:180-196, PartitionedMixin.hpp:692-761) ----
+ * cereal's BinaryOutputArchive writes arithmetic values as their raw bytes and container sizes as
+ * uint64 (cereal/macros.hpp:61-70, types/vector.hpp:40-85); Felt writes an Eigen vector as int32
+ * rows, int32 cols and a std::vector of its elements (PartitionedMixin.hpp:719-735). */
+struct ArcOut {
+    std::vector<uint8_t> b;
+    template <class T>
+    void put(T v) {
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    template <class T>
+    void put_n(const T* v, size_t n) {
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(v);
+        b.insert(b.end(), p, p + n * sizeof(T));
+    }
+    template <int D>
+    void vecdi(const Vi<D>& v) {                          // Eigen VecDi
+        put<int32_t>(D); put<int32_t>(1); put<uint64_t>(D);
+        for (int a = 0; a < D; a++) put<int32_t>(v[a]);
+    }
+    void lists(const std::vector<RawList>& ls, int nl) {  // Tuple<PosIdxList, N>
+        put<int32_t>(nl); put<int32_t>(1); put<uint64_t>(uint64_t(nl));
+        for (int k = 0; k < nl; k++) {
+            const Idx n = k < int(ls.size()) ? ls[k].size() : 0;
+            put<uint64_t>(n);
+            for (Idx i = 0; i < n; i++) put<uint64_t>(ls[k][i]);
+        }
+    }
+};
+
+struct ArcIn {
+    const uint8_t* p;
+    size_t n, at = 0;
+    bool ok = true;
+    template <class T>
+    T get() {
+        T v{};
+        if (at + sizeof(T) > n) { ok = false; return v; }
+        std::memcpy(&v, p + at, sizeof(T));
+        at += sizeof(T);
+        return v;
+    }
+    template <class T>
+    void get_n(std::vector<T>& out, uint64_t cnt) {
+        if (!ok || cnt > (n - at) / sizeof(T)) { ok = false; return; }
+        out.resize(size_t(cnt));
+        std::memcpy(out.data(), p + at, size_t(cnt) * sizeof(T));
+        at += size_t(cnt) * sizeof(T);
+    }
+    template <int D>
+    Vi<D> vecdi() {
+        Vi<D> v{};
+        if (get<int32_t>() != D || get<int32_t>() != 1 || get<uint64_t>() != uint64_t(D)) ok = false;
+        for (int a = 0; a < D && ok; a++) v[a] = get<int32_t>();
+        return v;
+    }
+    std::vector<std::vector<uint64_t>> lists(int nl) {
+        std::vector<std::vector<uint64_t>> ls;
+        if (get<int32_t>() != nl || get<int32_t>() != 1 || get<uint64_t>() != uint64_t(nl)) {
+            ok = false;
+            return ls;
+        }
+        ls.resize(size_t(nl));
+        for (int k = 0; k < nl && ok; k++) get_n(ls[size_t(k)], get<uint64_t>());
+        return ls;
+    }
+};
+
+/* Dimension and list count of an archive: the child size is the first Eigen vector, and the
+ * background child's list tuple gives N = 2L+1. */
+inline bool archive_shape(const uint8_t* src, size_t n, int* D, int* NL) {
+    ArcIn in{src, n};
+    in.get<float>();
+    const int32_t rows = in.get<int32_t>();
+    if (!in.ok || (rows != 2 && rows != 3)) return false;
+    in.at += 4 + 8 + 4 * size_t(rows);                  // cols, count, elements
+    in.get<float>();                                    // background child: bg
+    std::vector<float> d;
+    in.get_n(d, in.get<uint64_t>());                    // its data
+    in.get<uint64_t>();                                 // its lookup's background
+    std::vector<uint64_t> l;
+    in.get_n(l, in.get<uint64_t>());                    // its lookup data
+    const int32_t nl = in.get<int32_t>();
+    if (!in.ok || nl < 3 || nl % 2 == 0) return false;
+    *D = rows;
+    *NL = nl;
+    return true;
+}
+
+/* Child size (the archive's first Eigen vector) and grid size (the isogrid's Size, written just
+ * before Spatial's dx and two vectors at the end). */
+template <int D>
+bool archive_geometry(const uint8_t* src, size_t n, Vi<D>* size, Vi<D>* part) {
+    const size_t V = 16 + 4 * size_t(D);
+    if (n < 4 + V + 4 + 2 * V + 3 * V) return false;
+    ArcIn a{src, n, 4};
+    *part = a.vecdi<D>();
+    ArcIn b{src, n, n - (4 + 2 * V) - 3 * V};
+    *size = b.vecdi<D>();
+    return a.ok && b.ok;
+}
 
 /* SplitMix64 (Steele, Lea & Flood 2014). */
 inline uint64_t splitmix64(uint64_t x) {
@@ -713,6 +817,136 @@ public:
             if (hit != ray_miss()) return hit;
         }
         return ray_miss();
+    }
+
+    /* ---- save / load (Surface.hpp:180-196) ----
+     * The isogrid is written as the reference's IsoGrid::serialize does: Access background,
+     * Children (child size, then the children grid: its background child, every child, its lookup
+     * and its size), Size, Spatial (Partitioned.hpp:151-158).  A child is Activate, Data,
+     * LookupInterface, Size (Tracked.hpp:53-60); a lookup is Activate, Data, lists, Size
+     * (Lookup.hpp:170-177, 229-236).  The background child's sizes are never set by the reference
+     * (uninitialised Eigen vectors); they are written as zeros. */
+    void save(std::vector<uint8_t>& out) const {
+        ArcOut o;
+        const Vi<D> zero{};
+        const std::vector<RawList> none;
+        auto sizes = [&](const Vi<D>& sz, const Vi<D>& off) {
+            Vi<D> end{};
+            for (int a = 0; a < D; a++) end[a] = off[a] + sz[a];
+            o.vecdi<D>(sz); o.vecdi<D>(off); o.vecdi<D>(end);
+        };
+        o.put<float>(float(iso.grid_bg));
+        o.vecdi<D>(g.part);
+        // children grid: background child
+        o.put<float>(float(L + 1));
+        o.put<uint64_t>(0);
+        o.put<uint64_t>(NUL); o.put<uint64_t>(0); o.lists(none, NL);
+        o.vecdi<D>(zero); o.vecdi<D>(zero); o.vecdi<D>(zero);
+        o.vecdi<D>(zero); o.vecdi<D>(zero); o.vecdi<D>(zero);
+        o.put<uint64_t>(g.np);
+        for (Idx p = 0; p < g.np; p++) {
+            const Kid<R>& k = iso.kids[p];
+            const Vi<D> off = g.child_offset(p);
+            o.put<float>(float(k.bg));
+            o.put<uint64_t>(k.data.size());
+            for (R v : k.data) o.put<float>(float(v));
+            o.put<uint64_t>(NUL);
+            o.put<uint64_t>(k.look.size());
+            o.put_n(k.look.data(), k.look.size());
+            o.lists(k.lists, NL);
+            sizes(g.part, off);
+            sizes(g.part, off);
+        }
+        // children lookup: background tuple, per-partition tuples, parent lists, size
+        o.put<int32_t>(NL); o.put<int32_t>(1); o.put<uint64_t>(uint64_t(NL));
+        for (int k = 0; k < NL; k++) o.put<uint64_t>(NUL);
+        o.put<uint64_t>(g.np);
+        for (Idx p = 0; p < g.np; p++) {
+            o.put<int32_t>(NL); o.put<int32_t>(1); o.put<uint64_t>(uint64_t(NL));
+            for (int k = 0; k < NL; k++) o.put<uint64_t>(iso.pidx[p * NL + k]);
+        }
+        o.lists(iso.plists, NL);
+        sizes(g.csize, g.coff);
+        sizes(g.csize, g.coff);
+        sizes(g.size, g.offset);
+        o.put<float>(1.0f);                                // Spatial: dx, pos_min, pos_max
+        Vi<D> hi{};
+        for (int a = 0; a < D; a++) hi[a] = g.offset[a] + g.size[a];
+        o.vecdi<D>(g.offset); o.vecdi<D>(hi);
+        out.swap(o.b);
+    }
+    /* Fills a fresh surface of the archive's geometry; false if the archive does not match it. */
+    bool load(const uint8_t* src, size_t n) {
+        ArcIn in{src, n};
+        auto sizes = [&](const Vi<D>& sz, const Vi<D>& off, bool check) {
+            const Vi<D> a = in.vecdi<D>(), b = in.vecdi<D>(), c = in.vecdi<D>();
+            if (!check) return;
+            for (int i = 0; i < D; i++)
+                if (a[i] != sz[i] || b[i] != off[i] || c[i] != off[i] + sz[i]) in.ok = false;
+        };
+        iso.grid_bg = R(in.get<float>());
+        const Vi<D> part = in.vecdi<D>();
+        if (part != g.part) return false;
+        std::vector<float> fv;
+        std::vector<uint64_t> uv;
+        in.get<float>();                                   // background child
+        in.get_n(fv, in.get<uint64_t>());
+        in.get<uint64_t>();
+        in.get_n(uv, in.get<uint64_t>());
+        in.lists(NL);
+        for (int i = 0; i < 6; i++) in.vecdi<D>();
+        if (!in.ok || in.get<uint64_t>() != g.np) return false;
+        for (Idx p = 0; p < g.np && in.ok; p++) {
+            Kid<R>& k = iso.kids[p];
+            k.bg = R(in.get<float>());
+            in.get_n(fv, in.get<uint64_t>());
+            in.get<uint64_t>();
+            in.get_n(uv, in.get<uint64_t>());
+            auto ls = in.lists(NL);
+            if (!in.ok) return false;
+            k.active = !fv.empty();
+            if ((k.active && (fv.size() != g.pv || uv.size() != g.pv)) ||
+                (!k.active && !uv.empty()))
+                return false;
+            k.data.assign(fv.begin(), fv.end());
+            k.look.assign(uv.begin(), uv.end());
+            k.lists.assign(NL, RawList{});
+            for (int j = 0; j < NL; j++) {
+                if (!k.active && !ls[size_t(j)].empty()) return false;
+                for (uint64_t l : ls[size_t(j)]) {
+                    if (l >= g.pv) return false;
+                    k.lists[j].push_back(Idx(l));
+                }
+            }
+            const Vi<D> off = g.child_offset(p);
+            sizes(g.part, off, true);
+            sizes(g.part, off, true);
+        }
+        if (!in.ok) return false;
+        if (in.get<int32_t>() != NL || in.get<int32_t>() != 1 || in.get<uint64_t>() != uint64_t(NL))
+            return false;
+        for (int k = 0; k < NL; k++) in.get<uint64_t>();
+        if (in.get<uint64_t>() != g.np) return false;
+        for (Idx p = 0; p < g.np && in.ok; p++) {
+            if (in.get<int32_t>() != NL || in.get<int32_t>() != 1 ||
+                in.get<uint64_t>() != uint64_t(NL))
+                return false;
+            for (int k = 0; k < NL; k++) iso.pidx[p * NL + k] = Idx(in.get<uint64_t>());
+        }
+        auto pl = in.lists(NL);
+        if (!in.ok) return false;
+        iso.plists.assign(NL, RawList{});
+        for (int k = 0; k < NL; k++)
+            for (uint64_t p : pl[size_t(k)]) {
+                if (p >= g.np) return false;
+                iso.plists[k].push_back(Idx(p));
+            }
+        sizes(g.csize, g.coff, true);
+        sizes(g.csize, g.coff, true);
+        sizes(g.size, g.offset, true);
+        in.get<float>();
+        in.vecdi<D>(); in.vecdi<D>();
+        return in.ok && in.at == n;
     }
 
     /* ---- update API ---- */

@@ -102,6 +102,8 @@ struct Handle {
     virtual int geometry(int32_t* cs, uint64_t* np) = 0;
     virtual void set_omp(bool on) = 0;
     virtual int ray(size_t n, const float* origin, const float* dir, float* out) = 0;
+    virtual int save(std::vector<uint8_t>& out) = 0;
+    virtual bool load(const uint8_t* src, size_t n) = 0;
     /* polys */
     virtual int polys_create() = 0;
     virtual int polys_notify() = 0;
@@ -205,6 +207,11 @@ struct HandleT final : Handle {
         }
         return FELT_OK;
     }
+    int save(std::vector<uint8_t>& out) override {
+        s.save(out);
+        return FELT_OK;
+    }
+    bool load(const uint8_t* src, size_t n) override { return s.load(src, n); }
     int update_start() override {
         s.update_start();
         return FELT_OK;
@@ -421,6 +428,48 @@ OAPI int felt_oracle_surface_create(int D, int L, const int32_t* size, const int
     return FELT_OK;
 }
 OAPI int felt_oracle_surface_destroy(felt_oracle_surface* s) { delete s; return FELT_OK; }
+template <int D>
+static bool sane_geometry(const Vi<D>& sz, const Vi<D>& pt) {
+    double np = 1, pv = 1;
+    for (int a = 0; a < D; a++) {
+        if (sz[a] < 1 || pt[a] < 1 || sz[a] > (1 << 20)) return false;
+        np *= double((sz[a] + pt[a] - 1) / pt[a] + 1);
+        pv *= double(pt[a]);
+    }
+    return np <= double(1 << 26) && pv <= double(1 << 24);
+}
+/* Surface::save: the isogrid's cereal archive; with dst NULL only *n is set. */
+OAPI int felt_oracle_surface_save(felt_oracle_surface* s, uint8_t* dst, size_t cap, size_t* n) {
+    std::vector<uint8_t> b;
+    int rc = H(s)->save(b);
+    if (rc) return rc;
+    *n = b.size();
+    if (!dst) return FELT_OK;
+    if (cap < b.size()) return fail(FELT_E_CAPACITY, "archive buffer too small");
+    std::memcpy(dst, b.data(), b.size());
+    return FELT_OK;
+}
+/* Surface::load: a new (fp32) surface from a cereal archive of an isogrid. */
+OAPI int felt_oracle_surface_load(const uint8_t* src, size_t n, felt_oracle_surface** out) {
+    int D = 0, NL = 0;
+    if (!src || !out || !archive_shape(src, n, &D, &NL)) return fail(FELT_E_ARG, "not an isogrid archive");
+    const int L = (NL - 1) / 2;
+    std::unique_ptr<felt_oracle_surface> s(new felt_oracle_surface);
+    if (D == 2) {
+        Vi<2> sz{}, pt{};
+        if (!archive_geometry<2>(src, n, &sz, &pt) || !sane_geometry<2>(sz, pt))
+            return fail(FELT_E_ARG, "bad archive geometry");
+        s->h.reset(new HandleT<2, float>(L, sz, pt));
+    } else {
+        Vi<3> sz{}, pt{};
+        if (!archive_geometry<3>(src, n, &sz, &pt) || !sane_geometry<3>(sz, pt))
+            return fail(FELT_E_ARG, "bad archive geometry");
+        s->h.reset(new HandleT<3, float>(L, sz, pt));
+    }
+    if (!s->h->load(src, n)) return fail(FELT_E_ARG, "archive does not describe a valid isogrid");
+    *out = s.release();
+    return FELT_OK;
+}
 OAPI int felt_oracle_surface_set_omp(felt_oracle_surface* s, int on) { H(s)->set_omp(on != 0); return FELT_OK; }
 OAPI int felt_oracle_surface_geometry(felt_oracle_surface* s, int32_t* cs, uint64_t* np) { return H(s)->geometry(cs, np); }
 OAPI int felt_oracle_surface_seed(felt_oracle_surface* s, const int32_t* pos) { return H(s)->seed(pos); }

@@ -14,6 +14,7 @@ Operation encoding (replayed by tests/kat_replay.py):
     ["update_points", [[pos, v],...]]  update(fn) with fn returning v at pos and 0 elsewhere
     ["local", [[pos, v], ...]]         update_start(); delta(pos, v)...; update_end()
     ["box", lo, hi, v]                 update(lo, hi, fn) with fn returning v
+    ["save_load"]                      Surface::save, then continue on Surface::load of it
 """
 import json
 import re
@@ -81,6 +82,9 @@ def main(root):
     add("expand_0.6_contract_0.6_twice", cite(427, 461),
         ops=seed9 + [["update", -0.6], ["update", 0.6], ["update", 0.6]], grid=g(437),
         sizes=[0, 0, 0, 0, 0], **S9)
+    add("save_load_update", cite(464, 509),
+        ops=seed9 + [["update", -0.6], ["save_load"], ["update", 0.6]], grid=g(489),
+        sizes=[0, 0, 1, 4, 8], **S9)
     add("deactivate_centre", cite(519, 580), D=2, L=2, size=[21, 21], part=[2, 2],
         ops=seed9 + [["update", -1.0]] * 5, grid=g(540),
         active=[[[0, 1], True], [[0, -1], True], [[1, 0], True], [[-1, 0], True],

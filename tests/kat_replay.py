@@ -16,8 +16,12 @@ def load_kats():
 
 
 def replay(surface: Surface, ops):
+    """Runs the operations; returns the surface they end on (save_load replaces it)."""
     for op in ops:
         kind = op[0]
+        if kind == "save_load":
+            surface = type(surface).load(surface.b, surface.save())
+            continue
         if kind == "seed":
             surface.seed(op[1])
         elif kind == "update":
@@ -33,6 +37,7 @@ def replay(surface: Surface, ops):
             surface.update_box(op[1], op[2], speed_constant(op[3]))
         else:
             raise ValueError(kind)
+    return surface
 
 
 def grid_matches(snap: np.ndarray, expected, tol=1e-5):
@@ -54,7 +59,7 @@ def check_kat(make_surface, kat):
     """Returns a list of failure strings (empty when the scenario matches)."""
     s = make_surface(kat["D"], kat["L"], kat["size"], kat["part"])
     fails = []
-    replay(s, kat["ops"])
+    s = replay(s, kat["ops"])
     L = kat["L"]
     if "grid" in kat:
         ok, how = grid_matches(s.snapshot(), kat["grid"])
