@@ -408,6 +408,18 @@ API int felt_surface_save(felt_surface* s, uint8_t* dst, size_t cap, size_t* n) 
     return felt_save_impl(s, 0, dst, cap, n);
 }
 
+API int felt_surface_shape(felt_surface* s, int* D, int* L, int32_t* size, int32_t* part) {
+    CHECK_S(s);
+    if (!D || !L || !size || !part) return felt_fail(FELT_E_ARG, "null output");
+    *D = s->G.D;
+    *L = s->G.L;
+    for (int a = 0; a < s->G.D; a++) {
+        size[a] = s->mems[0].size[a];
+        part[a] = s->G.part[a];
+    }
+    return FELT_OK;
+}
+
 API int felt_surface_load(const uint8_t* src, size_t n, felt_surface** out) {
     if (!src || !out) return felt_fail(FELT_E_ARG, "null argument");
     int D = 0, L = 0, size[3] = {1, 1, 1}, part[3] = {1, 1, 1};

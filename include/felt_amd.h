@@ -131,6 +131,8 @@ int felt_surface_save(felt_surface* s, uint8_t* dst, size_t cap, size_t* n);
  * partition size come from the archive; its offset must be -size/2 (as every Surface's is), its
  * background L+1, and its parent lists must agree with its child lists, else FELT_E_ARG. */
 int felt_surface_load(const uint8_t* src, size_t n, felt_surface** out);
+/* D, L, grid size and partition size of a surface (D entries each; e.g. after a load). */
+int felt_surface_shape(felt_surface* s, int* D, int* L, int32_t* size, int32_t* part);
 
 /* ---- Read-back ---------------------------------------------------------------------- */
 /* isogrid().snapshot() into a caller buffer of prod(size) floats (PartitionedMixin.hpp:642-660). */

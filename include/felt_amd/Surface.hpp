@@ -8,6 +8,7 @@
 //   isogrid().snapshot(), isogrid().get(pos)      PartitionedMixin.hpp:594-660
 //   delta(k), status_change(k), layer_idx(id), stats(), is_intersected(p)  Surface.hpp:623-696
 //   ray(origin, dir), ray_miss                 Surface.hpp:468-616, 1805-1807
+//   save(ostream), load(istream)               Surface.hpp:180-196 (the same cereal archive)
 //   Polys(surface), notify(), march(), invalidate(), changes(), children().get(p)
 //                                            Polys.hpp:56-201, Poly.hpp:176-189
 // Functors: a felt_speed descriptor runs on the device; a host callable with one of the
@@ -19,7 +20,10 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <istream>
+#include <iterator>
 #include <limits>
+#include <ostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -112,6 +116,34 @@ public:
         m_h.reset(h);
         m_iso.reset(new IsoGrid(h, size));
     }
+
+    // Write the isogrid to a stream in the reference's on-disk format (a cereal binary archive).
+    void save(std::ostream& out) const {
+        std::size_t n = 0;
+        detail::check(felt_surface_save(m_h.get(), nullptr, 0, &n));
+        std::vector<uint8_t> buf(n);
+        detail::check(felt_surface_save(m_h.get(), buf.data(), buf.size(), &n));
+        out.write(reinterpret_cast<const char*>(buf.data()), std::streamsize(n));
+        out.flush();
+    }
+    // A surface from such a stream (read to its end).
+    static Surface load(std::istream& in) {
+        std::vector<uint8_t> buf{std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>()};
+        felt_surface* h = nullptr;
+        detail::check(felt_surface_load(buf.data(), buf.size(), &h));
+        std::unique_ptr<felt_surface, Del> owned(h);
+        int d = 0, l = 0;
+        int32_t size[3] = {0, 0, 0}, part[3] = {0, 0, 0};
+        detail::check(felt_surface_shape(h, &d, &l, size, part));
+        if (d != D || l != L) throw std::domain_error("archive holds a surface of another D or L");
+        VecDi<D> sz{}, pt{};
+        for (int a = 0; a < D; a++) {
+            sz[a] = size[a];
+            pt[a] = part[a];
+        }
+        return Surface(std::move(owned), sz, pt);
+    }
+    Surface(Surface&&) = default;
 
     void seed(const VecDi<D>& pos) { detail::check(felt_surface_seed(m_h.get(), pos.data())); }
 
@@ -235,6 +267,11 @@ private:
     template <class Fn>
     auto call(Fn& fn, const VecDi<D>&) -> decltype(fn()) {
         return fn();
+    }
+
+    Surface(std::unique_ptr<felt_surface, Del> h, const VecDi<D>& size, const VecDi<D>& part)
+        : m_size(size), m_part(part), m_h(std::move(h)) {
+        m_iso.reset(new IsoGrid(m_h.get(), size));
     }
 
     VecDi<D> m_size, m_part;

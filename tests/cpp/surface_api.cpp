@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <set>
+#include <sstream>
 
 #define CHECK(c)                                                   \
     do {                                                           \
@@ -104,6 +105,29 @@ int main() {
         (void)hipFree(d_org);
         (void)hipFree(d_dir);
         (void)hipFree(d_out);
+    }
+    {
+        // save / load round trip (test_Surface.cpp:464-509): expand by 0.6, save, load, then the
+        // loaded surface contracts by 0.6 to layer sizes 0/0/1/4/8
+        Surface<2, 2> surface(VecDi<2>{9, 9}, VecDi<2>{3, 3});
+        surface.seed(VecDi<2>{0, 0});
+        surface.update([]() { return -0.6f; });
+        std::stringstream ss;
+        surface.save(ss);
+        Surface<2, 2> loaded{Surface<2, 2>::load(ss)};
+        CHECK(loaded.isogrid().snapshot() == surface.isogrid().snapshot());
+        loaded.update([](const VecDi<2>&, const Surface<2, 2>::IsoGrid&) { return 0.6f; });
+        CHECK(loaded.layer_size(-2) == 0 && loaded.layer_size(-1) == 0 && loaded.layer_size(0) == 1 &&
+              loaded.layer_size(1) == 4 && loaded.layer_size(2) == 8);
+        bool threw = false;
+        try {
+            std::stringstream s3;
+            surface.save(s3);
+            Surface<3, 2>::load(s3);  // a 2D archive is not a Surface<3, 2>
+        } catch (const std::domain_error&) {
+            threw = true;
+        }
+        CHECK(threw);
     }
     std::printf("surface_api: all checks passed\n");
     return 0;
