@@ -84,6 +84,32 @@ int make_pool(int D, int L, const int32_t* part, int n, const int32_t* sizes, ui
     G.lstr[2] = 1;
     G.lstr[1] = G.part[2];
     G.lstr[0] = G.part[1] * G.part[2];
+    // brick tiles: 4x4x2 (3D) or 4x8 (2D), each edge cut down to a power of two dividing part
+    {
+        const int target[2][3] = {{4, 8, 1}, {4, 4, 2}};
+        int tv = 1;
+        for (int a = 0; a < 3; a++) {
+            int t = a < D ? target[D - 2][a] : 1;
+            while (t > 1 && G.part[a] % t != 0) t >>= 1;
+            G.tl[a] = t;
+            G.tmask[a] = t - 1;
+            G.tsh[a] = 0;
+            while ((1 << G.tsh[a]) < t) G.tsh[a]++;
+            tv *= t;
+        }
+        G.tvsh = 0;
+        while ((1 << G.tvsh) < tv) G.tvsh++;
+        for (int a = 0; a < 3; a++) {
+            int w = 1, ts = tv;
+            for (int b = a + 1; b < 3; b++) {
+                w *= G.tl[b];
+                ts *= G.part[b] / G.tl[b];
+            }
+            G.wstr[a] = w;
+            G.tstr[a] = ts;
+            G.fnt[a] = make_fastdiv((uint32_t)(G.part[a] / G.tl[a]));
+        }
+    }
     for (int a = 0; a < 3; a++) G.fpart[a] = make_fastdiv((uint32_t)G.part[a]);
     G.fpv = make_fastdiv((uint32_t)G.pv);
     std::vector<Mem> mems((size_t)n);

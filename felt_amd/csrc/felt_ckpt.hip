@@ -171,6 +171,20 @@ struct In {
 size_t vec_bytes(int D) { return 16 + 4 * (size_t)D; }
 size_t lists_bytes(int N, uint64_t entries) { return 16 + 8 * (size_t)N + 8 * (size_t)entries; }
 
+// perm[r] = storage leaf of the reference's row-major leaf PosIdx r.
+std::vector<int> leaf_perm(const Geo& G) {
+    std::vector<int> perm(G.pv);
+    for (int r = 0; r < G.pv; r++) {
+        int lc[3] = {0, 0, 0}, t = r;
+        for (int a = G.D - 1; a >= 0; a--) {
+            lc[a] = t % G.part[a];
+            t /= G.part[a];
+        }
+        perm[r] = G.D == 2 ? leaf_of<2>(G, lc) : leaf_of<3>(G, lc);
+    }
+    return perm;
+}
+
 // World offset of member-local partition lp.
 void child_offset(const Geo& G, const Mem& M, int lp, int* off) {
     int r = lp;
@@ -294,7 +308,9 @@ int felt_save_impl(felt_surface* s, int m, uint8_t* dst, size_t cap, size_t* nou
     for (int i = 0; i < 6; i++) o.vec(D, zero);
     o.put<uint64_t>((uint64_t)np);
     std::vector<uint64_t> look(pv);
+    std::vector<float> row(pv);
     std::vector<std::vector<uint32_t>> lists(N);
+    const std::vector<int> perm = leaf_perm(G);
     int j = 0;
     for (int p = 0; p < np; p++) {
         int off[3] = {0, 0, 0};
@@ -305,19 +321,21 @@ int felt_save_impl(felt_surface* s, int m, uint8_t* dst, size_t cap, size_t* nou
             const float* ph = &phi[(size_t)j * pv];
             const int8_t* ly = &lay[(size_t)j * pv];
             for (auto& l : lists) l.clear();
-            for (int l = 0; l < pv; l++) {
-                look[l] = kNull;
+            for (int r = 0; r < pv; r++) {  // the reference's leaf order
+                const int l = perm[r];
+                row[r] = ph[l];
+                look[r] = kNull;
                 if (ly[l] >= -G.L && ly[l] <= G.L) {
                     auto& lst = lists[ly[l] + G.L];
-                    look[l] = lst.size();
-                    lst.push_back((uint32_t)l);
+                    look[r] = lst.size();
+                    lst.push_back((uint32_t)r);
                 }
             }
             for (int k = 0; k < N; k++)
                 if ((int)lists[k].size() != cnt[(size_t)p * N + k])
                     return felt_fail(FELT_E_STATE, "layer counts and layer lists disagree");
             o.put<uint64_t>((uint64_t)pv);
-            std::memcpy(o.b + o.at, ph, (size_t)pv * 4);
+            std::memcpy(o.b + o.at, row.data(), (size_t)pv * 4);
             o.at += (size_t)pv * 4;
             o.put<uint64_t>(kNull);
             o.put<uint64_t>((uint64_t)pv);
@@ -397,6 +415,7 @@ int felt_load_impl(felt_surface* s, int m, const uint8_t* src, size_t n) {
     std::vector<size_t> data_at;
     std::vector<int8_t> lay;
     std::vector<uint8_t> seen(pv);
+    const std::vector<int> perm = leaf_perm(G);
     for (int p = 0; p < np; p++) {
         int off[3] = {0, 0, 0};
         child_offset(G, M, p, off);
@@ -437,7 +456,7 @@ int felt_load_impl(felt_surface* s, int m, const uint8_t* src, size_t n) {
                 }
                 for (int a = 0; a < D; a++)
                     if (q[a] < 0 || q[a] >= M.size[a]) return bad("tracked voxel outside the grid");
-                lay[base + l] = (int8_t)(k - G.L);
+                lay[base + perm[l]] = (int8_t)(k - G.L);
             }
             cnt[(size_t)p * N + k] = (int)len;
         }
@@ -492,11 +511,14 @@ int felt_load_impl(felt_surface* s, int m, const uint8_t* src, size_t n) {
         int8_t* d_lay = (int8_t*)(d_phi + (size_t)na * pv);
         int* d_parts = (int*)(((uintptr_t)(d_lay + (size_t)na * pv) + 15) & ~(uintptr_t)15);
         std::vector<int> parts(na);
+        std::vector<float> vals((size_t)na * pv);
         for (int j = 0; j < na; j++) {
             parts[j] = M.pbase + act[j];
-            HIPX(hipMemcpyAsync(d_phi + (size_t)j * pv, src + data_at[j], (size_t)pv * 4,
-                                hipMemcpyHostToDevice, s->st));
+            const uint8_t* d = src + data_at[j];
+            float* out = &vals[(size_t)j * pv];
+            for (int r = 0; r < pv; r++) std::memcpy(&out[perm[r]], d + 4 * (size_t)r, 4);
         }
+        HIPX(hipMemcpyAsync(d_phi, vals.data(), vals.size() * 4, hipMemcpyHostToDevice, s->st));
         HIPX(hipMemcpyAsync(d_lay, lay.data(), lay.size(), hipMemcpyHostToDevice, s->st));
         HIPX(hipMemcpyAsync(d_parts, parts.data(), na * sizeof(int), hipMemcpyHostToDevice, s->st));
         HIPX(hipMemcpyAsync(s->nlist, parts.data(), na * sizeof(int), hipMemcpyHostToDevice, s->st));

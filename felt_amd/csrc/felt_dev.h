@@ -14,13 +14,22 @@
 //   slot_part[cap]  brick slot -> partition, -1 when free
 //   slot_org[cap]   brick slot -> grid coordinates of its partition's first voxel, and member
 //                   (bits 0-29) with bit 30 set when the brick touches a face of its grid
-//   phi[cap*pv]     signed distances, one brick of pv = prod(part) voxels per slot, row-major
-//                   inside the brick with the last axis fastest (Util.hpp:23-40)
+//   phi[cap*pv]     signed distances, one brick of pv = prod(part) voxels per slot, stored in
+//                   tiles (below)
 //   lay[cap*pv]     layer list of each voxel (-L..L) or NONE: replaces the reference's per-child
 //                   lookup grid and per-layer leaf lists (LookupMixin.hpp:225-387)
 //   mem[nmem]       member geometry
 // A voxel is named by v = slot*pv + leaf.  Grid coordinates g are world coordinates minus the
 // member's grid offset, so 0 <= g < size; partition coordinates are g / part.
+//
+// Brick storage order.  A brick is cut into tiles of tl[0] x tl[1] x tl[2] voxels (4x4x2 in 3D,
+// 4x8 in 2D, each edge the largest power of two up to that which divides the partition edge);
+// tiles are row-major in the brick and voxels row-major in a tile, so one 128-byte line of phi
+// holds a 4x4x2 block.  The band is a thin sheet: a line then carries several band voxels of a
+// layer and most of their face neighbours, where a row-major line (32 voxels along one axis) is
+// crossed by the sheet about once.  Partition edges that are not multiples of 2 give 1-voxel tile
+// edges, i.e. the reference's row-major order (Util.hpp:23-40).  The reference's leaf PosIdx (row-
+// major in the partition) is kept wherever it is observable: mesh ordering and the on-disk format.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -66,7 +75,12 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
 struct Geo {
     int D, L, NL;
     int part[3];
-    int lstr[3];  // leaf strides inside a brick
+    int lstr[3];  // row-major leaf strides inside a partition (the reference's leaf PosIdx)
+    int tl[3], tsh[3], tmask[3];  // tile edge, its log2, edge - 1
+    int wstr[3];  // stride of a step inside a tile
+    int tstr[3];  // stride of a step to the next tile
+    int tvsh;     // log2 of the tile volume
+    FastDiv fnt[3];  // tiles per axis
     int pv;
     int np;       // partitions of all members
     int nmem;     // member surfaces
@@ -134,14 +148,61 @@ __device__ __forceinline__ int layer_of(float v, int L, int* nan_ctr) {
 
 __device__ __forceinline__ bool in_band(int k, int L) { return k >= -L && k <= L; }
 
+// Storage leaf of brick-local coordinates lc.
+template <int D>
+__host__ __device__ __forceinline__ int leaf_of(const Geo& G, const int lc[3]) {
+    int l = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) l += (lc[a] >> G.tsh[a]) * G.tstr[a] + (lc[a] & G.tmask[a]) * G.wstr[a];
+    return l;
+}
+
+// Brick-local coordinates of storage leaf l.
 template <int D>
 __device__ __forceinline__ void leaf_coords(const Geo& G, uint32_t l, int lc[3]) {
+    uint32_t w = l & ((1u << G.tvsh) - 1u), t = l >> G.tvsh;
+#pragma unroll
+    for (int a = D - 1; a >= 0; a--) {
+        const int in = (int)(w & (uint32_t)G.tmask[a]);
+        w >>= G.tsh[a];
+        const uint32_t q = fdiv(t, G.fnt[a]);
+        lc[a] = ((int)(t - q * (uint32_t)(G.part[a] >> G.tsh[a])) << G.tsh[a]) | in;
+        t = q;
+    }
+}
+
+// Brick-local coordinates of the reference's row-major leaf PosIdx l.
+template <int D>
+__device__ __forceinline__ void rm_coords(const Geo& G, uint32_t l, int lc[3]) {
 #pragma unroll
     for (int a = D - 1; a >= 0; a--) {
         const uint32_t q = fdiv(l, G.fpart[a]);
         lc[a] = (int)(l - q * (uint32_t)G.part[a]);
         l = q;
     }
+}
+
+template <int D>
+__host__ __device__ __forceinline__ int rm_of(const Geo& G, const int lc[3]) {
+    int l = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) l += lc[a] * G.lstr[a];
+    return l;
+}
+
+// Storage offset of a step d (components in {-1,0,1}) from brick-local coordinates lc that
+// stays in the brick.
+template <int D>
+__device__ __forceinline__ int step_offset(const Geo& G, const int lc[3], const int d[3]) {
+    int off = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        if (d[a] == 0) continue;
+        const int in = (lc[a] & G.tmask[a]) + d[a];
+        off += (in >= 0 && in <= G.tmask[a]) ? d[a] * G.wstr[a]
+                                              : d[a] * (G.tstr[a] - G.tmask[a] * G.wstr[a]);
+    }
+    return off;
 }
 
 template <int D>
@@ -198,13 +259,14 @@ __device__ __forceinline__ void locate(const Geo& G, const Pool& P, int m, const
     const int4 r0 = mem_row0(P, m), r1 = mem_row1(P, m);
     const int cs[3] = {r1.x, r1.y, r1.z};
     p = r0.w;
-    l = 0;
+    int lc[3] = {0, 0, 0};
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const int qa = (int)fdiv((uint32_t)q[a], G.fpart[a]);
         p += qa * cs[a];
-        l += (q[a] - qa * G.part[a]) * G.lstr[a];
+        lc[a] = q[a] - qa * G.part[a];
     }
+    l = leaf_of<D>(G, lc);
 }
 
 template <int D>
@@ -234,16 +296,14 @@ __device__ __forceinline__ float nb(const Geo& G, const Pool& P, const Vx& x, in
                                     int d2) {
     const int d[3] = {d0, d1, d2};
     bool same = true;
-    int off = 0;
     int q[3];
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const int la = x.loc[a] + d[a];
         q[a] = x.g[a] + d[a];
         if (la < 0 || la >= G.part[a] || q[a] >= x.size[a]) same = false;
-        off += d[a] * G.lstr[a];
     }
-    if (same) return P.phi[(int64_t)x.v + off];
+    if (same) return P.phi[(int64_t)x.v + step_offset<D>(G, x.loc, d)];
     return get_g<D>(G, P, x.m, x.size, q);
 }
 
@@ -267,27 +327,225 @@ __device__ __forceinline__ float distance(const Geo& G, const Pool& P, const Vx&
     return dn + dir;
 }
 
-// Mean curvature by difference of normals (NumericMixin.hpp:72-156), same operation order as
-// the reference with no contraction: forward normals, backward normals, then
-// (dn0 + (dn1 + dn2)) / 2 in 3D or (dn0 + dn1) / 2 in 2D.
+// distance() of U band voxels at once, with the gathers split into levels (brick origins and
+// member rows, then neighbour pages, then neighbour values) and every load of a level issued for
+// all U voxels and all 2D face neighbours before any is used: the dependent gathers of U voxels
+// overlap instead of running one after another.  Same result as distance() voxel by voxel.
+template <int D, int U>
+__device__ __forceinline__ void distance_u(const Geo& G, const Pool& P, const uint32_t* v,
+                                           float dir, float* out) {
+    constexpr int NN = 2 * D;
+    uint32_t slot[U];
+    int4 org[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        slot[u] = fdiv(v[u], G.fpv);
+        org[u] = P.slot_org[slot[u]];
+    }
+    float self[U];
+    int4 r0[U], r1[U];
+    int loc[U][3];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        self[u] = P.phi[v[u]];
+        const int m = org[u].w & kMemberMask;
+        r0[u] = mem_row0(P, m);
+        r1[u] = mem_row1(P, m);
+        loc[u][0] = loc[u][1] = loc[u][2] = 0;
+        leaf_coords<D>(G, v[u] - slot[u] * (uint32_t)G.pv, loc[u]);
+    }
+    // classify the neighbours: outside the grid, in the brick (index), or in the next brick
+    // (its partition and leaf)
+    int nidx[U][NN], npart[U][NN];
+    bool nval[U][NN];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int o[3] = {org[u].x, org[u].y, org[u].z};
+        const int sz[3] = {r0[u].x, r0[u].y, r0[u].z};
+        const int cs[3] = {r1[u].x, r1[u].y, r1[u].z};
+        int own = r0[u].w;
+#pragma unroll
+        for (int a = 0; a < D; a++) own += (int)fdiv((uint32_t)o[a], G.fpart[a]) * cs[a];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int sd = h ? 1 : -1, j = 2 * a + h;
+                const int qa = o[a] + loc[u][a] + sd;
+                const int la = loc[u][a] + sd;
+                nval[u][j] = qa >= 0 && qa < sz[a];
+                if (la >= 0 && la < G.part[a]) {
+                    int d[3] = {0, 0, 0};
+                    d[a] = sd;
+                    nidx[u][j] = (int)v[u] + step_offset<D>(G, loc[u], d);
+                    npart[u][j] = -1;
+                } else {
+                    int lc[3] = {loc[u][0], loc[u][1], loc[u][2]};
+                    lc[a] = sd < 0 ? G.part[a] - 1 : 0;
+                    nidx[u][j] = leaf_of<D>(G, lc);
+                    npart[u][j] = own + sd * cs[a];
+                }
+            }
+        }
+    }
+    int pg[U][NN];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < NN; j++)
+            pg[u][j] = (nval[u][j] && npart[u][j] >= 0) ? P.page[npart[u][j]] : -1;
+    float nv[U][NN];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < NN; j++) {
+            float x = 0.0f;
+            if (nval[u][j]) {
+                if (npart[u][j] < 0) x = P.phi[(uint32_t)nidx[u][j]];
+                else if (pg[u][j] < 0) x = P.pbg[npart[u][j]];
+                else x = P.phi[(size_t)pg[u][j] * G.pv + nidx[u][j]];
+            }
+            nv[u][j] = x;
+        }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        float dist = self[u] * dir;
+#pragma unroll
+        for (int j = 0; j < NN; j++)
+            if (nval[u][j] && nv[u][j] * dir < dist) dist = nv[u][j] * dir;
+        const float dn = dist * dir;
+        out[u] = dn + dir;
+    }
+}
+
+// A band voxel with what neighbour gathers need: coordinates in its brick and grid, its grid's
+// size, its member's partition strides and its own pool partition.
+struct VoxCtx {
+    uint32_t v;
+    int m, own;
+    int loc[3], g[3], sz[3], cs[3];
+};
+
 template <int D>
-__device__ __forceinline__ float curvature(const Geo& G, const Pool& P, const Vx& x) {
-    const float c = P.phi[x.v];
+__device__ __forceinline__ void vox_ctx(const Geo& G, const Pool& P, uint32_t v, VoxCtx& c) {
+    const uint32_t slot = fdiv(v, G.fpv);
+    const int4 o = P.slot_org[slot];
+    c.v = v;
+    c.m = o.w & kMemberMask;
+    const int4 r0 = mem_row0(P, c.m), r1 = mem_row1(P, c.m);
+    c.loc[0] = c.loc[1] = c.loc[2] = 0;
+    leaf_coords<D>(G, v - slot * (uint32_t)G.pv, c.loc);
+    const int org[3] = {o.x, o.y, o.z};
+    const int sz[3] = {r0.x, r0.y, r0.z}, cs[3] = {r1.x, r1.y, r1.z};
+    c.own = r0.w;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        c.g[a] = org[a] + c.loc[a];
+        c.sz[a] = sz[a];
+        c.cs[a] = cs[a];
+        if (a < D) c.own += (int)fdiv((uint32_t)org[a], G.fpart[a]) * cs[a];
+    }
+}
+
+// get() (background L+1 outside the grid) at K offsets off[j] (components in {-1,0,1}) of a
+// voxel, gathered in levels: every neighbour's brick or partition first, then the pages of the
+// neighbours in other bricks, then all values, so the K loads of a level are in flight together.
+template <int D, int K>
+__device__ __forceinline__ void gather(const Geo& G, const Pool& P, const VoxCtx& c,
+                                       const int (*off)[3], float* val) {
+    int idx[K], part[K];
+    bool in[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        bool inside = true, cross = false;
+        int lc[3] = {c.loc[0], c.loc[1], c.loc[2]};
+        int np = c.own;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            const int q = c.g[a] + off[j][a];
+            if (q < 0 || q >= c.sz[a]) inside = false;
+            lc[a] += off[j][a];
+            if (lc[a] < 0) {
+                lc[a] += G.part[a];
+                np -= c.cs[a];
+                cross = true;
+            } else if (lc[a] >= G.part[a]) {
+                lc[a] -= G.part[a];
+                np += c.cs[a];
+                cross = true;
+            }
+        }
+        in[j] = inside;
+        if (cross) {
+            idx[j] = leaf_of<D>(G, lc);
+            part[j] = np;
+        } else {
+            idx[j] = (int)c.v + step_offset<D>(G, c.loc, off[j]);
+            part[j] = -1;
+        }
+    }
+    int pg[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) pg[j] = (in[j] && part[j] >= 0) ? P.page[part[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        float x = (float)(G.L + 1);
+        if (in[j]) {
+            if (part[j] < 0) x = P.phi[(uint32_t)idx[j]];
+            else if (pg[j] < 0) x = P.pbg[part[j]];
+            else x = P.phi[(size_t)pg[j] * G.pv + idx[j]];
+        }
+        val[j] = x;
+    }
+}
+
+// The curvature stencil: 2D face neighbours (face(a, s) at 2a + (s > 0)), then for each axis
+// pair a < b the four edge neighbours (a, b) signs (-,-), (-,+), (+,-), (+,+).
+template <int D>
+struct Stencil;
+template <>
+struct Stencil<2> {
+    static constexpr int K = 8;
+};
+template <>
+struct Stencil<3> {
+    static constexpr int K = 18;
+};
+__device__ constexpr int kStencil3[18][3] = {
+    {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1},
+    {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}, {1, 1, 0},      // x,y
+    {-1, 0, -1}, {-1, 0, 1}, {1, 0, -1}, {1, 0, 1},      // x,z
+    {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}};     // y,z
+__device__ constexpr int kStencil2[8][3] = {
+    {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0},
+    {-1, -1, 0}, {-1, 1, 0}, {1, -1, 0}, {1, 1, 0}};
+
+template <int D>
+__device__ __forceinline__ int st_face(int a, int s) { return 2 * a + (s > 0); }
+// Edge neighbour with sign sa along axis a and sb along axis b (a != b).
+template <int D>
+__device__ __forceinline__ int st_edge(int a, int sa, int b, int sb) {
+    const int p = a < b ? a : b, q = a < b ? b : a;
+    const int sp = a < b ? sa : sb, sq = a < b ? sb : sa;
+    const int pair = D == 2 ? 0 : (p == 0 ? (q == 1 ? 0 : 1) : 2);
+    return 2 * D + 4 * pair + 2 * (sp > 0) + (sq > 0);
+}
+
+// Mean curvature by difference of normals (NumericMixin.hpp:72-156) from the gathered stencil,
+// same operation order as the reference with no contraction: forward normals, backward normals,
+// then (dn0 + (dn1 + dn2)) / 2 in 3D or (dn0 + dn1) / 2 in 2D.
+template <int D>
+__device__ __forceinline__ float curvature_of(float c, const float* v) {
     float nf[3], nbk[3];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        int e[3] = {0, 0, 0};
-        e[a] = 1;
-        const float va = nb<D>(G, P, x, e[0], e[1], e[2]) - c;
+        const float va = v[st_face<D>(a, 1)] - c;
         float sq = 0.0f;
 #pragma unroll
         for (int b = 0; b < D; b++) {
             if (b == a) continue;
-            int lo[3] = {e[0], e[1], e[2]}, hi[3] = {e[0], e[1], e[2]};
-            lo[b] -= 1;
-            hi[b] += 1;
-            const float vl = nb<D>(G, P, x, lo[0], lo[1], lo[2]);
-            const float vh = nb<D>(G, P, x, hi[0], hi[1], hi[2]);
+            const float vl = v[st_edge<D>(a, 1, b, -1)];
+            const float vh = v[st_edge<D>(a, 1, b, 1)];
             const float t = (vh - vl) / 2.0f;
             sq += t * t;
         }
@@ -295,18 +553,13 @@ __device__ __forceinline__ float curvature(const Geo& G, const Pool& P, const Vx
     }
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        int e[3] = {0, 0, 0};
-        e[a] = -1;
-        const float va = c - nb<D>(G, P, x, e[0], e[1], e[2]);
+        const float va = c - v[st_face<D>(a, -1)];
         float sq = 0.0f;
 #pragma unroll
         for (int b = 0; b < D; b++) {
             if (b == a) continue;
-            int lo[3] = {e[0], e[1], e[2]}, hi[3] = {e[0], e[1], e[2]};
-            lo[b] -= 1;
-            hi[b] += 1;
-            const float vl = nb<D>(G, P, x, lo[0], lo[1], lo[2]);
-            const float vh = nb<D>(G, P, x, hi[0], hi[1], hi[2]);
+            const float vl = v[st_edge<D>(a, -1, b, -1)];
+            const float vh = v[st_edge<D>(a, -1, b, 1)];
             const float t = (vh - vl) / 2.0f;
             sq += t * t;
         }

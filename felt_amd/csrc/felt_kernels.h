@@ -93,24 +93,20 @@ __device__ __forceinline__ float value_noise(const int w[3], unsigned long long 
     return val[0];
 }
 
-// The speed functor of member x.m at voxel x; `offset` is the member's grid offset (noise is a
-// function of world position).
+// The speed functor sp at a voxel whose curvature is curv (when the functor uses it) and whose
+// world position is w (for the noise).
 template <int D>
-__device__ __forceinline__ float eval_speed(const Geo& G, const Pool& P, const Speed& sp,
-                                            const Vx& x, const int offset[3]) {
+__device__ __forceinline__ float speed_from(const Speed& sp, float curv, const int w[3]) {
     switch (sp.kind) {
     case FELT_SPEED_CONSTANT:
         return sp.a;
     case FELT_SPEED_CURVATURE:
-        return sp.b * curvature<D>(G, P, x);
+        return sp.b * curv;
     case FELT_SPEED_CONST_CURV:
-        return sp.a + sp.b * curvature<D>(G, P, x);
+        return sp.a + sp.b * curv;
     default: {
-        int w[3];
-#pragma unroll
-        for (int a = 0; a < D; a++) w[a] = x.g[a] + offset[a];
         const float n = value_noise<D>(w, sp.seed, sp.cell);
-        float r = sp.a * (1.0f + sp.c * n) + sp.b * curvature<D>(G, P, x);
+        float r = sp.a * (1.0f + sp.c * n) + sp.b * curv;
         if (r < -1.0f) r = -1.0f;
         else if (r > 1.0f) r = 1.0f;
         return r;

@@ -102,14 +102,13 @@ __device__ __forceinline__ MemV mem_view(const Pool& P, int slot) {
 // get(pos) at grid position q of the member.
 template <int D>
 __device__ __forceinline__ float get_v(const Geo& G, const Pool& P, const MemV& V, const int q[3]) {
-    int l = 0;
+    int lc[3] = {0, 0, 0};
     bool in = true;
     for (int a = 0; a < D; a++) {
-        const int la = q[a] - V.org[a];
-        if (la < 0 || la >= G.part[a] || q[a] >= V.size[a]) in = false;
-        l += la * G.lstr[a];
+        lc[a] = q[a] - V.org[a];
+        if (lc[a] < 0 || lc[a] >= G.part[a] || q[a] >= V.size[a]) in = false;
     }
-    if (in) return P.phi[(size_t)V.slot * G.pv + l];
+    if (in) return P.phi[(size_t)V.slot * G.pv + leaf_of<D>(G, lc)];
     return get_g<D>(G, P, V.m, V.size, q);
 }
 
@@ -125,18 +124,17 @@ __device__ unsigned cube_mask(const Geo& G, const Pool& P, const MemV& V, const 
     return m;
 }
 
-// Leaf index (within the brick in slot s, whose first voxel is at grid position org) of the
-// band cube at grid base position b, or -1.
+// Row-major leaf PosIdx (within the brick in slot s, whose first voxel is at grid position org)
+// of the band cube at grid base position b, or -1.
 template <int D>
 __device__ int band_cube(const Geo& G, const Pool& P, const MemV& V, int s, const int org[3],
                          const int b[3]) {
-    int l = 0;
+    int lc[3] = {0, 0, 0};
     for (int a = 0; a < D; a++) {
-        const int la = b[a] - org[a];
-        if (la < 0 || la >= G.part[a] || b[a] >= V.size[a]) return -1;
-        l += la * G.lstr[a];
+        lc[a] = b[a] - org[a];
+        if (lc[a] < 0 || lc[a] >= G.part[a] || b[a] >= V.size[a]) return -1;
     }
-    return P.lay[(size_t)s * G.pv + l] != NONE ? l : -1;
+    return P.lay[(size_t)s * G.pv + leaf_of<D>(G, lc)] != NONE ? rm_of<D>(G, lc) : -1;
 }
 
 // Owner of the vertex on edge e of the cube at base b: the lowest-leaf band cube of the
@@ -366,9 +364,11 @@ __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, 
     if (s < 0) return;
     for (int l = threadIdx.x; l < G.pv; l += blockDim.x)
         if (P.lay[(size_t)s * G.pv + l] != NONE) {
+            int lc[3] = {0, 0, 0};
+            leaf_coords<D>(G, (uint32_t)l, lc);
             const int at = atomicAdd(&c, 1);
             item_j[at] = blockIdx.x;
-            item_l[at] = l;
+            item_l[at] = rm_of<D>(G, lc);  // items and their scratch are keyed by leaf PosIdx
         }
 }
 
@@ -391,7 +391,7 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
         const MemV V = mem_view(P, s);
         const int* org = V.org;
         int g[3] = {0, 0, 0};
-        leaf_coords<D>(G, (uint32_t)l, g);
+        rm_coords<D>(G, (uint32_t)l, g);
         for (int a = 0; a < D; a++) g[a] += org[a];
         const unsigned m = cube_mask<D>(G, P, V, g);
         const unsigned em = McT<D>::edges_of(m);
@@ -521,7 +521,7 @@ __global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, con
     const MemV V = mem_view(P, s);
     const int* org = V.org;
     int g[3] = {0, 0, 0};
-    leaf_coords<D>(G, (uint32_t)l, g);
+    rm_coords<D>(G, (uint32_t)l, g);
     for (int a = 0; a < D; a++) g[a] += org[a];
     const int vw = D == 3 ? 6 : 2;
     const unsigned om = own[jb + l];

@@ -324,12 +324,20 @@ template <int D>
 __global__ void k_speed(Geo G, Pool P, const uint32_t* list, const int* rng, const Speed* sp,
                         float* out) {
     FELT_SPAN_LOOP(i, rng) {
-        Vx x;
-        decode<D>(G, P, list[i], x);
-        const Speed s = sp[x.m];
-        const Mem& M = P.mem[x.m];
-        const int off[3] = {M.offset[0], M.offset[1], M.offset[2]};
-        out[i] = eval_speed<D>(G, P, s, x, off);
+        VoxCtx c;
+        vox_ctx<D>(G, P, list[i], c);
+        const Speed s = sp[c.m];
+        float curv = 0.0f;
+        if (s.kind != FELT_SPEED_CONSTANT) {
+            float v[Stencil<D>::K];
+            if (D == 3) gather<D, Stencil<D>::K>(G, P, c, kStencil3, v);
+            else gather<D, Stencil<D>::K>(G, P, c, kStencil2, v);
+            curv = curvature_of<D>(P.phi[c.v], v);
+        }
+        const Mem& M = P.mem[c.m];
+        int w[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) w[a] = c.g[a] + M.offset[a];
+        out[i] = speed_from<D>(s, curv, w);
     }
 }
 
@@ -347,16 +355,44 @@ __global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta,
 
 // update_distance pass 1 (Surface.hpp:1029-1073) over the entries of one layer: d = the
 // distance from the layer's neighbours.  Tagged lists hold every layer and are filtered by tag.
-template <int D, bool Tagged>
+// Each thread takes U entries per trip (a grid-stride apart) and computes all of them before
+// storing, so the loads of U independent voxels are in flight together: the pass is bound by
+// the latency of its dependent gathers (entry -> brick origin -> face neighbours), not by
+// bandwidth.  Entries past the span, or of another layer in a tagged list, are computed on a
+// valid voxel and not stored.
+#ifndef FELT_DIST_U
+#define FELT_DIST_U 1
+#endif
+template <int D, bool Tagged, int U = FELT_DIST_U>
 __device__ __forceinline__ void dist1_body(const Geo& G, const Pool& P, const uint32_t* list,
                                            const int8_t* tag, const int* rng, int layer,
                                            float* out) {
     const float side = layer < 0 ? -1.0f : 1.0f;
-    FELT_SPAN_LOOP(i, rng) {
-        if (Tagged && tag[i] != layer) continue;
-        Vx x;
-        decode<D>(G, P, list[i], x);
-        out[i] = distance<D>(G, P, x, P.phi[x.v], side);
+    const int lo = rng[0], hi = rng[1];
+    // slots of a trip are `step` apart: the grid stride for long spans, else the span split in U
+    // so that no thread computes a slot past the end while others sit idle
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int step = min(gridDim.x * blockDim.x, (hi - lo + U - 1) / U);
+    if (tid >= step) return;
+    for (int i0 = lo + tid; i0 < hi; i0 += U * step) {
+        uint32_t v[U];
+        bool st[U];
+        v[0] = list[i0];
+#pragma unroll
+        for (int u = 1; u < U; u++) {
+            const int i = i0 + u * step;
+            v[u] = i < hi ? list[i] : v[0];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * step;
+            st[u] = i < hi && (!Tagged || tag[i] == layer);
+        }
+        float d[U];
+        distance_u<D, U>(G, P, v, side, d);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (st[u]) out[i0 + u * step] = d[u];
     }
 }
 

@@ -161,7 +161,9 @@ __device__ __forceinline__ float aff_neighbour(const Geo& G, const Pool& P, cons
                                                bool& active) {
     const int la = x.loc[a] + sd;
     if (la >= 0 && la < G.part[a]) {  // same brick
-        vq = (uint32_t)((int64_t)x.v + sd * G.lstr[a]);
+        int d[3] = {0, 0, 0};
+        d[a] = sd;
+        vq = (uint32_t)((int64_t)x.v + step_offset<D>(G, x.loc, d));
         active = true;
         return P.phi[vq];
     }
