@@ -193,118 +193,139 @@ __global__ void k_bscan_apply(const int* pcnt, int np, int NL, const int* bs, in
     }
 }
 
-// Pass 4: blocks stride over the partitions; each active one writes its band voxels into their
-// segments (one shared counter per layer), reading 16 layer bytes per load when bricks are a
-// multiple of 16 voxels.  A voxel beyond its partition's count (layers and counts disagreeing)
-// is not written and is reported through err.
-__device__ __forceinline__ void band_put(const Pool& P, int* c, const int* e, int L, int k,
-                                         uint32_t v, uint32_t* band, int8_t* tag, Ctr* ctr) {
-    const int at = atomicAdd(&c[k + L], 1);
-    if (at < e[k + L]) {
-        band[at] = v;
-        tag[at] = (int8_t)k;
-    } else {
-        ctr->err = 2;
-    }
-}
+// Pass 4: one wavefront per partition writes the partition's band voxels into their layer
+// segments.  The scan gave the partition's first entry in every segment, so a wavefront needs no
+// shared counters and no barriers: it reads its brick's layer bytes 16 per lane (up to four
+// 1024-voxel chunks in flight), counts each lane's voxels per layer in 16-bit fields (layers 0-3
+// and 4-7 in two 64-bit words), scans the counts over the wave and writes each voxel at segment
+// start + running count + wave prefix, so entries follow the brick's storage order.  More than
+// 8 layers take a per-layer ballot path.  A voxel beyond its partition's count (layers and counts
+// disagreeing) is not written and is reported through err.
+constexpr int kScatterWaves = 4;  // wavefronts (partitions) per block
 
-// Each wavefront takes 16 consecutive voxels per lane (1024 per wave), counts the lane's voxels
-// per layer in 16-bit fields (layers 0-3 and 4-7 in two 64-bit words), scans the counts over
-// the wave and reserves each layer's entries with one shared atomic, so a wave's entries are in
-// leaf order.  Layer counts above 8 (L > 3) take the per-voxel path.
 template <int D>
-__global__ void k_band_scatter(Geo G, Pool P, const int* offsets, uint32_t* band, int8_t* tag,
-                               Ctr* ctr) {
-    __shared__ int c[32], e[32];
-    __shared__ int wb[16][8];
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    for (int p = blockIdx.x; p < G.np; p += gridDim.x) {
-        const int s = P.page[p];
-        if (s < 0) continue;  // uniform over the block
-        __syncthreads();      // the previous partition's counters are no longer read
-        if (threadIdx.x < G.NL) {
-            c[threadIdx.x] = offsets[p * G.NL + threadIdx.x];
-            e[threadIdx.x] = c[threadIdx.x] + P.pcnt[p * G.NL + threadIdx.x];
-        }
-        __syncthreads();
-        const uint32_t v0 = (uint32_t)s * (uint32_t)G.pv;
-        const int8_t* lay = P.lay + (size_t)v0;
-        if (G.NL > 8) {
-            for (int i = threadIdx.x; i < G.pv; i += blockDim.x) {
-                const int k = lay[i];
-                if (k != NONE) band_put(P, c, e, G.L, k, v0 + (uint32_t)i, band, tag, ctr);
-            }
-            continue;
-        }
-        for (int base = 0; base < G.pv; base += 16 * (int)blockDim.x) {
-            const int i0 = base + 16 * (int)threadIdx.x;
-            uint32_t q[4] = {0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u};
-            if ((G.pv & 15) == 0) {
-                if (i0 < G.pv) {
-                    const uint4 w = *reinterpret_cast<const uint4*>(lay + i0);
-                    q[0] = w.x;
-                    q[1] = w.y;
-                    q[2] = w.z;
-                    q[3] = w.w;
+__global__ void __launch_bounds__(64 * kScatterWaves)
+k_band_scatter(Geo G, Pool P, const int* offsets, uint32_t* band, int8_t* tag, Ctr* ctr) {
+    const int lane = __lane_id();
+    const int p = blockIdx.x * kScatterWaves + (threadIdx.x >> 6);
+    if (p >= G.np) return;
+    const int s = P.page[p];
+    if (s < 0) return;
+    const int NL = G.NL;
+    // lane k < NL: next free entry and end of layer k's entries of this partition
+    int nxt = 0, end = 0;
+    if (lane < NL) {
+        nxt = offsets[p * NL + lane];
+        end = nxt + P.pcnt[p * NL + lane];
+    }
+    const uint32_t v0 = (uint32_t)s * (uint32_t)G.pv;
+    const int8_t* lay = P.lay + (size_t)v0;
+    bool bad = false;
+    if (NL > 8) {
+        for (int i0 = 0; i0 < G.pv; i0 += 64) {
+            const int i = i0 + lane;
+            const int f = i < G.pv && lay[i] != NONE ? lay[i] + G.L : -1;
+            for (int k = 0; k < NL; k++) {
+                const unsigned long long m = __ballot(f == k);
+                if (!m) continue;
+                const int b = __shfl(nxt, k), e = __shfl(end, k);
+                if (f == k) {
+                    const int at = b + __popcll(m & ((1ull << lane) - 1ull));
+                    if (at < e) {
+                        band[at] = v0 + (uint32_t)i;
+                        tag[at] = (int8_t)(k - G.L);
+                    } else {
+                        bad = true;
+                    }
                 }
-            } else {
+                if (lane == k) nxt += __popcll(m);
+            }
+        }
+    } else {
+        constexpr int CH = 4;
+        for (int c0 = 0; c0 < G.pv; c0 += 1024 * CH) {
+            uint32_t q[CH][4];
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const int i0 = c0 + c * 1024 + 16 * lane;
+                q[c][0] = q[c][1] = q[c][2] = q[c][3] = 0x80808080u;
+                if ((G.pv & 15) == 0) {
+                    if (i0 < G.pv) {
+                        const uint4 w = *reinterpret_cast<const uint4*>(lay + i0);
+                        q[c][0] = w.x;
+                        q[c][1] = w.y;
+                        q[c][2] = w.z;
+                        q[c][3] = w.w;
+                    }
+                } else {
+                    for (int j = 0; j < 16; j++) {
+                        const uint32_t byte =
+                            i0 + j < G.pv ? (uint32_t)(uint8_t)lay[i0 + j] : 0x80u;
+                        q[c][j >> 2] = (q[c][j >> 2] & ~(0xffu << (8 * (j & 3)))) |
+                                       (byte << (8 * (j & 3)));
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CH; c++) {
+                const int i0 = c0 + c * 1024 + 16 * lane;
+                unsigned long long n0 = 0, n1 = 0;
+#pragma unroll
                 for (int j = 0; j < 16; j++) {
-                    const uint32_t byte = i0 + j < G.pv ? (uint32_t)(uint8_t)lay[i0 + j] : 0x80u;
-                    q[j >> 2] = (q[j >> 2] & ~(0xffu << (8 * (j & 3)))) | (byte << (8 * (j & 3)));
+                    const int k = (int)(int8_t)((q[c][j >> 2] >> (8 * (j & 3))) & 0xffu);
+                    if (k == NONE) continue;
+                    const int f = k + G.L;
+                    const unsigned long long one = 1ull << (16 * (f & 3));
+                    if (f < 4) n0 += one;
+                    else n1 += one;
                 }
-            }
-            unsigned long long n0 = 0, n1 = 0;
+                unsigned long long x0 = n0, x1 = n1;
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
-                if (k == NONE) continue;
-                const int f = k + G.L;
-                const unsigned long long one = 1ull << (16 * (f & 3));
-                if (f < 4) n0 += one;
-                else n1 += one;
-            }
-            unsigned long long x0 = n0, x1 = n1;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const unsigned long long y0 = __shfl_up(x0, o), y1 = __shfl_up(x1, o);
+                    if (lane >= o) {
+                        x0 += y0;
+                        x1 += y1;
+                    }
+                }
+                const unsigned long long t0 = __shfl(x0, 63), t1 = __shfl(x1, 63);
+                int b[8], e[8];
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned long long y0 = __shfl_up(x0, o), y1 = __shfl_up(x1, o);
-                if (lane >= o) {
-                    x0 += y0;
-                    x1 += y1;
+                for (int f = 0; f < 8; f++) {
+                    b[f] = __shfl(nxt, f);
+                    e[f] = __shfl(end, f);
                 }
-            }
-            const unsigned long long t0 = __shfl(x0, 63), t1 = __shfl(x1, 63);
-            if (lane < G.NL) {
-                const unsigned long long t = lane < 4 ? t0 : t1;
-                const int tot = (int)((t >> (16 * (lane & 3))) & 0xffffull);
-                wb[wave][lane] = tot ? atomicAdd(&c[lane], tot) : 0;
-            }
-            __syncthreads();
-            unsigned long long r0 = x0 - n0, r1 = x1 - n1;  // entries before this lane's
+                unsigned long long r0 = x0 - n0, r1 = x1 - n1;  // entries before this lane's
 #pragma unroll
-            for (int j = 0; j < 16; j++) {
-                const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
-                if (k == NONE) continue;
-                const int f = k + G.L;
-                const int sh = 16 * (f & 3);
-                int within;
-                if (f < 4) {
-                    within = (int)((r0 >> sh) & 0xffffull);
-                    r0 += 1ull << sh;
-                } else {
-                    within = (int)((r1 >> sh) & 0xffffull);
-                    r1 += 1ull << sh;
+                for (int j = 0; j < 16; j++) {
+                    const int k = (int)(int8_t)((q[c][j >> 2] >> (8 * (j & 3))) & 0xffu);
+                    if (k == NONE) continue;
+                    const int f = k + G.L;
+                    const int sh = 16 * (f & 3);
+                    int within;
+                    if (f < 4) {
+                        within = (int)((r0 >> sh) & 0xffffull);
+                        r0 += 1ull << sh;
+                    } else {
+                        within = (int)((r1 >> sh) & 0xffffull);
+                        r1 += 1ull << sh;
+                    }
+                    const int at = b[f] + within;
+                    if (at < e[f]) {
+                        band[at] = v0 + (uint32_t)(i0 + j);
+                        tag[at] = (int8_t)k;
+                    } else {
+                        bad = true;
+                    }
                 }
-                const int at = wb[wave][f] + within;
-                if (at < e[f]) {
-                    band[at] = v0 + (uint32_t)(i0 + j);
-                    tag[at] = (int8_t)k;
-                } else {
-                    ctr->err = 2;
+                if (lane < NL) {
+                    const unsigned long long t = lane < 4 ? t0 : t1;
+                    nxt += (int)((t >> (16 * (lane & 3))) & 0xffffull);
                 }
             }
-            __syncthreads();  // wb is rewritten by the next chunk
         }
     }
+    if (__ballot(bad) != 0ull && lane == 0) ctr->err = 2;
 }
 
 // ---- per-round processing --------------------------------------------------------------------
@@ -375,6 +396,7 @@ __device__ __forceinline__ void dist1_body(const Geo& G, const Pool& P, const ui
     const int step = min(gridDim.x * blockDim.x, (hi - lo + U - 1) / U);
     if (tid >= step) return;
     for (int i0 = lo + tid; i0 < hi; i0 += U * step) {
+        if (U == 1 && Tagged && tag[i0] != layer) continue;  // another layer's entry
         uint32_t v[U];
         bool st[U];
         v[0] = list[i0];

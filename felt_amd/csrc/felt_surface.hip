@@ -842,8 +842,9 @@ static void compact_band(felt_surface* s) {
     LAUNCH(k_bscan_top, dim3(1), dim3(BT), s->blk_cnt.ptr, nb, G.NL, s->bbase, s->ctr);
     LAUNCH(k_bscan_apply, dim3(nb), dim3(BT), s->P.pcnt, G.np, G.NL, s->blk_cnt.ptr,
            s->blk_off.ptr, s->bbase);
-    DISPATCH(G.D, k_band_scatter, dim3(std::min(G.np, 4096)), dim3(TPB), G, s->P, s->blk_off.ptr,
-             s->band.ptr, s->band_tag.ptr, s->ctr);
+    DISPATCH(G.D, k_band_scatter, dim3((G.np + kScatterWaves - 1) / kScatterWaves),
+             dim3(64 * kScatterWaves), G, s->P, s->blk_off.ptr, s->band.ptr, s->band_tag.ptr,
+             s->ctr);
 }
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
