@@ -452,7 +452,7 @@ __device__ __forceinline__ void vox_ctx(const Geo& G, const Pool& P, uint32_t v,
 // neighbours in other bricks, then all values, so the K loads of a level are in flight together.
 template <int D, int K>
 __device__ __forceinline__ void gather(const Geo& G, const Pool& P, const VoxCtx& c,
-                                       const int (*off)[3], float* val) {
+                                       const int (*off)[3], float* val, bool* in_grid = nullptr) {
     int idx[K], part[K];
     bool in[K];
 #pragma unroll
@@ -496,6 +496,7 @@ __device__ __forceinline__ void gather(const Geo& G, const Pool& P, const VoxCtx
             else x = P.phi[(size_t)pg[j] * G.pv + idx[j]];
         }
         val[j] = x;
+        if (in_grid) in_grid[j] = in[j];
     }
 }
 

@@ -16,7 +16,6 @@ struct Ctr {
     int lu_span[2][2];   // {0, entries} of the two lookups (affected / affected buffer)
     int flag;            // a non-zero-layer status change happened this round
     int need_n;          // partitions to activate
-    int touched_n;       // partitions that lost a voxel to untrack
     int nan_events;
     int act_n;           // active partitions
     int z_span[2];       // {0, zero-layer delta entries} of a bounded update

@@ -81,7 +81,6 @@ __global__ void k_step_reset(Ctr* c) {
     c->lu_span[1][0] = c->lu_span[1][1] = 0;
     c->z_span[0] = c->z_span[1] = 0;
     c->band_span[0] = c->band_span[1] = 0;
-    c->touched_n = 0;
     c->need_n = 0;
     c->act_n = 0;
     c->one = 1;
@@ -97,7 +96,6 @@ __global__ void k_round_gate(Ctr* c, int r) {
     if (live) {
         c->flag = 0;
         c->sc_n = 0;
-        c->touched_n = 0;
         c->need_n = 0;
     }
 }
@@ -112,12 +110,6 @@ __global__ void k_lu_clear(Ctr* c, int lu, const int* live) {
     if (*live) c->lu_span[lu][1] = 0;
 }
 
-__global__ void k_flush_reset(Ctr* c, const int* live) {
-    if (*live) {
-        c->touched_n = 0;
-        c->need_n = 0;
-    }
-}
 
 // ---- band compaction ---------------------------------------------------------------------
 // The band list is layer-major: segment k holds the layer k-L voxels, partitions in index
@@ -339,6 +331,18 @@ __global__ void k_tk_init(const int8_t* tag, int8_t* tk, const int* rng, const i
     FELT_SPAN_LOOP(i, rng) tk[i] = tag[i];
 }
 
+// Start of a lookup round: tk = tag over the lookup's span, and the first nmask words of smask
+// cleared (its status-change partitions start empty).
+__global__ void k_round_prep(const int8_t* tag, int8_t* tk, const int* rng, unsigned* smask,
+                             int nmask, const int* live) {
+    if (!*live) return;
+    const int lo = rng[0], n = max(rng[1] - lo, nmask);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (lo + i < rng[1]) tk[lo + i] = tag[lo + i];
+        if (i < nmask) smask[i] = 0u;
+    }
+}
+
 __device__ __forceinline__ int8_t clamp_k(int k) { return (int8_t)(k < -100 ? -100 : (k > 100 ? 100 : k)); }
 
 template <int D>
@@ -461,42 +465,48 @@ __global__ void k_dist2_lookup(Geo G, Pool P, const uint32_t* list, const int8_t
     dist2_body<true>(G, P, list, tag, rng, layer, d, tk, nan_ctr);
 }
 
-// Change compaction, pass 1: per block the number of status changes and of in-band targets;
-// raises the round flag for a change out of a non-zero layer (Surface.hpp:1116).  The grid is
-// sized for the list's capacity; blocks past the span's end count zero.
+// Change compaction, pass 1: the list is cut into tiles of BT * ITEMS entries; per tile the
+// number of status changes and of in-band targets.  Raises the round flag for a change out of a
+// non-zero layer (Surface.hpp:1116).  Blocks stride over the tiles the span needs, so a short
+// span costs few blocks whatever the grid.
+constexpr int kTile = BT * ITEMS;
+
 __global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, const int* rng, int* counts,
                             Ctr* ctr, const int* live) {
     if (!*live) return;
-    const int n = rng[1];
-    const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
-    int nsc = 0, nbf = 0;
-    bool fl = false;
+    const int n = rng[1], ntiles = (n + kTile - 1) / kTile;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int base = t * kTile + threadIdx.x;
+        int nsc = 0, nbf = 0;
+        bool fl = false;
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const int i = base + j * BT;
-        if (i < n) {
-            const int a = tag[i], b = tk[i];
-            if (a != b) {
-                nsc++;
-                if (b >= -G.L && b <= G.L) nbf++;
-                if (a != 0) fl = true;
+        for (int j = 0; j < ITEMS; j++) {
+            const int i = base + j * BT;
+            if (i < n) {
+                const int a = tag[i], b = tk[i];
+                if (a != b) {
+                    nsc++;
+                    if (b >= -G.L && b <= G.L) nbf++;
+                    if (a != 0) fl = true;
+                }
             }
         }
-    }
-    int tsc, tbf;
-    block_excl_scan(nsc, &tsc);
-    block_excl_scan(nbf, &tbf);
-    if (__ballot(fl) != 0ull && fl) ctr->flag = 1;
-    if (threadIdx.x == 0) {
-        counts[2 * blockIdx.x] = tsc;
-        counts[2 * blockIdx.x + 1] = tbf;
+        int tsc, tbf;
+        block_excl_scan(nsc, &tsc);
+        block_excl_scan(nbf, &tbf);
+        if (__ballot(fl) != 0ull && fl) ctr->flag = 1;
+        if (threadIdx.x == 0) {
+            counts[2 * t] = tsc;
+            counts[2 * t + 1] = tbf;
+        }
     }
 }
 
-// Pass 2 (single block): exclusive scan of the per-block counts; the totals become the round's
+// Pass 2 (single block): exclusive scan of the per-tile counts; the totals become the round's
 // status-change count and the buffer lookup's span.
-__global__ void k_chg_scan(int* counts, int nb, Ctr* ctr, int buf, const int* live) {
+__global__ void k_chg_scan(int* counts, const int* rng, Ctr* ctr, int buf, const int* live) {
     if (!*live) return;
+    const int nb = (rng[1] + kTile - 1) / kTile;
     __shared__ int carry[2];
     if (threadIdx.x == 0) carry[0] = carry[1] = 0;
     __syncthreads();
@@ -525,38 +535,43 @@ __global__ void k_chg_scan(int* counts, int nb, Ctr* ctr, int buf, const int* li
 }
 
 // Pass 3: status-change records (status_change, Surface.hpp:1138-1178) and the buffer entries
-// of the in-band targets, in list order within each block.
+// of the in-band targets, in list order.
 __global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, const int8_t* tk,
                               const int* rng, const int* offsets, SC sc, Lu buf, const int* live) {
     if (!*live) return;
-    const int n = rng[1];
-    const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
-    int nsc = 0, nbf = 0;
+    const int n = rng[1], ntiles = (n + kTile - 1) / kTile;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int base = t * kTile + threadIdx.x;
+        int8_t a[ITEMS], b[ITEMS];
+        int nsc = 0, nbf = 0;
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const int i = base + j * BT;
-        if (i < n && tag[i] != tk[i]) {
-            nsc++;
-            if (tk[i] >= -G.L && tk[i] <= G.L) nbf++;
+        for (int j = 0; j < ITEMS; j++) {
+            const int i = base + j * BT;
+            a[j] = b[j] = 0;
+            if (i < n) {
+                a[j] = tag[i];
+                b[j] = tk[i];
+                if (a[j] != b[j]) {
+                    nsc++;
+                    if (b[j] >= -G.L && b[j] <= G.L) nbf++;
+                }
+            }
         }
-    }
-    int tsc, tbf;
-    int osc = block_excl_scan(nsc, &tsc) + offsets[2 * blockIdx.x];
-    int obf = block_excl_scan(nbf, &tbf) + offsets[2 * blockIdx.x + 1];
+        int tsc, tbf;
+        int osc = block_excl_scan(nsc, &tsc) + offsets[2 * t];
+        int obf = block_excl_scan(nbf, &tbf) + offsets[2 * t + 1];
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const int i = base + j * BT;
-        if (i < n) {
-            const int a = tag[i], b = tk[i];
-            if (a != b) {
+        for (int j = 0; j < ITEMS; j++) {
+            const int i = base + j * BT;
+            if (i < n && a[j] != b[j]) {
                 const uint32_t v = lv[i];
                 sc.v[osc] = v;
-                sc.from[osc] = (int8_t)a;
-                sc.to[osc] = (int16_t)b;
+                sc.from[osc] = a[j];
+                sc.to[osc] = (int16_t)b[j];
                 osc++;
-                if (b >= -G.L && b <= G.L) {
+                if (b[j] >= -G.L && b[j] <= G.L) {
                     buf.v[obf] = v;
-                    buf.tag[obf] = (int8_t)b;
+                    buf.tag[obf] = b[j];
                     obf++;
                 }
             }
@@ -572,37 +587,13 @@ __global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, cons
         for (int i = i##_b + (int)threadIdx.x, valid = i < i##_n, i##_once = 1; i##_once;     \
              i##_once = 0)
 
-// Partitions holding status changes from each layer this round (status_change(k) lists), one
-// atomic per distinct partition of a wavefront.
-__global__ void k_smask(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask, const int* live) {
-    if (!*live) return;
-    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
-        int p = -1, from = 0;
-        if (valid) {
-            p = P.slot_part[fdiv(sc.v[i], G.fpv)];
-            from = sc.from[i] + G.L;
-        }
-        const int lane = __lane_id();
-        unsigned long long todo = __ballot(p >= 0);
-        while (todo) {
-            const int leader = __ffsll((long long)todo) - 1;
-            const int p0 = __shfl(p, leader);
-            const bool mine = p >= 0 && p == p0;
-            unsigned bits = 0;
-            for (int k = 0; k < G.NL; k++)
-                if (__ballot(mine && from == k)) bits |= 1u << k;
-            if (lane == leader) atomicOr(&smask[p0], bits);
-            todo &= ~__ballot(mine);
-        }
-    }
-}
-
 // flush_status_change (Surface.hpp:1183-1244) with Untrack::retrack / untrack
-// (PartitionedMixin.hpp:489-558).  A partition that loses voxels to untrack is queued for the
-// emptiness check; its background becomes Distance(to) of the untrack that comes last in the
-// reference's flush order (the highest from-layer).
-__global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, int* ukey, int* tflag, int* tlist,
-                        Ctr* ctr, const int* live) {
+// (PartitionedMixin.hpp:489-558).  Per distinct partition of a wavefront: one atomic per layer
+// count that changes, the from-layers of its changes into smask (its status_change(k) lists),
+// and for untracks the key of its new background (Distance(to) of the untrack that comes last in
+// the reference's flush order: the highest from-layer) and a flag for the emptiness check.
+__global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask, int* ukey,
+                        int* tflag, const int* live) {
     if (!*live) return;
     FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
         int p = -1, dec = -1, inc = -1, key = -1;
@@ -617,35 +608,45 @@ __global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, int* ukey, int* t
             } else {
                 P.lay[v] = NONE;
                 P.phi[v] = (float)to;
-                key = ((from + G.L) << 16) | ((to + 32768) & 0xffff);
+                key = (dec << 16) | ((to + 32768) & 0xffff);
             }
         }
-        agg_pcnt(P.pcnt, G.NL, p, dec, inc);
-        // untracks: one atomicMax per distinct partition of the wave
         const int lane = __lane_id();
-        const int pu = key >= 0 ? p : -1;
-        unsigned long long todo = __ballot(pu >= 0);
+        unsigned long long todo = __ballot(p >= 0);
         while (todo) {
             const int leader = __ffsll((long long)todo) - 1;
-            const int p0 = __shfl(pu, leader);
-            const bool mine = pu >= 0 && pu == p0;
+            const int p0 = __shfl(p, leader);
+            const bool mine = p >= 0 && p == p0;
+            unsigned from_bits = 0;
+            for (int k = 0; k < G.NL; k++) {
+                const unsigned long long md = __ballot(mine && dec == k);
+                const int d = __popcll(__ballot(mine && inc == k)) - __popcll(md);
+                if (md) from_bits |= 1u << k;
+                if (lane == leader && d != 0) atomicAdd(&P.pcnt[p0 * G.NL + k], d);
+            }
             int kmax = mine ? key : -1;
             for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
             if (lane == leader) {
-                atomicMax(&ukey[p0], kmax);
-                if (atomicExch(&tflag[p0], 1) == 0) tlist[atomicAdd(&ctr->touched_n, 1)] = p0;
+                atomicOr(&smask[p0], from_bits);
+                if (kmax >= 0) {
+                    atomicMax(&ukey[p0], kmax);
+                    tflag[p0] = 1;
+                }
             }
             todo &= ~__ballot(mine);
         }
     }
 }
 
-__global__ void k_deactivate(Geo G, Pool P, const int* tlist, const Ctr* ctrc, int* ukey,
-                             int* tflag, int* free_stack, Ctr* ctr, const int* live) {
+// Partitions flagged by the flush: a partition left with no voxel is deactivated with the
+// background of its key and its brick returned to the pool.  Also starts the expansion's
+// activation list empty.
+__global__ void k_deactivate(Geo G, Pool P, int* ukey, int* tflag, int* free_stack, Ctr* ctr,
+                             const int* live) {
     if (!*live) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ctrc->touched_n;
-         i += gridDim.x * blockDim.x) {
-        const int p = tlist[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->need_n = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < G.np; p += gridDim.x * blockDim.x) {
+        if (!tflag[p]) continue;
         tflag[p] = 0;
         const int key = ukey[p];
         ukey[p] = -1;
@@ -668,35 +669,51 @@ __device__ __forceinline__ bool expanding(const Geo& G, int from, int to) {
 }
 
 // expand_narrow_band (Surface.hpp:1249-1355), phase 1: queue inactive partitions that an
-// expanding voxel's out-of-band neighbours fall in.
+// expanding voxel's out-of-band neighbours fall in.  The neighbours' partitions, then their
+// pages, then the backgrounds of the inactive ones are loaded level by level.
 template <int D>
 __global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag, int* nlist,
                               Ctr* ctr, const int* live) {
     if (!*live) return;
+    constexpr int NN = 2 * D;
     FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
-        bool cand = false;
-        Vx x;
-        if (valid && expanding(G, sc.from[i], sc.to[i])) {
-            cand = true;
-            decode<D>(G, P, sc.v[i], x);
+        const bool cand = valid && expanding(G, sc.from[i], sc.to[i]);
+        int pq[NN];
+        bool q_new[NN];
+#pragma unroll
+        for (int j = 0; j < NN; j++) {
+            pq[j] = -1;
+            q_new[j] = false;
         }
-        for (int a = 0; a < D; a++) {
-            for (int s = -1; s <= 1; s += 2) {
-                bool q_new = false;
-                int pq = 0;
-                if (cand) {
-                    int q[3] = {x.g[0], x.g[1], x.g[2]};
-                    q[a] += s;
-                    if (inside<D>(x.size, q)) {
-                        pq = part_of<D>(G, P, x.m, q);
-                        if (P.page[pq] < 0 &&
-                            !in_band(layer_of(P.pbg[pq], G.L, &ctr->nan_events), G.L))
-                            q_new = atomicExch(&nflag[pq], 1) == 0;
-                    }
+        if (cand) {
+            VoxCtx x;
+            vox_ctx<D>(G, P, sc.v[i], x);
+#pragma unroll
+            for (int a = 0; a < D; a++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int sd = h ? 1 : -1, j = 2 * a + h;
+                    const int qa = x.g[a] + sd, la = x.loc[a] + sd;
+                    // only neighbours in another partition can be in an inactive one
+                    if (qa >= 0 && qa < x.sz[a] && (la < 0 || la >= G.part[a]))
+                        pq[j] = x.own + sd * x.cs[a];
                 }
-                const int at = wave_append(&ctr->need_n, q_new);
-                if (q_new) nlist[at] = pq;
-            }
+            int pg[NN];
+#pragma unroll
+            for (int j = 0; j < NN; j++) pg[j] = pq[j] >= 0 ? P.page[pq[j]] : 0;
+            float bg[NN];
+#pragma unroll
+            for (int j = 0; j < NN; j++) bg[j] = (pq[j] >= 0 && pg[j] < 0) ? P.pbg[pq[j]] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < NN; j++)
+                if (pq[j] >= 0 && pg[j] < 0 &&
+                    !in_band(layer_of(bg[j], G.L, &ctr->nan_events), G.L))
+                    q_new[j] = atomicExch(&nflag[pq[j]], 1) == 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NN; j++) {
+            const int at = wave_append(&ctr->need_n, q_new[j]);
+            if (q_new[j]) nlist[at] = pq[j];
         }
     }
 }
@@ -751,59 +768,94 @@ __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* n
 }
 
 // expand_narrow_band phase 2: claim each out-of-band neighbour once and track it in layer
-// +-L with its distance (the value does not depend on which expanding voxel claims it).
+// +-L with its distance (the value does not depend on which expanding voxel claims it: a new
+// voxel's nearest neighbour towards the zero curve is in the band already).  The neighbours'
+// pages, then their values, then the claims are issued level by level; each claimed voxel's
+// distance gathers its own face neighbours.
 template <int D>
 __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, const int* live) {
     if (!*live) return;
+    constexpr int NN = 2 * D;
     FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
-        bool cand = false;
-        int side = 1;
-        Vx x;
-        if (valid && expanding(G, sc.from[i], sc.to[i])) {
-            cand = true;
-            side = sc.from[i] < 0 ? -1 : 1;
-            decode<D>(G, P, sc.v[i], x);
-        }
+        const bool cand = valid && expanding(G, sc.from[i], sc.to[i]);
+        const int side = cand && sc.from[i] < 0 ? -1 : 1;
         const int8_t target = (int8_t)(G.L * side);
-        for (int a = 0; a < D; a++) {
-            for (int s = -1; s <= 1; s += 2) {
-                int claimed_p = -1;
-                if (cand) {
-                    int q[3] = {x.g[0], x.g[1], x.g[2]};
-                    q[a] += s;
-                    if (inside<D>(x.size, q)) {
-                        int pq, lq;
-                        locate<D>(G, P, x.m, q, pq, lq);
-                        const int sq = P.page[pq];
-                        if (sq >= 0) {  // else activation failed (pool exhausted, flagged)
-                            const uint32_t vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
-                            const float dq = __hip_atomic_load(&P.phi[vq], __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
-                            if (!in_band(layer_of(dq, G.L, &ctr->nan_events), G.L) &&
-                                cas_byte(&P.lay[vq], NONE, target)) {
-                                // the neighbour's brick may touch a grid face where x's
-                                // does not: give it the real grid size
-                                const int4 r0 = mem_row0(P, x.m);
-                                Vx y;
-                                y.v = vq;
-                                y.m = x.m;
-                                for (int b = 0; b < 3; b++) {
-                                    y.loc[b] = 0;
-                                    y.g[b] = q[b];
-                                    y.size[b] = b < D ? (b == 0 ? r0.x : (b == 1 ? r0.y : r0.z)) : 1;
-                                }
-                                leaf_coords<D>(G, (uint32_t)lq, y.loc);
-                                const float nd = distance<D>(G, P, y, dq, (float)side);
-                                __hip_atomic_store(&P.phi[vq], nd, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                                claimed_p = pq;
-                            }
-                        }
-                    }
+        int cp[NN];
+#pragma unroll
+        for (int j = 0; j < NN; j++) cp[j] = -1;
+        if (cand) {
+            VoxCtx x;
+            vox_ctx<D>(G, P, sc.v[i], x);
+            bool in[NN], cross[NN];
+            int pq[NN], lq[NN], lc[NN][3];
+#pragma unroll
+            for (int a = 0; a < D; a++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int sd = h ? 1 : -1, j = 2 * a + h;
+                    const int qa = x.g[a] + sd, la = x.loc[a] + sd;
+                    in[j] = qa >= 0 && qa < x.sz[a];
+                    cross[j] = la < 0 || la >= G.part[a];
+                    lc[j][0] = x.loc[0];
+                    lc[j][1] = x.loc[1];
+                    lc[j][2] = x.loc[2];
+                    lc[j][a] = la < 0 ? la + G.part[a] : (la >= G.part[a] ? la - G.part[a] : la);
+                    pq[j] = cross[j] ? x.own + sd * x.cs[a] : x.own;
+                    int d[3] = {0, 0, 0};
+                    d[a] = sd;
+                    lq[j] = cross[j] ? leaf_of<D>(G, lc[j]) : (int)x.v + step_offset<D>(G, x.loc, d);
                 }
-                agg_pcnt(P.pcnt, G.NL, claimed_p, -1, claimed_p >= 0 ? target + G.L : -1);
+            uint32_t vq[NN];
+#pragma unroll
+            for (int j = 0; j < NN; j++) {
+                const int sq = (in[j] && cross[j]) ? P.page[pq[j]] : 0;
+                // same brick: lq is the voxel id; an inactive partition failed activation (pool
+                // exhausted, flagged) and is skipped
+                if (!in[j] || sq < 0) in[j] = false;
+                vq[j] = cross[j] ? (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq[j] : (uint32_t)lq[j];
+            }
+            float dq[NN];
+#pragma unroll
+            for (int j = 0; j < NN; j++)
+                dq[j] = in[j] ? __hip_atomic_load(&P.phi[vq[j]], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : 0.0f;
+            bool cl[NN];
+#pragma unroll
+            for (int j = 0; j < NN; j++)
+                cl[j] = in[j] && !in_band(layer_of(dq[j], G.L, &ctr->nan_events), G.L) &&
+                        cas_byte(&P.lay[vq[j]], NONE, target);
+#pragma unroll
+            for (int j = 0; j < NN; j++) {
+                if (!cl[j]) continue;
+                VoxCtx y;
+                y.v = vq[j];
+                y.m = x.m;
+                y.own = pq[j];
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    y.loc[b] = lc[j][b];
+                    y.g[b] = x.g[b];
+                    y.sz[b] = x.sz[b];
+                    y.cs[b] = x.cs[b];
+                }
+                y.g[j / 2] += (j & 1) ? 1 : -1;
+                float nv[NN];
+                bool nin[NN];
+                gather<D, NN>(G, P, y, D == 3 ? kStencil3 : kStencil2, nv, nin);
+                const float dir = (float)side;
+                float dist = dq[j] * dir;
+#pragma unroll
+                for (int k = 0; k < NN; k++)
+                    if (nin[k] && nv[k] * dir < dist) dist = nv[k] * dir;
+                const float nd = dist * dir + dir;
+                __hip_atomic_store(&P.phi[vq[j]], nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                cp[j] = pq[j];
             }
         }
+#pragma unroll
+        for (int j = 0; j < NN; j++)
+            agg_pcnt(P.pcnt, G.NL, cp[j], -1, cp[j] >= 0 ? target + G.L : -1);
     }
 }
 

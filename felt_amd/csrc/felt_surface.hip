@@ -896,9 +896,9 @@ static void distance_passes(felt_surface* s, bool from_band, int look, const int
 static void compact_changes(felt_surface* s, const uint32_t* v, const int8_t* tag,
                             const int8_t* tk, const int* rng, int buf, const int* live) {
     const Geo& G = s->G;
-    const int nb = chg_blocks(s);
+    const int nb = std::min(chg_blocks(s), 512);
     LAUNCH(k_chg_count, dim3(nb), dim3(BT), G, tag, tk, rng, s->chg_cnt.ptr, s->ctr, live);
-    LAUNCH(k_chg_scan, dim3(1), dim3(BT), s->chg_cnt.ptr, nb, s->ctr, buf, live);
+    LAUNCH(k_chg_scan, dim3(1), dim3(BT), s->chg_cnt.ptr, rng, s->ctr, buf, live);
     LAUNCH(k_chg_scatter, dim3(nb), dim3(BT), G, v, tag, tk, rng, s->chg_cnt.ptr, sc_of(s),
            felt_lu(s, buf), live);
 }
@@ -908,12 +908,10 @@ static void flush_expand(felt_surface* s, const int* live) {
     const Geo& G = s->G;
     const SC sc = sc_of(s);
     const int grid = grid_for_lists(s);
-    LAUNCH(k_smask, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->smask, live);
-    LAUNCH(k_flush_reset, dim3(1), dim3(1), s->ctr, live);
-    PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->ukey, s->tflag,
-            s->tlist, s->ctr, live);
-    LAUNCH(k_deactivate, dim3(std::min(grid, blocks_for(G.np))), dim3(TPB), G, s->P, s->tlist,
-           s->ctr, s->ukey, s->tflag, s->free_stack, s->ctr, live);
+    PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->smask, s->ukey,
+            s->tflag, live);
+    LAUNCH(k_deactivate, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P, s->ukey,
+           s->tflag, s->free_stack, s->ctr, live);
     prof_begin(s, PK_EXPAND, 0);
     DISPATCH(G.D, k_expand_mark, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->nflag, s->nlist,
              s->ctr, live);
@@ -925,9 +923,13 @@ static void flush_expand(felt_surface* s, const int* live) {
 
 // A round over lookup `look` (rounds after the first): distance passes over its entries,
 // changes into lookup `buf`, then flush and expand.
-static void lookup_round(felt_surface* s, int look, int buf, const int* live) {
-    LAUNCH(k_tk_init, dim3(grid_for_lists(s)), dim3(TPB), s->lu_tag[look].ptr, s->lu_tk[look].ptr,
-           lu_span(s, look), live);
+// `clear_smask`: the round also clears smask (device-gated rounds; host rounds clear it first).
+static void lookup_round(felt_surface* s, int look, int buf, const int* live,
+                         bool clear_smask = false) {
+    const int nmask = clear_smask ? s->G.np : 0;
+    LAUNCH(k_round_prep, dim3(std::max(grid_for_lists(s), std::min(1024, blocks_for(nmask)))),
+           dim3(TPB), s->lu_tag[look].ptr, s->lu_tk[look].ptr, lu_span(s, look), s->smask, nmask,
+           live);
     distance_passes(s, false, look, live);
     compact_changes(s, s->lu_v[look].ptr, s->lu_tag[look].ptr, s->lu_tk[look].ptr,
                     lu_span(s, look), buf, live);
@@ -978,9 +980,7 @@ static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
         const int look = r % 2 == 1 ? 1 : 0, buf = 1 - look;
         LAUNCH(k_round_gate, dim3(1), dim3(1), s->ctr, r);
         const int* live = &s->ctr->live[r];
-        LAUNCH(k_fill_u32_live, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), s->smask, G.np,
-               0u, live);
-        lookup_round(s, look, buf, live);
+        lookup_round(s, look, buf, live, true);
         LAUNCH(k_lu_clear, dim3(1), dim3(1), s->ctr, look, live);
     }
     LAUNCH(k_zero_int, dim3(1), dim3(1), &s->ctr->band_total);
