@@ -13,6 +13,7 @@ namespace felt {
 struct Ctr {
     int free_top;        // free-slot stack height
     int sc_n;            // status-change records this round
+    int exp_n;           // of which expanding (from +-L to +-(L-1))
     int lu_span[2][2];   // {0, entries} of the two lookups (affected / affected buffer)
     int flag;            // a non-zero-layer status change happened this round
     int need_n;          // partitions to activate
@@ -33,6 +34,8 @@ struct SC {
     uint32_t* v;
     int8_t* from;
     int16_t* to;
+    uint32_t* xv;   // the expanding records (outermost layer to the next one in): voxel
+    int8_t* xside;  // and side
 };
 
 // A lookup: the reference's AffectedLookupGrid as a tagged voxel list (tag = layer).

@@ -465,6 +465,11 @@ __global__ void k_dist2_lookup(Geo G, Pool P, const uint32_t* list, const int8_t
     dist2_body<true>(G, P, list, tag, rng, layer, d, tk, nan_ctr);
 }
 
+__device__ __forceinline__ bool expanding(const Geo& G, int from, int to) {
+    const int side = from < 0 ? -1 : 1;
+    return (from == G.L || from == -G.L) && to == G.L * side - side;
+}
+
 // Change compaction, pass 1: the list is cut into tiles of BT * ITEMS entries; per tile the
 // number of status changes and of in-band targets.  Raises the round flag for a change out of a
 // non-zero layer (Surface.hpp:1116).  Blocks stride over the tiles the span needs, so a short
@@ -477,7 +482,7 @@ __global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, const in
     const int n = rng[1], ntiles = (n + kTile - 1) / kTile;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int base = t * kTile + threadIdx.x;
-        int nsc = 0, nbf = 0;
+        int nsc = 0, nbf = 0, nex = 0;
         bool fl = false;
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
@@ -488,16 +493,19 @@ __global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, const in
                     nsc++;
                     if (b >= -G.L && b <= G.L) nbf++;
                     if (a != 0) fl = true;
+                    if (expanding(G, a, b)) nex++;
                 }
             }
         }
-        int tsc, tbf;
+        int tsc, tbf, tex;
         block_excl_scan(nsc, &tsc);
         block_excl_scan(nbf, &tbf);
+        block_excl_scan(nex, &tex);
         if (__ballot(fl) != 0ull && fl) ctr->flag = 1;
         if (threadIdx.x == 0) {
-            counts[2 * t] = tsc;
-            counts[2 * t + 1] = tbf;
+            counts[3 * t] = tsc;
+            counts[3 * t + 1] = tbf;
+            counts[3 * t + 2] = tex;
         }
     }
 }
@@ -507,30 +515,30 @@ __global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, const in
 __global__ void k_chg_scan(int* counts, const int* rng, Ctr* ctr, int buf, const int* live) {
     if (!*live) return;
     const int nb = (rng[1] + kTile - 1) / kTile;
-    __shared__ int carry[2];
-    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __shared__ int carry[3];
+    if (threadIdx.x < 3) carry[threadIdx.x] = 0;
     __syncthreads();
     for (int b0 = 0; b0 < nb; b0 += BT) {
         const int b = b0 + threadIdx.x;
-        const int a = b < nb ? counts[2 * b] : 0, c = b < nb ? counts[2 * b + 1] : 0;
-        int ta, tc;
-        const int ea = block_excl_scan(a, &ta);
-        const int ec = block_excl_scan(c, &tc);
-        if (b < nb) {
-            counts[2 * b] = carry[0] + ea;
-            counts[2 * b + 1] = carry[1] + ec;
-        }
+        int v[3], t[3], e[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = b < nb ? counts[3 * b + c] : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) e[c] = block_excl_scan(v[c], &t[c]);
+        if (b < nb)
+#pragma unroll
+            for (int c = 0; c < 3; c++) counts[3 * b + c] = carry[c] + e[c];
         __syncthreads();
-        if (threadIdx.x == 0) {
-            carry[0] += ta;
-            carry[1] += tc;
-        }
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int c = 0; c < 3; c++) carry[c] += t[c];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         ctr->sc_n = carry[0];
         ctr->lu_span[buf][0] = 0;
         ctr->lu_span[buf][1] = carry[1];
+        ctr->exp_n = carry[2];
     }
 }
 
@@ -543,7 +551,7 @@ __global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, cons
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int base = t * kTile + threadIdx.x;
         int8_t a[ITEMS], b[ITEMS];
-        int nsc = 0, nbf = 0;
+        int nsc = 0, nbf = 0, nex = 0;
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
             const int i = base + j * BT;
@@ -554,12 +562,14 @@ __global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, cons
                 if (a[j] != b[j]) {
                     nsc++;
                     if (b[j] >= -G.L && b[j] <= G.L) nbf++;
+                    if (expanding(G, a[j], b[j])) nex++;
                 }
             }
         }
-        int tsc, tbf;
-        int osc = block_excl_scan(nsc, &tsc) + offsets[2 * t];
-        int obf = block_excl_scan(nbf, &tbf) + offsets[2 * t + 1];
+        int tsc, tbf, tex;
+        int osc = block_excl_scan(nsc, &tsc) + offsets[3 * t];
+        int obf = block_excl_scan(nbf, &tbf) + offsets[3 * t + 1];
+        int oex = block_excl_scan(nex, &tex) + offsets[3 * t + 2];
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
             const int i = base + j * BT;
@@ -573,6 +583,11 @@ __global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, cons
                     buf.v[obf] = v;
                     buf.tag[obf] = b[j];
                     obf++;
+                }
+                if (expanding(G, a[j], b[j])) {
+                    sc.xv[oex] = v;
+                    sc.xside[oex] = a[j] < 0 ? -1 : 1;
+                    oex++;
                 }
             }
         }
@@ -617,18 +632,23 @@ __global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask, 
             const int leader = __ffsll((long long)todo) - 1;
             const int p0 = __shfl(p, leader);
             const bool mine = p >= 0 && p == p0;
+            // only the layers the partition's records touch (usually two or three)
+            unsigned lm = mine ? ((1u << dec) | (inc >= 0 ? 1u << inc : 0u)) : 0u;
+            for (int o = 32; o > 0; o >>= 1) lm |= __shfl_xor(lm, o);
             unsigned from_bits = 0;
-            for (int k = 0; k < G.NL; k++) {
+            while (lm) {
+                const int k = __ffs(lm) - 1;
+                lm &= lm - 1u;
                 const unsigned long long md = __ballot(mine && dec == k);
                 const int d = __popcll(__ballot(mine && inc == k)) - __popcll(md);
                 if (md) from_bits |= 1u << k;
                 if (lane == leader && d != 0) atomicAdd(&P.pcnt[p0 * G.NL + k], d);
             }
-            int kmax = mine ? key : -1;
-            for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
-            if (lane == leader) {
-                atomicOr(&smask[p0], from_bits);
-                if (kmax >= 0) {
+            if (lane == leader) atomicOr(&smask[p0], from_bits);
+            if (__ballot(mine && key >= 0)) {
+                int kmax = mine ? key : -1;
+                for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
+                if (lane == leader) {
                     atomicMax(&ukey[p0], kmax);
                     tflag[p0] = 1;
                 }
@@ -663,10 +683,6 @@ __global__ void k_deactivate(Geo G, Pool P, int* ukey, int* tflag, int* free_sta
     }
 }
 
-__device__ __forceinline__ bool expanding(const Geo& G, int from, int to) {
-    const int side = from < 0 ? -1 : 1;
-    return (from == G.L || from == -G.L) && to == G.L * side - side;
-}
 
 // expand_narrow_band (Surface.hpp:1249-1355), phase 1: queue inactive partitions that an
 // expanding voxel's out-of-band neighbours fall in.  The neighbours' partitions, then their
@@ -676,8 +692,8 @@ __global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag,
                               Ctr* ctr, const int* live) {
     if (!*live) return;
     constexpr int NN = 2 * D;
-    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
-        const bool cand = valid && expanding(G, sc.from[i], sc.to[i]);
+    FELT_WAVE_LOOP(i, valid, ctrc->exp_n) {
+        const bool cand = valid;
         int pq[NN];
         bool q_new[NN];
 #pragma unroll
@@ -687,7 +703,7 @@ __global__ void k_expand_mark(Geo G, Pool P, SC sc, const Ctr* ctrc, int* nflag,
         }
         if (cand) {
             VoxCtx x;
-            vox_ctx<D>(G, P, sc.v[i], x);
+            vox_ctx<D>(G, P, sc.xv[i], x);
 #pragma unroll
             for (int a = 0; a < D; a++)
 #pragma unroll
@@ -776,16 +792,16 @@ template <int D>
 __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, const int* live) {
     if (!*live) return;
     constexpr int NN = 2 * D;
-    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
-        const bool cand = valid && expanding(G, sc.from[i], sc.to[i]);
-        const int side = cand && sc.from[i] < 0 ? -1 : 1;
+    FELT_WAVE_LOOP(i, valid, ctrc->exp_n) {
+        const bool cand = valid;
+        const int side = cand ? sc.xside[i] : 1;
         const int8_t target = (int8_t)(G.L * side);
         int cp[NN];
 #pragma unroll
         for (int j = 0; j < NN; j++) cp[j] = -1;
         if (cand) {
             VoxCtx x;
-            vox_ctx<D>(G, P, sc.v[i], x);
+            vox_ctx<D>(G, P, sc.xv[i], x);
             bool in[NN], cross[NN];
             int pq[NN], lq[NN], lc[NN][3];
 #pragma unroll

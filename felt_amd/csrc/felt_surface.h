@@ -74,6 +74,8 @@ struct felt_surface {
     DBuf<uint32_t> sc_v;
     DBuf<int8_t> sc_from;
     DBuf<int16_t> sc_to;
+    DBuf<uint32_t> sc_xv;
+    DBuf<int8_t> sc_xside;
     // scratch
     DBuf<float> vox;
     bool vox_zeroed = false;

@@ -769,6 +769,8 @@ void felt_surface_release(felt_surface* s) {
     s->sc_v.release();
     s->sc_from.release();
     s->sc_to.release();
+    s->sc_xv.release();
+    s->sc_xside.release();
     s->vox.release();
     s->tmp_i.release();
     s->tmp_f.release();
@@ -780,7 +782,9 @@ void felt_surface_release(felt_surface* s) {
 Lu felt_lu(felt_surface* s, int i) {
     return Lu{s->lu_v[i].ptr, s->lu_tag[i].ptr, s->lu_d[i].ptr, s->lu_tk[i].ptr};
 }
-static SC sc_of(felt_surface* s) { return SC{s->sc_v.ptr, s->sc_from.ptr, s->sc_to.ptr}; }
+static SC sc_of(felt_surface* s) {
+    return SC{s->sc_v.ptr, s->sc_from.ptr, s->sc_to.ptr, s->sc_xv.ptr, s->sc_xside.ptr};
+}
 static int* lu_span(felt_surface* s, int i) { return &s->ctr->lu_span[i][0]; }
 static const int* always(felt_surface* s) { return &s->ctr->one; }
 
@@ -804,13 +808,15 @@ static int ensure_step_capacity(felt_surface* s, size_t band) {
     if ((rc = s->sc_v.ensure(n))) return rc;
     if ((rc = s->sc_from.ensure(n))) return rc;
     if ((rc = s->sc_to.ensure(n))) return rc;
+    if ((rc = s->sc_xv.ensure(n))) return rc;
+    if ((rc = s->sc_xside.ensure(n))) return rc;
     if ((rc = s->zlist.ensure(n))) return rc;
     if ((rc = s->zdelta.ensure(n))) return rc;
     if ((rc = s->band.ensure(n))) return rc;
     if ((rc = s->band_d.ensure(n))) return rc;
     if ((rc = s->band_tag.ensure(n))) return rc;
     if ((rc = s->band_tk.ensure(n))) return rc;
-    if ((rc = s->chg_cnt.ensure(2 * (n / (BT * ITEMS) + 2)))) return rc;
+    if ((rc = s->chg_cnt.ensure(3 * (n / (BT * ITEMS) + 2)))) return rc;
     s->list_cap = n;
     s->graph_valid = false;
     return FELT_OK;
