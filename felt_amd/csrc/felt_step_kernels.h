@@ -185,136 +185,167 @@ __global__ void k_bscan_apply(const int* pcnt, int np, int NL, const int* bs, in
     }
 }
 
-// Pass 4: one wavefront per partition writes the partition's band voxels into their layer
-// segments.  The scan gave the partition's first entry in every segment, so a wavefront needs no
-// shared counters and no barriers: it reads its brick's layer bytes 16 per lane (up to four
-// 1024-voxel chunks in flight), counts each lane's voxels per layer in 16-bit fields (layers 0-3
-// and 4-7 in two 64-bit words), scans the counts over the wave and writes each voxel at segment
-// start + running count + wave prefix, so entries follow the brick's storage order.  More than
-// 8 layers take a per-layer ballot path.  A voxel beyond its partition's count (layers and counts
-// disagreeing) is not written and is reported through err.
-constexpr int kScatterWaves = 4;  // wavefronts (partitions) per block
+// Pass 4: band voxels written into their layer segments, the entries of each partition in its
+// brick's storage order.  A block holds kScatterBlock wavefronts; a partition gets wpp of them
+// (1, 2, 4 or 8, fewer for small bricks so a block serves several partitions).  In each round a
+// wavefront reads one 4096-voxel span of its brick, 64 consecutive layer bytes per lane, counts
+// each lane's voxels per layer in 16-bit fields (layers 0-3 and 4-7 in two 64-bit words) and
+// scans the counts over the wave; the wavefronts of a partition then exchange their span totals
+// through LDS, and each voxel goes to segment start (from the scan of partition counts) + the
+// totals of earlier rounds and earlier spans + its position within the span.
+// More than 8 layers take a per-layer ballot path in the partition's first wavefront.  A voxel
+// beyond its partition's count (layers and counts disagreeing) is not written and is reported
+// through err.
+constexpr int kScatterBlock = 8;     // wavefronts per block
+constexpr int kScatterSpan = 4096;   // voxels a wavefront reads per round
+
+__device__ __forceinline__ int field16(unsigned long long lo, unsigned long long hi, int f) {
+    return (int)(((f < 4 ? lo : hi) >> (16 * (f & 3))) & 0xffffull);
+}
 
 template <int D>
-__global__ void __launch_bounds__(64 * kScatterWaves)
-k_band_scatter(Geo G, Pool P, const int* offsets, uint32_t* band, int8_t* tag, Ctr* ctr) {
+__global__ void __launch_bounds__(64 * kScatterBlock)
+k_band_scatter(Geo G, Pool P, int wpp, const int* offsets, uint32_t* band, int8_t* tag,
+               Ctr* ctr) {
+    __shared__ unsigned long long span_tot[2][kScatterBlock][2];
     const int lane = __lane_id();
-    const int p = blockIdx.x * kScatterWaves + (threadIdx.x >> 6);
-    if (p >= G.np) return;
-    const int s = P.page[p];
-    if (s < 0) return;
+    const int wv = threadIdx.x >> 6;
+    const int wi = wv & (wpp - 1);
+    const int p = blockIdx.x * (kScatterBlock / wpp) + wv / wpp;
+    const int s = p < G.np ? P.page[p] : -1;
+    const bool act = s >= 0;
     const int NL = G.NL;
     // lane k < NL: next free entry and end of layer k's entries of this partition
     int nxt = 0, end = 0;
-    if (lane < NL) {
+    if (act && lane < NL) {
         nxt = offsets[p * NL + lane];
         end = nxt + P.pcnt[p * NL + lane];
     }
-    const uint32_t v0 = (uint32_t)s * (uint32_t)G.pv;
+    const uint32_t v0 = act ? (uint32_t)s * (uint32_t)G.pv : 0u;
     const int8_t* lay = P.lay + (size_t)v0;
     bool bad = false;
     if (NL > 8) {
-        for (int i0 = 0; i0 < G.pv; i0 += 64) {
-            const int i = i0 + lane;
-            const int f = i < G.pv && lay[i] != NONE ? lay[i] + G.L : -1;
-            for (int k = 0; k < NL; k++) {
-                const unsigned long long m = __ballot(f == k);
-                if (!m) continue;
-                const int b = __shfl(nxt, k), e = __shfl(end, k);
-                if (f == k) {
-                    const int at = b + __popcll(m & ((1ull << lane) - 1ull));
-                    if (at < e) {
-                        band[at] = v0 + (uint32_t)i;
-                        tag[at] = (int8_t)(k - G.L);
-                    } else {
-                        bad = true;
+        if (act && wi == 0) {
+            for (int i0 = 0; i0 < G.pv; i0 += 64) {
+                const int i = i0 + lane;
+                const int f = i < G.pv && lay[i] != NONE ? lay[i] + G.L : -1;
+                for (int k = 0; k < NL; k++) {
+                    const unsigned long long m = __ballot(f == k);
+                    if (!m) continue;
+                    const int b = __shfl(nxt, k), e = __shfl(end, k);
+                    if (f == k) {
+                        const int at = b + __popcll(m & ((1ull << lane) - 1ull));
+                        if (at < e) {
+                            band[at] = v0 + (uint32_t)i;
+                            tag[at] = (int8_t)(k - G.L);
+                        } else {
+                            bad = true;
+                        }
                     }
+                    if (lane == k) nxt += __popcll(m);
                 }
-                if (lane == k) nxt += __popcll(m);
             }
         }
     } else {
-        constexpr int CH = 4;
-        for (int c0 = 0; c0 < G.pv; c0 += 1024 * CH) {
-            uint32_t q[CH][4];
+        int e[8];
 #pragma unroll
-            for (int c = 0; c < CH; c++) {
-                const int i0 = c0 + c * 1024 + 16 * lane;
-                q[c][0] = q[c][1] = q[c][2] = q[c][3] = 0x80808080u;
+        for (int f = 0; f < 8; f++) e[f] = __shfl(end, f);
+        const int rounds = (G.pv + kScatterSpan * wpp - 1) / (kScatterSpan * wpp);
+        for (int r = 0; r < rounds; r++) {
+            // this lane's 64 consecutive voxels of the wavefront's span
+            const int i0 = (r * wpp + wi) * kScatterSpan + 64 * lane;
+            uint32_t q[16];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int ic = i0 + 16 * c;
+                q[4 * c] = q[4 * c + 1] = q[4 * c + 2] = q[4 * c + 3] = 0x80808080u;
+                if (!act || ic >= G.pv) continue;
                 if ((G.pv & 15) == 0) {
-                    if (i0 < G.pv) {
-                        const uint4 w = *reinterpret_cast<const uint4*>(lay + i0);
-                        q[c][0] = w.x;
-                        q[c][1] = w.y;
-                        q[c][2] = w.z;
-                        q[c][3] = w.w;
-                    }
+                    const uint4 w = *reinterpret_cast<const uint4*>(lay + ic);
+                    q[4 * c] = w.x;
+                    q[4 * c + 1] = w.y;
+                    q[4 * c + 2] = w.z;
+                    q[4 * c + 3] = w.w;
                 } else {
                     for (int j = 0; j < 16; j++) {
                         const uint32_t byte =
-                            i0 + j < G.pv ? (uint32_t)(uint8_t)lay[i0 + j] : 0x80u;
-                        q[c][j >> 2] = (q[c][j >> 2] & ~(0xffu << (8 * (j & 3)))) |
-                                       (byte << (8 * (j & 3)));
+                            ic + j < G.pv ? (uint32_t)(uint8_t)lay[ic + j] : 0x80u;
+                        q[4 * c + (j >> 2)] =
+                            (q[4 * c + (j >> 2)] & ~(0xffu << (8 * (j & 3)))) |
+                            (byte << (8 * (j & 3)));
                     }
                 }
             }
+            unsigned long long n0 = 0, n1 = 0;
 #pragma unroll
-            for (int c = 0; c < CH; c++) {
-                const int i0 = c0 + c * 1024 + 16 * lane;
-                unsigned long long n0 = 0, n1 = 0;
+            for (int j = 0; j < 64; j++) {
+                const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
+                if (k == NONE) continue;
+                const int f = k + G.L;
+                const unsigned long long one = 1ull << (16 * (f & 3));
+                if (f < 4) n0 += one;
+                else n1 += one;
+            }
+            unsigned long long x0 = n0, x1 = n1;
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const int k = (int)(int8_t)((q[c][j >> 2] >> (8 * (j & 3))) & 0xffu);
-                    if (k == NONE) continue;
-                    const int f = k + G.L;
-                    const unsigned long long one = 1ull << (16 * (f & 3));
-                    if (f < 4) n0 += one;
-                    else n1 += one;
-                }
-                unsigned long long x0 = n0, x1 = n1;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const unsigned long long y0 = __shfl_up(x0, o), y1 = __shfl_up(x1, o);
-                    if (lane >= o) {
-                        x0 += y0;
-                        x1 += y1;
-                    }
-                }
-                const unsigned long long t0 = __shfl(x0, 63), t1 = __shfl(x1, 63);
-                int b[8], e[8];
-#pragma unroll
-                for (int f = 0; f < 8; f++) {
-                    b[f] = __shfl(nxt, f);
-                    e[f] = __shfl(end, f);
-                }
-                unsigned long long r0 = x0 - n0, r1 = x1 - n1;  // entries before this lane's
-#pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    const int k = (int)(int8_t)((q[c][j >> 2] >> (8 * (j & 3))) & 0xffu);
-                    if (k == NONE) continue;
-                    const int f = k + G.L;
-                    const int sh = 16 * (f & 3);
-                    int within;
-                    if (f < 4) {
-                        within = (int)((r0 >> sh) & 0xffffull);
-                        r0 += 1ull << sh;
-                    } else {
-                        within = (int)((r1 >> sh) & 0xffffull);
-                        r1 += 1ull << sh;
-                    }
-                    const int at = b[f] + within;
-                    if (at < e[f]) {
-                        band[at] = v0 + (uint32_t)(i0 + j);
-                        tag[at] = (int8_t)k;
-                    } else {
-                        bad = true;
-                    }
-                }
-                if (lane < NL) {
-                    const unsigned long long t = lane < 4 ? t0 : t1;
-                    nxt += (int)((t >> (16 * (lane & 3))) & 0xffffull);
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned long long y0 = __shfl_up(x0, o), y1 = __shfl_up(x1, o);
+                if (lane >= o) {
+                    x0 += y0;
+                    x1 += y1;
                 }
             }
+            const unsigned long long w0 = __shfl(x0, 63), w1 = __shfl(x1, 63);
+            // the partition's spans of this round: totals before this wavefront's and in all
+            unsigned long long pre0 = 0, pre1 = 0, all0 = w0, all1 = w1;
+            if (wpp > 1) {
+                if (lane == 0) {
+                    span_tot[r & 1][wv][0] = w0;
+                    span_tot[r & 1][wv][1] = w1;
+                }
+                __syncthreads();
+                all0 = all1 = 0;
+                for (int k = 0; k < wpp; k++) {
+                    const unsigned long long a0 = span_tot[r & 1][wv - wi + k][0];
+                    const unsigned long long a1 = span_tot[r & 1][wv - wi + k][1];
+                    if (k < wi) {
+                        pre0 += a0;
+                        pre1 += a1;
+                    }
+                    all0 += a0;
+                    all1 += a1;
+                }
+            }
+            int b[8];
+#pragma unroll
+            for (int f = 0; f < 8; f++) b[f] = __shfl(nxt, f) + field16(pre0, pre1, f);
+            // the layer bytes are unpacked again below rather than kept unpacked across the scan
+#pragma unroll
+            for (int w = 0; w < 16; w++) asm volatile("" : "+v"(q[w]));
+            unsigned long long r0 = x0 - n0, r1 = x1 - n1;  // entries before this lane's
+#pragma unroll
+            for (int j = 0; j < 64; j++) {
+                const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
+                if (k == NONE) continue;
+                const int f = k + G.L;
+                const int sh = 16 * (f & 3);
+                int within;
+                if (f < 4) {
+                    within = (int)((r0 >> sh) & 0xffffull);
+                    r0 += 1ull << sh;
+                } else {
+                    within = (int)((r1 >> sh) & 0xffffull);
+                    r1 += 1ull << sh;
+                }
+                const int at = b[f] + within;
+                if (at < e[f]) {
+                    band[at] = v0 + (uint32_t)(i0 + j);
+                    tag[at] = (int8_t)k;
+                } else {
+                    bad = true;
+                }
+            }
+            if (lane < NL) nxt += field16(all0, all1, lane);
         }
     }
     if (__ballot(bad) != 0ull && lane == 0) ctr->err = 2;

@@ -848,9 +848,11 @@ static void compact_band(felt_surface* s) {
     LAUNCH(k_bscan_top, dim3(1), dim3(BT), s->blk_cnt.ptr, nb, G.NL, s->bbase, s->ctr);
     LAUNCH(k_bscan_apply, dim3(nb), dim3(BT), s->P.pcnt, G.np, G.NL, s->blk_cnt.ptr,
            s->blk_off.ptr, s->bbase);
-    DISPATCH(G.D, k_band_scatter, dim3((G.np + kScatterWaves - 1) / kScatterWaves),
-             dim3(64 * kScatterWaves), G, s->P, s->blk_off.ptr, s->band.ptr, s->band_tag.ptr,
-             s->ctr);
+    int wpp = 1;  // wavefronts per partition: one per 4096-voxel span, at most a block
+    while (wpp < kScatterBlock && wpp * kScatterSpan < G.pv) wpp *= 2;
+    const int ppb = kScatterBlock / wpp;
+    DISPATCH(G.D, k_band_scatter, dim3((G.np + ppb - 1) / ppb), dim3(64 * kScatterBlock), G,
+             s->P, wpp, s->blk_off.ptr, s->band.ptr, s->band_tag.ptr, s->ctr);
 }
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
