@@ -113,74 +113,142 @@ __global__ void k_lu_clear(Ctr* c, int lu, const int* live) {
 
 // ---- band compaction ---------------------------------------------------------------------
 // The band list is layer-major: segment k holds the layer k-L voxels, partitions in index
-// order within a segment.  Entry e = k*np + p of that order holds pcnt[p*NL + k] voxels, so
-// the segment offsets are an exclusive scan of the per-partition layer counts, and a block
-// per active partition then writes its voxels.
+// order within a segment.  Partition p holds pcnt[p*NL + k] voxels of layer k, so its offset
+// in segment k is the segment start + the layer-k counts of the partitions before it.  The scan
+// runs partition-major, one thread per partition and PB partitions per block, over NL+1
+// columns: the NL layer counts and a flag for partitions holding any band voxel, whose scan
+// places them in the list of partitions the scatter visits.
 
-constexpr int SCAN_ITEMS = 8;  // consecutive scan entries per thread
+constexpr int PB = 256;  // partitions per scan block
 
-// Sum of this thread's SCAN_ITEMS consecutive entries starting at e0.
-__device__ __forceinline__ int bscan_sum(const int* pcnt, int np, int NL, int e0, int n) {
-    int k = e0 / np, p = e0 - k * np, sum = 0;
-    for (int j = 0; j < SCAN_ITEMS && e0 + j < n; j++) {
-        sum += pcnt[p * NL + k];
-        if (++p == np) {
-            p = 0;
-            k++;
+// Column c (c < NL: layer count, c == NL: the band flag) of partition p.
+__device__ __forceinline__ int bscan_col(const int* pcnt, int np, int NL, int p, int c,
+                                         int flag) {
+    if (p >= np) return 0;
+    return c < NL ? pcnt[p * NL + c] : (c == NL ? flag : 0);
+}
+
+__device__ __forceinline__ int bscan_flag(const int* pcnt, int np, int NL, int p) {
+    if (p >= np) return 0;
+    int any = 0;
+    for (int k = 0; k < NL; k++) any |= pcnt[p * NL + k];
+    return any != 0;
+}
+
+// Exclusive scan over the block of 8 columns packed as 32-bit pairs (v[i]: columns 2i, 2i+1);
+// tot gets the block totals.
+__device__ __forceinline__ void bscan_block8(unsigned long long v[4], unsigned long long tot[4]) {
+    __shared__ unsigned long long wsum[PB / 64][4];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    unsigned long long x[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) x[i] = v[i];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const unsigned long long y = __shfl_up(x[i], o);
+            if (lane >= o) x[i] += y;
         }
     }
-    return sum;
+    if (lane == 63) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) wsum[w][i] = x[i];
+    }
+    __syncthreads();
+    unsigned long long pre[4] = {0, 0, 0, 0}, t[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < PB / 64; k++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const unsigned long long a = wsum[k][i];
+            if (k < w) pre[i] += a;
+            t[i] += a;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[i] = x[i] - v[i] + pre[i];
+        tot[i] = t[i];
+    }
+    __syncthreads();  // wsum is free again
 }
 
-// Pass 1: per block the total of its BT * SCAN_ITEMS entries.
-__global__ void k_bscan_blocks(const int* pcnt, int np, int NL, int* bs) {
-    const int n = np * NL;
-    const int e0 = (blockIdx.x * BT + threadIdx.x) * SCAN_ITEMS;
-    const int sum = e0 < n ? bscan_sum(pcnt, np, NL, e0, n) : 0;
-    int tot;
-    block_excl_scan(sum, &tot);
-    if (threadIdx.x == 0) bs[blockIdx.x] = tot;
+__device__ __forceinline__ int col32(const unsigned long long v[4], int j) {
+    return (int)(uint32_t)(v[j >> 1] >> (32 * (j & 1)));
 }
 
-// Pass 2 (single block): exclusive scan of the block totals in place; the total is the band
-// size (base[NL] and the band span's end).
+// Pass 1: per block the totals of its partitions' columns, bs[b*(NL+1) + c].
+__global__ void __launch_bounds__(PB) k_bscan_blocks(const int* pcnt, int np, int NL, int* bs) {
+    const int p = blockIdx.x * PB + threadIdx.x;
+    const int flag = bscan_flag(pcnt, np, NL, p);
+    for (int g = 0; g <= NL; g += 8) {
+        unsigned long long v[4], tot[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            v[i] = (unsigned long long)(uint32_t)bscan_col(pcnt, np, NL, p, g + 2 * i, flag) |
+                   ((unsigned long long)(uint32_t)bscan_col(pcnt, np, NL, p, g + 2 * i + 1, flag)
+                    << 32);
+        bscan_block8(v, tot);
+        if (threadIdx.x < 8 && g + threadIdx.x <= NL)
+            bs[(size_t)blockIdx.x * (NL + 1) + g + threadIdx.x] = col32(tot, threadIdx.x);
+    }
+}
+
+// Pass 2 (single block): the layer columns scanned in layer-major order (segment k's blocks
+// start where segment k-1's end), the flag column on its own.  base[k] is segment k's start,
+// base[NL] the band size; the flag total is the number of partitions in the visit list.
 __global__ void k_bscan_top(int* bs, int nb, int NL, int* base, Ctr* c) {
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int b0 = 0; b0 < nb; b0 += BT) {
-        const int b = b0 + threadIdx.x;
-        int tot;
-        const int ex = block_excl_scan(b < nb ? bs[b] : 0, &tot);
-        if (b < nb) bs[b] = carry + ex;
-        __syncthreads();
-        if (threadIdx.x == 0) carry += tot;
-        __syncthreads();
+    for (int k = 0; k <= NL; k++) {
+        if (k == NL) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                base[NL] = carry;
+                c->band_span[0] = 0;
+                c->band_span[1] = carry;
+                carry = 0;
+            }
+            __syncthreads();
+        } else if (threadIdx.x == 0) {
+            base[k] = carry;
+        }
+        for (int b0 = 0; b0 < nb; b0 += BT) {
+            const int b = b0 + threadIdx.x;
+            int tot;
+            const size_t at = (size_t)b * (NL + 1) + k;
+            const int ex = block_excl_scan(b < nb ? bs[at] : 0, &tot);
+            if (b < nb) bs[at] = carry + ex;
+            __syncthreads();
+            if (threadIdx.x == 0) carry += tot;
+            __syncthreads();
+        }
     }
-    if (threadIdx.x == 0) {
-        base[NL] = carry;
-        c->band_span[0] = 0;
-        c->band_span[1] = carry;
-    }
+    if (threadIdx.x == 0) c->act_n = carry;
 }
 
-// Pass 3: each entry's offset; the first entry of each layer gives that segment's start.
-__global__ void k_bscan_apply(const int* pcnt, int np, int NL, const int* bs, int* offsets,
-                              int* base) {
-    const int n = np * NL;
-    const int e0 = (blockIdx.x * BT + threadIdx.x) * SCAN_ITEMS;
-    const int sum = e0 < n ? bscan_sum(pcnt, np, NL, e0, n) : 0;
-    int tot;
-    int at = block_excl_scan(sum, &tot) + bs[blockIdx.x];
-    if (e0 >= n) return;
-    int k = e0 / np, p = e0 - k * np;
-    for (int j = 0; j < SCAN_ITEMS && e0 + j < n; j++) {
-        if (p == 0) base[k] = at;
-        offsets[p * NL + k] = at;
-        at += pcnt[p * NL + k];
-        if (++p == np) {
-            p = 0;
-            k++;
+// Pass 3: each partition's offset in every segment, and the visit list.
+__global__ void __launch_bounds__(PB) k_bscan_apply(const int* pcnt, int np, int NL,
+                                                    const int* bs, int* offsets, int* visit) {
+    const int p = blockIdx.x * PB + threadIdx.x;
+    const int flag = bscan_flag(pcnt, np, NL, p);
+    const int* bb = bs + (size_t)blockIdx.x * (NL + 1);
+    for (int g = 0; g <= NL; g += 8) {
+        unsigned long long v[4], tot[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            v[i] = (unsigned long long)(uint32_t)bscan_col(pcnt, np, NL, p, g + 2 * i, flag) |
+                   ((unsigned long long)(uint32_t)bscan_col(pcnt, np, NL, p, g + 2 * i + 1, flag)
+                    << 32);
+        bscan_block8(v, tot);
+        if (p >= np) continue;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int c = g + j;
+            if (c < NL) offsets[p * NL + c] = bb[c] + col32(v, j);
+            else if (c == NL && flag) visit[bb[NL] + col32(v, j)] = p;
         }
     }
 }
@@ -191,10 +259,11 @@ __global__ void k_bscan_apply(const int* pcnt, int np, int NL, const int* bs, in
 // wavefront reads one 4096-voxel span of its brick, 64 consecutive layer bytes per lane, counts
 // each lane's voxels per layer in 16-bit fields (layers 0-3 and 4-7 in two 64-bit words) and
 // scans the counts over the wave; the wavefronts of a partition then exchange their span totals
-// through LDS, and each voxel goes to segment start (from the scan of partition counts) + the
-// totals of earlier rounds and earlier spans + its position within the span.
-// More than 8 layers take a per-layer ballot path in the partition's first wavefront.  A voxel
-// beyond its partition's count (layers and counts disagreeing) is not written and is reported
+// through LDS.  A span's entries go to segment start (from the scan of partition counts) + the
+// totals of earlier rounds and earlier spans + their position within the span; they are first
+// ordered by layer in LDS so that the stores to each segment are contiguous across lanes.
+// More than 8 layers take a per-layer ballot path in the partition's first wavefront.  A
+// partition whose layers and counts disagree writes no entry beyond its counts and is reported
 // through err.
 constexpr int kScatterBlock = 8;     // wavefronts per block
 constexpr int kScatterSpan = 4096;   // voxels a wavefront reads per round
@@ -203,16 +272,16 @@ __device__ __forceinline__ int field16(unsigned long long lo, unsigned long long
     return (int)(((f < 4 ? lo : hi) >> (16 * (f & 3))) & 0xffffull);
 }
 
-template <int D>
-__global__ void __launch_bounds__(64 * kScatterBlock)
-k_band_scatter(Geo G, Pool P, int wpp, const int* offsets, uint32_t* band, int8_t* tag,
-               Ctr* ctr) {
-    __shared__ unsigned long long span_tot[2][kScatterBlock][2];
-    const int lane = __lane_id();
-    const int wv = threadIdx.x >> 6;
-    const int wi = wv & (wpp - 1);
-    const int p = blockIdx.x * (kScatterBlock / wpp) + wv / wpp;
-    const int s = p < G.np ? P.page[p] : -1;
+// The spans of partition p (an entry of the visit list, or -1) for the wavefronts wv - wi ..
+// wv - wi + wpp - 1 of the block; rr counts the block's rounds, whose parity picks the span_tot
+// buffer.  Returns whether this lane found counts and layers disagreeing.
+__device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, int p, int wpp,
+                                                  int wi, int wv, int lane, int& rr,
+                                                  const int* offsets, uint32_t* band,
+                                                  int8_t* tag,
+                                                  unsigned long long (*span_tot)[kScatterBlock][2],
+                                                  uint16_t* stage) {
+    const int s = p >= 0 ? P.page[p] : -1;
     const bool act = s >= 0;
     const int NL = G.NL;
     // lane k < NL: next free entry and end of layer k's entries of this partition
@@ -247,13 +316,11 @@ k_band_scatter(Geo G, Pool P, int wpp, const int* offsets, uint32_t* band, int8_
             }
         }
     } else {
-        int e[8];
-#pragma unroll
-        for (int f = 0; f < 8; f++) e[f] = __shfl(end, f);
         const int rounds = (G.pv + kScatterSpan * wpp - 1) / (kScatterSpan * wpp);
-        for (int r = 0; r < rounds; r++) {
+        for (int r = 0; r < rounds; r++, rr++) {
             // this lane's 64 consecutive voxels of the wavefront's span
-            const int i0 = (r * wpp + wi) * kScatterSpan + 64 * lane;
+            const int s0 = (r * wpp + wi) * kScatterSpan;
+            const int i0 = s0 + 64 * lane;
             uint32_t q[16];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
@@ -300,14 +367,14 @@ k_band_scatter(Geo G, Pool P, int wpp, const int* offsets, uint32_t* band, int8_
             unsigned long long pre0 = 0, pre1 = 0, all0 = w0, all1 = w1;
             if (wpp > 1) {
                 if (lane == 0) {
-                    span_tot[r & 1][wv][0] = w0;
-                    span_tot[r & 1][wv][1] = w1;
+                    span_tot[rr & 1][wv][0] = w0;
+                    span_tot[rr & 1][wv][1] = w1;
                 }
                 __syncthreads();
                 all0 = all1 = 0;
                 for (int k = 0; k < wpp; k++) {
-                    const unsigned long long a0 = span_tot[r & 1][wv - wi + k][0];
-                    const unsigned long long a1 = span_tot[r & 1][wv - wi + k][1];
+                    const unsigned long long a0 = span_tot[rr & 1][wv - wi + k][0];
+                    const unsigned long long a1 = span_tot[rr & 1][wv - wi + k][1];
                     if (k < wi) {
                         pre0 += a0;
                         pre1 += a1;
@@ -316,37 +383,85 @@ k_band_scatter(Geo G, Pool P, int wpp, const int* offsets, uint32_t* band, int8_
                     all1 += a1;
                 }
             }
-            int b[8];
-#pragma unroll
-            for (int f = 0; f < 8; f++) b[f] = __shfl(nxt, f) + field16(pre0, pre1, f);
+            // The round's entries of layer f fill [nxt_f, nxt_f + all_f): one bound check per
+            // layer, and a partition whose counts disagree with its layers writes nothing.
+            const bool skip = __ballot(lane < NL && nxt + field16(all0, all1, lane) > end) != 0ull;
+            bad = bad || skip;
+            // The span's entries ordered by layer, then storage order: layer f starts at
+            // st_f = sum of the span's counts of layers below f, packed like the counts, so
+            // entry = field f of (entries before this lane's + st) + running count.
+            const unsigned long long rep = 0x0001000100010001ull;
+            const unsigned long long lo_tot =
+                ((w0 & 0xffffull) + ((w0 >> 16) & 0xffffull) + ((w0 >> 32) & 0xffffull) +
+                 (w0 >> 48));
+            const unsigned long long st0 = (w0 << 16) + (w0 << 32) + (w0 << 48);
+            const unsigned long long st1 = lo_tot * rep + (w1 << 16) + (w1 << 32) + (w1 << 48);
+            const int total = (int)(lo_tot + (w1 & 0xffffull) + ((w1 >> 16) & 0xffffull) +
+                                    ((w1 >> 32) & 0xffffull) + (w1 >> 48));
             // the layer bytes are unpacked again below rather than kept unpacked across the scan
 #pragma unroll
             for (int w = 0; w < 16; w++) asm volatile("" : "+v"(q[w]));
-            unsigned long long r0 = x0 - n0, r1 = x1 - n1;  // entries before this lane's
+            unsigned long long r0 = x0 - n0 + st0, r1 = x1 - n1 + st1;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int j = 0; j < 64; j++) {
                 const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
                 if (k == NONE) continue;
                 const int f = k + G.L;
                 const int sh = 16 * (f & 3);
-                int within;
+                int e;
                 if (f < 4) {
-                    within = (int)((r0 >> sh) & 0xffffull);
+                    e = (int)((r0 >> sh) & 0xffffull);
                     r0 += 1ull << sh;
                 } else {
-                    within = (int)((r1 >> sh) & 0xffffull);
+                    e = (int)((r1 >> sh) & 0xffffull);
                     r1 += 1ull << sh;
                 }
-                const int at = b[f] + within;
-                if (at < e[f]) {
-                    band[at] = v0 + (uint32_t)(i0 + j);
-                    tag[at] = (int8_t)k;
-                } else {
-                    bad = true;
+                stage[e] = (uint16_t)(64 * lane + j);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // copy out: consecutive lanes write consecutive entries of a layer segment
+            const int first = nxt + field16(pre0, pre1, lane) - field16(st0, st1, lane);
+            for (int e0 = 0; e0 < total; e0 += 64) {
+                const int e = e0 + lane;
+                int f = 0;
+#pragma unroll
+                for (int g = 1; g < 8; g++) f += (g < NL && field16(st0, st1, g) <= e) ? 1 : 0;
+                const int at = __shfl(first, f) + e;
+                if (e < total && !skip) {
+                    band[at] = v0 + (uint32_t)(s0 + stage[e]);
+                    tag[at] = (int8_t)(f - G.L);
                 }
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
             if (lane < NL) nxt += field16(all0, all1, lane);
         }
+    }
+    return bad;
+}
+
+template <int D>
+__global__ void __launch_bounds__(64 * kScatterBlock)
+k_band_scatter(Geo G, Pool P, int wpp, const int* visit, const int* offsets, uint32_t* band,
+               int8_t* tag, Ctr* ctr) {
+    __shared__ unsigned long long span_tot[2][kScatterBlock][2];
+    __shared__ uint16_t span_voxel[kScatterBlock][kScatterSpan];  // a span's entries, staged
+    const int lane = __lane_id();
+    const int wv = threadIdx.x >> 6;
+    const int wi = wv & (wpp - 1);
+    const int ppb = kScatterBlock / wpp;
+    const int nvisit = ctr->act_n;
+    const int ntask = (nvisit + ppb - 1) / ppb;
+    int rr = 0;
+    bool bad = false;
+    for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
+        const int i = t * ppb + wv / wpp;
+        const int p = i < nvisit ? visit[i] : -1;
+        bad = scatter_partition(G, P, p, wpp, wi, wv, lane, rr, offsets, band, tag, span_tot,
+                                span_voxel[wv]) || bad;
     }
     if (__ballot(bad) != 0ull && lane == 0) ctr->err = 2;
 }

@@ -60,7 +60,7 @@ struct felt_surface {
     DBuf<float> band_d;
     DBuf<int8_t> band_tag, band_tk;
     DBuf<int> chg_cnt;
-    DBuf<int> blk_cnt, blk_off;
+    DBuf<int> blk_cnt, blk_off, visit;  // band scan: block totals, offsets, visit list
     std::vector<int> band_base;
     // bounded-update zero layer
     DBuf<uint32_t> zlist;

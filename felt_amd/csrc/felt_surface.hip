@@ -707,6 +707,7 @@ int felt_surface_init(felt_surface* s) {
     std::memset(s->h_speed, 0, G.nmem * sizeof(Speed));
     if (s->blk_cnt.ensure((size_t)np * G.NL + G.NL + 1)) return FELT_E_DEVICE;
     if (s->blk_off.ensure((size_t)np * G.NL)) return FELT_E_DEVICE;
+    if (s->visit.ensure((size_t)np)) return FELT_E_DEVICE;
     const int gb = 1024;
     LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->P.page, np, -1);
     float bg = (float)(G.L + 1);
@@ -759,6 +760,7 @@ void felt_surface_release(felt_surface* s) {
     for (int i = 0; i < 2; i++) s->lu_tk[i].release();
     s->blk_cnt.release();
     s->blk_off.release();
+    s->visit.release();
     s->zlist.release();
     s->zdelta.release();
     for (int i = 0; i < 2; i++) {
@@ -834,25 +836,25 @@ static int chg_blocks(felt_surface* s) {
     return (int)std::max<size_t>(1, (list_entries(s) + BT * ITEMS - 1) / (BT * ITEMS));
 }
 
-// Band compaction into per-layer segments [bbase[k], bbase[k+1]): scan of the partition
-// layer counts, then one block per partition.
-static int bscan_blocks(const Geo& G) {
-    return (int)(((int64_t)G.np * G.NL + (int64_t)BT * SCAN_ITEMS - 1) / ((int64_t)BT * SCAN_ITEMS));
-}
+// Band compaction into per-layer segments [bbase[k], bbase[k+1]): a partition-major scan of
+// the partition layer counts, which also lists the partitions holding band voxels, then the
+// scatter over that list.
+static int bscan_blocks(const Geo& G) { return (int)(((int64_t)G.np + PB - 1) / PB); }
 
 static void compact_band(felt_surface* s) {
     const Geo& G = s->G;
     const int nb = bscan_blocks(G);
-    PLAUNCH(PK_COMPACT, (long long)G.np * G.NL, k_bscan_blocks, dim3(nb), dim3(BT), s->P.pcnt,
+    PLAUNCH(PK_COMPACT, (long long)G.np * G.NL, k_bscan_blocks, dim3(nb), dim3(PB), s->P.pcnt,
             G.np, G.NL, s->blk_cnt.ptr);
     LAUNCH(k_bscan_top, dim3(1), dim3(BT), s->blk_cnt.ptr, nb, G.NL, s->bbase, s->ctr);
-    LAUNCH(k_bscan_apply, dim3(nb), dim3(BT), s->P.pcnt, G.np, G.NL, s->blk_cnt.ptr,
-           s->blk_off.ptr, s->bbase);
+    LAUNCH(k_bscan_apply, dim3(nb), dim3(PB), s->P.pcnt, G.np, G.NL, s->blk_cnt.ptr,
+           s->blk_off.ptr, s->visit.ptr);
     int wpp = 1;  // wavefronts per partition: one per 4096-voxel span, at most a block
     while (wpp < kScatterBlock && wpp * kScatterSpan < G.pv) wpp *= 2;
     const int ppb = kScatterBlock / wpp;
-    DISPATCH(G.D, k_band_scatter, dim3((G.np + ppb - 1) / ppb), dim3(64 * kScatterBlock), G,
-             s->P, wpp, s->blk_off.ptr, s->band.ptr, s->band_tag.ptr, s->ctr);
+    const int grid = (int)std::min<int64_t>(((int64_t)G.np + ppb - 1) / ppb, 1024);
+    DISPATCH(G.D, k_band_scatter, dim3(grid), dim3(64 * kScatterBlock), G, s->P, wpp,
+             s->visit.ptr, s->blk_off.ptr, s->band.ptr, s->band_tag.ptr, s->ctr);
 }
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
