@@ -36,7 +36,8 @@ struct felt_polys {
     long long last_spx = 0;
     DBuf<uint8_t> cmask;
     DBuf<uint16_t> own;
-    DBuf<int> vbase, tbase, tot, dparts, pstart, vst, item_j, item_l, nv, nt, vleaf, work;
+    DBuf<uint32_t> ocand;  // per band cube: 2 bits per crossed edge, its owner among the sharers
+    DBuf<int> vbase, tbase, tot, dparts, pstart, vst, item_j, item_l, nv, nt, vleaf, work, vwork;
     DBuf<long long> dseg;
     DBuf<uint8_t> bits;
     DBuf<int> plist;  // partition lists exchanged with the host

@@ -89,14 +89,58 @@ struct MemV {
     int offset[3];
     int slot;
     int org[3];
+    int pbase;
+    int cstr[3];
 };
 
 __device__ __forceinline__ MemV mem_view(const Pool& P, int slot) {
     const int4 o = P.slot_org[slot];
     const int m = o.w & kMemberMask;
     const Mem& M = P.mem[m];
-    return MemV{m, {M.size[0], M.size[1], M.size[2]}, {M.offset[0], M.offset[1], M.offset[2]},
-                slot, {o.x, o.y, o.z}};
+    return MemV{m,    {M.size[0], M.size[1], M.size[2]}, {M.offset[0], M.offset[1], M.offset[2]},
+                slot, {o.x, o.y, o.z},                   M.pbase,
+                {M.cstr[0], M.cstr[1], M.cstr[2]}};
+}
+
+// get(pos) at K grid positions, loaded in levels: positions in the marched brick are read
+// directly; the others need their partition's page first.  All K loads of a level are issued
+// before any is used.  Background L+1 outside the grid, the partition background when inactive.
+template <int D, int K>
+__device__ __forceinline__ void get_many(const Geo& G, const Pool& P, const MemV& V,
+                                         const int (*q)[3], float* out) {
+    uint32_t idx[K];
+    int part[K];
+    bool in[K], same[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        bool inside = true, mine = true;
+        int lc[3] = {0, 0, 0}, lq[3] = {0, 0, 0};
+        int p = V.pbase;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            if (q[j][a] < 0 || q[j][a] >= V.size[a]) inside = false;
+            lc[a] = q[j][a] - V.org[a];
+            if (lc[a] < 0 || lc[a] >= G.part[a]) mine = false;
+            const int qa = (int)fdiv((uint32_t)q[j][a], G.fpart[a]);
+            lq[a] = q[j][a] - qa * G.part[a];
+            p += qa * V.cstr[a];
+        }
+        in[j] = inside;
+        same[j] = inside && mine;
+        idx[j] = same[j] ? (uint32_t)V.slot * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, lc)
+                         : (uint32_t)leaf_of<D>(G, lq);
+        part[j] = p;
+    }
+    int pg[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) pg[j] = (in[j] && !same[j]) ? P.page[part[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        float x = (float)(G.L + 1);
+        if (same[j]) x = P.phi[idx[j]];
+        else if (in[j]) x = pg[j] < 0 ? P.pbg[part[j]] : P.phi[(size_t)pg[j] * G.pv + idx[j]];
+        out[j] = x;
+    }
 }
 
 // get(pos) at grid position q of the member.
@@ -138,14 +182,15 @@ __device__ int band_cube(const Geo& G, const Pool& P, const MemV& V, int s, cons
 }
 
 // Owner of the vertex on edge e of the cube at base b: the lowest-leaf band cube of the
-// partition among those sharing the edge.  Returns its leaf and the edge's index in its frame.
+// partition among those sharing the edge.  Returns its leaf, the edge's index in its frame and
+// which of the sharers (McT::axis_edge order) it is.
 template <int D>
 __device__ int edge_owner(const Geo& G, const Pool& P, const MemV& V, int s, const int org[3],
-                          const int b[3], int e, int* e_owner) {
+                          const int b[3], int e, int* e_owner, int* cand) {
     int pa[3] = {0, 0, 0};
     for (int a = 0; a < D; a++) pa[a] = b[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
     const int axis = McT<D>::edge(e, D);
-    int best = -1, best_e = e;
+    int best = -1, best_e = e, best_i = 0;
     for (int i = 0; i < McT<D>::per_axis; i++) {
         const int f = McT<D>::axis_edge(axis, i);
         int cb[3] = {0, 0, 0};
@@ -154,9 +199,11 @@ __device__ int edge_owner(const Geo& G, const Pool& P, const MemV& V, int s, con
         if (l >= 0 && (best < 0 || l < best)) {
             best = l;
             best_e = f;
+            best_i = i;
         }
     }
     *e_owner = best_e;
+    *cand = best_i;
     return best;
 }
 
@@ -243,6 +290,65 @@ __device__ void grad_i(const Geo& G, const Pool& P, const MemV& V, const int g[3
     }
 }
 
+// grad_f at a vertex interpolated on an edge along axis ax.  Its other coordinates are whole
+// numbers, and so are those of every probe (whole numbers +-1 are exact in fp32), so in the
+// N-linear interpolation at a probe every axis but ax has weight 0 and the value is
+// t*hi + (1-t)*lo between the two grid values along ax, t from the probe's own ax coordinate
+// (stepped -1, +2, -1 in fp32 as the reference steps it).  Only those 2 values per probe are
+// read, 14 in all, in one level-split gather.  The terms the reference multiplies by weight 0
+// are finite, so the result is the same number; only the sign of an exact zero can differ.
+template <int D>
+__device__ void grad_edge(const Geo& G, const Pool& P, const MemV& V, const float w[3], int ax,
+                          float out[3]) {
+    constexpr int NP = 1 + 2 * D;
+    float pr[NP][3];
+    float pt[3] = {w[0], w[1], w[2]};
+    pr[0][0] = pt[0], pr[0][1] = pt[1], pr[0][2] = pt[2];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        pt[a] -= 1.0f;
+        pr[1 + 2 * a][0] = pt[0], pr[1 + 2 * a][1] = pt[1], pr[1 + 2 * a][2] = pt[2];
+        pt[a] += 2.0f;
+        pr[2 + 2 * a][0] = pt[0], pr[2 + 2 * a][1] = pt[1], pr[2 + 2 * a][2] = pt[2];
+        pt[a] -= 1.0f;
+    }
+    int q[2 * NP][3];
+    float t[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        float x = 0.f;
+        q[2 * j][0] = q[2 * j][1] = q[2 * j][2] = 0;
+        for (int a = 0; a < D; a++) {
+            const float f = floorf(pr[j][a]);
+            if (a == ax) x = pr[j][a] - f;
+            q[2 * j][a] = (int)f - V.offset[a];
+        }
+        t[j] = x;
+#pragma unroll
+        for (int a = 0; a < 3; a++) q[2 * j + 1][a] = q[2 * j][a] + (a == ax ? 1 : 0);
+    }
+    float v[2 * NP];
+    get_many<D, 2 * NP>(G, P, V, q, v);
+    float val[NP];
+#pragma unroll
+    for (int j = 0; j < NP; j++) val[j] = t[j] * v[2 * j + 1] + (1.0f - t[j]) * v[2 * j];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        float back = val[0], fwd = val[0];
+        int order = 0;
+        if (inside_wf<D>(V, pr[1 + 2 * a])) {
+            back = val[1 + 2 * a];
+            order++;
+        }
+        if (inside_wf<D>(V, pr[2 + 2 * a])) {
+            fwd = val[2 + 2 * a];
+            order++;
+        }
+        out[a] = order != 0 ? (fwd - back) / (float)order : 0.0f;
+        out[a] = out[a] / 1.0f;
+    }
+}
+
 __device__ void normalize3(float n[3]) {
     const float sq = n[0] * n[0] + (n[1] * n[1] + n[2] * n[2]);
     if (sq > 0.0f) {
@@ -273,7 +379,7 @@ __device__ void make_vertex(const Geo& G, const Pool& P, const MemV& V, const in
             const float fa = (float)(ga[a] + V.offset[a]), fb = (float)(gb[a] + V.offset[a]);
             pos[a] = fa + (fb - fa) * mu;
         }
-        if (D == 3) grad_f<D>(G, P, V, pos, n);
+        if (D == 3) grad_edge<D>(G, P, V, pos, axis, n);
     }
     if (D == 3) {
         normalize3(n);
@@ -354,29 +460,47 @@ __global__ void k_mb_scan(const int* cnt, int nb, int* start) {
     if (threadIdx.x == 0) start[nb] = carry;
 }
 
+// The band voxels of each marched partition in its brick's storage order, from the partition's
+// first item: each thread takes a run of consecutive leaves, counts its band voxels, and a
+// block scan of the counts places its run.
 template <int D>
 __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, int* item_j,
                              int* item_l) {
-    __shared__ int c;
-    if (threadIdx.x == 0) c = start[blockIdx.x];
-    __syncthreads();
+    __shared__ int wsum[BT / 64];
     const int s = P.page[parts[blockIdx.x]];
     if (s < 0) return;
-    for (int l = threadIdx.x; l < G.pv; l += blockDim.x)
-        if (P.lay[(size_t)s * G.pv + l] != NONE) {
-            int lc[3] = {0, 0, 0};
-            leaf_coords<D>(G, (uint32_t)l, lc);
-            const int at = atomicAdd(&c, 1);
-            item_j[at] = blockIdx.x;
-            item_l[at] = rm_of<D>(G, lc);  // items and their scratch are keyed by leaf PosIdx
-        }
+    const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int per = (G.pv + blockDim.x - 1) / blockDim.x;
+    const int l0 = threadIdx.x * per, l1 = min(l0 + per, G.pv);
+    const int8_t* lay = P.lay + (size_t)s * G.pv;
+    int c = 0;
+    for (int l = l0; l < l1; l++) c += lay[l] != NONE;
+    int x = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    int at = start[blockIdx.x] + x - c;
+    for (int k = 0; k < w && k < nw; k++) at += wsum[k];
+    for (int l = l0; l < l1; l++) {
+        if (lay[l] == NONE) continue;
+        int lc[3] = {0, 0, 0};
+        leaf_coords<D>(G, (uint32_t)l, lc);
+        item_j[at] = blockIdx.x;
+        item_l[at] = rm_of<D>(G, lc);  // items and their scratch are keyed by leaf PosIdx
+        at++;
+    }
 }
 
 // Cube mask, owned edges and vertex / simplex counts of each band voxel; the voxels with
 // anything to emit are listed in `work` so that the emit pass runs on them alone.
 template <int D>
 __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
-                          int M, uint8_t* cmask, uint16_t* own, int* nv, int* nt, int* work,
+                          int M, uint8_t* cmask, uint16_t* own, uint32_t* ocand, int* nv,
+                          int* nt, int* work,
                           int* nwork) {
     __shared__ int s_cnt, s_base;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,14 +519,16 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
         for (int a = 0; a < D; a++) g[a] += org[a];
         const unsigned m = cube_mask<D>(G, P, V, g);
         const unsigned em = McT<D>::edges_of(m);
-        unsigned om = 0;
+        unsigned om = 0, oc = 0;
         for (int e = 0; e < McT<D>::nedge; e++) {
             if (!((em >> e) & 1)) continue;
-            int eo;
-            if (edge_owner<D>(G, P, V, s, org, g, e, &eo) == l) om |= 1u << e;
+            int eo, ci;
+            if (edge_owner<D>(G, P, V, s, org, g, e, &eo, &ci) == l) om |= 1u << e;
+            oc |= (unsigned)ci << (2 * e);
         }
         cmask[(size_t)j * G.pv + l] = (uint8_t)m;
         own[(size_t)j * G.pv + l] = (uint16_t)om;
+        ocand[(size_t)j * G.pv + l] = oc;
         nv[i] = __popc(om);
         nt[i] = McT<D>::nsimplex(m);
         busy = nt[i] > 0;  // an owned edge is a crossed edge, so it has simplices too
@@ -413,6 +539,16 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
     if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(nwork, s_cnt) : 0;
     __syncthreads();
     if (busy) work[s_base + local] = i;
+}
+
+// The owned edges of the busy voxels as item * 16 + edge, in vertex order.
+__global__ void k_m_vlist(const int* work, int nwork, const uint16_t* own, const int* item_j,
+                          const int* item_l, int pv, const int* vb, int* vwork) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nwork) return;
+    const int i = work[t];
+    unsigned om = own[(size_t)item_j[i] * pv + item_l[i]];
+    for (int at = vb[i]; om; at++, om &= om - 1u) vwork[at] = i * 16 + (__ffs(om) - 1);
 }
 
 // Device-wide exclusive scan of two int arrays: block sums, top-level scan, apply.
@@ -501,13 +637,36 @@ __global__ void k_m_partstart(const int* start, int nb, const int* vb, const int
     tst[j] = it < M ? tb[it] : tot[1];
 }
 
-// Vertices of owned edges and simplices of every band cube (Poly.hpp:198-287), simplex vertex
-// indices local to the partition.
+// One vertex per owned edge (Poly.hpp:296-346), written at its cube's vertex base + the edge's
+// rank among the cube's owned edges.
 template <int D>
-__global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
-                         const int* work, int nwork, const uint8_t* cmask, const uint16_t* own,
-                         const int* vb, const int* tb, const int* vleaf, const int* vst,
-                         float* vout, uint32_t* tout) {
+__global__ void __launch_bounds__(TPB) k_m_vert(Geo G, Pool P, const int* parts,
+                                                const int* item_j, const int* item_l,
+                                                const int* vwork, int nvw, const uint16_t* own,
+                                                const int* vb, float* vout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nvw) return;
+    const int x = vwork[t];
+    const int i = x >> 4, e = x & 15;
+    const int j = item_j[i], l = item_l[i];
+    const MemV V = mem_view(P, P.page[parts[j]]);
+    int ga[3] = {0, 0, 0};
+    rm_coords<D>(G, (uint32_t)l, ga);
+    for (int a = 0; a < D; a++) ga[a] += V.org[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
+    const unsigned om = own[(size_t)j * G.pv + l];
+    const int rank = __popc(om & ((1u << e) - 1u));
+    const int vw = D == 3 ? 6 : 2;
+    make_vertex<D>(G, P, V, ga, McT<D>::edge(e, D), vout + (size_t)(vb[i] + rank) * vw);
+}
+
+// Simplices of every band cube (Poly.hpp:198-287), vertex indices local to the partition: a
+// simplex edge's vertex belongs to the owner the count pass recorded for it.
+template <int D>
+__global__ void __launch_bounds__(TPB) k_m_tri(Geo G, const int* item_j, const int* item_l,
+                                               const int* work, int nwork, const uint8_t* cmask,
+                                               const uint16_t* own, const uint32_t* ocand,
+                                               const int* tb, const int* vleaf, const int* vst,
+                                               uint32_t* tout) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nwork) return;
     const int i = work[t];
@@ -515,31 +674,21 @@ __global__ void k_m_emit(Geo G, Pool P, const int* parts, const int* item_j, con
     const size_t jb = (size_t)j * G.pv;
     const unsigned m = cmask[jb + l];
     const int ns = McT<D>::nsimplex(m);
-    if (ns == 0) return;
-    const int p = parts[j];
-    const int s = P.page[p];
-    const MemV V = mem_view(P, s);
-    const int* org = V.org;
-    int g[3] = {0, 0, 0};
-    rm_coords<D>(G, (uint32_t)l, g);
-    for (int a = 0; a < D; a++) g[a] += org[a];
-    const int vw = D == 3 ? 6 : 2;
-    const unsigned om = own[jb + l];
-    int rank = 0;
-    for (int e = 0; e < McT<D>::nedge; e++) {
-        if (!((om >> e) & 1)) continue;
-        int ga[3] = {0, 0, 0};
-        for (int a = 0; a < D; a++) ga[a] = g[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
-        make_vertex<D>(G, P, V, ga, McT<D>::edge(e, D), vout + (size_t)(vb[i] + rank) * vw);
-        rank++;
-    }
+    int lc0[3] = {0, 0, 0};
+    rm_coords<D>(G, (uint32_t)l, lc0);
     const int base = vst[j];
+    const unsigned oc = ocand[jb + l];
     for (int t = 0; t < ns; t++) {
         for (int k = 0; k < D; k++) {
             const int e = McT<D>::simplex_edge(m, t, k);
-            int eo;
-            const int ow = edge_owner<D>(G, P, V, s, org, g, e, &eo);
-            const int idx = vleaf[jb + ow] + __popc((unsigned)own[jb + ow] & ((1u << eo) - 1u)) - base;
+            // sharer ci of the edge owns its vertex, as its edge eo
+            const int axis = McT<D>::edge(e, D);
+            const int eo = McT<D>::axis_edge(axis, (int)((oc >> (2 * e)) & 3u));
+            int lc[3] = {0, 0, 0};
+            for (int a = 0; a < D; a++) lc[a] = lc0[a] + McT<D>::edge(e, a) - McT<D>::edge(eo, a);
+            const int ow = rm_of<D>(G, lc);
+            const int idx =
+                vleaf[jb + ow] + __popc((unsigned)own[jb + ow] & ((1u << eo) - 1u)) - base;
             tout[(size_t)(tb[i] + t) * D + k] = (uint32_t)idx;
         }
     }
@@ -658,6 +807,8 @@ void felt_polys_release(felt_polys* q) {
     q->tarena.release();
     q->cmask.release();
     q->own.release();
+    q->ocand.release();
+    q->vwork.release();
     q->vbase.release();
     q->tbase.release();
     q->tot.release();
@@ -831,7 +982,7 @@ int felt_polys_march_impl(felt_polys* q) {
             q->alive.untrack(p);
         }
     }
-    const size_t per = (size_t)G.pv * (1 + 2 + 4);
+    const size_t per = (size_t)G.pv * (1 + 2 + 4 + 4);
     const size_t batch = std::max<size_t>(1, std::min<size_t>(8192, (size_t)(192u << 20) / per));
     const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
     q->last_spx = 0;
@@ -859,6 +1010,7 @@ int felt_polys_march_impl(felt_polys* q) {
         if ((rc = q->work.ensure(M))) return rc;
         if ((rc = q->cmask.ensure((size_t)nb * G.pv))) return rc;
         if ((rc = q->own.ensure((size_t)nb * G.pv))) return rc;
+        if ((rc = q->ocand.ensure((size_t)nb * G.pv))) return rc;
         if ((rc = q->vleaf.ensure((size_t)nb * G.pv))) return rc;
         DISPATCHP(G.D, k_mb_scatter, dim3(nb), dim3(bt), G, s->P, q->dparts.ptr, q->pstart.ptr,
                   q->item_j.ptr, q->item_l.ptr);
@@ -868,7 +1020,8 @@ int felt_polys_march_impl(felt_polys* q) {
         HIPX(hipMemsetAsync(nwork, 0, sizeof(int), st));
         prof_begin(s, PK_MARCH_COUNT, M);
         DISPATCHP(G.D, k_m_count, dim3((M + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
-                  q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->nv.ptr, q->nt.ptr,
+                  q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->ocand.ptr,
+                  q->nv.ptr, q->nt.ptr,
                   q->work.ptr, nwork);
         prof_end(s);
         hipLaunchKernelGGL(k_s2_blocks, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs);
@@ -899,12 +1052,21 @@ int felt_polys_march_impl(felt_polys* q) {
         q->vtop += NV;
         q->ttop += NT;
         q->last_spx += NT;
+        if (NV > 0 && (rc = q->vwork.ensure((size_t)NV))) return rc;
         prof_begin(s, PK_MARCH_EMIT, W);
-        if (W > 0)
-            DISPATCHP(G.D, k_m_emit, dim3((W + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
-                      q->item_j.ptr, q->item_l.ptr, q->work.ptr, W, q->cmask.ptr, q->own.ptr,
-                      q->vbase.ptr, q->tbase.ptr, q->vleaf.ptr, vst, q->varena.ptr + vb0 * q->vw,
-                      q->tarena.ptr + tb0 * G.D);
+        if (W > 0) {
+            const int nvw = (int)NV;
+            hipLaunchKernelGGL(k_m_vlist, dim3((W + TPB - 1) / TPB), dim3(TPB), 0, st,
+                               q->work.ptr, W, q->own.ptr, q->item_j.ptr, q->item_l.ptr, G.pv,
+                               q->vbase.ptr, q->vwork.ptr);
+            if (nvw > 0)
+                DISPATCHP(G.D, k_m_vert, dim3((nvw + TPB - 1) / TPB), dim3(TPB), G, s->P,
+                          q->dparts.ptr, q->item_j.ptr, q->item_l.ptr, q->vwork.ptr, nvw,
+                          q->own.ptr, q->vbase.ptr, q->varena.ptr + vb0 * q->vw);
+            DISPATCHP(G.D, k_m_tri, dim3((W + TPB - 1) / TPB), dim3(TPB), G, q->item_j.ptr,
+                      q->item_l.ptr, q->work.ptr, W, q->cmask.ptr, q->own.ptr, q->ocand.ptr,
+                      q->tbase.ptr, q->vleaf.ptr, vst, q->tarena.ptr + tb0 * G.D);
+        }
         prof_end(s);
     }
     HIPX(hipStreamSynchronize(st));
