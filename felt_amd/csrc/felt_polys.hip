@@ -461,8 +461,8 @@ __global__ void k_mb_scan(const int* cnt, int nb, int* start) {
 }
 
 // The band voxels of each marched partition in its brick's storage order, from the partition's
-// first item: each thread takes a run of consecutive leaves, counts its band voxels, and a
-// block scan of the counts places its run.
+// first item: each wavefront takes a segment of consecutive leaves, counts its band voxels by
+// ballot, the segment counts are exchanged once through LDS, and a second pass writes them.
 template <int D>
 __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, int* item_j,
                              int* item_l) {
@@ -470,28 +470,30 @@ __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, 
     const int s = P.page[parts[blockIdx.x]];
     if (s < 0) return;
     const int lane = __lane_id(), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int per = (G.pv + blockDim.x - 1) / blockDim.x;
-    const int l0 = threadIdx.x * per, l1 = min(l0 + per, G.pv);
+    const int seg = (((G.pv + nw - 1) / nw) + 63) & ~63;
+    const int l0 = w * seg, l1 = min(l0 + seg, G.pv);
     const int8_t* lay = P.lay + (size_t)s * G.pv;
     int c = 0;
-    for (int l = l0; l < l1; l++) c += lay[l] != NONE;
-    int x = c;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(x, o);
-        if (lane >= o) x += y;
+    for (int b = l0; b < l1; b += 64) {
+        const int l = b + lane;
+        c += __popcll(__ballot(l < l1 && lay[l] != NONE));
     }
-    if (lane == 63) wsum[w] = x;
+    if (lane == 0) wsum[w] = c;
     __syncthreads();
-    int at = start[blockIdx.x] + x - c;
-    for (int k = 0; k < w && k < nw; k++) at += wsum[k];
-    for (int l = l0; l < l1; l++) {
-        if (lay[l] == NONE) continue;
-        int lc[3] = {0, 0, 0};
-        leaf_coords<D>(G, (uint32_t)l, lc);
-        item_j[at] = blockIdx.x;
-        item_l[at] = rm_of<D>(G, lc);  // items and their scratch are keyed by leaf PosIdx
-        at++;
+    int at = start[blockIdx.x];
+    for (int k = 0; k < w; k++) at += wsum[k];
+    for (int b = l0; b < l1; b += 64) {
+        const int l = b + lane;
+        const bool in = l < l1 && lay[l] != NONE;
+        const unsigned long long m = __ballot(in);
+        if (in) {
+            int lc[3] = {0, 0, 0};
+            leaf_coords<D>(G, (uint32_t)l, lc);
+            const int e = at + __popcll(m & ((1ull << lane) - 1ull));
+            item_j[e] = blockIdx.x;
+            item_l[e] = rm_of<D>(G, lc);  // items and their scratch are keyed by leaf PosIdx
+        }
+        at += __popcll(m);
     }
 }
 

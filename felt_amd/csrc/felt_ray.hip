@@ -65,13 +65,35 @@ __device__ float interp(const Geo& G, const Pool& P, int m, const RayMem& M, con
         fl[a] = (int)f - M.off[a];
         t[a] = pos[a] - f;
     }
-    float v[1 << D];
+    // the 2^D corners in levels: partition and leaf, then page, then value
+    constexpr int NC = 1 << D;
+    float v[NC];
+    int part[NC], leaf[NC], pg[NC];
+    bool in[NC];
 #pragma unroll
-    for (int i = 0; i < (1 << D); i++) {
-        int q[3] = {0, 0, 0};
+    for (int i = 0; i < NC; i++) {
+        int lc[3] = {0, 0, 0};
+        int p = M.pbase;
+        bool inside = true;
 #pragma unroll
-        for (int a = 0; a < D; a++) q[a] = fl[a] + ((i >> a) & 1);
-        v[i] = get_g<D>(G, P, m, M.size, q);
+        for (int a = 0; a < D; a++) {
+            const int q = fl[a] + ((i >> a) & 1);
+            if (q < 0 || q >= M.size[a]) inside = false;
+            const int qa = (int)fdiv((uint32_t)q, G.fpart[a]);
+            lc[a] = q - qa * G.part[a];
+            p += qa * M.cstr[a];
+        }
+        in[i] = inside;
+        part[i] = p;
+        leaf[i] = leaf_of<D>(G, lc);
+    }
+#pragma unroll
+    for (int i = 0; i < NC; i++) pg[i] = in[i] ? P.page[part[i]] : -1;
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
+        float x = (float)(G.L + 1);
+        if (in[i]) x = pg[i] < 0 ? P.pbg[part[i]] : P.phi[(size_t)pg[i] * G.pv + leaf[i]];
+        v[i] = x;
     }
     int n = 1 << D;
 #pragma unroll

@@ -256,7 +256,8 @@ __global__ void __launch_bounds__(PB) k_bscan_apply(const int* pcnt, int np, int
 // Pass 4: band voxels written into their layer segments, the entries of each partition in its
 // brick's storage order.  A block holds kScatterBlock wavefronts; a partition gets wpp of them
 // (1, 2, 4 or 8, fewer for small bricks so a block serves several partitions).  In each round a
-// wavefront reads one 4096-voxel span of its brick, 64 consecutive layer bytes per lane, counts
+// wavefront reads one span of 64*VPL voxels of its brick, VPL consecutive layer bytes per lane
+// (VPL = 64 for bricks of more than 512 voxels, else 8), counts
 // each lane's voxels per layer in 16-bit fields (layers 0-3 and 4-7 in two 64-bit words) and
 // scans the counts over the wave; the wavefronts of a partition then exchange their span totals
 // through LDS.  A span's entries go to segment start (from the scan of partition counts) + the
@@ -265,8 +266,7 @@ __global__ void __launch_bounds__(PB) k_bscan_apply(const int* pcnt, int np, int
 // More than 8 layers take a per-layer ballot path in the partition's first wavefront.  A
 // partition whose layers and counts disagree writes no entry beyond its counts and is reported
 // through err.
-constexpr int kScatterBlock = 8;     // wavefronts per block
-constexpr int kScatterSpan = 4096;   // voxels a wavefront reads per round
+constexpr int kScatterBlock = 8;  // wavefronts per block
 
 __device__ __forceinline__ int field16(unsigned long long lo, unsigned long long hi, int f) {
     return (int)(((f < 4 ? lo : hi) >> (16 * (f & 3))) & 0xffffull);
@@ -275,6 +275,7 @@ __device__ __forceinline__ int field16(unsigned long long lo, unsigned long long
 // The spans of partition p (an entry of the visit list, or -1) for the wavefronts wv - wi ..
 // wv - wi + wpp - 1 of the block; rr counts the block's rounds, whose parity picks the span_tot
 // buffer.  Returns whether this lane found counts and layers disagreeing.
+template <int VPL>
 __device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, int p, int wpp,
                                                   int wi, int wv, int lane, int& rr,
                                                   const int* offsets, uint32_t* band,
@@ -316,36 +317,46 @@ __device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, i
             }
         }
     } else {
-        const int rounds = (G.pv + kScatterSpan * wpp - 1) / (kScatterSpan * wpp);
+        constexpr int SPAN = 64 * VPL;        // voxels a wavefront reads per round
+        constexpr int CB = VPL < 16 ? VPL : 16;  // bytes per load
+        constexpr int NW = VPL / 4;              // words of layer bytes per lane
+        const int rounds = (G.pv + SPAN * wpp - 1) / (SPAN * wpp);
         for (int r = 0; r < rounds; r++, rr++) {
-            // this lane's 64 consecutive voxels of the wavefront's span
-            const int s0 = (r * wpp + wi) * kScatterSpan;
-            const int i0 = s0 + 64 * lane;
-            uint32_t q[16];
+            // this lane's VPL consecutive voxels of the wavefront's span
+            const int s0 = (r * wpp + wi) * SPAN;
+            const int i0 = s0 + VPL * lane;
+            uint32_t q[NW];
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int ic = i0 + 16 * c;
-                q[4 * c] = q[4 * c + 1] = q[4 * c + 2] = q[4 * c + 3] = 0x80808080u;
+            for (int c = 0; c < VPL / CB; c++) {
+                const int ic = i0 + CB * c;
+                uint32_t* qc = q + c * (CB / 4);
+#pragma unroll
+                for (int k = 0; k < CB / 4; k++) qc[k] = 0x80808080u;
                 if (!act || ic >= G.pv) continue;
-                if ((G.pv & 15) == 0) {
-                    const uint4 w = *reinterpret_cast<const uint4*>(lay + ic);
-                    q[4 * c] = w.x;
-                    q[4 * c + 1] = w.y;
-                    q[4 * c + 2] = w.z;
-                    q[4 * c + 3] = w.w;
+                if ((G.pv & (CB - 1)) == 0) {
+                    if constexpr (CB == 16) {
+                        const uint4 w = *reinterpret_cast<const uint4*>(lay + ic);
+                        qc[0] = w.x;
+                        qc[1] = w.y;
+                        qc[2] = w.z;
+                        qc[3] = w.w;
+                    } else {
+                        const uint2 w = *reinterpret_cast<const uint2*>(lay + ic);
+                        qc[0] = w.x;
+                        qc[1] = w.y;
+                    }
                 } else {
-                    for (int j = 0; j < 16; j++) {
+                    for (int j = 0; j < CB; j++) {
                         const uint32_t byte =
                             ic + j < G.pv ? (uint32_t)(uint8_t)lay[ic + j] : 0x80u;
-                        q[4 * c + (j >> 2)] =
-                            (q[4 * c + (j >> 2)] & ~(0xffu << (8 * (j & 3)))) |
-                            (byte << (8 * (j & 3)));
+                        qc[j >> 2] = (qc[j >> 2] & ~(0xffu << (8 * (j & 3)))) |
+                                     (byte << (8 * (j & 3)));
                     }
                 }
             }
             unsigned long long n0 = 0, n1 = 0;
 #pragma unroll
-            for (int j = 0; j < 64; j++) {
+            for (int j = 0; j < VPL; j++) {
                 const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
                 if (k == NONE) continue;
                 const int f = k + G.L;
@@ -400,12 +411,12 @@ __device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, i
                                     ((w1 >> 32) & 0xffffull) + (w1 >> 48));
             // the layer bytes are unpacked again below rather than kept unpacked across the scan
 #pragma unroll
-            for (int w = 0; w < 16; w++) asm volatile("" : "+v"(q[w]));
+            for (int w = 0; w < NW; w++) asm volatile("" : "+v"(q[w]));
             unsigned long long r0 = x0 - n0 + st0, r1 = x1 - n1 + st1;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int j = 0; j < 64; j++) {
+            for (int j = 0; j < VPL; j++) {
                 const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
                 if (k == NONE) continue;
                 const int f = k + G.L;
@@ -418,7 +429,7 @@ __device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, i
                     e = (int)((r1 >> sh) & 0xffffull);
                     r1 += 1ull << sh;
                 }
-                stage[e] = (uint16_t)(64 * lane + j);
+                stage[e] = (uint16_t)(VPL * lane + j);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -443,12 +454,12 @@ __device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, i
     return bad;
 }
 
-template <int D>
+template <int D, int VPL>
 __global__ void __launch_bounds__(64 * kScatterBlock)
 k_band_scatter(Geo G, Pool P, int wpp, const int* visit, const int* offsets, uint32_t* band,
                int8_t* tag, Ctr* ctr) {
     __shared__ unsigned long long span_tot[2][kScatterBlock][2];
-    __shared__ uint16_t span_voxel[kScatterBlock][kScatterSpan];  // a span's entries, staged
+    __shared__ uint16_t span_voxel[kScatterBlock][64 * VPL];  // a span's entries, staged
     const int lane = __lane_id();
     const int wv = threadIdx.x >> 6;
     const int wi = wv & (wpp - 1);
@@ -460,8 +471,8 @@ k_band_scatter(Geo G, Pool P, int wpp, const int* visit, const int* offsets, uin
     for (int t = blockIdx.x; t < ntask; t += gridDim.x) {
         const int i = t * ppb + wv / wpp;
         const int p = i < nvisit ? visit[i] : -1;
-        bad = scatter_partition(G, P, p, wpp, wi, wv, lane, rr, offsets, band, tag, span_tot,
-                                span_voxel[wv]) || bad;
+        bad = scatter_partition<VPL>(G, P, p, wpp, wi, wv, lane, rr, offsets, band, tag,
+                                     span_tot, span_voxel[wv]) || bad;
     }
     if (__ballot(bad) != 0ull && lane == 0) ctr->err = 2;
 }

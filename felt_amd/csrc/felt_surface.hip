@@ -849,12 +849,17 @@ static void compact_band(felt_surface* s) {
     LAUNCH(k_bscan_top, dim3(1), dim3(BT), s->blk_cnt.ptr, nb, G.NL, s->bbase, s->ctr);
     LAUNCH(k_bscan_apply, dim3(nb), dim3(PB), s->P.pcnt, G.np, G.NL, s->blk_cnt.ptr,
            s->blk_off.ptr, s->visit.ptr);
-    int wpp = 1;  // wavefronts per partition: one per 4096-voxel span, at most a block
-    while (wpp < kScatterBlock && wpp * kScatterSpan < G.pv) wpp *= 2;
+    // voxels per lane: 64 for bricks of more than 512 voxels, else 8; wavefronts per
+    // partition: one per span of 64 lanes, at most a block
+    const int vpl = G.pv > 512 ? 64 : 8;
+    int wpp = 1;
+    while (wpp < kScatterBlock && wpp * 64 * vpl < G.pv) wpp *= 2;
     const int ppb = kScatterBlock / wpp;
     const int grid = (int)std::min<int64_t>(((int64_t)G.np + ppb - 1) / ppb, 1024);
-    DISPATCH(G.D, k_band_scatter, dim3(grid), dim3(64 * kScatterBlock), G, s->P, wpp,
-             s->visit.ptr, s->blk_off.ptr, s->band.ptr, s->band_tag.ptr, s->ctr);
+    auto kern = G.D == 2 ? (vpl == 64 ? k_band_scatter<2, 64> : k_band_scatter<2, 8>)
+                         : (vpl == 64 ? k_band_scatter<3, 64> : k_band_scatter<3, 8>);
+    LAUNCH(kern, dim3(grid), dim3(64 * kScatterBlock), G, s->P, wpp, s->visit.ptr,
+           s->blk_off.ptr, s->band.ptr, s->band_tag.ptr, s->ctr);
 }
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
