@@ -605,6 +605,14 @@ __device__ __forceinline__ int wave_append_n(int* counter, int k) {
     return base + x - k;
 }
 
+// Sets a 0/1 mark byte to 1 with one 32-bit atomic OR on its word; true when it was 0.
+__device__ __forceinline__ bool claim_mark(uint8_t* addr) {
+    const uintptr_t a = (uintptr_t)addr;
+    unsigned int* w = (unsigned int*)(a & ~(uintptr_t)3);
+    const int sh = (int)(a & 3) * 8;
+    return ((atomicOr(w, 1u << sh) >> sh) & 0xffu) == 0u;
+}
+
 // Compare-and-swap of one byte through its aligned 32-bit word.
 __device__ __forceinline__ bool cas_byte(int8_t* addr, int8_t expect, int8_t desired) {
     const uintptr_t a = (uintptr_t)addr;

@@ -181,10 +181,12 @@ __device__ __forceinline__ float aff_neighbour(const Geo& G, const Pool& P, cons
 
 // calc_affected (Surface.hpp:780-913), one breadth-first round: face neighbours of the last
 // round's voxels whose current layer is in the band join the set once.  Each lane claims its
-// new neighbours first, then the wave reserves all their entries with one atomic.
+// new neighbours first (one atomic OR on the mark's word), then the block reserves all their
+// entries with one atomic on the list's end.
 template <int D>
 __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j,
                                  uint8_t* amark) {
+    __shared__ int s_cnt, s_base;
     const int lo = ctr->front[j - 1], hi = ctr->front[j];
     const int n = hi - lo;
     const int stride = gridDim.x * blockDim.x;
@@ -208,8 +210,7 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
                     const float dq = aff_neighbour<D>(G, P, x, a, sd, q, vq, active);
                     const int kq = layer_of(dq, G.L, &ctr->nan_events);
                     // most neighbours are already in the set: test before the atomic
-                    if (in_band(kq, G.L) && active && amark[vq] == 0 &&
-                        cas_byte((int8_t*)&amark[vq], 0, 1)) {
+                    if (in_band(kq, G.L) && active && amark[vq] == 0 && claim_mark(&amark[vq])) {
                         const int d = 2 * a + t;
                         mask |= 1u << d;
                         tags |= (unsigned)(kq + 8) << (4 * d);
@@ -217,7 +218,13 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
                 }
             }
         }
-        int at = wave_append_n(&ctr->lu_span[lu_i][1], __popc(mask));
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        int at = wave_append_n(&s_cnt, __popc(mask));
+        __syncthreads();
+        if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(&ctr->lu_span[lu_i][1], s_cnt) : 0;
+        __syncthreads();
+        at += s_base;
         if (mask) {
 #pragma unroll
             for (int a = 0; a < D; a++) {
