@@ -940,9 +940,10 @@ __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* n
     }
 }
 
-// expand_narrow_band phase 2: claim each out-of-band neighbour once and track it in layer
-// +-L with its distance (the value does not depend on which expanding voxel claims it: a new
-// voxel's nearest neighbour towards the zero curve is in the band already).  The neighbours'
+// expand_narrow_band phase 2 (Surface.hpp:1249-1355): every neighbour whose value is outside the
+// band gets its distance, and joins layer +-L if it is in no list (the value does not depend on
+// which expanding voxel claims it: the claimed voxel's nearest neighbour towards the zero curve
+// is in the band already, so claims racing for one voxel write the same value).  The neighbours'
 // pages, then their values, then the claims are issued level by level; each claimed voxel's
 // distance gathers its own face neighbours.
 template <int D>
@@ -993,11 +994,15 @@ __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, 
                 dq[j] = in[j] ? __hip_atomic_load(&P.phi[vq[j]], __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT)
                               : 0.0f;
-            bool cl[NN];
+            // As the reference, a neighbour is claimed when its value is outside the band, even
+            // if it is tracked already (a value of exactly +-(L+0.5) rounds outside): its value
+            // is recomputed, and it joins the list only if it was in none (first list wins).
+            bool cl[NN], fresh[NN];
 #pragma unroll
-            for (int j = 0; j < NN; j++)
-                cl[j] = in[j] && !in_band(layer_of(dq[j], G.L, &ctr->nan_events), G.L) &&
-                        cas_byte(&P.lay[vq[j]], NONE, target);
+            for (int j = 0; j < NN; j++) {
+                cl[j] = in[j] && !in_band(layer_of(dq[j], G.L, &ctr->nan_events), G.L);
+                fresh[j] = cl[j] && cas_byte(&P.lay[vq[j]], NONE, target);
+            }
 #pragma unroll
             for (int j = 0; j < NN; j++) {
                 if (!cl[j]) continue;
@@ -1023,7 +1028,7 @@ __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, 
                     if (nin[k] && nv[k] * dir < dist) dist = nv[k] * dir;
                 const float nd = dist * dir + dir;
                 __hip_atomic_store(&P.phi[vq[j]], nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                cp[j] = pq[j];
+                if (fresh[j]) cp[j] = pq[j];
             }
         }
 #pragma unroll

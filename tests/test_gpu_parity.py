@@ -182,3 +182,55 @@ def test_polys_2d_parity():
         pg.march(); pc.march()
         d = parity.mesh_diff(pg, g, pc, c)
         assert not d, (i, d)
+
+
+@pytest.mark.parametrize("D,size,part", [(3, (96, 96, 64), (32, 32, 32)),
+                                         (2, (384, 256), (128, 128))],
+                         ids=["3d-32^3", "2d-128^2"])
+def test_large_partitions(D, size, part):
+    """Bricks large enough that several wavefronts share one in the band compaction (32^3 and
+    128^2 partitions): global and bounded steps compared after every other step, then a full
+    march compared as meshes."""
+    import felt_amd
+    L = 3 if D == 3 else 2
+    g, c = pair(D, L, size, part)
+    if D == 3:
+        cs, rs = [(-20.0, 10.0, 3.0), (18.0, -12.0, -6.0)], [22.0, 15.0]
+    else:
+        cs, rs = [(-60.0, 20.0), (70.0, -30.0)], [70.0, 45.0]
+    step_both(g, c, "load_dense", workloads.spheres_phi(size, L, cs, rs))
+    check(g, c, "load")
+    for i in range(12):
+        sp = speed_const_curv(-0.4, 0.2) if i % 3 else speed_constant(0.5)
+        if i % 4 == 3:
+            lo = [-s // 4 for s in size]
+            hi = [s // 8 for s in size]
+            step_both(g, c, "update_box", lo, hi, sp)
+        else:
+            step_both(g, c, "update", sp)
+        if i % 2:
+            check(g, c, f"step {i}", snapshot=i >= 9)
+    pg, pc = felt_amd.Polys(g), oracle_lib.polys(c)
+    pg.invalidate(); pc.invalidate()
+    pg.march(); pc.march()
+    d = parity.mesh_diff(pg, g, pc, c)
+    assert not d, d
+    assert pg.totals()[1] > 0
+    print(f"[large] {D}D {part}: band {g.stats().band_voxels}, simplices {pg.totals()[1]}",
+          flush=True)
+
+
+@pytest.mark.parametrize("D,L,size,part,cs,rs", [
+    (2, 2, (384, 256), (32, 32), [(-60.0, 20.0), (70.0, -30.0)], [70.0, 45.0]),
+    (3, 2, (160, 160, 32), (16, 16, 16), [(-30.0, 10.0, 0.0)], [70.0])],
+    ids=["2d", "3d"])
+def test_expansion_reclaims_half_values(D, L, size, part, cs, rs):
+    """A voxel claimed by the expansion with exactly -(L+0.5) is tracked in layer -L, but its
+    value rounds outside the band (iround(-(L+0.5) + FLT_EPSILON) = -(L+1) in fp32), so a later
+    expansion next to it recomputes its distance (Surface.hpp:1249-1355 tests the value, not
+    the lists).  These spheres produce such voxels in the first step."""
+    g, c = pair(D, L, size, part)
+    step_both(g, c, "load_dense", workloads.spheres_phi(size, L, cs, rs))
+    for i in range(3):
+        step_both(g, c, "update", speed_constant(0.5))
+        check(g, c, f"step {i}")
