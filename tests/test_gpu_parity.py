@@ -234,3 +234,27 @@ def test_expansion_reclaims_half_values(D, L, size, part, cs, rs):
     for i in range(3):
         step_both(g, c, "update", speed_constant(0.5))
         check(g, c, f"step {i}")
+
+
+@pytest.mark.parametrize("D,L,size,part", [(2, 1, (192, 128), (16, 16)),
+                                           (3, 1, (96, 96, 48), (16, 16, 16)),
+                                           (2, 4, (192, 128), (16, 16)),
+                                           (3, 4, (96, 96, 64), (32, 32, 32)),
+                                           (2, 5, (192, 128), (8, 8))],
+                         ids=["2d-L1", "3d-L1", "2d-L4", "3d-L4-32^3", "2d-L5"])
+def test_layer_counts(D, L, size, part):
+    """Other narrow-band widths: L=1, and L=4, 5 whose 2L+1 > 8 layers take the band
+    compaction's per-layer path; global and bounded steps compared after every step."""
+    g, c = pair(D, L, size, part)
+    if D == 3:
+        cs, rs = [(-10.0, 5.0, 0.0)], [30.0]
+    else:
+        cs, rs = [(-30.0, 10.0), (35.0, -15.0)], [35.0, 22.0]
+    step_both(g, c, "load_dense", workloads.spheres_phi(size, L, cs, rs))
+    for i in range(8):
+        sp = speed_const_curv(-0.4, 0.2) if i % 2 else speed_constant(0.5)
+        if i % 3 == 2:
+            step_both(g, c, "update_box", [-s // 4 for s in size], [s // 8 for s in size], sp)
+        else:
+            step_both(g, c, "update", sp)
+        check(g, c, f"L={L} step {i}")
