@@ -427,20 +427,16 @@ __device__ __forceinline__ int block_excl_scan(int val, int* total) {
     return res;
 }
 
-// Band voxels per marched partition.
-template <int D>
-__global__ void k_mb_count(Geo G, Pool P, const int* parts, int* cnt) {
-    __shared__ int c;
-    if (threadIdx.x == 0) c = 0;
-    __syncthreads();
-    const int s = P.page[parts[blockIdx.x]];
-    int mine = 0;
-    if (s >= 0)
-        for (int l = threadIdx.x; l < G.pv; l += blockDim.x)
-            mine += P.lay[(size_t)s * G.pv + l] != NONE;
-    atomicAdd(&c, mine);
-    __syncthreads();
-    if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+// Band voxels per marched partition: the sum of its layer counts (a voxel is in a layer list
+// exactly when its layer byte is not NONE).
+__global__ void k_mb_count(Geo G, Pool P, const int* parts, int nb, int* cnt) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nb) return;
+    const int p = parts[j];
+    int c = 0;
+    if (P.page[p] >= 0)
+        for (int k = 0; k < G.NL; k++) c += P.pcnt[p * G.NL + k];
+    cnt[j] = c;
 }
 
 // Single block: start[j] = exclusive prefix of cnt, start[nb] = total.
@@ -481,19 +477,30 @@ __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, 
     if (lane == 0) wsum[w] = c;
     __syncthreads();
     int at = start[blockIdx.x];
+    const int end = start[blockIdx.x + 1];  // from the layer counts: nothing is written past it
     for (int k = 0; k < w; k++) at += wsum[k];
     for (int b = l0; b < l1; b += 64) {
         const int l = b + lane;
         const bool in = l < l1 && lay[l] != NONE;
         const unsigned long long m = __ballot(in);
-        if (in) {
+        const int e = at + __popcll(m & ((1ull << lane) - 1ull));
+        if (in && e < end) {
             int lc[3] = {0, 0, 0};
             leaf_coords<D>(G, (uint32_t)l, lc);
-            const int e = at + __popcll(m & ((1ull << lane) - 1ull));
             item_j[e] = blockIdx.x;
             item_l[e] = rm_of<D>(G, lc);  // items and their scratch are keyed by leaf PosIdx
         }
         at += __popcll(m);
+    }
+    // counts above the layer bytes (never, while the two agree) leave slots that must still
+    // name a voxel of this brick for the passes that read them
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int k = 0; k < nw; k++) total += wsum[k];
+        for (int e = start[blockIdx.x] + total; e < end; e++) {
+            item_j[e] = blockIdx.x;
+            item_l[e] = 0;
+        }
     }
 }
 
@@ -995,7 +1002,8 @@ int felt_polys_march_impl(felt_polys* q) {
         if ((rc = q->vst.ensure(2 * (nb + 1) + 2))) return rc;
         HIPX(hipMemcpyAsync(q->dparts.ptr, todo.data() + b0, nb * sizeof(int),
                             hipMemcpyHostToDevice, st));
-        DISPATCHP(G.D, k_mb_count, dim3(nb), dim3(bt), G, s->P, q->dparts.ptr, q->pstart.ptr);
+        hipLaunchKernelGGL(k_mb_count, dim3((nb + TPB - 1) / TPB), dim3(TPB), 0, st, G, s->P,
+                           q->dparts.ptr, nb, q->pstart.ptr);
         hipLaunchKernelGGL(k_mb_scan, dim3(1), dim3(BT), 0, st, q->pstart.ptr, nb, q->pstart.ptr);
         int M = 0;
         HIPX(hipMemcpyAsync(&M, q->pstart.ptr + nb, sizeof(int), hipMemcpyDeviceToHost, st));
