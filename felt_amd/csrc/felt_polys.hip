@@ -992,7 +992,7 @@ int felt_polys_march_impl(felt_polys* q) {
         }
     }
     const size_t per = (size_t)G.pv * (1 + 2 + 4 + 4);
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(8192, (size_t)(192u << 20) / per));
+    const size_t batch = std::max<size_t>(1, std::min<size_t>(16384, (size_t)(768u << 20) / per));
     const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
     q->last_spx = 0;
     for (size_t b0 = 0; b0 < todo.size(); b0 += batch) {
