@@ -207,31 +207,6 @@ __device__ int edge_owner(const Geo& G, const Pool& P, const MemV& V, int s, con
     return best;
 }
 
-// Felt's N-linear interpolation in world coordinates (NumericMixin.hpp:403-481).
-template <int D>
-__device__ float interp_w(const Geo& G, const Pool& P, const MemV& V, const float w[3]) {
-    int fl[3] = {0, 0, 0};
-    float t[3] = {0.f, 0.f, 0.f};
-    for (int a = 0; a < D; a++) {
-        const float f = floorf(w[a]);
-        fl[a] = (int)f;
-        t[a] = w[a] - f;
-    }
-    float val[8];
-    for (int i = 0; i < (1 << D); i++) {
-        int q[3] = {0, 0, 0};
-        for (int a = 0; a < D; a++) q[a] = fl[a] + ((i >> a) & 1) - V.offset[a];
-        val[i] = get_v<D>(G, P, V, q);
-    }
-    int n = 1 << D;
-    for (int axis = 0; axis < D; axis++) {
-        const float wt = t[axis];
-        for (int i = 0; i < n / 2; i++) val[i] = wt * val[2 * i + 1] + (1.0f - wt) * val[2 * i];
-        n /= 2;
-    }
-    return val[0];
-}
-
 template <int D>
 __device__ bool inside_wf(const MemV& V, const float w[3]) {
     for (int a = 0; a < D; a++) {
@@ -239,31 +214,6 @@ __device__ bool inside_wf(const MemV& V, const float w[3]) {
         if (w[a] < (float)V.offset[a]) return false;
     }
     return true;
-}
-
-// Safe gradient at a real world position (NumericMixin.hpp:188-231), probe stepped -1,+2,-1.
-template <int D>
-__device__ void grad_f(const Geo& G, const Pool& P, const MemV& V, const float w[3],
-                       float out[3]) {
-    float pt[3] = {w[0], w[1], w[2]};
-    const float centre = interp_w<D>(G, P, V, w);
-    for (int a = 0; a < D; a++) {
-        float back = centre, fwd = centre;
-        int order = 0;
-        pt[a] -= 1.0f;
-        if (inside_wf<D>(V, pt)) {
-            back = interp_w<D>(G, P, V, pt);
-            order++;
-        }
-        pt[a] += 2.0f;
-        if (inside_wf<D>(V, pt)) {
-            fwd = interp_w<D>(G, P, V, pt);
-            order++;
-        }
-        pt[a] -= 1.0f;
-        out[a] = order != 0 ? (fwd - back) / (float)order : 0.0f;
-        out[a] = out[a] / 1.0f;
-    }
 }
 
 // Safe gradient at an integer grid position.
@@ -290,13 +240,15 @@ __device__ void grad_i(const Geo& G, const Pool& P, const MemV& V, const int g[3
     }
 }
 
-// grad_f at a vertex interpolated on an edge along axis ax.  Its other coordinates are whole
-// numbers, and so are those of every probe (whole numbers +-1 are exact in fp32), so in the
-// N-linear interpolation at a probe every axis but ax has weight 0 and the value is
-// t*hi + (1-t)*lo between the two grid values along ax, t from the probe's own ax coordinate
-// (stepped -1, +2, -1 in fp32 as the reference steps it).  Only those 2 values per probe are
-// read, 14 in all, in one level-split gather.  The terms the reference multiplies by weight 0
-// are finite, so the result is the same number; only the sign of an exact zero can differ.
+// The reference's safe gradient (NumericMixin.hpp:188-231) of the N-linear interpolation
+// (NumericMixin.hpp:403-481) at a vertex interpolated on an edge along axis ax: the probe is
+// stepped -1, +2, -1 along each axis in fp32 and the value at each probe interpolated.  The
+// vertex's other coordinates are whole numbers, and so are those of every probe (whole numbers
+// +-1 are exact in fp32), so at a probe every axis but ax has weight 0 and the value is
+// t*hi + (1-t)*lo between the two grid values along ax, t from the probe's own ax coordinate.
+// Only those 2 values per probe are read, 14 in all, in one level-split gather.  The terms the
+// reference multiplies by weight 0 are finite, so the result is the same number; only the sign
+// of an exact zero can differ.
 template <int D>
 __device__ void grad_edge(const Geo& G, const Pool& P, const MemV& V, const float w[3], int ax,
                           float out[3]) {
