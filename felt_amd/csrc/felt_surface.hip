@@ -119,6 +119,22 @@ __global__ void k_box_zero(Geo G, Pool P, const int* parts, const int* pbox, int
         const int* bx = boxes + 6 * pbox[j];
         if (threadIdx.x == 0) cnt = 0;
         __syncthreads();
+        if (G.pv <= 64 * (int)blockDim.x) {
+            // one pass: this thread's takes as bits, placed by a block prefix of their counts
+            unsigned long long m = 0;
+            for (int b = 0, it = 0; b < G.pv; b += blockDim.x, it++)
+                if (box_zero_take<D>(G, P, s, b + threadIdx.x, org, bx)) m |= 1ull << it;
+            int at = wave_append_n(&cnt, __popcll(m));
+            __syncthreads();
+            if (threadIdx.x == 0) base = cnt ? atomicAdd(&ctr->z_span[1], cnt) : 0;
+            __syncthreads();
+            at += base;
+            for (int it = 0; m; it++, m >>= 1)
+                if (m & 1ull)
+                    z[at++] = (uint32_t)s * (uint32_t)G.pv + (uint32_t)(it * blockDim.x + threadIdx.x);
+            __syncthreads();
+            continue;
+        }
         for (int b = 0; b < G.pv; b += blockDim.x)
             wave_append(&cnt, box_zero_take<D>(G, P, s, b + threadIdx.x, org, bx));
         __syncthreads();
@@ -504,12 +520,18 @@ __global__ void k_set_span(int* span, int n) {
 }
 
 // Tracked voxels of each listed member (one block per member).
+constexpr int kBandSlices = 16;  // blocks per member in k_member_band
+
+// Tracked voxels of member ids[blockIdx.x]: block (x, y) sums slice y of its layer counts into
+// out[x * kBandSlices + y].
 __global__ void k_member_band(Geo G, Pool P, const int* ids, long long* out) {
     const Mem& M = P.mem[ids[blockIdx.x]];
     const int64_t n = (int64_t)M.np * G.NL;
     const int* c = P.pcnt + (int64_t)M.pbase * G.NL;
     long long s = 0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += c[i];
+    for (int64_t i = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.y * blockDim.x)
+        s += c[i];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
     __shared__ long long part[16];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
@@ -517,7 +539,7 @@ __global__ void k_member_band(Geo G, Pool P, const int* ids, long long* out) {
     if (threadIdx.x == 0) {
         long long t = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
-        out[blockIdx.x] = t;
+        out[blockIdx.x * kBandSlices + blockIdx.y] = t;
     }
 }
 
@@ -1611,12 +1633,16 @@ int felt_member_bands(felt_surface* s, const int* ids, int n, std::vector<long l
     if (n == 0) return FELT_OK;
     int rc;
     if ((rc = s->tmp_i.ensure((size_t)n + 1))) return rc;
-    if ((rc = s->tmp_u.ensure(2 * (size_t)n + 2))) return rc;
+    if ((rc = s->tmp_u.ensure(2 * (size_t)n * kBandSlices + 2))) return rc;
     long long* d = reinterpret_cast<long long*>(s->tmp_u.ptr);
+    std::vector<long long> part((size_t)n * kBandSlices);
     HIPX(hipMemcpyAsync(s->tmp_i.ptr, ids, n * sizeof(int), hipMemcpyHostToDevice, s->st));
-    LAUNCH(k_member_band, dim3(n), dim3(TPB), s->G, s->P, s->tmp_i.ptr, d);
-    HIPX(hipMemcpyAsync(out.data(), d, n * sizeof(long long), hipMemcpyDeviceToHost, s->st));
+    LAUNCH(k_member_band, dim3(n, kBandSlices), dim3(TPB), s->G, s->P, s->tmp_i.ptr, d);
+    HIPX(hipMemcpyAsync(part.data(), d, part.size() * sizeof(long long), hipMemcpyDeviceToHost,
+                        s->st));
     HIPX(hipStreamSynchronize(s->st));
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < kBandSlices; k++) out[i] += part[(size_t)i * kBandSlices + k];
     return FELT_OK;
 }
 
