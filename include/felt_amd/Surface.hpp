@@ -17,7 +17,9 @@
 // Errors raise std::domain_error, as the reference does when FELT_EXCEPTIONS is defined.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <istream>
@@ -64,24 +66,215 @@ VecDf<D> constant_f(float v) {
 }
 }  // namespace detail
 
-// Host view of the isogrid for host functors and read-back.
+// Host view of the isogrid for host functors and read-back, with the reference's grid API for
+// functors (Partitioned.hpp:160-189; NumericMixin.hpp:73-500): get at integer and real
+// positions, interp, the gradients, curvature, divergence, dx, inside, neighs.  While a host
+// functor runs, reads come from one dense copy taken before the step (the functor sees the grid
+// as it was before the update, as in the reference); otherwise each get reads the device.
 template <int D>
 class IsoGridView {
 public:
     explicit IsoGridView(felt_surface* h, const VecDi<D>& size) : m_h(h), m_size(size) {
         for (int a = 0; a < D; a++) m_offset[a] = -1 * size[a] / 2;
+        std::size_t st = 1;
+        for (int a = D - 1; a >= 0; a--) {
+            m_stride[a] = st;
+            st *= std::size_t(size[a]);
+        }
     }
     const VecDi<D>& size() const { return m_size; }
     const VecDi<D>& offset() const { return m_offset; }
+    Distance dx() const { return 1.0f; }
     Distance get(const VecDi<D>& pos) const {
+        if (m_pinned && inside(pos)) {
+            std::size_t i = 0;
+            for (int a = 0; a < D; a++) i += std::size_t(pos[a] - m_offset[a]) * m_stride[a];
+            return m_cache[i];
+        }
         float v = 0.f;
         detail::check(felt_surface_get(m_h, 1, pos.data(), &v));
         return v;
+    }
+    // Real position: N-linear interpolation (NumericMixin.hpp:390-481).
+    Distance get(const VecDf<D>& pos) const { return interp(pos); }
+    Distance interp(const VecDf<D>& pos) const {
+        VecDi<D> fl{};
+        float t[D];
+        for (int a = 0; a < D; a++) {
+            const float f = std::floor(pos[a]);
+            fl[a] = int(f);
+            t[a] = pos[a] - f;
+        }
+        float val[1 << D];
+        for (int i = 0; i < (1 << D); i++) {
+            VecDi<D> q = fl;
+            for (int a = 0; a < D; a++) q[a] += (i >> a) & 1;
+            val[i] = get(q);
+        }
+        int n = 1 << D;
+        for (int a = 0; a < D; a++) {
+            for (int i = 0; i < n / 2; i++) val[i] = t[a] * val[2 * i + 1] + (1.0f - t[a]) * val[2 * i];
+            n /= 2;
+        }
+        return val[0];
     }
     bool inside(const VecDi<D>& pos) const {
         for (int a = 0; a < D; a++)
             if (pos[a] < m_offset[a] || pos[a] >= m_offset[a] + m_size[a]) return false;
         return true;
+    }
+    bool inside(const VecDf<D>& pos) const {  // Util.hpp:94-106
+        for (int a = 0; a < D; a++)
+            if (!(pos[a] < float(m_offset[a] + m_size[a]) && pos[a] >= float(m_offset[a])))
+                return false;
+        return true;
+    }
+    // Face neighbours, -1 then +1 along each axis in turn (NumericMixin.hpp:489-500).
+    template <class Fn>
+    static void neighs(VecDi<D> pos, Fn fn) {
+        for (int a = 0; a < D; a++) {
+            pos[a] -= 1;
+            fn(pos);
+            pos[a] += 2;
+            fn(pos);
+            pos[a] -= 1;
+        }
+    }
+    // Safe gradient: central difference where both neighbours are inside, else one-sided
+    // (NumericMixin.hpp:188-231).
+    template <class V>
+    VecDf<D> grad(const V& pos) const {
+        VecDf<D> g{};
+        V pt = pos;
+        const float centre = get(pos);
+        for (int a = 0; a < D; a++) {
+            float back = centre, fwd = centre;
+            int order = 0;
+            pt[a] -= 1;
+            if (inside(pt)) {
+                back = get(pt);
+                order++;
+            }
+            pt[a] += 2;
+            if (inside(pt)) {
+                fwd = get(pt);
+                order++;
+            }
+            pt[a] -= 1;
+            g[a] = order != 0 ? (fwd - back) / float(order) : 0.0f;
+            g[a] = g[a] / dx();
+        }
+        return g;
+    }
+    // Upwind (entropy satisfying) gradient (NumericMixin.hpp:243-273).
+    template <class V>
+    VecDf<D> gradE(const V& pos) const {
+        VecDf<D> g{};
+        V pt = pos;
+        const float centre = get(pos);
+        for (int a = 0; a < D; a++) {
+            pt[a] -= 1;
+            float back = get(pt);
+            pt[a] += 2;
+            float fwd = get(pt);
+            pt[a] -= 1;
+            back = std::max(centre - back, 0.0f);
+            fwd = std::min(fwd - centre, 0.0f);
+            g[a] = (fwd + back) / dx();
+        }
+        return g;
+    }
+    // Forward, backward and central differences (NumericMixin.hpp:279-358).
+    template <class V>
+    VecDf<D> gradF(const V& pos) const {
+        VecDf<D> g{};
+        V pt = pos;
+        const float centre = get(pos);
+        for (int a = 0; a < D; a++) {
+            pt[a] += 1;
+            g[a] = (get(pt) - centre) / dx();
+            pt[a] -= 1;
+        }
+        return g;
+    }
+    template <class V>
+    VecDf<D> gradB(const V& pos) const {
+        VecDf<D> g{};
+        V pt = pos;
+        const float centre = get(pos);
+        for (int a = 0; a < D; a++) {
+            pt[a] -= 1;
+            g[a] = (centre - get(pt)) / dx();
+            pt[a] += 1;
+        }
+        return g;
+    }
+    template <class V>
+    VecDf<D> gradC(const V& pos) const {
+        VecDf<D> g{};
+        V pt = pos;
+        for (int a = 0; a < D; a++) {
+            pt[a] -= 1;
+            const float back = get(pt);
+            pt[a] += 2;
+            const float fwd = get(pt);
+            pt[a] -= 1;
+            g[a] = ((fwd - back) / 2) / dx();
+        }
+        return g;
+    }
+    // Mean curvature by the difference of normals (NumericMixin.hpp:73-158); vector sums in
+    // Eigen's order, v0 + (v1 + v2).
+    template <class V>
+    Distance curv(const V& pos) const {
+        const float c = get(pos);
+        float nf[D], nb[D];
+        V dir = pos;
+        for (int a = 0; a < D; a++) {
+            dir[a] += 1;
+            const float va = get(dir) - c;
+            float sq = 0;
+            for (int b = 0; b < D; b++) {
+                if (b == a) continue;
+                V dn = dir;
+                dn[b] -= 1;
+                const float lo = get(dn);
+                dn[b] += 2;
+                const float hi = get(dn);
+                const float t = (hi - lo) / 2;
+                sq += t * t;
+            }
+            nf[a] = va / std::sqrt(va * va + sq);
+            dir[a] -= 1;
+        }
+        for (int a = 0; a < D; a++) {
+            dir[a] -= 1;
+            const float va = c - get(dir);
+            float sq = 0;
+            for (int b = 0; b < D; b++) {
+                if (b == a) continue;
+                V dn = dir;
+                dn[b] -= 1;
+                const float lo = get(dn);
+                dn[b] += 2;
+                const float hi = get(dn);
+                const float t = (hi - lo) / 2;
+                sq += t * t;
+            }
+            nb[a] = va / std::sqrt(va * va + sq);
+            dir[a] += 1;
+        }
+        float d[D];
+        for (int a = 0; a < D; a++) d[a] = nf[a] - nb[a];
+        return sum(d) / 2;
+    }
+    // Divergence of the gradient, sum(gradF - gradB) / dx^2 (NumericMixin.hpp:165-176).
+    template <class V>
+    Distance divergence(const V& pos) const {
+        const VecDf<D> f = gradF(pos), b = gradB(pos);
+        float d[D];
+        for (int a = 0; a < D; a++) d[a] = f[a] - b[a];
+        return sum(d) / (dx() * dx());
     }
     // Dense copy of the isogrid, row-major with the last axis fastest (Grid::Snapshot).
     std::vector<Distance> snapshot() const {
@@ -91,10 +284,23 @@ public:
         detail::check(felt_surface_snapshot(m_h, out.data(), out.size()));
         return out;
     }
+    // Reads from one dense copy until unpin(): for host functors during an update.
+    void pin() {
+        m_cache = snapshot();
+        m_pinned = true;
+    }
+    void unpin() {
+        m_pinned = false;
+        std::vector<Distance>().swap(m_cache);
+    }
 
 private:
+    static float sum(const float* v) { return D == 3 ? v[0] + (v[1] + v[2]) : v[0] + v[1]; }
     felt_surface* m_h;
     VecDi<D> m_size, m_offset{};
+    std::array<std::size_t, D> m_stride{};
+    std::vector<Distance> m_cache;
+    bool m_pinned = false;
 };
 
 template <int D, LayerId L>
@@ -154,8 +360,7 @@ public:
     template <class Fn, class = std::enable_if_t<!std::is_same<std::decay_t<Fn>, felt_speed>::value>>
     void update(Fn&& fn) {
         std::vector<int32_t> pos = layer_positions(0);
-        std::vector<float> val(pos.size() / D);
-        for (std::size_t i = 0; i < val.size(); i++) val[i] = call(fn, at(pos, i));
+        std::vector<float> val = evaluate(fn, pos);
         detail::check(felt_surface_update_deltas(m_h.get(), val.size(), pos.data(), val.data()));
     }
     template <class Fn, class = std::enable_if_t<!std::is_same<std::decay_t<Fn>, felt_speed>::value>>
@@ -163,8 +368,7 @@ public:
         std::vector<int32_t> pos = query([&](int32_t* out, std::size_t cap, std::size_t* n) {
             return felt_surface_zero_layer_in_box(m_h.get(), lo.data(), hi.data(), out, cap, n);
         });
-        std::vector<float> val(pos.size() / D);
-        for (std::size_t i = 0; i < val.size(); i++) val[i] = call(fn, at(pos, i));
+        std::vector<float> val = evaluate(fn, pos);
         update_start();
         if (!val.empty())
             detail::check(felt_surface_delta(m_h.get(), val.size(), pos.data(), val.data()));
@@ -254,6 +458,32 @@ private:
         VecDi<D> v{};
         for (int a = 0; a < D; a++) v[a] = pos[i * D + a];
         return v;
+    }
+    // The functor at every position of pos; a functor that reads the grid reads one copy of it.
+    template <class Fn>
+    std::vector<float> evaluate(Fn& fn, const std::vector<int32_t>& pos) {
+        std::vector<float> val(pos.size() / D);
+        constexpr bool reads_grid = takes_grid<Fn>(0);
+        if (reads_grid && !val.empty()) m_iso->pin();
+        try {
+            for (std::size_t i = 0; i < val.size(); i++) val[i] = call(fn, at(pos, i));
+        } catch (...) {
+            m_iso->unpin();
+            throw;
+        }
+        m_iso->unpin();
+        return val;
+    }
+    template <class Fn>
+    static constexpr auto takes_grid(int)
+        -> decltype(std::declval<Fn&>()(std::declval<const VecDi<D>&>(),
+                                        std::declval<const IsoGrid&>()),
+                    bool()) {
+        return true;
+    }
+    template <class Fn>
+    static constexpr bool takes_grid(...) {
+        return false;
     }
     // The reference's three functor shapes (Surface.hpp:1754-1802).
     template <class Fn>

@@ -38,6 +38,41 @@ int main() {
         CHECK(surface.delta(Surface<2, 2>::layer_idx(0)).size() >= 1);
     }
     {
+        // A host functor reading the grid's curvature (NumericMixin.hpp:73-158) steps exactly as
+        // the device functor of the same speed; the other accessors on the city-block ball
+        // |x|+|y|+|z| - 3 that three unit expansions of a seed leave.
+        using S = Surface<3, 2>;
+        S a(VecDi<3>{24, 24, 24}, VecDi<3>{8, 8, 8}), b(VecDi<3>{24, 24, 24}, VecDi<3>{8, 8, 8});
+        a.seed(VecDi<3>{0, 0, 0});
+        b.seed(VecDi<3>{0, 0, 0});
+        for (int i = 0; i < 3; i++) {
+            a.update(speed_constant(-1.0f));
+            b.update(speed_constant(-1.0f));
+        }
+        const S::IsoGrid& g = a.isogrid();
+        const VecDi<3> p2{2, 0, 0};
+        CHECK(g.get(p2) == -1.0f && g.get(VecDi<3>{1, 0, 0}) == -2.0f);
+        const VecDf<3> gc = g.gradC(p2), gf = g.gradF(p2), gb = g.gradB(p2), gs = g.grad(p2);
+        CHECK(gc[0] == 1.0f && gc[1] == 0.0f && gc[2] == 0.0f);
+        CHECK(gf[0] == 1.0f && gf[1] == 1.0f && gb[0] == 1.0f && gb[1] == -1.0f);
+        CHECK(gs[0] == gc[0] && gs[1] == gc[1] && gs[2] == gc[2]);
+        CHECK(g.divergence(p2) == 4.0f);
+        const VecDf<3> ge = g.gradE(p2);  // upwind: max(c - back, 0) + min(fwd - c, 0)
+        CHECK(ge[0] == 1.0f && ge[1] == 0.0f);
+        CHECK(g.interp(VecDf<3>{2.0f, 0.0f, 0.0f}) == -1.0f);
+        CHECK(std::fabs(g.interp(VecDf<3>{1.5f, 0.0f, 0.0f}) + 1.5f) < 1e-6f);
+        CHECK(std::fabs(g.get(VecDf<3>{1.5f, 0.25f, 0.0f}) + 1.25f) < 1e-6f);
+        CHECK(g.inside(VecDi<3>{-12, 11, 0}) && !g.inside(VecDi<3>{12, 0, 0}));
+        std::vector<VecDi<3>> nb;
+        S::IsoGrid::neighs(p2, [&](const VecDi<3>& q) { nb.push_back(q); });
+        CHECK(nb.size() == 6 && nb[0] == (VecDi<3>{1, 0, 0}) && nb[1] == (VecDi<3>{3, 0, 0}) &&
+              nb[4] == (VecDi<3>{2, 0, -1}));
+        a.update([](const VecDi<3>& p, const S::IsoGrid& grid) { return 0.2f * grid.curv(p); });
+        b.update(speed_curvature(0.2f));
+        CHECK(a.isogrid().snapshot() == b.isogrid().snapshot());
+        CHECK(a.isogrid().get(p2) == b.isogrid().get(p2));  // reads the device again
+    }
+    {
         // 3D layer counts 1/6/18/38/66 (test_Surface.cpp:1031-1053), device functor
         Surface<3, 3> surface(VecDi<3>{11, 11, 11}, VecDi<3>{3, 3, 3});
         surface.seed(VecDi<3>{0, 0, 0});
