@@ -6,7 +6,8 @@
 //   update(fn), update(lo, hi, fn)          Surface.hpp:254-386
 //   update_start(), delta(pos, v), update_end()   Surface.hpp:393-459
 //   isogrid().snapshot(), isogrid().get(pos)      PartitionedMixin.hpp:594-660
-//   delta(k), status_change(k), layer_idx(id), stats(), is_intersected(p)  Surface.hpp:623-696
+//   delta(k), status_change(k), affected(), layer_idx(id), stats(), is_intersected(p)
+//                                            Surface.hpp:623-696
 //   ray(origin, dir), ray_miss                 Surface.hpp:468-616, 1805-1807
 //   save(ostream), load(istream)               Surface.hpp:180-196 (the same cereal archive)
 //   Polys(surface), notify(), march(), invalidate(), changes(), children().get(p)
@@ -387,6 +388,18 @@ public:
         return parts(FELT_LIST_STATUS_CHANGE, layer_idx_);
     }
     static constexpr TupleIdx layer_idx(LayerId id) { return TupleIdx(id + L); }
+    // affected() (Surface.hpp:693): the affected set of the last bounded update, per layer index
+    // as the partitions of its lookup list (children().lookup().list(k)) and as positions.
+    struct Affected {
+        const Surface* s;
+        PosIdxList list(TupleIdx k) const { return s->parts(FELT_LIST_AFFECTED, k); }
+        std::vector<int32_t> leafs(TupleIdx k) const {
+            return query([&](int32_t* out, std::size_t cap, std::size_t* n) {
+                return felt_surface_affected_positions(s->m_h.get(), k, out, cap, n);
+            });
+        }
+    };
+    Affected affected() const { return Affected{this}; }
     // Cast a ray to the zero layer; ray_miss when nothing is hit.  dir is a unit vector.
     VecDf<D> ray(const VecDf<D>& origin, const VecDf<D>& dir) const {
         VecDf<D> hit{};

@@ -36,6 +36,13 @@ int main() {
         CHECK(snap.size() == 81 && snap[4 * 9 + 4] == -1.0f);
         surface.update(VecDi<2>{1, 0}, VecDi<2>{3, 3}, []() { return 0.0f; });
         CHECK(surface.delta(Surface<2, 2>::layer_idx(0)).size() >= 1);
+        // the affected set holds the box's zero-layer points and reaches every layer
+        const auto aff = surface.affected();
+        CHECK(!aff.list(Surface<2, 2>::layer_idx(0)).empty());
+        const std::size_t n0 = aff.leafs(Surface<2, 2>::layer_idx(0)).size() / 2;
+        std::size_t nall = 0;
+        for (int k = 0; k < 5; k++) nall += aff.leafs(k).size() / 2;
+        CHECK(n0 >= 1 && nall > n0);
     }
     {
         // A host functor reading the grid's curvature (NumericMixin.hpp:73-158) steps exactly as
