@@ -589,6 +589,12 @@ public:
             for (int a = 0; a < D; a++) c.m_spx[i].idxs[a] = t[i * D + a];
         return c;
     }
+    // children() (Polys.hpp:56): get(pos_idx) returns the child as child(pos_idx) does.
+    struct Children {
+        const Polys* p;
+        Child get(PosIdx pos_idx) const { return p->child(pos_idx); }
+    };
+    Children children() const { return Children{this}; }
 
 private:
     struct Del {

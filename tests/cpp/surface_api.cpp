@@ -109,7 +109,7 @@ int main() {
         polys.notify();
         polys.march();
         const PosIdx centre = (1 * 3 + 1) * 3 + 1;
-        const auto child = polys.child(centre);
+        const auto child = polys.children().get(centre);
         CHECK(child.vtxs().size() == 30 && child.spxs().size() == 56);
         CHECK(polys.changes().size() == 7);
         for (const auto& v : child.vtxs()) {
