@@ -429,8 +429,9 @@ def main():
     ap.add_argument("--plain-ticks", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs its surfaces on a single MI355X and "
-              "has no multi-GPU path (DESIGN.md describes how it would shard).", file=sys.stderr)
+        print("bench.py measures one GPU: the surfaces run on a single MI355X and there is no "
+              "multi-GPU path; independent surfaces would shard by surface id, one pool per GPU.",
+              file=sys.stderr)
         sys.exit(2)
     import felt_amd
     felt_amd.lib()
