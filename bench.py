@@ -401,6 +401,45 @@ def cpu_baseline(specs, budget_s):
                       "work), CPU restatement with OpenMP over partitions"}
 
 
+def run_traffic_probe(args):
+    """For rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs: a device-to-device copy of 256 MiB (the
+    counters' scale), then the mixed workload's 16^3 batch stepped `warmup` times and its six
+    first-round distance passes replayed `pass_reps` times."""
+    import ctypes
+    import felt_amd
+    from felt_amd import workloads
+    from felt_amd.batch import Batch
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = 256 << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) or \
+            hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3):  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log("[bench] traffic probe: 3 copies of 256 MiB")
+    specs = mixed_workload(args.surfaces)
+    ids = [i for i, sp in enumerate(specs) if sp.part[0] == 16]
+    bt = Batch(felt_amd.lib().lib, 3, 3, [specs[i].size for i in ids], (16,) * 3)
+    for j, i in enumerate(ids):
+        bt.load_spheres(j, specs[i].centres, specs[i].radii)
+    speeds = [workloads.speed_of(specs[i].speed) for i in ids]
+    for _ in range(args.warmup):
+        bt.update(speeds)
+    bt.sync()
+    us, items, n_launch = bt.time_pass(1, args.pass_reps)
+    bt.close()
+    res = {"surfaces": len(ids), "distance_pass_us": us, "entries_per_launch": items,
+           "launches_per_copy": n_launch, "copies": args.pass_reps}
+    log(f"[bench] traffic probe: distance pass {us:.1f} us per launch, {items:.0f} entries")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,6 +459,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-only", action="store_true")
     ap.add_argument("--single-only", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="copy of known size + the 16^3 batch's distance passes, for PMC runs")
     ap.add_argument("--served-surfaces", type=int, default=1000)
     ap.add_argument("--served-window", type=float, default=2.0)
     ap.add_argument("--rate", type=float, default=2000.0)
@@ -440,6 +481,9 @@ def main():
         return
     if args.single_only:
         print(json.dumps({"single_surface": run_single(args)}), flush=True)
+        return
+    if args.traffic_probe:
+        print(json.dumps({"traffic_probe": run_traffic_probe(args)}), flush=True)
         return
     specs, batched, plain, ktab, roof, cfg = run_mixed(args)
     log(f"[bench] batched {batched['value'] / 1e9:.2f} G/s" +
