@@ -197,6 +197,21 @@ def run_single(args):
     total = time.perf_counter() - t_all
     ktab, roof = time_passes(s, args.pass_reps)
     ray = ray_bench(s, size, args.rays) if args.rays > 0 else None
+    # checkpoint / resume in the reference's archive format: save the final state, load it into a
+    # new surface (host formatting and device gather / scatter included)
+    t0 = time.perf_counter()
+    data = s.save()
+    t_save = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s2 = felt_amd.Surface.load(felt_amd.lib(), data)
+    s2.sync()
+    t_load = time.perf_counter() - t0
+    same = s2.layer_size(0) == s.layer_size(0)
+    s2.close()
+    ckpt = {"archive_bytes": len(data), "save_s": t_save, "load_s": t_load,
+            "save_MB_per_s": len(data) / t_save / 1e6, "load_MB_per_s": len(data) / t_load / 1e6,
+            "zero_layer_matches": bool(same)}
+    del data
     res = {"workload": f"single large surface, plain GPU path: Surface<3,3> {size}^3 isogrid in "
                        f"{part}^3 partitions, harmonic-perturbed sphere r={150.0 * size / 512:g}, "
                        "speed clamp(-0.3(1+0.5n)+0.1curv), incremental marching cubes every 10 "
@@ -206,7 +221,7 @@ def run_single(args):
            "p50_step_ms": pctl(lat, 50), "p99_step_ms": pctl(lat, 99),
            "triangles_per_s": spx / march_s if march_s > 0 else None,
            "band_voxels_final": int(s.stats().band_voxels),
-           "roofline": roof, "kernels": ktab, "ray": ray}
+           "roofline": roof, "kernels": ktab, "ray": ray, "checkpoint": ckpt}
     polys.close()
     s.close()
     log(f"[bench] single: {res['value'] / 1e9:.2f} G voxel updates/s")

@@ -340,7 +340,7 @@ class Surface:
         self.b.call("surface_save", self.h, None, 0, C.byref(n))
         buf = (C.c_uint8 * max(n.value, 1))()
         self.b.call("surface_save", self.h, buf, n.value, C.byref(n))
-        return bytes(buf[: n.value])
+        return C.string_at(C.addressof(buf), n.value)
 
     @classmethod
     def load(cls, binding: "Binding", data: bytes) -> "Surface":

@@ -208,7 +208,7 @@ class Batch:
         self.api("felt_batch_save", self.h, int(i), None, 0, C.byref(n))
         buf = (C.c_uint8 * max(n.value, 1))()
         self.api("felt_batch_save", self.h, int(i), buf, n.value, C.byref(n))
-        return bytes(buf[: n.value])
+        return C.string_at(C.addressof(buf), n.value)
 
     def load(self, i: int, data: bytes) -> None:
         """Replace member i's state with an archive of the same geometry."""
