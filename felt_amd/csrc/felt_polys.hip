@@ -457,7 +457,7 @@ __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, 
 }
 
 // Cube mask, owned edges and vertex / simplex counts of each band voxel; the voxels with
-// anything to emit are listed in `work` so that the emit pass runs on them alone.
+// simplices are listed in `work` so that the simplex pass runs on them alone.
 template <int D>
 __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
                           int M, uint8_t* cmask, uint16_t* own, uint32_t* ocand, int* nv,
@@ -581,7 +581,7 @@ __global__ void k_s2_apply(const int* a, const int* b, int n, const int* bs, int
         }
 }
 
-// Vertex base of each band voxel, by leaf, for the owner lookups of the emit pass.
+// Vertex base of each band voxel, by leaf, for the owner lookups of the simplex pass.
 __global__ void k_m_leafbase(const int* item_j, const int* item_l, int M, int pv, const int* vb,
                              int* vleaf) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
