@@ -77,6 +77,11 @@ int main() {
         a.update([](const VecDi<3>& p, const S::IsoGrid& grid) { return 0.2f * grid.curv(p); });
         b.update(speed_curvature(0.2f));
         CHECK(a.isogrid().snapshot() == b.isogrid().snapshot());
+        // the same through the bounded form
+        const VecDi<3> lo{-4, -4, -2}, hi{2, 3, 4};
+        a.update(lo, hi, [](const VecDi<3>& p, const S::IsoGrid& grid) { return 0.3f * grid.curv(p); });
+        b.update(lo, hi, speed_curvature(0.3f));
+        CHECK(a.isogrid().snapshot() == b.isogrid().snapshot());
         CHECK(a.isogrid().get(p2) == b.isogrid().get(p2));  // reads the device again
     }
     {
