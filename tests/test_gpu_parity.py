@@ -185,12 +185,13 @@ def test_polys_2d_parity():
 
 
 @pytest.mark.parametrize("D,size,part", [(3, (96, 96, 64), (32, 32, 32)),
+                                         (3, (96, 96, 64), (16, 16, 32)),
                                          (2, (384, 256), (128, 128))],
-                         ids=["3d-32^3", "2d-128^2"])
+                         ids=["3d-32^3", "3d-16x16x32", "2d-128^2"])
 def test_large_partitions(D, size, part):
-    """Bricks large enough that several wavefronts share one in the band compaction (32^3 and
-    128^2 partitions): global and bounded steps compared after every other step, then a full
-    march compared as meshes."""
+    """Bricks large enough that several wavefronts share one in the band compaction (8, 2 and 4
+    per brick): global and bounded steps compared after every other step, then a full march
+    compared as meshes."""
     import felt_amd
     L = 3 if D == 3 else 2
     g, c = pair(D, L, size, part)
