@@ -35,6 +35,19 @@ PASS_SPEED = 0
 PASS_DIST1 = 1
 PASS_BAND = 2
 
+# Grid operators of felt_surface_stencil (NumericMixin.hpp:72-500)
+STENCIL_GET = 0
+STENCIL_CURV = 1
+STENCIL_DIVERGENCE = 2
+STENCIL_GRAD = 3
+STENCIL_GRAD_E = 4
+STENCIL_GRAD_F = 5
+STENCIL_GRAD_B = 6
+STENCIL_GRAD_C = 7
+STENCIL_NAMES = {"get": STENCIL_GET, "curv": STENCIL_CURV, "divergence": STENCIL_DIVERGENCE,
+                 "grad": STENCIL_GRAD, "gradE": STENCIL_GRAD_E, "gradF": STENCIL_GRAD_F,
+                 "gradB": STENCIL_GRAD_B, "gradC": STENCIL_GRAD_C}
+
 
 class FeltError(RuntimeError):
     """Raised for a non-zero status; the reference throws std::domain_error there."""
@@ -138,6 +151,9 @@ class Binding:
             "surface_load": (C.c_int, [C.POINTER(C.c_uint8), _SZ, C.POINTER(_P)]),
             "surface_time_ray": (C.c_int, [_P, _SZ, _F32P, _F32P, C.c_int,
                                            C.POINTER(C.c_double)]),
+            "surface_stencil": (C.c_int, [_P, C.c_int, C.c_float, _SZ, _I32P, _F32P]),
+            "surface_stencil_real": (C.c_int, [_P, C.c_int, C.c_float, _SZ, _F32P, _F32P]),
+            "surface_speed_eval": (C.c_int, [_P, C.POINTER(felt_speed), _SZ, _I32P, _F32P]),
             "polys_notify": (C.c_int, [_P]),
             "polys_march": (C.c_int, [_P]),
             "polys_invalidate": (C.c_int, [_P]),
@@ -375,6 +391,32 @@ class Surface:
         self.b.call("surface_time_ray", self.h, len(o), _ptr(o, _F32P), _ptr(d, _F32P),
                     int(reps), C.byref(us))
         return float(us.value)
+
+    def stencil(self, op, pos, dx: float = 1.0) -> np.ndarray:
+        """A grid operator (name or STENCIL_*) at integer positions: (n,) for get, curv and
+        divergence, (n, D) for the gradients (felt_surface_stencil)."""
+        return self._stencil("surface_stencil", op, np.int32, _I32P, pos, dx)
+
+    def stencil_real(self, op, pos, dx: float = 1.0) -> np.ndarray:
+        """The same at real positions, every sample interpolated (felt_surface_stencil_real)."""
+        return self._stencil("surface_stencil_real", op, np.float32, _F32P, pos, dx)
+
+    def _stencil(self, name, op, dtype, ptype, pos, dx):
+        op = STENCIL_NAMES.get(op, op)
+        p = np.ascontiguousarray(np.asarray(pos, dtype=dtype).reshape(-1, self.D))
+        width = self.D if op >= STENCIL_GRAD else 1
+        out = np.empty((len(p), width), dtype=np.float32)
+        self.b.call(name, self.h, int(op), float(dx), len(p), _ptr(p, ptype), _ptr(out, _F32P))
+        return out if width > 1 else out.reshape(-1)
+
+    def speed_eval(self, fn: felt_speed, pos) -> np.ndarray:
+        """fn(pos) of a device functor at positions of active partitions, computed by the
+        update kernel's own speed code (felt_surface_speed_eval)."""
+        p = np.ascontiguousarray(np.asarray(pos, dtype=np.int32).reshape(-1, self.D))
+        out = np.empty(len(p), dtype=np.float32)
+        self.b.call("surface_speed_eval", self.h, C.byref(fn), len(p), _ptr(p, _I32P),
+                    _ptr(out, _F32P))
+        return out
 
     def layer_size(self, layer: int) -> int:
         n = C.c_uint64()

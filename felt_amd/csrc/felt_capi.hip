@@ -20,10 +20,16 @@
 using namespace felt;
 
 #define API extern "C" __attribute__((visibility("default")))
-#define CHECK_S(s) \
-    if (!(s)) return felt_fail(FELT_E_ARG, "null surface handle")
-#define CHECK_P(p) \
-    if (!(p) || !(p)->s) return felt_fail(FELT_E_ARG, "null polys handle")
+// A surface accepts calls unless a device-functor step is open on it (felt_surface_update_begin
+// until commit or cancel); polys also refuse while an asynchronous batch step is in flight, whose
+// later rounds would change partitions after the march.
+#define CHECK_S(s)                                                                             \
+    if (!(s)) return felt_fail(FELT_E_ARG, "null surface handle");                            \
+    if ((s)->job) return felt_fail(FELT_E_STATE, "a device-functor step is open: commit or cancel it")
+#define CHECK_P(p)                                                                             \
+    if (!(p) || !(p)->s) return felt_fail(FELT_E_ARG, "null polys handle");                  \
+    if ((p)->s->job || (p)->s->step_pending)                                                  \
+        return felt_fail(FELT_E_STATE, "the surface has a step in flight")
 
 namespace {
 
@@ -400,6 +406,77 @@ API int felt_surface_get(felt_surface* s, size_t n, const int32_t* pos, float* o
             for (int a = 0; a < 3; a++) g[i * 3 + a] = -1;
         }
     return felt_get_impl(s, 0, n, g.data(), out);
+}
+
+// Positions the stencil kernels accept: integers within 2^30 and finite reals within 2^24 of the
+// origin, so that no neighbour or interpolation corner overflows.
+API int felt_surface_stencil(felt_surface* s, int op, float dx, size_t n, const int32_t* pos,
+                             float* out) {
+    CHECK_S(s);
+    if (n && (!pos || !out)) return felt_fail(FELT_E_ARG, "null position or output array");
+    for (size_t i = 0; i < n * (size_t)s->G.D; i++)
+        if (pos[i] > (1 << 30) || pos[i] < -(1 << 30))
+            return felt_fail(FELT_E_ARG, "stencil position beyond 2^30");
+    return felt_stencil_impl(s, 0, op, dx, n, pos, nullptr, out);
+}
+
+API int felt_surface_stencil_real(felt_surface* s, int op, float dx, size_t n, const float* pos,
+                                  float* out) {
+    CHECK_S(s);
+    if (n && (!pos || !out)) return felt_fail(FELT_E_ARG, "null position or output array");
+    for (size_t i = 0; i < n * (size_t)s->G.D; i++)
+        if (!std::isfinite(pos[i]) || std::fabs(pos[i]) > 16777216.0f)
+            return felt_fail(FELT_E_ARG, "stencil position not finite or beyond 2^24");
+    return felt_stencil_impl(s, 0, op, dx, n, nullptr, pos, out);
+}
+
+API int felt_surface_speed_eval(felt_surface* s, const felt_speed* fn, size_t n,
+                                const int32_t* pos, float* out) {
+    CHECK_S(s);
+    if (int rc = check_speed(fn)) return rc;
+    if (n && (!pos || !out)) return felt_fail(FELT_E_ARG, "null position or output array");
+    std::vector<int> g(n * 3);
+    for (size_t i = 0; i < n; i++)
+        if (!to_grid(s->mems[0], s->G.D, pos + i * s->G.D, g.data() + i * 3))
+            return felt_fail(FELT_E_ARG, "speed_eval position outside the grid");
+    return felt_speed_eval_impl(s, 0, fn, n, g.data(), out);
+}
+
+API int felt_surface_device_grid(felt_surface* s, felt_device_grid* out) {
+    CHECK_S(s);
+    if (!out) return felt_fail(FELT_E_ARG, "null output");
+    felt_fill_device_grid(s, 0, out);
+    return FELT_OK;
+}
+
+API int felt_surface_update_begin(felt_surface* s, felt_zero_job* job) {
+    CHECK_S(s);
+    if (!job) return felt_fail(FELT_E_ARG, "null job");
+    if (s->in_update) return felt_fail(FELT_E_STATE, "update(fn) inside update_start / update_end");
+    return felt_update_begin_impl(s, job);
+}
+
+API int felt_surface_update_box_begin(felt_surface* s, const int32_t* lo, const int32_t* hi,
+                                      felt_zero_job* job) {
+    CHECK_S(s);
+    if (!lo || !hi || !job) return felt_fail(FELT_E_ARG, "bad arguments");
+    if (s->in_update) return felt_fail(FELT_E_STATE, "update(fn) inside update_start / update_end");
+    int l3[3] = {0, 0, 0}, h3[3] = {0, 0, 0};
+    for (int a = 0; a < s->G.D; a++) {
+        l3[a] = lo[a];
+        h3[a] = hi[a];
+    }
+    return felt_update_box_begin_impl(s, l3, h3, job);
+}
+
+API int felt_surface_update_commit(felt_surface* s) {
+    if (!s) return felt_fail(FELT_E_ARG, "null surface handle");
+    return felt_update_commit_impl(s);
+}
+
+API int felt_surface_update_cancel(felt_surface* s) {
+    if (!s) return felt_fail(FELT_E_ARG, "null surface handle");
+    return felt_update_cancel_impl(s);
 }
 
 API int felt_surface_ray(felt_surface* s, size_t n, const float* origins, const float* dirs,

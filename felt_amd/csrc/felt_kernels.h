@@ -21,12 +21,14 @@ struct Ctr {
     int act_n;           // active partitions
     int z_span[2];       // {0, zero-layer delta entries} of a bounded update
     int band_span[2];    // {0, band entries}
-    int err;             // 1: brick pool exhausted
+    int err;             // 1: brick pool exhausted; 2: layer counts and lists disagree;
+                         // 3: step refused, its worst-case activations exceed the free bricks
     int misc;
     int one;             // constant 1: the live flag of a round that always runs
     int live[8];         // round r of an unrolled step runs while live[r] != 0
     int band_total;      // tracked voxels, refreshed at the end of an unrolled step
     int front[20];       // breadth-first frontier bounds of calc_affected
+    int reserve;         // bricks the step may activate at most (k_reserve_count)
 };
 
 // Status-change record: the reference's StatusChangeGrid entry (Surface.hpp:1138-1178).
@@ -114,6 +116,28 @@ __device__ __forceinline__ float speed_from(const Speed& sp, float curv, const i
         return r;
     }
     }
+}
+
+// The speed functor of voxel v's member (sp[member]) at v: the curvature stencil is gathered in
+// levels (gather) and reduced in the reference's order (curvature_of).  The update kernel and the
+// speed query both call this.
+template <int D>
+__device__ __forceinline__ float speed_voxel(const Geo& G, const Pool& P, uint32_t v,
+                                             const Speed* sp) {
+    VoxCtx c;
+    vox_ctx<D>(G, P, v, c);
+    const Speed s = sp[c.m];
+    float curv = 0.0f;
+    if (s.kind != FELT_SPEED_CONSTANT) {
+        float val[Stencil<D>::K];
+        if (D == 3) gather<D, Stencil<D>::K>(G, P, c, kStencil3, val);
+        else gather<D, Stencil<D>::K>(G, P, c, kStencil2, val);
+        curv = curvature_of<D>(P.phi[c.v], val);
+    }
+    const Mem& M = P.mem[c.m];
+    int w[3] = {0, 0, 0};
+    for (int a = 0; a < D; a++) w[a] = c.g[a] + M.offset[a];
+    return speed_from<D>(s, curv, w);
 }
 
 }  // namespace felt

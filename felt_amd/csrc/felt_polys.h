@@ -50,5 +50,7 @@ int felt_polys_invalidate_impl(felt_polys* q);
 int felt_polys_march_impl(felt_polys* q);
 int felt_polys_child_mesh_impl(felt_polys* q, uint64_t p, float* vtx, size_t vcap, uint32_t* spx,
                                size_t scap);
+// Device bytes the polygoniser holds: mesh arenas and march scratch.
+uint64_t felt_polys_device_bytes(const felt_polys* q);
 void felt_mc_table_host(uint64_t* out);
 void felt_ms_table_host(uint32_t* out);

@@ -1052,3 +1052,14 @@ int felt_polys_child_mesh_impl(felt_polys* q, uint64_t p, float* vtx, size_t vca
     HIPX(hipStreamSynchronize(q->s->st));
     return FELT_OK;
 }
+
+uint64_t felt_polys_device_bytes(const felt_polys* q) {
+    uint64_t b = q->varena.cap * sizeof(float) + q->tarena.cap * sizeof(uint32_t);
+    b += q->cmask.cap + q->own.cap * sizeof(uint16_t) + q->ocand.cap * sizeof(uint32_t);
+    b += q->bits.cap + q->dseg.cap * sizeof(long long);
+    for (const DBuf<int>* d : {&q->vbase, &q->tbase, &q->tot, &q->dparts, &q->pstart, &q->vst,
+                               &q->item_j, &q->item_l, &q->nv, &q->nt, &q->vleaf, &q->work,
+                               &q->vwork, &q->plist})
+        b += d->cap * sizeof(int);
+    return b;
+}

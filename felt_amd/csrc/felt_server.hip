@@ -16,6 +16,7 @@
 #include <cstring>
 #include <deque>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "../../include/felt_amd.h"
@@ -79,6 +80,8 @@ struct felt_server {
     std::vector<char> busy;            // a request of this surface is in flight
     std::vector<long long> band;       // tracked voxels per surface, refreshed after its steps
     std::vector<uint64_t> done;        // completed request ids not yet polled
+    std::vector<uint64_t> failed;      // failed request ids not yet polled
+    uint64_t n_failed = 0;
     std::vector<double> tick_ms, req_ms;
     double target_ms = 0.0;            // tick-time target of admission (0: none)
     TickModel model;
@@ -104,10 +107,21 @@ uint64_t list_bytes(const felt_server* v, long long extra) {
     return (uint64_t)felt_list_cap_for(v->s, (size_t)sum) * kListBytes;
 }
 
-// Bytes of everything but the step lists.
+// Bytes of everything but the step lists: bricks, tables, scratch and the polygoniser's mesh
+// arenas and march scratch.
 uint64_t resident_bytes(const felt_server* v) {
     const felt_surface* s = v->s;
-    return felt_device_bytes(s) - (uint64_t)s->list_cap * kListBytes;
+    uint64_t b = felt_device_bytes(s) - (uint64_t)s->list_cap * kListBytes;
+    if (v->polys) b += felt_polys_device_bytes(v->polys);
+    return b;
+}
+
+// A request leaves the server without completing: its surface is free again and its id is
+// reported by felt_server_poll_failed.
+void fail_request(felt_server* v, const Req& q) {
+    v->busy[q.r.surface] = 0;
+    v->failed.push_back(q.r.id);
+    v->n_failed++;
 }
 
 double work_of(const felt_server* v, const Req& q) { return (double)v->band[q.r.surface] * q.frac; }
@@ -128,7 +142,13 @@ void admit(felt_server* v) {
             ++it;
             continue;
         }
-        if (!v->active.empty() && fixed + list_bytes(v, v->band[m]) > v->budget) {
+        if (fixed + list_bytes(v, v->band[m]) > v->budget) {
+            if (v->active.empty()) {
+                // it would exceed the budget even alone: it can never run
+                fail_request(v, *it);
+                it = v->queue.erase(it);
+                continue;
+            }
             v->deferred++;
             break;  // keep arrival order: later requests wait too
         }
@@ -222,17 +242,22 @@ API int felt_server_set_target(felt_server* v, double tick_ms) {
     return FELT_OK;
 }
 
+// Every request is checked before any is queued: a call queues all n or none.
 API int felt_server_submit(felt_server* v, const felt_request* r, size_t n) {
     if (!v || (n && !r)) return felt_fail(FELT_E_ARG, "null argument");
+    for (size_t i = 0; i < n; i++) {
+        const felt_request& q = r[i];
+        const std::string at = "request " + std::to_string(i) + ": ";
+        if (q.surface < 0 || q.surface >= v->s->G.nmem)
+            return felt_fail(FELT_E_ARG, at + "surface out of range");
+        if (q.steps < 1) return felt_fail(FELT_E_ARG, at + "needs at least one step");
+        if (q.speed.kind < 0 || q.speed.kind > FELT_SPEED_NOISE_CURV ||
+            (q.speed.kind == FELT_SPEED_NOISE_CURV && q.speed.cell <= 0))
+            return felt_fail(FELT_E_ARG, at + "bad speed descriptor");
+    }
     const double t = now_s();
     for (size_t i = 0; i < n; i++) {
         const felt_request& q = r[i];
-        if (q.surface < 0 || q.surface >= v->s->G.nmem)
-            return felt_fail(FELT_E_ARG, "request surface out of range");
-        if (q.steps < 1) return felt_fail(FELT_E_ARG, "request needs at least one step");
-        if (q.speed.kind < 0 || q.speed.kind > FELT_SPEED_NOISE_CURV ||
-            (q.speed.kind == FELT_SPEED_NOISE_CURV && q.speed.cell <= 0))
-            return felt_fail(FELT_E_ARG, "bad speed descriptor");
         const Mem& M = v->s->mems[q.surface];
         double frac = 1.0;
         for (int a = 0; a < v->s->G.D; a++) {
@@ -274,16 +299,39 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
         v->phase_s[k] += t - tp;
         tp = t;
     };
+    // A step the brick pool cannot cover is refused with every surface unchanged: the requests
+    // are then stepped one at a time, and only those that cannot run even alone fail.
+    std::vector<char> ran(n, 1);
     int rc = felt_update_boxes_impl(v->s, n, ids.data(), lo.data(), hi.data(), sp.data(),
                                     bands.data());
+    if (rc == FELT_E_NOMEM) {
+        rc = FELT_OK;
+        for (int j = 0; j < n && !rc; j++) {
+            rc = felt_update_boxes_impl(v->s, 1, &ids[j], &lo[(size_t)j * 3], &hi[(size_t)j * 3],
+                                        &sp[j], &bands[j]);
+            if (rc == FELT_E_NOMEM) {
+                ran[j] = 0;
+                rc = FELT_OK;
+            }
+        }
+    }
     phase(0);
     if (!rc && v->polys) rc = felt_polys_notify_impl(v->polys);
     phase(1);
     if (!rc && v->polys) rc = felt_polys_march_impl(v->polys);
     phase(2);
-    if (!rc) rc = refresh_bands(v, ids);
+    std::vector<int> stepped_ids;
+    for (int j = 0; j < n; j++)
+        if (ran[j]) stepped_ids.push_back(ids[j]);
+    if (!rc) rc = refresh_bands(v, stepped_ids);
     phase(3);
-    if (rc) return rc;
+    if (rc) {
+        // the surfaces of the in-flight requests may be partly stepped: none of them continues
+        const std::string msg = g_felt_err;
+        for (const Req& q : v->active) fail_request(v, q);
+        v->active.clear();
+        return felt_fail(rc, msg);
+    }
     const double t1 = now_s();
     v->tick_ms.push_back((t1 - t0) * 1e3);
     v->model.observe((t1 - t0) * 1e3, work);
@@ -292,8 +340,11 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
     v->voxel_updates += v->s->last_updates;
     std::vector<Req> keep;
     keep.reserve(v->active.size());
-    for (Req& q : v->active) {
-        if (--q.left > 0) {
+    for (int j = 0; j < n; j++) {
+        Req& q = v->active[j];
+        if (!ran[j]) {
+            fail_request(v, q);
+        } else if (--q.left > 0) {
             keep.push_back(q);
         } else {
             v->busy[q.r.surface] = 0;
@@ -306,13 +357,43 @@ API int felt_server_tick(felt_server* v, int32_t* stepped) {
     return FELT_OK;
 }
 
-API int felt_server_poll(felt_server* v, uint64_t* ids, size_t cap, size_t* n) {
-    if (!v || !n) return felt_fail(FELT_E_ARG, "null argument");
-    const size_t k = std::min(cap, v->done.size());
-    if (k) std::copy(v->done.begin(), v->done.begin() + (long)k, ids);
-    v->done.erase(v->done.begin(), v->done.begin() + (long)k);
+namespace {
+int drain(std::vector<uint64_t>& from, uint64_t* ids, size_t cap, size_t* n) {
+    if (!n || (cap && !ids)) return felt_fail(FELT_E_ARG, "null argument");
+    const size_t k = std::min(cap, from.size());
+    if (k) std::copy(from.begin(), from.begin() + (long)k, ids);
+    from.erase(from.begin(), from.begin() + (long)k);
     *n = k;
     return FELT_OK;
+}
+}  // namespace
+
+API int felt_server_poll(felt_server* v, uint64_t* ids, size_t cap, size_t* n) {
+    if (!v) return felt_fail(FELT_E_ARG, "null server");
+    return drain(v->done, ids, cap, n);
+}
+
+API int felt_server_poll_failed(felt_server* v, uint64_t* ids, size_t cap, size_t* n) {
+    if (!v) return felt_fail(FELT_E_ARG, "null server");
+    return drain(v->failed, ids, cap, n);
+}
+
+// Withdraws a queued or in-flight request (the first with this id); an in-flight one stops
+// after the steps it has run.  It is reported neither as done nor as failed.
+API int felt_server_cancel(felt_server* v, uint64_t id) {
+    if (!v) return felt_fail(FELT_E_ARG, "null server");
+    for (auto it = v->queue.begin(); it != v->queue.end(); ++it)
+        if (it->r.id == id) {
+            v->queue.erase(it);
+            return FELT_OK;
+        }
+    for (auto it = v->active.begin(); it != v->active.end(); ++it)
+        if (it->r.id == id) {
+            v->busy[it->r.surface] = 0;
+            v->active.erase(it);
+            return FELT_OK;
+        }
+    return felt_fail(FELT_E_ARG, "no queued or in-flight request has this id");
 }
 
 API int felt_server_read_stats(felt_server* v, felt_server_stats* st) {
@@ -323,6 +404,7 @@ API int felt_server_read_stats(felt_server* v, felt_server_stats* st) {
     st->requests_queued = v->queue.size();
     st->requests_active = v->active.size();
     st->admissions_deferred = v->deferred;
+    st->requests_failed = v->n_failed;
     st->voxel_updates = v->voxel_updates;
     st->busy_s = v->busy_s;
     st->step_p50_ms = percentile(v->tick_ms, 50);
@@ -330,7 +412,7 @@ API int felt_server_read_stats(felt_server* v, felt_server_stats* st) {
     st->request_p50_ms = percentile(v->req_ms, 50);
     st->request_p99_ms = percentile(v->req_ms, 99);
     if (int rc = felt_download(v->s, &v->s->hc, v->s->ctr, sizeof(Ctr))) return rc;
-    st->device_bytes = felt_device_bytes(v->s);
+    st->device_bytes = felt_device_bytes(v->s) + (v->polys ? felt_polys_device_bytes(v->polys) : 0);
     st->budget_bytes = v->budget;
     st->bricks_total = (uint64_t)v->s->cap;
     st->bricks_used = (uint64_t)(v->s->cap - v->s->hc.free_top);

@@ -73,6 +73,51 @@ __device__ __forceinline__ void agg_pcnt(int* pcnt, int NL, int p, int dec, int 
 
 // ---- step and round control -----------------------------------------------------------------
 
+// ---- brick reservation ------------------------------------------------------------------
+// Every voxel that expands the band in a step (Surface.hpp:1249-1355) was tracked when the step
+// started: the first round processes the band (or, in a bounded step, the affected set), and
+// each later round only the voxels whose status the previous round changed; voxels the
+// expansion adds are tracked directly and are never processed again in the step.  So the
+// partitions a step can activate are inactive partitions face-adjacent to a partition holding
+// band voxels at the start (bounded step: holding affected voxels), and a partition activated in
+// the step is never deactivated in it; reactivating one the step deactivated reuses the brick it
+// freed.  Counting those partitions before anything is written lets a step that might exhaust the
+// brick pool be refused with the surface unchanged.
+
+// Inactive partitions with a face neighbour marked in `src` (nullptr: any active partition).
+template <int D>
+__global__ void k_reserve_count(Geo G, Pool P, const unsigned* src, Ctr* c) {
+    int n = 0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < G.np; p += gridDim.x * blockDim.x) {
+        if (P.page[p] >= 0) continue;
+        const Mem& M = P.mem[P.part_mem[p]];
+        int pc[3] = {0, 0, 0};
+        part_coords<D>(M, p - M.pbase, pc);
+        bool near = false;
+        for (int a = 0; a < D && !near; a++)
+            for (int sd = -1; sd <= 1 && !near; sd += 2) {
+                const int q = pc[a] + sd;
+                if (q < 0 || q >= M.csize[a]) continue;
+                const int pn = p + sd * M.cstr[a];
+                near = src ? src[pn] != 0u : P.page[pn] >= 0;
+            }
+        n += near ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_down(n, o);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&c->reserve, n);
+}
+
+// A global step whose reserve exceeds the free bricks does nothing: its lists are emptied and
+// its rounds switched off before any value is written.
+__global__ void k_reserve_decide(Ctr* c, int* bbase, int NL) {
+    if (c->reserve <= c->free_top) return;
+    c->err = 3;
+    c->one = 0;
+    for (int r = 0; r < 8; r++) c->live[r] = 0;
+    for (int k = 0; k <= NL; k++) bbase[k] = 0;
+    c->band_span[0] = c->band_span[1] = 0;
+}
+
 // Start of a step: counters of the previous step cleared, first round live.
 __global__ void k_step_reset(Ctr* c) {
     c->sc_n = 0;
@@ -86,6 +131,7 @@ __global__ void k_step_reset(Ctr* c) {
     c->one = 1;
     c->live[0] = 1;
     for (int r = 1; r < 8; r++) c->live[r] = 0;
+    c->reserve = 0;
 }
 
 // Round r runs only if round r-1 ran and changed an outer layer (Surface.hpp:752-760); its
@@ -505,22 +551,7 @@ __device__ __forceinline__ int8_t clamp_k(int k) { return (int8_t)(k < -100 ? -1
 template <int D>
 __global__ void k_speed(Geo G, Pool P, const uint32_t* list, const int* rng, const Speed* sp,
                         float* out) {
-    FELT_SPAN_LOOP(i, rng) {
-        VoxCtx c;
-        vox_ctx<D>(G, P, list[i], c);
-        const Speed s = sp[c.m];
-        float curv = 0.0f;
-        if (s.kind != FELT_SPEED_CONSTANT) {
-            float v[Stencil<D>::K];
-            if (D == 3) gather<D, Stencil<D>::K>(G, P, c, kStencil3, v);
-            else gather<D, Stencil<D>::K>(G, P, c, kStencil2, v);
-            curv = curvature_of<D>(P.phi[c.v], v);
-        }
-        const Mem& M = P.mem[c.m];
-        int w[3] = {0, 0, 0};
-        for (int a = 0; a < D; a++) w[a] = c.g[a] + M.offset[a];
-        out[i] = speed_from<D>(s, curv, w);
-    }
+    FELT_SPAN_LOOP(i, rng) out[i] = speed_voxel<D>(G, P, list[i], sp);
 }
 
 // update_zero_layer (Surface.hpp:920-979): phi += delta; tk = new layer.  tk is indexed like

@@ -97,6 +97,12 @@ struct felt_surface {
     bool use_graph = true;
     bool graph_valid = false;
     hipGraphExec_t graph_exec = nullptr;
+    // device-functor steps: the step's kernels before and after the caller's speed kernel, as two
+    // graphs (rebuilt whenever graph_valid is)
+    bool fn_graph_valid = false;
+    hipGraphExec_t gx_pre = nullptr, gx_post = nullptr;
+    int job = 0;  // a device-functor step is open: 1 global, 2 bounded
+
     bool in_update = false;
     std::vector<std::pair<std::array<int, 3>, float>> pending_deltas;
     std::map<std::array<int, 3>, size_t> pending_index;
@@ -138,6 +144,20 @@ int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t
                            const int32_t* hi, const felt_speed* fns,
                            const long long* bands = nullptr);
 int felt_delta_impl(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);
+// Device-functor steps (felt_surface_update_begin / _box_begin / _commit / _cancel).
+int felt_update_begin_impl(felt_surface* s, felt_zero_job* job);
+int felt_update_box_begin_impl(felt_surface* s, const int32_t* lo3, const int32_t* hi3,
+                               felt_zero_job* job);
+int felt_update_commit_impl(felt_surface* s);
+int felt_update_cancel_impl(felt_surface* s);
+// The isogrid of member m as device code reads it.
+void felt_fill_device_grid(const felt_surface* s, int m, felt_device_grid* out);
+// Grid operator op at n world positions of member m (ipos integer, or fpos real).
+int felt_stencil_impl(felt_surface* s, int m, int op, float dx, size_t n, const int32_t* ipos,
+                      const float* fpos, float* out);
+// The speed functor fn at n grid positions gpos (3 ints each) of member m.
+int felt_speed_eval_impl(felt_surface* s, int m, const felt_speed* fn, size_t n,
+                         const int32_t* gpos, float* out);
 int felt_update_end_public(felt_surface* s);
 // Per-member operations; positions are member grid coordinates (3 ints per entry) unless
 // stated.

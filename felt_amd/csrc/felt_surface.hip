@@ -684,7 +684,14 @@ static int read_ctr(felt_surface* s) {
                             hipMemcpyDeviceToHost, s->st));
     }
     HIPX(hipStreamSynchronize(s->st));
+    // an error is reported once: the next read starts clean (a refused step is recoverable)
+    if (s->hc.err) HIPX(hipMemsetAsync(&s->ctr->err, 0, sizeof(int), s->st));
     if (s->hc.err == 1) return felt_fail(FELT_E_NOMEM, "brick pool exhausted");
+    if (s->hc.err == 3)
+        return felt_fail(FELT_E_NOMEM, "step refused: it may activate " +
+                                           std::to_string(s->hc.reserve) + " partitions and " +
+                                           std::to_string(s->hc.free_top) +
+                                           " bricks are free; the surface is unchanged");
     if (s->hc.err) return felt_fail(FELT_E_STATE, "layer counts and layer lists disagree");
     return FELT_OK;
 }
@@ -772,8 +779,10 @@ void felt_surface_release(felt_surface* s) {
     for (hipEvent_t e : s->prof.spare) (void)hipEventDestroy(e);
     s->prof.pending.clear();
     s->prof.spare.clear();
-    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
-    s->graph_exec = nullptr;
+    for (hipGraphExec_t* g : {&s->graph_exec, &s->gx_pre, &s->gx_post}) {
+        if (*g) (void)hipGraphExecDestroy(*g);
+        *g = nullptr;
+    }
     if (s->h_speed) (void)hipHostFree(s->h_speed);
     s->h_speed = nullptr;
     void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
@@ -850,6 +859,7 @@ static int ensure_step_capacity(felt_surface* s, size_t band) {
     if ((rc = s->chg_cnt.ensure(3 * (n / (BT * ITEMS) + 2)))) return rc;
     s->list_cap = n;
     s->graph_valid = false;
+    s->fn_graph_valid = false;
     return FELT_OK;
 }
 
@@ -1006,14 +1016,21 @@ int felt_update_start_impl(felt_surface* s) {
 // band, then rounds 2..R gated on the previous round's flag.  Lookup roles follow
 // update_end_global (Surface.hpp:739-762): round 1 buffers into lookup 1; round r >= 2 reads
 // lookup (r even ? 1 : 0), buffers into the other, then empties the one it read.
-template <class ZeroFn>
-static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
+// Before the zero layer's deltas: update_start and the band lists.
+static void global_step_pre(felt_surface* s) {
     const Geo& G = s->G;
     felt_update_start_impl(s);
     compact_band(s);
+    DISPATCH(G.D, k_reserve_count, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P,
+             (const unsigned*)nullptr, s->ctr);
+    LAUNCH(k_reserve_decide, dim3(1), dim3(1), s->ctr, s->bbase, G.NL);
     LAUNCH(k_dmask_pcnt, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->dmask);
+}
+
+// After them: apply, the rounds, and the band total for sizing the next step.
+static void global_step_post(felt_surface* s) {
+    const Geo& G = s->G;
     const int* zrng = s->bbase + G.L;
-    zero_fn(zrng);
     PLAUNCH(PK_APPLY, -(long long)G.L - 1, k_apply, dim3(grid_for_lists(s)), dim3(TPB), G, s->P,
             s->band.ptr, s->band_d.ptr, zrng, s->band_tk.ptr, &s->ctr->nan_events);
     distance_passes(s, true, 0, always(s));
@@ -1029,6 +1046,13 @@ static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
     }
     LAUNCH(k_zero_int, dim3(1), dim3(1), &s->ctr->band_total);
     LAUNCH(k_band_total2, dim3(std::min(256, blocks_for(G.np))), dim3(TPB), G, s->P, s->ctr);
+}
+
+template <class ZeroFn>
+static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
+    global_step_pre(s);
+    zero_fn(s->bbase + s->G.L);
+    global_step_post(s);
 }
 
 // Rounds after the unrolled ones, driven from the host (rare).
@@ -1109,7 +1133,10 @@ int felt_global_step_finish(felt_surface* s) {
     int rc;
     if (!s->step_pending) return FELT_OK;
     s->step_pending = false;
-    if ((rc = read_ctr(s))) return rc;
+    if ((rc = read_ctr(s))) {
+        s->in_update = false;
+        return rc;
+    }
     if (s->prof.on && (rc = prof_collect(s))) return rc;
     int rounds = 1;
     for (int r = 1; r < s->unroll; r++) rounds += s->hc.live[r] ? 1 : 0;
@@ -1123,6 +1150,7 @@ int felt_global_step_finish(felt_surface* s) {
         if (s->auto_unroll && s->unroll < kMaxUnroll) {
             s->unroll = std::min(kMaxUnroll, rounds);
             s->graph_valid = false;
+            s->fn_graph_valid = false;
         }
     } else {
         s->band_now = (size_t)s->hc.band_total;
@@ -1164,6 +1192,88 @@ int felt_update_global(felt_surface* s, const felt_speed* fns) {
     int rc;
     if ((rc = felt_update_global_launch(s, fns))) return rc;
     return felt_global_step_finish(s);
+}
+
+// ---- device-functor steps: the caller's kernel writes the zero layer's deltas ----------------
+
+// Launches body's kernels: as the graph *gx, captured first unless `valid`, when graphs are on
+// and the step is not profiled; else one by one.
+template <class Body>
+static int launch_as_graph(felt_surface* s, hipGraphExec_t* gx, bool valid, Body&& body) {
+    if (!s->use_graph || s->prof.on) {
+        body();
+        HIPX(hipGetLastError());
+        return FELT_OK;
+    }
+    if (!valid) {
+        if (*gx) (void)hipGraphExecDestroy(*gx);
+        *gx = nullptr;
+        HIPX(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+        body();
+        hipGraph_t g = nullptr;
+        HIPX(hipStreamEndCapture(s->st, &g));
+        hipError_t e = hipGraphInstantiate(gx, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    }
+    HIPX(hipGraphLaunch(*gx, s->st));
+    return FELT_OK;
+}
+
+int felt_update_end_impl(felt_surface* s);
+static int boxes_list(felt_surface* s, int n, const int* ids, const int32_t* lo,
+                      const int32_t* hi, const felt_speed* fns, const long long* bands);
+
+int felt_update_begin_impl(felt_surface* s, felt_zero_job* job) {
+    int rc;
+    if ((rc = size_for_band(s))) return rc;
+    if ((rc = launch_as_graph(s, &s->gx_pre, s->fn_graph_valid, [&] { global_step_pre(s); })))
+        return rc;
+    felt_fill_device_grid(s, 0, &job->grid);
+    job->voxels = s->band.ptr;
+    job->span = s->bbase + s->G.L;
+    job->delta = s->band_d.ptr;
+    job->capacity = s->list_cap;
+    job->stream = (void*)s->st;
+    s->in_update = true;
+    s->job = 1;
+    return FELT_OK;
+}
+
+int felt_update_box_begin_impl(felt_surface* s, const int32_t* lo3, const int32_t* hi3,
+                               felt_zero_job* job) {
+    const int id = 0;
+    int rc;
+    if ((rc = boxes_list(s, 1, &id, lo3, hi3, nullptr, nullptr))) return rc;
+    felt_fill_device_grid(s, 0, &job->grid);
+    job->voxels = s->zlist.ptr;
+    job->span = &s->ctr->z_span[0];
+    job->delta = s->zdelta.ptr;
+    job->capacity = s->list_cap;
+    job->stream = (void*)s->st;
+    s->job = 2;
+    return FELT_OK;
+}
+
+int felt_update_commit_impl(felt_surface* s) {
+    const int job = s->job;
+    s->job = 0;
+    if (job == 2) return felt_update_end_impl(s);
+    if (job != 1) return felt_fail(FELT_E_STATE, "no device-functor step to commit");
+    int rc;
+    if ((rc = launch_as_graph(s, &s->gx_post, s->fn_graph_valid, [&] { global_step_post(s); })))
+        return rc;
+    s->fn_graph_valid = s->use_graph && !s->prof.on;
+    s->step_pending = true;
+    return felt_global_step_finish(s);
+}
+
+int felt_update_cancel_impl(felt_surface* s) {
+    if (!s->job) return felt_fail(FELT_E_STATE, "no device-functor step to cancel");
+    s->job = 0;
+    s->in_update = false;
+    HIPX(hipStreamSynchronize(s->st));
+    return FELT_OK;
 }
 
 // Times one pass of a global step on the current state (felt_surface_time_pass).
@@ -1288,13 +1398,27 @@ int felt_update_end_impl(felt_surface* s) {
         prof_end(s);
         LAUNCH(k_snap_front, dim3(1), dim3(1), s->ctr, 0, j + 1);
     }
+    // delta(k) partitions of the affected set, and the partitions the step may activate: the
+    // inactive neighbours of those
+    LAUNCH(k_dmask_lu, dim3(bfs_grid), dim3(TPB), G, s->P, A, lu_span(s, 0), s->dmask);
+    HIPX(hipMemsetAsync(&s->ctr->reserve, 0, sizeof(int), s->st));
+    DISPATCH(G.D, k_reserve_count, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P,
+             (const unsigned*)s->dmask, s->ctr);
     // the affected set's size sizes every grid of the rounds below
     if ((rc = read_ctr(s))) return rc;
     const int an = s->hc.lu_span[0][1];
     s->hint = an;
     const int grid = grid_for_lists(s);
     LAUNCH(k_unmark, dim3(grid), dim3(TPB), A, lu_span(s, 0), s->amark);
-    LAUNCH(k_dmask_lu, dim3(grid), dim3(TPB), G, s->P, A, lu_span(s, 0), s->dmask);
+    if (s->hc.reserve > s->hc.free_top) {
+        s->hint = -1;
+        s->in_update = false;
+        HIPX(hipStreamSynchronize(s->st));
+        return felt_fail(FELT_E_NOMEM, "bounded step refused: it may activate " +
+                                           std::to_string(s->hc.reserve) + " partitions and " +
+                                           std::to_string(s->hc.free_top) +
+                                           " bricks are free; the surface is unchanged");
+    }
     LAUNCH(k_tk_init, dim3(grid), dim3(TPB), A.tag, A.tk, lu_span(s, 0), always(s));
     prof_begin(s, PK_APPLY, 0);
     LAUNCH(k_apply, dim3(grid), dim3(TPB), G, s->P, s->zlist.ptr, s->zdelta.ptr,
@@ -1341,8 +1465,10 @@ int felt_band_total(felt_surface* s, size_t* out) {
 // overlap its box (clamped to the grid) are searched for zero-layer voxels with
 // lo <= x < min(hi + 1, grid_hi), which take that member's speed; then update_end over all of
 // them at once.
-int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t* lo,
-                           const int32_t* hi, const felt_speed* fns, const long long* bands) {
+// The listing half: list capacity, update_start, the zero-layer voxels of every box into zlist
+// (span ctr->z_span); fns[j] (when given) staged as member ids[j]'s speed.
+static int boxes_list(felt_surface* s, int n, const int* ids, const int32_t* lo,
+                      const int32_t* hi, const felt_speed* fns, const long long* bands) {
     const Geo& G = s->G;
     int rc;
     if (G.nmem == 1) {
@@ -1377,7 +1503,7 @@ int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t
             vlo[a] = l3[a] - M.offset[a];
             vhi[a] = std::min(M.size[a], h3[a] + 1 - M.offset[a]);
         }
-        s->h_speed[ids[j]] = to_speed(fns[j]);
+        if (fns) s->h_speed[ids[j]] = to_speed(fns[j]);
         if (empty) continue;
         const int b = (int)boxes.size() / 6;
         for (int a = 0; a < 3; a++) boxes.push_back(vlo[a]);
@@ -1403,6 +1529,14 @@ int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t
                  s->tmp_i.ptr, s->tmp_i.ptr + np, (int)np, s->tmp_i.ptr + 2 * np, s->zlist.ptr,
                  s->ctr);
     }
+    return FELT_OK;
+}
+
+int felt_update_boxes_impl(felt_surface* s, int n, const int* ids, const int32_t* lo,
+                           const int32_t* hi, const felt_speed* fns, const long long* bands) {
+    const Geo& G = s->G;
+    int rc;
+    if ((rc = boxes_list(s, n, ids, lo, hi, fns, bands))) return rc;
     // the zero-layer count stays on the device: the speed pass strides over its span
     HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, G.nmem * sizeof(Speed), hipMemcpyHostToDevice,
                         s->st));

@@ -31,7 +31,7 @@ class felt_server_stats(C.Structure):
                 ("request_p50_ms", C.c_double), ("request_p99_ms", C.c_double),
                 ("device_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
                 ("bricks_used", C.c_uint64), ("bricks_total", C.c_uint64),
-                ("phase_ms", C.c_double * 4)]
+                ("phase_ms", C.c_double * 4), ("requests_failed", C.c_uint64)]
 
 
 _SIGS = {
@@ -45,6 +45,8 @@ _SIGS = {
     "felt_server_submit": [_P, C.POINTER(felt_request), C.c_size_t],
     "felt_server_tick": [_P, C.POINTER(C.c_int32)],
     "felt_server_poll": [_P, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_size_t)],
+    "felt_server_poll_failed": [_P, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(C.c_size_t)],
+    "felt_server_cancel": [_P, C.c_uint64],
     "felt_server_read_stats": [_P, C.POINTER(felt_server_stats)],
 }
 
@@ -117,11 +119,22 @@ class Server:
         return int(n.value)
 
     def poll(self, cap: int = 4096):
-        out = np.empty(cap, dtype=np.uint64)
+        """Ids of the requests completed since the last poll."""
+        return self._drain("felt_server_poll", cap)
+
+    def poll_failed(self, cap: int = 4096):
+        """Ids of the requests that failed since the last poll_failed (felt_server_tick)."""
+        return self._drain("felt_server_poll_failed", cap)
+
+    def _drain(self, name, cap):
+        out = np.empty(max(cap, 1), dtype=np.uint64)
         n = C.c_size_t()
-        self.api("felt_server_poll", self.h, out.ctypes.data_as(C.POINTER(C.c_uint64)), cap,
-                 C.byref(n))
+        self.api(name, self.h, out.ctypes.data_as(C.POINTER(C.c_uint64)), cap, C.byref(n))
         return [int(x) for x in out[: n.value]]
+
+    def cancel(self, rid: int) -> None:
+        """Withdraw a queued or in-flight request."""
+        self.api("felt_server_cancel", self.h, int(rid))
 
     def stats(self) -> dict:
         st = felt_server_stats()

@@ -172,6 +172,92 @@ int felt_surface_ray_device(felt_surface* s, size_t n, const float* origins, con
  * surface's stream. */
 int felt_surface_time_ray(felt_surface* s, size_t n, const float* origins, const float* dirs,
                           int reps, double* us);
+/* ---- The isogrid's numeric API on the device (NumericMixin.hpp:72-500) ---------------- */
+/* Operators a speed functor calls on the grid (Partitioned.hpp:160-189), evaluated on the device
+ * at n positions with the code the device functor path compiles (include/felt_amd/device_grid.hpp):
+ *   FELT_STENCIL_GET         get(pos)                                      1 value
+ *   FELT_STENCIL_CURV        curv(pos), difference of normals (NumericMixin.hpp:72-156)    1 value
+ *   FELT_STENCIL_DIVERGENCE  divergence(pos) = sum(gradF - gradB) / dx^2 (:164-175)       1 value
+ *   FELT_STENCIL_GRAD        grad(pos), central or one-sided at the grid edge (:188-231) D values
+ *   FELT_STENCIL_GRAD_E      gradE(pos), upwind (entropy satisfying) (:242-273)         D values
+ *   FELT_STENCIL_GRAD_F, _B, _C   forward, backward, central differences (:279-358)    D values
+ * dx is the grid spacing the derivatives divide by (the reference's dx(); 1 for a Surface's
+ * isogrid).  felt_surface_stencil takes integer positions (n*D int32); felt_surface_stencil_real
+ * takes real positions (n*D floats), every sample of which is then an N-linear interpolation, as
+ * the reference's members templated on the position type do (get(VecDf) = interp, :390-481).
+ * out receives n values, or n*D for the gradients. */
+enum {
+    FELT_STENCIL_GET = 0,
+    FELT_STENCIL_CURV = 1,
+    FELT_STENCIL_DIVERGENCE = 2,
+    FELT_STENCIL_GRAD = 3,
+    FELT_STENCIL_GRAD_E = 4,
+    FELT_STENCIL_GRAD_F = 5,
+    FELT_STENCIL_GRAD_B = 6,
+    FELT_STENCIL_GRAD_C = 7
+};
+int felt_surface_stencil(felt_surface* s, int op, float dx, size_t n, const int32_t* pos,
+                         float* out);
+int felt_surface_stencil_real(felt_surface* s, int op, float dx, size_t n, const float* pos,
+                              float* out);
+/* fn(pos) of a felt_speed functor at n positions, computed by the device code the update kernel
+ * runs for each zero-layer voxel (Surface.hpp:265-294), without taking a step.  Every position
+ * must lie in an active partition, else FELT_E_ARG. */
+int felt_surface_speed_eval(felt_surface* s, const felt_speed* fn, size_t n, const int32_t* pos,
+                            float* out);
+
+/* ---- Device functors (Surface.hpp:254-386, 1754-1802) -------------------------------- */
+/* The isogrid as a kernel reads it: plain values and device pointers, valid until the surface's
+ * next update or destruction.  A voxel v of the pool is brick slot v / pv, storage leaf v % pv;
+ * a brick is cut into tiles of 2^tile_shift[a] voxels per axis, tiles row-major in the brick,
+ * voxels row-major in a tile.  include/felt_amd/device_grid.hpp wraps it as Felt's grid API. */
+typedef struct felt_device_grid {
+    int32_t D, L;
+    int32_t size[3];         /* grid size (unused axes 1) */
+    int32_t offset[3];       /* world position of grid position 0: -size/2 */
+    int32_t part[3];         /* partition size */
+    int32_t part_shift[3];   /* log2(part[a]) when a power of two, else -1 */
+    int32_t cstride[3];      /* children-grid strides, row-major, last axis fastest */
+    int32_t pbase;           /* pool partition of children-grid index 0 */
+    int32_t tile_shift[3];   /* log2 of the tile edge */
+    int32_t tile_stride[3];  /* leaf stride of one tile step */
+    int32_t in_stride[3];    /* leaf stride of one voxel step inside a tile */
+    int32_t tiles[3];        /* tiles per axis in a brick */
+    int32_t tile_vshift;     /* log2 of the tile volume */
+    uint32_t pv;             /* voxels per brick */
+    float dx;                /* grid spacing of the derivatives (1) */
+    const int32_t* page;     /* pool partition -> brick slot, or -1 when inactive */
+    const float* pbg;        /* pool partition -> value of an inactive partition */
+    const float* phi;        /* brick pool values */
+    const int32_t* slot_org; /* 4 per slot: grid position of the brick's first voxel, flags */
+} felt_device_grid;
+int felt_surface_device_grid(felt_surface* s, felt_device_grid* out);
+
+/* One update(fn) whose fn runs in the caller's own kernel:
+ *   felt_surface_update_begin (global step) or felt_surface_update_box_begin (bounded step,
+ *   Surface.hpp:310-386) queues update_start and the listing of the zero-layer voxels fn is
+ *   called for on the surface's stream, and describes them in *job: job->voxels[i] for
+ *   job->span[0] <= i < job->span[1] (span is device memory), each needing delta[i] = fn(pos of
+ *   voxel), at most job->capacity entries;
+ *   the caller launches its kernel on job->stream (a hipStream_t);
+ *   felt_surface_update_commit queues the rest of the step (apply, distance, flush, expand and
+ *   the round loop, Surface.hpp:404-430, 739-762) and finishes it as felt_surface_update does;
+ *   felt_surface_update_cancel instead abandons the step, leaving values and layers unchanged.
+ * Between begin and commit or cancel every other call on the surface fails with FELT_E_STATE. */
+typedef struct felt_zero_job {
+    felt_device_grid grid;
+    const uint32_t* voxels;
+    const int32_t* span;
+    float* delta;
+    uint64_t capacity;
+    void* stream;
+} felt_zero_job;
+int felt_surface_update_begin(felt_surface* s, felt_zero_job* job);
+int felt_surface_update_box_begin(felt_surface* s, const int32_t* lo, const int32_t* hi,
+                                  felt_zero_job* job);
+int felt_surface_update_commit(felt_surface* s);
+int felt_surface_update_cancel(felt_surface* s);
+
 /* Launch a global update(fn) with a device functor as one captured HIP graph (default on):
  * the first three rounds run without host reads, later rounds are launched from the host.
  * Off: the same kernels are launched one by one.  Not a reference member. */
@@ -322,6 +408,7 @@ typedef struct felt_server_stats {
     double request_p50_ms, request_p99_ms;  /* submit to completion */
     uint64_t device_bytes, budget_bytes, bricks_used, bricks_total;
     double phase_ms[4];  /* mean per tick: bounded update, notify, march, band refresh */
+    uint64_t requests_failed;      /* requests that left without completing (felt_server_tick) */
 } felt_server_stats;
 /* sizes: n*D grid sizes.  Bricks get at most three quarters of budget_bytes. */
 int felt_server_create(int D, int L, const int32_t* part, int32_t n, const int32_t* sizes,
@@ -338,11 +425,23 @@ int felt_server_start(felt_server* v);
  * time (a fixed part plus a cost per band voxel inside the in-flight boxes) stays within
  * tick_ms; a request is always admitted into an empty tick.  0 (the default) disables it. */
 int felt_server_set_target(felt_server* v, double tick_ms);
+/* Queues n requests; every one is checked first, so a call queues all of them or none. */
 int felt_server_submit(felt_server* v, const felt_request* r, size_t n);
-/* One batched step; *stepped = requests advanced (0 when nothing is in flight). */
+/* One batched step; *stepped = requests in the step (0 when nothing is in flight).
+ * A request fails -- leaves the server without completing, its id reported by
+ * felt_server_poll_failed -- when its step lists alone exceed the budget (at admission), or when
+ * its bounded step alone may activate more partitions than the brick pool has free (the step is
+ * then refused and its surface left unchanged; the other requests of the tick still run).  Any
+ * other error fails every in-flight request, whose surfaces may be partly stepped, and is
+ * returned. */
 int felt_server_tick(felt_server* v, int32_t* stepped);
-/* Ids of requests completed since the last poll (up to cap). */
+/* Ids of requests completed (felt_server_poll) or failed (felt_server_poll_failed) since the
+ * last such poll, up to cap; ids may be NULL only when cap is 0. */
 int felt_server_poll(felt_server* v, uint64_t* ids, size_t cap, size_t* n);
+int felt_server_poll_failed(felt_server* v, uint64_t* ids, size_t cap, size_t* n);
+/* Withdraws the queued or in-flight request with this id (the first one submitted); an
+ * in-flight request keeps the steps it has run.  FELT_E_ARG when there is none. */
+int felt_server_cancel(felt_server* v, uint64_t id);
 int felt_server_read_stats(felt_server* v, felt_server_stats* out);
 
 /* ---- Tables ------------------------------------------------------------------------- */

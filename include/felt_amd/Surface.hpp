@@ -12,9 +12,11 @@
 //   save(ostream), load(istream)               Surface.hpp:180-196 (the same cereal archive)
 //   Polys(surface), notify(), march(), invalidate(), changes(), children().get(p)
 //                                            Polys.hpp:56-201, Poly.hpp:176-189
-// Functors: a felt_speed descriptor runs on the device; a host callable with one of the
-// reference's call shapes -- fn(pos, isogrid), fn(pos) or fn() returning float -- is evaluated
-// on the host for every zero-layer voxel before the step, as the reference does on its CPU.
+// Functors: a felt_speed descriptor runs on the device; so does any callable wrapped as
+// device_fn(fn), compiled into the caller's own kernel (device.hpp); a host callable with one of
+// the reference's call shapes -- fn(pos, isogrid), fn(pos) or fn() returning float -- is
+// evaluated on the host for every zero-layer voxel before the step, as the reference does on its
+// CPU.
 // Errors raise std::domain_error, as the reference does when FELT_EXCEPTIONS is defined.
 #pragma once
 
@@ -35,19 +37,9 @@
 #include <vector>
 
 #include "../felt_amd.h"
+#include "types.hpp"
 
 namespace Felt {
-
-using Distance = float;
-using LayerId = int;
-using PosIdx = std::size_t;
-using TupleIdx = int;
-using ListIdx = std::size_t;
-using PosIdxList = std::vector<PosIdx>;
-template <int D>
-using VecDi = std::array<int, D>;
-template <int D>
-using VecDf = std::array<float, D>;
 
 namespace detail {
 inline void check(int rc) {
@@ -65,6 +57,29 @@ VecDf<D> constant_f(float v) {
     a.fill(v);
     return a;
 }
+}  // namespace detail
+
+// A functor evaluated on the GPU.  update(device_fn(fn)) and update(lo, hi, device_fn(fn))
+// compile fn into a kernel of the caller's own HIP translation unit, which must include
+// <felt_amd/device.hpp> and be built with hipcc -ffp-contract=off.  fn takes one of the
+// reference's call shapes (Surface.hpp:1754-1802) with the grid as Felt::device::Grid<D>:
+// fn(pos, grid), fn(pos) or fn(), returning float.
+template <class F>
+struct DeviceFn {
+    F fn;
+};
+template <class F>
+DeviceFn<std::decay_t<F>> device_fn(F&& f) {
+    return DeviceFn<std::decay_t<F>>{std::forward<F>(f)};
+}
+namespace detail {
+template <class T>
+struct is_device_fn : std::false_type {};
+template <class F>
+struct is_device_fn<DeviceFn<F>> : std::true_type {};
+// Launches the speed kernel of a device functor for a step's zero layer (device.hpp).
+template <int D, class F>
+struct DeviceSpeed;
 }  // namespace detail
 
 // Host view of the isogrid for host functors and read-back, with the reference's grid API for
@@ -358,13 +373,27 @@ public:
     void update(const VecDi<D>& lo, const VecDi<D>& hi, const felt_speed& fn) {
         detail::check(felt_surface_update_box(m_h.get(), lo.data(), hi.data(), &fn));
     }
-    template <class Fn, class = std::enable_if_t<!std::is_same<std::decay_t<Fn>, felt_speed>::value>>
+    template <class F>
+    void update(DeviceFn<F> f) {
+        felt_zero_job job{};
+        detail::check(felt_surface_update_begin(m_h.get(), &job));
+        run_device_job(job, f.fn);
+    }
+    template <class F>
+    void update(const VecDi<D>& lo, const VecDi<D>& hi, DeviceFn<F> f) {
+        felt_zero_job job{};
+        detail::check(felt_surface_update_box_begin(m_h.get(), lo.data(), hi.data(), &job));
+        run_device_job(job, f.fn);
+    }
+    template <class Fn, class = std::enable_if_t<!std::is_same<std::decay_t<Fn>, felt_speed>::value &&
+                                                 !detail::is_device_fn<std::decay_t<Fn>>::value>>
     void update(Fn&& fn) {
         std::vector<int32_t> pos = layer_positions(0);
         std::vector<float> val = evaluate(fn, pos);
         detail::check(felt_surface_update_deltas(m_h.get(), val.size(), pos.data(), val.data()));
     }
-    template <class Fn, class = std::enable_if_t<!std::is_same<std::decay_t<Fn>, felt_speed>::value>>
+    template <class Fn, class = std::enable_if_t<!std::is_same<std::decay_t<Fn>, felt_speed>::value &&
+                                                 !detail::is_device_fn<std::decay_t<Fn>>::value>>
     void update(const VecDi<D>& lo, const VecDi<D>& hi, Fn&& fn) {
         std::vector<int32_t> pos = query([&](int32_t* out, std::size_t cap, std::size_t* n) {
             return felt_surface_zero_layer_in_box(m_h.get(), lo.data(), hi.data(), out, cap, n);
@@ -443,6 +472,16 @@ private:
     struct Del {
         void operator()(felt_surface* h) const { felt_surface_destroy(h); }
     };
+    // The caller's speed kernel between begin and commit; a failed launch abandons the step.
+    template <class F>
+    void run_device_job(const felt_zero_job& job, F& fn) {
+        const std::string err = detail::DeviceSpeed<D, F>::launch(job, fn);
+        if (!err.empty()) {
+            felt_surface_update_cancel(m_h.get());
+            throw std::domain_error("device functor launch failed: " + err);
+        }
+        detail::check(felt_surface_update_commit(m_h.get()));
+    }
     template <class Q>
     static std::vector<int32_t> query(Q&& q) {
         std::size_t n = 0;

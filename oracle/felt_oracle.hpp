@@ -568,23 +568,29 @@ public:
         }
     }
 
-    /* ---- numeric stencils ---- */
+    /* ---- numeric stencils (NumericMixin.hpp:72-500) ----
+     * Each is templated on the position type as the reference's are: an integer position reads
+     * get(), a real one reads interp() (NumericMixin.hpp:390-393).  Vector sums are Eigen's
+     * v0 + (v1 + v2); derivatives divide by the grid spacing dx (1 for a Surface's isogrid). */
+    R at(const Vi<D>& p) const { return iso.get(p); }
+    R at(const Vf& p) const { return interp(p); }
     /* Mean curvature by difference of normals (NumericMixin.hpp:72-156). */
-    R curv(const Vi<D>& pos) const {
-        const R c = iso.get(pos);
-        Vi<D> dir = pos;
+    template <class P>
+    R curv(const P& pos) const {
+        const R c = at(pos);
+        P dir = pos;
         Vf nf{}, nb{};
         for (int a = 0; a < D; a++) {
             dir[a] += 1;
-            const R va = iso.get(dir) - c;
+            const R va = at(dir) - c;
             R sq = 0;
             for (int b = 0; b < D; b++) {
                 if (b == a) continue;
-                Vi<D> dn = dir;
+                P dn = dir;
                 dn[b] -= 1;
-                const R lo = iso.get(dn);
+                const R lo = at(dn);
                 dn[b] += 2;
-                const R hi = iso.get(dn);
+                const R hi = at(dn);
                 const R v = (hi - lo) / R(2);
                 sq += v * v;
             }
@@ -593,15 +599,15 @@ public:
         }
         for (int a = 0; a < D; a++) {
             dir[a] -= 1;
-            const R va = c - iso.get(dir);
+            const R va = c - at(dir);
             R sq = 0;
             for (int b = 0; b < D; b++) {
                 if (b == a) continue;
-                Vi<D> dn = dir;
+                P dn = dir;
                 dn[b] -= 1;
-                const R lo = iso.get(dn);
+                const R lo = at(dn);
                 dn[b] += 2;
-                const R hi = iso.get(dn);
+                const R hi = at(dn);
                 const R v = (hi - lo) / R(2);
                 sq += v * v;
             }
@@ -637,42 +643,95 @@ public:
         }
         return vals[0];
     }
-    /* Safe gradient at an integer position (NumericMixin.hpp:188-231). */
-    Vf grad(const Vi<D>& pos) const {
+    /* Safe gradient (NumericMixin.hpp:188-231): central where both neighbours are inside the grid,
+     * one-sided where one is, 0 where none is; a real position's probes step -1, +2, -1 in working
+     * precision as the reference's do. */
+    template <class P>
+    Vf grad(const P& pos, R dx = R(1)) const {
         Vf gv{};
-        Vi<D> pt = pos;
-        const R centre = iso.get(pos);
+        P pt = pos;
+        const R centre = at(pos);
         for (int a = 0; a < D; a++) {
             R back = centre, fwd = centre;
             int order = 0;
             pt[a] -= 1;
-            if (g.inside(pt)) { back = iso.get(pt); order++; }
+            if (g.inside(pt)) { back = at(pt); order++; }
             pt[a] += 2;
-            if (g.inside(pt)) { fwd = iso.get(pt); order++; }
+            if (g.inside(pt)) { fwd = at(pt); order++; }
             pt[a] -= 1;
             gv[a] = order != 0 ? (fwd - back) / R(order) : R(0);
         }
-        for (int a = 0; a < D; a++) gv[a] = gv[a] / R(1);
+        for (int a = 0; a < D; a++) gv[a] = gv[a] / dx;
         return gv;
     }
-    /* Safe gradient at a real position: samples go through interp, and the probe position is
-     * stepped -1, +2, -1 in working precision as the reference does. */
-    Vf grad(const Vf& pos) const {
+    /* Upwind gradient (NumericMixin.hpp:242-273): max(c - back, 0) + min(fwd - c, 0) per axis. */
+    template <class P>
+    Vf gradE(const P& pos, R dx = R(1)) const {
         Vf gv{};
-        Vf pt = pos;
-        const R centre = interp(pos);
+        P pt = pos;
+        const R centre = at(pos);
         for (int a = 0; a < D; a++) {
-            R back = centre, fwd = centre;
-            int order = 0;
-            pt[a] -= R(1);
-            if (g.inside(pt)) { back = interp(pt); order++; }
-            pt[a] += R(2);
-            if (g.inside(pt)) { fwd = interp(pt); order++; }
-            pt[a] -= R(1);
-            gv[a] = order != 0 ? (fwd - back) / R(order) : R(0);
+            pt[a] -= 1;
+            R back = at(pt);
+            pt[a] += 2;
+            R fwd = at(pt);
+            pt[a] -= 1;
+            back = std::max(centre - back, R(0));
+            fwd = std::min(fwd - centre, R(0));
+            gv[a] = fwd + back;
         }
-        for (int a = 0; a < D; a++) gv[a] = gv[a] / R(1);
+        for (int a = 0; a < D; a++) gv[a] = gv[a] / dx;
         return gv;
+    }
+    /* Forward, backward and central differences (NumericMixin.hpp:279-358). */
+    template <class P>
+    Vf gradF(const P& pos, R dx = R(1)) const {
+        Vf gv{};
+        P pt = pos;
+        const R centre = at(pos);
+        for (int a = 0; a < D; a++) {
+            pt[a] += 1;
+            gv[a] = at(pt) - centre;
+            pt[a] -= 1;
+        }
+        for (int a = 0; a < D; a++) gv[a] = gv[a] / dx;
+        return gv;
+    }
+    template <class P>
+    Vf gradB(const P& pos, R dx = R(1)) const {
+        Vf gv{};
+        P pt = pos;
+        const R centre = at(pos);
+        for (int a = 0; a < D; a++) {
+            pt[a] -= 1;
+            gv[a] = centre - at(pt);
+            pt[a] += 1;
+        }
+        for (int a = 0; a < D; a++) gv[a] = gv[a] / dx;
+        return gv;
+    }
+    template <class P>
+    Vf gradC(const P& pos, R dx = R(1)) const {
+        Vf gv{};
+        P pt = pos;
+        for (int a = 0; a < D; a++) {
+            pt[a] -= 1;
+            const R back = at(pt);
+            pt[a] += 2;
+            const R fwd = at(pt);
+            pt[a] -= 1;
+            gv[a] = (fwd - back) / R(2);
+        }
+        for (int a = 0; a < D; a++) gv[a] = gv[a] / dx;
+        return gv;
+    }
+    /* Divergence of the gradient: sum(gradF - gradB) / dx^2 (NumericMixin.hpp:164-175). */
+    template <class P>
+    R divergence(const P& pos, R dx = R(1)) const {
+        const Vf f = gradF(pos, dx), b = gradB(pos, dx);
+        Vf d{};
+        for (int a = 0; a < D; a++) d[a] = f[a] - b[a];
+        return sum(d) / (dx * dx);
     }
 
     /* ---- raycasting (Surface.hpp:468-616, 1450-1598) ---- */

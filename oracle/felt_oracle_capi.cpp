@@ -103,6 +103,9 @@ struct Handle {
     virtual void set_omp(bool on) = 0;
     virtual int ray(size_t n, const float* origin, const float* dir, float* out) = 0;
     virtual int save(std::vector<uint8_t>& out) = 0;
+    virtual int stencil(int op, float dx, size_t n, const int32_t* ipos, const float* fpos,
+                        float* out) = 0;
+    virtual int speed_eval(const felt_speed& sp, size_t n, const int32_t* pos, float* out) = 0;
     virtual bool load(const uint8_t* src, size_t n) = 0;
     /* polys */
     virtual int polys_create() = 0;
@@ -135,6 +138,46 @@ struct HandleT final : Handle {
     }
     int check_pos(const Vi<D>& v) const {
         if (!s.g.inside(v)) return fail(FELT_E_ARG, "position outside the grid");
+        return FELT_OK;
+    }
+    using Vf = Vr<D, R>;
+    /* Grid operator op at n positions (integer ipos or real fpos), felt_surface_stencil. */
+    template <class P>
+    void stencil_at(int op, R dx, const P& p, float* out) const {
+        Vf v{};
+        switch (op) {
+        case FELT_STENCIL_GET: *out = float(s.at(p)); return;
+        case FELT_STENCIL_CURV: *out = float(s.curv(p)); return;
+        case FELT_STENCIL_DIVERGENCE: *out = float(s.divergence(p, dx)); return;
+        case FELT_STENCIL_GRAD: v = s.grad(p, dx); break;
+        case FELT_STENCIL_GRAD_E: v = s.gradE(p, dx); break;
+        case FELT_STENCIL_GRAD_F: v = s.gradF(p, dx); break;
+        case FELT_STENCIL_GRAD_B: v = s.gradB(p, dx); break;
+        default: v = s.gradC(p, dx); break;
+        }
+        for (int a = 0; a < D; a++) out[a] = float(v[a]);
+    }
+    int stencil(int op, float dx, size_t n, const int32_t* ipos, const float* fpos,
+                float* out) override {
+        if (op < FELT_STENCIL_GET || op > FELT_STENCIL_GRAD_C) return fail(FELT_E_ARG, "unknown stencil operator");
+        const size_t w = op >= FELT_STENCIL_GRAD ? size_t(D) : 1;
+        for (size_t i = 0; i < n; i++) {
+            if (fpos) {
+                Vf p{};
+                for (int a = 0; a < D; a++) p[a] = R(fpos[i * D + a]);
+                stencil_at(op, R(dx), p, out + i * w);
+            } else {
+                stencil_at(op, R(dx), vec(ipos + i * D), out + i * w);
+            }
+        }
+        return FELT_OK;
+    }
+    int speed_eval(const felt_speed& sp, size_t n, const int32_t* pos, float* out) override {
+        for (size_t i = 0; i < n; i++) {
+            const Vi<D> p = vec(pos + i * D);
+            if (int rc = check_pos(p)) return rc;
+            out[i] = float(eval_speed<D, R>(sp, p, s));
+        }
         return FELT_OK;
     }
     int seed(const int32_t* pos) override {
@@ -534,9 +577,37 @@ OAPI float felt_oracle_value_noise(int D, const int32_t* pos, uint64_t seed, int
     if (D == 2) return value_noise<2>(Vi<2>{pos[0], pos[1]}, seed, cell);
     return value_noise<3>(Vi<3>{pos[0], pos[1], pos[2]}, seed, cell);
 }
-/* Curvature at a position of the current isogrid (fp32 surfaces only). */
-OAPI int felt_oracle_surface_curv(felt_oracle_surface* s, const int32_t* pos, float* out) {
-    if (auto* h3 = dynamic_cast<HandleT<3, float>*>(H(s).get())) { *out = h3->s.curv(h3->vec(pos)); return FELT_OK; }
-    if (auto* h2 = dynamic_cast<HandleT<2, float>*>(H(s).get())) { *out = h2->s.curv(h2->vec(pos)); return FELT_OK; }
-    return fail(FELT_E_ARG, "curv: fp32 surfaces only");
+/* Grid operators of the current isogrid (felt_surface_stencil / _stencil_real). */
+OAPI int felt_oracle_surface_stencil(felt_oracle_surface* s, int op, float dx, size_t n,
+                                     const int32_t* pos, float* out) {
+    return H(s)->stencil(op, dx, n, pos, nullptr, out);
+}
+OAPI int felt_oracle_surface_stencil_real(felt_oracle_surface* s, int op, float dx, size_t n,
+                                          const float* pos, float* out) {
+    return H(s)->stencil(op, dx, n, nullptr, pos, out);
+}
+/* The speed functor at n positions (felt_surface_speed_eval). */
+OAPI int felt_oracle_surface_speed_eval(felt_oracle_surface* s, const felt_speed* fn, size_t n,
+                                        const int32_t* pos, float* out) {
+    return H(s)->speed_eval(*fn, n, pos, out);
+}
+/* Row-major index of pos in a grid of the given size and offset, and its inverse
+ * (Util.hpp:23-83). */
+OAPI int felt_oracle_index(int D, const int32_t* size, const int32_t* offset, const int32_t* pos,
+                           uint64_t* idx, int32_t* back) {
+    if (D == 2) {
+        const Vi<2> sz{size[0], size[1]}, p{pos[0] - offset[0], pos[1] - offset[1]};
+        *idx = lin<2>(p, sz);
+        const Vi<2> q = unlin<2>(Idx(*idx), sz);
+        for (int a = 0; a < 2; a++) back[a] = q[a] + offset[a];
+    } else if (D == 3) {
+        const Vi<3> sz{size[0], size[1], size[2]};
+        const Vi<3> p{pos[0] - offset[0], pos[1] - offset[1], pos[2] - offset[2]};
+        *idx = lin<3>(p, sz);
+        const Vi<3> q = unlin<3>(Idx(*idx), sz);
+        for (int a = 0; a < 3; a++) back[a] = q[a] + offset[a];
+    } else {
+        return fail(FELT_E_ARG, "D must be 2 or 3");
+    }
+    return FELT_OK;
 }
