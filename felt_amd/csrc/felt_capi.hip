@@ -486,6 +486,13 @@ API int felt_surface_ray(felt_surface* s, size_t n, const float* origins, const 
     return felt_ray_impl(s, nullptr, 0, n, origins, dirs, out);
 }
 
+API int felt_surface_ray_noalloc(felt_surface* s, size_t n, const float* origins,
+                                 const float* dirs, float* out) {
+    CHECK_S(s);
+    if (n && (!origins || !dirs || !out)) return felt_fail(FELT_E_ARG, "null ray array");
+    return felt_ray_noalloc_impl(s, n, origins, dirs, out);
+}
+
 API int felt_surface_ray_device(felt_surface* s, size_t n, const float* origins,
                                 const float* dirs, float* out) {
     CHECK_S(s);

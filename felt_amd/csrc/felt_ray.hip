@@ -13,7 +13,9 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "felt_surface.h"
 
@@ -474,5 +476,63 @@ int felt_time_ray_impl(felt_surface* s, size_t n, const float* origins, const fl
     if (rc) return rc;
     if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, std::string("ray timing: ") + hipGetErrorString(e));
     *us = (double)ms * 1e3 / reps;
+    return FELT_OK;
+}
+
+// ---- rays without device allocation -------------------------------------------------------
+// felt_surface_ray_noalloc stages its rays through this array instead of device buffers the
+// library would allocate: origins, directions and hits of at most kNoallocRays rays, one block of
+// kNoallocRays * 3 floats each (2.25 MiB).  Every device holds its own copy until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress returns with that
+// device current, never through the array's name.
+constexpr size_t kNoallocRays = 65536;
+constexpr size_t kNoallocBlock = kNoallocRays * 3;
+static __device__ __align__(256) float scratch[3 * kNoallocBlock];
+
+namespace {
+
+std::mutex g_scratch_mu;            // every call on a device shares its array: one call at a time
+std::vector<float*> g_scratch_ptr;  // device address of `scratch` per device, resolved once
+
+int scratch_on(int dev, float** out) {
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        HIPX(hipGetDevice(&cur));
+        if (cur != dev) HIPX(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return felt_fail(FELT_E_DEVICE,
+                             std::string("hipGetSymbolAddress: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FELT_OK;
+}
+
+}  // namespace
+
+int felt_ray_noalloc_impl(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                          float* out) {
+    if (n == 0) return FELT_OK;
+    if (n > kNoallocRays)
+        return felt_fail(FELT_E_ARG, "felt_surface_ray_noalloc takes at most 65536 rays per call");
+    const int D = s->G.D;
+    int rc;
+    if ((rc = check_rays(D, n, origins, dirs))) return rc;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    float* base = nullptr;
+    if ((rc = scratch_on(s->device, &base))) return rc;
+    const size_t nf = n * D;
+    float* d_org = base;
+    float* d_dir = base + kNoallocBlock;
+    float* d_out = base + 2 * kNoallocBlock;
+    HIPX(hipMemcpyAsync(d_org, origins, nf * sizeof(float), hipMemcpyHostToDevice, s->st));
+    HIPX(hipMemcpyAsync(d_dir, dirs, nf * sizeof(float), hipMemcpyHostToDevice, s->st));
+    if ((rc = felt_ray_device(s, nullptr, 0, n, d_org, d_dir, d_out))) return rc;
+    HIPX(hipMemcpyAsync(out, d_out, nf * sizeof(float), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipStreamSynchronize(s->st));
     return FELT_OK;
 }

@@ -47,7 +47,8 @@ struct felt_surface {
     felt::Pool P{};
     std::vector<felt::Mem> mems;     // member geometry (host copy of P.mem)
     hipStream_t st = nullptr;
-    int cap = 0;  // brick pool slots
+    int device = 0;  // the HIP device the handle was created on
+    int cap = 0;     // brick pool slots
     int* free_stack = nullptr;
     felt::Ctr* ctr = nullptr;
     felt::Ctr hc{};  // host copy of the counters
@@ -194,6 +195,9 @@ int felt_save_impl(felt_surface* s, int m, uint8_t* dst, size_t cap, size_t* n);
 int felt_load_impl(felt_surface* s, int m, const uint8_t* src, size_t n);
 int felt_member_clear_impl(felt_surface* s, int m);
 int felt_alloc_listed(felt_surface* s, int grid);
+// felt_surface_ray without device allocation: rays staged through a fixed device array.
+int felt_ray_noalloc_impl(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                          float* out);
 // Average device time of one launch over n rays of member 0, reps launches between two events.
 int felt_time_ray_impl(felt_surface* s, size_t n, const float* origins, const float* dirs, int reps,
                        double* us);

@@ -703,6 +703,7 @@ static int zero_ctr_field(felt_surface* s, int* field) {
 
 int felt_surface_init(felt_surface* s) {
     Geo& G = s->G;
+    HIPX(hipGetDevice(&s->device));
     HIPX(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     const int64_t np = G.np, cap = s->cap;
     HIPX(hipMalloc(&s->P.page, np * sizeof(int)));

@@ -163,6 +163,14 @@ int felt_surface_stats(felt_surface* s, felt_stats* out);
  * finite, start beyond 2^20 or whose direction is not of unit length fail with FELT_E_ARG. */
 int felt_surface_ray(felt_surface* s, size_t n, const float* origins, const float* dirs,
                      float* out);
+/* felt_surface_ray with no device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (allocation-free serving or picking loops, programs
+ * that forbid hipMalloc after start-up).  Rays are staged through one fixed device array per GPU
+ * (room for 65536 rays, 2.25 MiB, held until the process ends) instead of buffers the library
+ * would allocate; n above 65536 fails with FELT_E_ARG.  Every call on a GPU shares that GPU's
+ * array, so calls are serialised by a process-wide lock; results equal felt_surface_ray's. */
+int felt_surface_ray_noalloc(felt_surface* s, size_t n, const float* origins, const float* dirs,
+                             float* out);
 /* The same with origins, dirs and out already in device memory (hipMalloc'd, n*D floats each);
  * returns once the results are written. */
 int felt_surface_ray_device(felt_surface* s, size_t n, const float* origins, const float* dirs,
