@@ -254,27 +254,6 @@ def ray_bench(s, size, n):
 SERVED_SEED = 0xFE1704
 
 
-def served_requests(specs, window_s, rate, seed):
-    """Poisson arrivals: (time, surface, box, speed index, steps) per request."""
-    import math
-    from felt_amd import workloads
-    rng = workloads.SplitMix64(seed)
-    out, t = [], 0.0
-    while True:
-        t += -math.log(1.0 - rng.uniform()) / rate
-        if t >= window_s:
-            break
-        i = rng.randint(0, len(specs) - 1)
-        lo, hi = [], []
-        for s in specs[i].size:
-            side = max(1, int(rng.uniform(0.05, 1.0) * s))
-            a = rng.randint(-(s // 2), s - s // 2 - side)
-            lo.append(a)
-            hi.append(a + side - 1)
-        out.append((t, i, lo, hi, rng.randint(0, 2), rng.randint(1, 50)))
-    return out
-
-
 def run_served(args):
     """Served stream: resident surfaces, Poisson request arrivals, continuous batching."""
     import felt_amd
@@ -284,7 +263,7 @@ def run_served(args):
     specs = workloads.mixed_surfaces(args.served_surfaces, SERVED_SEED, 64, 384, fixed_part=16)
     sizes = [sp.size for sp in specs]
     budget = int(args.budget_frac * 288e9)
-    arrivals = served_requests(specs, args.served_window, args.rate, SERVED_SEED ^ 0x5EED)
+    arrivals = workloads.served_requests(specs, args.served_window, args.rate, SERVED_SEED ^ 0x5EED)
 
     def make(max_active, target_ms=0.0):
         t = time.time()

@@ -182,3 +182,26 @@ def mixed_surfaces(n: int, seed: int, edge_lo: int, edge_hi: int, D: int = 3, L:
             cs.append(tuple(rng.uniform(-edge / 2, edge / 2) for _ in range(D)))
         out.append(SurfaceSpec(D, L, (edge,) * D, (p,) * D, cs, rs, rng.randint(0, 2)))
     return out
+
+
+def served_requests(specs, window_s: float, rate: float, seed: int):
+    """Poisson request arrivals at `rate` per second over `window_s` seconds: (arrival time,
+    surface, box lo, box hi, speed index, steps) per request.  The surface is uniform over specs,
+    the box side U[0.05, 1] of the grid edge per axis placed uniformly inside the grid, the speed
+    uniform over speed_of's three, the step count U{1..50}."""
+    rng = SplitMix64(seed)
+    out, t = [], 0.0
+    while True:
+        t += -math.log(1.0 - rng.uniform()) / rate
+        if t >= window_s:
+            break
+        i = rng.randint(0, len(specs) - 1)
+        lo, hi = [], []
+        for s in specs[i].size:
+            side = max(1, int(rng.uniform(0.05, 1.0) * s))
+            a = rng.randint(-(s // 2), s - s // 2 - side)
+            lo.append(a)
+            hi.append(a + side - 1)
+        out.append((t, i, lo, hi, rng.randint(0, 2), rng.randint(1, 50)))
+    return out
+

@@ -9,16 +9,46 @@ from felt_amd._binding import LIST_DELTA, LIST_ISOGRID, LIST_STATUS_CHANGE
 PHI_TOL = 1e-5
 
 
+def position_keys(pos, size) -> np.ndarray:
+    """Sorted row-major indices (last axis fastest) of world positions in a grid of `size`."""
+    pos = np.asarray(pos, dtype=np.int64).reshape(-1, len(size))
+    key = np.zeros(len(pos), dtype=np.int64)
+    for a, n in enumerate(size):
+        key = key * n + (pos[:, a] + n // 2)
+    return np.sort(key)
+
+
+def keys_diff(name, a, b, size):
+    """A message when the sorted key arrays a and b differ, with a few positions of each side."""
+    if np.array_equal(a, b):
+        return None
+
+    def pos(keys):
+        out = []
+        for k in keys[:5].tolist():
+            p = []
+            for n in reversed(size):
+                p.append(k % n - n // 2)
+                k //= n
+            out.append(tuple(reversed(p)))
+        return out
+    return (f"{name}: gpu-only {pos(np.setdiff1d(a, b))} cpu-only {pos(np.setdiff1d(b, a))}"
+            f" ({len(a)} vs {len(b)})")
+
+
+def layer_diff(g, c, k):
+    return keys_diff(f"layer {k}", position_keys(g.layer_positions(k), g.size),
+                     position_keys(c.layer_positions(k), c.size), g.size)
+
+
 def state_diff(g, c, snapshot=True):
     """List of differences between GPU surface g and oracle surface c (empty when equal)."""
     out = []
     L = g.L
     for k in range(-L, L + 1):
-        gs = {tuple(int(x) for x in p) for p in g.layer_positions(k)}
-        cs = {tuple(int(x) for x in p) for p in c.layer_positions(k)}
-        if gs != cs:
-            out.append(f"layer {k}: gpu-only {sorted(gs - cs)[:5]} cpu-only {sorted(cs - gs)[:5]}"
-                       f" ({len(gs)} vs {len(cs)})")
+        d = layer_diff(g, c, k)
+        if d:
+            out.append(d)
     if not np.array_equal(g.partition_active(), c.partition_active()):
         out.append("active partitions differ")
     for kind, name in ((LIST_ISOGRID, "isogrid"), (LIST_DELTA, "delta"),
@@ -60,9 +90,17 @@ def mesh_sets(polys, surface, ndigits=4):
 def mesh_diff(pg, sg, pc, sc, tol=1e-5, changes=True):
     """Unordered comparison of two Polys meshes, child by child (and of their last march's
     changed partitions when `changes`)."""
+    out = mesh_sets_diff(mesh_sets(pg, sg), mesh_sets(pc, sc), sg.D, tol)
+    if pg.totals() != pc.totals():
+        out.append(f"totals {pg.totals()} vs {pc.totals()}")
+    if changes and set(int(x) for x in pg.changes()) != set(int(x) for x in pc.changes()):
+        out.append("changes() differ")
+    return out
+
+
+def mesh_sets_diff(a, b, D, tol=1e-5):
+    """Unordered comparison of two meshes given as {child: (vertices, simplices)}."""
     out = []
-    D = sg.D
-    a, b = mesh_sets(pg, sg), mesh_sets(pc, sc)
     if set(a) != set(b):
         out.append(f"active children differ: gpu-only {sorted(set(a) - set(b))[:5]} "
                    f"cpu-only {sorted(set(b) - set(a))[:5]}")
@@ -85,8 +123,4 @@ def mesh_diff(pg, sg, pc, sc, tol=1e-5, changes=True):
             return {tuple(tuple(key[i]) for i in tri) for tri in t.tolist()}
         if spx_set(vg, tg) != spx_set(vc, tc):
             out.append(f"child {p}: simplex sets differ")
-    if pg.totals() != pc.totals():
-        out.append(f"totals {pg.totals()} vs {pc.totals()}")
-    if changes and set(int(x) for x in pg.changes()) != set(int(x) for x in pc.changes()):
-        out.append("changes() differ")
     return out
