@@ -16,10 +16,12 @@ struct Changes {
     void reset();
 };
 
-// One Poly::Single child: its mesh lives in the vertex / simplex arenas.
+// One Poly::Single child: its mesh lives in a segment of the vertex / simplex arenas, of
+// capacity vcap / tcap entries, reused in place while a new mesh of the child fits.
 struct PolyKid {
     bool active = false;
     long long voff = 0, nv = 0, toff = 0, nt = 0;
+    long long vcap = 0, tcap = 0;
 };
 
 struct felt_polys {
@@ -30,9 +32,12 @@ struct felt_polys {
     Changes* pending = nullptr;
     Changes* done = nullptr;
     int vw = 6;
-    DBuf<float> varena;
-    DBuf<uint32_t> tarena;
-    long long vtop = 0, ttop = 0;
+    DBuf<float> varena, varena2;      // the arena and the spare a compaction copies into
+    DBuf<uint32_t> tarena, tarena2;
+    long long vtop = 0, ttop = 0;     // bump pointers (entries)
+    DBuf<long long> dst;              // per marched partition: vertex and simplex destinations
+    HBuf<long long> hdst;
+    HBuf<int> hbuf;                   // staging of partition lists and counts
     long long last_spx = 0;
     DBuf<uint8_t> cmask;
     DBuf<uint16_t> own;
@@ -48,6 +53,8 @@ void felt_polys_release(felt_polys* q);
 int felt_polys_notify_impl(felt_polys* q);
 int felt_polys_invalidate_impl(felt_polys* q);
 int felt_polys_march_impl(felt_polys* q);
+// Allocates a full march batch's scratch and a spare arena ahead of time.
+int felt_polys_reserve(felt_polys* q);
 int felt_polys_child_mesh_impl(felt_polys* q, uint64_t p, float* vtx, size_t vcap, uint32_t* spx,
                                size_t scap);
 // Device bytes the polygoniser holds: mesh arenas and march scratch.

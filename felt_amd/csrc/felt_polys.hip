@@ -598,13 +598,14 @@ __global__ void k_m_partstart(const int* start, int nb, const int* vb, const int
     tst[j] = it < M ? tb[it] : tot[1];
 }
 
-// One vertex per owned edge (Poly.hpp:296-346), written at its cube's vertex base + the edge's
-// rank among the cube's owned edges.
+// One vertex per owned edge (Poly.hpp:296-346): its cube's vertex base + the edge's rank among
+// the cube's owned edges numbers it within its partition, whose vertices start at vdst[j].
 template <int D>
 __global__ void __launch_bounds__(TPB) k_m_vert(Geo G, Pool P, const int* parts,
                                                 const int* item_j, const int* item_l,
                                                 const int* vwork, int nvw, const uint16_t* own,
-                                                const int* vb, float* vout) {
+                                                const int* vb, const int* vst,
+                                                const long long* vdst, float* varena) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nvw) return;
     const int x = vwork[t];
@@ -617,17 +618,20 @@ __global__ void __launch_bounds__(TPB) k_m_vert(Geo G, Pool P, const int* parts,
     const unsigned om = own[(size_t)j * G.pv + l];
     const int rank = __popc(om & ((1u << e) - 1u));
     const int vw = D == 3 ? 6 : 2;
-    make_vertex<D>(G, P, V, ga, McT<D>::edge(e, D), vout + (size_t)(vb[i] + rank) * vw);
+    const long long at = vdst[j] + (vb[i] + rank - vst[j]);
+    make_vertex<D>(G, P, V, ga, McT<D>::edge(e, D), varena + (size_t)at * vw);
 }
 
 // Simplices of every band cube (Poly.hpp:198-287), vertex indices local to the partition: a
-// simplex edge's vertex belongs to the owner the count pass recorded for it.
+// simplex edge's vertex belongs to the owner the count pass recorded for it.  Partition j's
+// simplices start at tdst[j].
 template <int D>
 __global__ void __launch_bounds__(TPB) k_m_tri(Geo G, const int* item_j, const int* item_l,
                                                const int* work, int nwork, const uint8_t* cmask,
                                                const uint16_t* own, const uint32_t* ocand,
                                                const int* tb, const int* vleaf, const int* vst,
-                                               uint32_t* tout) {
+                                               const int* tst, const long long* tdst,
+                                               uint32_t* tarena) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nwork) return;
     const int i = work[t];
@@ -638,6 +642,7 @@ __global__ void __launch_bounds__(TPB) k_m_tri(Geo G, const int* item_j, const i
     int lc0[3] = {0, 0, 0};
     rm_coords<D>(G, (uint32_t)l, lc0);
     const int base = vst[j];
+    uint32_t* tout = tarena + (size_t)(tdst[j] + (tb[i] - tst[j])) * D;
     const unsigned oc = ocand[jb + l];
     for (int t = 0; t < ns; t++) {
         for (int k = 0; k < D; k++) {
@@ -650,7 +655,7 @@ __global__ void __launch_bounds__(TPB) k_m_tri(Geo G, const int* item_j, const i
             const int ow = rm_of<D>(G, lc);
             const int idx =
                 vleaf[jb + ow] + __popc((unsigned)own[jb + ow] & ((1u << eo) - 1u)) - base;
-            tout[(size_t)(tb[i] + t) * D + k] = (uint32_t)idx;
+            tout[(size_t)t * D + k] = (uint32_t)idx;
         }
     }
 }
@@ -766,6 +771,11 @@ int felt_polys_init(felt_polys* q) {
 void felt_polys_release(felt_polys* q) {
     q->varena.release();
     q->tarena.release();
+    q->varena2.release();
+    q->tarena2.release();
+    q->dst.release();
+    q->hdst.release();
+    q->hbuf.release();
     q->cmask.release();
     q->own.release();
     q->ocand.release();
@@ -840,73 +850,102 @@ int felt_polys_invalidate_impl(felt_polys* q) {
     return FELT_OK;
 }
 
-// Makes room for `nv` more vertices and `nt` more simplices, compacting live segments into a
-// larger arena when the bump pointer would overflow.
-static int arena_reserve(felt_polys* q, long long nv, long long nt) {
+// Segment capacity for a mesh of n entries: a re-marched partition whose mesh grows a little
+// still fits in place.
+static long long seg_cap(long long n) { return n ? n + n / 4 + 8 : 0; }
+
+// Compacts the arenas into the spare ones: every live segment is copied with a fresh capacity,
+// children whose mesh count is zero (about to be re-marched) give up their segment, and at least
+// extra_v / extra_t entries are left free after the live ones.  The spare arenas only grow, so
+// a compaction in steady state allocates nothing.
+static int arena_compact(felt_polys* q, long long extra_v, long long extra_t) {
     hipStream_t st = q->s->st;
-    if (q->vtop + nv <= (long long)(q->varena.cap / q->vw) &&
-        q->ttop + nt <= (long long)(q->tarena.cap / q->s->G.D))
-        return FELT_OK;
+    const int D = q->s->G.D, vw = q->vw;
     long long live_v = 0, live_t = 0;
-    std::vector<long long> sv, dv, lv, stt, dt, lt;
+    size_t n = 0;
     for (int64_t p : q->alive.list) {
-        PolyKid& k = q->kids[p];
-        if (k.nv) {
-            sv.push_back(k.voff * q->vw);
-            dv.push_back(live_v * q->vw);
-            lv.push_back(k.nv * q->vw);
-        }
-        if (k.nt) {
-            stt.push_back(k.toff * q->s->G.D);
-            dt.push_back(live_t * q->s->G.D);
-            lt.push_back(k.nt * q->s->G.D);
-        }
-        live_v += k.nv;
-        live_t += k.nt;
+        const PolyKid& k = q->kids[p];
+        live_v += seg_cap(k.nv);
+        live_t += seg_cap(k.nt);
+        n += k.nv || k.nt ? 1 : 0;
     }
-    const long long cap_v = std::max<long long>(1024, 2 * (live_v + nv));
-    const long long cap_t = std::max<long long>(1024, 2 * (live_t + nt));
-    DBuf<float> nvb;
-    DBuf<uint32_t> ntb;
+    const long long cap_v = std::max<long long>(4096, 2 * (live_v + extra_v));
+    const long long cap_t = std::max<long long>(4096, 2 * (live_t + extra_t));
     int rc;
-    if ((rc = nvb.ensure((size_t)cap_v * q->vw))) return rc;
-    if ((rc = ntb.ensure((size_t)cap_t * q->s->G.D))) { nvb.release(); return rc; }
-    auto copy = [&](std::vector<long long>& a, std::vector<long long>& b, std::vector<long long>& c,
-                    bool is_f) -> int {
-        if (a.empty()) return FELT_OK;
-        const size_t n = a.size();
-        long long* d = nullptr;
-        HIPX(hipMalloc(&d, 3 * n * sizeof(long long)));
-        HIPX(hipMemcpyAsync(d, a.data(), n * 8, hipMemcpyHostToDevice, st));
-        HIPX(hipMemcpyAsync(d + n, b.data(), n * 8, hipMemcpyHostToDevice, st));
-        HIPX(hipMemcpyAsync(d + 2 * n, c.data(), n * 8, hipMemcpyHostToDevice, st));
-        const int grid = (int)std::min<size_t>(n, 65535);
-        if (is_f)
-            hipLaunchKernelGGL(k_copy_segments, dim3(grid), dim3(TPB), 0, st, d, d + n, d + 2 * n,
-                               (int)n, q->varena.ptr, nvb.ptr);
-        else
-            hipLaunchKernelGGL(k_copy_segments_u, dim3(grid), dim3(TPB), 0, st, d, d + n,
-                               d + 2 * n, (int)n, q->tarena.ptr, ntb.ptr);
-        HIPX(hipStreamSynchronize(st));
-        HIPX(hipFree(d));
-        return FELT_OK;
-    };
-    if ((rc = copy(sv, dv, lv, true))) return rc;
-    if ((rc = copy(stt, dt, lt, false))) return rc;
+    if ((rc = q->varena2.ensure((size_t)cap_v * vw))) return rc;
+    if ((rc = q->tarena2.ensure((size_t)cap_t * D))) return rc;
+    // copy plan: (source, destination, length) of the vertex then the simplex segments
+    if ((rc = q->hdst.ensure(6 * n + 6))) return rc;
+    if ((rc = q->dst.ensure(6 * n + 6))) return rc;
+    long long* h = q->hdst.ptr;
     long long cv = 0, ct = 0;
+    size_t j = 0;
     for (int64_t p : q->alive.list) {
         PolyKid& k = q->kids[p];
+        if (!k.nv && !k.nt) {
+            k.voff = k.toff = k.vcap = k.tcap = 0;
+            continue;
+        }
+        h[j] = k.voff * vw;
+        h[n + j] = cv * vw;
+        h[2 * n + j] = k.nv * vw;
+        h[3 * n + j] = k.toff * D;
+        h[4 * n + j] = ct * D;
+        h[5 * n + j] = k.nt * D;
         k.voff = cv;
         k.toff = ct;
-        cv += k.nv;
-        ct += k.nt;
+        k.vcap = seg_cap(k.nv);
+        k.tcap = seg_cap(k.nt);
+        cv += k.vcap;
+        ct += k.tcap;
+        j++;
     }
-    q->varena.release();
-    q->tarena.release();
-    q->varena = nvb;
-    q->tarena = ntb;
-    q->vtop = live_v;
-    q->ttop = live_t;
+    if (n) {
+        HIPX(hipMemcpyAsync(q->dst.ptr, h, 6 * n * sizeof(long long), hipMemcpyHostToDevice, st));
+        const int grid = (int)std::min<size_t>(n, 65535);
+        const long long* d = q->dst.ptr;
+        hipLaunchKernelGGL(k_copy_segments, dim3(grid), dim3(TPB), 0, st, d, d + n, d + 2 * n,
+                           (int)n, q->varena.ptr, q->varena2.ptr);
+        hipLaunchKernelGGL(k_copy_segments_u, dim3(grid), dim3(TPB), 0, st, d + 3 * n, d + 4 * n,
+                           d + 5 * n, (int)n, q->tarena.ptr, q->tarena2.ptr);
+        HIPX(hipGetLastError());
+        HIPX(hipStreamSynchronize(st));  // the plan's pinned host copy may be reused after this
+    }
+    std::swap(q->varena, q->varena2);
+    std::swap(q->tarena, q->tarena2);
+    q->vtop = cv;
+    q->ttop = ct;
+    return FELT_OK;
+}
+
+// Partitions marched per batch: the batch's per-voxel scratch (corner masks, owners, owner
+// candidates, vertex bases: 11 B per voxel) stays under 768 MB.
+static size_t march_batch(const Geo& G) {
+    const size_t per = (size_t)G.pv * (1 + 2 + 4 + 4);
+    return std::max<size_t>(1, std::min<size_t>(16384, (size_t)(768u << 20) / per));
+}
+
+// Allocates up front what a march of a full batch needs and a spare arena as large as the
+// arena, so that marches in steady state do not allocate (a server calls this at start).
+int felt_polys_reserve(felt_polys* q) {
+    const Geo& G = q->s->G;
+    const size_t nb = std::min<size_t>(march_batch(G), (size_t)G.np);
+    const size_t vox = nb * (size_t)G.pv;
+    const size_t items = vox / 2 + 1;  // band voxels of the batch's partitions (most are fewer)
+    int rc;
+    for (DBuf<int>* d : {&q->dparts, &q->pstart})
+        if ((rc = d->ensure(nb + 1))) return rc;
+    if ((rc = q->vst.ensure(2 * (nb + 1) + 2))) return rc;
+    for (DBuf<int>* d : {&q->item_j, &q->item_l, &q->nv, &q->nt, &q->vbase, &q->tbase, &q->work})
+        if ((rc = d->ensure(items))) return rc;
+    if ((rc = q->cmask.ensure(vox))) return rc;
+    if ((rc = q->own.ensure(vox))) return rc;
+    if ((rc = q->ocand.ensure(vox))) return rc;
+    if ((rc = q->vleaf.ensure(vox))) return rc;
+    if ((rc = q->dst.ensure(2 * nb + 2))) return rc;
+    if ((rc = q->hdst.ensure(2 * nb + 2))) return rc;
+    if ((rc = q->varena2.ensure(q->varena.cap))) return rc;
+    if ((rc = q->tarena2.ensure(q->tarena.cap))) return rc;
     return FELT_OK;
 }
 
@@ -943,8 +982,7 @@ int felt_polys_march_impl(felt_polys* q) {
             q->alive.untrack(p);
         }
     }
-    const size_t per = (size_t)G.pv * (1 + 2 + 4 + 4);
-    const size_t batch = std::max<size_t>(1, std::min<size_t>(16384, (size_t)(768u << 20) / per));
+    const size_t batch = march_batch(G);
     const int bt = std::min(BT, std::max(64, ((G.pv + 63) / 64) * 64));
     q->last_spx = 0;
     for (size_t b0 = 0; b0 < todo.size(); b0 += batch) {
@@ -1002,17 +1040,51 @@ int felt_polys_march_impl(felt_polys* q) {
         HIPX(hipMemcpyAsync(&W, nwork, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPX(hipStreamSynchronize(st));
         const long long NV = hv[nb], NT = hv[(nb + 1) + nb];
-        if ((rc = arena_reserve(q, NV, NT))) return rc;
-        const long long vb0 = q->vtop, tb0 = q->ttop;
-        for (int j = 0; j < nb; j++) {
-            PolyKid& k = q->kids[todo[b0 + j]];
-            k.voff = vb0 + hv[j];
-            k.nv = hv[j + 1] - hv[j];
-            k.toff = tb0 + hv[(nb + 1) + j];
-            k.nt = hv[(nb + 1) + j + 1] - hv[(nb + 1) + j];
+        // Destinations: a partition whose new mesh fits its segment is rewritten in place, the
+        // others get a new segment with slack at the bump pointers (after a compaction when the
+        // arenas are full).
+        auto place = [&](bool dry, long long* need_v, long long* need_t) {
+            *need_v = *need_t = 0;
+            for (int j = 0; j < nb; j++) {
+                PolyKid& k = q->kids[todo[b0 + j]];
+                const long long nv = hv[j + 1] - hv[j];
+                const long long nt = hv[(nb + 1) + j + 1] - hv[(nb + 1) + j];
+                const bool fits = nv <= k.vcap && nt <= k.tcap;
+                if (!fits) {
+                    *need_v += seg_cap(nv);
+                    *need_t += seg_cap(nt);
+                }
+                if (dry) continue;
+                if (!fits) {
+                    k.vcap = seg_cap(nv);
+                    k.tcap = seg_cap(nt);
+                    k.voff = q->vtop;
+                    k.toff = q->ttop;
+                    q->vtop += k.vcap;
+                    q->ttop += k.tcap;
+                }
+                k.nv = nv;
+                k.nt = nt;
+                q->hdst.ptr[j] = k.voff;
+                q->hdst.ptr[nb + j] = k.toff;
+            }
+        };
+        long long need_v, need_t;
+        place(true, &need_v, &need_t);
+        if (q->vtop + need_v > (long long)(q->varena.cap / q->vw) ||
+            q->ttop + need_t > (long long)(q->tarena.cap / G.D)) {
+            long long all_v = 0, all_t = 0;
+            for (int j = 0; j < nb; j++) {
+                all_v += seg_cap(hv[j + 1] - hv[j]);
+                all_t += seg_cap(hv[(nb + 1) + j + 1] - hv[(nb + 1) + j]);
+            }
+            if ((rc = arena_compact(q, all_v, all_t))) return rc;
         }
-        q->vtop += NV;
-        q->ttop += NT;
+        if ((rc = q->hdst.ensure(2 * (size_t)nb + 2))) return rc;
+        if ((rc = q->dst.ensure(2 * (size_t)nb + 2))) return rc;
+        place(false, &need_v, &need_t);
+        HIPX(hipMemcpyAsync(q->dst.ptr, q->hdst.ptr, 2 * (size_t)nb * sizeof(long long),
+                            hipMemcpyHostToDevice, st));
         q->last_spx += NT;
         if (NV > 0 && (rc = q->vwork.ensure((size_t)NV))) return rc;
         prof_begin(s, PK_MARCH_EMIT, W);
@@ -1024,10 +1096,10 @@ int felt_polys_march_impl(felt_polys* q) {
             if (nvw > 0)
                 DISPATCHP(G.D, k_m_vert, dim3((nvw + TPB - 1) / TPB), dim3(TPB), G, s->P,
                           q->dparts.ptr, q->item_j.ptr, q->item_l.ptr, q->vwork.ptr, nvw,
-                          q->own.ptr, q->vbase.ptr, q->varena.ptr + vb0 * q->vw);
+                          q->own.ptr, q->vbase.ptr, vst, q->dst.ptr, q->varena.ptr);
             DISPATCHP(G.D, k_m_tri, dim3((W + TPB - 1) / TPB), dim3(TPB), G, q->item_j.ptr,
                       q->item_l.ptr, q->work.ptr, W, q->cmask.ptr, q->own.ptr, q->ocand.ptr,
-                      q->tbase.ptr, q->vleaf.ptr, vst, q->tarena.ptr + tb0 * G.D);
+                      q->tbase.ptr, q->vleaf.ptr, vst, tst, q->dst.ptr + nb, q->tarena.ptr);
         }
         prof_end(s);
     }
@@ -1054,7 +1126,8 @@ int felt_polys_child_mesh_impl(felt_polys* q, uint64_t p, float* vtx, size_t vca
 }
 
 uint64_t felt_polys_device_bytes(const felt_polys* q) {
-    uint64_t b = q->varena.cap * sizeof(float) + q->tarena.cap * sizeof(uint32_t);
+    uint64_t b = (q->varena.cap + q->varena2.cap) * sizeof(float) +
+                 (q->tarena.cap + q->tarena2.cap) * sizeof(uint32_t) + q->dst.cap * sizeof(long long);
     b += q->cmask.cap + q->own.cap * sizeof(uint16_t) + q->ocand.cap * sizeof(uint32_t);
     b += q->bits.cap + q->dseg.cap * sizeof(long long);
     for (const DBuf<int>* d : {&q->vbase, &q->tbase, &q->tot, &q->dparts, &q->pstart, &q->vst,

@@ -13,8 +13,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -56,11 +58,22 @@ struct Req {
 
 // Tick-time model of the latency target: t = fixed + unit * work, work = sum over in-flight
 // requests of band * box share (about the affected set).  Both terms are running averages.
+// The target bounds the upper tail of tick time, not its mean: admission keeps the prediction
+// times a margin -- the running mean plus 2.5 standard deviations of actual / predicted time --
+// within the target.
 struct TickModel {
     double fixed_ms = 2.0;
     double unit_ms = 2e-7;
+    double r1 = 1.0, r2 = 1.0;  // running mean of actual / predicted, and of its square
     double predict(double work) const { return fixed_ms + unit_ms * work; }
+    double margin() const {
+        const double sd = std::sqrt(std::max(0.0, r2 - r1 * r1));
+        return std::max(1.0, r1 + 2.5 * sd);
+    }
     void observe(double ms, double work) {
+        const double r = ms / std::max(1e-3, predict(work));
+        r1 = 0.95 * r1 + 0.05 * r;
+        r2 = 0.95 * r2 + 0.05 * r * r;
         const double f = std::max(0.0, ms - unit_ms * work);
         fixed_ms = 0.95 * fixed_ms + 0.05 * std::min(f, ms);
         if (work > 0) unit_ms = 0.9 * unit_ms + 0.1 * std::max(0.0, ms - fixed_ms) / work;
@@ -154,7 +167,7 @@ void admit(felt_server* v) {
         }
         const double w = work_of(v, *it);
         if (!v->active.empty() && v->target_ms > 0 &&
-            v->model.predict(work + w) > v->target_ms)
+            v->model.predict(work + w) * v->model.margin() > v->target_ms)
             break;  // the tick would overrun the latency target
         work += w;
         v->busy[m] = 1;
@@ -232,7 +245,20 @@ API int felt_server_start(felt_server* v) {
     if (v->polys) {
         if (int rc = felt_polys_invalidate_impl(v->polys)) return rc;
         if (int rc = felt_polys_march_impl(v->polys)) return rc;
+        if (int rc = felt_polys_reserve(v->polys)) return rc;
     }
+    // Step lists for the largest in-flight set (the max_active largest bands), within what the
+    // budget leaves beside the resident bytes: a tick then grows no list.
+    std::vector<long long> b(v->band);
+    std::sort(b.begin(), b.end(), std::greater<long long>());
+    long long want = 0;
+    for (size_t i = 0; i < b.size() && (int)i < v->max_active; i++) want += b[i];
+    const uint64_t fixed = resident_bytes(v);
+    const uint64_t room = v->budget > fixed ? (v->budget - fixed) / kListBytes : 0;
+    size_t band = (size_t)want;
+    while (band > 0 && felt_list_cap_for(v->s, band) > room) band -= band / 8 + 1;
+    if (band > 0)
+        if (int rc = felt_reserve_lists(v->s, band)) return rc;
     return FELT_OK;
 }
 

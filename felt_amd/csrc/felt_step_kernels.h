@@ -107,6 +107,30 @@ __global__ void k_reserve_count(Geo G, Pool P, const unsigned* src, Ctr* c) {
     if ((threadIdx.x & 63) == 0 && n) atomicAdd(&c->reserve, n);
 }
 
+// The same over the partitions of members ids[blockIdx.y] only.
+template <int D>
+__global__ void k_reserve_count_members(Geo G, Pool P, const unsigned* src, const int* ids,
+                                        Ctr* c) {
+    const Mem& M = P.mem[ids[blockIdx.y]];
+    int n = 0;
+    for (int lp = blockIdx.x * blockDim.x + threadIdx.x; lp < M.np; lp += gridDim.x * blockDim.x) {
+        const int p = M.pbase + lp;
+        if (P.page[p] >= 0) continue;
+        int pc[3] = {0, 0, 0};
+        part_coords<D>(M, lp, pc);
+        bool near = false;
+        for (int a = 0; a < D && !near; a++)
+            for (int sd = -1; sd <= 1 && !near; sd += 2) {
+                const int q = pc[a] + sd;
+                if (q < 0 || q >= M.csize[a]) continue;
+                near = src[p + sd * M.cstr[a]] != 0u;
+            }
+        n += near ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_down(n, o);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&c->reserve, n);
+}
+
 // A global step whose reserve exceeds the free bricks does nothing: its lists are emptied and
 // its rounds switched off before any value is written.
 __global__ void k_reserve_decide(Ctr* c, int* bbase, int NL) {

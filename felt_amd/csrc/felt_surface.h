@@ -22,6 +22,16 @@ struct DBuf {
     void release();
 };
 
+// Grow-only pinned host buffer for transfers on a handle's stream (contents not preserved when
+// it grows).  A transfer from or into it is complete once the stream has been synchronised.
+template <class T>
+struct HBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n);
+    void release();
+};
+
 // Optional per-kernel timing with HIP events on the surface's stream.
 enum ProfKernel { PK_SPEED, PK_APPLY, PK_DIST1, PK_DIST2, PK_FLUSH, PK_EXPAND, PK_COMPACT,
                   PK_AFFECTED, PK_MARCH_COUNT, PK_MARCH_EMIT, PK_DIST1_LATE, PK_DIST2_LATE,
@@ -51,7 +61,8 @@ struct felt_surface {
     int cap = 0;     // brick pool slots
     int* free_stack = nullptr;
     felt::Ctr* ctr = nullptr;
-    felt::Ctr hc{};  // host copy of the counters
+    felt::Ctr hc{};            // host copy of the counters
+    felt::Ctr* hcp = nullptr;  // pinned staging of that copy
     uint8_t* amark = nullptr;
     int *ukey = nullptr, *tflag = nullptr, *tlist = nullptr, *nflag = nullptr, *nlist = nullptr;
     unsigned *dmask = nullptr, *smask = nullptr;
@@ -83,6 +94,9 @@ struct felt_surface {
     DBuf<int> tmp_i;
     DBuf<float> tmp_f;
     DBuf<uint32_t> tmp_u;
+    DBuf<int> step_ids;            // members of the current bounded step (device)
+    std::vector<int> h_step_ids;   // and on the host
+    int n_step_ids = -1;           // -1: every member
     int* bbase = nullptr;           // band segment starts, NL+1 entries (device)
     felt::Speed* d_speed = nullptr;  // per-member speed descriptors read by the speed kernel
     felt::Speed* h_speed = nullptr;  // pinned staging copy (nmem entries)
@@ -178,6 +192,8 @@ int felt_band_total(felt_surface* s, size_t* out);
 int felt_download(felt_surface* s, void* dst, const void* src, size_t bytes);
 // Tracked voxels of members ids[0..n).
 int felt_member_bands(felt_surface* s, const int* ids, int n, std::vector<long long>& out);
+// Step lists sized for a step over `band` listed voxels.
+int felt_reserve_lists(felt_surface* s, size_t band);
 // Capacity the step lists would grow to for a step over `band` listed voxels.
 size_t felt_list_cap_for(const felt_surface* s, size_t band);
 // Device bytes held by the handle.

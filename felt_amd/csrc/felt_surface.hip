@@ -113,7 +113,8 @@ __global__ void k_box_zero(Geo G, Pool P, const int* parts, const int* pbox, int
     for (int j = blockIdx.x; j < nparts; j += gridDim.x) {
         const int p = parts[j];
         const int s = P.page[p];
-        if (s < 0) continue;  // uniform over the block
+        // uniform over the block: inactive partitions and those without zero-layer voxels
+        if (s < 0 || P.pcnt[p * G.NL + G.L] == 0) continue;
         const int4 o = P.slot_org[s];
         const int org[3] = {o.x, o.y, o.z};
         const int* bx = boxes + 6 * pbox[j];
@@ -628,7 +629,7 @@ int prof_collect(felt_surface* s) {
 template <class T>
 int DBuf<T>::ensure(size_t n) {
     if (n <= cap) return FELT_OK;
-    size_t nc = std::max(n, cap + cap / 2);
+    size_t nc = std::max(n, 2 * cap);
     nc = std::max<size_t>(nc, 256);
     if (ptr) HIPX(hipFree(ptr));
     ptr = nullptr;
@@ -643,6 +644,25 @@ void DBuf<T>::release() {
     ptr = nullptr;
     cap = 0;
 }
+template <class T>
+int HBuf<T>::ensure(size_t n) {
+    if (n <= cap) return FELT_OK;
+    const size_t nc = std::max<size_t>(std::max(n, cap + cap / 2), 256);
+    if (ptr) HIPX(hipHostFree(ptr));
+    ptr = nullptr;
+    cap = 0;
+    HIPX(hipHostMalloc((void**)&ptr, nc * sizeof(T), hipHostMallocDefault));
+    cap = nc;
+    return FELT_OK;
+}
+template <class T>
+void HBuf<T>::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+}
+template struct HBuf<int>;
+template struct HBuf<long long>;
 template struct DBuf<uint32_t>;
 template struct DBuf<int8_t>;
 template struct DBuf<int16_t>;
@@ -677,13 +697,14 @@ template struct DBuf<uint8_t>;
     } while (0)
 
 static int read_ctr(felt_surface* s) {
-    HIPX(hipMemcpyAsync(&s->hc, s->ctr, sizeof(Ctr), hipMemcpyDeviceToHost, s->st));
+    HIPX(hipMemcpyAsync(s->hcp, s->ctr, sizeof(Ctr), hipMemcpyDeviceToHost, s->st));
     if (s->prof.on && s->bbase) {
         s->hbase.resize(s->G.NL + 1);
         HIPX(hipMemcpyAsync(s->hbase.data(), s->bbase, (s->G.NL + 1) * sizeof(int),
                             hipMemcpyDeviceToHost, s->st));
     }
     HIPX(hipStreamSynchronize(s->st));
+    s->hc = *s->hcp;
     // an error is reported once: the next read starts clean (a refused step is recoverable)
     if (s->hc.err) HIPX(hipMemsetAsync(&s->ctr->err, 0, sizeof(int), s->st));
     if (s->hc.err == 1) return felt_fail(FELT_E_NOMEM, "brick pool exhausted");
@@ -741,6 +762,7 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->bbase, (G.NL + 2) * sizeof(int)));
     HIPX(hipMalloc(&s->d_speed, G.nmem * sizeof(Speed)));
     HIPX(hipHostMalloc((void**)&s->h_speed, G.nmem * sizeof(Speed), hipHostMallocDefault));
+    HIPX(hipHostMalloc((void**)&s->hcp, sizeof(Ctr), hipHostMallocDefault));
     std::memset(s->h_speed, 0, G.nmem * sizeof(Speed));
     if (s->blk_cnt.ensure((size_t)np * G.NL + G.NL + 1)) return FELT_E_DEVICE;
     if (s->blk_off.ensure((size_t)np * G.NL)) return FELT_E_DEVICE;
@@ -786,6 +808,8 @@ void felt_surface_release(felt_surface* s) {
     }
     if (s->h_speed) (void)hipHostFree(s->h_speed);
     s->h_speed = nullptr;
+    if (s->hcp) (void)hipHostFree(s->hcp);
+    s->hcp = nullptr;
     void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
                     s->P.slot_org, (void*)s->P.mem, (void*)s->P.part_mem, s->amark, s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag,
                     s->nlist, s->dmask, s->smask, s->act, s->bbase, s->d_speed};
@@ -816,6 +840,7 @@ void felt_surface_release(felt_surface* s) {
     s->tmp_i.release();
     s->tmp_f.release();
     s->tmp_u.release();
+    s->step_ids.release();
     if (s->st) (void)hipStreamDestroy(s->st);
     s->st = nullptr;
 }
@@ -835,6 +860,11 @@ size_t felt_list_cap_for(const felt_surface* s, size_t band) {
     if (band + 64 <= s->list_cap) return s->list_cap;
     return std::max<size_t>(band + band / 4 + 1024, s->list_cap + s->list_cap / 2);
 }
+
+static int ensure_step_capacity(felt_surface* s, size_t band);
+
+// Step lists sized for `band` listed voxels ahead of time (servers reserve at start).
+int felt_reserve_lists(felt_surface* s, size_t band) { return ensure_step_capacity(s, band); }
 
 static int ensure_step_capacity(felt_surface* s, size_t band) {
     if (band + 64 <= s->list_cap) return FELT_OK;
@@ -1403,8 +1433,16 @@ int felt_update_end_impl(felt_surface* s) {
     // inactive neighbours of those
     LAUNCH(k_dmask_lu, dim3(bfs_grid), dim3(TPB), G, s->P, A, lu_span(s, 0), s->dmask);
     HIPX(hipMemsetAsync(&s->ctr->reserve, 0, sizeof(int), s->st));
-    DISPATCH(G.D, k_reserve_count, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P,
-             (const unsigned*)s->dmask, s->ctr);
+    if (s->n_step_ids > 0 && s->n_step_ids <= 65535) {
+        // only the stepped members' partitions can be activated
+        int most = 0;
+        for (int j = 0; j < s->n_step_ids; j++) most = std::max(most, s->mems[s->h_step_ids[j]].np);
+        DISPATCH(G.D, k_reserve_count_members, dim3(std::min(256, blocks_for(most)), s->n_step_ids),
+                 dim3(TPB), G, s->P, (const unsigned*)s->dmask, s->step_ids.ptr, s->ctr);
+    } else {
+        DISPATCH(G.D, k_reserve_count, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P,
+                 (const unsigned*)s->dmask, s->ctr);
+    }
     // the affected set's size sizes every grid of the rounds below
     if ((rc = read_ctr(s))) return rc;
     const int an = s->hc.lu_span[0][1];
@@ -1486,6 +1524,11 @@ static int boxes_list(felt_surface* s, int n, const int* ids, const int32_t* lo,
         if ((rc = ensure_step_capacity(s, (size_t)sum))) return rc;
     }
     if ((rc = felt_update_start_impl(s))) return rc;
+    if ((rc = s->step_ids.ensure((size_t)n + 1))) return rc;
+    s->h_step_ids.assign(ids, ids + n);
+    s->n_step_ids = n;
+    HIPX(hipMemcpyAsync(s->step_ids.ptr, s->h_step_ids.data(), (size_t)n * sizeof(int),
+                        hipMemcpyHostToDevice, s->st));
     std::vector<int> parts, pbox, boxes;
     for (int j = 0; j < n; j++) {
         const Mem& M = s->mems[ids[j]];
@@ -1565,6 +1608,7 @@ int felt_delta_impl(felt_surface* s, size_t m, const int32_t* gpos, const float*
 
 int felt_update_end_public(felt_surface* s) {
     int rc;
+    s->n_step_ids = -1;
     if (!s->in_update) return felt_fail(FELT_E_STATE, "update_end() without update_start()");
     if ((rc = size_for_band(s))) return rc;
     if ((rc = ensure_step_capacity(s, s->band_now + s->pending_deltas.size()))) return rc;

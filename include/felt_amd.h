@@ -430,8 +430,12 @@ int felt_server_polys(felt_server* v, felt_polys** out);
 /* Reads every surface's band size and, when polygonising, marches every surface once. */
 int felt_server_start(felt_server* v);
 /* Latency target for admission: a request joins a tick only while a running model of tick
- * time (a fixed part plus a cost per band voxel inside the in-flight boxes) stays within
- * tick_ms; a request is always admitted into an empty tick.  0 (the default) disables it. */
+ * time (a fixed part plus a cost per band voxel inside the in-flight boxes), times a margin for
+ * its error (the running mean plus 2.5 standard deviations of actual / predicted tick time),
+ * stays within tick_ms, so that the target bounds the tail of tick time rather than its mean;
+ * a request is always admitted into an empty tick.  0 (the default) disables it.
+ * felt_server_start also reserves the step lists the in-flight set can need (within the
+ * budget), the march's scratch and a spare mesh arena, so that ticks do not allocate. */
 int felt_server_set_target(felt_server* v, double tick_ms);
 /* Queues n requests; every one is checked first, so a call queues all of them or none. */
 int felt_server_submit(felt_server* v, const felt_request* r, size_t n);

@@ -121,11 +121,13 @@ def test_served_budget_defers_admission():
     from felt_amd.server import Server
     sizes = [(32, 32, 32), (48, 48, 48), (32, 48, 32), (48, 48, 48)]
     part = (16, 16, 16)
-    probe, _ = setup(sizes, part, 4, 4 << 30, 8)
+    # a server admitting one request at a time reserves step lists for the largest band: its
+    # device bytes are the resident bytes plus those lists; half a band more is the budget
+    probe, _ = setup(sizes, part, 4, 4 << 30, 1)
     resident = probe.stats()["device_bytes"]
     bands = [sum(probe.batch.layer_size(i, k) for k in range(-3, 4)) for i in range(len(sizes))]
     probe.close()
-    budget = resident + int((max(bands) * 1.25 + 1024) * 45 * 1.5)
+    budget = resident + int(max(bands) * 0.5 * 45)
     srv, fields = setup(sizes, part, 4, budget, 8)
     reqs = make_requests(sizes, 16, 12, 3)
     srv.submit(reqs)
