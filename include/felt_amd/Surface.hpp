@@ -57,6 +57,49 @@ VecDf<D> constant_f(float v) {
     a.fill(v);
     return a;
 }
+// Calls q(buffer, capacity, &n) for positions of D ints, once more with a larger buffer when
+// the first call reports the count it needed.
+template <int D, class Q>
+std::vector<int32_t> query_positions(Q&& q) {
+    std::size_t n = 0;
+    std::vector<int32_t> out(64 * D);
+    int rc = q(out.data(), out.size() / D, &n);
+    if (rc == FELT_E_CAPACITY) {
+        out.resize((n + 1) * D);
+        rc = q(out.data(), out.size() / D, &n);
+    }
+    check(rc);
+    out.resize(n * D);
+    return out;
+}
+inline PosIdxList partition_list(felt_surface* h, int kind, TupleIdx k) {
+    std::size_t n = 0;
+    std::vector<uint64_t> buf(64);
+    int rc = felt_surface_partition_list(h, kind, k, buf.data(), buf.size(), &n);
+    if (rc == FELT_E_CAPACITY) {
+        buf.resize(n + 1);
+        rc = felt_surface_partition_list(h, kind, k, buf.data(), buf.size(), &n);
+    }
+    check(rc);
+    return PosIdxList(buf.begin(), buf.begin() + std::ptrdiff_t(n));
+}
+// Row-major index of pos in a grid of `size` at `offset`, the last axis fastest (Util.hpp:23-40),
+// and its inverse (Util.hpp:56-80).
+template <int D>
+PosIdx index_of(const VecDi<D>& pos, const VecDi<D>& size, const VecDi<D>& offset) {
+    int64_t idx = 0;
+    for (int a = 0; a < D; a++) idx = idx * size[a] + (pos[a] - offset[a]);
+    return PosIdx(idx);
+}
+template <int D>
+VecDi<D> position_of(PosIdx idx, const VecDi<D>& size, const VecDi<D>& offset) {
+    VecDi<D> pos{};
+    for (int a = D - 1; a >= 0; a--) {
+        pos[a] = int32_t(idx % PosIdx(size[a])) + offset[a];
+        idx /= PosIdx(size[a]);
+    }
+    return pos;
+}
 }  // namespace detail
 
 // A functor evaluated on the GPU.  update(device_fn(fn)) and update(lo, hi, device_fn(fn))
@@ -90,12 +133,20 @@ struct DeviceSpeed;
 template <int D>
 class IsoGridView {
 public:
-    explicit IsoGridView(felt_surface* h, const VecDi<D>& size) : m_h(h), m_size(size) {
+    IsoGridView(felt_surface* h, const VecDi<D>& size, const VecDi<D>& child_size, int L)
+        : m_h(h), m_size(size), m_child_size(child_size), m_L(L) {
         for (int a = 0; a < D; a++) m_offset[a] = -1 * size[a] / 2;
         std::size_t st = 1;
         for (int a = D - 1; a >= 0; a--) {
             m_stride[a] = st;
             st *= std::size_t(size[a]);
+        }
+        int32_t cs[3] = {1, 1, 1};
+        uint64_t np = 0;
+        detail::check(felt_surface_geometry(h, cs, &np));
+        for (int a = 0; a < D; a++) {
+            m_children_size[a] = cs[a];
+            m_children_offset[a] = m_offset[a] / child_size[a];  // truncating, as C++ int division
         }
     }
     const VecDi<D>& size() const { return m_size; }
@@ -300,6 +351,134 @@ public:
         detail::check(felt_surface_snapshot(m_h, out.data(), out.size()));
         return out;
     }
+
+    // ---- partitions (Partitioned.hpp:160-166; PartitionedMixin.hpp:57-124, 177-244) ----------
+    // Each child (partition) has child_size(); child c sits at (c - children().offset()) *
+    // child_size() + offset() in world space.
+    class ChildView {
+    public:
+        ChildView(const IsoGridView* g, PosIdx pos_idx) : m_g(g), m_pos_idx(pos_idx) {
+            const VecDi<D> c = detail::position_of<D>(pos_idx, g->m_children_size,
+                                                      g->m_children_offset);
+            for (int a = 0; a < D; a++)
+                m_offset[a] = (c[a] - g->m_children_offset[a]) * g->m_child_size[a] +
+                              g->m_offset[a];
+        }
+        const VecDi<D>& size() const { return m_g->m_child_size; }
+        const VecDi<D>& offset() const { return m_offset; }
+        bool inside(const VecDi<D>& pos) const {
+            for (int a = 0; a < D; a++)
+                if (pos[a] < m_offset[a] || pos[a] >= m_offset[a] + size()[a]) return false;
+            return true;
+        }
+        PosIdx index(const VecDi<D>& pos) const { return detail::index_of<D>(pos, size(), m_offset); }
+        VecDi<D> index(PosIdx idx) const { return detail::position_of<D>(idx, size(), m_offset); }
+        // Whether the partition holds storage (Tracked.hpp:16-79); an inactive one reads as its
+        // background.
+        bool is_active() const { return m_g->children().lookup().is_active(m_pos_idx); }
+        Distance get(const VecDi<D>& pos) const { return m_g->get(pos); }
+        // Leaf PosIdx (in this child) of the voxels in tracking list k = layer + L.
+        PosIdxList list(TupleIdx k) const {
+            const std::vector<int32_t> pos = m_g->layer_positions(int(k) - m_g->m_L);
+            PosIdxList out;
+            for (std::size_t i = 0; i * D < pos.size(); i++) {
+                VecDi<D> p{};
+                for (int a = 0; a < D; a++) p[a] = pos[i * D + a];
+                if (inside(p)) out.push_back(index(p));
+            }
+            return out;
+        }
+
+    private:
+        const IsoGridView* m_g;
+        PosIdx m_pos_idx;
+        VecDi<D> m_offset{};
+    };
+    class ChildrenView {
+    public:
+        explicit ChildrenView(const IsoGridView* g) : m_g(g) {}
+        const VecDi<D>& size() const { return m_g->m_children_size; }
+        const VecDi<D>& offset() const { return m_g->m_children_offset; }
+        bool inside(const VecDi<D>& pos) const {
+            for (int a = 0; a < D; a++)
+                if (pos[a] < offset()[a] || pos[a] >= offset()[a] + size()[a]) return false;
+            return true;
+        }
+        PosIdx index(const VecDi<D>& pos) const { return detail::index_of<D>(pos, size(), offset()); }
+        VecDi<D> index(PosIdx idx) const { return detail::position_of<D>(idx, size(), offset()); }
+        ChildView get(PosIdx pos_idx) const { return ChildView(m_g, pos_idx); }
+        ChildView get(const VecDi<D>& pos) const { return ChildView(m_g, index(pos)); }
+        // The children grid's tracking lists: partitions with voxels in list k.
+        struct Lookup {
+            const IsoGridView* g;
+            PosIdxList list(TupleIdx k) const {
+                return detail::partition_list(g->m_h, FELT_LIST_ISOGRID, k);
+            }
+            bool is_tracked(PosIdx pos_idx, TupleIdx k) const {
+                uint64_t n = 0;
+                detail::check(felt_surface_partition_layer_size(g->m_h, pos_idx,
+                                                                int(k) - g->m_L, &n));
+                return n != 0;
+            }
+            bool is_active(PosIdx pos_idx) const {
+                std::vector<uint8_t> act(g->num_children());
+                detail::check(felt_surface_partition_active(g->m_h, act.data(), act.size()));
+                return pos_idx < act.size() && act[pos_idx] != 0;
+            }
+        };
+        Lookup lookup() const { return Lookup{m_g}; }
+
+    private:
+        const IsoGridView* m_g;
+    };
+    ChildrenView children() const { return ChildrenView(this); }
+    const VecDi<D>& child_size() const { return m_child_size; }
+    // Partition holding a leaf position (PartitionedMixin.hpp:221-244).
+    VecDi<D> pos_child(const VecDi<D>& pos_leaf) const {
+        VecDi<D> c{};
+        for (int a = 0; a < D; a++)
+            c[a] = (pos_leaf[a] - m_offset[a]) / m_child_size[a] + m_children_offset[a];
+        return c;
+    }
+    PosIdx pos_idx_child(const VecDi<D>& pos_leaf) const {
+        return detail::index_of<D>(pos_child(pos_leaf), m_children_size, m_children_offset);
+    }
+    // fn(pos) for every voxel of tracking list k = layer + L, partition by partition in the
+    // order of children().lookup().list(k) (PartitionedMixin.hpp:177-190); the order within a
+    // partition is unspecified, as the reference's tracking order is.
+    template <class Fn>
+    void leafs(TupleIdx k, Fn&& fn) const {
+        const PosIdxList parts = children().lookup().list(k);
+        const std::vector<int32_t> pos = layer_positions(int(k) - m_L);
+        std::vector<std::pair<PosIdx, std::size_t>> order;
+        order.reserve(pos.size() / D);
+        for (std::size_t i = 0; i * D < pos.size(); i++) {
+            VecDi<D> p{};
+            for (int a = 0; a < D; a++) p[a] = pos[i * D + a];
+            order.emplace_back(pos_idx_child(p), i);
+        }
+        std::sort(order.begin(), order.end());
+        for (PosIdx pc : parts) {
+            auto it = std::lower_bound(order.begin(), order.end(), std::make_pair(pc, std::size_t(0)));
+            for (; it != order.end() && it->first == pc; ++it) {
+                VecDi<D> p{};
+                for (int a = 0; a < D; a++) p[a] = pos[it->second * D + a];
+                fn(p);
+            }
+        }
+    }
+    std::size_t num_children() const {
+        std::size_t n = 1;
+        for (int a = 0; a < D; a++) n *= std::size_t(m_children_size[a]);
+        return n;
+    }
+    // Positions tracked in layer `layer` (-L..L), D ints each, order unspecified.
+    std::vector<int32_t> layer_positions(int layer) const {
+        return detail::query_positions<D>([&](int32_t* out, std::size_t cap, std::size_t* n) {
+            return felt_surface_layer_positions(m_h, layer, out, cap, n);
+        });
+    }
+
     // Reads from one dense copy until unpin(): for host functors during an update.
     void pin() {
         m_cache = snapshot();
@@ -313,7 +492,8 @@ public:
 private:
     static float sum(const float* v) { return D == 3 ? v[0] + (v[1] + v[2]) : v[0] + v[1]; }
     felt_surface* m_h;
-    VecDi<D> m_size, m_offset{};
+    VecDi<D> m_size, m_offset{}, m_child_size{}, m_children_size{}, m_children_offset{};
+    int m_L;
     std::array<std::size_t, D> m_stride{};
     std::vector<Distance> m_cache;
     bool m_pinned = false;
@@ -336,7 +516,7 @@ public:
         felt_surface* h = nullptr;
         detail::check(felt_surface_create(D, L, size.data(), size_partition.data(), &h));
         m_h.reset(h);
-        m_iso.reset(new IsoGrid(h, size));
+        m_iso.reset(new IsoGrid(h, size, size_partition, L));
     }
 
     // Write the isogrid to a stream in the reference's on-disk format (a cereal binary archive).
@@ -484,27 +664,10 @@ private:
     }
     template <class Q>
     static std::vector<int32_t> query(Q&& q) {
-        std::size_t n = 0;
-        std::vector<int32_t> out(64 * D);
-        int rc = q(out.data(), out.size() / D, &n);
-        if (rc == FELT_E_CAPACITY) {
-            out.resize((n + 1) * D);
-            rc = q(out.data(), out.size() / D, &n);
-        }
-        detail::check(rc);
-        out.resize(n * D);
-        return out;
+        return detail::query_positions<D>(std::forward<Q>(q));
     }
     PosIdxList parts(int kind, TupleIdx k) const {
-        std::size_t n = 0;
-        std::vector<uint64_t> buf(64);
-        int rc = felt_surface_partition_list(m_h.get(), kind, k, buf.data(), buf.size(), &n);
-        if (rc == FELT_E_CAPACITY) {
-            buf.resize(n + 1);
-            rc = felt_surface_partition_list(m_h.get(), kind, k, buf.data(), buf.size(), &n);
-        }
-        detail::check(rc);
-        return PosIdxList(buf.begin(), buf.begin() + std::ptrdiff_t(n));
+        return detail::partition_list(m_h.get(), kind, k);
     }
     static VecDi<D> at(const std::vector<int32_t>& pos, std::size_t i) {
         VecDi<D> v{};
@@ -553,7 +716,7 @@ private:
 
     Surface(std::unique_ptr<felt_surface, Del> h, const VecDi<D>& size, const VecDi<D>& part)
         : m_size(size), m_part(part), m_h(std::move(h)) {
-        m_iso.reset(new IsoGrid(m_h.get(), size));
+        m_iso.reset(new IsoGrid(m_h.get(), size, part, L));
     }
 
     VecDi<D> m_size, m_part;

@@ -177,5 +177,57 @@ int main() {
         CHECK(threw);
     }
     std::printf("surface_api: all checks passed\n");
+    {
+        // Partition geometry, as test_Grid.cpp:1188-1201 and 1205-1215 state it for a 9x9x9 grid
+        // in 3x3x3 partitions: child (-1,-1,-1) at offset (-4,-4,-4), child (1,1,1) at (2,2,2),
+        // and the partitions holding (1,-4,-1), (2,-3,-2), (3,-2,-3), (4,-1,-4).
+        using S = Surface<3, 2>;
+        S s(VecDi<3>{9, 9, 9}, VecDi<3>{3, 3, 3});
+        const S::IsoGrid& g = s.isogrid();
+        CHECK(g.child_size() == (VecDi<3>{3, 3, 3}));
+        CHECK(g.children().size() == (VecDi<3>{3, 3, 3}));
+        CHECK(g.children().offset() == (VecDi<3>{-1, -1, -1}));
+        CHECK(g.children().get(VecDi<3>{-1, -1, -1}).offset() == (VecDi<3>{-4, -4, -4}));
+        CHECK(g.children().get(VecDi<3>{1, 1, 1}).offset() == (VecDi<3>{2, 2, 2}));
+        CHECK(g.children().get(VecDi<3>{1, 1, 1}).size() == (VecDi<3>{3, 3, 3}));
+        CHECK(g.pos_child(VecDi<3>{1, -4, -1}) == (VecDi<3>{0, -1, 0}));
+        CHECK(g.pos_child(VecDi<3>{2, -3, -2}) == (VecDi<3>{1, -1, -1}));
+        CHECK(g.pos_child(VecDi<3>{3, -2, -3}) == (VecDi<3>{1, -1, -1}));
+        CHECK(g.pos_child(VecDi<3>{4, -1, -4}) == (VecDi<3>{1, 0, -1}));
+        CHECK(g.pos_idx_child(VecDi<3>{4, -1, -4}) == g.children().index(VecDi<3>{1, 0, -1}));
+        CHECK(g.children().index(g.children().index(PosIdx(17))) == 17);
+        // the seed's partition is the only active one and holds the zero layer's one voxel
+        s.seed(VecDi<3>{0, 0, 0});
+        const PosIdx centre = g.pos_idx_child(VecDi<3>{0, 0, 0});
+        CHECK(centre == 13);
+        const auto child = g.children().get(centre);
+        CHECK(child.is_active() && !g.children().get(PosIdx(0)).is_active());
+        CHECK(child.list(S::layer_idx(0)).size() == 1);
+        CHECK(child.index(child.list(S::layer_idx(0))[0]) == (VecDi<3>{0, 0, 0}));
+        CHECK(g.children().lookup().list(S::layer_idx(0)) == PosIdxList{centre});
+        CHECK(g.children().lookup().is_tracked(centre, S::layer_idx(1)));
+        // leafs(k, fn) visits each voxel of the layer once, partition by partition
+        for (int k = 0; k < 5; k++) {
+            std::set<std::array<int, 3>> seen;
+            PosIdx last = 0;
+            std::size_t n = 0, runs = 0;
+            g.leafs(TupleIdx(k), [&](const VecDi<3>& p) {
+                seen.insert({p[0], p[1], p[2]});
+                const PosIdx pc = g.pos_idx_child(p);
+                if (n == 0 || pc != last) runs++;
+                last = pc;
+                n++;
+            });
+            CHECK(n == seen.size() && n == s.layer_size(k - 2));
+            CHECK(runs == g.children().lookup().list(TupleIdx(k)).size());
+        }
+    }
+    {
+        // test_Surface.cpp:35-45: a 7x7 isogrid in 3x3 partitions has 9 children of 3x3
+        Surface<2, 2> s(VecDi<2>{7, 7}, VecDi<2>{3, 3});
+        CHECK(s.isogrid().num_children() == 9);
+        CHECK(s.isogrid().children().get(VecDi<2>{0, 0}).size() == (VecDi<2>{3, 3}));
+        CHECK(s.isogrid().get(VecDi<2>{0, 0}) == 3);
+    }
     return 0;
 }

@@ -41,8 +41,9 @@ def layer_diff(g, c, k):
                      position_keys(c.layer_positions(k), c.size), g.size)
 
 
-def state_diff(g, c, snapshot=True):
-    """List of differences between GPU surface g and oracle surface c (empty when equal)."""
+def state_diff(g, c, snapshot=True, exact=False):
+    """List of differences between GPU surface g and oracle surface c (empty when equal).  With
+    `exact` the snapshots must be bit-identical rather than within PHI_TOL."""
     out = []
     L = g.L
     for k in range(-L, L + 1):
@@ -66,6 +67,10 @@ def state_diff(g, c, snapshot=True):
             i = int(np.nanargmax(d))
             out.append(f"phi max diff {float(np.nanmax(d))} at flat {i}: gpu {sg.reshape(-1)[i]}"
                        f" cpu {sc.reshape(-1)[i]}")
+        elif exact and not np.array_equal(sg.view(np.uint32), sc.view(np.uint32)):
+            i = int(np.argmax(sg.view(np.uint32) != sc.view(np.uint32)))
+            out.append(f"phi not bit-identical at flat {i}: gpu {sg.reshape(-1)[i]!r} "
+                       f"cpu {sc.reshape(-1)[i]!r}")
     sa, sb = g.stats(), c.stats()
     if (sa.active_isogrid_partitions, sa.active_delta_partitions) != \
             (sb.active_isogrid_partitions, sb.active_delta_partitions):

@@ -74,6 +74,8 @@ def test_single_large_surface_quarter_scale():
         d = parity.mesh_sets_diff(m, c32["mesh"][step], 3)
         assert not d, f"mesh at step {step}: {d[:4]}"
     assert worst64 < 1e-4
+    # both sides round every operation the same way: the distances are bit-identical
+    assert identical == len(phis), f"phi bit-identical at {identical}/{len(phis)} checks only"
     print(f"[c2/4] 200 steps: layers and lists equal every step, phi bit-identical at "
           f"{identical}/{len(phis)} checks, max |phi - phi_fp64| = {worst64:.3g}", flush=True)
 
