@@ -99,6 +99,15 @@ def run_mixed(args):
     for _ in range(args.warmup):
         for bt, speeds in batches:
             bt.update(speeds)
+    # launch shapes tuned on the warmed-up state (felt_batch_autotune), then one more untimed
+    # step so the step graphs are captured with them
+    tuned = {}
+    if args.tune_reps > 0:
+        t = time.time()
+        for (p, _), (bt, speeds) in zip(sorted(groups.items()), batches):
+            tuned[f"{p}^3"] = bt.autotune(args.tune_reps)
+            bt.update(speeds)
+        log(f"[bench] tuned in {time.time() - t:.1f}s: {tuned}")
     lat, ups = [], 0
     t_all = time.perf_counter()
     for step in range(args.steps):
@@ -127,6 +136,8 @@ def run_mixed(args):
                        f"0x{MIXED_SEED:X}; one update(fn) of every surface per step",
            "surfaces": len(specs),
            "batches": {f"{p}^3": len(ids) for p, ids in sorted(groups.items())},
+           "launch_config": tuned or "default (256 threads per block, grid cap 4096, "
+                                     "band compaction one wavefront per 512 voxels)",
            "device_bytes": int(sum(m[0] for m in mem)),
            "bricks_in_use": int(sum(m[1] for m in mem))}
     if args.plain_steps <= 0:
@@ -446,6 +457,8 @@ def main():
     ap.add_argument("--part", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--pass-reps", type=int, default=20)
+    ap.add_argument("--tune-reps", type=int, default=5,
+                    help="felt_batch_autotune replays per candidate (0: default launch shapes)")
     ap.add_argument("--rays", type=int, default=1 << 20, help="rays cast on the single surface "
                     "(0 skips)")
     ap.add_argument("--no-cpu", action="store_true")

@@ -71,6 +71,15 @@ class felt_stats(C.Structure):
                 ("nan_events", C.c_uint64)]
 
 
+class felt_launch_config(C.Structure):
+    _fields_ = [("threads_per_block", C.c_int32), ("grid_cap", C.c_int32),
+                ("voxels_per_lane", C.c_int32), ("threads_per_brick", C.c_int32),
+                ("us_default", C.c_double), ("us_tuned", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def speed_constant(a: float) -> felt_speed:
     return felt_speed(SPEED_CONSTANT, a, 0.0, 0.0, 0, 0, 0)
 
@@ -142,6 +151,8 @@ class Binding:
             "surface_sync": (C.c_int, [_P]),
             "surface_step_info": (C.c_int, [_P, _U64P, _U64P]),
             "surface_set_graph": (C.c_int, [_P, C.c_int]),
+            "surface_autotune": (C.c_int, [_P, C.c_int, C.POINTER(felt_launch_config)]),
+            "surface_set_launch_config": (C.c_int, [_P, C.POINTER(felt_launch_config)]),
             "surface_load_spheres": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double)]),
             "surface_time_pass": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
@@ -177,6 +188,8 @@ class Binding:
         for name, (res, args) in sig.items():
             if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info",
                                                             "surface_set_graph",
+                                                            "surface_autotune",
+                                                            "surface_set_launch_config",
                                                             "surface_time_pass",
                                                             "surface_load_spheres",
                                                             "surface_ray",
@@ -486,6 +499,18 @@ class Surface:
         one by one."""
         if "surface_set_graph" in self.b.fn:
             self.b.call("surface_set_graph", self.h, int(bool(enable)))
+
+    def autotune(self, reps: int = 10) -> dict:
+        """Times launch shapes on the current state and keeps the fastest
+        (felt_surface_autotune); returns the chosen felt_launch_config as a dict."""
+        out = felt_launch_config()
+        self.b.call("surface_autotune", self.h, int(reps), C.byref(out))
+        return out.as_dict()
+
+    def set_launch_config(self, threads_per_block: int, grid_cap: int,
+                          voxels_per_lane: int) -> None:
+        cfg = felt_launch_config(threads_per_block, grid_cap, voxels_per_lane, 0, 0.0, 0.0)
+        self.b.call("surface_set_launch_config", self.h, C.byref(cfg))
 
     def step_info(self):
         """(voxel updates, rounds) of the last update call."""

@@ -12,7 +12,7 @@ from typing import Sequence
 
 import numpy as np
 
-from ._binding import FeltError, felt_speed, felt_stats
+from ._binding import FeltError, felt_launch_config, felt_speed, felt_stats
 
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
@@ -48,6 +48,8 @@ _SIGS = {
     "felt_batch_time_pass": [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
                              C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "felt_batch_sync": [_P],
+    "felt_batch_autotune": [_P, C.c_int, C.POINTER(felt_launch_config)],
+    "felt_batch_set_launch_config": [_P, C.POINTER(felt_launch_config)],
     "felt_batch_polys_create": [_P, C.POINTER(_P)],
     "felt_batch_polys_changes": [_P, C.c_int32, _U64P, _SZ, _SZP],
     "felt_batch_polys_child_info": [_P, C.c_int32, C.c_uint64, C.POINTER(C.c_int), _U64P, _U64P],
@@ -232,6 +234,18 @@ class Batch:
         self.api("felt_batch_time_pass", self.h, int(pass_id), int(reps), C.byref(us),
                  C.byref(items), C.byref(n))
         return float(us.value), float(items.value), int(n.value)
+
+    def autotune(self, reps: int = 10) -> dict:
+        """felt_batch_autotune: times launch shapes on the members' current state and keeps the
+        fastest; returns the chosen felt_launch_config as a dict."""
+        out = felt_launch_config()
+        self.api("felt_batch_autotune", self.h, int(reps), C.byref(out))
+        return out.as_dict()
+
+    def set_launch_config(self, threads_per_block: int, grid_cap: int,
+                          voxels_per_lane: int) -> None:
+        cfg = felt_launch_config(threads_per_block, grid_cap, voxels_per_lane, 0, 0.0, 0.0)
+        self.api("felt_batch_set_launch_config", self.h, C.byref(cfg))
 
     def sync(self) -> None:
         self.api("felt_batch_sync", self.h)

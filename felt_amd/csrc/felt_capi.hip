@@ -707,6 +707,19 @@ API int felt_surface_time_pass(felt_surface* s, int pass, int reps, double* us_p
     return felt_time_pass_impl(s, pass, reps, us_per_launch, items_per_launch, launches);
 }
 
+API int felt_surface_autotune(felt_surface* s, int reps, felt_launch_config* out) {
+    CHECK_S(s);
+    if (!out) return felt_fail(FELT_E_ARG, "null output");
+    if (int rc = prof_collect(s)) return rc;
+    return felt_autotune_impl(s, reps, out);
+}
+
+API int felt_surface_set_launch_config(felt_surface* s, const felt_launch_config* cfg) {
+    CHECK_S(s);
+    if (!cfg) return felt_fail(FELT_E_ARG, "null config");
+    return felt_set_launch_cfg_impl(s, cfg);
+}
+
 API int felt_surface_sync(felt_surface* s) {
     CHECK_S(s);
     return felt_sync_impl(s);
@@ -1001,6 +1014,18 @@ API int felt_batch_set_graph(felt_batch* b, int enable) {
     CHECK_IDLE(b);
     b->s->use_graph = enable != 0;
     return FELT_OK;
+}
+
+API int felt_batch_autotune(felt_batch* b, int reps, felt_launch_config* out) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    return felt_surface_autotune(b->s, reps, out);
+}
+
+API int felt_batch_set_launch_config(felt_batch* b, const felt_launch_config* cfg) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    return felt_surface_set_launch_config(b->s, cfg);
 }
 
 API int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launch,

@@ -52,6 +52,13 @@ struct Prof {
     long long items[PK_COUNT] = {};
 };
 
+// Launch shape of the per-entry list passes and of band compaction (felt_launch_config).
+struct LaunchCfg {
+    int tpb = 256;   // threads per block
+    int gcap = 4096; // largest grid
+    int vpl = 0;     // band compaction voxels per lane; 0: 64 for bricks above 512 voxels, else 8
+};
+
 struct felt_surface {
     felt::Geo G{};
     felt::Pool P{};
@@ -110,6 +117,7 @@ struct felt_surface {
     bool auto_unroll = false;        // raise `unroll` when steps need more rounds (batches)
     bool step_pending = false;       // a global step is queued and not yet finished
     bool use_graph = true;
+    LaunchCfg cfg;                   // felt_surface_autotune / set_launch_config
     bool graph_valid = false;
     hipGraphExec_t graph_exec = nullptr;
     // device-functor steps: the step's kernels before and after the caller's speed kernel, as two
@@ -149,6 +157,8 @@ int felt_update_global(felt_surface* s, const felt_speed* fns);
 // and any rounds beyond the unrolled ones).
 int felt_update_global_launch(felt_surface* s, const felt_speed* fns);
 int felt_global_step_finish(felt_surface* s);
+int felt_autotune_impl(felt_surface* s, int reps, felt_launch_config* out);
+int felt_set_launch_cfg_impl(felt_surface* s, const felt_launch_config* c);
 int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double* items,
                         int* launches);
 int felt_update_global_deltas(felt_surface* s, size_t m, const int32_t* gpos, const float* vals);

@@ -12,6 +12,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -902,6 +905,12 @@ static size_t list_entries(felt_surface* s) {
 static int grid_for_lists(felt_surface* s) {
     return (int)std::min<size_t>(4096, std::max<size_t>(1, (list_entries(s) + TPB - 1) / TPB));
 }
+// Grid of the tuned per-entry passes (LaunchCfg: block size and grid cap).
+static int list_grid(felt_surface* s) {
+    const size_t tpb = (size_t)s->cfg.tpb;
+    return (int)std::min<size_t>((size_t)s->cfg.gcap,
+                                 std::max<size_t>(1, (list_entries(s) + tpb - 1) / tpb));
+}
 static int chg_blocks(felt_surface* s) {
     return (int)std::max<size_t>(1, (list_entries(s) + BT * ITEMS - 1) / (BT * ITEMS));
 }
@@ -910,6 +919,20 @@ static int chg_blocks(felt_surface* s) {
 // the partition layer counts, which also lists the partitions holding band voxels, then the
 // scatter over that list.
 static int bscan_blocks(const Geo& G) { return (int)(((int64_t)G.np + PB - 1) / PB); }
+
+// Wavefronts per partition of the band scatter: one per span of 64 lanes x vpl voxels, at
+// most a block.
+static int scatter_wpp(const Geo& G, int vpl) {
+    int wpp = 1;
+    while (wpp < kScatterBlock && wpp * 64 * vpl < G.pv) wpp *= 2;
+    return wpp;
+}
+template <int D>
+static auto scatter_kernel(int vpl) {
+    return vpl == 64 ? k_band_scatter<D, 64>
+                     : vpl == 32 ? k_band_scatter<D, 32>
+                                 : vpl == 16 ? k_band_scatter<D, 16> : k_band_scatter<D, 8>;
+}
 
 static void compact_band(felt_surface* s) {
     const Geo& G = s->G;
@@ -921,13 +944,11 @@ static void compact_band(felt_surface* s) {
            s->blk_off.ptr, s->visit.ptr);
     // voxels per lane: 64 for bricks of more than 512 voxels, else 8; wavefronts per
     // partition: one per span of 64 lanes, at most a block
-    const int vpl = G.pv > 512 ? 64 : 8;
-    int wpp = 1;
-    while (wpp < kScatterBlock && wpp * 64 * vpl < G.pv) wpp *= 2;
+    const int vpl = s->cfg.vpl ? s->cfg.vpl : (G.pv > 512 ? 64 : 8);
+    const int wpp = scatter_wpp(G, vpl);
     const int ppb = kScatterBlock / wpp;
     const int grid = (int)std::min<int64_t>(((int64_t)G.np + ppb - 1) / ppb, 1024);
-    auto kern = G.D == 2 ? (vpl == 64 ? k_band_scatter<2, 64> : k_band_scatter<2, 8>)
-                         : (vpl == 64 ? k_band_scatter<3, 64> : k_band_scatter<3, 8>);
+    auto kern = G.D == 2 ? scatter_kernel<2>(vpl) : scatter_kernel<3>(vpl);
     LAUNCH(kern, dim3(grid), dim3(64 * kScatterBlock), G, s->P, wpp, s->visit.ptr,
            s->blk_off.ptr, s->band.ptr, s->band_tag.ptr, s->ctr);
 }
@@ -936,7 +957,7 @@ static void compact_band(felt_surface* s) {
 // 1..L, each a Jacobi pass (distance into d, then phi = d and tk = its layer).
 static void distance_passes(felt_surface* s, bool from_band, int look, const int* live) {
     const Geo& G = s->G;
-    const int grid = grid_for_lists(s);
+    const int grid = list_grid(s), tpb = s->cfg.tpb;
     for (int pass = 0; pass < 2; pass++) {
         const int side = pass == 0 ? -1 : 1;
         for (int m = 1; m <= G.L; m++) {
@@ -962,14 +983,14 @@ static void distance_passes(felt_surface* s, bool from_band, int look, const int
             // only the first round's launches are attributed item counts (band segments)
             if (from_band) {
                 const long long spec = -(long long)(layer + G.L) - 1;
-                PDISPATCH(PK_DIST1, spec, G.D, k_dist1_band, dim3(grid), dim3(TPB), G, s->P, list,
+                PDISPATCH(PK_DIST1, spec, G.D, k_dist1_band, dim3(grid), dim3(tpb), G, s->P, list,
                           rng, layer, d, live);
-                PLAUNCH(PK_DIST2, spec, k_dist2_band, dim3(grid), dim3(TPB), G, s->P, list, rng,
+                PLAUNCH(PK_DIST2, spec, k_dist2_band, dim3(grid), dim3(tpb), G, s->P, list, rng,
                         layer, d, tk, &s->ctr->nan_events, live);
             } else {
-                PDISPATCH(PK_DIST1_LATE, 0, G.D, k_dist1_lookup, dim3(grid), dim3(TPB), G, s->P,
+                PDISPATCH(PK_DIST1_LATE, 0, G.D, k_dist1_lookup, dim3(grid), dim3(tpb), G, s->P,
                           list, tag, rng, layer, d, live);
-                PLAUNCH(PK_DIST2_LATE, 0, k_dist2_lookup, dim3(grid), dim3(TPB), G, s->P, list, tag,
+                PLAUNCH(PK_DIST2_LATE, 0, k_dist2_lookup, dim3(grid), dim3(tpb), G, s->P, list, tag,
                         rng, layer, d, tk, &s->ctr->nan_events, live);
             }
         }
@@ -992,17 +1013,17 @@ static void compact_changes(felt_surface* s, const uint32_t* v, const int8_t* ta
 static void flush_expand(felt_surface* s, const int* live) {
     const Geo& G = s->G;
     const SC sc = sc_of(s);
-    const int grid = grid_for_lists(s);
-    PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->smask, s->ukey,
+    const int grid = list_grid(s), tpb = s->cfg.tpb;
+    PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->smask, s->ukey,
             s->tflag, live);
     LAUNCH(k_deactivate, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P, s->ukey,
            s->tflag, s->free_stack, s->ctr, live);
     prof_begin(s, PK_EXPAND, 0);
-    DISPATCH(G.D, k_expand_mark, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->nflag, s->nlist,
+    DISPATCH(G.D, k_expand_mark, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->nflag, s->nlist,
              s->ctr, live);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
            s->free_stack, s->ctr, live);
-    DISPATCH(G.D, k_expand_claim, dim3(grid), dim3(TPB), G, s->P, sc, s->ctr, s->ctr, live);
+    DISPATCH(G.D, k_expand_claim, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->ctr, live);
     prof_end(s);
 }
 
@@ -1012,8 +1033,8 @@ static void flush_expand(felt_surface* s, const int* live) {
 static void lookup_round(felt_surface* s, int look, int buf, const int* live,
                          bool clear_smask = false) {
     const int nmask = clear_smask ? s->G.np : 0;
-    LAUNCH(k_round_prep, dim3(std::max(grid_for_lists(s), std::min(1024, blocks_for(nmask)))),
-           dim3(TPB), s->lu_tag[look].ptr, s->lu_tk[look].ptr, lu_span(s, look), s->smask, nmask,
+    LAUNCH(k_round_prep, dim3(std::max(list_grid(s), std::min(1024, blocks_for(nmask)))),
+           dim3(s->cfg.tpb), s->lu_tag[look].ptr, s->lu_tk[look].ptr, lu_span(s, look), s->smask, nmask,
            live);
     distance_passes(s, false, look, live);
     compact_changes(s, s->lu_v[look].ptr, s->lu_tag[look].ptr, s->lu_tk[look].ptr,
@@ -1062,7 +1083,7 @@ static void global_step_pre(felt_surface* s) {
 static void global_step_post(felt_surface* s) {
     const Geo& G = s->G;
     const int* zrng = s->bbase + G.L;
-    PLAUNCH(PK_APPLY, -(long long)G.L - 1, k_apply, dim3(grid_for_lists(s)), dim3(TPB), G, s->P,
+    PLAUNCH(PK_APPLY, -(long long)G.L - 1, k_apply, dim3(list_grid(s)), dim3(s->cfg.tpb), G, s->P,
             s->band.ptr, s->band_d.ptr, zrng, s->band_tk.ptr, &s->ctr->nan_events);
     distance_passes(s, true, 0, always(s));
     compact_changes(s, s->band.ptr, s->band_tag.ptr, s->band_tk.ptr, &s->ctr->band_span[0], 1,
@@ -1214,8 +1235,8 @@ int felt_update_global_launch(felt_surface* s, const felt_speed* fns) {
                         s->st));
     s->speed_valid = true;
     return global_step_launch(s, [&](const int* rng) {
-        PDISPATCH(PK_SPEED, -(long long)s->G.L - 1, s->G.D, k_speed, dim3(grid_for_lists(s)),
-                  dim3(TPB), s->G, s->P, s->band.ptr, rng, s->d_speed, s->band_d.ptr);
+        PDISPATCH(PK_SPEED, -(long long)s->G.L - 1, s->G.D, k_speed, dim3(list_grid(s)),
+                  dim3(s->cfg.tpb), s->G, s->P, s->band.ptr, rng, s->d_speed, s->band_d.ptr);
     }, s->use_graph);
 }
 
@@ -1333,7 +1354,7 @@ int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double*
         n = 0;
         entries = 0;
         if (pass == FELT_PASS_SPEED) {
-            DISPATCH(G.D, k_speed, dim3(grid_for_lists(s)), dim3(TPB), G, s->P, s->band.ptr,
+            DISPATCH(G.D, k_speed, dim3(list_grid(s)), dim3(s->cfg.tpb), G, s->P, s->band.ptr,
                      s->bbase + G.L, s->d_speed, s->band_d.ptr);
             n = 1;
             entries = hb[G.L + 1] - hb[G.L];
@@ -1341,7 +1362,7 @@ int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double*
             for (int layer = -G.L; layer <= G.L; layer++) {
                 if (layer == 0) continue;
                 const int k = layer + G.L;
-                DISPATCH(G.D, k_dist1_band, dim3(grid_for_lists(s)), dim3(TPB), G, s->P,
+                DISPATCH(G.D, k_dist1_band, dim3(list_grid(s)), dim3(s->cfg.tpb), G, s->P,
                          s->band.ptr, s->bbase + k, layer, s->band_d.ptr, always(s));
                 n++;
                 entries += hb[k + 1] - hb[k];
@@ -1376,6 +1397,150 @@ int felt_time_pass_impl(felt_surface* s, int pass, int reps, double* us, double*
     *us = (double)ms * 1000.0 / ((double)reps * n);
     *items = (double)entries / n;
     *launches = n;
+    return FELT_OK;
+}
+
+// ---- launch-shape tuning (felt_surface_autotune) ------------------------------------------
+
+namespace {
+
+// Process-wide choices of felt_surface_autotune, keyed by what they depend on: the device,
+// D, L, the brick size and the band size to within a factor of two.
+struct TuneKey {
+    int device, D, L, pv, band_log2;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(device, D, L, pv, band_log2) < std::tie(o.device, o.D, o.L, o.pv, o.band_log2);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, LaunchCfg> g_tune;
+
+TuneKey tune_key(const felt_surface* s) {
+    int lg = 0;
+    for (size_t b = s->band_now; b > 1; b >>= 1) lg++;
+    return TuneKey{s->device, s->G.D, s->G.L, s->G.pv, lg};
+}
+
+bool valid_cfg(const LaunchCfg& c) {
+    return c.tpb >= 64 && c.tpb <= 1024 && c.tpb % 64 == 0 && c.gcap >= 1 &&
+           (c.vpl == 0 || c.vpl == 8 || c.vpl == 16 || c.vpl == 32 || c.vpl == 64);
+}
+
+void fill_out(const felt_surface* s, const LaunchCfg& c, double us_def, double us_tuned,
+              felt_launch_config* out) {
+    const int vpl = c.vpl ? c.vpl : (s->G.pv > 512 ? 64 : 8);
+    out->threads_per_block = c.tpb;
+    out->grid_cap = c.gcap;
+    out->voxels_per_lane = vpl;
+    out->threads_per_brick = 64 * scatter_wpp(s->G, vpl);
+    out->us_default = us_def;
+    out->us_tuned = us_tuned;
+}
+
+void apply_cfg(felt_surface* s, const LaunchCfg& c) {
+    s->cfg = c;
+    s->graph_valid = false;
+    s->fn_graph_valid = false;
+}
+
+// Microseconds of one step's worth of a pass (per-launch time x launches).
+int pass_us(felt_surface* s, int pass, int reps, double* us) {
+    double per = 0, items = 0;
+    int n = 0;
+    if (int rc = felt_time_pass_impl(s, pass, reps, &per, &items, &n)) return rc;
+    *us = per * n;
+    return FELT_OK;
+}
+
+}  // namespace
+
+int felt_set_launch_cfg_impl(felt_surface* s, const felt_launch_config* c) {
+    LaunchCfg n{c->threads_per_block, c->grid_cap, c->voxels_per_lane};
+    if (!valid_cfg(n) || n.vpl == 0)
+        return felt_fail(FELT_E_ARG, "threads_per_block must be a multiple of 64 in [64, 1024], "
+                                     "grid_cap >= 1, voxels_per_lane 8, 16, 32 or 64");
+    apply_cfg(s, n);
+    return FELT_OK;
+}
+
+// Coordinate search on the current state: block size x grid cap over the first-round distance
+// passes (plus the speed pass when a device functor has run), then voxels per lane over band
+// compaction.  Each candidate is timed by felt_time_pass_impl (graph replays of `reps`
+// copies); the surface is unchanged.
+int felt_autotune_impl(felt_surface* s, int reps, felt_launch_config* out) {
+    if (s->in_update) return felt_fail(FELT_E_STATE, "tuning inside update_start/update_end");
+    if (reps < 0) return felt_fail(FELT_E_ARG, "reps must be >= 0");
+    int rc;
+    if ((rc = size_for_band(s))) return rc;
+    const TuneKey key = tune_key(s);
+    if (reps == 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) apply_cfg(s, it->second);
+        fill_out(s, s->cfg, 0.0, 0.0, out);
+        return FELT_OK;
+    }
+    const LaunchCfg start = s->cfg;
+    auto cost = [&](const LaunchCfg& c, bool lists, double* us) -> int {
+        apply_cfg(s, c);
+        double a = 0, b = 0;
+        if (lists) {
+            if (int r = pass_us(s, FELT_PASS_DIST1, reps, &a)) return r;
+            if (s->speed_valid)
+                if (int r = pass_us(s, FELT_PASS_SPEED, reps, &b)) return r;
+        } else if (int r = pass_us(s, FELT_PASS_BAND, reps, &a)) {
+            return r;
+        }
+        *us = a + b;
+        return FELT_OK;
+    };
+    const LaunchCfg def{};
+    double def_lists = 0, def_band = 0;
+    if ((rc = cost(def, true, &def_lists)) || (rc = cost(def, false, &def_band))) {
+        apply_cfg(s, start);
+        return rc;
+    }
+    LaunchCfg best = def;
+    double best_lists = def_lists;
+    for (int tpb : {128, 256, 512}) {
+        for (int gcap : {2048, 4096, 8192, 16384}) {
+            const LaunchCfg c{tpb, gcap, 0};
+            if (c.tpb == def.tpb && c.gcap == def.gcap) continue;
+            double us = 0;
+            if ((rc = cost(c, true, &us))) {
+                apply_cfg(s, start);
+                return rc;
+            }
+            if (us < best_lists) {
+                best_lists = us;
+                best = c;
+            }
+        }
+    }
+    double best_band = def_band;
+    const int def_vpl = s->G.pv > 512 ? 64 : 8;
+    int best_vpl = 0;
+    for (int vpl : {8, 16, 32, 64}) {
+        if (vpl == def_vpl) continue;
+        double us = 0;
+        LaunchCfg c = best;
+        c.vpl = vpl;
+        if ((rc = cost(c, false, &us))) {
+            apply_cfg(s, start);
+            return rc;
+        }
+        if (us < best_band) {
+            best_band = us;
+            best_vpl = vpl;
+        }
+    }
+    best.vpl = best_vpl;
+    apply_cfg(s, best);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = best;
+    }
+    fill_out(s, best, def_lists + def_band, best_lists + best_band, out);
     return FELT_OK;
 }
 

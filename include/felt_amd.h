@@ -298,6 +298,35 @@ int felt_surface_profile_read(felt_surface* s, int kernel, uint64_t* launches, d
 #define FELT_PASS_BAND 2
 int felt_surface_time_pass(felt_surface* s, int pass, int reps, double* us_per_launch,
                            double* items_per_launch, int* launches);
+/* Launch shape of a handle's step kernels, chosen by felt_surface_autotune.  The reference's
+ * only tuning knob is the OpenMP chunk threshold (Surface.hpp:19-35); this is its counterpart
+ * here.  Not a reference member.
+ * threads_per_block, grid_cap: block size and largest grid of the per-entry passes over the
+ *   layer lists (speed, zero-layer apply, the distance passes, flush, expansion);
+ * voxels_per_lane: band compaction, voxels of a brick each lane scans (8, 16, 32 or 64), which
+ *   sets the threads per brick: ceil(brick voxels / voxels_per_lane), at most 512;
+ * threads_per_brick, us_default, us_tuned: outputs of felt_surface_autotune, the threads per
+ *   brick of the chosen compaction and the timed passes (microseconds per step's worth of
+ *   them) with the default shape (256, 4096, by brick size) and with the chosen one. */
+typedef struct felt_launch_config {
+    int32_t threads_per_block;
+    int32_t grid_cap;
+    int32_t voxels_per_lane;
+    int32_t threads_per_brick;
+    double us_default;
+    double us_tuned;
+} felt_launch_config;
+/* Times candidate launch shapes on the surface's current state (as felt_surface_time_pass
+ * does, `reps` copies per graph; the surface is unchanged) and keeps the fastest: block size x
+ * grid cap over the distance passes (and the speed pass when a device functor has run), then
+ * voxels per lane over band compaction.  The choice is also kept in a process-wide cache under
+ * (device, D, L, partition size, log2 of the band size); with reps == 0 nothing is timed and
+ * the cached choice for the current state is applied if there is one (else the handle keeps
+ * its shape; `out` then reports it with zero times).  Every shape gives identical results. */
+int felt_surface_autotune(felt_surface* s, int reps, felt_launch_config* out);
+/* Sets a launch shape directly (threads_per_block a multiple of 64 in [64, 1024], grid_cap >=
+ * 1, voxels_per_lane 8, 16, 32 or 64; the outputs are ignored). */
+int felt_surface_set_launch_config(felt_surface* s, const felt_launch_config* cfg);
 /* Block until all queued device work of the handle is done. */
 int felt_surface_sync(felt_surface* s);
 
@@ -377,6 +406,8 @@ int felt_batch_step_info(felt_batch* b, uint64_t* voxel_updates, uint64_t* round
 int felt_batch_set_graph(felt_batch* b, int enable);
 int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launch,
                          double* items_per_launch, int* launches);
+int felt_batch_autotune(felt_batch* b, int reps, felt_launch_config* out);
+int felt_batch_set_launch_config(felt_batch* b, const felt_launch_config* cfg);
 int felt_batch_sync(felt_batch* b);
 /* Polygonisation of every member: felt_polys_notify / march / invalidate / totals / destroy
  * act on all members at once; the per-member queries below take the member index. */
