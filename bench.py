@@ -79,7 +79,7 @@ def time_passes(obj, reps):
 def run_mixed(args):
     import felt_amd
     from felt_amd import workloads
-    from felt_amd.batch import Batch
+    from felt_amd.batch import Batch, Group
     specs = mixed_workload(args.surfaces)
     groups = {}
     for i, sp in enumerate(specs):
@@ -108,23 +108,47 @@ def run_mixed(args):
             tuned[f"{p}^3"] = bt.autotune(args.tune_reps)
             bt.update(speeds)
         log(f"[bench] tuned in {time.time() - t:.1f}s: {tuned}")
+    # Timed: one graph launch per batch, the batches on their own streams so they overlap.
+    # Beside it, the same batches as one group (felt_group_update_async: one graph launch per
+    # step); the runtime overlaps two streams better than two independent chains of one graph.
+    group = Group([bt for bt, _ in batches]) if args.group_steps > 0 else None
+    all_speeds = [speeds for _, speeds in batches]
+
+    def one_step(grouped):
+        if grouped:
+            group.update_async(all_speeds)
+            group.wait()
+        else:
+            # every batch's step queued (one stream each) before waiting, so they overlap
+            for bt, speeds in batches:
+                bt.update_async(speeds)
+            for bt, _ in batches:
+                bt.wait()
+        return sum(bt.step_info()[0] for bt, _ in batches)
+
     lat, ups = [], 0
     t_all = time.perf_counter()
     for step in range(args.steps):
         t0 = time.perf_counter()
-        # queue every batch's step (one stream each) before waiting, so they overlap
-        for bt, speeds in batches:
-            bt.update_async(speeds)
-        for bt, _ in batches:
-            bt.wait()
-            ups += bt.step_info()[0]
+        ups += one_step(False)
         lat.append(time.perf_counter() - t0)
         if (step + 1) % 25 == 0:
             log(f"[bench] batched step {step + 1}/{args.steps}")
     total = time.perf_counter() - t_all
     batched = {"value": ups / total, "ms_per_step": total / args.steps * 1e3,
                "p50_step_ms": pctl(lat, 50), "p99_step_ms": pctl(lat, 99),
-               "voxel_updates_per_step": ups / args.steps}
+               "voxel_updates_per_step": ups / args.steps,
+               "launch": "one graph launch per batch, the two batches on two streams"}
+    if group is not None:
+        one_step(True)  # captures the group's graph
+        t0 = time.perf_counter()
+        ups_g = sum(one_step(True) for _ in range(args.group_steps))
+        t_g = time.perf_counter() - t0
+        batched["group_launch"] = {"steps": args.group_steps,
+                                   "ms_per_step": t_g / args.group_steps * 1e3,
+                                   "value": ups_g / t_g,
+                                   "launch": "one graph launch per step for both batches"}
+        group.close()
     big = max(batches, key=lambda b: b[0].memory()[1])[0]
     ktab, roof = time_passes(big, args.pass_reps)
     mem = [b.memory() for b, _ in batches]
@@ -457,6 +481,8 @@ def main():
     ap.add_argument("--part", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--pass-reps", type=int, default=20)
+    ap.add_argument("--group-steps", type=int, default=25,
+                    help="steps timed with the batches as one group (one launch per step)")
     ap.add_argument("--tune-reps", type=int, default=5,
                     help="felt_batch_autotune replays per candidate (0: default launch shapes)")
     ap.add_argument("--rays", type=int, default=1 << 20, help="rays cast on the single surface "
@@ -512,6 +538,8 @@ def main():
         "p50_step_ms": batched["p50_step_ms"],
         "p99_step_ms": batched["p99_step_ms"],
         "voxel_updates_per_step": batched["voxel_updates_per_step"],
+        "launch": batched["launch"],
+        "group_launch": batched.get("group_launch"),
         "plain": plain,
         "batched_vs_plain": batched["value"] / plain["value"] if plain else None,
         "roofline": roof,

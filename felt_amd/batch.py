@@ -48,6 +48,10 @@ _SIGS = {
     "felt_batch_time_pass": [_P, C.c_int, C.c_int, C.POINTER(C.c_double),
                              C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "felt_batch_sync": [_P],
+    "felt_group_create": [C.POINTER(_P), C.c_int32, C.POINTER(_P)],
+    "felt_group_destroy": [_P],
+    "felt_group_update_async": [_P, C.POINTER(C.POINTER(felt_speed))],
+    "felt_group_wait": [_P],
     "felt_batch_autotune": [_P, C.c_int, C.POINTER(felt_launch_config)],
     "felt_batch_set_launch_config": [_P, C.POINTER(felt_launch_config)],
     "felt_batch_polys_create": [_P, C.POINTER(_P)],
@@ -249,6 +253,47 @@ class Batch:
 
     def sync(self) -> None:
         self.api("felt_batch_sync", self.h)
+
+
+class Group:
+    """Batches stepped by one graph launch per step (felt_group_*).  The batches must outlive
+    the group."""
+
+    def __init__(self, batches: Sequence[Batch]):
+        self.batches = list(batches)
+        self.api = self.batches[0].api
+        arr = (_P * len(self.batches))(*[b.h for b in self.batches])
+        h = _P()
+        self.api("felt_group_create", arr, len(self.batches), C.byref(h))
+        self.h = h
+        self._keep = None
+
+    def update_async(self, speeds: Sequence[Sequence[felt_speed]]) -> None:
+        """Queue update(fn) of every member of every batch; speeds[k] are batch k's."""
+        rows = [(felt_speed * len(sp))(*sp) for sp in speeds]
+        ptrs = (C.POINTER(felt_speed) * len(rows))(*[C.cast(r, C.POINTER(felt_speed))
+                                                     for r in rows])
+        self._keep = (rows, ptrs)
+        self.api("felt_group_update_async", self.h, ptrs)
+
+    def wait(self) -> None:
+        self.api("felt_group_wait", self.h)
+        self._keep = None
+
+    def update(self, speeds) -> None:
+        self.update_async(speeds)
+        self.wait()
+
+    def close(self):
+        if self.h:
+            self.api("felt_group_destroy", self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BatchPolys:

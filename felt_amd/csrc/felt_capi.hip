@@ -1002,6 +1002,74 @@ API int felt_batch_wait(felt_batch* b) {
     return felt_global_step_finish(b->s);
 }
 
+API int felt_group_create(felt_batch* const* batches, int32_t n, felt_group** out) {
+    if (!out || !batches || n < 1) return felt_fail(FELT_E_ARG, "need batches and an output");
+    *out = nullptr;
+    auto* g = new felt_group();
+    for (int32_t k = 0; k < n; k++) {
+        felt_batch* b = batches[k];
+        if (!b || !b->s) {
+            delete g;
+            return felt_fail(FELT_E_ARG, "null batch handle");
+        }
+        for (felt_surface* t : g->s)
+            if (t == b->s) {
+                delete g;
+                return felt_fail(FELT_E_ARG, "a batch appears twice in the group");
+            }
+        if (b->s->device != batches[0]->s->device) {
+            delete g;
+            return felt_fail(FELT_E_ARG, "the batches of a group must be on one device");
+        }
+        g->s.push_back(b->s);
+    }
+    g->ver.assign(g->s.size(), 0);
+    g->ready.assign(g->s.size(), nullptr);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipSetDevice(g->s[0]->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+    for (hipEvent_t& ev : g->ready)
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    (void)hipSetDevice(dev);
+    if (e != hipSuccess) {
+        felt_group_release(g);
+        delete g;
+        return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    }
+    *out = g;
+    return FELT_OK;
+}
+
+API int felt_group_destroy(felt_group* g) {
+    if (!g) return FELT_OK;
+    if (g->pending) (void)felt_group_wait_impl(g);
+    felt_group_release(g);
+    delete g;
+    return FELT_OK;
+}
+
+API int felt_group_update_async(felt_group* g, const felt_speed* const* speeds) {
+    if (!g) return felt_fail(FELT_E_ARG, "null group handle");
+    if (!speeds) return felt_fail(FELT_E_ARG, "null speeds");
+    if (g->pending) return felt_fail(FELT_E_STATE, "a step is in flight: felt_group_wait first");
+    for (size_t k = 0; k < g->s.size(); k++) {
+        if (g->s[k]->step_pending || g->s[k]->in_update)
+            return felt_fail(FELT_E_STATE, "a batch of the group is busy");
+        if (!speeds[k]) return felt_fail(FELT_E_ARG, "null speeds");
+        for (int i = 0; i < g->s[k]->G.nmem; i++)
+            if (int rc = check_speed(speeds[k] + i)) return rc;
+    }
+    return felt_group_launch_impl(g, speeds);
+}
+
+API int felt_group_wait(felt_group* g) {
+    if (!g) return felt_fail(FELT_E_ARG, "null group handle");
+    return felt_group_wait_impl(g);
+}
+
 API int felt_batch_step_info(felt_batch* b, uint64_t* voxel_updates, uint64_t* rounds) {
     CHECK_B(b);
     *voxel_updates = b->s->last_updates;

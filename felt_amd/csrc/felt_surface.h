@@ -120,6 +120,8 @@ struct felt_surface {
     LaunchCfg cfg;                   // felt_surface_autotune / set_launch_config
     bool graph_valid = false;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;       // the captured step behind graph_exec (a group's child)
+    unsigned graph_ver = 0;           // counts its captures
     // device-functor steps: the step's kernels before and after the caller's speed kernel, as two
     // graphs (rebuilt whenever graph_valid is)
     bool fn_graph_valid = false;
@@ -138,6 +140,22 @@ struct felt_surface {
 struct felt_batch {
     felt_surface* s = nullptr;
 };
+
+// Batches stepped by one graph launch (felt_group_*): one graph holding each batch's step
+// kernels as an independent chain.
+struct felt_group {
+    std::vector<felt_surface*> s;
+    hipStream_t st = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned> ver;        // each child's graph_ver when `exec` was built
+    hipEvent_t done = nullptr;        // end of the group launch, awaited by each batch's stream
+    hipEvent_t fork = nullptr;        // start of the group's capture, joined by each batch
+    std::vector<hipEvent_t> ready;    // each batch's speed upload
+    bool pending = false;
+};
+int felt_group_launch_impl(felt_group* g, const felt_speed* const* speeds);
+int felt_group_wait_impl(felt_group* g);
+void felt_group_release(felt_group* g);
 
 // Brackets one launch with events when profiling is on.
 void prof_begin(felt_surface* s, int kernel, long long items);

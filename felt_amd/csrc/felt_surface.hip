@@ -809,6 +809,8 @@ void felt_surface_release(felt_surface* s) {
         if (*g) (void)hipGraphExecDestroy(*g);
         *g = nullptr;
     }
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph = nullptr;
     if (s->h_speed) (void)hipHostFree(s->h_speed);
     s->h_speed = nullptr;
     if (s->hcp) (void)hipHostFree(s->hcp);
@@ -1146,6 +1148,23 @@ static int size_for_band(felt_surface* s) {
 // layer's deltas into band_d over the span rng.  With `use_graph` the kernel sequence is
 // captured once and replayed while list capacities, the speed layout and the unrolled round
 // count are unchanged.  Returns with the step queued on the handle's stream.
+// Captures the step's kernels as s->graph and instantiates it as s->graph_exec.
+template <class ZeroFn>
+static int capture_step(felt_surface* s, ZeroFn&& zero_fn) {
+    if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+    HIPX(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    global_step_kernels(s, zero_fn);
+    HIPX(hipStreamEndCapture(s->st, &s->graph));
+    hipError_t e = hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    s->graph_valid = true;
+    s->graph_ver++;
+    return FELT_OK;
+}
+
 template <class ZeroFn>
 static int global_step_launch(felt_surface* s, ZeroFn&& zero_fn, bool use_graph) {
     int rc;
@@ -1156,20 +1175,7 @@ static int global_step_launch(felt_surface* s, ZeroFn&& zero_fn, bool use_graph)
     if (s->prof.on && s->prof.hold_ticks > 0)
         LAUNCH(k_hold, dim3(1), dim3(64), s->prof.hold_ticks);
     if (graph) {
-        if (!s->graph_valid) {
-            if (s->graph_exec) {
-                (void)hipGraphExecDestroy(s->graph_exec);
-                s->graph_exec = nullptr;
-            }
-            HIPX(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-            global_step_kernels(s, zero_fn);
-            hipGraph_t g = nullptr;
-            HIPX(hipStreamEndCapture(s->st, &g));
-            hipError_t e = hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
-            s->graph_valid = true;
-        }
+        if (!s->graph_valid && (rc = capture_step(s, zero_fn))) return rc;
         HIPX(hipGraphLaunch(s->graph_exec, s->st));
     } else {
         global_step_kernels(s, zero_fn);
@@ -1225,7 +1231,8 @@ static Speed to_speed(const felt_speed& f) {
     return Speed{f.kind, f.a, f.b, f.c, (unsigned long long)f.seed, f.cell};
 }
 
-int felt_update_global_launch(felt_surface* s, const felt_speed* fns) {
+// Tagged speeds of the members into d_speed (queued on the handle's stream).
+static int upload_speeds(felt_surface* s, const felt_speed* fns) {
     for (int m = 0; m < s->G.nmem; m++) {
         if (fns[m].kind == FELT_SPEED_NOISE_CURV && fns[m].cell <= 0)
             return felt_fail(FELT_E_ARG, "noise cell must be positive");
@@ -1234,10 +1241,107 @@ int felt_update_global_launch(felt_surface* s, const felt_speed* fns) {
     HIPX(hipMemcpyAsync(s->d_speed, s->h_speed, s->G.nmem * sizeof(Speed), hipMemcpyHostToDevice,
                         s->st));
     s->speed_valid = true;
-    return global_step_launch(s, [&](const int* rng) {
+    return FELT_OK;
+}
+
+// The zero layer's deltas of a tagged-speed step: k_speed over the band's zero-layer segment.
+static auto speed_zero_fn(felt_surface* s) {
+    return [s](const int* rng) {
         PDISPATCH(PK_SPEED, -(long long)s->G.L - 1, s->G.D, k_speed, dim3(list_grid(s)),
                   dim3(s->cfg.tpb), s->G, s->P, s->band.ptr, rng, s->d_speed, s->band_d.ptr);
-    }, s->use_graph);
+    };
+}
+
+int felt_update_global_launch(felt_surface* s, const felt_speed* fns) {
+    if (int rc = upload_speeds(s, fns)) return rc;
+    return global_step_launch(s, speed_zero_fn(s), s->use_graph);
+}
+
+// ---- groups: several batches' steps as one graph launch -------------------------------------
+
+// Queues one tagged-speed step of every batch of the group as one launch of a graph holding
+// each batch's step kernels as an independent chain (no edges between the chains, so they run
+// concurrently).  The group's stream waits for each batch's speed upload; each batch's stream
+// then waits for the launch, so felt_group_wait (felt_global_step_finish per batch) reads each
+// batch's counters after its step.  A batch that does not replay graphs (graphs off, or
+// kernel profiling on) makes the group queue the batches' steps one by one instead.
+int felt_group_launch_impl(felt_group* g, const felt_speed* const* speeds) {
+    int rc;
+    bool graphs = true;
+    for (felt_surface* s : g->s) graphs = graphs && s->use_graph && !s->prof.on;
+    if (!graphs) {
+        for (size_t k = 0; k < g->s.size(); k++)
+            if ((rc = felt_update_global_launch(g->s[k], speeds[k]))) return rc;
+        g->pending = true;
+        return FELT_OK;
+    }
+    bool rebuild = g->exec == nullptr;
+    for (size_t k = 0; k < g->s.size(); k++) {
+        felt_surface* s = g->s[k];
+        if ((rc = upload_speeds(s, speeds[k]))) return rc;
+        if ((rc = size_for_band(s))) return rc;
+        if (!s->graph_valid && (rc = capture_step(s, speed_zero_fn(s)))) return rc;
+        rebuild = rebuild || s->graph_ver != g->ver[k];
+        HIPX(hipEventRecord(g->ready[k], s->st));
+        HIPX(hipStreamWaitEvent(g->st, g->ready[k], 0));
+    }
+    if (rebuild) {
+        // one capture on the group's stream forking onto each batch's stream and joining
+        // back: the graph holds each batch's step kernels as an independent chain
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        g->exec = nullptr;
+        HIPX(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipEventRecord(g->fork, g->st);
+        for (size_t k = 0; k < g->s.size() && e == hipSuccess; k++) {
+            felt_surface* s = g->s[k];
+            e = hipStreamWaitEvent(s->st, g->fork, 0);
+            if (e != hipSuccess) break;
+            global_step_kernels(s, speed_zero_fn(s));
+            e = hipEventRecord(g->ready[k], s->st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(g->st, g->ready[k], 0);
+            g->ver[k] = s->graph_ver;
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(g->st, &graph);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return felt_fail(FELT_E_DEVICE, hipGetErrorString(e));
+    }
+    HIPX(hipGraphLaunch(g->exec, g->st));
+    HIPX(hipEventRecord(g->done, g->st));
+    for (felt_surface* s : g->s) {
+        HIPX(hipStreamWaitEvent(s->st, g->done, 0));
+        s->step_pending = true;
+    }
+    g->pending = true;
+    return FELT_OK;
+}
+
+// Finishes every batch's step; the first error is returned after all have been finished.
+int felt_group_wait_impl(felt_group* g) {
+    if (!g->pending) return FELT_OK;
+    g->pending = false;
+    int first = FELT_OK;
+    for (felt_surface* s : g->s) {
+        const int rc = felt_global_step_finish(s);
+        if (rc && !first) first = rc;
+    }
+    return first;
+}
+
+void felt_group_release(felt_group* g) {
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->done) (void)hipEventDestroy(g->done);
+    if (g->fork) (void)hipEventDestroy(g->fork);
+    g->fork = nullptr;
+    for (hipEvent_t e : g->ready) (void)hipEventDestroy(e);
+    if (g->st) (void)hipStreamDestroy(g->st);
+    g->exec = nullptr;
+    g->done = nullptr;
+    g->ready.clear();
+    g->st = nullptr;
 }
 
 int felt_update_global(felt_surface* s, const felt_speed* fns) {

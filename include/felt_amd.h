@@ -387,6 +387,18 @@ int felt_batch_update_boxes(felt_batch* b, int32_t n, const int32_t* ids, const 
  * felt_batch_wait, every other call on the batch fails with FELT_E_STATE. */
 int felt_batch_update_async(felt_batch* b, const felt_speed* speeds);
 int felt_batch_wait(felt_batch* b);
+/* A group of batches stepped by one launch: one HIP graph holding each batch's step kernels as
+ * an independent chain, replayed once per step on the group's own stream, so a workload split
+ * over batches of different partition sizes is one graph launch per step.  The
+ * batches must be on one device, distinct, and outlive the group.  Not a reference member. */
+typedef struct felt_group felt_group;
+int felt_group_create(felt_batch* const* batches, int32_t n, felt_group** out);
+int felt_group_destroy(felt_group* g);
+/* update(fn) of every member of every batch: speeds[k] holds batch k's per-member speeds (as
+ * felt_batch_update_async).  Until felt_group_wait, the batches refuse other calls
+ * (FELT_E_STATE), as after felt_batch_update_async. */
+int felt_group_update_async(felt_group* g, const felt_speed* const* speeds);
+int felt_group_wait(felt_group* g);
 int felt_batch_snapshot(felt_batch* b, int32_t i, float* dst, size_t n);
 int felt_batch_layer_size(felt_batch* b, int32_t i, int layer, uint64_t* n);
 int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos, size_t cap,
