@@ -456,20 +456,14 @@ __global__ void k_mb_scatter(Geo G, Pool P, const int* parts, const int* start, 
     }
 }
 
-// Cube mask, owned edges and vertex / simplex counts of each band voxel; the voxels with
-// simplices are listed in `work` so that the simplex pass runs on them alone.
+// Cube mask, owned edges and vertex / simplex counts of each band voxel (the scan that follows
+// lists the voxels with simplices, so that the simplex pass runs on them alone).
 template <int D>
 __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, const int* item_l,
                           int M, uint8_t* cmask, uint16_t* own, uint32_t* ocand, int* nv,
-                          int* nt, int* work,
-                          int* nwork) {
-    __shared__ int s_cnt, s_base;
+                          int* nt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < M;
-    bool busy = false;
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    if (valid) {
+    if (i < M) {
         const int j = item_j[i], l = item_l[i];
         const int p = parts[j];
         const int s = P.page[p];
@@ -491,15 +485,8 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
         own[(size_t)j * G.pv + l] = (uint16_t)om;
         ocand[(size_t)j * G.pv + l] = oc;
         nv[i] = __popc(om);
-        nt[i] = McT<D>::nsimplex(m);
-        busy = nt[i] > 0;  // an owned edge is a crossed edge, so it has simplices too
+        nt[i] = McT<D>::nsimplex(m);  // an owned edge is a crossed edge, so it has simplices
     }
-    // the busy voxels, listed with one global atomic per block
-    const int local = wave_append(&s_cnt, busy);
-    __syncthreads();
-    if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(nwork, s_cnt) : 0;
-    __syncthreads();
-    if (busy) work[s_base + local] = i;
 }
 
 // The owned edges of the busy voxels as item * 16 + edge, in vertex order.
@@ -512,70 +499,73 @@ __global__ void k_m_vlist(const int* work, int nwork, const uint16_t* own, const
     for (int at = vb[i]; om; at++, om &= om - 1u) vwork[at] = i * 16 + (__ffs(om) - 1);
 }
 
-// Device-wide exclusive scan of two int arrays: block sums, top-level scan, apply.
-__global__ void k_s2_blocks(const int* a, const int* b, int n, int* bs) {
+// Device-wide exclusive scans of the vertex counts a, the simplex counts b and the busy flags
+// (b > 0), in three kernels (block sums, top-level scan, apply) and without atomics, so every
+// output is in item order: the vertex and simplex bases, and the list of busy items.
+__global__ void k_s3_blocks(const int* a, const int* b, int n, int* bs) {
     const int base = blockIdx.x * BT * ITEMS + threadIdx.x;
-    int sa = 0, sb = 0;
+    int sa = 0, sb = 0, sc = 0;
     for (int k = 0; k < ITEMS; k++) {
         const int i = base + k * BT;
         if (i < n) {
             sa += a[i];
             sb += b[i];
+            sc += b[i] > 0;
         }
     }
-    int ta, tb;
+    int ta, tb, tc;
     block_excl_scan(sa, &ta);
     block_excl_scan(sb, &tb);
+    block_excl_scan(sc, &tc);
     if (threadIdx.x == 0) {
-        bs[2 * blockIdx.x] = ta;
-        bs[2 * blockIdx.x + 1] = tb;
+        bs[3 * blockIdx.x] = ta;
+        bs[3 * blockIdx.x + 1] = tb;
+        bs[3 * blockIdx.x + 2] = tc;
     }
 }
 
-__global__ void k_s2_top(int* bs, int nblk, int* tot) {
-    __shared__ int carry[2];
-    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+__global__ void k_s3_top(int* bs, int nblk, int* tot) {
+    __shared__ int carry[3];
+    if (threadIdx.x < 3) carry[threadIdx.x] = 0;
     __syncthreads();
     for (int b0 = 0; b0 < nblk; b0 += BT) {
         const int b = b0 + threadIdx.x;
-        const int x = b < nblk ? bs[2 * b] : 0, y = b < nblk ? bs[2 * b + 1] : 0;
-        int tx, ty;
-        const int ex = block_excl_scan(x, &tx);
-        const int ey = block_excl_scan(y, &ty);
-        if (b < nblk) {
-            bs[2 * b] = carry[0] + ex;
-            bs[2 * b + 1] = carry[1] + ey;
+        int t[3], e[3];
+        for (int c = 0; c < 3; c++) {
+            const int x = b < nblk ? bs[3 * b + c] : 0;
+            e[c] = block_excl_scan(x, &t[c]);
         }
+        if (b < nblk)
+            for (int c = 0; c < 3; c++) bs[3 * b + c] = carry[c] + e[c];
         __syncthreads();
-        if (threadIdx.x == 0) {
-            carry[0] += tx;
-            carry[1] += ty;
-        }
+        if (threadIdx.x == 0)
+            for (int c = 0; c < 3; c++) carry[c] += t[c];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        tot[0] = carry[0];
-        tot[1] = carry[1];
-    }
+    if (threadIdx.x < 3) tot[threadIdx.x] = carry[threadIdx.x];
 }
 
 // Each thread scans ITEMS consecutive entries so that the output order is the input order.
-__global__ void k_s2_apply(const int* a, const int* b, int n, const int* bs, int* ea, int* eb) {
+__global__ void k_s3_apply(const int* a, const int* b, int n, const int* bs, int* ea, int* eb,
+                           int* work) {
     const int i0 = (blockIdx.x * BT + threadIdx.x) * ITEMS;
-    int sa = 0, sb = 0;
+    int sa = 0, sb = 0, sc = 0;
     for (int k = 0; k < ITEMS; k++)
         if (i0 + k < n) {
             sa += a[i0 + k];
             sb += b[i0 + k];
+            sc += b[i0 + k] > 0;
         }
-    int ta, tb;
-    int ra = block_excl_scan(sa, &ta) + bs[2 * blockIdx.x];
-    int rb = block_excl_scan(sb, &tb) + bs[2 * blockIdx.x + 1];
+    int ta, tb, tc;
+    int ra = block_excl_scan(sa, &ta) + bs[3 * blockIdx.x];
+    int rb = block_excl_scan(sb, &tb) + bs[3 * blockIdx.x + 1];
+    int rc = block_excl_scan(sc, &tc) + bs[3 * blockIdx.x + 2];
     for (int k = 0; k < ITEMS; k++)
         if (i0 + k < n) {
             const int va = a[i0 + k], vb = b[i0 + k];
             ea[i0 + k] = ra;
             eb[i0 + k] = rb;
+            if (vb > 0) work[rc++] = i0 + k;
             ra += va;
             rb += vb;
         }
@@ -1006,7 +996,7 @@ int felt_polys_march_impl(felt_polys* q) {
         if ((rc = q->nt.ensure(M))) return rc;
         if ((rc = q->vbase.ensure(M))) return rc;
         if ((rc = q->tbase.ensure(M))) return rc;
-        if ((rc = q->tot.ensure(2 * (size_t)nblk + 3))) return rc;
+        if ((rc = q->tot.ensure(3 * (size_t)nblk + 3))) return rc;
         if ((rc = q->work.ensure(M))) return rc;
         if ((rc = q->cmask.ensure((size_t)nb * G.pv))) return rc;
         if ((rc = q->own.ensure((size_t)nb * G.pv))) return rc;
@@ -1015,19 +1005,17 @@ int felt_polys_march_impl(felt_polys* q) {
         DISPATCHP(G.D, k_mb_scatter, dim3(nb), dim3(bt), G, s->P, q->dparts.ptr, q->pstart.ptr,
                   q->item_j.ptr, q->item_l.ptr);
         int* bs = q->tot.ptr;
-        int* totals = q->tot.ptr + 2 * nblk;
-        int* nwork = q->tot.ptr + 2 * nblk + 2;
-        HIPX(hipMemsetAsync(nwork, 0, sizeof(int), st));
+        int* totals = q->tot.ptr + 3 * nblk;  // vertices, simplices, busy items
+        int* nwork = totals + 2;
         prof_begin(s, PK_MARCH_COUNT, M);
         DISPATCHP(G.D, k_m_count, dim3((M + TPB - 1) / TPB), dim3(TPB), G, s->P, q->dparts.ptr,
                   q->item_j.ptr, q->item_l.ptr, M, q->cmask.ptr, q->own.ptr, q->ocand.ptr,
-                  q->nv.ptr, q->nt.ptr,
-                  q->work.ptr, nwork);
+                  q->nv.ptr, q->nt.ptr);
         prof_end(s);
-        hipLaunchKernelGGL(k_s2_blocks, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs);
-        hipLaunchKernelGGL(k_s2_top, dim3(1), dim3(BT), 0, st, bs, nblk, totals);
-        hipLaunchKernelGGL(k_s2_apply, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs,
-                           q->vbase.ptr, q->tbase.ptr);
+        hipLaunchKernelGGL(k_s3_blocks, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs);
+        hipLaunchKernelGGL(k_s3_top, dim3(1), dim3(BT), 0, st, bs, nblk, totals);
+        hipLaunchKernelGGL(k_s3_apply, dim3(nblk), dim3(BT), 0, st, q->nv.ptr, q->nt.ptr, M, bs,
+                           q->vbase.ptr, q->tbase.ptr, q->work.ptr);
         hipLaunchKernelGGL(k_m_leafbase, dim3((M + TPB - 1) / TPB), dim3(TPB), 0, st, q->item_j.ptr,
                            q->item_l.ptr, M, G.pv, q->vbase.ptr, q->vleaf.ptr);
         int* vst = q->vst.ptr;

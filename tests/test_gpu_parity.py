@@ -172,6 +172,42 @@ def test_polys_incremental_parity():
     print(f"[polys] ok, triangles {pg.totals()[1]}", flush=True)
 
 
+def test_mesh_order_deterministic():
+    """Two surfaces stepped and marched the same way hold the same meshes in the same order:
+    every child's vertex and simplex arrays equal element by element, and changes() equal in
+    order (the march lists and numbers its work with scans, not atomics)."""
+    import felt_amd
+    rng = workloads.SplitMix64(23)
+    size, part, L = (64, 56, 48), (8, 8, 8), 3
+    cs = [tuple(rng.uniform(-15, 15) for _ in range(3)) for _ in range(3)]
+    phi = workloads.spheres_phi(size, L, cs, [9.0, 11.5, 7.0])
+    gs = []
+    for _ in range(2):
+        g = felt_amd.Surface(3, L, size, part)
+        g.load_dense(phi)
+        pg = felt_amd.Polys(g)
+        pg.invalidate()
+        pg.march()
+        gs.append((g, pg))
+    for i in range(8):
+        sp = speed_const_curv(-0.4, 0.2) if i % 2 else speed_noise_curv(-0.3, 0.1, 0.5, 5, 8)
+        for g, pg in gs:
+            g.update(sp)
+            pg.notify()
+            pg.march()
+        (ga, pa), (gb, pb) = gs
+        assert [int(x) for x in pa.changes()] == [int(x) for x in pb.changes()], f"step {i}"
+        for p in range(ga.num_partitions):
+            ia, ib = pa.child_info(p), pb.child_info(p)
+            assert ia == ib, (i, p)
+            if not ia[0]:
+                continue
+            (va, ta), (vb, tb) = pa.child_mesh(p), pb.child_mesh(p)
+            assert np.array_equal(va.view(np.uint32), vb.view(np.uint32)), (i, p)
+            assert np.array_equal(ta, tb), (i, p)
+    assert pa.totals()[1] > 0
+
+
 def test_polys_2d_parity():
     import felt_amd
     size, part, L = (64, 48), (8, 8), 2
