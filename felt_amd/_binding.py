@@ -367,17 +367,20 @@ class Surface:
         """Surface::save: the isogrid as the reference's cereal binary archive."""
         n = C.c_size_t()
         self.b.call("surface_save", self.h, None, 0, C.byref(n))
-        buf = (C.c_uint8 * max(n.value, 1))()
-        self.b.call("surface_save", self.h, buf, n.value, C.byref(n))
-        return C.string_at(C.addressof(buf), n.value)
+        # numpy buffers: ctypes.string_at takes a C int length and cuts archives of 2 GiB or more
+        buf = np.empty(max(n.value, 1), dtype=np.uint8)
+        self.b.call("surface_save", self.h, buf.ctypes.data_as(C.POINTER(C.c_uint8)), n.value,
+                    C.byref(n))
+        return buf[:n.value].tobytes()
 
     @classmethod
     def load(cls, binding: "Binding", data: bytes) -> "Surface":
         """Surface::load: a new surface from a cereal archive of an isogrid."""
         D, L, size, part = archive_geometry(data)
-        buf = (C.c_uint8 * (len(data) + 1)).from_buffer_copy(data + b"\0")
+        buf = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
         h = _P()
-        binding.call("surface_load", buf, len(data), C.byref(h))
+        binding.call("surface_load", buf.ctypes.data_as(C.POINTER(C.c_uint8)), len(data),
+                     C.byref(h))
         obj = Surface.__new__(cls)
         obj.b = binding
         obj.D, obj.L, obj.NL = D, L, 2 * L + 1

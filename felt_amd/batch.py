@@ -20,6 +20,7 @@ _F32P = C.POINTER(C.c_float)
 _F64P = C.POINTER(C.c_double)
 _U64P = C.POINTER(C.c_uint64)
 _U32P = C.POINTER(C.c_uint32)
+_U8P = C.POINTER(C.c_uint8)
 _SZ = C.c_size_t
 _SZP = C.POINTER(C.c_size_t)
 
@@ -212,14 +213,15 @@ class Batch:
         """Surface::save of member i: the reference's cereal archive of its isogrid."""
         n = C.c_size_t()
         self.api("felt_batch_save", self.h, int(i), None, 0, C.byref(n))
-        buf = (C.c_uint8 * max(n.value, 1))()
-        self.api("felt_batch_save", self.h, int(i), buf, n.value, C.byref(n))
-        return C.string_at(C.addressof(buf), n.value)
+        # numpy buffers: ctypes.string_at takes a C int length and cuts archives of 2 GiB or more
+        buf = np.empty(max(n.value, 1), dtype=np.uint8)
+        self.api("felt_batch_save", self.h, int(i), _ptr(buf, _U8P), n.value, C.byref(n))
+        return buf[:n.value].tobytes()
 
     def load(self, i: int, data: bytes) -> None:
         """Replace member i's state with an archive of the same geometry."""
-        buf = (C.c_uint8 * (len(data) + 1)).from_buffer_copy(data + b"\0")
-        self.api("felt_batch_load", self.h, int(i), buf, len(data))
+        buf = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+        self.api("felt_batch_load", self.h, int(i), _ptr(buf, _U8P), len(data))
 
     def clear(self, i: int) -> None:
         """Release member i's bricks; it becomes an empty grid."""

@@ -41,9 +41,12 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <stdexcept>
 #include <utility>
 #include <vector>
 
@@ -1115,9 +1118,27 @@ public:
             flush_status_change();
             expand_narrow_band();
             rounds++;
+            round_check(rounds);
         }
         last_updates = n;
         last_rounds = rounds;
+    }
+
+    /* Test-only round diagnostics: FELT_ORACLE_ROUND_TRACE=n prints every n-th round of a
+     * step's round loop to stderr; FELT_ORACLE_MAX_ROUNDS=m stops a loop of more than m rounds
+     * with std::runtime_error (the reference's loop has no bound). */
+    static void round_check(uint64_t rounds) {
+        static const long long every = std::getenv("FELT_ORACLE_ROUND_TRACE")
+                                           ? std::atoll(std::getenv("FELT_ORACLE_ROUND_TRACE"))
+                                           : 0;
+        static const long long cap = std::getenv("FELT_ORACLE_MAX_ROUNDS")
+                                         ? std::atoll(std::getenv("FELT_ORACLE_MAX_ROUNDS"))
+                                         : 0;
+        if (every > 0 && rounds % (uint64_t)every == 0) {
+            std::fprintf(stderr, "[oracle] round %llu\n", (unsigned long long)rounds);
+            std::fflush(stderr);
+        }
+        if (cap > 0 && rounds > (uint64_t)cap) throw std::runtime_error("round cap reached");
     }
 
     /* update_end_global (Surface.hpp:739-762). */
@@ -1138,6 +1159,7 @@ public:
                 look->reset(iso);
                 std::swap(look, buf);
                 rounds++;
+                round_check(rounds);
             }
         }
         last_rounds = rounds;

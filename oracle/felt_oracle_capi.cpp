@@ -9,6 +9,7 @@
 
 #include <cstring>
 #include <memory>
+#include <stdexcept>
 #include <string>
 
 #include "../include/felt_amd.h"
@@ -517,9 +518,20 @@ OAPI int felt_oracle_surface_set_omp(felt_oracle_surface* s, int on) { H(s)->set
 OAPI int felt_oracle_surface_geometry(felt_oracle_surface* s, int32_t* cs, uint64_t* np) { return H(s)->geometry(cs, np); }
 OAPI int felt_oracle_surface_seed(felt_oracle_surface* s, const int32_t* pos) { return H(s)->seed(pos); }
 OAPI int felt_oracle_surface_load_dense(felt_oracle_surface* s, const float* phi, size_t n) { return H(s)->load_dense(phi, n); }
-OAPI int felt_oracle_surface_update(felt_oracle_surface* s, const felt_speed* fn) { return H(s)->update(*fn); }
+// A round loop stopped by FELT_ORACLE_MAX_ROUNDS comes back as FELT_E_STATE.
+template <class F>
+static int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::runtime_error& e) {
+        return fail(FELT_E_STATE, e.what());
+    }
+}
+OAPI int felt_oracle_surface_update(felt_oracle_surface* s, const felt_speed* fn) {
+    return guarded([&] { return H(s)->update(*fn); });
+}
 OAPI int felt_oracle_surface_update_box(felt_oracle_surface* s, const int32_t* lo, const int32_t* hi, const felt_speed* fn) {
-    return H(s)->update_box(lo, hi, *fn);
+    return guarded([&] { return H(s)->update_box(lo, hi, *fn); });
 }
 OAPI int felt_oracle_surface_update_deltas(felt_oracle_surface* s, size_t n, const int32_t* pos, const float* d) {
     return H(s)->update_deltas(n, pos, d);
