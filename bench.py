@@ -263,6 +263,157 @@ def run_single(args):
     return res
 
 
+def inside_merged(surface, pool=4):
+    """Whether the inside region (phi < 0) is one face-connected component.  A 4x-pooled mask
+    (OR over 4^3 blocks) can only join components, so while it has several the fronts have not
+    merged; only when it has one is the full-resolution mask labelled."""
+    from scipy import ndimage
+    inside = surface.snapshot() < 0
+    n = inside.shape
+    if all(x % pool == 0 for x in n):
+        shape = []
+        for x in n:
+            shape += [x // pool, pool]
+        coarse = inside.reshape(shape).any(axis=tuple(range(1, 2 * len(n), 2)))
+        if ndimage.label(coarse)[1] != 1:
+            return False
+    return ndimage.label(inside)[1] == 1
+
+
+def run_large_3d(args):
+    """The large topology-changing workload at full size: Surface<3,3> 1024^3 in 32^3
+    partitions, 64 spheres r U[20,60] (seed 0xFE1705), -0.5 until the inside region is one
+    face-connected component, then 0.2*curvature, 300 steps of update(fn) + 16 random 64^3
+    update(lo,hi,fn) + notify after each + march.  The merge check (host labelling of the
+    snapshot) runs every `check` steps outside the timed region.  A step whose round loop
+    cycles (the reference's would never end) stops the run; the steps before it are reported."""
+    import felt_amd
+    from felt_amd import workloads
+    from felt_amd._binding import FeltError, speed_constant, speed_curvature
+    N, part, L, steps, check = args.large_size, 32, 3, args.large_steps, args.large_check
+    rng = workloads.SplitMix64(0xFE1705)
+    cs = [tuple(rng.uniform(-N / 2, N / 2) for _ in range(3)) for _ in range(64)]
+    rs = [rng.uniform(20, 60) * N / 1024 for _ in range(64)]
+    box = 64 * N // 1024
+    t = time.time()
+    s = felt_amd.Surface(3, L, (N,) * 3, (part,) * 3)
+    s.load_spheres(cs, rs)
+    # a step whose round loop cycles (the reference's would not end) fails after 1000 rounds
+    s.set_round_limit(1000)
+    polys = felt_amd.Polys(s)
+    polys.invalidate()
+    polys.march()
+    s.sync()
+    log(f"[bench] large 3d: {N}^3 loaded in {time.time() - t:.1f}s, band "
+        f"{s.stats().band_voxels}")
+    switch, lat, ups, spx, march_s, check_s = None, [], 0, 0, 0.0, 0.0
+    stopped, done = None, 0
+    t_all = time.perf_counter()
+    for step in range(steps):
+        sp = speed_constant(-0.5) if switch is None else speed_curvature(0.2)
+        boxes = []
+        for _ in range(16):
+            lo = [rng.randint(-N // 2, N // 2 - box) for _ in range(3)]
+            boxes.append((lo, [x + box - 1 for x in lo]))
+        t0 = time.perf_counter()
+        try:
+            s.update(sp)
+            ups += s.step_info()[0]
+            polys.notify()
+            for lo, hi in boxes:
+                s.update_box(lo, hi, sp)
+                ups += s.step_info()[0]
+                polys.notify()
+        except FeltError as e:
+            stopped = {"step": step + 1, "error": str(e)}
+            log(f"[bench] large 3d: step {step + 1} stopped: {e}")
+            t_stop = time.perf_counter() - t0
+            break
+        done = step + 1
+        tm = time.perf_counter()
+        polys.march()
+        march_s += time.perf_counter() - tm
+        lat.append(time.perf_counter() - t0)
+        spx += sum(polys.child_info(int(p))[2] for p in polys.changes())
+        if switch is None and (step + 1) % check == 0:
+            tc = time.perf_counter()
+            if inside_merged(s):
+                switch = step
+            check_s += time.perf_counter() - tc
+        if (step + 1) % 25 == 0:
+            log(f"[bench] large 3d step {step + 1}/{steps}, switch {switch}, band "
+                f"{s.stats().band_voxels}")
+    total = time.perf_counter() - t_all - check_s - (t_stop if stopped else 0.0)
+    ktab, roof = time_passes(s, args.pass_reps)
+    res = {"workload": f"Surface<3,3> {N}^3 in {part}^3 partitions, 64 spheres r=U[20,60]*{N}/1024"
+                       ", seed 0xFE1705; -0.5 until the inside region is one face-connected "
+                       f"component (checked every {check} steps), then 0.2*curvature; per step "
+                       f"update(fn), 16 random {box}^3 update(lo,hi,fn), notify after each, "
+                       "incremental marching cubes",
+           "steps": done, "stopped": stopped, "first_step_ms": lat[0] * 1e3 if lat else None,
+           "switch_after_step": None if switch is None else switch + 1,
+           "value": ups / total, "ms_per_step": total / max(done, 1) * 1e3,
+           "p50_step_ms": pctl(lat, 50), "p99_step_ms": pctl(lat, 99),
+           "voxel_updates_per_step": ups / max(done, 1),
+           "triangles_per_s": spx / march_s if march_s > 0 else None,
+           "band_voxels_final": int(s.stats().band_voxels),
+           "roofline": roof, "kernels": ktab}
+    polys.close()
+    s.close()
+    log(f"[bench] large 3d: {res['value'] / 1e9:.2f} G voxel updates/s, "
+        f"{res['ms_per_step']:.2f} ms/step, switch after step {res['switch_after_step']}")
+    return res
+
+
+def run_large_2d(args):
+    """Surface<2,2> 16384^2 in 64^2 partitions, 512 circles r U[4,64] (seed 0xFE1706), 300
+    steps of update(-0.5) with notify and incremental marching squares."""
+    import felt_amd
+    from felt_amd import workloads
+    from felt_amd._binding import speed_constant
+    N, part, steps = args.large2d_size, 64, args.large_steps
+    rng = workloads.SplitMix64(0xFE1706)
+    cs = [tuple(rng.uniform(-N / 2, N / 2) for _ in range(2)) for _ in range(512)]
+    rs = [rng.uniform(4, 64) * N / 16384 for _ in range(512)]
+    s = felt_amd.Surface(2, 2, (N,) * 2, (part,) * 2)
+    s.load_spheres(cs, rs)
+    polys = felt_amd.Polys(s)
+    polys.invalidate()
+    polys.march()
+    s.sync()
+    sp = speed_constant(-0.5)
+    lat, ups, spx, march_s = [], 0, 0, 0.0
+    t_all = time.perf_counter()
+    for step in range(steps):
+        t0 = time.perf_counter()
+        s.update(sp)
+        ups += s.step_info()[0]
+        polys.notify()
+        tm = time.perf_counter()
+        polys.march()
+        march_s += time.perf_counter() - tm
+        lat.append(time.perf_counter() - t0)
+        spx += sum(polys.child_info(int(p))[2] for p in polys.changes())
+        if (step + 1) % 50 == 0:
+            log(f"[bench] large 2d step {step + 1}/{steps}, band {s.stats().band_voxels}")
+    total = time.perf_counter() - t_all
+    ktab, roof = time_passes(s, args.pass_reps)
+    res = {"workload": f"Surface<2,2> {N}^2 in {part}^2 partitions, 512 circles "
+                       f"r=U[4,64]*{N}/16384, seed 0xFE1706, {steps} steps of update(-0.5), "
+                       "notify, incremental marching squares",
+           "steps": steps, "first_step_ms": lat[0] * 1e3, "value": ups / total,
+           "ms_per_step": total / steps * 1e3,
+           "p50_step_ms": pctl(lat, 50), "p99_step_ms": pctl(lat, 99),
+           "voxel_updates_per_step": ups / steps,
+           "segments_per_s": spx / march_s if march_s > 0 else None,
+           "band_voxels_final": int(s.stats().band_voxels), "roofline": roof, "kernels": ktab}
+    polys.close()
+    s.close()
+    log(f"[bench] large 2d: {res['value'] / 1e9:.2f} G voxel updates/s, "
+        f"{res['ms_per_step']:.2f} ms/step")
+    return res
+
+
 def ray_bench(s, size, n):
     """Surface::ray throughput on the final state: n rays from a shell of radius 0.45*size about
     the centre, aimed at points within 0.1*size of it; rays resident on the device."""
@@ -476,11 +627,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--surfaces", type=int, default=256)
-    ap.add_argument("--single-steps", type=int, default=60)
+    ap.add_argument("--single-steps", type=int, default=200)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--part", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--pass-reps", type=int, default=20)
+    ap.add_argument("--large", action="store_true",
+                    help="also run the full-size topology-changing workloads (1024^3 3D, "
+                         "16384^2 2D); minutes of host-side merge checks")
+    ap.add_argument("--large-only", action="store_true")
+    ap.add_argument("--large-size", type=int, default=1024)
+    ap.add_argument("--large2d-size", type=int, default=16384)
+    ap.add_argument("--large-steps", type=int, default=300)
+    ap.add_argument("--large-check", type=int, default=5,
+                    help="steps between merge checks of the 3D workload")
     ap.add_argument("--group-steps", type=int, default=25,
                     help="steps timed with the batches as one group (one launch per step)")
     ap.add_argument("--tune-reps", type=int, default=5,
@@ -518,6 +678,10 @@ def main():
     if args.traffic_probe:
         print(json.dumps({"traffic_probe": run_traffic_probe(args)}), flush=True)
         return
+    if args.large_only:
+        print(json.dumps({"large_3d": run_large_3d(args), "large_2d": run_large_2d(args)}),
+              flush=True)
+        return
     specs, batched, plain, ktab, roof, cfg = run_mixed(args)
     log(f"[bench] batched {batched['value'] / 1e9:.2f} G/s" +
         (f", plain {plain['value'] / 1e9:.2f} G/s" if plain else ""))
@@ -549,6 +713,9 @@ def main():
         result["served"] = run_served(args)
     if not args.no_single:
         result["single_surface"] = run_single(args)
+    if args.large:
+        result["large_3d"] = run_large_3d(args)
+        result["large_2d"] = run_large_2d(args)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(specs, args.cpu_seconds)
     print(json.dumps(result), flush=True)

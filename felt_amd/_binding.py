@@ -152,6 +152,7 @@ class Binding:
             "surface_step_info": (C.c_int, [_P, _U64P, _U64P]),
             "surface_set_graph": (C.c_int, [_P, C.c_int]),
             "surface_autotune": (C.c_int, [_P, C.c_int, C.POINTER(felt_launch_config)]),
+            "surface_set_round_limit": (C.c_int, [_P, C.c_int32]),
             "surface_set_launch_config": (C.c_int, [_P, C.POINTER(felt_launch_config)]),
             "surface_load_spheres": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double)]),
@@ -189,6 +190,7 @@ class Binding:
             if not hasattr(L, prefix + name) and name in ("device_count", "surface_step_info",
                                                             "surface_set_graph",
                                                             "surface_autotune",
+                                                            "surface_set_round_limit",
                                                             "surface_set_launch_config",
                                                             "surface_time_pass",
                                                             "surface_load_spheres",
@@ -514,6 +516,10 @@ class Surface:
                           voxels_per_lane: int) -> None:
         cfg = felt_launch_config(threads_per_block, grid_cap, voxels_per_lane, 0, 0.0, 0.0)
         self.b.call("surface_set_launch_config", self.h, C.byref(cfg))
+
+    def set_round_limit(self, max_rounds: int) -> None:
+        """Most rounds one update may take before it fails (felt_surface_set_round_limit)."""
+        self.b.call("surface_set_round_limit", self.h, int(max_rounds))
 
     def step_info(self):
         """(voxel updates, rounds) of the last update call."""

@@ -54,6 +54,7 @@ _SIGS = {
     "felt_group_update_async": [_P, C.POINTER(C.POINTER(felt_speed))],
     "felt_group_wait": [_P],
     "felt_batch_autotune": [_P, C.c_int, C.POINTER(felt_launch_config)],
+    "felt_batch_set_round_limit": [_P, C.c_int32],
     "felt_batch_set_launch_config": [_P, C.POINTER(felt_launch_config)],
     "felt_batch_polys_create": [_P, C.POINTER(_P)],
     "felt_batch_polys_changes": [_P, C.c_int32, _U64P, _SZ, _SZP],
@@ -252,6 +253,9 @@ class Batch:
                           voxels_per_lane: int) -> None:
         cfg = felt_launch_config(threads_per_block, grid_cap, voxels_per_lane, 0, 0.0, 0.0)
         self.api("felt_batch_set_launch_config", self.h, C.byref(cfg))
+
+    def set_round_limit(self, max_rounds: int) -> None:
+        self.api("felt_batch_set_round_limit", self.h, int(max_rounds))
 
     def sync(self) -> None:
         self.api("felt_batch_sync", self.h)

@@ -720,6 +720,13 @@ API int felt_surface_set_launch_config(felt_surface* s, const felt_launch_config
     return felt_set_launch_cfg_impl(s, cfg);
 }
 
+API int felt_surface_set_round_limit(felt_surface* s, int32_t max_rounds) {
+    CHECK_S(s);
+    if (max_rounds < 1) return felt_fail(FELT_E_ARG, "max_rounds must be at least 1");
+    s->max_rounds = max_rounds;
+    return FELT_OK;
+}
+
 API int felt_surface_sync(felt_surface* s) {
     CHECK_S(s);
     return felt_sync_impl(s);
@@ -1094,6 +1101,12 @@ API int felt_batch_set_launch_config(felt_batch* b, const felt_launch_config* cf
     CHECK_B(b);
     CHECK_IDLE(b);
     return felt_surface_set_launch_config(b->s, cfg);
+}
+
+API int felt_batch_set_round_limit(felt_batch* b, int32_t max_rounds) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    return felt_surface_set_round_limit(b->s, max_rounds);
 }
 
 API int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launch,

@@ -118,6 +118,7 @@ struct felt_surface {
     bool step_pending = false;       // a global step is queued and not yet finished
     bool use_graph = true;
     LaunchCfg cfg;                   // felt_surface_autotune / set_launch_config
+    int max_rounds = 100000;         // rounds of one step before it fails (set_round_limit)
     bool graph_valid = false;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;       // the captured step behind graph_exec (a group's child)

@@ -28,7 +28,6 @@ using namespace felt::step;
 namespace {
 
 constexpr int TPB = 256;
-constexpr int kMaxRounds = 100000;
 constexpr int kMaxUnroll = 8;  // Ctr::live has one flag per unrolled round
 
 __global__ void k_fill_i32(int* p, int64_t n, int v) {
@@ -1109,6 +1108,20 @@ static void global_step_kernels(felt_surface* s, ZeroFn&& zero_fn) {
     global_step_post(s);
 }
 
+// The round loop of update_end / update_end_global (Surface.hpp:739-762) runs while any voxel
+// changes layer, with no bound in the reference.  A band can cycle: e.g. an inside blob with no
+// zero layer whose centre touches an out-of-band outside voxel flips sign every round, since
+// the distance pass (Surface.hpp:1386-1413) takes the neighbour nearest the zero curve on either
+// side.  The reference's release build loops forever there (its debug build throws at the
+// first jump of more than one layer, Surface.hpp:1044-1059); a step here gives up after
+// max_rounds rounds and reports it, leaving the surface as the last round left it.
+static int round_limit_error(felt_surface* s) {
+    return felt_fail(FELT_E_STATE, "round loop did not converge within " +
+                                       std::to_string(s->max_rounds) +
+                                       " rounds: status changes keep cycling (the reference's "
+                                       "loop would not end)");
+}
+
 // Rounds after the unrolled ones, driven from the host (rare).
 static int finish_rounds(felt_surface* s, int look, int buf, int* rounds) {
     int rc = FELT_OK;
@@ -1123,8 +1136,8 @@ static int finish_rounds(felt_surface* s, int look, int buf, int* rounds) {
         std::swap(look, buf);
         (*rounds)++;
         if ((rc = read_ctr(s))) break;
-        if (*rounds > kMaxRounds) {
-            rc = felt_fail(FELT_E_STATE, "round loop did not converge");
+        if (*rounds > s->max_rounds) {
+            rc = round_limit_error(s);
             break;
         }
     }
@@ -1203,7 +1216,11 @@ int felt_global_step_finish(felt_surface* s) {
     if (s->hc.live[s->unroll - 1] && s->hc.flag) {
         const int r = s->unroll;  // next round number
         const int look = r % 2 == 1 ? 1 : 0, buf = 1 - look;
-        if ((rc = finish_rounds(s, look, buf, &rounds))) return rc;
+        if ((rc = finish_rounds(s, look, buf, &rounds))) {
+            s->in_update = false;
+            s->band_known = false;
+            return rc;
+        }
         s->band_known = false;
         if (s->auto_unroll && s->unroll < kMaxUnroll) {
             s->unroll = std::min(kMaxUnroll, rounds);
@@ -1746,13 +1763,17 @@ int felt_update_end_impl(felt_surface* s) {
         lookup_round(s, look, buf, always(s));
         rounds++;
         if ((rc = read_ctr(s))) break;
-        if (rounds > kMaxRounds) {
-            rc = felt_fail(FELT_E_STATE, "round loop did not converge");
+        if (rounds > s->max_rounds) {
+            rc = round_limit_error(s);
             break;
         }
     }
     s->hint = -1;
-    if (rc) return rc;
+    if (rc) {
+        s->in_update = false;
+        s->band_known = false;
+        return rc;
+    }
     s->last_updates = (uint64_t)an;
     s->last_rounds = rounds;
     s->in_update = false;

@@ -327,6 +327,12 @@ int felt_surface_autotune(felt_surface* s, int reps, felt_launch_config* out);
 /* Sets a launch shape directly (threads_per_block a multiple of 64 in [64, 1024], grid_cap >=
  * 1, voxels_per_lane 8, 16, 32 or 64; the outputs are ignored). */
 int felt_surface_set_launch_config(felt_surface* s, const felt_launch_config* cfg);
+/* Most rounds one update may take (default 100000; at least 1).  The reference's round loop
+ * (Surface.hpp:739-762) runs while any voxel changes layer and can cycle forever (an inside
+ * blob without a zero layer that touches an out-of-band outside voxel flips sign each round);
+ * a step here fails with FELT_E_STATE after this many rounds instead.  Not a reference
+ * member. */
+int felt_surface_set_round_limit(felt_surface* s, int32_t max_rounds);
 /* Block until all queued device work of the handle is done. */
 int felt_surface_sync(felt_surface* s);
 
@@ -420,6 +426,7 @@ int felt_batch_time_pass(felt_batch* b, int pass, int reps, double* us_per_launc
                          double* items_per_launch, int* launches);
 int felt_batch_autotune(felt_batch* b, int reps, felt_launch_config* out);
 int felt_batch_set_launch_config(felt_batch* b, const felt_launch_config* cfg);
+int felt_batch_set_round_limit(felt_batch* b, int32_t max_rounds);
 int felt_batch_sync(felt_batch* b);
 /* Polygonisation of every member: felt_polys_notify / march / invalidate / totals / destroy
  * act on all members at once; the per-member queries below take the member index. */

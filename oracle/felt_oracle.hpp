@@ -42,7 +42,6 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -465,6 +464,8 @@ bool archive_geometry(const uint8_t* src, size_t n, Vi<D>* size, Vi<D>* part) {
     *size = b.vecdi<D>();
     return a.ok && b.ok;
 }
+
+inline long long g_round_trace = 0, g_round_cap = 0, g_round_dump = 0;
 
 /* SplitMix64 (Steele, Lea & Flood 2014). */
 inline uint64_t splitmix64(uint64_t x) {
@@ -1124,16 +1125,31 @@ public:
         last_rounds = rounds;
     }
 
-    /* Test-only round diagnostics: FELT_ORACLE_ROUND_TRACE=n prints every n-th round of a
-     * step's round loop to stderr; FELT_ORACLE_MAX_ROUNDS=m stops a loop of more than m rounds
-     * with std::runtime_error (the reference's loop has no bound). */
-    static void round_check(uint64_t rounds) {
-        static const long long every = std::getenv("FELT_ORACLE_ROUND_TRACE")
-                                           ? std::atoll(std::getenv("FELT_ORACLE_ROUND_TRACE"))
-                                           : 0;
-        static const long long cap = std::getenv("FELT_ORACLE_MAX_ROUNDS")
-                                         ? std::atoll(std::getenv("FELT_ORACLE_MAX_ROUNDS"))
-                                         : 0;
+    /* Round diagnostics of the test harness (felt_oracle_set_round_diag): every g_round_trace-th
+     * round of a step's round loop is printed to stderr, rounds g_round_dump.. +2 print their
+     * status changes, and a loop of more than g_round_cap rounds stops with std::runtime_error
+     * (the reference's loop has no bound).  Zero turns each off. */
+    void round_check(uint64_t rounds) {
+        const long long every = g_round_trace, cap = g_round_cap, dump = g_round_dump;
+        if (dump > 0 && rounds >= (uint64_t)dump && rounds < (uint64_t)dump + 3) {
+            int shown = 0;
+            for (int from = -L; from <= L && shown < 40; from++) {
+                const int kf = layer_idx(from);
+                for (Idx p : status.plists[kf]) {
+                    for (Idx l : status.kids[p].lists[kf]) {
+                        if (shown++ >= 40) break;
+                        const Vi<D> pos = g.pos_of(p, l);
+                        std::fprintf(stderr, "[oracle] round %llu change (",
+                                     (unsigned long long)rounds);
+                        for (int a = 0; a < D; a++)
+                            std::fprintf(stderr, a ? ",%d" : "%d", pos[a]);
+                        std::fprintf(stderr, ") %d -> %d, phi %.9g\n", from,
+                                     status.kids[p].data[l], double(iso.get(pos)));
+                    }
+                }
+            }
+            std::fflush(stderr);
+        }
         if (every > 0 && rounds % (uint64_t)every == 0) {
             std::fprintf(stderr, "[oracle] round %llu\n", (unsigned long long)rounds);
             std::fflush(stderr);

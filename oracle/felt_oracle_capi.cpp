@@ -443,6 +443,15 @@ struct felt_oracle_surface {
 
 OAPI const char* felt_oracle_last_error(void) { return g_err.c_str(); }
 
+/* Round-loop diagnostics for the tests: trace every n-th round, stop loops longer than cap
+ * rounds (FELT_E_STATE), print the status changes of rounds dump..dump+2; 0 turns each off. */
+OAPI int felt_oracle_set_round_diag(long long trace, long long cap, long long dump) {
+    g_round_trace = trace;
+    g_round_cap = cap;
+    g_round_dump = dump;
+    return FELT_OK;
+}
+
 OAPI int felt_oracle_set_threads(int n) {
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);

@@ -1,5 +1,6 @@
 """Loads the CPU restatement (oracle/libfelt_oracle.so) behind the same Python mirror as the
 HIP library.  Test infrastructure: only tests, smoke() and bench.py's CPU baseline use it."""
+import ctypes
 import os
 import subprocess
 
@@ -18,6 +19,9 @@ def binding() -> Binding:
         _binding = Binding(LIB, prefix="felt_oracle_", polys_on_surface=True,
                            create_has_precision=True)
         _binding.lib.felt_oracle_set_threads(1)
+        f = _binding.lib.felt_oracle_set_round_diag
+        f.argtypes = [ctypes.c_longlong] * 3
+        f.restype = ctypes.c_int
     return _binding
 
 
