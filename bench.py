@@ -56,8 +56,26 @@ def mixed_workload(n):
     return workloads.mixed_surfaces(n, MIXED_SEED, 32, 256)
 
 
-def time_passes(obj, reps):
-    """Per-pass timing on the current state; returns (table, roofline of the dominant pass)."""
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "distance_pass_traffic_batched_16.md")
+
+
+def dist_traffic_per_entry():
+    """(fetch, write) HBM bytes per entry of the first-round distance pass, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy of known size), or None
+    when the summary is absent."""
+    import re
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"Per-entry HBM bytes.*?fetch ([0-9.]+) write ([0-9.]+)", text, re.S)
+    return (float(m.group(1)), float(m.group(2))) if m else None
+
+
+def time_passes(obj, reps, traffic=False):
+    """Per-pass timing on the current state; returns (table, roofline of the dominant pass).
+    With `traffic` (the mixed workload's 16^3 batch, the one the counter summary measured) the
+    roofline carries the HBM bytes per launch."""
     ktab = {}
     for k, (name, bpi) in PASSES.items():
         us, items, n = obj.time_pass(k, reps)
@@ -65,12 +83,20 @@ def time_passes(obj, reps):
     dom = max((k for k in ktab if k != 2), key=lambda k: ktab[k][2] * ktab[k][3])
     name, bpi, n, us, items = ktab[dom]
     achieved = bpi * items / (us / 1e6)
+    per = dist_traffic_per_entry() if traffic and name == "distance pass 1" else None
     roof = {"bound": "hbm", "kernel": name, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK,
+            "traffic": (per[0] + per[1]) * items if per else None,
             "bytes_per_launch": bpi * items, "avg_launch_us": us,
             "timing": f"the pass's launches of one step on the final state, {reps} copies "
                       "captured in one HIP graph and replayed between two HIP events on the "
                       "handle's stream"}
+    if per:
+        roof["traffic_source"] = (
+            f"HBM bytes per launch = entries x ({per[0]:.2f} fetched + {per[1]:.2f} written) B, "
+            "per-entry figures from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of bench.py "
+            "--traffic-probe scaled by a 256 MiB copy (profiles/"
+            "distance_pass_traffic_batched_16.md)")
     table = {v[0]: {"launches_per_step": v[2], "us_per_launch": v[3], "items_per_launch": v[4]}
              for v in ktab.values()}
     return table, roof
@@ -150,7 +176,7 @@ def run_mixed(args):
                                    "launch": "one graph launch per step for both batches"}
         group.close()
     big = max(batches, key=lambda b: b[0].memory()[1])[0]
-    ktab, roof = time_passes(big, args.pass_reps)
+    ktab, roof = time_passes(big, args.pass_reps, traffic=True)
     mem = [b.memory() for b, _ in batches]
     for bt, _ in batches:
         bt.close()
