@@ -38,18 +38,31 @@ def test_c1_plain_config(graph):
     one."""
     import felt_amd
     g, c = pair(3, 3, (128,) * 3, (16,) * 3)
+    # the FP64 oracle beside it (graph run only): the largest |phi - phi_fp64| is reported
+    c64 = oracle_lib.surface(3, 3, (128,) * 3, (16,) * 3, fp64=True) if graph else None
     g.set_graph(graph)
     step_both(g, c, "seed", (0, 0, 0))
+    if c64 is not None:
+        c64.seed((0, 0, 0))
     check(g, c, "seed")
     t0 = time.time()
+    worst64 = 0.0
     for i in range(80):
         sp = speed_constant(-0.5) if i < 40 else speed_curvature(0.2)
         step_both(g, c, "update", sp)
         check(g, c, f"step {i}", exact=True)
+        if c64 is not None:
+            c64.update(sp)
+            if i % 10 == 9:
+                worst64 = max(worst64, float(np.abs(g.snapshot().astype(np.float64) -
+                                                    c64.snapshot().astype(np.float64)).max()))
         if i % 10 == 9:
             print(f"[c1] step {i + 1} ok, band {g.stats().band_voxels}, "
                   f"{time.time() - t0:.1f}s", flush=True)
     assert c.stats().nan_events == g.stats().nan_events
+    if c64 is not None:
+        assert worst64 < 1e-4
+        print(f"[c1] max |phi - phi_fp64| over the run = {worst64:.3g}", flush=True)
     pg, pc = felt_amd.Polys(g), oracle_lib.polys(c)
     pg.invalidate(); pc.invalidate()
     pg.march(); pc.march()
