@@ -59,7 +59,7 @@ struct Req {
 // Tick-time model of the latency target: t = fixed + unit * work, work = sum over in-flight
 // requests of band * box share (about the affected set).  Both terms are running averages.
 // The target bounds the upper tail of tick time, not its mean: admission keeps the prediction
-// times a margin -- the running mean plus 2.5 standard deviations of actual / predicted time --
+// times a margin -- the running mean plus 3 standard deviations of actual / predicted time --
 // within the target.
 struct TickModel {
     double fixed_ms = 2.0;
@@ -68,7 +68,7 @@ struct TickModel {
     double predict(double work) const { return fixed_ms + unit_ms * work; }
     double margin() const {
         const double sd = std::sqrt(std::max(0.0, r2 - r1 * r1));
-        return std::max(1.0, r1 + 2.5 * sd);
+        return std::max(1.0, r1 + 3.0 * sd);
     }
     void observe(double ms, double work) {
         const double r = ms / std::max(1e-3, predict(work));

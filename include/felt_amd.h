@@ -481,7 +481,7 @@ int felt_server_polys(felt_server* v, felt_polys** out);
 int felt_server_start(felt_server* v);
 /* Latency target for admission: a request joins a tick only while a running model of tick
  * time (a fixed part plus a cost per band voxel inside the in-flight boxes), times a margin for
- * its error (the running mean plus 2.5 standard deviations of actual / predicted tick time),
+ * its error (the running mean plus 3 standard deviations of actual / predicted tick time),
  * stays within tick_ms, so that the target bounds the tail of tick time rather than its mean;
  * a request is always admitted into an empty tick.  0 (the default) disables it.
  * felt_server_start also reserves the step lists the in-flight set can need (within the
