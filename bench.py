@@ -198,6 +198,8 @@ def run_mixed(args):
     for sp in specs:
         s = felt_amd.Surface(3, 3, sp.size, sp.part)
         s.load_spheres(sp.centres, sp.radii)
+        if args.plain_launches:
+            s.set_graph(False)
         plain.append((s, workloads.speed_of(sp.speed)))
     log(f"[bench] plain: {len(plain)} surfaces loaded in {time.time() - t:.1f}s")
     for _ in range(args.warmup):
@@ -667,6 +669,9 @@ def main():
     ap.add_argument("--large-steps", type=int, default=300)
     ap.add_argument("--large-check", type=int, default=5,
                     help="steps between merge checks of the 3D workload")
+    ap.add_argument("--plain-launches", action="store_true",
+                    help="plain surfaces launch their kernels one by one instead of replaying a "
+                         "captured graph")
     ap.add_argument("--group-steps", type=int, default=25,
                     help="steps timed with the batches as one group (one launch per step)")
     ap.add_argument("--tune-reps", type=int, default=5,
