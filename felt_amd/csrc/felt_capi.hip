@@ -86,6 +86,9 @@ int make_pool(int D, int L, const int32_t* part, int n, const int32_t* sizes, ui
         pv *= G.part[a];
     }
     if (pv > (1 << 24)) return felt_fail(FELT_E_ARG, "partition too large");
+    // a brick's face-neighbour record holds its in-grid extent along axes 0 and 1 in 16 bits
+    if (G.part[0] > 65535 || G.part[1] > 65535)
+        return felt_fail(FELT_E_ARG, "partition edges along the first two axes are at most 65535");
     G.pv = (int)pv;
     G.lstr[2] = 1;
     G.lstr[1] = G.part[2];

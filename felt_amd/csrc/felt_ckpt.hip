@@ -42,7 +42,7 @@ constexpr uint64_t kNull = ~0ull;  // Felt::null_idx (Common.hpp:102)
 // Member m back to an empty grid: bricks returned to the free stack, every partition inactive
 // with background `bg`, no layer counts, no delta or status partitions.
 __global__ void k_member_clear(Geo G, Pool P, int pbase, int np, float bg, int* free_stack,
-                               Ctr* ctr, unsigned* dmask, unsigned* smask) {
+                               int* dlist, Ctr* ctr, unsigned* dmask, unsigned* smask) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
         const int p = pbase + i;
         const int s = P.page[p];
@@ -51,6 +51,7 @@ __global__ void k_member_clear(Geo G, Pool P, int pbase, int np, float bg, int* 
             P.slot_part[s] = -1;
             const int t = atomicAdd(&ctr->free_top, 1);
             free_stack[t] = s;
+            dlist[atomicAdd(&ctr->dead_n, 1)] = p;
         }
         P.pbg[p] = bg;
         for (int k = 0; k < G.NL; k++) P.pcnt[(size_t)p * G.NL + k] = 0;
@@ -224,9 +225,13 @@ int felt_archive_geometry(const uint8_t* src, size_t n, int* D, int* L, int* siz
 
 int felt_member_clear_impl(felt_surface* s, int m) {
     const Mem& M = s->mems[m];
+    HIPX(hipMemsetAsync(&s->ctr->dead_n, 0, sizeof(int), s->st));
     k_member_clear<<<std::min(1024, (M.np + TPB - 1) / TPB), TPB, 0, s->st>>>(
-        s->G, s->P, M.pbase, M.np, (float)(s->G.L + 1), s->free_stack, s->ctr, s->dmask, s->smask);
+        s->G, s->P, M.pbase, M.np, (float)(s->G.L + 1), s->free_stack, s->dlist, s->ctr, s->dmask,
+        s->smask);
     HIPX(hipGetLastError());
+    int rc;
+    if ((rc = felt_nbr_fix(s))) return rc;
     HIPX(hipStreamSynchronize(s->st));
     s->band_known = false;
     s->hint = -1;

@@ -109,7 +109,19 @@ struct Pool {
     int4* slot_org;
     const Mem* mem;
     const int* part_mem;
+    int4* nbr;  // two per slot: the brick's face-neighbour record (below)
 };
+
+// Face-neighbour record of an active brick, two int4 per slot, kept in step with the page table
+// by k_nbr_fix after every activation and deactivation.  Field j = 2a + h names the neighbour
+// partition one step along axis a (h = 0: -1, h = 1: +1): its brick slot when active,
+// kNbrOut when it lies outside the member's children grid, else kNbrInactive - p for pool
+// partition p (whose voxels read pbg[p]).  Fields 0-3 are the first int4, 4-5 the x and y of the
+// second; its z holds the in-grid extent of the brick along axes 0 and 1 (16 bits each) and its
+// w along axis 2: min(part, size - origin), less than part only for a brick the grid cuts.  A
+// distance gather then needs no page-table, brick-origin or member read.
+constexpr int kNbrOut = -1;
+constexpr int kNbrInactive = -2;
 
 // slot_org.w: the member and whether the brick touches a face of its member's grid.
 constexpr int kMemberMask = 0x3fffffff;
@@ -327,95 +339,60 @@ __device__ __forceinline__ float distance(const Geo& G, const Pool& P, const Vx&
     return dn + dir;
 }
 
-// distance() of U band voxels at once, with the gathers split into levels (brick origins and
-// member rows, then neighbour pages, then neighbour values) and every load of a level issued for
-// all U voxels and all 2D face neighbours before any is used: the dependent gathers of U voxels
-// overlap instead of running one after another.  Same result as distance() voxel by voxel.
-template <int D, int U>
-__device__ __forceinline__ void distance_u(const Geo& G, const Pool& P, const uint32_t* v,
-                                           float dir, float* out) {
+// distance() of one band voxel through its brick's face-neighbour record: the record, the
+// voxel's own value and its in-brick neighbours are loaded together, then the values of the
+// neighbours in other bricks (or their partitions' backgrounds), so the gather is two dependent
+// levels after the list entry.  Same result as distance().
+template <int D>
+__device__ __forceinline__ float distance_rec(const Geo& G, const Pool& P, uint32_t v, float dir) {
     constexpr int NN = 2 * D;
-    uint32_t slot[U];
-    int4 org[U];
+    const uint32_t slot = fdiv(v, G.fpv);
+    int loc[3] = {0, 0, 0};
+    leaf_coords<D>(G, v - slot * (uint32_t)G.pv, loc);
+    const int4 ra = P.nbr[2 * (size_t)slot], rb = P.nbr[2 * (size_t)slot + 1];
+    const float self = P.phi[v];
+    const int rec[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
+    const int ext[3] = {(int)((unsigned)rb.z & 0xffffu), (int)((unsigned)rb.z >> 16), rb.w};
+    // per neighbour: whether it is in the grid, and its voxel id, or for a neighbour in an
+    // inactive partition that partition (bg >= 0)
+    bool in[NN];
+    uint32_t vid[NN];
+    int bg[NN];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        slot[u] = fdiv(v[u], G.fpv);
-        org[u] = P.slot_org[slot[u]];
-    }
-    float self[U];
-    int4 r0[U], r1[U];
-    int loc[U][3];
+    for (int a = 0; a < D; a++) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-        self[u] = P.phi[v[u]];
-        const int m = org[u].w & kMemberMask;
-        r0[u] = mem_row0(P, m);
-        r1[u] = mem_row1(P, m);
-        loc[u][0] = loc[u][1] = loc[u][2] = 0;
-        leaf_coords<D>(G, v[u] - slot[u] * (uint32_t)G.pv, loc[u]);
-    }
-    // classify the neighbours: outside the grid, in the brick (index), or in the next brick
-    // (its partition and leaf)
-    int nidx[U][NN], npart[U][NN];
-    bool nval[U][NN];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        const int o[3] = {org[u].x, org[u].y, org[u].z};
-        const int sz[3] = {r0[u].x, r0[u].y, r0[u].z};
-        const int cs[3] = {r1[u].x, r1[u].y, r1[u].z};
-        int own = r0[u].w;
-#pragma unroll
-        for (int a = 0; a < D; a++) own += (int)fdiv((uint32_t)o[a], G.fpart[a]) * cs[a];
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int sd = h ? 1 : -1, j = 2 * a + h;
-                const int qa = o[a] + loc[u][a] + sd;
-                const int la = loc[u][a] + sd;
-                nval[u][j] = qa >= 0 && qa < sz[a];
-                if (la >= 0 && la < G.part[a]) {
-                    int d[3] = {0, 0, 0};
-                    d[a] = sd;
-                    nidx[u][j] = (int)v[u] + step_offset<D>(G, loc[u], d);
-                    npart[u][j] = -1;
-                } else {
-                    int lc[3] = {loc[u][0], loc[u][1], loc[u][2]};
-                    lc[a] = sd < 0 ? G.part[a] - 1 : 0;
-                    nidx[u][j] = leaf_of<D>(G, lc);
-                    npart[u][j] = own + sd * cs[a];
-                }
+        for (int h = 0; h < 2; h++) {
+            const int sd = h ? 1 : -1, j = 2 * a + h;
+            const int la = loc[a] + sd;
+            bg[j] = -1;
+            if (la >= 0 && la < G.part[a]) {
+                int d[3] = {0, 0, 0};
+                d[a] = sd;
+                in[j] = la < ext[a];
+                vid[j] = v + (uint32_t)step_offset<D>(G, loc, d);
+            } else {
+                const int t = rec[j];
+                in[j] = t != kNbrOut;
+                int lc[3] = {loc[0], loc[1], loc[2]};
+                lc[a] = sd < 0 ? G.part[a] - 1 : 0;
+                vid[j] = (uint32_t)max(t, 0) * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, lc);
+                if (t < kNbrOut) bg[j] = kNbrInactive - t;
             }
         }
     }
-    int pg[U][NN];
+    float nv[NN];
 #pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int j = 0; j < NN; j++)
-            pg[u][j] = (nval[u][j] && npart[u][j] >= 0) ? P.page[npart[u][j]] : -1;
-    float nv[U][NN];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-#pragma unroll
-        for (int j = 0; j < NN; j++) {
-            float x = 0.0f;
-            if (nval[u][j]) {
-                if (npart[u][j] < 0) x = P.phi[(uint32_t)nidx[u][j]];
-                else if (pg[u][j] < 0) x = P.pbg[npart[u][j]];
-                else x = P.phi[(size_t)pg[u][j] * G.pv + nidx[u][j]];
-            }
-            nv[u][j] = x;
-        }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-        float dist = self[u] * dir;
-#pragma unroll
-        for (int j = 0; j < NN; j++)
-            if (nval[u][j] && nv[u][j] * dir < dist) dist = nv[u][j] * dir;
-        const float dn = dist * dir;
-        out[u] = dn + dir;
+    for (int j = 0; j < NN; j++) {
+        float x = 0.0f;
+        if (in[j]) x = bg[j] >= 0 ? P.pbg[bg[j]] : P.phi[vid[j]];
+        nv[j] = x;
     }
+    float dist = self * dir;
+#pragma unroll
+    for (int j = 0; j < NN; j++)
+        if (in[j] && nv[j] * dir < dist) dist = nv[j] * dir;
+    const float dn = dist * dir;
+    return dn + dir;
 }
 
 // A band voxel with what neighbour gathers need: coordinates in its brick and grid, its grid's

@@ -29,6 +29,7 @@ struct Ctr {
     int band_total;      // tracked voxels, refreshed at the end of an unrolled step
     int front[20];       // breadth-first frontier bounds of calc_affected
     int reserve;         // bricks the step may activate at most (k_reserve_count)
+    int dead_n;          // partitions deactivated since the neighbour records were last fixed
 };
 
 // Status-change record: the reference's StatusChangeGrid entry (Surface.hpp:1138-1178).

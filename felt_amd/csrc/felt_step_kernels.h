@@ -592,45 +592,16 @@ __global__ void k_apply(Geo G, Pool P, const uint32_t* list, const float* delta,
 
 // update_distance pass 1 (Surface.hpp:1029-1073) over the entries of one layer: d = the
 // distance from the layer's neighbours.  Tagged lists hold every layer and are filtered by tag.
-// Each thread takes U entries per trip (a grid-stride apart) and computes all of them before
-// storing, so the loads of U independent voxels are in flight together: the pass is bound by
-// the latency of its dependent gathers (entry -> brick origin -> face neighbours), not by
-// bandwidth.  Entries past the span, or of another layer in a tagged list, are computed on a
-// valid voxel and not stored.
-#ifndef FELT_DIST_U
-#define FELT_DIST_U 1
-#endif
-template <int D, bool Tagged, int U = FELT_DIST_U>
+// The pass is bound by its gathers (entry -> brick record and own value -> neighbours in other
+// bricks), not by arithmetic.
+template <int D, bool Tagged>
 __device__ __forceinline__ void dist1_body(const Geo& G, const Pool& P, const uint32_t* list,
                                            const int8_t* tag, const int* rng, int layer,
                                            float* out) {
     const float side = layer < 0 ? -1.0f : 1.0f;
-    const int lo = rng[0], hi = rng[1];
-    // slots of a trip are `step` apart: the grid stride for long spans, else the span split in U
-    // so that no thread computes a slot past the end while others sit idle
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int step = min(gridDim.x * blockDim.x, (hi - lo + U - 1) / U);
-    if (tid >= step) return;
-    for (int i0 = lo + tid; i0 < hi; i0 += U * step) {
-        if (U == 1 && Tagged && tag[i0] != layer) continue;  // another layer's entry
-        uint32_t v[U];
-        bool st[U];
-        v[0] = list[i0];
-#pragma unroll
-        for (int u = 1; u < U; u++) {
-            const int i = i0 + u * step;
-            v[u] = i < hi ? list[i] : v[0];
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int i = i0 + u * step;
-            st[u] = i < hi && (!Tagged || tag[i] == layer);
-        }
-        float d[U];
-        distance_u<D, U>(G, P, v, side, d);
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (st[u]) out[i0 + u * step] = d[u];
+    FELT_SPAN_LOOP(i, rng) {
+        if (Tagged && tag[i] != layer) continue;  // another layer's entry
+        out[i] = distance_rec<D>(G, P, list[i], side);
     }
 }
 
@@ -873,8 +844,8 @@ __global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask, 
 // Partitions flagged by the flush: a partition left with no voxel is deactivated with the
 // background of its key and its brick returned to the pool.  Also starts the expansion's
 // activation list empty.
-__global__ void k_deactivate(Geo G, Pool P, int* ukey, int* tflag, int* free_stack, Ctr* ctr,
-                             const int* live) {
+__global__ void k_deactivate(Geo G, Pool P, int* ukey, int* tflag, int* free_stack, int* dlist,
+                             Ctr* ctr, const int* live) {
     if (!*live) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr->need_n = 0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < G.np; p += gridDim.x * blockDim.x) {
@@ -891,6 +862,7 @@ __global__ void k_deactivate(Geo G, Pool P, int* ukey, int* tflag, int* free_sta
             P.pbg[p] = (float)((key & 0xffff) - 32768);
             const int t = atomicAdd(&ctr->free_top, 1);
             free_stack[t] = s;
+            dlist[atomicAdd(&ctr->dead_n, 1)] = p;
         }
     }
 }
@@ -995,6 +967,52 @@ __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* n
     }
 }
 
+// Brings the face-neighbour records (felt_dev.h) of the listed partitions and of their active
+// face neighbours in line with the page table: the first n_a partitions of la (activated) and
+// the first n_b of lb (deactivated).  Any partition may be listed, in either list, any number
+// of times: every record field written is a function of the page table, which no kernel changes
+// while this one runs, so racing writers of a field write the same value.
+template <int D>
+__global__ void k_nbr_fix(Geo G, Pool P, const int* la, const int* n_a, const int* lb,
+                          const int* n_b, const int* live) {
+    if (!*live) return;
+    const int na = *n_a, n = na + *n_b;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int p = i < na ? la[i] : lb[i - na];
+        const int m = P.part_mem[p];
+        const Mem& M = P.mem[m];
+        int pc[3] = {0, 0, 0};
+        part_coords<D>(M, p - M.pbase, pc);
+        const int s = P.page[p];
+        const int self = s >= 0 ? s : kNbrInactive - p;
+        int rec[6] = {kNbrOut, kNbrOut, kNbrOut, kNbrOut, kNbrOut, kNbrOut};
+        int* words = reinterpret_cast<int*>(P.nbr);
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int sd = h ? 1 : -1, j = 2 * a + h;
+                const int c = pc[a] + sd;
+                if (c < 0 || c >= M.csize[a]) continue;
+                const int q = p + sd * M.cstr[a];
+                const int t = P.page[q];
+                // a partition of the children grid that starts past the grid's end (the +1
+                // rule, PartitionedMixin.hpp:131-141) holds no grid position: kNbrOut
+                if (c * G.part[a] < M.size[a]) rec[j] = t >= 0 ? t : kNbrInactive - q;
+                // the neighbour's field that faces p
+                if (t >= 0) words[8 * (size_t)t + (j ^ 1)] = self;
+            }
+        if (s >= 0) {
+            int ext[3] = {1, 1, 1};
+#pragma unroll
+            for (int a = 0; a < D; a++)
+                ext[a] = max(0, min(G.part[a], M.size[a] - pc[a] * G.part[a]));
+            P.nbr[2 * (size_t)s] = make_int4(rec[0], rec[1], rec[2], rec[3]);
+            P.nbr[2 * (size_t)s + 1] = make_int4(rec[4], rec[5], ext[0] | (ext[1] << 16), ext[2]);
+        }
+    }
+}
+
 // expand_narrow_band phase 2 (Surface.hpp:1249-1355): every neighbour whose value is outside the
 // band gets its distance, and joins layer +-L if it is in no list (the value does not depend on
 // which expanding voxel claims it: the claimed voxel's nearest neighbour towards the zero curve
@@ -1004,6 +1022,8 @@ __global__ void k_alloc(Geo G, Pool P, const int* nlist, const Ctr* ctrc, int* n
 template <int D>
 __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, const int* live) {
     if (!*live) return;
+    // the round's deactivations are in the neighbour records by now (k_nbr_fix ran)
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->dead_n = 0;
     constexpr int NN = 2 * D;
     FELT_WAVE_LOOP(i, valid, ctrc->exp_n) {
         const bool cand = valid;

@@ -74,6 +74,7 @@ struct felt_surface {
     int *ukey = nullptr, *tflag = nullptr, *tlist = nullptr, *nflag = nullptr, *nlist = nullptr;
     unsigned *dmask = nullptr, *smask = nullptr;
     int* act = nullptr;
+    int* dlist = nullptr;  // partitions deactivated since the neighbour records were fixed
     // band compaction
     DBuf<uint32_t> band;
     DBuf<float> band_d;
@@ -240,6 +241,9 @@ int felt_save_impl(felt_surface* s, int m, uint8_t* dst, size_t cap, size_t* n);
 int felt_load_impl(felt_surface* s, int m, const uint8_t* src, size_t n);
 int felt_member_clear_impl(felt_surface* s, int m);
 int felt_alloc_listed(felt_surface* s, int grid);
+// Face-neighbour records brought in line with the page table for the partitions activated
+// (s->nlist) and deactivated (s->dlist) since they were last fixed; queued on the stream.
+int felt_nbr_fix(felt_surface* s);
 // felt_surface_ray without device allocation: rays staged through a fixed device array.
 int felt_ray_noalloc_impl(felt_surface* s, size_t n, const float* origins, const float* dirs,
                           float* out);

@@ -734,6 +734,7 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->P.pcnt, np * G.NL * sizeof(int)));
     HIPX(hipMalloc(&s->P.slot_part, cap * sizeof(int)));
     HIPX(hipMalloc(&s->P.slot_org, cap * sizeof(int4)));
+    HIPX(hipMalloc(&s->P.nbr, cap * 2 * sizeof(int4)));
     {
         Mem* dm = nullptr;
         int* pm = nullptr;
@@ -761,6 +762,7 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->dmask, np * sizeof(unsigned)));
     HIPX(hipMalloc(&s->smask, np * sizeof(unsigned)));
     HIPX(hipMalloc(&s->act, np * sizeof(int)));
+    HIPX(hipMalloc(&s->dlist, np * sizeof(int)));
     HIPX(hipMalloc(&s->bbase, (G.NL + 2) * sizeof(int)));
     HIPX(hipMalloc(&s->d_speed, G.nmem * sizeof(Speed)));
     HIPX(hipHostMalloc((void**)&s->h_speed, G.nmem * sizeof(Speed), hipHostMallocDefault));
@@ -783,6 +785,8 @@ int felt_surface_init(felt_surface* s) {
     LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->dmask, np, 0);
     LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->smask, np, 0);
     HIPX(hipMemsetAsync(s->P.lay, 0x80, cap * (int64_t)G.pv, s->st));
+    // records of slots never activated read as "no neighbour" (every field kNbrOut)
+    HIPX(hipMemsetAsync(s->P.nbr, 0xff, cap * 2 * sizeof(int4), s->st));
     HIPX(hipMemsetAsync(s->amark, 0, cap * (int64_t)G.pv, s->st));
     LAUNCH(k_init_free, dim3(gb), dim3(TPB), s->free_stack, (int)cap);
     Ctr c;
@@ -815,8 +819,9 @@ void felt_surface_release(felt_surface* s) {
     if (s->hcp) (void)hipHostFree(s->hcp);
     s->hcp = nullptr;
     void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
-                    s->P.slot_org, (void*)s->P.mem, (void*)s->P.part_mem, s->amark, s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag,
-                    s->nlist, s->dmask, s->smask, s->act, s->bbase, s->d_speed};
+                    s->P.slot_org, (void*)s->P.mem, (void*)s->P.part_mem, s->P.nbr, s->amark,
+                    s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag, s->nlist,
+                    s->dmask, s->smask, s->act, s->dlist, s->bbase, s->d_speed};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s->band.release();
@@ -1018,12 +1023,14 @@ static void flush_expand(felt_surface* s, const int* live) {
     PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->smask, s->ukey,
             s->tflag, live);
     LAUNCH(k_deactivate, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P, s->ukey,
-           s->tflag, s->free_stack, s->ctr, live);
+           s->tflag, s->free_stack, s->dlist, s->ctr, live);
     prof_begin(s, PK_EXPAND, 0);
     DISPATCH(G.D, k_expand_mark, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->nflag, s->nlist,
              s->ctr, live);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
            s->free_stack, s->ctr, live);
+    DISPATCH(G.D, k_nbr_fix, dim3(std::min(256, blocks_for(G.np))), dim3(TPB), G, s->P, s->nlist,
+             &s->ctr->need_n, s->dlist, &s->ctr->dead_n, live);
     DISPATCH(G.D, k_expand_claim, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->ctr, live);
     prof_end(s);
 }
@@ -1952,6 +1959,7 @@ int felt_track_points(felt_surface* s, int m, const std::vector<int>& gpos,
              s->nlist, s->ctr);
     LAUNCH(k_alloc, dim3(std::min(1024, G.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
            s->free_stack, s->ctr, always(s));
+    if ((rc = felt_nbr_fix(s))) return rc;
     DISPATCH(G.D, k_track_points, dim3(blocks_for(n)), dim3(TPB), G, s->P, m, dpos, s->tmp_f.ptr,
              dlay, n);
     return read_ctr(s);
@@ -1976,6 +1984,7 @@ static int load_field(felt_surface* s, int m, const F& f) {
            s->nlist, s->nflag, s->ctr);
     LAUNCH(k_alloc, dim3(std::min(1024, M.np)), dim3(TPB), G, s->P, s->nlist, s->ctr, s->nflag,
            s->free_stack, s->ctr, always(s));
+    if ((rc = felt_nbr_fix(s))) return rc;
     hipLaunchKernelGGL(fill, dim3(M.np), dim3(bt), 0, s->st, G, s->P, m, f, s->tmp_i.ptr, s->ctr);
     return read_ctr(s);
 }
@@ -1985,6 +1994,16 @@ int felt_alloc_listed(felt_surface* s, int grid) {
     LAUNCH(k_alloc, dim3(std::max(1, grid)), dim3(TPB), s->G, s->P, s->nlist, s->ctr, s->nflag,
            s->free_stack, s->ctr, always(s));
     HIPX(hipGetLastError());
+    return felt_nbr_fix(s);
+}
+
+int felt_nbr_fix(felt_surface* s) {
+    const Geo& G = s->G;
+    DISPATCH(G.D, k_nbr_fix, dim3(std::min(256, blocks_for(G.np))), dim3(TPB), G, s->P, s->nlist,
+             &s->ctr->need_n, s->dlist, &s->ctr->dead_n, always(s));
+    HIPX(hipGetLastError());
+    // outside a round nothing else clears the deactivation list
+    HIPX(hipMemsetAsync(&s->ctr->dead_n, 0, sizeof(int), s->st));
     return FELT_OK;
 }
 

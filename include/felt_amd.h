@@ -86,7 +86,8 @@ const char* felt_last_error(void);
 int felt_device_count(int* n);
 
 /* ---- Surface lifecycle -------------------------------------------------------------- */
-/* Surface(size, size_partition) (Surface.hpp:163-173).  D is 2 or 3, 1 <= L <= 8. */
+/* Surface(size, size_partition) (Surface.hpp:163-173).  D is 2 or 3, 1 <= L <= 8; a partition
+ * holds at most 2^24 voxels and its first two edges are at most 65535. */
 int felt_surface_create(int D, int L, const int32_t* size, const int32_t* part, felt_surface** out);
 int felt_surface_destroy(felt_surface* s);
 /* Geometry: children-grid size (3 values, unused axes 1) and partition count. */

@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 
 import oracle_lib
+import progress
 import oracle_pool
 import parity
 from felt_amd import workloads
@@ -55,12 +56,12 @@ def test_large_3d_eighth_scale():
                 phis[step] = g.snapshot()
             if step == switch or step in (100, 299):
                 meshes[step] = parity.mesh_sets(pg, g)
-            if step % 50 == 49:
-                print(f"[c5a/8] gpu step {step + 1} (switch {switch}), band {bands[-1]}, "
-                      f"{time.time() - t0:.0f}s", flush=True)
+            if step % 10 == 9:
+                progress.say(f"[c5a/8] gpu step {step + 1} (switch {switch}), band {bands[-1]}, "
+                             f"{time.time() - t0:.0f}s")
 
         switch = oracle_pool.c5aq_run(g, pg, on_step)
-        want = fut.result()
+        want = progress.wait(fut, "c5a/8")
     assert switch == want["switch"] and switch is not None and switch > 10
     for step, (a, b) in enumerate(zip(states, want["state"])):
         assert a[0] == b[0], f"step {step}: layers differ ({[x[0] for x in a[0]]} vs " \
@@ -74,8 +75,8 @@ def test_large_3d_eighth_scale():
         d = parity.mesh_sets_diff(m, want["mesh"][step], 3)
         assert not d, f"mesh at step {step}: {d[:4]}"
     assert min(bands) > 0 and max(len(m) for m in meshes.values()) > 0
-    print(f"[c5a/8] 300 steps equal the oracle after every step; fronts merged at step "
-          f"{switch + 1}; band {min(bands)}..{max(bands)} voxels", flush=True)
+    progress.say(f"[c5a/8] 300 steps equal the oracle after every step; fronts merged at "
+                 f"step {switch + 1}; band {min(bands)}..{max(bands)} voxels")
 
 
 def test_large_2d_eighth_scale():
@@ -96,8 +97,8 @@ def test_large_2d_eighth_scale():
         c.update(sp)
         check(g, c, f"step {step}", snapshot=step % 50 == 49)
         if step % 50 == 49:
-            print(f"[c5b/8] step {step + 1} ok, band {g.stats().band_voxels}, "
-                  f"{time.time() - t0:.0f}s", flush=True)
+            progress.say(f"[c5b/8] step {step + 1} ok, band {g.stats().band_voxels}, "
+                         f"{time.time() - t0:.0f}s")
     pg, pc = felt_amd.Polys(g), oracle_lib.polys(c)
     for p in (pg, pc):
         p.invalidate()

@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 import oracle_pool
+import progress
 import parity
 from felt_amd import workloads
 
@@ -57,9 +58,8 @@ def test_single_large_surface_quarter_scale():
                 phis[step] = g.snapshot()
             if step in cfg["mesh_steps"]:
                 meshes[step] = parity.mesh_sets(pg, g)
-        print(f"[c2/4] gpu done in {time.time() - t0:.0f}s, band {g.stats().band_voxels}",
-              flush=True)
-        c32, c64 = f32.result(), f64.result()
+        progress.say(f"[c2/4] gpu done in {time.time() - t0:.0f}s, band {g.stats().band_voxels}")
+        c32, c64 = progress.wait(f32, "c2/4"), progress.wait(f64, "c2/4 fp64")
     for step, (gs, cs) in enumerate(zip(states, c32["state"]), 1):
         assert gs[0] == cs[0], _fail_digest(f"step {step}", gs[0], cs[0], 3)
         assert gs[1] == cs[1], f"step {step}: active partitions or partition lists differ"
@@ -77,7 +77,7 @@ def test_single_large_surface_quarter_scale():
     # both sides round every operation the same way: the distances are bit-identical
     assert identical == len(phis), f"phi bit-identical at {identical}/{len(phis)} checks only"
     print(f"[c2/4] 200 steps: layers and lists equal every step, phi bit-identical at "
-          f"{identical}/{len(phis)} checks, max |phi - phi_fp64| = {worst64:.3g}", flush=True)
+          f"{identical}/{len(phis)} checks, max |phi - phi_fp64| = {worst64:.3g}")
 
 
 @pytest.mark.timeout(600)
@@ -117,16 +117,16 @@ def test_batched_mixed_workload_256():
         for bt, ids, _ in batches:
             for j, i in enumerate(ids):
                 got[(i, "phi")] = oracle_pool.phi_digest(bt.snapshot(j))
-        print(f"[c3] gpu {steps} steps of {n} surfaces in {len(batches)} batches, "
-              f"{time.time() - t0:.0f}s with digests", flush=True)
+        progress.say(f"[c3] gpu {steps} steps of {n} surfaces in {len(batches)} batches, "
+                     f"{time.time() - t0:.0f}s with digests")
         want = {}
         for f in futs:
-            want.update(f.result())
+            want.update(progress.wait(f, "c3"))
     bad = [key for key in want if got[key] != want[key]]
     assert not bad, f"{len(bad)} member checkpoints differ, first {sorted(bad, key=str)[:6]}"
     assert len(want) == n * (len(checkpoints) + 1)
-    print(f"[c3] all {n} members equal their oracles (layers at {checkpoints}, phi at {steps})",
-          flush=True)
+    progress.say(f"[c3] all {n} members equal their oracles (layers at {checkpoints}, "
+                 f"phi at {steps})")
 
 
 @pytest.mark.timeout(600)
@@ -181,11 +181,11 @@ def test_served_stream_subset():
                                       parity.mesh_sets(pg, g), 3)
             assert not d, f"surface {i} mesh vs alone: {d[:4]}"
             g.close()
-        want = [f.result() for f in futs]
+        want = [progress.wait(f, "c4") for f in futs]
     for i, s in enumerate(specs):
         layers = oracle_pool.layer_digests(lambda k: srv.batch.layer_positions(i, k), s.size, 3)
         assert layers == want[i][0], _fail_digest(f"surface {i}", layers, want[i][0], 3)
         assert oracle_pool.phi_digest(srv.batch.snapshot(i)) == want[i][1], f"surface {i} phi"
     print(f"[c4] {len(reqs)} requests over {len(specs)} surfaces in {ticks} ticks "
           f"(p99 tick {st['step_p99_ms']:.1f} ms): every surface equals its oracle and its "
-          f"plain run alone", flush=True)
+          f"plain run alone")
