@@ -690,7 +690,9 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0)
     ap.add_argument("--budget-frac", type=float, default=0.25)
     ap.add_argument("--max-active", type=int, default=1000)
-    ap.add_argument("--tick-target", type=float, default=5.0)
+    # admission aims at 4.5 ms so that the tick p99 stays under the 5 ms latency target with
+    # margin for a noisier host
+    ap.add_argument("--tick-target", type=float, default=4.5)
     ap.add_argument("--plain-ticks", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
