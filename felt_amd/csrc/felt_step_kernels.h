@@ -553,6 +553,21 @@ k_band_scatter(Geo G, Pool P, int wpp, const int* visit, const int* offsets, uin
     for (int i = (rng)[0] + blockIdx.x * blockDim.x + threadIdx.x, i##_hi = (rng)[1]; i < i##_hi; \
          i += gridDim.x * blockDim.x)
 
+// The same loop with the span cut into eight contiguous parts, one per XCD: the hardware deals
+// workgroups to the eight XCDs in turn (block b runs on XCD b mod 8), so block b works in part
+// b mod 8, and entries next to each other in the list -- the same brick, or neighbouring bricks
+// -- are processed on one XCD and share its L2, instead of each brick's lines being fetched
+// into several XCDs' L2s.  Blocks past the last multiple of eight have no part; a grid of fewer
+// than eight blocks works on the whole span as one part.
+#define FELT_XCD_SPAN_LOOP(i, rng)                                                             \
+    for (int i##_k = gridDim.x >= 8u ? 8 : 1, i##_x = (int)blockIdx.x % i##_k,                 \
+             i##_l = (int)blockIdx.x / i##_k, i##_per = (int)gridDim.x / i##_k,                \
+             i##_n = (rng)[1] - (rng)[0],                                                      \
+             i = (rng)[0] + (int)((long long)i##_n * i##_x / i##_k) + i##_l * (int)blockDim.x + \
+                 (int)threadIdx.x,                                                            \
+             i##_hi = (rng)[0] + (int)((long long)i##_n * (i##_x + 1) / i##_k);                \
+         i##_l < i##_per && i < i##_hi; i += i##_per * (int)blockDim.x)
+
 __global__ void k_tk_init(const int8_t* tag, int8_t* tk, const int* rng, const int* live) {
     if (!*live) return;
     FELT_SPAN_LOOP(i, rng) tk[i] = tag[i];
@@ -599,9 +614,15 @@ __device__ __forceinline__ void dist1_body(const Geo& G, const Pool& P, const ui
                                            const int8_t* tag, const int* rng, int layer,
                                            float* out) {
     const float side = layer < 0 ? -1.0f : 1.0f;
-    FELT_SPAN_LOOP(i, rng) {
-        if (Tagged && tag[i] != layer) continue;  // another layer's entry
-        out[i] = distance_rec<D>(G, P, list[i], side);
+    if (Tagged) {
+        // a lookup's entries of one layer cluster where the previous round's list held their
+        // source layers: the interleaved loop keeps every XCD busy
+        FELT_SPAN_LOOP(i, rng) {
+            if (tag[i] != layer) continue;  // another layer's entry
+            out[i] = distance_rec<D>(G, P, list[i], side);
+        }
+    } else {
+        FELT_XCD_SPAN_LOOP(i, rng) out[i] = distance_rec<D>(G, P, list[i], side);
     }
 }
 
@@ -610,12 +631,20 @@ template <bool Tagged>
 __device__ __forceinline__ void dist2_body(const Geo& G, const Pool& P, const uint32_t* list,
                                            const int8_t* tag, const int* rng, int layer,
                                            const float* d, int8_t* tk, int* nan_ctr) {
-    FELT_SPAN_LOOP(i, rng) {
-        if (Tagged && tag[i] != layer) continue;
-        const uint32_t v = list[i];
-        const float x = d[i];
-        P.phi[v] = x;
-        tk[i] = clamp_k(layer_of(x, G.L, nan_ctr));
+    if (Tagged) {
+        FELT_SPAN_LOOP(i, rng) {
+            if (tag[i] != layer) continue;
+            const uint32_t v = list[i];
+            const float x = d[i];
+            P.phi[v] = x;
+            tk[i] = clamp_k(layer_of(x, G.L, nan_ctr));
+        }
+    } else {
+        FELT_XCD_SPAN_LOOP(i, rng) {
+            const float x = d[i];
+            P.phi[list[i]] = x;
+            tk[i] = clamp_k(layer_of(x, G.L, nan_ctr));
+        }
     }
 }
 

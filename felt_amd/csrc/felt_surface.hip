@@ -911,11 +911,20 @@ static size_t list_entries(felt_surface* s) {
 static int grid_for_lists(felt_surface* s) {
     return (int)std::min<size_t>(4096, std::max<size_t>(1, (list_entries(s) + TPB - 1) / TPB));
 }
-// Grid of the tuned per-entry passes (LaunchCfg: block size and grid cap).
+// Grid of the tuned per-entry passes (LaunchCfg: block size and grid cap): the speed pass and
+// the first round's distance passes, the ones the tuner times.
 static int list_grid(felt_surface* s) {
     const size_t tpb = (size_t)s->cfg.tpb;
     return (int)std::min<size_t>((size_t)s->cfg.gcap,
                                  std::max<size_t>(1, (list_entries(s) + tpb - 1) / tpb));
+}
+// Grid of the other per-entry passes, in blocks of kListTpb threads: their lists (lookups,
+// status changes, expansions) are usually far shorter than their capacity, and a wider grid
+// only adds blocks that exit at once.
+constexpr int kListTpb = 256;
+static int fixed_grid(felt_surface* s) {
+    return (int)std::min<size_t>(4096, std::max<size_t>(1, (list_entries(s) + kListTpb - 1) /
+                                                                kListTpb));
 }
 static int chg_blocks(felt_surface* s) {
     return (int)std::max<size_t>(1, (list_entries(s) + BT * ITEMS - 1) / (BT * ITEMS));
@@ -963,7 +972,8 @@ static void compact_band(felt_surface* s) {
 // 1..L, each a Jacobi pass (distance into d, then phi = d and tk = its layer).
 static void distance_passes(felt_surface* s, bool from_band, int look, const int* live) {
     const Geo& G = s->G;
-    const int grid = list_grid(s), tpb = s->cfg.tpb;
+    const int grid = from_band ? list_grid(s) : fixed_grid(s);
+    const int tpb = from_band ? s->cfg.tpb : kListTpb;
     for (int pass = 0; pass < 2; pass++) {
         const int side = pass == 0 ? -1 : 1;
         for (int m = 1; m <= G.L; m++) {
@@ -1019,7 +1029,7 @@ static void compact_changes(felt_surface* s, const uint32_t* v, const int8_t* ta
 static void flush_expand(felt_surface* s, const int* live) {
     const Geo& G = s->G;
     const SC sc = sc_of(s);
-    const int grid = list_grid(s), tpb = s->cfg.tpb;
+    const int grid = fixed_grid(s), tpb = kListTpb;
     PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->smask, s->ukey,
             s->tflag, live);
     LAUNCH(k_deactivate, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P, s->ukey,
@@ -1041,8 +1051,8 @@ static void flush_expand(felt_surface* s, const int* live) {
 static void lookup_round(felt_surface* s, int look, int buf, const int* live,
                          bool clear_smask = false) {
     const int nmask = clear_smask ? s->G.np : 0;
-    LAUNCH(k_round_prep, dim3(std::max(list_grid(s), std::min(1024, blocks_for(nmask)))),
-           dim3(s->cfg.tpb), s->lu_tag[look].ptr, s->lu_tk[look].ptr, lu_span(s, look), s->smask, nmask,
+    LAUNCH(k_round_prep, dim3(std::max(fixed_grid(s), std::min(1024, blocks_for(nmask)))),
+           dim3(kListTpb), s->lu_tag[look].ptr, s->lu_tk[look].ptr, lu_span(s, look), s->smask, nmask,
            live);
     distance_passes(s, false, look, live);
     compact_changes(s, s->lu_v[look].ptr, s->lu_tag[look].ptr, s->lu_tk[look].ptr,
@@ -1091,7 +1101,7 @@ static void global_step_pre(felt_surface* s) {
 static void global_step_post(felt_surface* s) {
     const Geo& G = s->G;
     const int* zrng = s->bbase + G.L;
-    PLAUNCH(PK_APPLY, -(long long)G.L - 1, k_apply, dim3(list_grid(s)), dim3(s->cfg.tpb), G, s->P,
+    PLAUNCH(PK_APPLY, -(long long)G.L - 1, k_apply, dim3(fixed_grid(s)), dim3(kListTpb), G, s->P,
             s->band.ptr, s->band_d.ptr, zrng, s->band_tk.ptr, &s->ctr->nan_events);
     distance_passes(s, true, 0, always(s));
     compact_changes(s, s->band.ptr, s->band_tag.ptr, s->band_tk.ptr, &s->ctr->band_span[0], 1,
