@@ -353,40 +353,35 @@ __device__ __forceinline__ float distance_rec(const Geo& G, const Pool& P, uint3
     const float self = P.phi[v];
     const int rec[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
     const int ext[3] = {(int)((unsigned)rb.z & 0xffffu), (int)((unsigned)rb.z >> 16), rb.w};
-    // per neighbour: whether it is in the grid, and its voxel id, or for a neighbour in an
-    // inactive partition that partition (bg >= 0)
+    // per neighbour: whether it is in the grid, and where its value is -- a voxel of this brick
+    // or of the neighbour brick, or the background of an inactive neighbour partition.  Every
+    // neighbour is loaded (one outside the grid reads this voxel and is masked afterwards), so
+    // the loads are issued together with no branch between them.
     bool in[NN];
-    uint32_t vid[NN];
-    int bg[NN];
+    const float* src[NN];
 #pragma unroll
     for (int a = 0; a < D; a++) {
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int sd = h ? 1 : -1, j = 2 * a + h;
             const int la = loc[a] + sd;
-            bg[j] = -1;
-            if (la >= 0 && la < G.part[a]) {
-                int d[3] = {0, 0, 0};
-                d[a] = sd;
-                in[j] = la < ext[a];
-                vid[j] = v + (uint32_t)step_offset<D>(G, loc, d);
-            } else {
-                const int t = rec[j];
-                in[j] = t != kNbrOut;
-                int lc[3] = {loc[0], loc[1], loc[2]};
-                lc[a] = sd < 0 ? G.part[a] - 1 : 0;
-                vid[j] = (uint32_t)max(t, 0) * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, lc);
-                if (t < kNbrOut) bg[j] = kNbrInactive - t;
-            }
+            const bool cross = la < 0 || la >= G.part[a];
+            int d[3] = {0, 0, 0};
+            d[a] = sd;
+            int lc[3] = {loc[0], loc[1], loc[2]};
+            lc[a] = sd < 0 ? G.part[a] - 1 : 0;
+            const int t = rec[j];
+            const uint32_t own = v + (uint32_t)step_offset<D>(G, loc, d);
+            const uint32_t other = (uint32_t)max(t, 0) * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, lc);
+            in[j] = cross ? t != kNbrOut : la < ext[a];
+            src[j] = !cross ? P.phi + own
+                            : (t >= 0 ? P.phi + other : (t < kNbrOut ? P.pbg + (kNbrInactive - t)
+                                                                    : P.phi + v));
         }
     }
     float nv[NN];
 #pragma unroll
-    for (int j = 0; j < NN; j++) {
-        float x = 0.0f;
-        if (in[j]) x = bg[j] >= 0 ? P.pbg[bg[j]] : P.phi[vid[j]];
-        nv[j] = x;
-    }
+    for (int j = 0; j < NN; j++) nv[j] = *src[j];
     float dist = self * dir;
 #pragma unroll
     for (int j = 0; j < NN; j++)
