@@ -456,18 +456,25 @@ __device__ __forceinline__ void gather(const Geo& G, const Pool& P, const VoxCtx
             part[j] = -1;
         }
     }
+    // every page entry and value is loaded, with no branch between the loads: a neighbour
+    // outside the grid reads the voxel itself (and its own partition's page) and is replaced
+    // by L+1 afterwards
     int pg[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) pg[j] = (in[j] && part[j] >= 0) ? P.page[part[j]] : -1;
+    for (int j = 0; j < K; j++) pg[j] = P.page[(in[j] && part[j] >= 0) ? part[j] : c.own];
+    float raw[K];
 #pragma unroll
     for (int j = 0; j < K; j++) {
-        float x = (float)(G.L + 1);
-        if (in[j]) {
-            if (part[j] < 0) x = P.phi[(uint32_t)idx[j]];
-            else if (pg[j] < 0) x = P.pbg[part[j]];
-            else x = P.phi[(size_t)pg[j] * G.pv + idx[j]];
-        }
-        val[j] = x;
+        const float* src = P.phi + c.v;
+        if (in[j])
+            src = part[j] < 0 ? P.phi + (uint32_t)idx[j]
+                              : (pg[j] < 0 ? P.pbg + part[j]
+                                           : P.phi + ((size_t)pg[j] * G.pv + idx[j]));
+        raw[j] = *src;
+    }
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        val[j] = in[j] ? raw[j] : (float)(G.L + 1);
         if (in_grid) in_grid[j] = in[j];
     }
 }
