@@ -131,16 +131,23 @@ __device__ __forceinline__ void get_many(const Geo& G, const Pool& P, const MemV
                          : (uint32_t)leaf_of<D>(G, lq);
         part[j] = p;
     }
+    // every page entry and value is loaded, with no branch between the loads: a position
+    // outside the grid reads the brick's first voxel (and its member's first page entry) and
+    // is replaced by L+1 afterwards
     int pg[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) pg[j] = (in[j] && !same[j]) ? P.page[part[j]] : 0;
+    for (int j = 0; j < K; j++) pg[j] = P.page[(in[j] && !same[j]) ? part[j] : V.pbase];
+    float raw[K];
 #pragma unroll
     for (int j = 0; j < K; j++) {
-        float x = (float)(G.L + 1);
-        if (same[j]) x = P.phi[idx[j]];
-        else if (in[j]) x = pg[j] < 0 ? P.pbg[part[j]] : P.phi[(size_t)pg[j] * G.pv + idx[j]];
-        out[j] = x;
+        const float* src = P.phi + (size_t)V.slot * G.pv;
+        if (same[j]) src = P.phi + idx[j];
+        else if (in[j])
+            src = pg[j] < 0 ? P.pbg + part[j] : P.phi + ((size_t)pg[j] * G.pv + idx[j]);
+        raw[j] = *src;
     }
+#pragma unroll
+    for (int j = 0; j < K; j++) out[j] = in[j] ? raw[j] : (float)(G.L + 1);
 }
 
 // get(pos) at grid position q of the member.
@@ -159,12 +166,19 @@ __device__ __forceinline__ float get_v(const Geo& G, const Pool& P, const MemV& 
 // Corner mask of the cube based at grid position g (bit i: phi(corner i) > 0, Poly.hpp:356-368).
 template <int D>
 __device__ unsigned cube_mask(const Geo& G, const Pool& P, const MemV& V, const int g[3]) {
-    unsigned m = 0;
-    for (int c = 0; c < McT<D>::ncorner; c++) {
-        int q[3] = {0, 0, 0};
-        for (int a = 0; a < D; a++) q[a] = g[a] + McT<D>::calc_off(a) + McT<D>::corner(c, a);
-        m |= (unsigned)(get_v<D>(G, P, V, q) > 0.0f) << c;
+    constexpr int NC = McT<D>::ncorner;
+    int q[NC][3];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        q[c][0] = q[c][1] = q[c][2] = 0;
+#pragma unroll
+        for (int a = 0; a < D; a++) q[c][a] = g[a] + McT<D>::calc_off(a) + McT<D>::corner(c, a);
     }
+    float v[NC];
+    get_many<D, NC>(G, P, V, q, v);
+    unsigned m = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) m |= (unsigned)(v[c] > 0.0f) << c;
     return m;
 }
 
