@@ -172,30 +172,45 @@ __global__ void k_lu_seed(Lu lu, const uint32_t* z, const int* z_n, uint8_t* ama
 
 __global__ void k_snap_front(Ctr* ctr, int lu_i, int j) { ctr->front[j] = ctr->lu_span[lu_i][1]; }
 
-// Face neighbour d = (axis a, side sd) of voxel x for the affected search: its id and value
-// when its partition is active (active = true), else the partition background.
+// The face neighbours of band voxel v for the affected search, through its brick's face-neighbour
+// record: whether each is in the grid, whether its partition is active, its voxel id (active
+// ones) and its value (the partition background when inactive).  Every value is loaded, with no
+// branch between the loads (a neighbour outside the grid reads v itself).
 template <int D>
-__device__ __forceinline__ float aff_neighbour(const Geo& G, const Pool& P, const Vx& x, int a,
-                                               int sd, const int q[3], uint32_t& vq,
-                                               bool& active) {
-    const int la = x.loc[a] + sd;
-    if (la >= 0 && la < G.part[a]) {  // same brick
-        int d[3] = {0, 0, 0};
-        d[a] = sd;
-        vq = (uint32_t)((int64_t)x.v + step_offset<D>(G, x.loc, d));
-        active = true;
-        return P.phi[vq];
+__device__ __forceinline__ void aff_neighbours(const Geo& G, const Pool& P, uint32_t v, bool* in,
+                                               bool* active, uint32_t* vq, float* val) {
+    constexpr int NN = 2 * D;
+    const uint32_t slot = fdiv(v, G.fpv);
+    int loc[3] = {0, 0, 0};
+    leaf_coords<D>(G, v - slot * (uint32_t)G.pv, loc);
+    const int4 ra = P.nbr[2 * (size_t)slot], rb = P.nbr[2 * (size_t)slot + 1];
+    const int rec[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
+    const int ext[3] = {(int)((unsigned)rb.z & 0xffffu), (int)((unsigned)rb.z >> 16), rb.w};
+    const float* src[NN];
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int sd = h ? 1 : -1, j = 2 * a + h;
+            const int la = loc[a] + sd;
+            const bool cross = la < 0 || la >= G.part[a];
+            int d[3] = {0, 0, 0};
+            d[a] = sd;
+            int lc[3] = {loc[0], loc[1], loc[2]};
+            lc[a] = sd < 0 ? G.part[a] - 1 : 0;
+            const int t = rec[j];
+            const uint32_t own = v + (uint32_t)step_offset<D>(G, loc, d);
+            const uint32_t other = (uint32_t)max(t, 0) * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, lc);
+            in[j] = cross ? t != kNbrOut : la < ext[a];
+            active[j] = !cross || t >= 0;
+            vq[j] = !cross ? own : other;
+            src[j] = !cross ? P.phi + own
+                            : (t >= 0 ? P.phi + other : (t < kNbrOut ? P.pbg + (kNbrInactive - t)
+                                                                    : P.phi + v));
+        }
     }
-    int pq, lq;
-    locate<D>(G, P, x.m, q, pq, lq);
-    const int sq = P.page[pq];
-    if (sq < 0) {
-        active = false;
-        return P.pbg[pq];
-    }
-    vq = (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq;
-    active = true;
-    return P.phi[vq];
+#pragma unroll
+    for (int j = 0; j < NN; j++) val[j] = *src[j];
 }
 
 // calc_affected (Surface.hpp:780-913), one breadth-first round: face neighbours of the last
@@ -205,6 +220,7 @@ __device__ __forceinline__ float aff_neighbour(const Geo& G, const Pool& P, cons
 template <int D>
 __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j,
                                  uint8_t* amark) {
+    constexpr int NN = 2 * D;
     __shared__ int s_cnt, s_base;
     const int lo = ctr->front[j - 1], hi = ctr->front[j];
     const int n = hi - lo;
@@ -212,28 +228,29 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const int i = base + threadIdx.x;
         const bool valid = i < n;
-        Vx x;
         unsigned mask = 0, tags = 0;  // claimed directions; their layers, 4 bits each
+        uint32_t vq[NN];
         if (valid) {
-            decode<D>(G, P, lu.v[lo + i], x);
+            const uint32_t v = lu.v[lo + i];
+            bool in[NN], active[NN];
+            float dq[NN];
+            aff_neighbours<D>(G, P, v, in, active, vq, dq);
+            int kq[NN];
+            bool cand[NN];
 #pragma unroll
-            for (int a = 0; a < D; a++) {
+            for (int d = 0; d < NN; d++) {
+                kq[d] = in[d] ? layer_of(dq[d], G.L, &ctr->nan_events) : G.L + 1;
+                cand[d] = in[d] && active[d] && in_band(kq[d], G.L);
+            }
+            // most neighbours are already in the set: test before the atomic
+            uint8_t am[NN];
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    const int sd = t == 0 ? -1 : 1;
-                    int q[3] = {x.g[0], x.g[1], x.g[2]};
-                    q[a] += sd;
-                    if (!inside<D>(x.size, q)) continue;
-                    uint32_t vq = 0;
-                    bool active;
-                    const float dq = aff_neighbour<D>(G, P, x, a, sd, q, vq, active);
-                    const int kq = layer_of(dq, G.L, &ctr->nan_events);
-                    // most neighbours are already in the set: test before the atomic
-                    if (in_band(kq, G.L) && active && amark[vq] == 0 && claim_mark(&amark[vq])) {
-                        const int d = 2 * a + t;
-                        mask |= 1u << d;
-                        tags |= (unsigned)(kq + 8) << (4 * d);
-                    }
+            for (int d = 0; d < NN; d++) am[d] = amark[cand[d] ? vq[d] : v];
+#pragma unroll
+            for (int d = 0; d < NN; d++) {
+                if (cand[d] && am[d] == 0 && claim_mark(&amark[vq[d]])) {
+                    mask |= 1u << d;
+                    tags |= (unsigned)(kq[d] + 8) << (4 * d);
                 }
             }
         }
@@ -244,24 +261,12 @@ __global__ void k_affected_round(Geo G, Pool P, Lu lu, Ctr* ctr, int lu_i, int j
         if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(&ctr->lu_span[lu_i][1], s_cnt) : 0;
         __syncthreads();
         at += s_base;
-        if (mask) {
 #pragma unroll
-            for (int a = 0; a < D; a++) {
-#pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    const int d = 2 * a + t;
-                    if (!((mask >> d) & 1u)) continue;
-                    const int sd = t == 0 ? -1 : 1;
-                    int q[3] = {x.g[0], x.g[1], x.g[2]};
-                    q[a] += sd;
-                    uint32_t vq = 0;
-                    bool active;
-                    (void)aff_neighbour<D>(G, P, x, a, sd, q, vq, active);
-                    lu.v[at] = vq;
-                    lu.tag[at] = (int8_t)((int)((tags >> (4 * d)) & 15u) - 8);
-                    at++;
-                }
-            }
+        for (int d = 0; d < NN; d++) {
+            if (!((mask >> d) & 1u)) continue;
+            lu.v[at] = vq[d];
+            lu.tag[at] = (int8_t)((int)((tags >> (4 * d)) & 15u) - 8);
+            at++;
         }
     }
 }
