@@ -233,6 +233,8 @@ def run_single(args):
                                         amp=6.0 * size / 512.0)
     log(f"[bench] single: field {size}^3 built in {time.time() - t:.1f}s")
     s = felt_amd.Surface(3, L, (size,) * 3, (part,) * 3)
+    if args.plain_launches:
+        s.set_graph(False)
     s.load_dense(phi)
     del phi
     polys = felt_amd.Polys(s)
@@ -670,7 +672,8 @@ def main():
     ap.add_argument("--large-check", type=int, default=5,
                     help="steps between merge checks of the 3D workload")
     ap.add_argument("--plain-launches", action="store_true",
-                    help="plain surfaces launch their kernels one by one instead of replaying a "
+                    help="plain surfaces (the mixed workload's plain path and the single "
+                         "surface) launch their kernels one by one instead of replaying a "
                          "captured graph")
     ap.add_argument("--group-steps", type=int, default=25,
                     help="steps timed with the batches as one group (one launch per step)")
