@@ -148,6 +148,7 @@ class Binding:
             "surface_partition_active": (C.c_int, [_P, _U8P, _SZ]),
             "surface_partition_layer_size": (C.c_int, [_P, C.c_uint64, C.c_int, _U64P]),
             "surface_stats": (C.c_int, [_P, C.POINTER(felt_stats)]),
+            "surface_check_records": (C.c_int, [_P, _U64P]),
             "surface_sync": (C.c_int, [_P]),
             "surface_step_info": (C.c_int, [_P, _U64P, _U64P]),
             "surface_set_graph": (C.c_int, [_P, C.c_int]),
@@ -197,7 +198,8 @@ class Binding:
                                                             "surface_ray",
                                                             "surface_time_ray",
                                                             "surface_save",
-                                                            "surface_load"):
+                                                            "surface_load",
+                                                            "surface_check_records"):
                 continue
             f = getattr(L, prefix + name)
             f.restype = res
@@ -498,6 +500,12 @@ class Surface:
         st = felt_stats()
         self.b.call("surface_stats", self.h, C.byref(st))
         return st
+
+    def check_records(self) -> int:
+        """Active bricks whose face-neighbour record disagrees with the page table."""
+        n = C.c_uint64()
+        self.b.call("surface_check_records", self.h, C.byref(n))
+        return int(n.value)
 
     def set_graph(self, enable: bool) -> None:
         """Capture global device-functor steps as one HIP graph (default) or launch kernels

@@ -40,6 +40,7 @@ _SIGS = {
     "felt_batch_layer_size": [_P, C.c_int32, C.c_int, _U64P],
     "felt_batch_layer_positions": [_P, C.c_int32, C.c_int, _I32P, _SZ, _SZP],
     "felt_batch_stats": [_P, C.c_int32, C.POINTER(felt_stats)],
+    "felt_batch_check_records": [_P, _U64P],
     "felt_batch_ray": [_P, _SZ, _I32P, _F32P, _F32P, _F32P],
     "felt_batch_save": [_P, C.c_int32, C.POINTER(C.c_uint8), _SZ, _SZP],
     "felt_batch_load": [_P, C.c_int32, C.POINTER(C.c_uint8), _SZ],
@@ -197,6 +198,12 @@ class Batch:
         st = felt_stats()
         self.api("felt_batch_stats", self.h, int(i), C.byref(st))
         return st
+
+    def check_records(self) -> int:
+        """Active bricks whose face-neighbour record disagrees with the page table."""
+        n = C.c_uint64()
+        self.api("felt_batch_check_records", self.h, C.byref(n))
+        return int(n.value)
 
     def ray(self, members, origins, dirs) -> np.ndarray:
         """Surface::ray of ray i against member members[i]; a miss reads FLT_MAX."""

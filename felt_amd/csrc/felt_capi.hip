@@ -657,6 +657,12 @@ API int felt_surface_stats(felt_surface* s, felt_stats* st) {
     return stats_m(s, 0, st);
 }
 
+API int felt_surface_check_records(felt_surface* s, uint64_t* mismatches) {
+    CHECK_S(s);
+    if (!mismatches) return felt_fail(FELT_E_ARG, "null argument");
+    return felt_nbr_check(s, mismatches);
+}
+
 API int felt_surface_set_graph(felt_surface* s, int enable) {
     CHECK_S(s);
     s->use_graph = enable != 0;
@@ -989,6 +995,13 @@ API int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t*
     CHECK_IDLE(b);
     CHECK_M(b, i);
     return layer_positions_m(b->s, i, layer, pos, cap, n);
+}
+
+API int felt_batch_check_records(felt_batch* b, uint64_t* mismatches) {
+    CHECK_B(b);
+    CHECK_IDLE(b);
+    if (!mismatches) return felt_fail(FELT_E_ARG, "null argument");
+    return felt_nbr_check(b->s, mismatches);
 }
 
 API int felt_batch_stats(felt_batch* b, int32_t i, felt_stats* st) {

@@ -1042,6 +1042,38 @@ __global__ void k_nbr_fix(Geo G, Pool P, const int* la, const int* n_a, const in
     }
 }
 
+// Counts the active bricks whose face-neighbour record differs from what the page table implies
+// (a consistency check for tests: k_nbr_fix keeps every active brick's record in step).
+template <int D>
+__global__ void k_nbr_check(Geo G, Pool P, int cap, int* bad) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += gridDim.x * blockDim.x) {
+        const int p = P.slot_part[s];
+        if (p < 0) continue;
+        const Mem& M = P.mem[P.part_mem[p]];
+        int pc[3] = {0, 0, 0};
+        part_coords<D>(M, p - M.pbase, pc);
+        int want[8] = {kNbrOut, kNbrOut, kNbrOut, kNbrOut, kNbrOut, kNbrOut, 0, 1};
+        for (int a = 0; a < D; a++)
+            for (int h = 0; h < 2; h++) {
+                const int sd = h ? 1 : -1, c = pc[a] + sd;
+                if (c < 0 || c >= M.csize[a] || c * G.part[a] >= M.size[a]) continue;
+                const int q = p + sd * M.cstr[a];
+                const int t = P.page[q];
+                want[2 * a + h] = t >= 0 ? t : kNbrInactive - q;
+            }
+        int ext[3] = {1, 1, 1};
+        for (int a = 0; a < D; a++) ext[a] = max(0, min(G.part[a], M.size[a] - pc[a] * G.part[a]));
+        want[6] = ext[0] | (ext[1] << 16);
+        want[7] = ext[2];
+        const int4 ra = P.nbr[2 * (size_t)s], rb = P.nbr[2 * (size_t)s + 1];
+        const int have[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+        bool ok = true;
+        for (int j = 0; j < 8; j++)
+            if ((j < 2 * D || j >= 6) && have[j] != want[j]) ok = false;
+        if (!ok) atomicAdd(bad, 1);
+    }
+}
+
 // expand_narrow_band phase 2 (Surface.hpp:1249-1355): every neighbour whose value is outside the
 // band gets its distance, and joins layer +-L if it is in no list (the value does not depend on
 // which expanding voxel claims it: the claimed voxel's nearest neighbour towards the zero curve

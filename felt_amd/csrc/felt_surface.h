@@ -244,6 +244,8 @@ int felt_alloc_listed(felt_surface* s, int grid);
 // Face-neighbour records brought in line with the page table for the partitions activated
 // (s->nlist) and deactivated (s->dlist) since they were last fixed; queued on the stream.
 int felt_nbr_fix(felt_surface* s);
+// Active bricks whose face-neighbour record disagrees with the page table (0 when consistent).
+int felt_nbr_check(felt_surface* s, uint64_t* bad);
 // felt_surface_ray without device allocation: rays staged through a fixed device array.
 int felt_ray_noalloc_impl(felt_surface* s, size_t n, const float* origins, const float* dirs,
                           float* out);

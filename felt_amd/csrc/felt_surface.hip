@@ -2012,6 +2012,17 @@ int felt_alloc_listed(felt_surface* s, int grid) {
     return felt_nbr_fix(s);
 }
 
+int felt_nbr_check(felt_surface* s, uint64_t* bad) {
+    const Geo& G = s->G;
+    HIPX(hipMemsetAsync(&s->ctr->misc, 0, sizeof(int), s->st));
+    DISPATCH(G.D, k_nbr_check, dim3(std::min(1024, std::max(1, (s->cap + TPB - 1) / TPB))),
+             dim3(TPB), G, s->P, s->cap, &s->ctr->misc);
+    int rc = read_ctr(s);
+    if (rc) return rc;
+    *bad = (uint64_t)s->hc.misc;
+    return FELT_OK;
+}
+
 int felt_nbr_fix(felt_surface* s) {
     const Geo& G = s->G;
     DISPATCH(G.D, k_nbr_fix, dim3(std::min(256, blocks_for(G.np))), dim3(TPB), G, s->P, s->nlist,

@@ -158,6 +158,11 @@ int felt_surface_partition_active(felt_surface* s, uint8_t* out, size_t n);
 /* Number of layer-k voxels in partition pos_idx (layer_size helper, test_Surface.cpp:1720-1737). */
 int felt_surface_partition_layer_size(felt_surface* s, uint64_t pos_idx, int layer, uint64_t* n);
 int felt_surface_stats(felt_surface* s, felt_stats* out);
+/* Not a reference member: a consistency check of the device layout, for tests.  Counts the active
+ * bricks whose face-neighbour record (the slots of the six neighbouring bricks, or which neighbour
+ * partitions are inactive or outside the grid, and the in-grid extent) disagrees with the page
+ * table; 0 after every operation. */
+int felt_surface_check_records(felt_surface* s, uint64_t* mismatches);
 /* ray(origin, dir) for n rays (Surface.hpp:468-616): origins and dirs hold n*D floats in world
  * coordinates, dirs unit vectors; out receives n*D floats, the hit on the zero curve or FLT_MAX
  * in every component for a miss (Surface::ray_miss, Surface.hpp:1805-1807).  Rays that are not
@@ -412,6 +417,8 @@ int felt_batch_layer_positions(felt_batch* b, int32_t i, int layer, int32_t* pos
                                size_t* n);
 /* Stats of member i; last_voxel_updates / last_rounds are those of the last batch step. */
 int felt_batch_stats(felt_batch* b, int32_t i, felt_stats* out);
+/* felt_surface_check_records over every member of the batch. */
+int felt_batch_check_records(felt_batch* b, uint64_t* mismatches);
 /* felt_surface_save / load for member i of a batch.  A load replaces the member's state (its
  * size, partition size and L must match the archive's); felt_batch_clear returns the member's
  * bricks to the pool and leaves it the empty grid of a newly created surface (for eviction). */
