@@ -339,18 +339,18 @@ __device__ __forceinline__ float distance(const Geo& G, const Pool& P, const Vx&
     return dn + dir;
 }
 
-// distance() of one band voxel through its brick's face-neighbour record: the record, the
-// voxel's own value and its in-brick neighbours are loaded together, then the values of the
-// neighbours in other bricks (or their partitions' backgrounds), so the gather is two dependent
-// levels after the list entry.  Same result as distance().
+// distance() of one band voxel through its brick's face-neighbour record, with the voxel's own
+// value given: the record and the voxel's in-brick neighbours are loaded together, then the
+// values of the neighbours in other bricks (or their partitions' backgrounds), so the gather is
+// two dependent levels after the list entry.  Same result as distance().
 template <int D>
-__device__ __forceinline__ float distance_rec(const Geo& G, const Pool& P, uint32_t v, float dir) {
+__device__ __forceinline__ float distance_rec_self(const Geo& G, const Pool& P, uint32_t v,
+                                                   float self, float dir) {
     constexpr int NN = 2 * D;
     const uint32_t slot = fdiv(v, G.fpv);
     int loc[3] = {0, 0, 0};
     leaf_coords<D>(G, v - slot * (uint32_t)G.pv, loc);
     const int4 ra = P.nbr[2 * (size_t)slot], rb = P.nbr[2 * (size_t)slot + 1];
-    const float self = P.phi[v];
     const int rec[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
     const int ext[3] = {(int)((unsigned)rb.z & 0xffffu), (int)((unsigned)rb.z >> 16), rb.w};
     // per neighbour: whether it is in the grid, and where its value is -- a voxel of this brick
@@ -388,6 +388,12 @@ __device__ __forceinline__ float distance_rec(const Geo& G, const Pool& P, uint3
         if (in[j] && nv[j] * dir < dist) dist = nv[j] * dir;
     const float dn = dist * dir;
     return dn + dir;
+}
+
+// The same, reading the voxel's own value alongside the record.
+template <int D>
+__device__ __forceinline__ float distance_rec(const Geo& G, const Pool& P, uint32_t v, float dir) {
+    return distance_rec_self<D>(G, P, v, P.phi[v], dir);
 }
 
 // A band voxel with what neighbour gathers need: coordinates in its brick and grid, its grid's

@@ -1078,8 +1078,9 @@ __global__ void k_nbr_check(Geo G, Pool P, int cap, int* bad) {
 // band gets its distance, and joins layer +-L if it is in no list (the value does not depend on
 // which expanding voxel claims it: the claimed voxel's nearest neighbour towards the zero curve
 // is in the band already, so claims racing for one voxel write the same value).  The neighbours'
-// pages, then their values, then the claims are issued level by level; each claimed voxel's
-// distance gathers its own face neighbours.
+// bricks come from the expanding voxel's brick record, then their values and the claims are
+// issued level by level; each claimed voxel's distance gathers its own face neighbours through
+// its brick's record.
 template <int D>
 __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, const int* live) {
     if (!*live) return;
@@ -1094,36 +1095,37 @@ __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, 
 #pragma unroll
         for (int j = 0; j < NN; j++) cp[j] = -1;
         if (cand) {
-            VoxCtx x;
-            vox_ctx<D>(G, P, sc.xv[i], x);
-            bool in[NN], cross[NN];
-            int pq[NN], lq[NN], lc[NN][3];
+            // the expanding voxel's face neighbours through its brick's record: in this brick, or
+            // in an active neighbour brick (an inactive one failed activation -- pool exhausted,
+            // flagged -- and is skipped, as one outside the grid)
+            const uint32_t vx = sc.xv[i];
+            const uint32_t slot = fdiv(vx, G.fpv);
+            int loc[3] = {0, 0, 0};
+            leaf_coords<D>(G, vx - slot * (uint32_t)G.pv, loc);
+            const int4 ra = P.nbr[2 * (size_t)slot], rb = P.nbr[2 * (size_t)slot + 1];
+            const int rec[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
+            const int ext[3] = {(int)((unsigned)rb.z & 0xffffu), (int)((unsigned)rb.z >> 16), rb.w};
+            const int own = P.slot_part[slot];
+            bool in[NN];
+            uint32_t vq[NN];
+            int sq[NN];
 #pragma unroll
             for (int a = 0; a < D; a++)
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const int sd = h ? 1 : -1, j = 2 * a + h;
-                    const int qa = x.g[a] + sd, la = x.loc[a] + sd;
-                    in[j] = qa >= 0 && qa < x.sz[a];
-                    cross[j] = la < 0 || la >= G.part[a];
-                    lc[j][0] = x.loc[0];
-                    lc[j][1] = x.loc[1];
-                    lc[j][2] = x.loc[2];
-                    lc[j][a] = la < 0 ? la + G.part[a] : (la >= G.part[a] ? la - G.part[a] : la);
-                    pq[j] = cross[j] ? x.own + sd * x.cs[a] : x.own;
+                    const int la = loc[a] + sd;
+                    const bool cross = la < 0 || la >= G.part[a];
                     int d[3] = {0, 0, 0};
                     d[a] = sd;
-                    lq[j] = cross[j] ? leaf_of<D>(G, lc[j]) : (int)x.v + step_offset<D>(G, x.loc, d);
+                    int lc[3] = {loc[0], loc[1], loc[2]};
+                    lc[a] = sd < 0 ? G.part[a] - 1 : 0;
+                    const int t = rec[j];
+                    in[j] = cross ? t >= 0 : la < ext[a];
+                    sq[j] = cross ? max(t, 0) : (int)slot;
+                    vq[j] = cross ? (uint32_t)max(t, 0) * (uint32_t)G.pv + (uint32_t)leaf_of<D>(G, lc)
+                                  : vx + (uint32_t)step_offset<D>(G, loc, d);
                 }
-            uint32_t vq[NN];
-#pragma unroll
-            for (int j = 0; j < NN; j++) {
-                const int sq = (in[j] && cross[j]) ? P.page[pq[j]] : 0;
-                // same brick: lq is the voxel id; an inactive partition failed activation (pool
-                // exhausted, flagged) and is skipped
-                if (!in[j] || sq < 0) in[j] = false;
-                vq[j] = cross[j] ? (uint32_t)sq * (uint32_t)G.pv + (uint32_t)lq[j] : (uint32_t)lq[j];
-            }
             float dq[NN];
 #pragma unroll
             for (int j = 0; j < NN; j++)
@@ -1142,29 +1144,9 @@ __global__ void k_expand_claim(Geo G, Pool P, SC sc, const Ctr* ctrc, Ctr* ctr, 
 #pragma unroll
             for (int j = 0; j < NN; j++) {
                 if (!cl[j]) continue;
-                VoxCtx y;
-                y.v = vq[j];
-                y.m = x.m;
-                y.own = pq[j];
-#pragma unroll
-                for (int b = 0; b < 3; b++) {
-                    y.loc[b] = lc[j][b];
-                    y.g[b] = x.g[b];
-                    y.sz[b] = x.sz[b];
-                    y.cs[b] = x.cs[b];
-                }
-                y.g[j / 2] += (j & 1) ? 1 : -1;
-                float nv[NN];
-                bool nin[NN];
-                gather<D, NN>(G, P, y, D == 3 ? kStencil3 : kStencil2, nv, nin);
-                const float dir = (float)side;
-                float dist = dq[j] * dir;
-#pragma unroll
-                for (int k = 0; k < NN; k++)
-                    if (nin[k] && nv[k] * dir < dist) dist = nv[k] * dir;
-                const float nd = dist * dir + dir;
+                const float nd = distance_rec_self<D>(G, P, vq[j], dq[j], (float)side);
                 __hip_atomic_store(&P.phi[vq[j]], nd, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (fresh[j]) cp[j] = pq[j];
+                if (fresh[j]) cp[j] = sq[j] == (int)slot ? own : P.slot_part[sq[j]];
             }
         }
 #pragma unroll
