@@ -182,41 +182,59 @@ __device__ unsigned cube_mask(const Geo& G, const Pool& P, const MemV& V, const 
     return m;
 }
 
-// Row-major leaf PosIdx (within the brick in slot s, whose first voxel is at grid position org)
-// of the band cube at grid base position b, or -1.
+// Band bits of the cubes at b + d, d in {-1, 0, 1}^D with at least one zero component (every cube
+// that can share an edge with the cube at b): bit (d0+1)*9 + (d1+1)*3 + (d2+1) is set when that
+// cube's base voxel lies in the marched brick and in the grid and is in a layer list.  The layer
+// bytes are loaded together, with no branch between the loads (a cube outside reads the brick's
+// first voxel and is masked).
 template <int D>
-__device__ int band_cube(const Geo& G, const Pool& P, const MemV& V, int s, const int org[3],
-                         const int b[3]) {
-    int lc[3] = {0, 0, 0};
-    for (int a = 0; a < D; a++) {
-        lc[a] = b[a] - org[a];
-        if (lc[a] < 0 || lc[a] >= G.part[a] || b[a] >= V.size[a]) return -1;
+__device__ __forceinline__ unsigned band_mask_around(const Geo& G, const Pool& P, const MemV& V,
+                                                     int s, const int org[3], const int b[3]) {
+    constexpr int NK = D == 3 ? 27 : 9;
+    const size_t base = (size_t)s * G.pv;
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        const int d[3] = {D == 3 ? k / 9 - 1 : k / 3 - 1, D == 3 ? (k / 3) % 3 - 1 : k % 3 - 1,
+                          D == 3 ? k % 3 - 1 : 0};
+        if (d[0] != 0 && d[1] != 0 && (D == 2 || d[2] != 0)) continue;  // shares no edge
+        int lc[3] = {0, 0, 0};
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            lc[a] = b[a] + d[a] - org[a];
+            if (lc[a] < 0 || lc[a] >= G.part[a] || b[a] + d[a] >= V.size[a]) ok = false;
+        }
+        const int8_t lay = P.lay[ok ? base + leaf_of<D>(G, lc) : base];
+        if (ok && lay != NONE) m |= 1u << k;
     }
-    return P.lay[(size_t)s * G.pv + leaf_of<D>(G, lc)] != NONE ? rm_of<D>(G, lc) : -1;
+    return m;
 }
 
-// Owner of the vertex on edge e of the cube at base b: the lowest-leaf band cube of the
-// partition among those sharing the edge.  Returns its leaf, the edge's index in its frame and
-// which of the sharers (McT::axis_edge order) it is.
+// Owner of the vertex on edge e of the cube at base b: the lowest-leaf band cube of the partition
+// among those sharing the edge (the cubes at b + edge(e) - edge(f) for the edges f along e's
+// axis), found from the band bits of the cubes around b (band_mask_around).  Returns the owner's
+// row-major leaf PosIdx, and which of the sharers (McT::axis_edge order) it is in cand.
 template <int D>
-__device__ int edge_owner(const Geo& G, const Pool& P, const MemV& V, int s, const int org[3],
-                          const int b[3], int e, int* e_owner, int* cand) {
-    int pa[3] = {0, 0, 0};
-    for (int a = 0; a < D; a++) pa[a] = b[a] + McT<D>::calc_off(a) + McT<D>::edge(e, a);
+__device__ __forceinline__ int edge_owner_m(const Geo& G, unsigned mask, const int org[3],
+                                            const int b[3], int e, int* cand) {
     const int axis = McT<D>::edge(e, D);
-    int best = -1, best_e = e, best_i = 0;
+    int best = -1, best_i = 0;
     for (int i = 0; i < McT<D>::per_axis; i++) {
         const int f = McT<D>::axis_edge(axis, i);
-        int cb[3] = {0, 0, 0};
-        for (int a = 0; a < D; a++) cb[a] = pa[a] - McT<D>::edge(f, a) - McT<D>::calc_off(a);
-        const int l = band_cube<D>(G, P, V, s, org, cb);
-        if (l >= 0 && (best < 0 || l < best)) {
+        int d[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) d[a] = McT<D>::edge(e, a) - McT<D>::edge(f, a);
+        const int k = D == 3 ? (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1)
+                             : (d[0] + 1) * 3 + (d[1] + 1);
+        if (!((mask >> k) & 1u)) continue;
+        int lc[3] = {0, 0, 0};
+        for (int a = 0; a < D; a++) lc[a] = b[a] + d[a] - org[a];
+        const int l = rm_of<D>(G, lc);
+        if (best < 0 || l < best) {
             best = l;
-            best_e = f;
             best_i = i;
         }
     }
-    *e_owner = best_e;
     *cand = best_i;
     return best;
 }
@@ -488,11 +506,12 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
         for (int a = 0; a < D; a++) g[a] += org[a];
         const unsigned m = cube_mask<D>(G, P, V, g);
         const unsigned em = McT<D>::edges_of(m);
+        const unsigned bm = em ? band_mask_around<D>(G, P, V, s, org, g) : 0u;
         unsigned om = 0, oc = 0;
         for (int e = 0; e < McT<D>::nedge; e++) {
             if (!((em >> e) & 1)) continue;
-            int eo, ci;
-            if (edge_owner<D>(G, P, V, s, org, g, e, &eo, &ci) == l) om |= 1u << e;
+            int ci;
+            if (edge_owner_m<D>(G, bm, org, g, e, &ci) == l) om |= 1u << e;
             oc |= (unsigned)ci << (2 * e);
         }
         cmask[(size_t)j * G.pv + l] = (uint8_t)m;
