@@ -667,20 +667,37 @@ __global__ void __launch_bounds__(TPB) k_m_tri(Geo G, const int* item_j, const i
     const int base = vst[j];
     uint32_t* tout = tarena + (size_t)(tdst[j] + (tb[i] - tst[j])) * D;
     const unsigned oc = ocand[jb + l];
-    for (int t = 0; t < ns; t++) {
-        for (int k = 0; k < D; k++) {
+    // every simplex vertex's owner first, then all their loads together (no branch between
+    // them: the slots past the cube's simplices read the cube itself), then the stores
+    constexpr int NV = (D == 3 ? 5 : 2) * D;  // at most 5 triangles (3D) or 2 segments (2D)
+    int ow[NV], eo[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        const int t = q / D, k = q % D;
+        ow[q] = l;
+        eo[q] = 0;
+        if (t < ns) {
             const int e = McT<D>::simplex_edge(m, t, k);
             // sharer ci of the edge owns its vertex, as its edge eo
             const int axis = McT<D>::edge(e, D);
-            const int eo = McT<D>::axis_edge(axis, (int)((oc >> (2 * e)) & 3u));
+            eo[q] = McT<D>::axis_edge(axis, (int)((oc >> (2 * e)) & 3u));
             int lc[3] = {0, 0, 0};
-            for (int a = 0; a < D; a++) lc[a] = lc0[a] + McT<D>::edge(e, a) - McT<D>::edge(eo, a);
-            const int ow = rm_of<D>(G, lc);
-            const int idx =
-                vleaf[jb + ow] + __popc((unsigned)own[jb + ow] & ((1u << eo) - 1u)) - base;
-            tout[(size_t)t * D + k] = (uint32_t)idx;
+            for (int a = 0; a < D; a++)
+                lc[a] = lc0[a] + McT<D>::edge(e, a) - McT<D>::edge(eo[q], a);
+            ow[q] = rm_of<D>(G, lc);
         }
     }
+    int vl[NV];
+    unsigned om[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        vl[q] = vleaf[jb + ow[q]];
+        om[q] = own[jb + ow[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++)
+        if (q / D < ns)
+            tout[q] = (uint32_t)(vl[q] + __popc(om[q] & ((1u << eo[q]) - 1u)) - base);
 }
 
 // Copies live mesh segments into a fresh arena when the old one is compacted.
