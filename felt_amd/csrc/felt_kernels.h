@@ -30,6 +30,11 @@ struct Ctr {
     int front[20];       // breadth-first frontier bounds of calc_affected
     int reserve;         // bricks the step may activate at most (k_reserve_count)
     int dead_n;          // partitions deactivated since the neighbour records were last fixed
+    // A bounded step's affected set sorted by layer for its first round: the entries of layer
+    // k - L (k != L) are seg0[k] .. seg0[k + 1]; seg0_cnt and seg0_cur count and place them.
+    int seg0[18];
+    int seg0_cnt[17];
+    int seg0_cur[17];
 };
 
 // Status-change record: the reference's StatusChangeGrid entry (Surface.hpp:1138-1178).

@@ -90,6 +90,9 @@ struct felt_surface {
     DBuf<int8_t> lu_tag[2];
     DBuf<float> lu_d[2];
     DBuf<int8_t> lu_tk[2];
+    // a bounded step's affected set sorted by layer (first round only)
+    DBuf<uint32_t> srt_v;
+    DBuf<int8_t> srt_tag;
     // status-change records
     DBuf<uint32_t> sc_v;
     DBuf<int8_t> sc_from;

@@ -170,6 +170,71 @@ __global__ void k_lu_seed(Lu lu, const uint32_t* z, const int* z_n, uint8_t* ama
     }
 }
 
+// A bounded step's affected set sorted by layer for its first round's distance passes (the
+// delta list at its front stays in place and in order; the other entries go to their layer's
+// segment, in an order that may differ from run to run -- the passes treat the set as a set).
+// Counts per layer, one atomic per layer per block.
+constexpr int kSegItems = 16;  // entries per thread of the sort
+__global__ void k_seg_count(const int8_t* tag, const int* z_n, const int* n, int L, Ctr* ctr) {
+    __shared__ int cnt[17];
+    const int NL = 2 * L + 1;
+    if (threadIdx.x < NL) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int lo = *z_n, hi = *n;
+    const int chunk = blockDim.x * kSegItems;
+    for (int b = lo + blockIdx.x * chunk; b < hi; b += gridDim.x * chunk)
+        for (int i = b + threadIdx.x; i < min(b + chunk, hi); i += blockDim.x)
+            atomicAdd(&cnt[tag[i] + L], 1);
+    __syncthreads();
+    if (threadIdx.x < NL && cnt[threadIdx.x]) atomicAdd(&ctr->seg0_cnt[threadIdx.x], cnt[threadIdx.x]);
+}
+
+__global__ void k_seg_scan(const int* z_n, int NL, Ctr* ctr) {
+    int at = *z_n;
+    for (int k = 0; k < NL; k++) {
+        ctr->seg0[k] = at;
+        ctr->seg0_cur[k] = at;
+        at += ctr->seg0_cnt[k];
+        ctr->seg0_cnt[k] = 0;
+    }
+    ctr->seg0[NL] = at;
+}
+
+// Each block claims, per layer, one run of the layer's segment for its chunk, and places its
+// entries in the run by a block-local counter.
+__global__ void k_seg_scatter(const uint32_t* v, const int8_t* tag, const int* z_n, const int* n,
+                              int L, uint32_t* out_v, int8_t* out_tag, Ctr* ctr) {
+    __shared__ int cnt[17], base[17];
+    const int NL = 2 * L + 1;
+    const int lo = *z_n, hi = *n;
+    // the delta list keeps its place
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < lo; i += gridDim.x * blockDim.x) {
+        out_v[i] = v[i];
+        out_tag[i] = tag[i];
+    }
+    const int chunk = blockDim.x * kSegItems;
+    for (int b = lo + blockIdx.x * chunk; b < hi; b += gridDim.x * chunk) {
+        const int e = min(b + chunk, hi);
+        if (threadIdx.x < NL) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        for (int i = b + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&cnt[tag[i] + L], 1);
+        __syncthreads();
+        if (threadIdx.x < NL) {
+            base[threadIdx.x] =
+                cnt[threadIdx.x] ? atomicAdd(&ctr->seg0_cur[threadIdx.x], cnt[threadIdx.x]) : 0;
+            cnt[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
+            const int k = tag[i] + L;
+            const int at = base[k] + atomicAdd(&cnt[k], 1);
+            out_v[at] = v[i];
+            out_tag[at] = tag[i];
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_snap_front(Ctr* ctr, int lu_i, int j) { ctr->front[j] = ctr->lu_span[lu_i][1]; }
 
 // The face neighbours of band voxel v for the affected search, through its brick's face-neighbour
@@ -835,6 +900,8 @@ void felt_surface_release(felt_surface* s) {
     s->band_tk.release();
     s->chg_cnt.release();
     for (int i = 0; i < 2; i++) s->lu_tk[i].release();
+    s->srt_v.release();
+    s->srt_tag.release();
     s->blk_cnt.release();
     s->blk_off.release();
     s->visit.release();
@@ -890,6 +957,8 @@ static int ensure_step_capacity(felt_surface* s, size_t band) {
         if ((rc = s->lu_d[i].ensure(n))) return rc;
         if ((rc = s->lu_tk[i].ensure(n))) return rc;
     }
+    if ((rc = s->srt_v.ensure(n))) return rc;
+    if ((rc = s->srt_tag.ensure(n))) return rc;
     if ((rc = s->sc_v.ensure(n))) return rc;
     if ((rc = s->sc_from.ensure(n))) return rc;
     if ((rc = s->sc_to.ensure(n))) return rc;
@@ -927,6 +996,9 @@ static int list_grid(felt_surface* s) {
 // status changes, expansions) are usually far shorter than their capacity, and a wider grid
 // only adds blocks that exit at once.
 constexpr int kListTpb = 256;
+// A bounded step sorts its affected set by layer when it holds at least this many entries
+// beyond the delta list.
+constexpr int kSortedMin = 1 << 16;
 static int fixed_grid(felt_surface* s) {
     return (int)std::min<size_t>(4096, std::max<size_t>(1, (list_entries(s) + kListTpb - 1) /
                                                                 kListTpb));
@@ -975,7 +1047,10 @@ static void compact_band(felt_surface* s) {
 
 // The layered distance transform of one round (Surface.hpp:988-1003): layers -1..-L then
 // 1..L, each a Jacobi pass (distance into d, then phi = d and tk = its layer).
-static void distance_passes(felt_surface* s, bool from_band, int look, const int* live) {
+// With `sorted` (a bounded step's first round), the lookup's entries are read from the affected set
+// sorted by layer (srt_v, segments seg0) and the passes touch only their layer's segment.
+static void distance_passes(felt_surface* s, bool from_band, int look, const int* live,
+                            bool sorted = false) {
     const Geo& G = s->G;
     const int grid = from_band ? list_grid(s) : fixed_grid(s);
     const int tpb = from_band ? s->cfg.tpb : kListTpb;
@@ -1008,6 +1083,12 @@ static void distance_passes(felt_surface* s, bool from_band, int look, const int
                           rng, layer, d, live);
                 PLAUNCH(PK_DIST2, spec, k_dist2_band, dim3(grid), dim3(tpb), G, s->P, list, rng,
                         layer, d, tk, &s->ctr->nan_events, live);
+            } else if (sorted) {
+                const int* seg = &s->ctr->seg0[layer + G.L];
+                PDISPATCH(PK_DIST1_LATE, 0, G.D, k_dist1_band, dim3(grid), dim3(tpb), G, s->P,
+                          s->srt_v.ptr, seg, layer, d, live);
+                PLAUNCH(PK_DIST2_LATE, 0, k_dist2_band, dim3(grid), dim3(tpb), G, s->P, s->srt_v.ptr,
+                        seg, layer, d, tk, &s->ctr->nan_events, live);
             } else {
                 PDISPATCH(PK_DIST1_LATE, 0, G.D, k_dist1_lookup, dim3(grid), dim3(tpb), G, s->P,
                           list, tag, rng, layer, d, live);
@@ -1766,14 +1847,30 @@ int felt_update_end_impl(felt_surface* s) {
                                            std::to_string(s->hc.free_top) +
                                            " bricks are free; the surface is unchanged");
     }
-    LAUNCH(k_tk_init, dim3(grid), dim3(TPB), A.tag, A.tk, lu_span(s, 0), always(s));
+    // A large affected set is sorted by layer first, so that each of the round's 2L distance
+    // passes reads its own layer's entries only; the round then processes the sorted copy (same
+    // entries, the delta list still first).
+    const bool sorted = an - s->hc.z_span[1] >= kSortedMin;
+    const uint32_t* rv = A.v;
+    const int8_t* rtag = A.tag;
+    if (sorted) {
+        const int sg = std::min(1024, std::max(1, (an + TPB * kSegItems - 1) / (TPB * kSegItems)));
+        LAUNCH(k_seg_count, dim3(sg), dim3(TPB), A.tag, &s->ctr->z_span[1], &s->ctr->lu_span[0][1],
+               G.L, s->ctr);
+        LAUNCH(k_seg_scan, dim3(1), dim3(1), &s->ctr->z_span[1], G.NL, s->ctr);
+        LAUNCH(k_seg_scatter, dim3(sg), dim3(TPB), A.v, A.tag, &s->ctr->z_span[1],
+               &s->ctr->lu_span[0][1], G.L, s->srt_v.ptr, s->srt_tag.ptr, s->ctr);
+        rv = s->srt_v.ptr;
+        rtag = s->srt_tag.ptr;
+    }
+    LAUNCH(k_tk_init, dim3(grid), dim3(TPB), rtag, A.tk, lu_span(s, 0), always(s));
     prof_begin(s, PK_APPLY, 0);
     LAUNCH(k_apply, dim3(grid), dim3(TPB), G, s->P, s->zlist.ptr, s->zdelta.ptr,
            &s->ctr->z_span[0], A.tk, &s->ctr->nan_events);
     prof_end(s);
     int look = 0, buf = 1;
-    distance_passes(s, false, look, always(s));
-    compact_changes(s, A.v, A.tag, A.tk, lu_span(s, 0), buf, always(s));
+    distance_passes(s, false, look, always(s), sorted);
+    compact_changes(s, rv, rtag, A.tk, lu_span(s, 0), buf, always(s));
     flush_expand(s, always(s));
     if ((rc = read_ctr(s))) { s->hint = -1; return rc; }
     int rounds = 1;
@@ -2163,13 +2260,14 @@ int felt_member_bands(felt_surface* s, const int* ids, int n, std::vector<long l
 uint64_t felt_device_bytes(const felt_surface* s) {
     const Geo& G = s->G;
     const uint64_t np = (uint64_t)G.np, cap = (uint64_t)s->cap, pv = (uint64_t)G.pv;
-    // phi, lay, amark per voxel; slot tables; partition tables
-    uint64_t b = cap * pv * (sizeof(float) + 2) + cap * (sizeof(int) * 2 + sizeof(int4));
-    b += np * (sizeof(int) * 9 + sizeof(float) + sizeof(int) * G.NL * 3);
+    // phi, lay, amark per voxel; slot tables (partition, free stack, origin, neighbour record);
+    // partition tables
+    uint64_t b = cap * pv * (sizeof(float) + 2) + cap * (sizeof(int) * 2 + sizeof(int4) * 3);
+    b += np * (sizeof(int) * 10 + sizeof(float) + sizeof(int) * G.NL * 3);
     b += (uint64_t)s->mems.size() * (sizeof(Mem) + sizeof(Speed));
     // per-step lists: band (4+4+1+1), two lookups (4+1+4+1), status changes (4+1+2),
-    // zero-layer list (4+4)
-    b += (uint64_t)s->list_cap * (10 + 2 * 10 + 7 + 8);
+    // zero-layer list (4+4), a bounded step's affected set sorted by layer (4+1)
+    b += (uint64_t)s->list_cap * (10 + 2 * 10 + 7 + 8 + 5);
     b += (uint64_t)s->vox.cap * sizeof(float);
     return b;
 }

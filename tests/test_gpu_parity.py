@@ -310,3 +310,19 @@ def test_layer_counts(D, L, size, part):
         else:
             step_both(g, c, "update", sp)
         check(g, c, f"L={L} step {i}")
+
+
+def test_bounded_steps_over_large_affected_sets():
+    """Bounded steps whose affected set is large enough (over 65536 voxels beyond the delta list)
+    for the device to sort it by layer before the first round's distance passes."""
+    size, part, L = (128, 128, 96), (16, 16, 16), 3
+    cs, rs = [(-8.0, 5.0, 2.0), (20.0, -12.0, -6.0)], [34.0, 22.0]
+    g, c = pair(3, L, size, part)
+    step_both(g, c, "load_dense", workloads.spheres_phi(size, L, cs, rs))
+    lo, hi = [-64, -64, -48], [63, 63, 47]
+    for i in range(4):
+        sp = speed_const_curv(-0.4, 0.2) if i % 2 == 0 else speed_curvature(0.3)
+        step_both(g, c, "update_box", lo, hi, sp)
+        check(g, c, f"bounded step {i}")
+    assert g.step_info()[0] > 70000  # voxel updates of the last step: the affected set
+    print(f"[bounded-large] ok, affected {g.step_info()[0]}", flush=True)
