@@ -14,6 +14,8 @@
 //   slot_part[cap]  brick slot -> partition, -1 when free
 //   slot_org[cap]   brick slot -> grid coordinates of its partition's first voxel, and member
 //                   (bits 0-29) with bit 30 set when the brick touches a face of its grid
+//   nbr[2*cap]      brick slot -> its face-neighbour record (the six neighbour bricks, or which
+//                   neighbour partitions are inactive or outside the grid; its in-grid extent)
 //   phi[cap*pv]     signed distances, one brick of pv = prod(part) voxels per slot, stored in
 //                   tiles (below)
 //   lay[cap*pv]     layer list of each voxel (-L..L) or NONE: replaces the reference's per-child
