@@ -19,7 +19,7 @@ namespace felt {
 namespace step {
 
 constexpr int BT = 1024;  // threads of the scan / compaction blocks
-constexpr int ITEMS = 4;  // list entries per thread in the change compaction
+constexpr int ITEMS = 16;  // consecutive list entries per thread in the change compaction
 
 // Exclusive scan of one int per thread over a block of BT threads; returns the block total.
 __device__ __forceinline__ int block_excl_scan(int val, int* total) {
@@ -51,6 +51,49 @@ __device__ __forceinline__ int block_excl_scan(int val, int* total) {
     *total = wsum[0];
     __syncthreads();
     return res;
+}
+
+// Exclusive scans of three ints per thread over a block of BT threads, with one set of barriers
+// for the three; tot[c] is the block total of column c.
+__device__ __forceinline__ void block_excl_scan3(const int v[3], int ex[3], int tot[3]) {
+    constexpr int NW = BT / 64;
+    __shared__ int wsum3[3][NW];
+    __shared__ int wpre3[3][NW + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x[3] = {v[0], v[1], v[2]};
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int y = __shfl_up(x[c], o);
+            if (lane >= o) x[c] += y;
+        }
+    }
+    if (lane == 63)
+#pragma unroll
+        for (int c = 0; c < 3; c++) wsum3[c][wave] = x[c];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int w = lane < NW ? wsum3[c][lane] : 0;
+            int z = w;
+#pragma unroll
+            for (int o = 1; o < NW; o <<= 1) {
+                const int y = __shfl_up(z, o);
+                if (lane >= o) z += y;
+            }
+            if (lane < NW) wpre3[c][lane] = z - w;
+            if (lane == NW - 1) wpre3[c][NW] = z;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        ex[c] = wpre3[c][wave] + x[c] - v[c];
+        tot[c] = wpre3[c][NW];
+    }
+    __syncthreads();  // the shared arrays are free again
 }
 
 // Adds +1 to pcnt[p][inc] and -1 to pcnt[p][dec] (-1 = none) for every active lane, with one
@@ -682,69 +725,97 @@ __device__ __forceinline__ bool expanding(const Geo& G, int from, int to) {
     return (from == G.L || from == -G.L) && to == G.L * side - side;
 }
 
-// Change compaction, pass 1: the list is cut into tiles of BT * ITEMS entries; per tile the
-// number of status changes and of in-band targets.  Raises the round flag for a change out of a
-// non-zero layer (Surface.hpp:1116).  Blocks stride over the tiles the span needs, so a short
-// span costs few blocks whatever the grid.
+// Change compaction, pass 1: the list is cut into tiles of BT * ITEMS entries, each thread taking
+// ITEMS consecutive entries (one 16-byte load of the old and of the new layer bytes); per tile the
+// number of status changes, of in-band targets and of expansions.  Raises the round flag for a
+// change out of a non-zero layer (Surface.hpp:1116).  Blocks stride over the tiles the span
+// needs, so a short span costs few blocks whatever the grid.
 constexpr int kTile = BT * ITEMS;
 
-__global__ void k_chg_count(Geo G, const int8_t* tag, const int8_t* tk, const int* rng, int* counts,
-                            Ctr* ctr, const int* live) {
+// The layer bytes (old, new) of a thread's ITEMS entries starting at i0; entries past n read as
+// unchanged.  The lists are allocation bases (256-byte aligned), so whole groups load as vectors.
+__device__ __forceinline__ void chg_load(const int8_t* tag, const int8_t* tk, int i0, int n,
+                                         uint32_t a[ITEMS / 4], uint32_t b[ITEMS / 4]) {
+    static_assert(ITEMS == 16, "one 16-byte load per array");
+    if (i0 + ITEMS <= n) {
+        const uint4 x = *reinterpret_cast<const uint4*>(tag + i0);
+        const uint4 y = *reinterpret_cast<const uint4*>(tk + i0);
+        a[0] = x.x, a[1] = x.y, a[2] = x.z, a[3] = x.w;
+        b[0] = y.x, b[1] = y.y, b[2] = y.z, b[3] = y.w;
+    } else {
+#pragma unroll
+        for (int w = 0; w < ITEMS / 4; w++) a[w] = b[w] = 0u;
+        for (int j = 0; j < ITEMS && i0 + j < n; j++) {
+            const int sh = 8 * (j & 3);
+            a[j >> 2] |= (uint32_t)(uint8_t)tag[i0 + j] << sh;
+            b[j >> 2] |= (uint32_t)(uint8_t)tk[i0 + j] << sh;
+        }
+    }
+}
+
+__device__ __forceinline__ int byte_of(const uint32_t w[], int j) {
+    return (int)(int8_t)(w[j >> 2] >> (8 * (j & 3)));
+}
+
+// Per-thread counts of the status changes, in-band targets and expansions among its entries.
+__device__ __forceinline__ void chg_counts(const Geo& G, const uint32_t a[], const uint32_t b[],
+                                           int cnt[3], bool* fl) {
+    cnt[0] = cnt[1] = cnt[2] = 0;
+#pragma unroll
+    for (int w = 0; w < ITEMS / 4; w++) {
+        if (a[w] == b[w]) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int x = byte_of(a, 4 * w + q), y = byte_of(b, 4 * w + q);
+            if (x == y) continue;
+            cnt[0]++;
+            if (y >= -G.L && y <= G.L) cnt[1]++;
+            if (x != 0) *fl = true;
+            if (expanding(G, x, y)) cnt[2]++;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BT) k_chg_count(Geo G, const int8_t* tag, const int8_t* tk,
+                                                  const int* rng, int* counts, Ctr* ctr,
+                                                  const int* live) {
     if (!*live) return;
     const int n = rng[1], ntiles = (n + kTile - 1) / kTile;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int base = t * kTile + threadIdx.x;
-        int nsc = 0, nbf = 0, nex = 0;
+        const int i0 = t * kTile + threadIdx.x * ITEMS;
+        uint32_t a[ITEMS / 4], b[ITEMS / 4];
+        chg_load(tag, tk, i0, n, a, b);
+        int cnt[3], ex[3], tot[3];
         bool fl = false;
-#pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            const int i = base + j * BT;
-            if (i < n) {
-                const int a = tag[i], b = tk[i];
-                if (a != b) {
-                    nsc++;
-                    if (b >= -G.L && b <= G.L) nbf++;
-                    if (a != 0) fl = true;
-                    if (expanding(G, a, b)) nex++;
-                }
-            }
-        }
-        int tsc, tbf, tex;
-        block_excl_scan(nsc, &tsc);
-        block_excl_scan(nbf, &tbf);
-        block_excl_scan(nex, &tex);
+        chg_counts(G, a, b, cnt, &fl);
+        block_excl_scan3(cnt, ex, tot);
         if (__ballot(fl) != 0ull && fl) ctr->flag = 1;
         if (threadIdx.x == 0) {
-            counts[3 * t] = tsc;
-            counts[3 * t + 1] = tbf;
-            counts[3 * t + 2] = tex;
+            counts[3 * t] = tot[0];
+            counts[3 * t + 1] = tot[1];
+            counts[3 * t + 2] = tot[2];
         }
     }
 }
 
 // Pass 2 (single block): exclusive scan of the per-tile counts; the totals become the round's
 // status-change count and the buffer lookup's span.
-__global__ void k_chg_scan(int* counts, const int* rng, Ctr* ctr, int buf, const int* live) {
+__global__ void __launch_bounds__(BT) k_chg_scan(int* counts, const int* rng, Ctr* ctr, int buf,
+                                                 const int* live) {
     if (!*live) return;
     const int nb = (rng[1] + kTile - 1) / kTile;
-    __shared__ int carry[3];
-    if (threadIdx.x < 3) carry[threadIdx.x] = 0;
-    __syncthreads();
+    int carry[3] = {0, 0, 0};
     for (int b0 = 0; b0 < nb; b0 += BT) {
         const int b = b0 + threadIdx.x;
-        int v[3], t[3], e[3];
+        int v[3], e[3], t[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) v[c] = b < nb ? counts[3 * b + c] : 0;
-#pragma unroll
-        for (int c = 0; c < 3; c++) e[c] = block_excl_scan(v[c], &t[c]);
+        block_excl_scan3(v, e, t);
         if (b < nb)
 #pragma unroll
             for (int c = 0; c < 3; c++) counts[3 * b + c] = carry[c] + e[c];
-        __syncthreads();
-        if (threadIdx.x == 0)
 #pragma unroll
-            for (int c = 0; c < 3; c++) carry[c] += t[c];
-        __syncthreads();
+        for (int c = 0; c < 3; c++) carry[c] += t[c];
     }
     if (threadIdx.x == 0) {
         ctr->sc_n = carry[0];
@@ -756,49 +827,44 @@ __global__ void k_chg_scan(int* counts, const int* rng, Ctr* ctr, int buf, const
 
 // Pass 3: status-change records (status_change, Surface.hpp:1138-1178) and the buffer entries
 // of the in-band targets, in list order.
-__global__ void k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag, const int8_t* tk,
-                              const int* rng, const int* offsets, SC sc, Lu buf, const int* live) {
+__global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag,
+                                                    const int8_t* tk, const int* rng,
+                                                    const int* offsets, SC sc, Lu buf,
+                                                    const int* live) {
     if (!*live) return;
     const int n = rng[1], ntiles = (n + kTile - 1) / kTile;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int base = t * kTile + threadIdx.x;
-        int8_t a[ITEMS], b[ITEMS];
-        int nsc = 0, nbf = 0, nex = 0;
+        const int i0 = t * kTile + threadIdx.x * ITEMS;
+        uint32_t a[ITEMS / 4], b[ITEMS / 4];
+        chg_load(tag, tk, i0, n, a, b);
+        int cnt[3], ex[3], tot[3];
+        bool fl = false;
+        chg_counts(G, a, b, cnt, &fl);
+        block_excl_scan3(cnt, ex, tot);
+        int osc = ex[0] + offsets[3 * t], obf = ex[1] + offsets[3 * t + 1],
+            oex = ex[2] + offsets[3 * t + 2];
+        if (cnt[0] == 0) continue;
 #pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            const int i = base + j * BT;
-            a[j] = b[j] = 0;
-            if (i < n) {
-                a[j] = tag[i];
-                b[j] = tk[i];
-                if (a[j] != b[j]) {
-                    nsc++;
-                    if (b[j] >= -G.L && b[j] <= G.L) nbf++;
-                    if (expanding(G, a[j], b[j])) nex++;
-                }
-            }
-        }
-        int tsc, tbf, tex;
-        int osc = block_excl_scan(nsc, &tsc) + offsets[3 * t];
-        int obf = block_excl_scan(nbf, &tbf) + offsets[3 * t + 1];
-        int oex = block_excl_scan(nex, &tex) + offsets[3 * t + 2];
+        for (int w = 0; w < ITEMS / 4; w++) {
+            if (a[w] == b[w]) continue;
 #pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            const int i = base + j * BT;
-            if (i < n && a[j] != b[j]) {
-                const uint32_t v = lv[i];
+            for (int q = 0; q < 4; q++) {
+                const int j = 4 * w + q;
+                const int x = byte_of(a, j), y = byte_of(b, j);
+                if (x == y) continue;
+                const uint32_t v = lv[i0 + j];
                 sc.v[osc] = v;
-                sc.from[osc] = a[j];
-                sc.to[osc] = (int16_t)b[j];
+                sc.from[osc] = (int8_t)x;
+                sc.to[osc] = (int16_t)y;
                 osc++;
-                if (b[j] >= -G.L && b[j] <= G.L) {
+                if (y >= -G.L && y <= G.L) {
                     buf.v[obf] = v;
-                    buf.tag[obf] = b[j];
+                    buf.tag[obf] = (int8_t)y;
                     obf++;
                 }
-                if (expanding(G, a[j], b[j])) {
+                if (expanding(G, x, y)) {
                     sc.xv[oex] = v;
-                    sc.xside[oex] = a[j] < 0 ? -1 : 1;
+                    sc.xside[oex] = x < 0 ? -1 : 1;
                     oex++;
                 }
             }
