@@ -562,6 +562,12 @@ __device__ __forceinline__ float curvature_of(float c, const float* v) {
     return s / 2.0f;
 }
 
+// Number of lanes below this one whose bit is set in the wavefront mask m.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Wave-aggregated append: one atomic per wavefront.  Every lane of the wave must call it.
 __device__ __forceinline__ int wave_append(int* counter, bool pred) {
     const unsigned long long m = __ballot(pred);

@@ -826,12 +826,18 @@ __global__ void __launch_bounds__(BT) k_chg_scan(int* counts, const int* rng, Ct
 }
 
 // Pass 3: status-change records (status_change, Surface.hpp:1138-1178) and the buffer entries
-// of the in-band targets, in list order.
+// of the in-band targets, in list order.  Each lane stages the wavefront-local indices of its
+// changed entries in LDS, in list order; the wavefront then reads them back 64 at a time and
+// writes its records as contiguous runs (the entries' values come again from the cache), so
+// the records keep list order -- which the flush and the expansion read with locality -- and
+// every store instruction writes one run.
 __global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag,
                                                     const int8_t* tk, const int* rng,
                                                     const int* offsets, SC sc, Lu buf,
                                                     const int* live) {
     if (!*live) return;
+    __shared__ uint16_t stage[BT / 64][64 * ITEMS];
+    const int lane = __lane_id(), wv = threadIdx.x >> 6;
     const int n = rng[1], ntiles = (n + kTile - 1) / kTile;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int i0 = t * kTile + threadIdx.x * ITEMS;
@@ -841,34 +847,53 @@ __global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, const uint32_t* lv, c
         bool fl = false;
         chg_counts(G, a, b, cnt, &fl);
         block_excl_scan3(cnt, ex, tot);
-        int osc = ex[0] + offsets[3 * t], obf = ex[1] + offsets[3 * t + 1],
-            oex = ex[2] + offsets[3 * t + 2];
-        if (cnt[0] == 0) continue;
+        // the wavefront's first record of each kind, and its number of status changes
+        const int w0 = __shfl(ex[0], 0);
+        const int nw = __shfl(ex[0] + cnt[0], 63) - w0;
+        if (nw == 0) continue;
+        int obf = __shfl(ex[1], 0) + offsets[3 * t + 1], oex = __shfl(ex[2], 0) + offsets[3 * t + 2];
+        const int osc = w0 + offsets[3 * t];
+        int at = ex[0] - w0;
 #pragma unroll
         for (int w = 0; w < ITEMS / 4; w++) {
             if (a[w] == b[w]) continue;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int j = 4 * w + q;
-                const int x = byte_of(a, j), y = byte_of(b, j);
-                if (x == y) continue;
-                const uint32_t v = lv[i0 + j];
-                sc.v[osc] = v;
-                sc.from[osc] = (int8_t)x;
-                sc.to[osc] = (int16_t)y;
-                osc++;
-                if (y >= -G.L && y <= G.L) {
-                    buf.v[obf] = v;
-                    buf.tag[obf] = (int8_t)y;
-                    obf++;
-                }
-                if (expanding(G, x, y)) {
-                    sc.xv[oex] = v;
-                    sc.xside[oex] = x < 0 ? -1 : 1;
-                    oex++;
-                }
+                if (byte_of(a, j) != byte_of(b, j)) stage[wv][at++] = (uint16_t)(lane * ITEMS + j);
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int e0 = t * kTile + wv * 64 * ITEMS;  // the wavefront's first entry
+        for (int k0 = 0; k0 < nw; k0 += 64) {
+            const int k = k0 + lane;
+            const bool ok = k < nw;
+            const int e = ok ? e0 + stage[wv][k] : e0;
+            const uint32_t v = lv[e];
+            const int x = tag[e], y = tk[e];
+            const bool inb = ok && y >= -G.L && y <= G.L;
+            const bool xp = ok && expanding(G, x, y);
+            if (ok) {
+                sc.v[osc + k] = v;
+                sc.from[osc + k] = (int8_t)x;
+                sc.to[osc + k] = (int16_t)y;
+            }
+            const unsigned long long m1 = __ballot(inb), m2 = __ballot(xp);
+            if (inb) {
+                buf.v[obf + lanes_below(m1)] = v;
+                buf.tag[obf + lanes_below(m1)] = (int8_t)y;
+            }
+            if (xp) {
+                sc.xv[oex + lanes_below(m2)] = v;
+                sc.xside[oex + lanes_below(m2)] = x < 0 ? -1 : 1;
+            }
+            obf += __popcll(m1);
+            oex += __popcll(m2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
