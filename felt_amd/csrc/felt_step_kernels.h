@@ -381,6 +381,13 @@ __global__ void __launch_bounds__(PB) k_bscan_apply(const int* pcnt, int np, int
 // through err.
 constexpr int kScatterBlock = 8;  // wavefronts per block
 
+// The four bytes of x as the four 16-bit fields of a 64-bit word.
+__device__ __forceinline__ unsigned long long spread16(uint32_t x) {
+    return (unsigned long long)(x & 0xffu) | ((unsigned long long)(x & 0xff00u) << 8) |
+           ((unsigned long long)(x & 0xff0000u) << 16) |
+           ((unsigned long long)(x & 0xff000000u) << 24);
+}
+
 __device__ __forceinline__ int field16(unsigned long long lo, unsigned long long hi, int f) {
     return (int)(((f < 4 ? lo : hi) >> (16 * (f & 3))) & 0xffffull);
 }
@@ -467,16 +474,24 @@ __device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, i
                     }
                 }
             }
-            unsigned long long n0 = 0, n1 = 0;
+            // Each layer byte becomes its field f = k + L (0..NL-1, NL <= 7), and a voxel outside
+            // the band field 7, four bytes at a time: adding L byte-wise takes the band's bytes
+            // to 0..2L and NONE (0x80) to 0x80 + L, whose top bit marks it.  q then holds 8f per
+            // byte.  The counts go into 8-bit fields (at most VPL <= 64 per field) with no
+            // branch, then into 16-bit fields for the scan (layers 0-3 and 4-7 in two words).
+            const uint32_t Lrep = (uint32_t)G.L * 0x01010101u;
 #pragma unroll
-            for (int j = 0; j < VPL; j++) {
-                const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
-                if (k == NONE) continue;
-                const int f = k + G.L;
-                const unsigned long long one = 1ull << (16 * (f & 3));
-                if (f < 4) n0 += one;
-                else n1 += one;
+            for (int w = 0; w < NW; w++) {
+                const uint32_t x = q[w];
+                const uint32_t u = ((x & 0x7f7f7f7fu) + Lrep) ^ (x & 0x80808080u);
+                const uint32_t out = (u >> 7) & 0x01010101u;
+                q[w] = ((u & 0x07070707u) | ((out << 3) - out)) << 3;
             }
+            unsigned long long c8 = 0;
+#pragma unroll
+            for (int j = 0; j < VPL; j++) c8 += 1ull << ((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
+            const unsigned long long n0 = spread16((uint32_t)c8),
+                                     n1 = spread16((uint32_t)(c8 >> 32) & 0x00ffffffu);
             unsigned long long x0 = n0, x1 = n1;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
@@ -522,26 +537,24 @@ __device__ __forceinline__ bool scatter_partition(const Geo& G, const Pool& P, i
             const unsigned long long st1 = lo_tot * rep + (w1 << 16) + (w1 << 32) + (w1 << 48);
             const int total = (int)(lo_tot + (w1 & 0xffffull) + ((w1 >> 16) & 0xffffull) +
                                     ((w1 >> 32) & 0xffffull) + (w1 >> 48));
-            // the layer bytes are unpacked again below rather than kept unpacked across the scan
+            // the field bytes are read again below rather than kept unpacked across the scan
 #pragma unroll
             for (int w = 0; w < NW; w++) asm volatile("" : "+v"(q[w]));
+            // r0, r1: each field's next position in the stage.  Field 7 (outside the band)
+            // starts at total, so those voxels are written past the span's entries, into the
+            // stage's VPL spare slots, with no branch.
             unsigned long long r0 = x0 - n0 + st0, r1 = x1 - n1 + st1;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int j = 0; j < VPL; j++) {
-                const int k = (int)(int8_t)((q[j >> 2] >> (8 * (j & 3))) & 0xffu);
-                if (k == NONE) continue;
-                const int f = k + G.L;
-                const int sh = 16 * (f & 3);
-                int e;
-                if (f < 4) {
-                    e = (int)((r0 >> sh) & 0xffffull);
-                    r0 += 1ull << sh;
-                } else {
-                    e = (int)((r1 >> sh) & 0xffffull);
-                    r1 += 1ull << sh;
-                }
+                const uint32_t sh = ((q[j >> 2] >> (8 * (j & 3))) & 0xffu) << 1;  // 16 f
+                const bool hi = sh >= 64u;
+                const unsigned long long r = hi ? r1 : r0;
+                const int e = (int)((r >> (sh & 63u)) & 0xffffull);
+                const unsigned long long rn = r + (1ull << (sh & 63u));
+                r0 = hi ? r0 : rn;
+                r1 = hi ? rn : r1;
                 stage[e] = (uint16_t)(VPL * lane + j);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -572,7 +585,8 @@ __global__ void __launch_bounds__(64 * kScatterBlock)
 k_band_scatter(Geo G, Pool P, int wpp, const int* visit, const int* offsets, uint32_t* band,
                int8_t* tag, Ctr* ctr) {
     __shared__ unsigned long long span_tot[2][kScatterBlock][2];
-    __shared__ uint16_t span_voxel[kScatterBlock][64 * VPL];  // a span's entries, staged
+    // a span's entries, staged, and VPL spare slots for the voxels outside the band
+    __shared__ uint16_t span_voxel[kScatterBlock][64 * VPL + VPL];
     const int lane = __lane_id();
     const int wv = threadIdx.x >> 6;
     const int wi = wv & (wpp - 1);
