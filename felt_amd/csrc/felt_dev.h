@@ -487,6 +487,109 @@ __device__ __forceinline__ void gather(const Geo& G, const Pool& P, const VoxCtx
     }
 }
 
+// Storage leaf term of brick-local coordinate x along axis a: the leaf of coordinates lc is the
+// sum of its axes' terms (leaf_of).  Both products are below pv <= 2^24, so 24-bit multiplies
+// are exact.
+__device__ __forceinline__ int leaf_term(const Geo& G, int a, int x) {
+    return (int)(__umul24((uint32_t)(x >> G.tsh[a]), (uint32_t)G.tstr[a]) +
+                 __umul24((uint32_t)(x & G.tmask[a]), (uint32_t)G.wstr[a]));
+}
+
+// gather<D, K>(G, P, c, off, val) for offsets whose components are compile-time constants, with
+// the work shared between the K neighbours: per axis, the leaf terms of the coordinate and its
+// two steps (wrapped into the neighbouring brick), whether each step leaves the brick or the
+// grid; per brick that a neighbour can fall in (the voxel's own, the one across each axis the
+// voxel borders, the one across each pair of those), one page lookup.  A neighbour's value is
+// then read from its brick's base plus the sum of its axes' leaf terms, with no branch.
+template <int D, int K>
+__device__ __forceinline__ void gather_sep(const Geo& G, const Pool& P, const VoxCtx& c,
+                                           const int (*off)[3], float* val) {
+    int F[3][3];     // leaf term of the coordinate stepped by -1, 0, +1
+    bool inn[3][3];  // the stepped coordinate is inside the grid
+    int dir[3];      // the side of the brick the voxel borders along the axis (0: neither; every
+                     // partition edge is at least 2, so a voxel borders at most one side)
+    bool bord[3];    // the step across that side stays inside the grid
+    int leaf = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const int l = c.loc[a], pa = G.part[a];
+        const bool lo = l == 0, hi = l == pa - 1;
+        F[a][0] = leaf_term(G, a, lo ? pa - 1 : l - 1);
+        F[a][1] = leaf_term(G, a, l);
+        F[a][2] = leaf_term(G, a, hi ? 0 : l + 1);
+        leaf += F[a][1];
+        inn[a][0] = c.g[a] > 0;
+        inn[a][1] = true;
+        inn[a][2] = c.g[a] + 1 < c.sz[a];
+        dir[a] = lo ? -1 : (hi ? 1 : 0);
+        bord[a] = lo ? inn[a][0] : (hi && inn[a][2]);
+    }
+    const uint32_t own_base = c.v - (uint32_t)leaf;
+    // bricks: [0] own; [1 + a] across axis a; [1 + D + pair(a, b)] across axes a and b
+    constexpr int NB = D == 3 ? 7 : 4;
+    int bp[NB];  // partition index (valid when the brick is inside the grid)
+    bool bin[NB];
+    bp[0] = c.own;
+    bin[0] = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        bp[1 + a] = c.own + dir[a] * c.cs[a];
+        bin[1 + a] = bord[a];
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++)
+#pragma unroll
+        for (int b = a + 1; b < D; b++) {
+            const int k = 1 + D + (D == 2 ? 0 : (a == 0 ? b - 1 : 2));
+            bp[k] = c.own + dir[a] * c.cs[a] + dir[b] * c.cs[b];
+            bin[k] = bin[1 + a] && bin[1 + b];
+        }
+    int pg[NB];
+    pg[0] = 0;
+#pragma unroll
+    for (int k = 1; k < NB; k++) pg[k] = P.page[bin[k] ? bp[k] : c.own];
+    // brick k's values: phi + its base (active) or its background (inactive); a brick outside
+    // the grid reads the voxel's own page and is never used, as its neighbours are outside too
+    const float* src_b[NB];
+    src_b[0] = P.phi + own_base;
+#pragma unroll
+    for (int k = 1; k < NB; k++)
+        src_b[k] = pg[k] >= 0 ? P.phi + (uint32_t)pg[k] * (uint32_t)G.pv : P.pbg + bp[k];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        int l = 0, a0 = -1, a1 = -1;  // a0, a1: the axes the offset steps along (at most two)
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            const int o = off[j][a];
+            l += F[a][o + 1];
+            inside = inside && inn[a][o + 1];
+            if (o != 0) {
+                if (a0 < 0) a0 = a;
+                else a1 = a;
+            }
+        }
+        // the brick: across the axes along which the step leaves the voxel's brick
+        const float* src = src_b[0];
+        bool inactive = false;
+        if (a0 >= 0) {
+            const bool c0 = off[j][a0] == dir[a0];
+            src = c0 ? src_b[1 + a0] : src;
+            inactive = c0 && pg[1 + a0] < 0;
+            if (a1 >= 0) {
+                const bool c1 = off[j][a1] == dir[a1];
+                const int kp = 1 + D + (D == 2 ? 0 : (a0 == 0 ? a1 - 1 : 2));
+                src = c1 ? (c0 ? src_b[kp] : src_b[1 + a1]) : src;
+                inactive = c1 ? (c0 ? pg[kp] < 0 : pg[1 + a1] < 0) : inactive;
+            }
+        }
+        // an inactive brick's background is one value; a neighbour outside the grid reads the
+        // voxel itself and is replaced by L+1
+        const float raw = *(inside ? (inactive ? src : src + l) : P.phi + c.v);
+        val[j] = inside ? raw : (float)(G.L + 1);
+    }
+}
+
 // The curvature stencil: 2D face neighbours (face(a, s) at 2a + (s > 0)), then for each axis
 // pair a < b the four edge neighbours (a, b) signs (-,-), (-,+), (+,-), (+,+).
 template <int D>

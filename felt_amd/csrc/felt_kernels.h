@@ -136,8 +136,12 @@ __device__ __forceinline__ float speed_voxel(const Geo& G, const Pool& P, uint32
     float curv = 0.0f;
     if (s.kind != FELT_SPEED_CONSTANT) {
         float val[Stencil<D>::K];
-        if (D == 3) gather<D, Stencil<D>::K>(G, P, c, kStencil3, val);
-        else gather<D, Stencil<D>::K>(G, P, c, kStencil2, val);
+        const int (*off)[3] = D == 3 ? kStencil3 : kStencil2;
+        // partition edges of 1 (a voxel on both sides of its brick) take the general gather
+        if (G.part[0] > 1 && (D < 2 || G.part[1] > 1) && (D < 3 || G.part[2] > 1))
+            gather_sep<D, Stencil<D>::K>(G, P, c, off, val);
+        else
+            gather<D, Stencil<D>::K>(G, P, c, off, val);
         curv = curvature_of<D>(P.phi[c.v], val);
     }
     const Mem& M = P.mem[c.m];
