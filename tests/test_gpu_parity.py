@@ -125,6 +125,23 @@ def test_uneven_partitions_and_grid_edge():
         check(g, c, f"step {i}")
 
 
+@pytest.mark.parametrize("D,size,part", [(3, (14, 12, 10), (2, 2, 2)),
+                                         (3, (13, 11, 9), (1, 4, 2)),
+                                         (3, (12, 13, 14), (3, 1, 5)),
+                                         (2, (21, 17), (1, 3)),
+                                         (2, (20, 18), (2, 5))])
+def test_thin_partitions(D, size, part):
+    """Partition edges of 1 and 2 (every voxel on a brick face; with an edge of 1 on both faces
+    along that axis): the curvature stencil and the expansion reach across bricks on every step."""
+    print(f"[thin] D={D} size={size} part={part}", flush=True)
+    g, c = pair(D, 2, size, part)
+    step_both(g, c, "seed", (1,) * D)
+    for i in range(9):
+        speed = speed_const_curv(-0.9, 0.1) if i < 6 else speed_curvature(0.3)
+        step_both(g, c, "update", speed)
+        check(g, c, f"step {i}")
+
+
 def test_collapse_and_reexpand():
     g, c = pair(3, 3, (40,) * 3, (8,) * 3)
     step_both(g, c, "load_dense", workloads.spheres_phi((40,) * 3, 3, [(0.5, -1.0, 2.0)], [9.0]))
