@@ -37,11 +37,9 @@ struct Ctr {
     int seg0_cur[17];
 };
 
-// Status-change record: the reference's StatusChangeGrid entry (Surface.hpp:1138-1178).
+// Expansion records (expand_narrow_band, Surface.hpp:1249-1355): the status changes out of the
+// outermost layers, kept for the expansion after the flush.
 struct SC {
-    uint32_t* v;
-    int8_t* from;
-    int16_t* to;
     uint32_t* xv;   // the expanding records (outermost layer to the next one in): voxel
     int8_t* xside;  // and side
 };

@@ -31,10 +31,6 @@ using namespace felt;
 
 namespace {
 
-// Bytes of step lists per listed voxel: band (index, delta, tag, new layer), two lookups,
-// status-change records, zero-layer list, the affected set sorted by layer.
-constexpr uint64_t kListBytes = 10 + 2 * 10 + 7 + 8 + 5;
-
 double now_s() {
     using clk = std::chrono::steady_clock;
     return std::chrono::duration<double>(clk::now().time_since_epoch()).count();

@@ -839,16 +839,69 @@ __global__ void __launch_bounds__(BT) k_chg_scan(int* counts, const int* rng, Ct
     }
 }
 
-// Pass 3: status-change records (status_change, Surface.hpp:1138-1178) and the buffer entries
-// of the in-band targets, in list order.  Each lane stages the wavefront-local indices of its
-// changed entries in LDS, in list order; the wavefront then reads them back 64 at a time and
-// writes its records as contiguous runs (the entries' values come again from the cache), so
-// the records keep list order -- which the flush and the expansion read with locality -- and
-// every store instruction writes one run.
-__global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, const uint32_t* lv, const int8_t* tag,
-                                                    const int8_t* tk, const int* rng,
-                                                    const int* offsets, SC sc, Lu buf,
-                                                    const int* live) {
+// flush_status_change (Surface.hpp:1183-1244) with Untrack::retrack / untrack
+// (PartitionedMixin.hpp:489-558) of one status change per lane (p = -1: none) -- voxel v of
+// partition p from layer index dec to layer `to`: its layer byte (and, for an untrack, its value
+// Distance(to)), then per distinct partition of the wavefront one atomic per layer count that
+// changes, the from-layers of its changes into smask (its status_change(k) lists), and for
+// untracks the key of its new background (Distance(to) of the untrack that comes last in the
+// reference's flush order: the highest from-layer) and a flag for the emptiness check.  Every
+// lane of the wavefront calls it.
+__device__ __forceinline__ void flush_change(const Geo& G, const Pool& P, uint32_t v, int p, int dec,
+                                             int to, unsigned* smask, int* ukey, int* tflag) {
+    int inc = -1, key = -1;
+    if (p >= 0) {
+        if (to >= -G.L && to <= G.L) {
+            P.lay[v] = (int8_t)to;
+            inc = to + G.L;
+        } else {
+            P.lay[v] = NONE;
+            P.phi[v] = (float)to;
+            key = (dec << 16) | ((to + 32768) & 0xffff);
+        }
+    }
+    const int lane = __lane_id();
+    unsigned long long todo = __ballot(p >= 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int p0 = __shfl(p, leader);
+        const bool mine = p >= 0 && p == p0;
+        // only the layers the partition's records touch (usually two or three)
+        unsigned lm = mine ? ((1u << dec) | (inc >= 0 ? 1u << inc : 0u)) : 0u;
+        for (int o = 32; o > 0; o >>= 1) lm |= __shfl_xor(lm, o);
+        unsigned from_bits = 0;
+        while (lm) {
+            const int k = __ffs(lm) - 1;
+            lm &= lm - 1u;
+            const unsigned long long md = __ballot(mine && dec == k);
+            const int d = __popcll(__ballot(mine && inc == k)) - __popcll(md);
+            if (md) from_bits |= 1u << k;
+            if (lane == leader && d != 0) atomicAdd(&P.pcnt[p0 * G.NL + k], d);
+        }
+        if (lane == leader) atomicOr(&smask[p0], from_bits);
+        if (__ballot(mine && key >= 0)) {
+            int kmax = mine ? key : -1;
+            for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
+            if (lane == leader) {
+                atomicMax(&ukey[p0], kmax);
+                tflag[p0] = 1;
+            }
+        }
+        todo &= ~__ballot(mine);
+    }
+}
+
+// Pass 3: the status changes (status_change, Surface.hpp:1138-1178) flushed, and the buffer
+// entries of the in-band targets and the expansion records, in list order.  Each lane stages the
+// wavefront-local indices of its changed entries in LDS, in list order; the wavefront then reads
+// them back 64 at a time (the entries' values come again from the cache), so the flush's layer
+// byte stores and the buffer's entries keep list order and every store instruction of the
+// buffer writes one contiguous run.
+__global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, Pool P, const uint32_t* lv,
+                                                    const int8_t* tag, const int8_t* tk,
+                                                    const int* rng, const int* offsets, SC sc,
+                                                    Lu buf, unsigned* smask, int* ukey,
+                                                    int* tflag, const int* live) {
     if (!*live) return;
     __shared__ uint16_t stage[BT / 64][64 * ITEMS];
     const int lane = __lane_id(), wv = threadIdx.x >> 6;
@@ -866,7 +919,6 @@ __global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, const uint32_t* lv, c
         const int nw = __shfl(ex[0] + cnt[0], 63) - w0;
         if (nw == 0) continue;
         int obf = __shfl(ex[1], 0) + offsets[3 * t + 1], oex = __shfl(ex[2], 0) + offsets[3 * t + 2];
-        const int osc = w0 + offsets[3 * t];
         int at = ex[0] - w0;
 #pragma unroll
         for (int w = 0; w < ITEMS / 4; w++) {
@@ -889,11 +941,8 @@ __global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, const uint32_t* lv, c
             const int x = tag[e], y = tk[e];
             const bool inb = ok && y >= -G.L && y <= G.L;
             const bool xp = ok && expanding(G, x, y);
-            if (ok) {
-                sc.v[osc + k] = v;
-                sc.from[osc + k] = (int8_t)x;
-                sc.to[osc + k] = (int16_t)y;
-            }
+            flush_change(G, P, v, ok ? P.slot_part[fdiv(v, G.fpv)] : -1, x + G.L, y, smask, ukey,
+                         tflag);
             const unsigned long long m1 = __ballot(inb), m2 = __ballot(xp);
             if (inb) {
                 buf.v[obf + lanes_below(m1)] = v;
@@ -918,62 +967,6 @@ __global__ void __launch_bounds__(BT) k_chg_scatter(Geo G, const uint32_t* lv, c
          i##_b += gridDim.x * blockDim.x)                                                      \
         for (int i = i##_b + (int)threadIdx.x, valid = i < i##_n, i##_once = 1; i##_once;     \
              i##_once = 0)
-
-// flush_status_change (Surface.hpp:1183-1244) with Untrack::retrack / untrack
-// (PartitionedMixin.hpp:489-558).  Per distinct partition of a wavefront: one atomic per layer
-// count that changes, the from-layers of its changes into smask (its status_change(k) lists),
-// and for untracks the key of its new background (Distance(to) of the untrack that comes last in
-// the reference's flush order: the highest from-layer) and a flag for the emptiness check.
-__global__ void k_flush(Geo G, Pool P, SC sc, const Ctr* ctrc, unsigned* smask, int* ukey,
-                        int* tflag, const int* live) {
-    if (!*live) return;
-    FELT_WAVE_LOOP(i, valid, ctrc->sc_n) {
-        int p = -1, dec = -1, inc = -1, key = -1;
-        if (valid) {
-            const uint32_t v = sc.v[i];
-            const int from = sc.from[i], to = sc.to[i];
-            p = P.slot_part[fdiv(v, G.fpv)];
-            dec = from + G.L;
-            if (to >= -G.L && to <= G.L) {
-                P.lay[v] = (int8_t)to;
-                inc = to + G.L;
-            } else {
-                P.lay[v] = NONE;
-                P.phi[v] = (float)to;
-                key = (dec << 16) | ((to + 32768) & 0xffff);
-            }
-        }
-        const int lane = __lane_id();
-        unsigned long long todo = __ballot(p >= 0);
-        while (todo) {
-            const int leader = __ffsll((long long)todo) - 1;
-            const int p0 = __shfl(p, leader);
-            const bool mine = p >= 0 && p == p0;
-            // only the layers the partition's records touch (usually two or three)
-            unsigned lm = mine ? ((1u << dec) | (inc >= 0 ? 1u << inc : 0u)) : 0u;
-            for (int o = 32; o > 0; o >>= 1) lm |= __shfl_xor(lm, o);
-            unsigned from_bits = 0;
-            while (lm) {
-                const int k = __ffs(lm) - 1;
-                lm &= lm - 1u;
-                const unsigned long long md = __ballot(mine && dec == k);
-                const int d = __popcll(__ballot(mine && inc == k)) - __popcll(md);
-                if (md) from_bits |= 1u << k;
-                if (lane == leader && d != 0) atomicAdd(&P.pcnt[p0 * G.NL + k], d);
-            }
-            if (lane == leader) atomicOr(&smask[p0], from_bits);
-            if (__ballot(mine && key >= 0)) {
-                int kmax = mine ? key : -1;
-                for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o));
-                if (lane == leader) {
-                    atomicMax(&ukey[p0], kmax);
-                    tflag[p0] = 1;
-                }
-            }
-            todo &= ~__ballot(mine);
-        }
-    }
-}
 
 // Partitions flagged by the flush: a partition left with no voxel is deactivated with the
 // background of its key and its brick returned to the pool.  Also starts the expansion's

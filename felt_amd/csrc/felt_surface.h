@@ -59,6 +59,11 @@ struct LaunchCfg {
     int vpl = 0;     // band compaction voxels per lane; 0: 64 for bricks above 512 voxels, else 8
 };
 
+// Device bytes of the step lists per listed voxel: band (index, delta, tag, new layer), two
+// lookups (index, tag, distance, new layer), expansion records (voxel, side), zero-layer list
+// (index, delta), a bounded step's affected set sorted by layer (index, tag).
+constexpr uint64_t kListBytes = 10 + 2 * 10 + 5 + 8 + 5;
+
 struct felt_surface {
     felt::Geo G{};
     felt::Pool P{};
@@ -94,9 +99,6 @@ struct felt_surface {
     DBuf<uint32_t> srt_v;
     DBuf<int8_t> srt_tag;
     // status-change records
-    DBuf<uint32_t> sc_v;
-    DBuf<int8_t> sc_from;
-    DBuf<int16_t> sc_to;
     DBuf<uint32_t> sc_xv;
     DBuf<int8_t> sc_xside;
     // scratch

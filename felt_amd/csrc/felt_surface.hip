@@ -912,9 +912,6 @@ void felt_surface_release(felt_surface* s) {
         s->lu_tag[i].release();
         s->lu_d[i].release();
     }
-    s->sc_v.release();
-    s->sc_from.release();
-    s->sc_to.release();
     s->sc_xv.release();
     s->sc_xside.release();
     s->vox.release();
@@ -930,7 +927,7 @@ Lu felt_lu(felt_surface* s, int i) {
     return Lu{s->lu_v[i].ptr, s->lu_tag[i].ptr, s->lu_d[i].ptr, s->lu_tk[i].ptr};
 }
 static SC sc_of(felt_surface* s) {
-    return SC{s->sc_v.ptr, s->sc_from.ptr, s->sc_to.ptr, s->sc_xv.ptr, s->sc_xside.ptr};
+    return SC{s->sc_xv.ptr, s->sc_xside.ptr};
 }
 static int* lu_span(felt_surface* s, int i) { return &s->ctr->lu_span[i][0]; }
 static const int* always(felt_surface* s) { return &s->ctr->one; }
@@ -959,9 +956,6 @@ static int ensure_step_capacity(felt_surface* s, size_t band) {
     }
     if ((rc = s->srt_v.ensure(n))) return rc;
     if ((rc = s->srt_tag.ensure(n))) return rc;
-    if ((rc = s->sc_v.ensure(n))) return rc;
-    if ((rc = s->sc_from.ensure(n))) return rc;
-    if ((rc = s->sc_to.ensure(n))) return rc;
     if ((rc = s->sc_xv.ensure(n))) return rc;
     if ((rc = s->sc_xside.ensure(n))) return rc;
     if ((rc = s->zlist.ensure(n))) return rc;
@@ -1099,25 +1093,23 @@ static void distance_passes(felt_surface* s, bool from_band, int look, const int
     }
 }
 
-// Status changes of the round's processed list -> status-change records and the buffer
-// lookup `buf` (its previous contents are replaced).
+// Status changes of the round's processed list: flushed (flush_status_change), with the buffer
+// lookup `buf` (its previous contents are replaced) and the expansion records.
 static void compact_changes(felt_surface* s, const uint32_t* v, const int8_t* tag,
                             const int8_t* tk, const int* rng, int buf, const int* live) {
     const Geo& G = s->G;
     const int nb = std::min(chg_blocks(s), 512);
     LAUNCH(k_chg_count, dim3(nb), dim3(BT), G, tag, tk, rng, s->chg_cnt.ptr, s->ctr, live);
     LAUNCH(k_chg_scan, dim3(1), dim3(BT), s->chg_cnt.ptr, rng, s->ctr, buf, live);
-    LAUNCH(k_chg_scatter, dim3(nb), dim3(BT), G, v, tag, tk, rng, s->chg_cnt.ptr, sc_of(s),
-           felt_lu(s, buf), live);
+    PLAUNCH(PK_FLUSH, 0, k_chg_scatter, dim3(nb), dim3(BT), G, s->P, v, tag, tk, rng,
+            s->chg_cnt.ptr, sc_of(s), felt_lu(s, buf), s->smask, s->ukey, s->tflag, live);
 }
 
-// flush_status_change then expand_narrow_band.
+// The rest of the flush (partitions left empty) then expand_narrow_band.
 static void flush_expand(felt_surface* s, const int* live) {
     const Geo& G = s->G;
     const SC sc = sc_of(s);
     const int grid = fixed_grid(s), tpb = kListTpb;
-    PLAUNCH(PK_FLUSH, 0, k_flush, dim3(grid), dim3(tpb), G, s->P, sc, s->ctr, s->smask, s->ukey,
-            s->tflag, live);
     LAUNCH(k_deactivate, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P, s->ukey,
            s->tflag, s->free_stack, s->dlist, s->ctr, live);
     prof_begin(s, PK_EXPAND, 0);
@@ -2265,9 +2257,7 @@ uint64_t felt_device_bytes(const felt_surface* s) {
     uint64_t b = cap * pv * (sizeof(float) + 2) + cap * (sizeof(int) * 2 + sizeof(int4) * 3);
     b += np * (sizeof(int) * 10 + sizeof(float) + sizeof(int) * G.NL * 3);
     b += (uint64_t)s->mems.size() * (sizeof(Mem) + sizeof(Speed));
-    // per-step lists: band (4+4+1+1), two lookups (4+1+4+1), status changes (4+1+2),
-    // zero-layer list (4+4), a bounded step's affected set sorted by layer (4+1)
-    b += (uint64_t)s->list_cap * (10 + 2 * 10 + 7 + 8 + 5);
+    b += (uint64_t)s->list_cap * kListBytes;
     b += (uint64_t)s->vox.cap * sizeof(float);
     return b;
 }

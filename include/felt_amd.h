@@ -280,9 +280,9 @@ int felt_surface_set_graph(felt_surface* s, int enable);
 int felt_surface_step_info(felt_surface* s, uint64_t* voxel_updates, uint64_t* rounds);
 /* Kernel timing with HIP events on the handle's stream, for measurement.  Not a reference
  * member (the reference has no profiling hooks).  Kernel ids: 0 speed, 1 zero-layer apply,
- * 2 distance pass 1 and 3 distance pass 2 of a step's first round, 4 flush, 5 expand (three
- * kernels), 6 band count, 7 affected search, 8 march count, 9 march emit, 10 / 11 distance
- * passes of later rounds.  `items` sums the list entries the launches processed (first-round
+ * 2 distance pass 1 and 3 distance pass 2 of a step's first round, 4 flush (with the round's
+ * lookup and expansion records), 5 expand (three kernels), 6 band count, 7 affected search,
+ * 8 march count, 9 march emit, 10 / 11 distance passes of later rounds.  `items` sums the list entries the launches processed (first-round
  * and march kernels).  Events are attached to the dispatches (hipExtLaunchKernelGGL); a
  * profiled global step launches its kernels one by one instead of replaying its graph.
  * Enabling or disabling clears the totals. */
