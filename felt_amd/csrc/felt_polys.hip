@@ -182,6 +182,118 @@ __device__ unsigned cube_mask(const Geo& G, const Pool& P, const MemV& V, const 
     return m;
 }
 
+// Per-axis view of the cube based at grid position g, whose base voxel is at brick-local lc of the
+// marched brick: the leaf terms (leaf_of is a sum of per-axis terms) of lc - 1, lc and lc + 1
+// (the last wrapped into the next brick), whether lc is the brick's last coordinate, and whether
+// g - 1 / g + 1 stay in the brick / the grid.
+struct CubeAxes {
+    int Fm[3], F0[3], F1[3];
+    bool hi[3], okm[3], okp[3], inp[3];
+};
+
+template <int D>
+__device__ __forceinline__ void cube_axes(const Geo& G, const MemV& V, const int lc[3],
+                                          const int g[3], CubeAxes& A) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const int l = lc[a];
+        A.hi[a] = l == G.part[a] - 1;
+        A.F0[a] = leaf_term(G, a, l);
+        A.F1[a] = leaf_term(G, a, A.hi[a] ? 0 : l + 1);
+        A.Fm[a] = leaf_term(G, a, l > 0 ? l - 1 : 0);
+        A.inp[a] = g[a] + 1 < V.size[a];
+        A.okm[a] = l > 0;
+        A.okp[a] = !A.hi[a] && A.inp[a];
+    }
+}
+
+// Corner c's offset from the cube's base voxel (calc_off + corner, each component 0 or 1) as
+// bits, bit a for axis a: c_corner3 shifted by pos_calc's (0, 0, 1), and c_corner2.
+__device__ constexpr unsigned kCornerUp3[8] = {4, 5, 1, 0, 6, 7, 3, 2};
+__device__ constexpr unsigned kCornerUp2[4] = {0, 1, 3, 2};
+
+// cube_mask for the cube whose base voxel is in the marched brick (partition p): a corner's
+// leaf is the sum of its axes' terms, and the bricks its corners can fall in (the marched one
+// and those across the + faces it borders, their edges and corner) are looked up once each.
+template <int D>
+__device__ unsigned cube_mask_at(const Geo& G, const Pool& P, const MemV& V, int p,
+                                 const CubeAxes& A) {
+    constexpr int NC = McT<D>::ncorner, NB = 1 << D;
+    unsigned hm = 0;  // axes along which the + corners leave the brick
+#pragma unroll
+    for (int a = 0; a < D; a++) hm |= (unsigned)A.hi[a] << a;
+    // brick k (bit a: across axis a): partition p + the crossed axes' strides; one outside the
+    // grid reads the marched brick's page and is never used (its corners are outside too)
+    const float* src[NB];
+    bool inact[NB];
+    src[0] = P.phi + (uint32_t)V.slot * (uint32_t)G.pv;
+    inact[0] = false;
+#pragma unroll
+    for (int k = 1; k < NB; k++) {
+        bool ok = true;
+        int pk = p;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+            if ((k >> a) & 1) {
+                ok = ok && A.hi[a] && A.inp[a];
+                pk += V.cstr[a];
+            }
+        const int pg = P.page[ok ? pk : p];
+        src[k] = pg >= 0 ? P.phi + (uint32_t)pg * (uint32_t)G.pv : P.pbg + pk;
+        inact[k] = pg < 0;
+    }
+    unsigned m = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        const unsigned cm = D == 3 ? kCornerUp3[c] : kCornerUp2[c];
+        int leaf = 0;
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            const bool up = (cm >> a) & 1u;
+            leaf += up ? A.F1[a] : A.F0[a];
+            inside = inside && (!up || A.inp[a]);
+        }
+        const unsigned kb = cm & hm;  // the corner's brick
+        const float* sp = src[0];
+        bool ia = false;
+#pragma unroll
+        for (unsigned k = 1; k < (unsigned)NB; k++) {
+            if ((k & ~cm) != 0u) continue;  // not a subset of the corner's axes
+            sp = kb == k ? src[k] : sp;
+            ia = kb == k ? inact[k] : ia;
+        }
+        const float v = *(inside ? (ia ? sp : sp + leaf) : src[0]);
+        m |= (unsigned)(inside ? v > 0.0f : (float)(G.L + 1) > 0.0f) << c;
+    }
+    return m;
+}
+
+// band_mask_around for a cube base at brick-local lc of brick slot s, from its axes' terms.
+template <int D>
+__device__ __forceinline__ unsigned band_mask_at(const Geo& G, const Pool& P, int s,
+                                                 const CubeAxes& A) {
+    constexpr int NK = D == 3 ? 27 : 9;
+    const size_t base = (size_t)s * G.pv;
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        const int d[3] = {D == 3 ? k / 9 - 1 : k / 3 - 1, D == 3 ? (k / 3) % 3 - 1 : k % 3 - 1,
+                          D == 3 ? k % 3 - 1 : 0};
+        if (d[0] != 0 && d[1] != 0 && (D == 2 || d[2] != 0)) continue;  // shares no edge
+        int leaf = 0;
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            leaf += d[a] < 0 ? A.Fm[a] : (d[a] > 0 ? A.F1[a] : A.F0[a]);
+            ok = ok && (d[a] < 0 ? A.okm[a] : (d[a] > 0 ? A.okp[a] : true));
+        }
+        const int8_t lay = P.lay[ok ? base + (size_t)leaf : base];
+        if (ok && lay != NONE) m |= 1u << k;
+    }
+    return m;
+}
+
 // Band bits of the cubes at b + d, d in {-1, 0, 1}^D with at least one zero component (every cube
 // that can share an edge with the cube at b): bit (d0+1)*9 + (d1+1)*3 + (d2+1) is set when that
 // cube's base voxel lies in the marched brick and in the grid and is in a layer list.  The layer
@@ -501,12 +613,14 @@ __global__ void k_m_count(Geo G, Pool P, const int* parts, const int* item_j, co
         const int s = P.page[p];
         const MemV V = mem_view(P, s);
         const int* org = V.org;
-        int g[3] = {0, 0, 0};
-        rm_coords<D>(G, (uint32_t)l, g);
-        for (int a = 0; a < D; a++) g[a] += org[a];
-        const unsigned m = cube_mask<D>(G, P, V, g);
+        int g[3] = {0, 0, 0}, lc[3] = {0, 0, 0};
+        rm_coords<D>(G, (uint32_t)l, lc);
+        for (int a = 0; a < D; a++) g[a] = lc[a] + org[a];
+        CubeAxes A;
+        cube_axes<D>(G, V, lc, g, A);
+        const unsigned m = cube_mask_at<D>(G, P, V, p, A);
         const unsigned em = McT<D>::edges_of(m);
-        const unsigned bm = em ? band_mask_around<D>(G, P, V, s, org, g) : 0u;
+        const unsigned bm = em ? band_mask_at<D>(G, P, s, A) : 0u;
         unsigned om = 0, oc = 0;
         for (int e = 0; e < McT<D>::nedge; e++) {
             if (!((em >> e) & 1)) continue;
