@@ -30,6 +30,7 @@ struct Ctr {
     int front[20];       // breadth-first frontier bounds of calc_affected
     int reserve;         // bricks the step may activate at most (k_reserve_count)
     int dead_n;          // partitions deactivated since the neighbour records were last fixed
+    unsigned stamp;      // this step's stamp in the reserve count's partition stamps
     // A bounded step's affected set sorted by layer for its first round: the entries of layer
     // k - L (k != L) are seg0[k] .. seg0[k + 1]; seg0_cnt and seg0_cur count and place them.
     int seg0[18];

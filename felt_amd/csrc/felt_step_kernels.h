@@ -174,6 +174,27 @@ __global__ void k_reserve_count_members(Geo G, Pool P, const unsigned* src, cons
     if ((threadIdx.x & 63) == 0 && n) atomicAdd(&c->reserve, n);
 }
 
+// The count for a global step (any active partition marks its inactive face neighbours), from
+// the other side: each active brick's face-neighbour record names its inactive neighbour
+// partitions, which are stamped with the step's stamp and counted by the brick that stamps each
+// first.  A partition past the grid's end (the +1 rule) is recorded as outside and never
+// counted: it holds no grid voxel, so no step can activate it.
+template <int D>
+__global__ void k_reserve_count_active(Geo G, Pool P, int cap, unsigned* stamp, Ctr* c) {
+    const unsigned st = c->stamp;
+    int n = 0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += gridDim.x * blockDim.x) {
+        if (P.slot_part[s] < 0) continue;
+        const int4 ra = P.nbr[2 * (size_t)s], rb = P.nbr[2 * (size_t)s + 1];
+        const int rec[6] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y};
+#pragma unroll
+        for (int j = 0; j < 2 * D; j++)
+            if (rec[j] <= kNbrInactive) n += atomicExch(&stamp[kNbrInactive - rec[j]], st) != st;
+    }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_down(n, o);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(&c->reserve, n);
+}
+
 // A global step whose reserve exceeds the free bricks does nothing: its lists are emptied and
 // its rounds switched off before any value is written.
 __global__ void k_reserve_decide(Ctr* c, int* bbase, int NL) {
@@ -187,6 +208,8 @@ __global__ void k_reserve_decide(Ctr* c, int* bbase, int NL) {
 
 // Start of a step: counters of the previous step cleared, first round live.
 __global__ void k_step_reset(Ctr* c) {
+    // a fresh stamp per step for k_reserve_count_active (0 is the stamps' initial value)
+    c->stamp = c->stamp == 0xffffffffu ? 1u : c->stamp + 1u;
     c->sc_n = 0;
     c->flag = 0;
     c->lu_span[0][0] = c->lu_span[0][1] = 0;
@@ -221,6 +244,26 @@ __global__ void k_fill_u32_live(unsigned* p, int n, unsigned v, const int* live)
 // Empties a lookup (plookup->reset, Surface.hpp:424, 758).
 __global__ void k_lu_clear(Ctr* c, int lu, const int* live) {
     if (*live) c->lu_span[lu][1] = 0;
+}
+
+// Between unrolled rounds r and r + 1: round r's lookup lu emptied (when round r ran), then
+// round r + 1's gate.
+__global__ void k_round_next(Ctr* c, int lu, int r) {
+    if (c->live[r]) c->lu_span[lu][1] = 0;
+    const int live = c->live[r] && c->flag;
+    c->live[r + 1] = live;
+    if (live) {
+        c->flag = 0;
+        c->sc_n = 0;
+        c->need_n = 0;
+    }
+}
+
+// After the last unrolled round r: its lookup lu emptied (when it ran), and the band total
+// cleared for k_band_total2.
+__global__ void k_round_end(Ctr* c, int lu, int r) {
+    if (c->live[r]) c->lu_span[lu][1] = 0;
+    c->band_total = 0;
 }
 
 
@@ -790,6 +833,31 @@ __device__ __forceinline__ void chg_counts(const Geo& G, const uint32_t a[], con
     }
 }
 
+// Pass 2 (single block): exclusive scan of the per-tile counts; the totals become the round's
+// status-change count and the buffer lookup's span.
+__device__ __forceinline__ void chg_scan_block(int* counts, const int* rng, Ctr* ctr, int buf) {
+    const int nb = (rng[1] + kTile - 1) / kTile;
+    int carry[3] = {0, 0, 0};
+    for (int b0 = 0; b0 < nb; b0 += BT) {
+        const int b = b0 + threadIdx.x;
+        int v[3], e[3], t[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = b < nb ? counts[3 * b + c] : 0;
+        block_excl_scan3(v, e, t);
+        if (b < nb)
+#pragma unroll
+            for (int c = 0; c < 3; c++) counts[3 * b + c] = carry[c] + e[c];
+#pragma unroll
+        for (int c = 0; c < 3; c++) carry[c] += t[c];
+    }
+    if (threadIdx.x == 0) {
+        ctr->sc_n = carry[0];
+        ctr->lu_span[buf][0] = 0;
+        ctr->lu_span[buf][1] = carry[1];
+        ctr->exp_n = carry[2];
+    }
+}
+
 __global__ void __launch_bounds__(BT) k_chg_count(Geo G, const int8_t* tag, const int8_t* tk,
                                                   const int* rng, int* counts, Ctr* ctr,
                                                   const int* live) {
@@ -812,31 +880,13 @@ __global__ void __launch_bounds__(BT) k_chg_count(Geo G, const int8_t* tag, cons
     }
 }
 
-// Pass 2 (single block): exclusive scan of the per-tile counts; the totals become the round's
-// status-change count and the buffer lookup's span.
+// Pass 2 as a kernel of its own (a block that finished last could scan instead, but the
+// device-scope fence each block would need before taking its ticket writes back the L2 of its
+// XCD, which costs far more than a launch).
 __global__ void __launch_bounds__(BT) k_chg_scan(int* counts, const int* rng, Ctr* ctr, int buf,
                                                  const int* live) {
     if (!*live) return;
-    const int nb = (rng[1] + kTile - 1) / kTile;
-    int carry[3] = {0, 0, 0};
-    for (int b0 = 0; b0 < nb; b0 += BT) {
-        const int b = b0 + threadIdx.x;
-        int v[3], e[3], t[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = b < nb ? counts[3 * b + c] : 0;
-        block_excl_scan3(v, e, t);
-        if (b < nb)
-#pragma unroll
-            for (int c = 0; c < 3; c++) counts[3 * b + c] = carry[c] + e[c];
-#pragma unroll
-        for (int c = 0; c < 3; c++) carry[c] += t[c];
-    }
-    if (threadIdx.x == 0) {
-        ctr->sc_n = carry[0];
-        ctr->lu_span[buf][0] = 0;
-        ctr->lu_span[buf][1] = carry[1];
-        ctr->exp_n = carry[2];
-    }
+    chg_scan_block(counts, rng, ctr, buf);
 }
 
 // flush_status_change (Surface.hpp:1183-1244) with Untrack::retrack / untrack

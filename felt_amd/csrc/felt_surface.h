@@ -77,6 +77,7 @@ struct felt_surface {
     felt::Ctr* hcp = nullptr;  // pinned staging of that copy
     uint8_t* amark = nullptr;
     int *ukey = nullptr, *tflag = nullptr, *tlist = nullptr, *nflag = nullptr, *nlist = nullptr;
+    unsigned* rstamp = nullptr;  // per partition: the last step stamp the reserve count put there
     unsigned *dmask = nullptr, *smask = nullptr;
     int* act = nullptr;
     int* dlist = nullptr;  // partitions deactivated since the neighbour records were fixed

@@ -833,6 +833,7 @@ int felt_surface_init(felt_surface* s) {
     HIPX(hipMalloc(&s->smask, np * sizeof(unsigned)));
     HIPX(hipMalloc(&s->act, np * sizeof(int)));
     HIPX(hipMalloc(&s->dlist, np * sizeof(int)));
+    HIPX(hipMalloc(&s->rstamp, np * sizeof(unsigned)));
     HIPX(hipMalloc(&s->bbase, (G.NL + 2) * sizeof(int)));
     HIPX(hipMalloc(&s->d_speed, G.nmem * sizeof(Speed)));
     HIPX(hipHostMalloc((void**)&s->h_speed, G.nmem * sizeof(Speed), hipHostMallocDefault));
@@ -854,6 +855,7 @@ int felt_surface_init(felt_surface* s) {
     LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), s->nflag, np, 0);
     LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->dmask, np, 0);
     LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->smask, np, 0);
+    LAUNCH(k_fill_i32, dim3(gb), dim3(TPB), (int*)s->rstamp, np, 0);
     HIPX(hipMemsetAsync(s->P.lay, 0x80, cap * (int64_t)G.pv, s->st));
     // records of slots never activated read as "no neighbour" (every field kNbrOut)
     HIPX(hipMemsetAsync(s->P.nbr, 0xff, cap * 2 * sizeof(int4), s->st));
@@ -891,7 +893,7 @@ void felt_surface_release(felt_surface* s) {
     void* ptrs[] = {s->P.page, s->P.pbg, s->P.pcnt, s->P.slot_part, s->P.phi, s->P.lay,
                     s->P.slot_org, (void*)s->P.mem, (void*)s->P.part_mem, s->P.nbr, s->amark,
                     s->free_stack, s->ctr, s->ukey, s->tflag, s->tlist, s->nflag, s->nlist,
-                    s->dmask, s->smask, s->act, s->dlist, s->bbase, s->d_speed};
+                    s->dmask, s->smask, s->act, s->dlist, s->rstamp, s->bbase, s->d_speed};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     s->band.release();
@@ -1169,8 +1171,8 @@ static void global_step_pre(felt_surface* s) {
     const Geo& G = s->G;
     felt_update_start_impl(s);
     compact_band(s);
-    DISPATCH(G.D, k_reserve_count, dim3(std::min(1024, blocks_for(G.np))), dim3(TPB), G, s->P,
-             (const unsigned*)nullptr, s->ctr);
+    DISPATCH(G.D, k_reserve_count_active, dim3(std::max(1, std::min(1024, blocks_for(s->cap)))),
+             dim3(TPB), G, s->P, (int)s->cap, s->rstamp, s->ctr);
     LAUNCH(k_reserve_decide, dim3(1), dim3(1), s->ctr, s->bbase, G.NL);
     LAUNCH(k_dmask_pcnt, dim3(blocks_for(G.np)), dim3(TPB), G, s->P, s->dmask);
 }
@@ -1185,14 +1187,15 @@ static void global_step_post(felt_surface* s) {
     compact_changes(s, s->band.ptr, s->band_tag.ptr, s->band_tk.ptr, &s->ctr->band_span[0], 1,
                     always(s));
     flush_expand(s, always(s));
+    // each round's gate shares a launch with the previous round's lookup clear
+    if (s->unroll > 1) LAUNCH(k_round_gate, dim3(1), dim3(1), s->ctr, 1);
     for (int r = 1; r < s->unroll; r++) {
         const int look = r % 2 == 1 ? 1 : 0, buf = 1 - look;
-        LAUNCH(k_round_gate, dim3(1), dim3(1), s->ctr, r);
-        const int* live = &s->ctr->live[r];
-        lookup_round(s, look, buf, live, true);
-        LAUNCH(k_lu_clear, dim3(1), dim3(1), s->ctr, look, live);
+        lookup_round(s, look, buf, &s->ctr->live[r], true);
+        if (r + 1 < s->unroll) LAUNCH(k_round_next, dim3(1), dim3(1), s->ctr, look, r);
+        else LAUNCH(k_round_end, dim3(1), dim3(1), s->ctr, look, r);
     }
-    LAUNCH(k_zero_int, dim3(1), dim3(1), &s->ctr->band_total);
+    if (s->unroll <= 1) LAUNCH(k_zero_int, dim3(1), dim3(1), &s->ctr->band_total);
     LAUNCH(k_band_total2, dim3(std::min(256, blocks_for(G.np))), dim3(TPB), G, s->P, s->ctr);
 }
 
@@ -2255,7 +2258,7 @@ uint64_t felt_device_bytes(const felt_surface* s) {
     // phi, lay, amark per voxel; slot tables (partition, free stack, origin, neighbour record);
     // partition tables
     uint64_t b = cap * pv * (sizeof(float) + 2) + cap * (sizeof(int) * 2 + sizeof(int4) * 3);
-    b += np * (sizeof(int) * 10 + sizeof(float) + sizeof(int) * G.NL * 3);
+    b += np * (sizeof(int) * 11 + sizeof(float) + sizeof(int) * G.NL * 3);
     b += (uint64_t)s->mems.size() * (sizeof(Mem) + sizeof(Speed));
     b += (uint64_t)s->list_cap * kListBytes;
     b += (uint64_t)s->vox.cap * sizeof(float);
