@@ -207,7 +207,7 @@ __global__ void k_reserve_decide(Ctr* c, int* bbase, int NL) {
 }
 
 // Start of a step: counters of the previous step cleared, first round live.
-__global__ void k_step_reset(Ctr* c) {
+__device__ __forceinline__ void step_reset(Ctr* c) {
     // a fresh stamp per step for k_reserve_count_active (0 is the stamps' initial value)
     c->stamp = c->stamp == 0xffffffffu ? 1u : c->stamp + 1u;
     c->sc_n = 0;
@@ -223,6 +223,18 @@ __global__ void k_step_reset(Ctr* c) {
     for (int r = 1; r < 8; r++) c->live[r] = 0;
     c->reserve = 0;
 }
+
+// update_start in one launch: k_step_reset's work by the first thread, and the partitions' delta
+// and status-change masks cleared.
+__global__ void k_step_start(Ctr* c, unsigned* dmask, unsigned* smask, int np) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) step_reset(c);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+        dmask[p] = 0u;
+        smask[p] = 0u;
+    }
+}
+
+__global__ void k_step_reset(Ctr* c) { step_reset(c); }
 
 // Round r runs only if round r-1 ran and changed an outer layer (Surface.hpp:752-760); its
 // status-change state starts empty (m_grid_status_change.reset).
@@ -673,18 +685,6 @@ __global__ void k_tk_init(const int8_t* tag, int8_t* tk, const int* rng, const i
     FELT_SPAN_LOOP(i, rng) tk[i] = tag[i];
 }
 
-// Start of a lookup round: tk = tag over the lookup's span, and the first nmask words of smask
-// cleared (its status-change partitions start empty).
-__global__ void k_round_prep(const int8_t* tag, int8_t* tk, const int* rng, unsigned* smask,
-                             int nmask, const int* live) {
-    if (!*live) return;
-    const int lo = rng[0], n = max(rng[1] - lo, nmask);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (lo + i < rng[1]) tk[lo + i] = tag[lo + i];
-        if (i < nmask) smask[i] = 0u;
-    }
-}
-
 __device__ __forceinline__ int8_t clamp_k(int k) { return (int8_t)(k < -100 ? -100 : (k > 100 ? 100 : k)); }
 
 template <int D>
@@ -767,6 +767,22 @@ template <int D>
 __global__ void k_dist1_lookup(Geo G, Pool P, const uint32_t* list, const int8_t* tag,
                                const int* rng, int layer, float* out, const int* live) {
     if (!*live) return;
+    dist1_body<D, true>(G, P, list, tag, rng, layer, out);
+}
+
+// The first distance pass of a lookup round with the round's preparation in the same launch: tk =
+// tag over the lookup's span, and the first nmask words of smask cleared (its status-change
+// partitions start empty).  The pass reads neither tk nor smask, so the two need no order.
+template <int D>
+__global__ void k_dist1_lookup_prep(Geo G, Pool P, const uint32_t* list, const int8_t* tag,
+                                    const int* rng, int layer, float* out, int8_t* tk,
+                                    unsigned* smask, int nmask, const int* live) {
+    if (!*live) return;
+    const int lo = rng[0], n = max(rng[1] - lo, nmask);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (lo + i < rng[1]) tk[lo + i] = tag[lo + i];
+        if (i < nmask) smask[i] = 0u;
+    }
     dist1_body<D, true>(G, P, list, tag, rng, layer, out);
 }
 

@@ -1045,8 +1045,10 @@ static void compact_band(felt_surface* s) {
 // 1..L, each a Jacobi pass (distance into d, then phi = d and tk = its layer).
 // With `sorted` (a bounded step's first round), the lookup's entries are read from the affected set
 // sorted by layer (srt_v, segments seg0) and the passes touch only their layer's segment.
+// `nmask` >= 0 (a lookup round, unsorted): the round's preparation (tk = tag, the first nmask
+// words of smask cleared) rides on its first distance launch.
 static void distance_passes(felt_surface* s, bool from_band, int look, const int* live,
-                            bool sorted = false) {
+                            bool sorted = false, int nmask = -1) {
     const Geo& G = s->G;
     const int grid = from_band ? list_grid(s) : fixed_grid(s);
     const int tpb = from_band ? s->cfg.tpb : kListTpb;
@@ -1085,6 +1087,11 @@ static void distance_passes(felt_surface* s, bool from_band, int look, const int
                           s->srt_v.ptr, seg, layer, d, live);
                 PLAUNCH(PK_DIST2_LATE, 0, k_dist2_band, dim3(grid), dim3(tpb), G, s->P, s->srt_v.ptr,
                         seg, layer, d, tk, &s->ctr->nan_events, live);
+            } else if (nmask >= 0 && pass == 0 && m == 1) {
+                PDISPATCH(PK_DIST1_LATE, 0, G.D, k_dist1_lookup_prep, dim3(grid), dim3(tpb), G,
+                          s->P, list, tag, rng, layer, d, tk, s->smask, nmask, live);
+                PLAUNCH(PK_DIST2_LATE, 0, k_dist2_lookup, dim3(grid), dim3(tpb), G, s->P, list, tag,
+                        rng, layer, d, tk, &s->ctr->nan_events, live);
             } else {
                 PDISPATCH(PK_DIST1_LATE, 0, G.D, k_dist1_lookup, dim3(grid), dim3(tpb), G, s->P,
                           list, tag, rng, layer, d, live);
@@ -1131,10 +1138,7 @@ static void flush_expand(felt_surface* s, const int* live) {
 static void lookup_round(felt_surface* s, int look, int buf, const int* live,
                          bool clear_smask = false) {
     const int nmask = clear_smask ? s->G.np : 0;
-    LAUNCH(k_round_prep, dim3(std::max(fixed_grid(s), std::min(1024, blocks_for(nmask)))),
-           dim3(kListTpb), s->lu_tag[look].ptr, s->lu_tk[look].ptr, lu_span(s, look), s->smask, nmask,
-           live);
-    distance_passes(s, false, look, live);
+    distance_passes(s, false, look, live, false, nmask);
     compact_changes(s, s->lu_v[look].ptr, s->lu_tag[look].ptr, s->lu_tk[look].ptr,
                     lu_span(s, look), buf, live);
     flush_expand(s, live);
@@ -1150,11 +1154,8 @@ static int reset_round(felt_surface* s) {
 
 // update_start (Surface.hpp:393-399): delta, affected, buffer and status-change state cleared.
 int felt_update_start_impl(felt_surface* s) {
-    LAUNCH(k_step_reset, dim3(1), dim3(1), s->ctr);
-    LAUNCH(k_fill_i32, dim3(std::min(1024, blocks_for(s->G.np))), dim3(TPB), (int*)s->dmask,
-           (int64_t)s->G.np, 0);
-    LAUNCH(k_fill_i32, dim3(std::min(1024, blocks_for(s->G.np))), dim3(TPB), (int*)s->smask,
-           (int64_t)s->G.np, 0);
+    LAUNCH(k_step_start, dim3(std::min(1024, blocks_for(s->G.np))), dim3(TPB), s->ctr, s->dmask,
+           s->smask, s->G.np);
     s->in_update = true;
     s->pending_deltas.clear();
     s->pending_index.clear();
