@@ -81,13 +81,18 @@ __device__ __forceinline__ float value_noise(const int w[3], unsigned long long 
     }
     const unsigned long long mul[3] = {0x9E3779B97F4A7C15ULL, 0xC2B2AE3D27D4EB4FULL,
                                        0x165667B19E3779F9ULL};
+    // corner i's key, sum over a of (base[a] + bit a of i) * mul[a] mod 2^64: the base's
+    // products once, plus the multipliers of the corner's bits
+    unsigned long long key0 = 0;
+#pragma unroll
+    for (int a = 0; a < D; a++) key0 += (unsigned long long)(long long)base[a] * mul[a];
     float val[8];
 #pragma unroll
     for (int i = 0; i < (1 << D); i++) {
-        unsigned long long key = 0;
+        unsigned long long key = key0;
 #pragma unroll
         for (int a = 0; a < D; a++)
-            key += (unsigned long long)(long long)(base[a] + ((i >> a) & 1)) * mul[a];
+            if ((i >> a) & 1) key += mul[a];
         const unsigned long long h = splitmix64(seed ^ key);
         const float u = (float)(h >> 40) * (1.0f / 16777216.0f);
         val[i] = u * 2.0f - 1.0f;
